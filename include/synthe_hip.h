@@ -1,0 +1,168 @@
+/*
+ * synthe_hip.h -- C ABI of libsynthe_hip.so, the MI355X (gfx950) hot path of
+ * the synthe_py spectrum synthesiser: line opacity (TRANSP + ASYNTH), continuum
+ * opacity, the JOSH formal solve with its scattering iteration, emergent flux.
+ *
+ * Conventions
+ *  - Every array argument is a device pointer (hipMalloc'd or a torch CUDA
+ *    tensor's data_ptr()) unless the comment says "host".  Shapes are row
+ *    major; [D][W] means depth-major (depth d, wavelength w at d*W + w).
+ *  - Outputs are written, never accumulated into; the caller allocates them.
+ *  - Calls are asynchronous on `stream` (a hipStream_t; NULL = default
+ *    stream) unless noted.  Calls on one context are serialised by its
+ *    internal workspace; use one context per host thread for concurrency.
+ *  - Return value: SYNTHE_OK (0) or a negative SYNTHE_E* code;
+ *    synthe_last_error() gives the message of the calling thread's last
+ *    failure.  No C++ exception crosses the ABI.
+ *  - Pairs (line, depth) that fail a gate are skipped, never an error, as in
+ *    the reference (physics/line_opacity.py:111-184).
+ *
+ * Reference interfaces each entry point replaces are cited as
+ * path:line relative to the reference's synthe_py/ package.
+ */
+#ifndef SYNTHE_HIP_H
+#define SYNTHE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SYNTHE_ABI_VERSION 1
+
+#define SYNTHE_OK 0
+#define SYNTHE_EINVAL (-1) /* bad size, pointer or argument */
+#define SYNTHE_EHIP (-2)   /* a HIP runtime call failed */
+#define SYNTHE_ENOMEM (-3) /* device allocation failed */
+#define SYNTHE_ERANGE (-4) /* input exceeds a compiled limit (e.g. depth count) */
+
+#define SYNTHE_MAX_DEPTH 128 /* largest depth count the RT kernels accept */
+
+typedef struct synthe_ctx synthe_ctx;
+typedef void *synthe_stream; /* hipStream_t */
+
+/* Per-line arrays, length n_lines, in catalog order (device pointers).
+ * Mirrors the SoA built by physics/line_opacity.py:740-784. */
+typedef struct {
+  long n_lines;
+  const double *wl;          /* index wavelength, nm */
+  const double *gf;          /* 10**log gf */
+  const double *cgf;         /* 0.026538/1.77245 * gf / nu */
+  const double *gamma_rad;   /* normalised by 12.5664 nu */
+  const double *gamma_stark;
+  const double *gamma_vdw;
+  const double *elo;         /* lower-level energy, cm^-1 */
+  const int32_t *elem;       /* Z - 1, or -1 */
+  const int32_t *ion;        /* ion stage, 1-based */
+  const int32_t *ltype;      /* line type (0 = plain Voigt line) */
+  const uint8_t *process;    /* 1 if the line takes the plain-line path */
+  const int64_t *center_t;   /* centre index, engine/opacity.py:550-592 */
+  const int64_t *center_w;   /* wing centre index, physics/line_opacity.py:1030-1059 */
+} synthe_lines;
+
+/* Per-depth arrays, length n_depths (device pointers). */
+typedef struct {
+  long n_depths;
+  const double *temperature; /* K */
+  const double *rhox;        /* column mass, g cm^-2 */
+  const double *rho;         /* mass density, g cm^-3 */
+  const double *xne;         /* electron density */
+  const double *txnxn;       /* van der Waals perturber density */
+  const double *hckt;        /* hc/kT, cm */
+  const double *pop;         /* [D][n_ion][n_elem] N/U */
+  const double *dop;         /* [D][n_ion][n_elem] Doppler width / c */
+  long n_ion, n_elem;
+} synthe_depths;
+
+/* Message of the calling thread's last failure ("" if none). */
+const char *synthe_last_error(void);
+int synthe_abi_version(void);
+
+/* Create a context on the current HIP device: builds the Voigt (H0/H1/H2,
+ * physics/tables.py:214-246), FASTEX (physics/tables.py:13-59) and JOSH
+ * (physics/josh_tables.py) tables and uploads them.  No reference
+ * counterpart (the reference builds these lazily in Python). */
+int synthe_create(synthe_ctx **out);
+int synthe_destroy(synthe_ctx *ctx);
+
+/* Host copies of the context's tables (host pointers; any may be NULL):
+ * h0/h1/h2 [2001], extab/extabf [1001]. */
+int synthe_copy_tables(const synthe_ctx *ctx, double *h0, double *h1, double *h2, double *extab,
+                       double *extabf);
+
+/* Geometric wavelength grid, computed on the HOST exactly as
+ * engine/opacity.py:526-547 (_build_wavelength_grid).  Writes at most `cap`
+ * values to the host array `out` (may be NULL) and returns the count
+ * (negative on error). */
+long synthe_wavelength_grid(double start, double end, double resolution, double *out, long cap);
+
+/* physics/continuum.py:73-303 (build_depth_continuum, coefficient path).
+ * wl [W]; wledge [E] (absolute value taken); half_edge, delta_edge [E-1] or
+ * NULL to derive them; coeff_abs, coeff_scat [D][E-1][3] (log10 when
+ * coeff_log10).  Outputs cont_abs, cont_scat [D][W]. */
+int synthe_continuum(synthe_ctx *ctx, long n_depths, long n_wl, const double *wl, long n_edges,
+                     const double *wledge, const double *half_edge, const double *delta_edge,
+                     const double *coeff_abs, const double *coeff_scat, int coeff_log10,
+                     double *cont_abs, double *cont_scat, synthe_stream stream);
+
+/* physics/line_opacity.py:620-860 (compute_transp) with the Boltzmann factor
+ * FASTEX(E_low * HCKT) of physics/populations.py:172 fused in.
+ * kapmin [D][W] = cont_abs + cont_scat.  Outputs transp [L][D] and
+ * valid [L][D] (1/0). */
+int synthe_transp(synthe_ctx *ctx, const synthe_lines *lines, const synthe_depths *depths,
+                  long n_wl, const double *kapmin, double cutoff, double microturb_kms,
+                  double *transp, uint8_t *valid, synthe_stream stream);
+
+/* physics/line_opacity.py:946-1340 (compute_asynth_from_transp): centre adds,
+ * wing accumulation for plain lines, times the stimulated-emission factor.
+ * wl [W] is the grid; transp/valid [L][D] from synthe_transp;
+ * center_order [L] = lines stably sorted by center_t and center_start [W+1]
+ * its CSR offsets per pixel; wing_order [L] = lines stably sorted by
+ * center_w (all host-computed index arrays, device copies).  Output
+ * asynth [D][W].  stats (device, may be NULL) receives 4 doubles:
+ * line x depth x lambda evaluations, wing pairs, sum of MAXSTEP, max MAXSTEP. */
+int synthe_asynth(synthe_ctx *ctx, const synthe_lines *lines, const synthe_depths *depths,
+                  long n_wl, const double *wl, const double *transp, const uint8_t *valid,
+                  const double *kapmin, double cutoff, const int64_t *center_order,
+                  const int64_t *center_start, const int64_t *wing_order, double *asynth,
+                  double *stats, synthe_stream stream);
+
+/* engine/radiative.py:282-494 (solve_lte_spectrum) with
+ * physics/josh_solver.py:367-767 (solve_josh_flux) per wavelength: the
+ * continuum-only and the total JOSH solves.  All [D][W] except wl [W],
+ * temperature and rhox [D]; line_source may be NULL (Planck).  Outputs
+ * flux_total, flux_cont [W] per Hz; iters [W] (may be NULL) gets the summed
+ * JOSH iteration count of the two solves, +1000 if the total solve took the
+ * label-401 branch. */
+int synthe_lte_spectrum(synthe_ctx *ctx, long n_depths, long n_wl, const double *wl,
+                        const double *temperature, const double *rhox, const double *cont_abs,
+                        const double *cont_scat, const double *line_abs, const double *line_scat,
+                        const double *line_source, double *flux_total, double *flux_cont,
+                        int32_t *iters, synthe_stream stream);
+
+/* physics/josh_solver.py:367-767 (solve_josh_flux) over n_cols independent
+ * columns: each input is [n][n_cols] (column c at j*n_cols + c); rhox [n]
+ * shared.  Output flux [n_cols]; iters [n_cols] may be NULL. */
+int synthe_josh_flux_batch(synthe_ctx *ctx, long n, long n_cols, const double *acont,
+                           const double *scont, const double *aline, const double *sline,
+                           const double *sigmac, const double *sigmal, const double *rhox,
+                           double *flux, int32_t *iters, synthe_stream stream);
+
+/* engine/opacity.py:2215-2228 (frequency quantities) and
+ * physics/bfudge.py:312-406 in LTE (BFUDGE = 1): slinec [D][W] =
+ * BNU * STIM / (1 - EHVKT).  Any output may be NULL. */
+int synthe_source_function(synthe_ctx *ctx, long n_depths, long n_wl, const double *wl,
+                           const double *temperature, double *bnu, double *stim, double *ehvkt,
+                           double *slinec, synthe_stream stream);
+
+/* engine/opacity.py:3852-3861: flux[W] *= 2.99792458e17 / max(wl^2, 1e-40),
+ * in place. */
+int synthe_flux_to_lambda(synthe_ctx *ctx, long n_wl, const double *wl, double *flux,
+                          synthe_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SYNTHE_HIP_H */

@@ -1,0 +1,169 @@
+"""ctypes binding of libsynthe_hip.so (include/synthe_hip.h).
+
+The library is the only compute path: if it is missing, or no GPU is
+visible, every entry point raises NativeUnavailable.  There is no CPU
+fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes as ct
+import threading
+from pathlib import Path
+from typing import Optional
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libsynthe_hip.so"
+
+SYNTHE_OK = 0
+SYMBOLS = (
+    "synthe_last_error", "synthe_abi_version", "synthe_create", "synthe_destroy",
+    "synthe_copy_tables", "synthe_wavelength_grid", "synthe_continuum", "synthe_transp",
+    "synthe_asynth", "synthe_lte_spectrum", "synthe_josh_flux_batch", "synthe_source_function",
+    "synthe_flux_to_lambda",
+)
+
+
+class NativeUnavailable(RuntimeError):
+    """libsynthe_hip.so is missing or no HIP device is usable."""
+
+
+class SyntheError(RuntimeError):
+    """A libsynthe_hip entry point returned a non-zero status."""
+
+    def __init__(self, fn: str, code: int, msg: str):
+        super().__init__(f"{fn} failed ({code}): {msg}")
+        self.code = code
+
+
+_VP = ct.c_void_p
+
+
+class Lines(ct.Structure):
+    _fields_ = [("n_lines", ct.c_long), ("wl", _VP), ("gf", _VP), ("cgf", _VP),
+                ("gamma_rad", _VP), ("gamma_stark", _VP), ("gamma_vdw", _VP), ("elo", _VP),
+                ("elem", _VP), ("ion", _VP), ("ltype", _VP), ("process", _VP),
+                ("center_t", _VP), ("center_w", _VP)]
+
+
+class Depths(ct.Structure):
+    _fields_ = [("n_depths", ct.c_long), ("temperature", _VP), ("rhox", _VP), ("rho", _VP),
+                ("xne", _VP), ("txnxn", _VP), ("hckt", _VP), ("pop", _VP), ("dop", _VP),
+                ("n_ion", ct.c_long), ("n_elem", ct.c_long)]
+
+
+_lib = None
+_lib_lock = threading.Lock()
+
+
+def load_library(path: Optional[Path] = None):
+    """Load the shared library and declare prototypes (no GPU needed)."""
+    global _lib
+    with _lib_lock:
+        if _lib is not None and path is None:
+            return _lib
+        p = Path(path) if path else LIB_PATH
+        if not p.exists():
+            raise NativeUnavailable(f"{p} is not built: run python -m kurucz_py_joss_submission_amd.build")
+        lib = ct.CDLL(str(p))
+        for name in SYMBOLS:
+            getattr(lib, name)  # raises AttributeError if an export is missing
+        lib.synthe_last_error.restype = ct.c_char_p
+        lib.synthe_abi_version.restype = ct.c_int
+        lib.synthe_create.argtypes = [ct.POINTER(_VP)]
+        lib.synthe_destroy.argtypes = [_VP]
+        lib.synthe_copy_tables.argtypes = [_VP] + [_VP] * 5
+        lib.synthe_wavelength_grid.restype = ct.c_long
+        lib.synthe_wavelength_grid.argtypes = [ct.c_double, ct.c_double, ct.c_double, _VP, ct.c_long]
+        lib.synthe_continuum.argtypes = [_VP, ct.c_long, ct.c_long, _VP, ct.c_long, _VP, _VP, _VP, _VP,
+                                         _VP, ct.c_int, _VP, _VP, _VP]
+        lib.synthe_transp.argtypes = [_VP, ct.POINTER(Lines), ct.POINTER(Depths), ct.c_long, _VP,
+                                      ct.c_double, ct.c_double, _VP, _VP, _VP]
+        lib.synthe_asynth.argtypes = [_VP, ct.POINTER(Lines), ct.POINTER(Depths), ct.c_long, _VP, _VP,
+                                      _VP, _VP, ct.c_double, _VP, _VP, _VP, _VP, _VP, _VP]
+        lib.synthe_lte_spectrum.argtypes = [_VP, ct.c_long, ct.c_long] + [_VP] * 12
+        lib.synthe_josh_flux_batch.argtypes = [_VP, ct.c_long, ct.c_long] + [_VP] * 10
+        lib.synthe_source_function.argtypes = [_VP, ct.c_long, ct.c_long] + [_VP] * 7
+        lib.synthe_flux_to_lambda.argtypes = [_VP, ct.c_long, _VP, _VP, _VP]
+        for name in SYMBOLS[2:]:
+            if name != "synthe_wavelength_grid":
+                getattr(lib, name).restype = ct.c_int
+        if path is None:
+            _lib = lib
+        return lib
+
+
+def check(fn: str, rc: int) -> None:
+    if rc != SYNTHE_OK:
+        msg = load_library().synthe_last_error().decode(errors="replace")
+        raise SyntheError(fn, rc, msg)
+
+
+class Context:
+    """A libsynthe_hip context (device tables + workspace) on the current GPU."""
+
+    def __init__(self):
+        import torch
+
+        if not torch.cuda.is_available():
+            raise NativeUnavailable("no HIP device is visible: the synthe hot path needs a GPU")
+        self.lib = load_library()
+        torch.cuda.init()
+        h = _VP()
+        check("synthe_create", self.lib.synthe_create(ct.byref(h)))
+        self.handle = h
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self.lib.synthe_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def tables(self):
+        h0, h1, h2 = (np.zeros(2001) for _ in range(3))
+        e, f = np.zeros(1001), np.zeros(1001)
+        check("synthe_copy_tables", self.lib.synthe_copy_tables(
+            self.handle, h0.ctypes.data, h1.ctypes.data, h2.ctypes.data, e.ctypes.data, f.ctypes.data))
+        return h0, h1, h2, e, f
+
+
+_ctx: Optional[Context] = None
+_ctx_lock = threading.Lock()
+
+
+def context() -> Context:
+    global _ctx
+    with _ctx_lock:
+        if _ctx is None:
+            _ctx = Context()
+        return _ctx
+
+
+def stream_handle():
+    import torch
+
+    return _VP(torch.cuda.current_stream().cuda_stream)
+
+
+def ptr(t) -> Optional[int]:
+    """Device address of a torch tensor (None passes NULL)."""
+    if t is None:
+        return None
+    return t.data_ptr()
+
+
+def wavelength_grid(start: float, end: float, resolution: float) -> np.ndarray:
+    """engine/opacity.py:526-547 through the library's host routine."""
+    lib = load_library()
+    n = lib.synthe_wavelength_grid(start, end, resolution, None, 0)
+    if n < 0:
+        check("synthe_wavelength_grid", int(n))
+    out = np.zeros(int(n), dtype=np.float64)
+    lib.synthe_wavelength_grid(start, end, resolution, out.ctypes.data, int(n))
+    return out

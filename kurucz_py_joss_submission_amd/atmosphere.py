@@ -1,0 +1,148 @@
+"""Atmosphere bundles: the per-depth inputs of the hot path.
+
+The on-disk bundle is the reference's ``.npz`` schema (the subset the hot path
+reads; synthe_py/io/atmosphere.py:82-506 load_cached, keys written by
+tools/convert_atm_to_npz.py:2755-2830):
+
+=================  ===========================  ==============================
+key                shape                        meaning
+=================  ===========================  ==============================
+depth              [D]                          RHOX column mass, g cm^-2
+temperature        [D]                          K
+gas_pressure       [D]                          dyn cm^-2
+electron_density   [D]                          cm^-3
+mass_density       [D]                          g cm^-3
+turbulent_velocity [D]                          cm s^-1
+hckt               [D]                          hc/kT, cm
+txnxn              [D] (optional)               vdW perturber density
+xnf_h, xnf_he1,    [D]                          for TXNXN when absent
+xnf_h2
+population_per_ion [D][6][NELEM]                N_ion / U_ion, cm^-3
+doppler_per_ion    [D][6][NELEM]                Doppler width / c
+wledge             [E]                          continuum edges, nm
+half_edge          [E-1] (optional)
+delta_edge         [E-1] (optional)
+cont_abs_coeff     [D][E-1][3]                  log10 kappa_abs at edge nodes
+cont_scat_coeff    [D][E-1][3]                  log10 sigma at edge nodes
+cont_coeff_log10   scalar bool
+=================  ===========================  ==============================
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+from pathlib import Path
+from typing import Dict, Optional
+
+import numpy as np
+
+
+@dataclass
+class AtmosphereModel:
+    """Hot-path subset of the reference AtmosphereModel (io/atmosphere.py:16-79)."""
+
+    depth: np.ndarray
+    temperature: np.ndarray
+    gas_pressure: np.ndarray
+    electron_density: np.ndarray
+    mass_density: np.ndarray
+    turbulent_velocity: np.ndarray
+    hckt: Optional[np.ndarray] = None
+    txnxn: Optional[np.ndarray] = None
+    xnf_h: Optional[np.ndarray] = None
+    xnf_he1: Optional[np.ndarray] = None
+    xnf_h2: Optional[np.ndarray] = None
+    population_per_ion: Optional[np.ndarray] = None
+    doppler_per_ion: Optional[np.ndarray] = None
+    continuum_wledge: Optional[np.ndarray] = None
+    continuum_half_edge: Optional[np.ndarray] = None
+    continuum_delta_edge: Optional[np.ndarray] = None
+    continuum_abs_coeff: Optional[np.ndarray] = None
+    continuum_scat_coeff: Optional[np.ndarray] = None
+    continuum_coeff_log10: bool = True
+    metadata: Optional[Dict[str, float]] = None
+
+    @property
+    def layers(self) -> int:
+        return int(self.temperature.size)
+
+    def hckt_or_default(self) -> np.ndarray:
+        """HCKT per depth; the reference falls back to 1.4388/T (populations.py:166-171)."""
+        if self.hckt is not None and self.hckt.size == self.layers:
+            return np.asarray(self.hckt, dtype=np.float64)
+        t = np.maximum(self.temperature.astype(np.float64), 1.0)
+        return 1.4388 / t
+
+    def txnxn_or_default(self) -> np.ndarray:
+        """TXNXN per depth (populations.py:190-199)."""
+        if self.txnxn is not None:
+            return np.asarray(self.txnxn, dtype=np.float64)
+        n = self.layers
+        out = np.empty(n, dtype=np.float64)
+        for i in range(n):
+            temp = max(float(self.temperature[i]), 1.0)
+            h = float(self.xnf_h[i]) if self.xnf_h is not None else 0.0
+            he = float(self.xnf_he1[i]) if self.xnf_he1 is not None else 0.0
+            h2 = float(self.xnf_h2[i]) if self.xnf_h2 is not None else 0.0
+            out[i] = (h + 0.42 * he + 0.85 * h2) * (temp / 10_000.0) ** 0.3
+        return out
+
+    def to_npz_dict(self) -> Dict[str, np.ndarray]:
+        d = {
+            "depth": self.depth, "temperature": self.temperature,
+            "gas_pressure": self.gas_pressure, "electron_density": self.electron_density,
+            "mass_density": self.mass_density, "turbulent_velocity": self.turbulent_velocity,
+        }
+        opt = {
+            "hckt": self.hckt, "txnxn": self.txnxn, "xnf_h": self.xnf_h,
+            "xnf_he1": self.xnf_he1, "xnf_h2": self.xnf_h2,
+            "population_per_ion": self.population_per_ion,
+            "doppler_per_ion": self.doppler_per_ion, "wledge": self.continuum_wledge,
+            "half_edge": self.continuum_half_edge, "delta_edge": self.continuum_delta_edge,
+            "cont_abs_coeff": self.continuum_abs_coeff,
+            "cont_scat_coeff": self.continuum_scat_coeff,
+        }
+        d.update({k: v for k, v in opt.items() if v is not None})
+        d["cont_coeff_log10"] = np.array(bool(self.continuum_coeff_log10))
+        for k, v in (self.metadata or {}).items():
+            d[f"meta_{k}"] = np.array(v)
+        return d
+
+    def save(self, path: Path) -> None:
+        np.savez(path, **self.to_npz_dict())
+
+
+def load_cached(path: Path) -> AtmosphereModel:
+    """Load a bundle ``.npz`` (io/atmosphere.py:82-506, hot-path keys only)."""
+    with np.load(path, allow_pickle=False) as data:
+        def pick(*names, required=True):
+            for n in names:
+                if n in data:
+                    return np.asarray(data[n])
+            if required:
+                raise KeyError(f"Required atmosphere field missing: {' / '.join(names)}")
+            return None
+
+        meta = {k[5:]: data[k].item() for k in data.files if k.startswith("meta_")}
+        return AtmosphereModel(
+            depth=pick("depth", "rhox", "qrhox").astype(np.float64),
+            temperature=pick("temperature", "qt").astype(np.float64),
+            gas_pressure=pick("gas_pressure", "qp").astype(np.float64),
+            electron_density=pick("electron_density", "qxne").astype(np.float64),
+            mass_density=pick("mass_density", "qrho").astype(np.float64),
+            turbulent_velocity=pick("turbulent_velocity", "qvturb").astype(np.float64),
+            hckt=pick("hckt", "qhckt", required=False),
+            txnxn=pick("txnxn", required=False),
+            xnf_h=pick("xnf_h", "qxnf_h", required=False),
+            xnf_he1=pick("xnf_he1", required=False),
+            xnf_h2=pick("xnf_h2", "qxnf_h2", required=False),
+            population_per_ion=pick("population_per_ion", required=False),
+            doppler_per_ion=pick("doppler_per_ion", required=False),
+            continuum_wledge=pick("wledge", required=False),
+            continuum_half_edge=pick("half_edge", required=False),
+            continuum_delta_edge=pick("delta_edge", required=False),
+            continuum_abs_coeff=pick("cont_abs_coeff", required=False),
+            continuum_scat_coeff=pick("cont_scat_coeff", required=False),
+            continuum_coeff_log10=bool(data["cont_coeff_log10"]) if "cont_coeff_log10" in data else False,
+            metadata=meta,
+        )

@@ -1,0 +1,1167 @@
+// libsynthe_hip: HIP kernels for gfx950 and the C ABI of include/synthe_hip.h.
+//
+// Pipeline for one spectrum (all device-resident):
+//   k_continuum      build_depth_continuum            [D][W]
+//   k_source         frequency quantities, SLINEC     [D][W]
+//   k_transp_wing    TRANSP + per-(line,depth) wing setup (probe), one thread per pair
+//   k_class_count / k_class_scan / k_regions / k_class_scatter
+//                    per-depth lists of wing pairs, bucketed by reach class and
+//                    ordered by centre: a deterministic, atomic-free transpose
+//                    from line-major to wavelength-major
+//   k_gather         centre adds + wing gather per (depth, pixel), times STIM
+//   k_lte            JOSH formal solve, two per wavelength, one wavelength per lane
+//
+// Numerics follow the reference operation by operation (see synthe_device.h);
+// the build uses -ffp-contract=off.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "_tables_gen.h"
+#include "synthe_device.h"
+#include "synthe_hip.h"
+
+using namespace synthe;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess) return fail(SYNTHE_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr int kClasses = 9;  // reach classes: MAXSTEP <= 16 * 4^k
+constexpr int kBlock = 256;
+
+// ---------------------------------------------------------------- host tables
+// physics/tables.py:275-391 (_map1_interpolate): the MAP1 variant that builds H1TAB.
+void map1_table(const double* xold, const double* fold, int nold, const double* xnew, double* fnew,
+                int nnew) {
+#define XO(i) xold[(i)-1]
+#define FO(i) fold[(i)-1]
+  int l = 2, ll = 0;
+  double a = 0, b = 0, c = 0, abac = 0, bbac = 0, cbac = 0, afor = 0, bfor = 0, cfor = 0;
+  for (int k = 1; k <= nnew; ++k) {
+    double x = xnew[k - 1];
+    while (true) {
+      if (x < XO(l)) break;
+      ++l;
+      if (l > nold) break;
+    }
+    if (l > nold) {
+      if (l != ll) {
+        l = nold;
+        c = 0.0;
+        b = (FO(l) - FO(l - 1)) / (XO(l) - XO(l - 1));
+        a = FO(l) - XO(l) * b;
+        ll = l;
+      }
+      fnew[k - 1] = a + (b + c * x) * x;
+      continue;
+    }
+    if (l == ll) {
+      fnew[k - 1] = a + (b + c * x) * x;
+      continue;
+    }
+    if (l == 2) {
+      c = 0.0;
+      b = (FO(l) - FO(l - 1)) / (XO(l) - XO(l - 1));
+      a = FO(l) - XO(l) * b;
+      ll = l;
+      fnew[k - 1] = a + (b + c * x) * x;
+      continue;
+    }
+    int l1 = l - 1;
+    bool done = false;
+    if (l > ll + 1 || l == 3) {
+      int l2 = l - 2;
+      double d = (FO(l1) - FO(l2)) / (XO(l1) - XO(l2));
+      cbac = FO(l) / ((XO(l) - XO(l1)) * (XO(l) - XO(l2))) +
+             (FO(l2) / (XO(l) - XO(l2)) - FO(l1) / (XO(l) - XO(l1))) / (XO(l1) - XO(l2));
+      bbac = d - (XO(l1) + XO(l2)) * cbac;
+      abac = FO(l2) - XO(l2) * d + XO(l1) * XO(l2) * cbac;
+      if (l == nold) done = true;
+    } else {
+      cbac = cfor;
+      bbac = bfor;
+      abac = afor;
+      if (l == nold) done = true;
+    }
+    if (done) {
+      c = cbac;
+      b = bbac;
+      a = abac;
+      ll = l;
+      fnew[k - 1] = a + (b + c * x) * x;
+      continue;
+    }
+    double d = (FO(l) - FO(l1)) / (XO(l) - XO(l1));
+    cfor = FO(l + 1) / ((XO(l + 1) - XO(l)) * (XO(l + 1) - XO(l1))) +
+           (FO(l1) / (XO(l + 1) - XO(l1)) - FO(l) / (XO(l + 1) - XO(l))) / (XO(l) - XO(l1));
+    bfor = d - (XO(l) + XO(l1)) * cfor;
+    afor = FO(l1) - XO(l1) * d + XO(l) * XO(l1) * cfor;
+    double wt = 0.0;
+    if (fabs(cfor) != 0.0) wt = fabs(cfor) / (fabs(cfor) + fabs(cbac));
+    a = afor + wt * (abac - afor);
+    b = bfor + wt * (bbac - bfor);
+    c = cfor + wt * (cbac - cfor);
+    ll = l;
+    fnew[k - 1] = a + (b + c * x) * x;
+  }
+#undef XO
+#undef FO
+}
+
+// ------------------------------------------------------------------- kernels
+struct LinesDev {
+  long n;
+  const double *wl, *gf, *cgf, *grad, *gstark, *gvdw, *elo;
+  const int32_t *elem, *ion, *ltype;
+  const uint8_t* process;
+  const int64_t *center_t, *center_w;
+};
+
+struct DepthsDev {
+  int n;
+  const double *temp, *rhox, *rho, *xne, *txnxn, *hckt, *pop, *dop;
+  int nion, nelem;
+};
+
+__device__ __forceinline__ int reach_class(int m) {
+  int k = 0;
+  long r = 16;
+  while (m > r && k < kClasses - 1) {
+    r <<= 2;
+    ++k;
+  }
+  return k;
+}
+
+// physics/continuum.py:73-303, coefficient path, one thread per wavelength.
+__global__ void k_continuum(int D, long W, const double* __restrict__ wl, int E,
+                            const double* __restrict__ wledge, const double* __restrict__ half_in,
+                            const double* __restrict__ delta_in, const double* __restrict__ cabs,
+                            const double* __restrict__ cscat, int log10flag,
+                            double* __restrict__ out_abs, double* __restrict__ out_scat) {
+  long w = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (w >= W) return;
+  double x = wl[w];
+  double ax = fabs(x);
+  int lo = 0, hi = E;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (fabs(wledge[mid]) <= ax) lo = mid + 1;
+    else hi = mid;
+  }
+  int e = lo - 1;
+  e = e < 0 ? 0 : (e > E - 2 ? E - 2 : e);
+  double left = fabs(wledge[e]), right = fabs(wledge[e + 1]);
+  double h = half_in ? half_in[e] : 0.5 * (left + right);
+  double dl;
+  if (delta_in) dl = delta_in[e];
+  else {
+    double dd = right - left;
+    dl = 0.5 * dd * dd;
+  }
+  if (dl == 0.0) dl = 1e-20;
+  double c1 = (x - h) * (x - right) / dl;
+  double c2 = (left - x) * (x - right) * 2.0 / dl;
+  double c3 = (x - left) * (x - h) / dl;
+  for (int d = 0; d < D; ++d) {
+    const double* ca = cabs + ((long)d * (E - 1) + e) * 3;
+    const double* cs = cscat + ((long)d * (E - 1) + e) * 3;
+    double la = ca[0] * c1 + ca[1] * c2 + ca[2] * c3;
+    double ls = cs[0] * c1 + cs[1] * c2 + cs[2] * c3;
+    double va = la, vs = ls;
+    if (log10flag) {
+      if (la > 38.0) la = 38.0;
+      if (ls > 38.0) ls = 38.0;
+      va = pow(10.0, la);
+      vs = pow(10.0, ls);
+      if (va < 0.0) va = 0.0;
+      if (va > 3.4e38) va = 3.4e38;
+      if (vs < 0.0) vs = 0.0;
+      if (vs > 3.4e38) vs = 3.4e38;
+      if (!isfinite(va)) va = 3.4e38;
+      if (!isfinite(vs)) vs = 3.4e38;
+    }
+    out_abs[(long)d * W + w] = va;
+    out_scat[(long)d * W + w] = vs;
+  }
+}
+
+// engine/opacity.py:2215-2228 and physics/bfudge.py:312-406 (LTE).
+__global__ void k_source(int D, long W, const double* __restrict__ wl,
+                         const double* __restrict__ temp, double* bnu_o, double* stim_o,
+                         double* ehvkt_o, double* slinec_o) {
+  long w = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  int d = blockIdx.y;
+  if (w >= W) return;
+  double hkt = kHPlanck / (kKBoltz * temp[d]);
+  double freq = kCLightCm / (wl[w] * kNmToCm);
+  double ehvkt = exp(-freq * hkt);
+  double stim = 1.0 - ehvkt;
+  double freq15 = freq / 1.0e15;
+  double bnu = 1.47439e-02 * pow(freq15, 3.0) * ehvkt / stim;
+  long i = (long)d * W + w;
+  if (bnu_o) bnu_o[i] = bnu;
+  if (stim_o) stim_o[i] = stim;
+  if (ehvkt_o) ehvkt_o[i] = ehvkt;
+  if (slinec_o) slinec_o[i] = (bnu * stim) / (1.0 - ehvkt);
+}
+
+// TRANSP (physics/line_opacity.py:56-184) and the ASYNTH wing setup
+// (line_opacity.py:1160-1272 prep, 410-537 phases 1-2) for one (line, depth)
+// pair per thread.  mode bit 0: compute transp/valid (else read transp_in /
+// valid_in); bit 1: compute wing parameters and the reach class.
+__global__ void k_transp_wing(LinesDev ln, DepthsDev dp, long W, const double* __restrict__ kapmin,
+                              double cutoff, double micro, double resolu, DeviceTables tb, int mode,
+                              double* transp, uint8_t* valid, const double* transp_in,
+                              const uint8_t* valid_in, WingParams* wp, uint8_t* cls) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long L = ln.n;
+  int D = dp.n;
+  if (i >= L * D) return;
+  long l = i / D;
+  int d = (int)(i - l * D);
+  double tv = 0.0;
+  bool ok = false;
+  if (mode & 1) {
+    double kapcen = 0.0;
+    int ei = ln.elem[l], nel = ln.ion[l];
+    if (ln.process[l] && ei >= 0 && ei < dp.nelem && nel > 0 && nel <= dp.nion) {
+      long c = ln.center_t[l];
+      long cc = c < 0 ? 0 : (c > W - 1 ? W - 1 : c);
+      long pi = ((long)d * dp.nion + (nel - 1)) * dp.nelem + ei;
+      double pv = dp.pop[pi];
+      double dv = dp.dop[pi];
+      double lw = ln.wl[l];
+      if (micro > 0.0) dv = sqrt(dv * dv + micro * micro);
+      double r = dp.rho[d];
+      if (pv > 0.0 && dv > 0.0 && r > 0.0) {
+        double xnfdop = pv / (r * dv);
+        double dw = dv * lw;
+        double boltz = fastex(ln.elo[l] * dp.hckt[d], tb.extab, tb.extabf);
+        int lt = ln.ltype[l];
+        double k0pre = lt == 1 ? ln.gvdw[l] * ln.gf[l] * (pv / r) : ln.cgf[l] * xnfdop;
+        double km = kapmin[(long)d * W + cc] * cutoff;
+        double post = k0pre * boltz;
+        if (!(k0pre < km) && !(post < km) && post > 0.0) {
+          double dopple = lw > 0 ? dw / lw : 1e-6;
+          double adamp = 0.0;
+          if (dw > 0 && dopple > 0) {
+            double gt = ln.grad[l] + ln.gstark[l] * dp.xne[d] + ln.gvdw[l] * dp.txnxn[d];
+            adamp = gt / dopple;
+          }
+          if (adamp >= 0.0) {
+            if (lt == 1) kapcen = post;
+            else if (adamp < 0.2) kapcen = post * (1.0 - 1.128 * adamp);
+            else kapcen = post * voigt(0.0, adamp, tb.h0, tb.h1, tb.h2);
+            ok = true;
+          }
+        }
+      }
+    }
+    tv = ok ? kapcen : 0.0;
+    transp[i] = tv;
+    valid[i] = ok ? 1 : 0;
+  } else {
+    tv = transp_in[i];
+    ok = valid_in[i] != 0;
+  }
+  if (!(mode & 2)) return;
+  uint8_t k_out = 0xFF;
+  long c = ln.center_w[l];
+  int ei = ln.elem[l], nel = ln.ion[l];
+  if (ok && tv > 0.0 && ln.ltype[l] == 0 && c >= -kMaxProfileSteps && c <= W - 1 + kMaxProfileSteps &&
+      ei >= 0 && ei < dp.nelem && nel > 0 && nel <= dp.nion) {
+    long pi = ((long)d * dp.nion + (nel - 1)) * dp.nelem + ei;
+    double pv = dp.pop[pi];
+    double dv = dp.dop[pi];
+    double lw = ln.wl[l];
+    if (pv > 0.0 && dv > 0.0) {
+      double dw = dv * lw;
+      double dopple_p = lw > 0 ? dw / lw : 1e-6;
+      double adamp = 0.0;
+      if (dopple_p > 0 && lw > 0) {
+        double gt = ln.grad[l] + ln.gstark[l] * dp.xne[d] + ln.gvdw[l] * dp.txnxn[d];
+        adamp = gt / dopple_p;
+      }
+      if (adamp < 1e-12) adamp = 1e-12;
+      long cw = c < 0 ? 0 : (c > W - 1 ? W - 1 : c);
+      double kref = kapmin[(long)d * W + cw] * cutoff;
+      double vc = adamp < 0.2 ? 1.0 - 1.128 * adamp : voigt(0.0, adamp, tb.h0, tb.h1, tb.h2);
+      double kappa0 = vc > 0 ? tv / vc : tv;
+      if (dw > 0.0) {
+        double dopple = lw > 0.0 ? dw / lw : 1e-10;
+        long n10dop = (long)(10.0 * dopple * resolu);
+        double dvoigt = dopple > 0 ? 1.0 / (dopple * resolu) : 1.0;
+        double tabstep = 200.0 * dvoigt;
+        long cut = -1;
+        double p_n10 = 0.0;
+        double tabi = 0.5;
+        for (long k = 1; k <= n10dop; ++k) {
+          double p;
+          double x = (double)k * dvoigt;
+          if (adamp < 0.2) {
+            tabi += tabstep;
+            long idx = (long)tabi;
+            if (x > 10.0) p = kappa0 * (0.5642 * adamp / (x * x));
+            else {
+              idx = idx < 0 ? 0 : (idx > kVoigtN - 1 ? kVoigtN - 1 : idx);
+              p = kappa0 * (tb.h0[idx] + adamp * tb.h1[idx]);
+            }
+          } else {
+            p = kappa0 * voigt(x, adamp, tb.h0, tb.h1, tb.h2);
+          }
+          if (k == n10dop) p_n10 = p;
+          if (p < kref) {
+            cut = k;
+            break;
+          }
+        }
+        long maxstep;
+        int use_far;
+        double x_far;
+        if (cut != -1) {
+          maxstep = cut;
+          use_far = 0;
+          x_far = 0.0;
+        } else {
+          use_far = 1;
+          if (n10dop > 0 && p_n10 > 0.0 && kref > 0.0) {
+            x_far = p_n10 * ((double)n10dop * (double)n10dop);
+            double ms = sqrt(x_far / kref) + 1.0;
+            maxstep = ms < 2.0e6 ? (long)ms : kMaxProfileSteps;
+          } else {
+            x_far = 0.0;
+            maxstep = 0;
+          }
+          if (maxstep > kMaxProfileSteps) maxstep = kMaxProfileSteps;
+        }
+        if (maxstep > 0) {
+          WingParams p;
+          p.kappa0 = kappa0;
+          p.adamp = adamp;
+          p.dvoigt = dvoigt;
+          p.x_far = x_far;
+          p.c = (int32_t)c;
+          p.maxstep = (int32_t)maxstep;
+          p.n10dop = (int32_t)n10dop;
+          p.use_far = use_far;
+          wp[i] = p;
+          k_out = (uint8_t)reach_class((int)maxstep);
+        }
+      }
+    }
+  }
+  cls[i] = k_out;
+}
+
+// Per (block of 256 centre-sorted lines, depth): pair counts per reach class.
+__global__ __launch_bounds__(kBlock) void k_class_count(long L, int D, const int64_t* __restrict__ order,
+                                                        const uint8_t* __restrict__ cls,
+                                                        int32_t* __restrict__ counts) {
+  __shared__ int sh[kBlock / 64][kClasses];
+  long s = blockIdx.x * (long)kBlock + threadIdx.x;
+  long l = s < L ? order[s] : -1;
+  int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int d = 0; d < D; ++d) {
+    int k = l >= 0 ? cls[l * D + d] : 0xFF;
+    for (int kk = 0; kk < kClasses; ++kk) {
+      unsigned long long m = __ballot(k == kk);
+      if (lane == 0) sh[wave][kk] = __popcll(m);
+    }
+    __syncthreads();
+    if (threadIdx.x < kClasses) {
+      int t = 0;
+      for (int q = 0; q < kBlock / 64; ++q) t += sh[q][threadIdx.x];
+      counts[((long)blockIdx.x * D + d) * kClasses + threadIdx.x] = t;
+    }
+    __syncthreads();
+  }
+}
+
+// Exclusive scan over blocks of counts[blk][d][k] for one (d, k) per workgroup;
+// totals[d][k] gets the sum.
+__global__ __launch_bounds__(kBlock) void k_class_scan(long nblk, int D, int32_t* __restrict__ counts,
+                                                       int64_t* __restrict__ totals) {
+  __shared__ int64_t sh[kBlock];
+  int dk = blockIdx.x;
+  int d = dk / kClasses, kk = dk % kClasses;
+  int64_t carry = 0;
+  for (long base = 0; base < nblk; base += kBlock) {
+    long b = base + threadIdx.x;
+    int64_t v = b < nblk ? counts[(b * D + d) * kClasses + kk] : 0;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < kBlock; off <<= 1) {
+      int64_t t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+      __syncthreads();
+      sh[threadIdx.x] += t;
+      __syncthreads();
+    }
+    int64_t incl = sh[threadIdx.x];
+    if (b < nblk) counts[(b * D + d) * kClasses + kk] = (int32_t)(carry + incl - v);
+    carry += sh[kBlock - 1];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) totals[dk] = carry;
+}
+
+// regions[d][k] = start of the (d, k) list; regions[d][kClasses] = end.
+__global__ void k_regions(int D, long L, const int64_t* __restrict__ totals, int64_t* __restrict__ regions) {
+  int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= D) return;
+  int64_t pos = (int64_t)d * L;
+  for (int k = 0; k < kClasses; ++k) {
+    regions[d * (kClasses + 1) + k] = pos;
+    pos += totals[d * kClasses + k];
+  }
+  regions[d * (kClasses + 1) + kClasses] = pos;
+}
+
+// Stable scatter of wing pairs into their (depth, class) lists in centre order.
+__global__ __launch_bounds__(kBlock) void k_class_scatter(
+    long L, int D, const int64_t* __restrict__ order, const uint8_t* __restrict__ cls,
+    const int32_t* __restrict__ offsets, const int64_t* __restrict__ regions,
+    const WingParams* __restrict__ wp, int2* __restrict__ list_cm, WingParams* __restrict__ list_p) {
+  __shared__ int sh[kBlock / 64][kClasses];
+  long s = blockIdx.x * (long)kBlock + threadIdx.x;
+  long l = s < L ? order[s] : -1;
+  int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  unsigned long long ltmask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int d = 0; d < D; ++d) {
+    int k = l >= 0 ? cls[l * D + d] : 0xFF;
+    int rank = 0;
+    for (int kk = 0; kk < kClasses; ++kk) {
+      unsigned long long m = __ballot(k == kk);
+      if (lane == 0) sh[wave][kk] = __popcll(m);
+      if (k == kk) rank = __popcll(m & ltmask);
+    }
+    __syncthreads();
+    if (k != 0xFF) {
+      int before = 0;
+      for (int q = 0; q < wave; ++q) before += sh[q][k];
+      int64_t pos = regions[d * (kClasses + 1) + k] +
+                    offsets[((long)blockIdx.x * D + d) * kClasses + k] + before + rank;
+      WingParams p = wp[l * D + d];
+      list_cm[pos] = make_int2(p.c, p.maxstep);
+      list_p[pos] = p;
+    }
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ long lower_bound_c(const int2* __restrict__ a, long lo, long hi, long key) {
+  while (lo < hi) {
+    long mid = (lo + hi) >> 1;
+    if ((long)a[mid].x < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Centre adds (catalog order, line_opacity.py:1091-1102) then wings of every
+// (depth, class) list whose reach can touch pixel w, then * STIM
+// (line_opacity.py:1338).  One thread per (pixel, depth); fixed summation order.
+__global__ __launch_bounds__(kBlock) void k_gather(
+    long W, int D, const int64_t* __restrict__ corder, const int64_t* __restrict__ cstart,
+    const double* __restrict__ transp, const uint8_t* __restrict__ valid,
+    const int2* __restrict__ list_cm, const WingParams* __restrict__ list_p,
+    const int64_t* __restrict__ regions, const double* __restrict__ grid,
+    const double* __restrict__ temp, DeviceTables tb, double* __restrict__ asynth,
+    unsigned long long* __restrict__ evals) {
+  long w = blockIdx.x * (long)kBlock + threadIdx.x;
+  int d = blockIdx.y;
+  unsigned long long n_eval = 0;
+  if (w < W) {
+    double acc = 0.0;
+    for (long j = cstart[w]; j < cstart[w + 1]; ++j) {
+      long l = corder[j];
+      if (valid[l * D + d]) {
+        acc += transp[l * D + d];
+        ++n_eval;
+      }
+    }
+    const int64_t* reg = regions + d * (kClasses + 1);
+    long r = 16;
+    for (int kk = 0; kk < kClasses; ++kk, r <<= 2) {
+      long beg = reg[kk], end = reg[kk + 1];
+      if (beg == end) continue;
+      long j = lower_bound_c(list_cm, beg, end, w - r);
+      for (; j < end; ++j) {
+        int2 cm = list_cm[j];
+        if ((long)cm.x > w + r) break;
+        long k = w - (long)cm.x;
+        k = k < 0 ? -k : k;
+        if (k >= 1 && k <= cm.y) {
+          WingParams p = list_p[j];
+          acc += wing_value(p, k, tb.h0, tb.h1, tb.h2);
+          ++n_eval;
+        }
+      }
+    }
+    double T = temp[d] > 1.0 ? temp[d] : 1.0;
+    double hkt = kHPlanck / (kKBoltz * T);
+    double fg = kCLightNm / grid[w];
+    asynth[(long)d * W + w] = acc * (1.0 - exp(-fg * hkt));
+  }
+  if (evals) {
+    __shared__ unsigned long long red[kBlock / 64];
+    for (int off = 32; off > 0; off >>= 1) n_eval += __shfl_down(n_eval, off);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = n_eval;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long t = 0;
+      for (int q = 0; q < kBlock / 64; ++q) t += red[q];
+      atomicAdd(evals, t);
+    }
+  }
+}
+
+// Wing statistics over the lists: pairs, sum and max of MAXSTEP.
+__global__ void k_wing_stats(long n_entries_cap, int D, long L, const int64_t* __restrict__ regions,
+                             const int2* __restrict__ list_cm, unsigned long long* out) {
+  int d = blockIdx.y;
+  long beg = regions[d * (kClasses + 1)], end = regions[d * (kClasses + 1) + kClasses];
+  unsigned long long cnt = 0, sum = 0, mx = 0;
+  for (long j = beg + blockIdx.x * (long)blockDim.x + threadIdx.x; j < end;
+       j += (long)gridDim.x * blockDim.x) {
+    unsigned long long m = (unsigned long long)list_cm[j].y;
+    ++cnt;
+    sum += m;
+    mx = m > mx ? m : mx;
+  }
+  atomicAdd(out + 0, cnt);
+  atomicAdd(out + 1, sum);
+  atomicMax(out + 2, mx);
+}
+
+__global__ void k_stats_to_double(const unsigned long long* in, double* out) {
+  if (threadIdx.x == 0) {
+    out[0] = (double)in[3];
+    out[1] = (double)in[0];
+    out[2] = (double)in[1];
+    out[3] = (double)in[2];
+  }
+}
+
+// --------------------------------------------------------------- JOSH solve
+// Inputs of one solve_josh_flux call, read through a depth index list so the
+// masked/reversed arrays of solve_lte_frequency are never materialised.
+struct SolveIn {
+  const double *acont, *scont, *aline, *sline, *sigmac, *sigmal;  // strided by `stride`
+  const double* rhox;                                             // per depth
+  long stride;
+  const double* scont_local;  // per-lane Planck values [n] (NULL: use scont)
+  const double* sline_local;  // per-lane line source [n] (NULL: use sline)
+  bool zero_line;             // aline = sigmal = 0 (continuum-only solve)
+};
+
+// One wavelength per lane.  xs lives in LDS ([k][lane]) so the Gauss-Seidel
+// sweep can index it dynamically; COEFJ is read with wave-uniform addresses.
+template <int kLanes>
+__device__ double josh_solve(const SolveIn& in, const int* idx, int n, DeviceTables tb,
+                             float (*s_xs)[kLanes], int lane, double* abtot, double* alpha,
+                             double* snubar, double* rho, double* taunu, double* pa, double* pb,
+                             double* pc, int* iters, int* took401) {
+  for (int j = 0; j < n; ++j) {
+    long di = (long)idx[j] * in.stride;
+    double ac = in.acont[di];
+    double al = in.zero_line ? 0.0 : in.aline[di];
+    double sc = in.sigmac[di];
+    double sl = in.zero_line ? 0.0 : in.sigmal[di];
+    double scn = in.scont_local ? in.scont_local[j] : in.scont[di];
+    double sln = in.sline_local ? in.sline_local[j] : in.sline[di];
+    double ab = ac + al + sc + sl;
+    ab = ab > kEps38 ? ab : (ab != ab ? ab : kEps38);
+    abtot[j] = ab;
+    double sct = sc + sl;
+    double a = ab > 0.0 ? sct / ab : 0.0;
+    a = a < 0.0 ? 0.0 : (a > 1.0 ? 1.0 : a);
+    alpha[j] = a;
+    double dn = ac + al;
+    snubar[j] = dn > 0 ? (ac * scn + al * sln) / dn : scn;
+    rho[j] = in.rhox[idx[j]];
+  }
+  bool rev = n > 1 && rho[0] > rho[n - 1];
+  double start;
+  if (rev) {
+    for (int j = 0; j < n / 2; ++j) {
+      double t = rho[j]; rho[j] = rho[n - 1 - j]; rho[n - 1 - j] = t;
+      t = abtot[j]; abtot[j] = abtot[n - 1 - j]; abtot[n - 1 - j] = t;
+      t = snubar[j]; snubar[j] = snubar[n - 1 - j]; snubar[n - 1 - j] = t;
+      t = alpha[j]; alpha[j] = alpha[n - 1 - j]; alpha[n - 1 - j] = t;
+    }
+    start = abtot[n - 1] * rho[n - 1];
+  } else {
+    start = abtot[0] * rho[0];
+  }
+  // physics/josh_solver.py:170-189 (_integ)
+  parcoe(abtot, rho, n, pa, pb, pc);
+  taunu[0] = start;
+  for (int i = 0; i + 1 < n; ++i) {
+    double dx = rho[i + 1] - rho[i];
+    double term = pa[i] + 0.5 * pb[i] * (rho[i + 1] + rho[i]);
+    term += (pc[i] / 3.0) * ((rho[i + 1] + rho[i]) * rho[i + 1] + rho[i] * rho[i]);
+    taunu[i + 1] = taunu[i] + term * dx;
+  }
+  if (taunu[0] > tb.xtau[kNxtau - 1]) {
+    // label-401 branch (josh_solver.py:700-727): DERIV iteration on TAUNU.
+    double* snu = pa;
+    double* hnu = pb;
+    double* jnu = pc;
+    double* tmp = abtot;
+    for (int j = 0; j < n; ++j) snu[j] = snubar[j];
+    int it = 0;
+    for (int l = 0; l < kNxtau; ++l) {
+      deriv(taunu, snu, n, hnu);
+      for (int j = 0; j < n; ++j) hnu[j] = hnu[j] / 3.0;
+      deriv(taunu, hnu, n, jnu);
+      for (int j = 0; j < n; ++j) {
+        double jn = jnu[j] + snu[j];
+        double sn = (1.0 - alpha[j]) * snubar[j] + alpha[j] * jn;
+        double den = fabs(sn) > kEps38 ? fabs(sn) : kEps38;
+        tmp[j] = fabs(sn - snu[j]) / den;
+        snu[j] = sn;
+      }
+      // numpy pairwise summation (n <= 128: eight partial sums)
+      double err;
+      if (n < 8) {
+        err = 0.0;
+        for (int j = 0; j < n; ++j) err += tmp[j];
+      } else {
+        double r0 = tmp[0], r1 = tmp[1], r2 = tmp[2], r3 = tmp[3], r4 = tmp[4], r5 = tmp[5],
+               r6 = tmp[6], r7 = tmp[7];
+        int j = 8;
+        for (; j < n - (n % 8); j += 8) {
+          r0 += tmp[j]; r1 += tmp[j + 1]; r2 += tmp[j + 2]; r3 += tmp[j + 3];
+          r4 += tmp[j + 4]; r5 += tmp[j + 5]; r6 += tmp[j + 6]; r7 += tmp[j + 7];
+        }
+        err = ((r0 + r1) + (r2 + r3)) + ((r4 + r5) + (r6 + r7));
+        for (; j < n; ++j) err += tmp[j];
+      }
+      it = l + 1;
+      if (err < 1.0e-5) break;
+    }
+    *iters = it;
+    *took401 = 1;
+    return hnu[0];
+  }
+  *took401 = 0;
+  double xsbar[kNxtau], xalpha[kNxtau];
+  map1_xtau(taunu, snubar, n, tb.xtau, xsbar);
+  map1_xtau(taunu, alpha, n, tb.xtau, xalpha);
+  double s0 = snubar[0] > kEps38 ? snubar[0] : kEps38;
+  double a0 = alpha[0] < 0.0 ? 0.0 : (alpha[0] > 1.0 ? 1.0 : alpha[0]);
+  float xal32[kNxtau], xsb32[kNxtau], dg32[kNxtau];
+#pragma unroll
+  for (int k = 0; k < kNxtau; ++k) {
+    double xb = xsbar[k] > kEps38 ? xsbar[k] : kEps38;
+    double xa = xalpha[k] < 0.0 ? 0.0 : (xalpha[k] > 1.0 ? 1.0 : xalpha[k]);
+    if (tb.xtau[k] < taunu[0]) {
+      xb = s0;
+      xa = a0;
+    }
+    s_xs[k][lane] = (float)xb;
+    xal32[k] = (float)xa;
+    xsb32[k] = (float)(xb * (1.0 - xa));
+    dg32[k] = 1.0f - xal32[k] * tb.diag32[k];
+  }
+  // physics/josh_solver.py:24-92: float32 arrays, float64 accumulation.
+  const double tol = (double)(float)1.0e-5;
+  const double eps = (double)(float)kEps38;
+  int it = kNxtau;
+  for (int sweep = 0; sweep < kNxtau; ++sweep) {
+    bool err = false;
+#pragma unroll
+    for (int kr = 0; kr < kNxtau; ++kr) {
+      const int k = kNxtau - 1 - kr;
+      const float* row = tb.coefj32 + k * kNxtau;
+      double delxs = 0.0;
+      for (int m = 0; m < kNxtau; ++m) delxs += (double)(row[m] * s_xs[m][lane]);
+      float xk = s_xs[k][lane];
+      delxs = (delxs * (double)xal32[k] + (double)xsb32[k] - (double)xk) / (double)dg32[k];
+      double ex = xk != 0.0f ? fabs(delxs / (double)xk) : INFINITY;
+      if (ex > tol) err = true;
+      double nv = (double)xk + delxs;
+      if (eps > nv) nv = eps;
+      s_xs[k][lane] = (float)nv;
+    }
+    if (!err) {
+      it = sweep + 1;
+      break;
+    }
+  }
+  *iters = it;
+  double flux = 0.0;
+  for (int k = 0; k < kNxtau; ++k) flux += tb.ch[k] * (double)s_xs[k][lane];
+  return flux;
+}
+
+__device__ __forceinline__ double planck_one(double freq, double f3, double T) {
+  double hkt = kHPlanck / (kKBoltz * T);
+  double ehvkt = exp(-freq * hkt);
+  double stim = 1.0 - ehvkt;
+  double b = 1.47439e-2 * f3 * ehvkt / stim;
+  return b != b ? 0.0 : b;
+}
+
+constexpr int kRtLanes = 64;
+
+// engine/radiative.py:42-218 (solve_lte_frequency) per wavelength.
+__global__ __launch_bounds__(kRtLanes) void k_lte(int D, long W, const double* __restrict__ wl,
+                                                  const double* __restrict__ temp,
+                                                  const double* __restrict__ rhox,
+                                                  const double* __restrict__ ca,
+                                                  const double* __restrict__ cs,
+                                                  const double* __restrict__ la,
+                                                  const double* __restrict__ ls,
+                                                  const double* __restrict__ src,
+                                                  double* __restrict__ ft, double* __restrict__ fc,
+                                                  int32_t* __restrict__ iters, DeviceTables tb) {
+  __shared__ float s_xs[kNxtau][kRtLanes];
+  long w = blockIdx.x * (long)kRtLanes + threadIdx.x;
+  if (w >= W) return;
+  int lane = threadIdx.x;
+  const double maxr4 = 3.4e38;
+  int idx[kMaxDepth];
+  int n = 0;
+  for (int d = 0; d < D; ++d) {
+    long i = (long)d * W + w;
+    double r = rhox[d], a1 = ca[i], a2 = cs[i], a3 = la[i], a4 = ls[i];
+    bool okd = r >= 0.0 && isfinite(r) && isfinite(a1) && isfinite(a2) && isfinite(a3) &&
+               isfinite(a4) && a1 < maxr4 && a2 < maxr4 && a3 < maxr4 && a4 < maxr4;
+    if (okd && r > 0) idx[n++] = d;
+  }
+  if (n == 0) {
+    ft[w] = 0.0;
+    fc[w] = 0.0;
+    if (iters) iters[w] = 0;
+    return;
+  }
+  if (n > 1 && !(rhox[idx[0]] < rhox[idx[n - 1]])) {
+    for (int j = 0; j < n / 2; ++j) {
+      int t = idx[j];
+      idx[j] = idx[n - 1 - j];
+      idx[n - 1 - j] = t;
+    }
+  }
+  double wv = wl[w];
+  double freq = kCLightNm / (wv > 1e-12 ? wv : 1e-12);
+  double f3 = pow(freq / 1.0e15, 3.0);
+  double pl[kMaxDepth], sl[kMaxDepth];
+  for (int j = 0; j < n; ++j) {
+    int d = idx[j];
+    pl[j] = planck_one(freq, f3, temp[d]);
+    if (src) {
+      double s = src[(long)d * W + w];
+      sl[j] = isfinite(s) ? s : pl[j];
+    } else {
+      sl[j] = pl[j];
+    }
+  }
+  double abtot[kMaxDepth], alpha[kMaxDepth], snubar[kMaxDepth], rho[kMaxDepth], taunu[kMaxDepth];
+  double pa[kMaxDepth], pb[kMaxDepth], pc[kMaxDepth];
+  SolveIn cin{ca + w, nullptr, la + w, nullptr, cs + w, ls + w, rhox, W, pl, pl, true};
+  int it_c = 0, it_t = 0, b401c = 0, b401t = 0;
+  double fcont = josh_solve<kRtLanes>(cin, idx, n, tb, s_xs, lane, abtot, alpha, snubar, rho, taunu,
+                                      pa, pb, pc, &it_c, &b401c);
+  SolveIn tin{ca + w, nullptr, la + w, nullptr, cs + w, ls + w, rhox, W, pl, sl, false};
+  double ftot = josh_solve<kRtLanes>(tin, idx, n, tb, s_xs, lane, abtot, alpha, snubar, rho, taunu,
+                                     pa, pb, pc, &it_t, &b401t);
+  ft[w] = ftot;
+  fc[w] = fcont;
+  if (iters) iters[w] = it_c + it_t + (b401t ? 1000 : 0);
+}
+
+// solve_josh_flux over independent columns (physics/josh_solver.py:367-767).
+__global__ __launch_bounds__(kRtLanes) void k_josh_batch(int n, long ncol, const double* acont,
+                                                         const double* scont, const double* aline,
+                                                         const double* sline, const double* sigmac,
+                                                         const double* sigmal, const double* rhox,
+                                                         double* flux, int32_t* iters,
+                                                         DeviceTables tb) {
+  __shared__ float s_xs[kNxtau][kRtLanes];
+  long c = blockIdx.x * (long)kRtLanes + threadIdx.x;
+  if (c >= ncol) return;
+  int idx[kMaxDepth];
+  for (int j = 0; j < n; ++j) idx[j] = j;
+  double abtot[kMaxDepth], alpha[kMaxDepth], snubar[kMaxDepth], rho[kMaxDepth], taunu[kMaxDepth];
+  double pa[kMaxDepth], pb[kMaxDepth], pc[kMaxDepth];
+  SolveIn in{acont + c, scont + c, aline + c, sline + c, sigmac + c, sigmal + c, rhox, ncol,
+             nullptr, nullptr, false};
+  int it = 0, b401 = 0;
+  double f = 0.0;
+  if (n > 0)
+    f = josh_solve<kRtLanes>(in, idx, n, tb, s_xs, threadIdx.x, abtot, alpha, snubar, rho, taunu,
+                             pa, pb, pc, &it, &b401);
+  flux[c] = f;
+  if (iters) iters[c] = it + (b401 ? 1000 : 0);
+}
+
+__global__ void k_flux_to_lambda(long W, const double* __restrict__ wl, double* flux) {
+  long w = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (w >= W) return;
+  double w2 = wl[w] * wl[w];
+  flux[w] = flux[w] * (kCLightNm / (w2 > 1e-40 ? w2 : 1e-40));
+}
+
+unsigned int blocks_for(long n, int b) { return (unsigned int)((n + b - 1) / b); }
+
+}  // namespace
+
+// ----------------------------------------------------------------- context
+struct synthe_ctx {
+  int device = 0;
+  double* d_tab = nullptr;  // h0 h1 h2 extab extabf xtau ch
+  float* d_tab32 = nullptr; // coefj32 diag32
+  DeviceTables tb{};
+  std::vector<double> h0, h1, h2, extab, extabf;
+  void* ws = nullptr;
+  size_t ws_size = 0;
+};
+
+static int ensure_ws(synthe_ctx* ctx, size_t bytes) {
+  if (ctx->ws_size >= bytes) return SYNTHE_OK;
+  if (ctx->ws) {
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipFree(ctx->ws));
+    ctx->ws = nullptr;
+    ctx->ws_size = 0;
+  }
+  size_t sz = bytes + bytes / 8 + (1 << 20);
+  if (hipMalloc(&ctx->ws, sz) != hipSuccess) {
+    ctx->ws = nullptr;
+    return fail(SYNTHE_ENOMEM, "workspace allocation of " + std::to_string(sz) + " bytes failed");
+  }
+  ctx->ws_size = sz;
+  return SYNTHE_OK;
+}
+
+static size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
+
+extern "C" {
+
+const char* synthe_last_error(void) { return g_err.c_str(); }
+
+int synthe_abi_version(void) { return SYNTHE_ABI_VERSION; }
+
+int synthe_create(synthe_ctx** out) {
+  if (!out) return fail(SYNTHE_EINVAL, "synthe_create: out is NULL");
+  *out = nullptr;
+  synthe_ctx* ctx = new synthe_ctx();
+  if (hipGetDevice(&ctx->device) != hipSuccess) {
+    delete ctx;
+    return fail(SYNTHE_EHIP, "synthe_create: no HIP device");
+  }
+  // physics/tables.py:224-241 (Voigt) and 20-27 (FASTEX)
+  ctx->h0.assign(kVoigtN, 0.0);
+  ctx->h1.assign(kVoigtN, 0.0);
+  ctx->h2.assign(kVoigtN, 0.0);
+  for (int i = 0; i < kVoigtN; ++i) ctx->h0[i] = (double)i / 200.0;
+  map1_table(kTabvi, kTabh1, 81, ctx->h0.data(), ctx->h1.data(), kVoigtN);
+  for (int i = 0; i < kVoigtN; ++i) {
+    double q = (double)i / 200.0;
+    double vv = q * q;
+    ctx->h0[i] = exp(-vv);
+    ctx->h2[i] = ctx->h0[i] - (vv + vv) * ctx->h0[i];
+  }
+  ctx->extab.assign(kFastexN, 0.0);
+  ctx->extabf.assign(kFastexN, 0.0);
+  for (int i = 0; i < kFastexN; ++i) {
+    ctx->extab[i] = exp(-(double)i);
+    ctx->extabf[i] = exp(-(double)i * 0.001);
+  }
+  std::vector<double> host;
+  host.insert(host.end(), ctx->h0.begin(), ctx->h0.end());
+  host.insert(host.end(), ctx->h1.begin(), ctx->h1.end());
+  host.insert(host.end(), ctx->h2.begin(), ctx->h2.end());
+  host.insert(host.end(), ctx->extab.begin(), ctx->extab.end());
+  host.insert(host.end(), ctx->extabf.begin(), ctx->extabf.end());
+  host.insert(host.end(), kXtau, kXtau + kNxtau);
+  host.insert(host.end(), kCh, kCh + kNxtau);
+  std::vector<float> h32(kNxtau * kNxtau + kNxtau);
+  for (int i = 0; i < kNxtau * kNxtau; ++i) h32[i] = (float)kCoefj[i];
+  for (int k = 0; k < kNxtau; ++k) h32[kNxtau * kNxtau + k] = (float)kCoefj[k * kNxtau + k];
+  if (hipMalloc(&ctx->d_tab, host.size() * sizeof(double)) != hipSuccess ||
+      hipMalloc(&ctx->d_tab32, h32.size() * sizeof(float)) != hipSuccess) {
+    synthe_destroy(ctx);
+    return fail(SYNTHE_ENOMEM, "synthe_create: table allocation failed");
+  }
+  if (hipMemcpy(ctx->d_tab, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(ctx->d_tab32, h32.data(), h32.size() * sizeof(float), hipMemcpyHostToDevice) !=
+          hipSuccess) {
+    synthe_destroy(ctx);
+    return fail(SYNTHE_EHIP, "synthe_create: table upload failed");
+  }
+  double* p = ctx->d_tab;
+  ctx->tb.h0 = p;
+  ctx->tb.h1 = p + kVoigtN;
+  ctx->tb.h2 = p + 2 * kVoigtN;
+  ctx->tb.extab = p + 3 * kVoigtN;
+  ctx->tb.extabf = p + 3 * kVoigtN + kFastexN;
+  ctx->tb.xtau = p + 3 * kVoigtN + 2 * kFastexN;
+  ctx->tb.ch = ctx->tb.xtau + kNxtau;
+  ctx->tb.coefj32 = ctx->d_tab32;
+  ctx->tb.diag32 = ctx->d_tab32 + kNxtau * kNxtau;
+  *out = ctx;
+  return SYNTHE_OK;
+}
+
+int synthe_destroy(synthe_ctx* ctx) {
+  if (!ctx) return SYNTHE_OK;
+  if (ctx->d_tab) (void)hipFree(ctx->d_tab);
+  if (ctx->d_tab32) (void)hipFree(ctx->d_tab32);
+  if (ctx->ws) (void)hipFree(ctx->ws);
+  delete ctx;
+  return SYNTHE_OK;
+}
+
+int synthe_copy_tables(const synthe_ctx* ctx, double* h0, double* h1, double* h2, double* extab,
+                       double* extabf) {
+  if (!ctx) return fail(SYNTHE_EINVAL, "synthe_copy_tables: ctx is NULL");
+  if (h0) memcpy(h0, ctx->h0.data(), kVoigtN * sizeof(double));
+  if (h1) memcpy(h1, ctx->h1.data(), kVoigtN * sizeof(double));
+  if (h2) memcpy(h2, ctx->h2.data(), kVoigtN * sizeof(double));
+  if (extab) memcpy(extab, ctx->extab.data(), kFastexN * sizeof(double));
+  if (extabf) memcpy(extabf, ctx->extabf.data(), kFastexN * sizeof(double));
+  return SYNTHE_OK;
+}
+
+long synthe_wavelength_grid(double start, double end, double resolution, double* out, long cap) {
+  if (!(resolution > 0.0)) return fail(SYNTHE_EINVAL, "resolution must be positive");
+  if (!(start > 0.0) || !(end >= start)) return fail(SYNTHE_EINVAL, "need 0 < start <= end");
+  double ratio = 1.0 + 1.0 / resolution;
+  double rlog = log(ratio);
+  double ix_floor = floor(log(start) / rlog);
+  if (exp(ix_floor * rlog) < start) ix_floor += 1.0;
+  double wl = exp(ix_floor * rlog);
+  long n = 0;
+  double lim = end * (1.0 + 1e-9);
+  while (wl <= lim) {
+    if (out && n < cap) out[n] = wl;
+    ++n;
+    wl *= ratio;
+  }
+  return n;
+}
+
+int synthe_continuum(synthe_ctx* ctx, long n_depths, long n_wl, const double* wl, long n_edges,
+                     const double* wledge, const double* half_edge, const double* delta_edge,
+                     const double* coeff_abs, const double* coeff_scat, int coeff_log10,
+                     double* cont_abs, double* cont_scat, synthe_stream stream) {
+  if (!ctx) return fail(SYNTHE_EINVAL, "synthe_continuum: ctx is NULL");
+  if (n_depths <= 0 || n_wl < 0 || n_edges < 2)
+    return fail(SYNTHE_EINVAL, "synthe_continuum: need n_depths > 0 and at least two edges");
+  if (n_wl == 0) return SYNTHE_OK;
+  if (!wl || !wledge || !coeff_abs || !coeff_scat || !cont_abs || !cont_scat)
+    return fail(SYNTHE_EINVAL, "synthe_continuum: NULL array");
+  hipLaunchKernelGGL(k_continuum, dim3(blocks_for(n_wl, kBlock)), dim3(kBlock), 0, (hipStream_t)stream,
+                     (int)n_depths, n_wl, wl, (int)n_edges, wledge, half_edge, delta_edge, coeff_abs,
+                     coeff_scat, coeff_log10, cont_abs, cont_scat);
+  HIP_TRY(hipGetLastError());
+  return SYNTHE_OK;
+}
+
+int synthe_source_function(synthe_ctx* ctx, long n_depths, long n_wl, const double* wl,
+                           const double* temperature, double* bnu, double* stim, double* ehvkt,
+                           double* slinec, synthe_stream stream) {
+  if (!ctx || !wl || !temperature) return fail(SYNTHE_EINVAL, "synthe_source_function: NULL argument");
+  if (n_depths <= 0 || n_wl < 0 || n_depths > 65535)
+    return fail(SYNTHE_EINVAL, "synthe_source_function: bad sizes");
+  if (n_wl == 0) return SYNTHE_OK;
+  hipLaunchKernelGGL(k_source, dim3(blocks_for(n_wl, kBlock), (unsigned)n_depths), dim3(kBlock), 0,
+                     (hipStream_t)stream, (int)n_depths, n_wl, wl, temperature, bnu, stim, ehvkt, slinec);
+  HIP_TRY(hipGetLastError());
+  return SYNTHE_OK;
+}
+
+static int check_lines_depths(const synthe_lines* ln, const synthe_depths* dp, const char* who) {
+  if (!ln || !dp) return fail(SYNTHE_EINVAL, std::string(who) + ": lines/depths is NULL");
+  if (ln->n_lines < 0 || dp->n_depths <= 0 || dp->n_depths > 65535 || dp->n_ion <= 0 || dp->n_elem <= 0)
+    return fail(SYNTHE_EINVAL, std::string(who) + ": bad sizes");
+  if (ln->n_lines > 0 &&
+      (!ln->wl || !ln->gf || !ln->cgf || !ln->gamma_rad || !ln->gamma_stark || !ln->gamma_vdw ||
+       !ln->elo || !ln->elem || !ln->ion || !ln->ltype || !ln->process || !ln->center_t ||
+       !ln->center_w))
+    return fail(SYNTHE_EINVAL, std::string(who) + ": NULL line array");
+  if (!dp->temperature || !dp->rhox || !dp->rho || !dp->xne || !dp->txnxn || !dp->hckt || !dp->pop ||
+      !dp->dop)
+    return fail(SYNTHE_EINVAL, std::string(who) + ": NULL depth array");
+  return SYNTHE_OK;
+}
+
+static LinesDev to_dev(const synthe_lines* l) {
+  return LinesDev{l->n_lines, l->wl,   l->gf,     l->cgf,   l->gamma_rad, l->gamma_stark, l->gamma_vdw,
+                  l->elo,     l->elem, l->ion,    l->ltype, l->process,   l->center_t,    l->center_w};
+}
+
+static DepthsDev to_dev(const synthe_depths* d) {
+  return DepthsDev{(int)d->n_depths, d->temperature, d->rhox, d->rho, d->xne, d->txnxn, d->hckt,
+                   d->pop,           d->dop,         (int)d->n_ion, (int)d->n_elem};
+}
+
+int synthe_transp(synthe_ctx* ctx, const synthe_lines* lines, const synthe_depths* depths, long n_wl,
+                  const double* kapmin, double cutoff, double microturb_kms, double* transp,
+                  uint8_t* valid, synthe_stream stream) {
+  if (!ctx) return fail(SYNTHE_EINVAL, "synthe_transp: ctx is NULL");
+  int rc = check_lines_depths(lines, depths, "synthe_transp");
+  if (rc) return rc;
+  if (n_wl <= 0 || !kapmin || !transp || !valid)
+    return fail(SYNTHE_EINVAL, "synthe_transp: kapmin/transp/valid required and n_wl > 0");
+  long pairs = lines->n_lines * depths->n_depths;
+  if (pairs == 0) return SYNTHE_OK;
+  double micro = microturb_kms > 0.0 ? microturb_kms / 299792.458 : 0.0;
+  hipLaunchKernelGGL(k_transp_wing, dim3(blocks_for(pairs, kBlock)), dim3(kBlock), 0,
+                     (hipStream_t)stream, to_dev(lines), to_dev(depths), n_wl, kapmin, cutoff, micro,
+                     300000.0, ctx->tb, 1, transp, valid, nullptr, nullptr, nullptr, nullptr);
+  HIP_TRY(hipGetLastError());
+  return SYNTHE_OK;
+}
+
+int synthe_asynth(synthe_ctx* ctx, const synthe_lines* lines, const synthe_depths* depths, long n_wl,
+                  const double* wl, const double* transp, const uint8_t* valid, const double* kapmin,
+                  double cutoff, const int64_t* center_order, const int64_t* center_start,
+                  const int64_t* wing_order, double* asynth, double* stats, synthe_stream stream) {
+  if (!ctx) return fail(SYNTHE_EINVAL, "synthe_asynth: ctx is NULL");
+  int rc = check_lines_depths(lines, depths, "synthe_asynth");
+  if (rc) return rc;
+  if (n_wl <= 0 || !wl || !asynth || !center_start)
+    return fail(SYNTHE_EINVAL, "synthe_asynth: wl/asynth/center_start required and n_wl > 0");
+  long L = lines->n_lines;
+  int D = (int)depths->n_depths;
+  if (L > 0 && (!transp || !valid || !kapmin || !center_order || !wing_order))
+    return fail(SYNTHE_EINVAL, "synthe_asynth: NULL transp/valid/kapmin/order array");
+  hipStream_t st = (hipStream_t)stream;
+  // resolu from the grid (line_opacity.py:233-238) is computed on the host
+  // from the first two grid points.
+  double g01[2] = {0.0, 0.0};
+  double resolu = 300000.0;
+  if (n_wl > 1) {
+    HIP_TRY(hipMemcpyAsync(g01, wl, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    double ratio = g01[1] / g01[0];
+    if (ratio > 1.0) resolu = 1.0 / (ratio - 1.0);
+  }
+  long pairs = L * (long)D;
+  long nblk = (L + kBlock - 1) / kBlock;
+  size_t off = 0;
+  size_t o_wp = off; off = align_up(off + (size_t)pairs * sizeof(WingParams));
+  size_t o_cls = off; off = align_up(off + (size_t)pairs);
+  size_t o_cnt = off; off = align_up(off + (size_t)(nblk > 0 ? nblk : 1) * D * kClasses * sizeof(int32_t));
+  size_t o_tot = off; off = align_up(off + (size_t)D * kClasses * sizeof(int64_t));
+  size_t o_reg = off; off = align_up(off + (size_t)D * (kClasses + 1) * sizeof(int64_t));
+  size_t o_lcm = off; off = align_up(off + (size_t)pairs * sizeof(int2));
+  size_t o_lp = off; off = align_up(off + (size_t)pairs * sizeof(WingParams));
+  size_t o_ev = off; off = align_up(off + 4 * sizeof(unsigned long long));
+  rc = ensure_ws(ctx, off);
+  if (rc) return rc;
+  char* base = (char*)ctx->ws;
+  WingParams* wp = (WingParams*)(base + o_wp);
+  uint8_t* cls = (uint8_t*)(base + o_cls);
+  int32_t* cnt = (int32_t*)(base + o_cnt);
+  int64_t* tot = (int64_t*)(base + o_tot);
+  int64_t* reg = (int64_t*)(base + o_reg);
+  int2* lcm = (int2*)(base + o_lcm);
+  WingParams* lp = (WingParams*)(base + o_lp);
+  unsigned long long* ev = (unsigned long long*)(base + o_ev);
+  HIP_TRY(hipMemsetAsync(ev, 0, 4 * sizeof(unsigned long long), st));
+  if (pairs > 0) {
+    hipLaunchKernelGGL(k_transp_wing, dim3(blocks_for(pairs, kBlock)), dim3(kBlock), 0, st,
+                       to_dev(lines), to_dev(depths), n_wl, kapmin, cutoff, 0.0, resolu, ctx->tb, 2,
+                       nullptr, nullptr, transp, valid, wp, cls);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_class_count, dim3((unsigned)nblk), dim3(kBlock), 0, st, L, D, wing_order, cls,
+                       cnt);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_class_scan, dim3((unsigned)(D * kClasses)), dim3(kBlock), 0, st, nblk, D, cnt,
+                       tot);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_regions, dim3(blocks_for(D, 64)), dim3(64), 0, st, D, L, tot, reg);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_class_scatter, dim3((unsigned)nblk), dim3(kBlock), 0, st, L, D, wing_order, cls,
+                       cnt, reg, wp, lcm, lp);
+    HIP_TRY(hipGetLastError());
+  } else {
+    HIP_TRY(hipMemsetAsync(reg, 0, (size_t)D * (kClasses + 1) * sizeof(int64_t), st));
+  }
+  hipLaunchKernelGGL(k_gather, dim3(blocks_for(n_wl, kBlock), (unsigned)D), dim3(kBlock), 0, st, n_wl, D,
+                     center_order, center_start, transp, valid, lcm, lp, reg, wl, depths->temperature,
+                     ctx->tb, asynth, stats ? ev + 3 : nullptr);
+  HIP_TRY(hipGetLastError());
+  if (stats) {
+    if (pairs > 0) {
+      hipLaunchKernelGGL(k_wing_stats, dim3(64, (unsigned)D), dim3(kBlock), 0, st, pairs, D, L, reg, lcm,
+                         ev);
+      HIP_TRY(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_stats_to_double, dim3(1), dim3(64), 0, st, ev, stats);
+    HIP_TRY(hipGetLastError());
+  }
+  return SYNTHE_OK;
+}
+
+int synthe_lte_spectrum(synthe_ctx* ctx, long n_depths, long n_wl, const double* wl,
+                        const double* temperature, const double* rhox, const double* cont_abs,
+                        const double* cont_scat, const double* line_abs, const double* line_scat,
+                        const double* line_source, double* flux_total, double* flux_cont,
+                        int32_t* iters, synthe_stream stream) {
+  if (!ctx) return fail(SYNTHE_EINVAL, "synthe_lte_spectrum: ctx is NULL");
+  if (n_depths <= 0 || n_wl < 0) return fail(SYNTHE_EINVAL, "synthe_lte_spectrum: bad sizes");
+  if (n_depths > kMaxDepth)
+    return fail(SYNTHE_ERANGE, "synthe_lte_spectrum: more than SYNTHE_MAX_DEPTH depths");
+  if (n_wl == 0) return SYNTHE_OK;
+  if (!wl || !temperature || !rhox || !cont_abs || !cont_scat || !line_abs || !line_scat ||
+      !flux_total || !flux_cont)
+    return fail(SYNTHE_EINVAL, "synthe_lte_spectrum: NULL array");
+  hipLaunchKernelGGL(k_lte, dim3(blocks_for(n_wl, kRtLanes)), dim3(kRtLanes), 0, (hipStream_t)stream,
+                     (int)n_depths, n_wl, wl, temperature, rhox, cont_abs, cont_scat, line_abs, line_scat,
+                     line_source, flux_total, flux_cont, iters, ctx->tb);
+  HIP_TRY(hipGetLastError());
+  return SYNTHE_OK;
+}
+
+int synthe_josh_flux_batch(synthe_ctx* ctx, long n, long n_cols, const double* acont,
+                           const double* scont, const double* aline, const double* sline,
+                           const double* sigmac, const double* sigmal, const double* rhox,
+                           double* flux, int32_t* iters, synthe_stream stream) {
+  if (!ctx) return fail(SYNTHE_EINVAL, "synthe_josh_flux_batch: ctx is NULL");
+  if (n < 0 || n_cols < 0) return fail(SYNTHE_EINVAL, "synthe_josh_flux_batch: bad sizes");
+  if (n > kMaxDepth) return fail(SYNTHE_ERANGE, "synthe_josh_flux_batch: more than SYNTHE_MAX_DEPTH depths");
+  if (n_cols == 0) return SYNTHE_OK;
+  if (!flux || (n > 0 && (!acont || !scont || !aline || !sline || !sigmac || !sigmal || !rhox)))
+    return fail(SYNTHE_EINVAL, "synthe_josh_flux_batch: NULL array");
+  hipLaunchKernelGGL(k_josh_batch, dim3(blocks_for(n_cols, kRtLanes)), dim3(kRtLanes), 0,
+                     (hipStream_t)stream, (int)n, n_cols, acont, scont, aline, sline, sigmac, sigmal, rhox,
+                     flux, iters, ctx->tb);
+  HIP_TRY(hipGetLastError());
+  return SYNTHE_OK;
+}
+
+int synthe_flux_to_lambda(synthe_ctx* ctx, long n_wl, const double* wl, double* flux,
+                          synthe_stream stream) {
+  if (!ctx || !wl || !flux) return fail(SYNTHE_EINVAL, "synthe_flux_to_lambda: NULL argument");
+  if (n_wl <= 0) return n_wl == 0 ? SYNTHE_OK : fail(SYNTHE_EINVAL, "synthe_flux_to_lambda: bad size");
+  hipLaunchKernelGGL(k_flux_to_lambda, dim3(blocks_for(n_wl, kBlock)), dim3(kBlock), 0, (hipStream_t)stream,
+                     n_wl, wl, flux);
+  HIP_TRY(hipGetLastError());
+  return SYNTHE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,250 @@
+"""Device-resident spectrum plans: inputs uploaded once, every stage on the GPU.
+
+A SpectrumPlan holds one atmosphere, one line list and one wavelength grid in
+HBM and runs the hot path (continuum -> source function -> TRANSP -> ASYNTH ->
+JOSH RT -> unit conversion) as a chain of libsynthe_hip launches on the
+current torch stream.  PyTorch is used only to own device memory and the stream.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import Dict, Optional
+
+import numpy as np
+
+from . import _native as N
+from .atmosphere import AtmosphereModel
+from .lines import LineCatalog, LineSoA, build_line_soa
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def to_device(a: np.ndarray, dtype=None):
+    torch = _torch()
+    arr = np.ascontiguousarray(a if dtype is None else np.asarray(a, dtype=dtype))
+    return torch.from_numpy(arr).to("cuda", non_blocking=False)
+
+
+def center_csr(center_t: np.ndarray, n_wl: int):
+    """Lines stably sorted by centre index and per-pixel offsets (host prep)."""
+    order = np.argsort(center_t, kind="stable").astype(np.int64)
+    ct_sorted = center_t[order]
+    start = np.searchsorted(ct_sorted, np.arange(n_wl + 1), side="left").astype(np.int64)
+    return order, start
+
+
+@dataclass
+class DepthArrays:
+    temperature: object
+    rhox: object
+    rho: object
+    xne: object
+    txnxn: object
+    hckt: object
+    pop: object
+    dop: object
+    n_ion: int
+    n_elem: int
+    wledge: object
+    half_edge: Optional[object]
+    delta_edge: Optional[object]
+    coeff_abs: object
+    coeff_scat: object
+    coeff_log10: bool
+
+    @property
+    def n(self) -> int:
+        return int(self.temperature.shape[0])
+
+    def struct(self) -> N.Depths:
+        p = N.ptr
+        return N.Depths(self.n, p(self.temperature), p(self.rhox), p(self.rho), p(self.xne),
+                        p(self.txnxn), p(self.hckt), p(self.pop), p(self.dop), self.n_ion,
+                        self.n_elem)
+
+
+def upload_atmosphere(atm: AtmosphereModel) -> DepthArrays:
+    if atm.population_per_ion is None or atm.doppler_per_ion is None:
+        raise ValueError("atmosphere bundle lacks population_per_ion / doppler_per_ion")
+    if atm.continuum_wledge is None or atm.continuum_abs_coeff is None or atm.continuum_scat_coeff is None:
+        raise ValueError("Continuum coefficients are missing from atmosphere model "
+                         "(need wledge, cont_abs_coeff, cont_scat_coeff)")
+    f8 = np.float64
+    wledge = np.asarray(atm.continuum_wledge, f8)
+    E = wledge.size
+    if E < 2:
+        raise ValueError("Continuum edge table requires at least two wavelength entries")
+    half = atm.continuum_half_edge
+    delta = atm.continuum_delta_edge
+    pop = np.asarray(atm.population_per_ion, f8)
+    ca = np.asarray(atm.continuum_abs_coeff, f8)
+    cs = np.asarray(atm.continuum_scat_coeff, f8)
+    if ca.shape[1] != E - 1 or cs.shape[1] != E - 1:
+        raise ValueError("Continuum coefficient grid does not match edge table dimensions")
+    return DepthArrays(
+        temperature=to_device(atm.temperature, f8),
+        rhox=to_device(atm.depth, f8),
+        rho=to_device(atm.mass_density, f8),
+        xne=to_device(atm.electron_density, f8),
+        txnxn=to_device(atm.txnxn_or_default(), f8),
+        hckt=to_device(atm.hckt_or_default(), f8),
+        pop=to_device(pop, f8),
+        dop=to_device(atm.doppler_per_ion, f8),
+        n_ion=int(pop.shape[1]),
+        n_elem=int(pop.shape[2]),
+        wledge=to_device(wledge, f8),
+        half_edge=to_device(half, f8) if half is not None and np.size(half) == E - 1 else None,
+        delta_edge=to_device(delta, f8) if delta is not None and np.size(delta) == E - 1 else None,
+        coeff_abs=to_device(ca, f8),
+        coeff_scat=to_device(cs, f8),
+        coeff_log10=bool(atm.continuum_coeff_log10),
+    )
+
+
+@dataclass
+class LineArrays:
+    soa: LineSoA
+    t: Dict[str, object]
+    corder: object
+    cstart: object
+    worder: object
+
+    @property
+    def n(self) -> int:
+        return len(self.soa)
+
+    def struct(self) -> N.Lines:
+        p = N.ptr
+        t = self.t
+        return N.Lines(self.n, p(t["wl"]), p(t["gf"]), p(t["cgf"]), p(t["gamma_rad"]),
+                       p(t["gamma_stark"]), p(t["gamma_vdw"]), p(t["elo"]), p(t["elem"]),
+                       p(t["ion"]), p(t["ltype"]), p(t["process"]), p(t["center_t"]),
+                       p(t["center_w"]))
+
+
+def upload_lines(soa: LineSoA, n_wl: int) -> LineArrays:
+    t = {k: to_device(getattr(soa, k)) for k in ("wl", "gf", "cgf", "gamma_rad", "gamma_stark",
+                                                  "gamma_vdw", "elo", "elem", "ion", "ltype",
+                                                  "process", "center_t", "center_w")}
+    corder, cstart = center_csr(soa.center_t, n_wl)
+    worder = np.argsort(soa.center_w, kind="stable").astype(np.int64)
+    return LineArrays(soa, t, to_device(corder), to_device(cstart), to_device(worder))
+
+
+class SpectrumPlan:
+    """One spectrum's device-resident inputs, buffers and stage launches."""
+
+    def __init__(self, atm: AtmosphereModel, catalog: LineCatalog, wl_start: float, wl_end: float,
+                 resolution: float, cutoff: float = 1e-3, microturb_kms: float = 0.0,
+                 rhoxj: float = 0.0, grid: Optional[np.ndarray] = None):
+        torch = _torch()
+        self.ctx = N.context()
+        self.grid = N.wavelength_grid(wl_start, wl_end, resolution) if grid is None else np.asarray(grid)
+        self.W = int(self.grid.size)
+        self.D = atm.layers
+        self.cutoff = float(cutoff)
+        self.microturb = float(microturb_kms)
+        self.rhoxj = float(rhoxj)
+        self.atm = atm
+        self.depth = upload_atmosphere(atm)
+        self.soa = build_line_soa(catalog, self.grid, self.depth.n_elem,
+                                  grid_origin=float(self.grid[0]) if self.W else None)
+        self.lines = upload_lines(self.soa, self.W)
+        f8 = torch.float64
+        D, W, L = self.D, self.W, len(self.soa)
+        dev = "cuda"
+        self.d_grid = to_device(self.grid, np.float64)
+        self.cont_abs = torch.empty((D, W), dtype=f8, device=dev)
+        self.cont_scat = torch.empty((D, W), dtype=f8, device=dev)
+        self.kapmin = torch.empty((D, W), dtype=f8, device=dev)
+        self.slinec = torch.empty((D, W), dtype=f8, device=dev)
+        self.transp = torch.empty((L, D), dtype=f8, device=dev)
+        self.valid = torch.empty((L, D), dtype=torch.uint8, device=dev)
+        self.asynth = torch.empty((D, W), dtype=f8, device=dev)
+        self.flux_hz = torch.empty(W, dtype=f8, device=dev)
+        self.cont_hz = torch.empty(W, dtype=f8, device=dev)
+        self.flux = torch.empty(W, dtype=f8, device=dev)
+        self.cont = torch.empty(W, dtype=f8, device=dev)
+        self.iters = torch.empty(W, dtype=torch.int32, device=dev)
+        self.stats = torch.zeros(4, dtype=f8, device=dev)
+        rx = np.asarray(atm.depth, np.float64)
+        fscat = np.exp(-rx / self.rhoxj) if self.rhoxj > 0.0 else np.zeros_like(rx)
+        self.fscat = fscat
+        self.one_minus_fscat = to_device((1.0 - fscat)[:, None], np.float64)
+        self.fscat_col = to_device(fscat[:, None], np.float64)
+        self.line_abs = self.asynth
+        self.line_scat = torch.zeros((D, W), dtype=f8, device=dev) if self.rhoxj <= 0.0 else \
+            torch.empty((D, W), dtype=f8, device=dev)
+
+    # ---------------------------------------------------------------- stages
+    def run_continuum(self, stream=None):
+        s = stream if stream is not None else N.stream_handle()
+        d = self.depth
+        N.check("synthe_continuum", self.ctx.lib.synthe_continuum(
+            self.ctx.handle, self.D, self.W, N.ptr(self.d_grid), int(d.wledge.shape[0]), N.ptr(d.wledge),
+            N.ptr(d.half_edge), N.ptr(d.delta_edge), N.ptr(d.coeff_abs), N.ptr(d.coeff_scat),
+            int(d.coeff_log10), N.ptr(self.cont_abs), N.ptr(self.cont_scat), s))
+        self.kapmin.copy_(self.cont_abs)
+        self.kapmin.add_(self.cont_scat)
+        N.check("synthe_source_function", self.ctx.lib.synthe_source_function(
+            self.ctx.handle, self.D, self.W, N.ptr(self.d_grid), N.ptr(d.temperature), None, None,
+            None, N.ptr(self.slinec), s))
+
+    def run_transp(self, stream=None):
+        s = stream if stream is not None else N.stream_handle()
+        ls, ds = self.lines.struct(), self.depth.struct()
+        N.check("synthe_transp", self.ctx.lib.synthe_transp(
+            self.ctx.handle, ct_byref(ls), ct_byref(ds), self.W, N.ptr(self.kapmin), self.cutoff,
+            self.microturb, N.ptr(self.transp), N.ptr(self.valid), s))
+
+    def run_asynth(self, stream=None, stats: bool = False):
+        s = stream if stream is not None else N.stream_handle()
+        ls, ds = self.lines.struct(), self.depth.struct()
+        N.check("synthe_asynth", self.ctx.lib.synthe_asynth(
+            self.ctx.handle, ct_byref(ls), ct_byref(ds), self.W, N.ptr(self.d_grid),
+            N.ptr(self.transp), N.ptr(self.valid), N.ptr(self.kapmin), self.cutoff,
+            N.ptr(self.lines.corder), N.ptr(self.lines.cstart), N.ptr(self.lines.worder),
+            N.ptr(self.asynth), N.ptr(self.stats) if stats else None, s))
+        if self.rhoxj > 0.0:
+            self.line_scat.copy_(self.asynth * self.fscat_col)
+            self.line_abs = self.asynth * self.one_minus_fscat
+
+    def run_rt(self, stream=None):
+        s = stream if stream is not None else N.stream_handle()
+        d = self.depth
+        N.check("synthe_lte_spectrum", self.ctx.lib.synthe_lte_spectrum(
+            self.ctx.handle, self.D, self.W, N.ptr(self.d_grid), N.ptr(d.temperature), N.ptr(d.rhox),
+            N.ptr(self.cont_abs), N.ptr(self.cont_scat), N.ptr(self.line_abs), N.ptr(self.line_scat),
+            N.ptr(self.slinec), N.ptr(self.flux_hz), N.ptr(self.cont_hz), N.ptr(self.iters), s))
+        self.flux.copy_(self.flux_hz)
+        self.cont.copy_(self.cont_hz)
+        for t in (self.flux, self.cont):
+            N.check("synthe_flux_to_lambda", self.ctx.lib.synthe_flux_to_lambda(
+                self.ctx.handle, self.W, N.ptr(self.d_grid), N.ptr(t), s))
+
+    def run(self, stats: bool = False):
+        if self.W == 0:
+            return
+        self.run_continuum()
+        if len(self.soa) > 0:
+            self.run_transp()
+            self.run_asynth(stats=stats)
+        else:
+            self.asynth.zero_()
+        self.run_rt()
+
+    def result(self):
+        return dict(wavelength=self.grid.copy(), flux=self.flux.cpu().numpy(),
+                    continuum=self.cont.cpu().numpy(), flux_hz=self.flux_hz.cpu().numpy(),
+                    cont_hz=self.cont_hz.cpu().numpy())
+
+
+def ct_byref(s):
+    import ctypes
+
+    return ctypes.byref(s)

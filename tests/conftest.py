@@ -1,0 +1,58 @@
+"""Shared fixtures: the oracle (CPU checker), seeded workloads, GPU gating."""
+
+from __future__ import annotations
+
+import os
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "oracle"))
+
+CACHE = ROOT / "_work" / "test_cache" if (ROOT / "_work").exists() else None
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device) and libsynthe_hip.so")
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    import oracle as O
+
+    O.build()
+    return O
+
+
+def make_small(name="small", teff=5750.0, logg=4.44, n_lines=2000, wl0=500.0, wl1=501.0, seed=7,
+               n_layers=72, mh=0.0):
+    from kurucz_py_joss_submission_amd import synthetic as s
+
+    return s.make_workload(name, teff, logg, n_lines, wl0, wl1, seed=seed, n_layers=n_layers,
+                           cache_dir=CACHE, mh=mh)
+
+
+@pytest.fixture(scope="session")
+def small_workload():
+    return make_small()
+
+
+@pytest.fixture(scope="session")
+def c1_workload():
+    """Config 1: Teff 5750, log g 4.44, 72 layers, 20,000 lines over 500-510 nm."""
+    return make_small("c1", 5750.0, 4.44, 20000, 500.0, 510.0, seed=1)
+
+
+@pytest.fixture(scope="session")
+def gpu():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from kurucz_py_joss_submission_amd import _native
+
+    return _native.context()
