@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "_tables_gen.h"
+#include "rt_core.h"
 #include "synthe_device.h"
 #include "synthe_hip.h"
 
@@ -723,8 +724,12 @@ __device__ __forceinline__ double planck_one(double freq, double f3, double T) {
 
 constexpr int kRtLanes = 64;
 
-// engine/radiative.py:42-218 (solve_lte_frequency) per wavelength.
-__global__ __launch_bounds__(kRtLanes) void k_lte(int D, long W, const double* __restrict__ wl,
+// engine/radiative.py:42-218 (solve_lte_frequency) per wavelength, general
+// form with per-lane depth arrays: runs only for wavelengths the streaming
+// kernels flag (masked or non-increasing depths, fewer than 4 depths, or a
+// label-401 solve).  flags == NULL runs every wavelength.
+__global__ __launch_bounds__(kRtLanes) void k_lte(int D, long W, const uint8_t* __restrict__ flags,
+                                                  const double* __restrict__ wl,
                                                   const double* __restrict__ temp,
                                                   const double* __restrict__ rhox,
                                                   const double* __restrict__ ca,
@@ -737,6 +742,7 @@ __global__ __launch_bounds__(kRtLanes) void k_lte(int D, long W, const double* _
   __shared__ float s_xs[kNxtau][kRtLanes];
   long w = blockIdx.x * (long)kRtLanes + threadIdx.x;
   if (w >= W) return;
+  if (flags && flags[w] == 0) return;
   int lane = threadIdx.x;
   const double maxr4 = 3.4e38;
   int idx[kMaxDepth];
@@ -787,6 +793,175 @@ __global__ __launch_bounds__(kRtLanes) void k_lte(int D, long W, const double* _
   ft[w] = ftot;
   fc[w] = fcont;
   if (iters) iters[w] = it_c + it_t + (b401t ? 1000 : 0);
+}
+
+// ----------------------------------------------------- streaming RT kernels
+// Per-depth values of one solve_josh_flux call read straight from the [D][W]
+// inputs (josh_solver.py:411-441): abtot, alpha, snubar; rho = rhox.
+struct ColSrc {
+  const double *ca, *cs, *la, *ls, *src, *temp, *rhox;
+  long W;
+  double freq, f3;
+  bool total;  // false: continuum-only solve (aline = sigmal = 0, sline = B)
+  __device__ __forceinline__ void get(int j, double* f, double* x, double* sb, double* al) {
+    long i = (long)j * W;
+    double ac = ca[i], sc = cs[i];
+    double aln = total ? la[i] : 0.0;
+    double sln = total ? ls[i] : 0.0;
+    double B = planck_one(freq, f3, temp[j]);
+    double sl = B;
+    if (total && src) {
+      double s = src[i];
+      sl = isfinite(s) ? s : B;
+    }
+    double ab = ac + aln + sc + sln;
+    ab = ab > kEps38 ? ab : (ab != ab ? ab : kEps38);
+    double a = ab > 0.0 ? (sc + sln) / ab : 0.0;
+    a = a < 0.0 ? 0.0 : (a > 1.0 ? 1.0 : a);
+    double dn = ac + aln;
+    *f = ab;
+    *x = rhox[j];
+    *sb = dn > 0 ? (ac * B + aln * sl) / dn : B;
+    *al = a;
+  }
+};
+
+// MAP1 results -> the float32 iteration inputs (josh_solver.py:617-680):
+// XSBAR = max(., EPS), XALPHA clipped, surface masking for XTAU < TAUNU(0).
+struct ColOut {
+  float *xs0, *xal, *xsbm;  // [51][W], column w already applied
+  long W;
+  const double* xtau;
+  double tau0, s0, a0;
+  __device__ __forceinline__ void put(int k, double xsbar, double xalpha) {
+    double xb = xsbar > kEps38 ? xsbar : kEps38;
+    double xa = xalpha < 0.0 ? 0.0 : (xalpha > 1.0 ? 1.0 : xalpha);
+    if (xtau[k] < tau0) {
+      xb = s0;
+      xa = a0;
+    }
+    long i = (long)k * W;
+    xs0[i] = (float)xb;
+    xal[i] = (float)xa;
+    xsbm[i] = (float)(xb * (1.0 - xa));
+  }
+};
+
+struct TauOnlyOut {
+  __device__ __forceinline__ void put(int, double, double) {}
+};
+
+constexpr int kRtBlock = 256;
+
+// Fast-path set-up of both JOSH solves per wavelength: validity check, then
+// the streaming INTEG + MAP1.  flags[w]: 1 = needs the general kernel.
+__global__ __launch_bounds__(kRtBlock) void k_rt_prep(int D, long W, const double* __restrict__ wl,
+                                                      const double* __restrict__ temp,
+                                                      const double* __restrict__ rhox,
+                                                      const double* __restrict__ ca,
+                                                      const double* __restrict__ cs,
+                                                      const double* __restrict__ la,
+                                                      const double* __restrict__ ls,
+                                                      const double* __restrict__ src,
+                                                      const double* __restrict__ xtau, float* ws,
+                                                      uint8_t* __restrict__ flags) {
+  long w = blockIdx.x * (long)kRtBlock + threadIdx.x;
+  if (w >= W) return;
+  const double maxr4 = 3.4e38;
+  bool ok = D >= 4 && rhox[0] < rhox[D - 1];
+  for (int d = 0; d < D && ok; ++d) {
+    long i = (long)d * W + w;
+    double r = rhox[d], a1 = ca[i], a2 = cs[i], a3 = la[i], a4 = ls[i];
+    ok = r > 0.0 && isfinite(r) && isfinite(a1) && isfinite(a2) && isfinite(a3) && isfinite(a4) &&
+         a1 < maxr4 && a2 < maxr4 && a3 < maxr4 && a4 < maxr4;
+  }
+  if (!ok) {
+    flags[w] = 1;
+    return;
+  }
+  double wv = wl[w];
+  double freq = kCLightNm / (wv > 1e-12 ? wv : 1e-12);
+  double f3 = pow(freq / 1.0e15, 3.0);
+  uint8_t fl = 0;
+  for (int s = 0; s < 2; ++s) {
+    ColSrc in{ca + w, cs + w, la + w, ls + w, src ? src + w : nullptr, temp, rhox, W, freq, f3, s == 1};
+    double f0, x0, sb0, al0;
+    in.get(0, &f0, &x0, &sb0, &al0);
+    float* base = ws + (long)s * 3 * kNxtau * W + w;
+    ColOut out{base, base + (long)kNxtau * W, base + 2L * kNxtau * W, W, xtau, 0.0,
+               sb0 > kEps38 ? sb0 : kEps38, al0 < 0.0 ? 0.0 : (al0 > 1.0 ? 1.0 : al0)};
+    out.tau0 = f0 * x0;
+    if (out.tau0 > xtau[kNxtau - 1]) {
+      fl = 1;  // label 401: general kernel
+      break;
+    }
+    double tau0;
+    rt::integ_map1_stream(D, xtau, in, out, &tau0);
+  }
+  flags[w] = fl;
+}
+
+// Gauss-Seidel scattering iteration (josh_solver.py:24-92, Numba's float32
+// arrays with float64 accumulation) and flux = CH . XS, both solves.
+__global__ __launch_bounds__(kRtLanes) void k_rt_iter(long W, const uint8_t* __restrict__ flags,
+                                                      const float* __restrict__ ws,
+                                                      double* __restrict__ ft,
+                                                      double* __restrict__ fc,
+                                                      int32_t* __restrict__ iters, DeviceTables tb) {
+  __shared__ float s_xs[kNxtau][kRtLanes];
+  __shared__ float s_cj[kNxtau * kNxtau + kNxtau];  // COEFJ rows, then its diagonal
+  for (int i = threadIdx.x; i < kNxtau * kNxtau; i += kRtLanes) s_cj[i] = tb.coefj32[i];
+  if (threadIdx.x < kNxtau) s_cj[kNxtau * kNxtau + threadIdx.x] = tb.diag32[threadIdx.x];
+  __syncthreads();
+  long w = blockIdx.x * (long)kRtLanes + threadIdx.x;
+  if (w >= W) return;
+  if (flags[w] != 0) return;
+  const int lane = threadIdx.x;
+  const double tol = (double)(float)1.0e-5;
+  const double eps = (double)(float)kEps38;
+  int it_sum = 0;
+  for (int s = 0; s < 2; ++s) {
+    const float* base = ws + (long)s * 3 * kNxtau * W + w;
+    float xal[kNxtau], xsb[kNxtau];
+#pragma unroll
+    for (int k = 0; k < kNxtau; ++k) {
+      s_xs[k][lane] = base[(long)k * W];
+      xal[k] = base[(long)(kNxtau + k) * W];
+      xsb[k] = base[(long)(2 * kNxtau + k) * W];
+    }
+    int it = kNxtau;
+    for (int sweep = 0; sweep < kNxtau; ++sweep) {
+      bool err = false;
+#pragma unroll
+      for (int kr = 0; kr < kNxtau; ++kr) {
+        const int k = kNxtau - 1 - kr;
+        // compiler-only fence: keeps the COEFJ row reads inside the sweep
+        // instead of hoisting all 2601 of them into registers
+        __asm__ volatile("" ::: "memory");
+        const float* row = s_cj + k * kNxtau;
+        double delxs = 0.0;
+#pragma unroll 3
+        for (int m = 0; m < kNxtau; ++m) delxs += (double)(row[m] * s_xs[m][lane]);
+        const float dg = 1.0f - xal[k] * s_cj[kNxtau * kNxtau + k];
+        const float xk = s_xs[k][lane];
+        delxs = (delxs * (double)xal[k] + (double)xsb[k] - (double)xk) / (double)dg;
+        double ex = xk != 0.0f ? fabs(delxs / (double)xk) : INFINITY;
+        if (ex > tol) err = true;
+        double nv = (double)xk + delxs;
+        if (eps > nv) nv = eps;
+        s_xs[k][lane] = (float)nv;
+      }
+      if (!err) {
+        it = sweep + 1;
+        break;
+      }
+    }
+    double flux = 0.0;
+    for (int k = 0; k < kNxtau; ++k) flux += tb.ch[k] * (double)s_xs[k][lane];
+    (s == 0 ? fc : ft)[w] = flux;
+    it_sum += it;
+  }
+  if (iters) iters[w] = it_sum;
 }
 
 // solve_josh_flux over independent columns (physics/josh_solver.py:367-767).
@@ -1130,9 +1305,22 @@ int synthe_lte_spectrum(synthe_ctx* ctx, long n_depths, long n_wl, const double*
   if (!wl || !temperature || !rhox || !cont_abs || !cont_scat || !line_abs || !line_scat ||
       !flux_total || !flux_cont)
     return fail(SYNTHE_EINVAL, "synthe_lte_spectrum: NULL array");
-  hipLaunchKernelGGL(k_lte, dim3(blocks_for(n_wl, kRtLanes)), dim3(kRtLanes), 0, (hipStream_t)stream,
-                     (int)n_depths, n_wl, wl, temperature, rhox, cont_abs, cont_scat, line_abs, line_scat,
-                     line_source, flux_total, flux_cont, iters, ctx->tb);
+  hipStream_t st = (hipStream_t)stream;
+  size_t need = align_up((size_t)2 * 3 * kNxtau * n_wl * sizeof(float)) + align_up((size_t)n_wl);
+  int rc = ensure_ws(ctx, need);
+  if (rc) return rc;
+  float* ws = (float*)ctx->ws;
+  uint8_t* flags = (uint8_t*)ctx->ws + align_up((size_t)2 * 3 * kNxtau * n_wl * sizeof(float));
+  hipLaunchKernelGGL(k_rt_prep, dim3(blocks_for(n_wl, kRtBlock)), dim3(kRtBlock), 0, st, (int)n_depths,
+                     n_wl, wl, temperature, rhox, cont_abs, cont_scat, line_abs, line_scat, line_source,
+                     ctx->tb.xtau, ws, flags);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_rt_iter, dim3(blocks_for(n_wl, kRtLanes)), dim3(kRtLanes), 0, st, n_wl, flags, ws,
+                     flux_total, flux_cont, iters, ctx->tb);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_lte, dim3(blocks_for(n_wl, kRtLanes)), dim3(kRtLanes), 0, st, (int)n_depths, n_wl,
+                     flags, wl, temperature, rhox, cont_abs, cont_scat, line_abs, line_scat, line_source,
+                     flux_total, flux_cont, iters, ctx->tb);
   HIP_TRY(hipGetLastError());
   return SYNTHE_OK;
 }

@@ -278,12 +278,14 @@ def lte_spectrum(grid, atm, cont_abs, cont_scat, line_abs, line_scat, line_sourc
     D, W = atm.layers, grid.size
     ft, fc = np.zeros(W), np.zeros(W)
     iters = np.zeros(1)
+    iw = np.zeros(W, dtype=np.int32)
     args = [_c(a, np.float64) for a in (cont_abs, cont_scat, line_abs, line_scat)]
     ls = _c(line_source, np.float64) if line_source is not None else None
     lib().orc_lte_spectrum(ct.c_long(D), ct.c_long(W), p(_c(grid, np.float64)),
                            p(_c(atm.temperature, np.float64)), p(_c(atm.depth, np.float64)),
                            *[p(a) for a in args], p(ls) if ls is not None else None,
-                           p(ft), p(fc), p(iters))
+                           p(ft), p(fc), p(iters), p(iw, _I32))
+    lte_spectrum.last_iters = iw
     return ft, fc, float(iters[0])
 
 
@@ -353,7 +355,7 @@ def synthesize(atm, catalog, wl_start, wl_end, resolution, cutoff=1e-3, microtur
     stages = {}
     if keep_stages:
         stages = dict(cont_abs=ca, cont_scat=cs, transp=tr, valid=valid, asynth=asy, slinec=slinec,
-                      center_t=soa.center_t, center_w=soa.center_w)
+                      center_t=soa.center_t, center_w=soa.center_w, iters=lte_spectrum.last_iters)
     return OracleResult(grid, flux, cont, ft, fc, stages, st, tm)
 
 

@@ -1015,11 +1015,12 @@ double orc_lte_frequency(long D, long W, long w, double wavelength, const double
 }
 
 /* engine/radiative.py:282-494 (solve_lte_spectrum), wavelength-parallel.
- * iters (may be NULL) gets the summed JOSH iterations of both solves. */
+ * iters (may be NULL) gets the summed JOSH iterations of both solves; iters_w
+ * (may be NULL) per wavelength, +1000 when the total solve took label 401. */
 void orc_lte_spectrum(long D, long W, const double *wl, const double *temp, const double *rhox,
                       const double *cont_abs, const double *cont_scat, const double *line_abs,
                       const double *line_scat, const double *line_source, double *flux_total,
-                      double *flux_cont, double *iters) {
+                      double *flux_cont, double *iters, int32_t *iters_w) {
     double itsum = 0.0;
 #pragma omp parallel for schedule(dynamic, 64) reduction(+ : itsum)
     for (long w = 0; w < W; ++w) {
@@ -1029,6 +1030,7 @@ void orc_lte_spectrum(long D, long W, const double *wl, const double *temp, cons
                                           line_abs, line_scat, line_source, &fc, info2);
         flux_cont[w] = fc;
         itsum += info2[0] + info2[1];
+        if (iters_w) iters_w[w] = info2[0] + info2[1] + (info2[2] >= 2 ? 1000 : 0);
     }
     if (iters) *iters = itsum;
 }
