@@ -1,0 +1,181 @@
+"""run_synthesis: the reference's orchestrator for the hot path, on the GPU.
+
+Mirrors synthe_py/config.py (SynthesisConfig and friends),
+engine/buffers.py:36-43 (SynthResult), engine/opacity.py:2084-3914
+(run_synthesis, hot-path stages only) and io/export.py:12-21
+(write_spec_file).  The atmosphere must be a bundle .npz (the .atm -> .npz
+converter is an offline step of the reference); the line list is a GFALL
+file, or a cached .npz of a parsed catalog.
+"""
+
+from __future__ import annotations
+
+import logging
+import time
+from dataclasses import dataclass, field
+from pathlib import Path
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from . import lines as L
+from .atmosphere import AtmosphereModel, load_cached
+
+logger = logging.getLogger(__name__)
+
+
+@dataclass
+class WavelengthGrid:
+    start: float
+    end: float
+    resolution: float
+    velocity_microturb: float = 0.0
+    vacuum: bool = True
+
+
+@dataclass
+class LineDataConfig:
+    atomic_catalog: Path
+    molecular_catalogs: List[Path] = field(default_factory=list)
+    include_predicted: bool = False
+    cache_directory: Optional[Path] = None
+    allow_tfort_runtime: bool = False
+
+
+@dataclass
+class AtmosphereInput:
+    model_path: Path
+    format: str = "atlas12"
+    npz_path: Optional[Path] = None
+
+
+@dataclass
+class OutputConfig:
+    spec_path: Path
+    diagnostics_path: Optional[Path] = None
+
+
+@dataclass
+class SynthesisConfig:
+    """config.py:54-134 (SynthesisConfig)."""
+
+    wavelength_grid: WavelengthGrid
+    line_data: LineDataConfig
+    atmosphere: AtmosphereInput
+    output: OutputConfig
+    cutoff: float = 1e-3
+    linout: int = 30
+    nlte: bool = False
+    scattering_iterations: int = 8
+    scattering_tolerance: float = 1e-3
+    rhoxj_scale: float = 0.0
+    log_level: str = "INFO"
+    enable_helium_wings: bool = True
+    skip_hydrogen_wings: bool = False
+    line_filter: bool = True
+    wavelength_subsample: int = 1
+    n_workers: Optional[int] = None
+
+    @classmethod
+    def from_cli(cls, spec_path: Path, diagnostics_path: Optional[Path], atmosphere_path: Path,
+                 atomic_catalog: Path, wl_start: float, wl_end: float, resolution: float,
+                 velocity_microturb: float = 0.0, vacuum: bool = True, cutoff: float = 1e-3,
+                 linout: int = 30, nlte: bool = False, scattering_iterations: int = 8,
+                 scattering_tolerance: float = 1e-3, rhoxj_scale: float = 0.0,
+                 enable_helium_wings: bool = True, skip_hydrogen_wings: bool = False,
+                 line_filter: bool = True, wavelength_subsample: int = 1,
+                 npz_path: Optional[Path] = None, n_workers: Optional[int] = None,
+                 allow_tfort_runtime: bool = False) -> "SynthesisConfig":
+        return cls(
+            wavelength_grid=WavelengthGrid(wl_start, wl_end, resolution, velocity_microturb, vacuum),
+            line_data=LineDataConfig(atomic_catalog=atomic_catalog, allow_tfort_runtime=allow_tfort_runtime),
+            atmosphere=AtmosphereInput(model_path=atmosphere_path, npz_path=npz_path),
+            output=OutputConfig(spec_path=spec_path, diagnostics_path=diagnostics_path),
+            cutoff=cutoff, linout=linout, nlte=nlte, scattering_iterations=scattering_iterations,
+            scattering_tolerance=scattering_tolerance, rhoxj_scale=rhoxj_scale,
+            enable_helium_wings=enable_helium_wings, skip_hydrogen_wings=skip_hydrogen_wings,
+            line_filter=line_filter, wavelength_subsample=wavelength_subsample, n_workers=n_workers)
+
+
+@dataclass
+class SynthResult:
+    """engine/buffers.py:36-43."""
+
+    wavelength: np.ndarray
+    intensity: np.ndarray
+    continuum: np.ndarray
+    timings: Dict[str, float] = field(default_factory=dict)
+
+
+def write_spec_file(result: SynthResult, destination: Path) -> None:
+    """io/export.py:12-21: three columns, '%15.8f %11.6E %11.6E'."""
+    data = np.column_stack((result.wavelength, result.intensity, result.continuum))
+    np.savetxt(destination, data, fmt="%15.8f %11.6E %11.6E")
+
+
+def _load_atmosphere(cfg: SynthesisConfig) -> AtmosphereModel:
+    p = cfg.atmosphere.npz_path or cfg.atmosphere.model_path
+    p = Path(p)
+    if p.suffix != ".npz":
+        cand = p.with_suffix(".npz")
+        if not cand.exists():
+            raise FileNotFoundError(
+                f"Could not find cached .npz atmosphere file for {p}; convert the atmosphere with "
+                "the bundle builder (kurucz_py_joss_submission_amd.synthetic.build_bundle) or pass --npz")
+        p = cand
+    return load_cached(p)
+
+
+def _load_catalog(cfg: SynthesisConfig) -> L.LineCatalog:
+    p = Path(cfg.line_data.atomic_catalog)
+    cat = L.LineCatalog.load_npz(p) if p.suffix == ".npz" else L.load_catalog(p)
+    if cfg.line_filter:
+        cat = L.filter_by_range(cat, cfg.wavelength_grid.start, cfg.wavelength_grid.end)
+    return cat
+
+
+def run_synthesis(cfg: SynthesisConfig) -> SynthResult:
+    """engine/opacity.py:2084-3914, hot-path stages, all on the GPU."""
+    from .device import SpectrumPlan
+
+    timings: Dict[str, float] = {}
+    t_all = time.perf_counter()
+    t = time.perf_counter()
+    atm = _load_atmosphere(cfg)
+    timings["atmosphere load"] = time.perf_counter() - t
+    if cfg.wavelength_subsample != 1:
+        raise NotImplementedError("wavelength_subsample > 1 is not on the hot path")
+    t = time.perf_counter()
+    cat = _load_catalog(cfg)
+    timings["line catalog"] = time.perf_counter() - t
+    logger.info("Catalog: %d lines", len(cat))
+    import torch
+
+    t = time.perf_counter()
+    wg = cfg.wavelength_grid
+    plan = SpectrumPlan(atm, cat, wg.start, wg.end, wg.resolution, cutoff=cfg.cutoff,
+                        microturb_kms=wg.velocity_microturb, rhoxj=cfg.rhoxj_scale)
+    torch.cuda.synchronize()
+    timings["upload"] = time.perf_counter() - t
+    t = time.perf_counter()
+    plan.run()
+    torch.cuda.synchronize()
+    timings["spectrum (GPU)"] = time.perf_counter() - t
+    r = plan.result()
+    res = SynthResult(wavelength=r["wavelength"], intensity=r["flux"], continuum=r["continuum"],
+                      timings=timings)
+    if cfg.output.spec_path is not None:
+        write_spec_file(res, Path(cfg.output.spec_path))
+    if cfg.output.diagnostics_path is not None:
+        Path(cfg.output.diagnostics_path).parent.mkdir(parents=True, exist_ok=True)
+        np.savez(cfg.output.diagnostics_path, wavelength=r["wavelength"],
+                 continuum_absorption=plan.cont_abs.cpu().numpy(),
+                 continuum_scattering=plan.cont_scat.cpu().numpy(),
+                 line_opacity=plan.line_abs.cpu().numpy(),
+                 line_scattering=plan.line_scat.cpu().numpy(),
+                 line_source=plan.slinec.cpu().numpy(), flux_total=r["flux"],
+                 flux_continuum=r["continuum"])
+    timings["total pipeline"] = time.perf_counter() - t_all
+    for k, v in timings.items():
+        logger.info("Timing: %s in %.3fs", k, v)
+    return res

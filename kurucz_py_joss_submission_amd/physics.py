@@ -1,0 +1,234 @@
+"""Drop-in replacements for the reference's hot-path physics functions.
+
+Same names, argument meaning, return shapes and error behaviour as the
+reference (paths relative to its synthe_py/ package); the compute runs in
+libsynthe_hip on the GPU.  Inputs and outputs are numpy arrays, so callers of
+the reference can switch module imports without other changes:
+
+* compute_transp                 physics/line_opacity.py:620-860
+* compute_asynth_from_transp     physics/line_opacity.py:946-1340
+* build_depth_continuum          physics/continuum.py:73-303
+* solve_josh_flux                physics/josh_solver.py:367-767
+* solve_lte_frequency            engine/radiative.py:42-218
+* solve_lte_spectrum             engine/radiative.py:282-494
+* compute_bfudge_and_slinec      physics/bfudge.py:312-406 (LTE: BFUDGE = 1)
+
+The Boltzmann factors and TXNXN that the reference takes from a Populations
+object (physics/populations.py:131-240) are computed on the device from the
+atmosphere's HCKT and TXNXN (or its xnf_h/xnf_he1/xnf_h2 fallback), so the
+``populations`` arguments are accepted and ignored.
+"""
+
+from __future__ import annotations
+
+from typing import Optional, Tuple
+
+import numpy as np
+
+from . import _native as N
+from .atmosphere import AtmosphereModel
+from .lines import LineCatalog, build_line_soa, nearest_grid_indices
+
+
+def _t():
+    import torch
+
+    return torch
+
+
+def _dev(a, dtype=np.float64):
+    from .device import to_device
+
+    return to_device(np.asarray(a, dtype=dtype), dtype)
+
+
+def _host(t) -> np.ndarray:
+    return t.cpu().numpy()
+
+
+def build_depth_continuum(atmosphere: AtmosphereModel, wavelength_grid: np.ndarray):
+    """physics/continuum.py:73-303 -> (absorption[D,W], scattering[D,W], None, None)."""
+    from .device import upload_atmosphere
+
+    torch = _t()
+    wl = np.asarray(wavelength_grid, dtype=np.float64)
+    d = upload_atmosphere(atmosphere)
+    D, W = atmosphere.layers, wl.size
+    ca = torch.empty((D, W), dtype=torch.float64, device="cuda")
+    cs = torch.empty((D, W), dtype=torch.float64, device="cuda")
+    ctx = N.context()
+    N.check("synthe_continuum", ctx.lib.synthe_continuum(
+        ctx.handle, D, W, N.ptr(_dev(wl)), int(d.wledge.shape[0]), N.ptr(d.wledge),
+        N.ptr(d.half_edge), N.ptr(d.delta_edge), N.ptr(d.coeff_abs), N.ptr(d.coeff_scat),
+        int(d.coeff_log10), N.ptr(ca), N.ptr(cs), N.stream_handle()))
+    return _host(ca), _host(cs), None, None
+
+
+def compute_transp(catalog: LineCatalog, populations, atmosphere: AtmosphereModel,
+                   cutoff: float = 1e-3, continuum_absorption: Optional[np.ndarray] = None,
+                   wavelength_grid: Optional[np.ndarray] = None,
+                   continuum_absorption_full: Optional[np.ndarray] = None,
+                   wavelength_grid_full: Optional[np.ndarray] = None,
+                   microturb_kms: float = 0.0) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """physics/line_opacity.py:620-860 -> (transp[L,D], valid_mask[L,D], center_indices[L])."""
+    if continuum_absorption is None or wavelength_grid is None:
+        raise ValueError("continuum_absorption and wavelength_grid are required for compute_transp. "
+                         "Fortran always uses KAPMIN = CONTINUUM * CUTOFF with no fallback.")
+    if continuum_absorption_full is not None and wavelength_grid_full is not None:
+        raise NotImplementedError("full-grid KAPMIN (continuum_absorption_full) is not on the hot path")
+    from .device import upload_atmosphere, upload_lines
+
+    torch = _t()
+    wl = np.asarray(wavelength_grid, dtype=np.float64)
+    d = upload_atmosphere(atmosphere)
+    soa = build_line_soa(catalog, wl, d.n_elem)
+    lines = upload_lines(soa, wl.size)
+    L, D = len(soa), atmosphere.layers
+    tr = torch.zeros((L, D), dtype=torch.float64, device="cuda")
+    va = torch.zeros((L, D), dtype=torch.uint8, device="cuda")
+    if L:
+        ctx = N.context()
+        ls, ds = lines.struct(), d.struct()
+        import ctypes
+
+        N.check("synthe_transp", ctx.lib.synthe_transp(
+            ctx.handle, ctypes.byref(ls), ctypes.byref(ds), wl.size,
+            N.ptr(_dev(continuum_absorption)), float(cutoff), float(microturb_kms), N.ptr(tr),
+            N.ptr(va), N.stream_handle()))
+    return _host(tr), _host(va).astype(bool), soa.center_t.copy()
+
+
+def compute_asynth_from_transp(transp: np.ndarray, catalog: LineCatalog, atmosphere: AtmosphereModel,
+                               wavelength_grid: np.ndarray, valid_mask: Optional[np.ndarray] = None,
+                               populations=None, cutoff: float = 1e-3,
+                               continuum_absorption: Optional[np.ndarray] = None,
+                               continuum_absorption_full: Optional[np.ndarray] = None,
+                               wavelength_grid_full: Optional[np.ndarray] = None,
+                               metal_tables=None, grid_origin: Optional[float] = None) -> np.ndarray:
+    """physics/line_opacity.py:946-1340 -> asynth[D,W] (stimulated emission applied).
+
+    Without continuum_absorption the cutoff is off, as in the reference: near
+    wings run to N10DOP and far wings are skipped (KAPMIN_REF = 0)."""
+    if continuum_absorption_full is not None and wavelength_grid_full is not None:
+        raise NotImplementedError("full-grid KAPMIN (continuum_absorption_full) is not on the hot path")
+    if metal_tables is not None:
+        raise NotImplementedError("WCON/WTAIL merged-continuum tapers (fort.19 lines) are out of scope")
+    from .device import upload_atmosphere, upload_lines
+
+    torch = _t()
+    wl = np.asarray(wavelength_grid, dtype=np.float64)
+    D, W = atmosphere.layers, wl.size
+    d = upload_atmosphere(atmosphere)
+    soa = build_line_soa(catalog, wl, d.n_elem, grid_origin=grid_origin)
+    lines = upload_lines(soa, W)
+    L = len(soa)
+    tr = np.asarray(transp, dtype=np.float64)
+    if tr.shape != (L, D):
+        raise ValueError(f"transp has shape {tr.shape}, expected {(L, D)}")
+    va = np.ones((L, D), dtype=np.uint8) if valid_mask is None else np.asarray(valid_mask, dtype=np.uint8)
+    use_cutoff = continuum_absorption is not None and np.shape(continuum_absorption) == (D, W)
+    kapmin = np.asarray(continuum_absorption, np.float64) if use_cutoff else np.zeros((D, W))
+    out = torch.empty((D, W), dtype=torch.float64, device="cuda")
+    import ctypes
+
+    ctx = N.context()
+    ls, ds = lines.struct(), d.struct()
+    N.check("synthe_asynth", ctx.lib.synthe_asynth(
+        ctx.handle, ctypes.byref(ls), ctypes.byref(ds), W, N.ptr(_dev(wl)), N.ptr(_dev(tr)),
+        N.ptr(_dev(va, np.uint8)), N.ptr(_dev(kapmin)), float(cutoff) if use_cutoff else 0.0,
+        N.ptr(lines.corder), N.ptr(lines.cstart), N.ptr(lines.worder), N.ptr(out), None,
+        N.stream_handle()))
+    return _host(out)
+
+
+def compute_bfudge_and_slinec(atmosphere: AtmosphereModel, params, bnu: np.ndarray, stim: np.ndarray,
+                              ehvkt: np.ndarray, *, clip=None):
+    """physics/bfudge.py:312-406 in LTE (PH1 = PC1 = PSI1 = 0): BFUDGE = 1,
+    SLINEC = BNU * STIM / (BFUDGE - EHVKT).  Elementwise on the host inputs."""
+    for name in ("ph1", "pc1", "psi1"):
+        if params is not None and float(getattr(params, name, 0.0)) != 0.0:
+            raise NotImplementedError("NLTE departure exponents (PH1/PC1/PSI1 != 0) are out of scope")
+    layers = bnu.shape[0]
+    bfudge = np.ones(layers, dtype=np.float64)
+    with np.errstate(over="ignore", under="ignore", divide="ignore", invalid="ignore"):
+        slinec = (bnu * stim) / (bfudge[:, None] - ehvkt)
+    return bfudge, slinec
+
+
+def source_function(wavelength_grid: np.ndarray, temperature: np.ndarray):
+    """engine/opacity.py:2215-2228 on the device -> (bnu, stim, ehvkt, slinec) [D,W]."""
+    torch = _t()
+    wl = np.asarray(wavelength_grid, np.float64)
+    T = np.asarray(temperature, np.float64)
+    D, W = T.size, wl.size
+    outs = [torch.empty((D, W), dtype=torch.float64, device="cuda") for _ in range(4)]
+    ctx = N.context()
+    N.check("synthe_source_function", ctx.lib.synthe_source_function(
+        ctx.handle, D, W, N.ptr(_dev(wl)), N.ptr(_dev(T)), *[N.ptr(o) for o in outs],
+        N.stream_handle()))
+    return tuple(_host(o) for o in outs)
+
+
+def solve_lte_spectrum(wavelength_nm: np.ndarray, temperature: np.ndarray, column_mass: np.ndarray,
+                       cont_abs: np.ndarray, cont_scat: np.ndarray, line_opacity: np.ndarray,
+                       line_scattering: np.ndarray, line_source: Optional[np.ndarray] = None,
+                       n_workers: Optional[int] = None) -> Tuple[np.ndarray, np.ndarray]:
+    """engine/radiative.py:282-494 -> (flux_total[W], flux_cont[W]) per Hz.
+
+    n_workers is accepted for signature compatibility; the GPU solves every
+    wavelength in parallel."""
+    torch = _t()
+    wl = np.asarray(wavelength_nm, np.float64)
+    W = wl.size
+    D = int(np.asarray(temperature).size)
+    ft = torch.empty(W, dtype=torch.float64, device="cuda")
+    fc = torch.empty(W, dtype=torch.float64, device="cuda")
+    if W == 0:
+        return np.zeros(0), np.zeros(0)
+    ctx = N.context()
+    arrs = [_dev(a) for a in (cont_abs, cont_scat, line_opacity, line_scattering)]
+    ls = _dev(line_source) if line_source is not None else None
+    N.check("synthe_lte_spectrum", ctx.lib.synthe_lte_spectrum(
+        ctx.handle, D, W, N.ptr(_dev(wl)), N.ptr(_dev(temperature)), N.ptr(_dev(column_mass)),
+        *[N.ptr(a) for a in arrs], N.ptr(ls), N.ptr(ft), N.ptr(fc), None, N.stream_handle()))
+    return _host(ft), _host(fc)
+
+
+def solve_lte_frequency(wavelength_nm: float, temperature: np.ndarray, column_mass: np.ndarray,
+                        cont_abs: np.ndarray, cont_scat: np.ndarray, line_opacity: np.ndarray,
+                        line_scattering: np.ndarray, line_source: Optional[np.ndarray] = None
+                        ) -> Tuple[float, float]:
+    """engine/radiative.py:42-218 for one wavelength -> (flux_total, flux_cont)."""
+    col = lambda a: None if a is None else np.asarray(a, np.float64).reshape(-1, 1)  # noqa: E731
+    ft, fc = solve_lte_spectrum(np.array([wavelength_nm], np.float64), temperature, column_mass,
+                                col(cont_abs), col(cont_scat), col(line_opacity), col(line_scattering),
+                                col(line_source))
+    return float(ft[0]), float(fc[0])
+
+
+def solve_josh_flux(acont: np.ndarray, scont: np.ndarray, aline: np.ndarray, sline: np.ndarray,
+                    sigmac: np.ndarray, sigmal: np.ndarray, column_mass: np.ndarray) -> float:
+    """physics/josh_solver.py:367-767 for one frequency -> HNU (surface flux)."""
+    a = np.asarray(acont, np.float64)
+    if a.size == 0:
+        return 0.0
+    return float(solve_josh_flux_batch(*(np.asarray(x, np.float64).reshape(-1, 1) for x in
+                                         (acont, scont, aline, sline, sigmac, sigmal)),
+                                       np.asarray(column_mass, np.float64))[0])
+
+
+def solve_josh_flux_batch(acont, scont, aline, sline, sigmac, sigmal, column_mass) -> np.ndarray:
+    """solve_josh_flux over columns: inputs [n, ncol], column_mass [n] -> flux[ncol]."""
+    torch = _t()
+    n, ncol = np.shape(acont)
+    out = torch.empty(ncol, dtype=torch.float64, device="cuda")
+    ctx = N.context()
+    devs = [_dev(x) for x in (acont, scont, aline, sline, sigmac, sigmal, column_mass)]
+    N.check("synthe_josh_flux_batch", ctx.lib.synthe_josh_flux_batch(
+        ctx.handle, n, ncol, *[N.ptr(x) for x in devs], N.ptr(out), None, N.stream_handle()))
+    return _host(out)
+
+
+def nearest_grid_indices_ref(grid, values):
+    """engine/opacity.py:550-592 (host)."""
+    return nearest_grid_indices(np.asarray(grid, np.float64), np.asarray(values, np.float64))

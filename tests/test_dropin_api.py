@@ -1,0 +1,92 @@
+"""The reference-signature drop-ins (numpy in, numpy out) against the oracle, on the GPU.
+
+Each function keeps the reference's name, arguments and error behaviour
+(physics/line_opacity.py:620-630, 946-959; physics/continuum.py:73;
+physics/josh_solver.py:367; engine/radiative.py:282; engine/opacity.py:2084).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+
+def _rel(a, b, floor=1e-300):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b)) / np.maximum(np.abs(np.asarray(b)), floor)))
+
+
+def test_missing_continuum_raises_like_reference(small_workload):
+    """compute_transp raises ValueError without continuum/grid (line_opacity.py:685-689)."""
+    from kurucz_py_joss_submission_amd import physics
+
+    with pytest.raises(ValueError):
+        physics.compute_transp(small_workload.catalog, None, small_workload.atmosphere)
+
+
+@pytest.mark.gpu
+def test_physics_dropins_match_oracle(gpu, oracle, small_workload):
+    from kurucz_py_joss_submission_amd import lines as L
+    from kurucz_py_joss_submission_amd import physics
+
+    wk = small_workload
+    atm = wk.atmosphere
+    grid = oracle.wavelength_grid(wk.wl_start, wk.wl_end, wk.resolution)
+    ca, cs, _, _ = physics.build_depth_continuum(atm, grid)
+    oca, ocs = oracle.continuum(atm, grid)
+    assert _rel(ca, oca) < 1e-14 and _rel(cs, ocs) < 1e-14
+    km = oca + ocs
+    tr, va, ci = physics.compute_transp(wk.catalog, None, atm, 1e-3, km, grid)
+    soa = L.build_line_soa(wk.catalog, grid, atm.population_per_ion.shape[2], grid_origin=float(grid[0]))
+    otr, ova = oracle.transp(soa, atm, grid, km)
+    assert np.array_equal(va, ova) and _rel(tr, otr) < 1e-14
+    assert np.array_equal(ci, oracle.nearest_grid_indices(grid, soa.wl))
+    asy = physics.compute_asynth_from_transp(otr, wk.catalog, atm, grid, ova, None, 1e-3, km,
+                                             grid_origin=float(grid[0]))
+    oasy, _ = oracle.asynth(soa, atm, grid, otr, ova, km)
+    assert _rel(asy, oasy) < 1e-12
+    _, _, _, sl = oracle.freq_quantities(grid, atm.temperature)
+    ft, fc = physics.solve_lte_spectrum(grid, atm.temperature, atm.depth, oca, ocs, oasy,
+                                        np.zeros_like(oasy), sl)
+    oft, ofc, _ = oracle.lte_spectrum(grid, atm, oca, ocs, oasy, np.zeros_like(oasy), sl)
+    assert _rel(ft, oft) < 1e-12 and _rel(fc, ofc) < 1e-12
+    w = grid.size // 3
+    f1 = physics.solve_josh_flux(oca[:, w], sl[:, w], oasy[:, w], sl[:, w], ocs[:, w],
+                                 np.zeros(atm.layers), atm.depth)
+    # solve_josh_flux with SCONT = SLINE = SLINEC differs from the LTE spectrum's
+    # SCONT = Planck only through the source term; check it against the oracle directly
+    of1, _, _ = oracle.solve_josh_flux(oca[:, w], sl[:, w], oasy[:, w], sl[:, w], ocs[:, w],
+                                       np.zeros(atm.layers), atm.depth)
+    assert abs(f1 - of1) <= 1e-12 * abs(of1)
+    # no-cutoff ASYNTH (continuum_absorption=None): near wings only
+    asy_nc = physics.compute_asynth_from_transp(otr, wk.catalog, atm, grid, ova, None, 1e-3, None,
+                                                grid_origin=float(grid[0]))
+    oasy_nc, st = oracle.asynth(soa, atm, grid, otr, ova, np.zeros_like(km), 0.0)
+    assert _rel(asy_nc, oasy_nc) < 1e-12
+
+
+@pytest.mark.gpu
+def test_run_synthesis_cli(gpu, oracle, small_workload, tmp_path):
+    """cli.main -> run_synthesis -> .spec with the reference's 3-column format."""
+    from kurucz_py_joss_submission_amd import cli, synthetic
+
+    wk = small_workload
+    atm_path = tmp_path / "model.npz"
+    wk.atmosphere.save(atm_path)
+    d = synthetic.synthetic_lines(1500, 500.0, 501.0, seed=5)
+    lines_path = tmp_path / "lines.gfall"
+    synthetic.write_line_list(lines_path, d)
+    spec = tmp_path / "out.spec"
+    diag = tmp_path / "diag.npz"
+    assert cli.main([str(atm_path), str(lines_path), "--wl-start", "500", "--wl-end", "501",
+                     "--spec", str(spec), "--diagnostics", str(diag)]) == 0
+    rows = np.loadtxt(spec)
+    from kurucz_py_joss_submission_amd import lines as L
+
+    cat = L.filter_by_range(L.load_catalog(lines_path), 500.0, 501.0)
+    ref = oracle.synthesize(wk.atmosphere, cat, 500.0, 501.0, 300000.0)
+    assert rows.shape == (ref.wavelength.size, 3)
+    np.testing.assert_allclose(rows[:, 0], ref.wavelength, rtol=1e-9)
+    np.testing.assert_allclose(rows[:, 1], ref.flux, rtol=1e-6)  # 7 significant digits in .spec
+    np.testing.assert_allclose(rows[:, 2], ref.continuum, rtol=1e-6)
+    z = np.load(diag)
+    assert _rel(z["flux_total"], ref.flux) < 1e-6
