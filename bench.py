@@ -100,7 +100,20 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a plain device copy of 1 GiB (3 times), to scale HBM counters")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+
+        a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": a.numel() * 8, "copy_bytes_written": a.numel() * 8,
+                          "launches": 3}))
+        return 0
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU layout "
               "is a design note, not a measured path", file=sys.stderr)

@@ -161,6 +161,12 @@ int synthe_source_function(synthe_ctx *ctx, long n_depths, long n_wl, const doub
 int synthe_flux_to_lambda(synthe_ctx *ctx, long n_wl, const double *wl, double *flux,
                           synthe_stream stream);
 
+/* Diagnostic (no reference counterpart): evaluates the far-wing quotient
+ * kernel routine q_fast = X / b and IEEE q_ieee = X / b on n device inputs,
+ * so tests can check they agree bit for bit. */
+int synthe_diag_div_pos(long n, const double *x, const double *b, double *q_fast, double *q_ieee,
+                        synthe_stream stream);
+
 #ifdef __cplusplus
 }
 #endif

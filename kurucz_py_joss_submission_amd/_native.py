@@ -21,7 +21,7 @@ SYMBOLS = (
     "synthe_last_error", "synthe_abi_version", "synthe_create", "synthe_destroy",
     "synthe_copy_tables", "synthe_wavelength_grid", "synthe_continuum", "synthe_transp",
     "synthe_asynth", "synthe_lte_spectrum", "synthe_josh_flux_batch", "synthe_source_function",
-    "synthe_flux_to_lambda",
+    "synthe_flux_to_lambda", "synthe_diag_div_pos",
 )
 
 
@@ -86,6 +86,7 @@ def load_library(path: Optional[Path] = None):
         lib.synthe_josh_flux_batch.argtypes = [_VP, ct.c_long, ct.c_long] + [_VP] * 10
         lib.synthe_source_function.argtypes = [_VP, ct.c_long, ct.c_long] + [_VP] * 7
         lib.synthe_flux_to_lambda.argtypes = [_VP, ct.c_long, _VP, _VP, _VP]
+        lib.synthe_diag_div_pos.argtypes = [ct.c_long] + [_VP] * 5
         for name in SYMBOLS[2:]:
             if name != "synthe_wavelength_grid":
                 getattr(lib, name).restype = ct.c_int

@@ -103,9 +103,26 @@ __device__ __forceinline__ double near_wing(const WingParams& p, long k, const d
   return p.kappa0 * voigt(x, p.adamp, h0t, h1t, h2t);
 }
 
+// x / b for x in [2^-900, 2^900] and b in [1, 2^60] (the far-wing X / k^2):
+// reciprocal, two Newton steps and a Markstein remainder correction, which is
+// the IEEE division sequence without its operand scaling and special-value
+// fix-up -- both are identities in this range -- so the quotient is the
+// correctly rounded one.  Outside the range it falls back to division.
+__device__ __forceinline__ double div_pos(double x, double b) {
+  if (!(x >= 0x1p-900 && x <= 0x1p900 && b >= 1.0 && b <= 0x1p60)) return x / b;
+  double r = __builtin_amdgcn_rcp(b);
+  double e = __builtin_fma(-b, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-b, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  double q = x * r;
+  double rem = __builtin_fma(-b, q, x);
+  return __builtin_fma(rem, r, q);
+}
+
 __device__ __forceinline__ double wing_value(const WingParams& p, long k, const double* h0t,
                                              const double* h1t, const double* h2t) {
-  if (p.use_far && k > p.n10dop) return p.x_far / ((double)k * (double)k);
+  if (p.use_far && k > p.n10dop) return div_pos(p.x_far, (double)k * (double)k);
   return near_wing(p, k, h0t, h1t, h2t);
 }
 

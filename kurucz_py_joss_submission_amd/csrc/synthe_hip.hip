@@ -467,68 +467,147 @@ __global__ __launch_bounds__(kBlock) void k_class_scatter(
   }
 }
 
-__device__ __forceinline__ long lower_bound_c(const int2* __restrict__ a, long lo, long hi, long key) {
-  while (lo < hi) {
-    long mid = (lo + hi) >> 1;
-    if ((long)a[mid].x < key) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo;
-}
+constexpr int kGT = 256;           // threads per gather workgroup
+constexpr int kGP = 4;             // consecutive pixels per thread
+constexpr int kGTile = kGT * kGP;  // pixels per workgroup
 
-// Centre adds (catalog order, line_opacity.py:1091-1102) then wings of every
-// (depth, class) list whose reach can touch pixel w, then * STIM
-// (line_opacity.py:1338).  One thread per (pixel, depth); fixed summation order.
-__global__ __launch_bounds__(kBlock) void k_gather(
+// Centre adds (catalog order, line_opacity.py:1091-1102), then the wings of
+// every (depth, reach class) list entry that can reach the tile, then * STIM
+// (line_opacity.py:1338).  One workgroup per (1024-pixel tile, depth); list
+// entries are staged through LDS in chunks and read by all lanes (broadcast).
+// Each pixel sums centres first, then classes 0..8 in list (centre) order:
+// a fixed order, no atomics, so results are run-to-run identical.
+template <bool kStats>
+__global__ __launch_bounds__(kGT) void k_gather(
     long W, int D, const int64_t* __restrict__ corder, const int64_t* __restrict__ cstart,
-    const double* __restrict__ transp, const uint8_t* __restrict__ valid,
-    const int2* __restrict__ list_cm, const WingParams* __restrict__ list_p,
-    const int64_t* __restrict__ regions, const double* __restrict__ grid,
-    const double* __restrict__ temp, DeviceTables tb, double* __restrict__ asynth,
-    unsigned long long* __restrict__ evals) {
-  long w = blockIdx.x * (long)kBlock + threadIdx.x;
-  int d = blockIdx.y;
-  unsigned long long n_eval = 0;
-  if (w < W) {
-    double acc = 0.0;
-    for (long j = cstart[w]; j < cstart[w + 1]; ++j) {
-      long l = corder[j];
-      if (valid[l * D + d]) {
-        acc += transp[l * D + d];
-        ++n_eval;
-      }
-    }
-    const int64_t* reg = regions + d * (kClasses + 1);
-    long r = 16;
-    for (int kk = 0; kk < kClasses; ++kk, r <<= 2) {
-      long beg = reg[kk], end = reg[kk + 1];
-      if (beg == end) continue;
-      long j = lower_bound_c(list_cm, beg, end, w - r);
-      for (; j < end; ++j) {
-        int2 cm = list_cm[j];
-        if ((long)cm.x > w + r) break;
-        long k = w - (long)cm.x;
-        k = k < 0 ? -k : k;
-        if (k >= 1 && k <= cm.y) {
-          WingParams p = list_p[j];
-          acc += wing_value(p, k, tb.h0, tb.h1, tb.h2);
-          ++n_eval;
+    const int64_t* __restrict__ center_t, const double* __restrict__ transp,
+    const uint8_t* __restrict__ valid, const int2* __restrict__ list_cm,
+    const WingParams* __restrict__ list_p, const int64_t* __restrict__ regions,
+    const double* __restrict__ grid, const double* __restrict__ temp, DeviceTables tb,
+    double* __restrict__ asynth, unsigned long long* __restrict__ evals) {
+  __shared__ long s_rng[kClasses][2];
+  __shared__ int s_c[kGT], s_m[kGT], s_n10[kGT], s_far[kGT];
+  __shared__ double s_k0[kGT], s_a[kGT], s_dv[kGT], s_x[kGT];
+  __shared__ unsigned long long s_red[kGT / 64];
+  const int d = blockIdx.y;
+  const long t0 = blockIdx.x * (long)kGTile;
+  const long w0 = t0 + (long)kGP * threadIdx.x;
+  double acc[kGP];
+#pragma unroll
+  for (int p = 0; p < kGP; ++p) acc[p] = 0.0;
+  unsigned long long nev = 0;
+  if (w0 < W) {
+#pragma unroll
+    for (int p = 0; p < kGP; ++p) {
+      long w = w0 + p;
+      if (w < W) {
+        for (long j = cstart[w]; j < cstart[w + 1]; ++j) {
+          long l = corder[j];
+          if (valid[l * D + d]) {
+            acc[p] += transp[l * D + d];
+            if (kStats) ++nev;
+          }
         }
       }
     }
+  }
+  if (threadIdx.x < 2 * kClasses) {
+    int kk = threadIdx.x >> 1;
+    bool upper = threadIdx.x & 1;
+    long r = 16L << (2 * kk);
+    long beg = regions[d * (kClasses + 1) + kk], end = regions[d * (kClasses + 1) + kk + 1];
+    long key = upper ? t0 + kGTile - 1 + r : t0 - r;
+    long lo = beg, hi = end;
+    while (lo < hi) {  // first entry with c > key (upper) or c >= key (lower)
+      long mid = (lo + hi) >> 1;
+      long c = list_cm[mid].x;
+      if (upper ? c <= key : c < key) lo = mid + 1;
+      else hi = mid;
+    }
+    s_rng[kk][upper] = lo;
+  }
+  __syncthreads();
+  for (int kk = 0; kk < kClasses; ++kk) {
+    const long lo = s_rng[kk][0], hi = s_rng[kk][1];
+    for (long base = lo; base < hi; base += kGT) {
+      const int count = (int)(hi - base < kGT ? hi - base : kGT);
+      if (threadIdx.x < count) {
+        long j = base + threadIdx.x;
+        int2 cm = list_cm[j];
+        const WingParams& wp = list_p[j];
+        s_c[threadIdx.x] = cm.x;
+        s_m[threadIdx.x] = cm.y;
+        s_n10[threadIdx.x] = wp.n10dop;
+        s_far[threadIdx.x] = wp.use_far;
+        s_k0[threadIdx.x] = wp.kappa0;
+        s_a[threadIdx.x] = wp.adamp;
+        s_dv[threadIdx.x] = wp.dvoigt;
+        s_x[threadIdx.x] = wp.x_far;
+      }
+      __syncthreads();
+      if (w0 < W) {
+        const bool full = w0 + (kGP - 1) < W;
+        for (int q = 0; q < count; ++q) {
+          const int c = s_c[q];
+          const int M = s_m[q];
+          const int dd0 = (int)(w0 - c);
+          if (dd0 + (kGP - 1) < -M || dd0 > M) continue;
+          const int n10 = s_n10[q];
+          // all kGP pixels in the far wing (X / k^2) of this line on one side
+          if (full && s_far[q] &&
+              ((dd0 > n10 && dd0 + (kGP - 1) <= M) || (dd0 + (kGP - 1) < -n10 && dd0 >= -M))) {
+            const double x = s_x[q];
+            const double kd = (double)dd0;
+#pragma unroll
+            for (int pp = 0; pp < kGP; ++pp) {
+              const double k = fabs(kd + (double)pp);
+              acc[pp] += div_pos(x, k * k);
+            }
+            if (kStats) nev += kGP;
+            continue;
+          }
+          WingParams p;
+          p.c = (int32_t)c;
+          p.maxstep = (int32_t)M;
+          p.n10dop = s_n10[q];
+          p.use_far = s_far[q];
+          p.kappa0 = s_k0[q];
+          p.adamp = s_a[q];
+          p.dvoigt = s_dv[q];
+          p.x_far = s_x[q];
+#pragma unroll
+          for (int pp = 0; pp < kGP; ++pp) {
+            long dd = dd0 + pp;
+            long k = dd < 0 ? -dd : dd;
+            if (k >= 1 && k <= M && w0 + pp < W) {
+              acc[pp] += wing_value(p, k, tb.h0, tb.h1, tb.h2);
+              if (kStats) ++nev;
+            }
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (w0 < W) {
     double T = temp[d] > 1.0 ? temp[d] : 1.0;
     double hkt = kHPlanck / (kKBoltz * T);
-    double fg = kCLightNm / grid[w];
-    asynth[(long)d * W + w] = acc * (1.0 - exp(-fg * hkt));
+#pragma unroll
+    for (int p = 0; p < kGP; ++p) {
+      long w = w0 + p;
+      if (w < W) {
+        double fg = kCLightNm / grid[w];
+        asynth[(long)d * W + w] = acc[p] * (1.0 - exp(-fg * hkt));
+      }
+    }
   }
-  if (evals) {
-    __shared__ unsigned long long red[kBlock / 64];
-    for (int off = 32; off > 0; off >>= 1) n_eval += __shfl_down(n_eval, off);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = n_eval;
+  if (kStats) {
+    for (int off = 32; off > 0; off >>= 1) nev += __shfl_down(nev, off);
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = nev;
     __syncthreads();
     if (threadIdx.x == 0) {
       unsigned long long t = 0;
-      for (int q = 0; q < kBlock / 64; ++q) t += red[q];
+      for (int q = 0; q < kGT / 64; ++q) t += s_red[q];
       atomicAdd(evals, t);
     }
   }
@@ -725,10 +804,11 @@ __device__ __forceinline__ double planck_one(double freq, double f3, double T) {
 constexpr int kRtLanes = 64;
 
 // engine/radiative.py:42-218 (solve_lte_frequency) per wavelength, general
-// form with per-lane depth arrays: runs only for wavelengths the streaming
-// kernels flag (masked or non-increasing depths, fewer than 4 depths, or a
-// label-401 solve).  flags == NULL runs every wavelength.
-__global__ __launch_bounds__(kRtLanes) void k_lte(int D, long W, const uint8_t* __restrict__ flags,
+// form with per-lane depth arrays: runs only for the wavelengths the streaming
+// kernels list (masked or non-increasing depths, fewer than 4 depths, or a
+// label-401 solve), packed densely into waves.  list == NULL runs every one.
+__global__ __launch_bounds__(kRtLanes) void k_lte(int D, long W, const int* __restrict__ list,
+                                                  const int* __restrict__ list_n,
                                                   const double* __restrict__ wl,
                                                   const double* __restrict__ temp,
                                                   const double* __restrict__ rhox,
@@ -741,8 +821,11 @@ __global__ __launch_bounds__(kRtLanes) void k_lte(int D, long W, const uint8_t* 
                                                   int32_t* __restrict__ iters, DeviceTables tb) {
   __shared__ float s_xs[kNxtau][kRtLanes];
   long w = blockIdx.x * (long)kRtLanes + threadIdx.x;
+  if (list) {
+    if (w >= *list_n) return;
+    w = list[w];
+  }
   if (w >= W) return;
-  if (flags && flags[w] == 0) return;
   int lane = threadIdx.x;
   const double maxr4 = 3.4e38;
   int idx[kMaxDepth];
@@ -864,7 +947,8 @@ __global__ __launch_bounds__(kRtBlock) void k_rt_prep(int D, long W, const doubl
                                                       const double* __restrict__ ls,
                                                       const double* __restrict__ src,
                                                       const double* __restrict__ xtau, float* ws,
-                                                      uint8_t* __restrict__ flags) {
+                                                      uint8_t* __restrict__ flags,
+                                                      int* __restrict__ list, int* __restrict__ list_n) {
   long w = blockIdx.x * (long)kRtBlock + threadIdx.x;
   if (w >= W) return;
   const double maxr4 = 3.4e38;
@@ -877,6 +961,7 @@ __global__ __launch_bounds__(kRtBlock) void k_rt_prep(int D, long W, const doubl
   }
   if (!ok) {
     flags[w] = 1;
+    list[atomicAdd(list_n, 1)] = (int)w;  // order is irrelevant: one slot per wavelength
     return;
   }
   double wv = wl[w];
@@ -899,6 +984,19 @@ __global__ __launch_bounds__(kRtBlock) void k_rt_prep(int D, long W, const doubl
     rt::integ_map1_stream(D, xtau, in, out, &tau0);
   }
   flags[w] = fl;
+  if (fl) list[atomicAdd(list_n, 1)] = (int)w;
+}
+
+__global__ void k_zero_int(int* p) {
+  if (threadIdx.x == 0) *p = 0;
+}
+
+// Diagnostic: the far-wing quotient div_pos against IEEE division.
+__global__ void k_check_div(long n, const double* x, const double* b, double* q_fast, double* q_ieee) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  q_fast[i] = div_pos(x[i], b[i]);
+  q_ieee[i] = x[i] / b[i];
 }
 
 // Gauss-Seidel scattering iteration (josh_solver.py:24-92, Numba's float32
@@ -1276,9 +1374,14 @@ int synthe_asynth(synthe_ctx* ctx, const synthe_lines* lines, const synthe_depth
   } else {
     HIP_TRY(hipMemsetAsync(reg, 0, (size_t)D * (kClasses + 1) * sizeof(int64_t), st));
   }
-  hipLaunchKernelGGL(k_gather, dim3(blocks_for(n_wl, kBlock), (unsigned)D), dim3(kBlock), 0, st, n_wl, D,
-                     center_order, center_start, transp, valid, lcm, lp, reg, wl, depths->temperature,
-                     ctx->tb, asynth, stats ? ev + 3 : nullptr);
+  if (stats)
+    hipLaunchKernelGGL(k_gather<true>, dim3(blocks_for(n_wl, kGTile), (unsigned)D), dim3(kGT), 0, st, n_wl,
+                       D, center_order, center_start, lines->center_t, transp, valid, lcm, lp, reg, wl,
+                       depths->temperature, ctx->tb, asynth, ev + 3);
+  else
+    hipLaunchKernelGGL(k_gather<false>, dim3(blocks_for(n_wl, kGTile), (unsigned)D), dim3(kGT), 0, st, n_wl,
+                       D, center_order, center_start, lines->center_t, transp, valid, lcm, lp, reg, wl,
+                       depths->temperature, ctx->tb, asynth, nullptr);
   HIP_TRY(hipGetLastError());
   if (stats) {
     if (pairs > 0) {
@@ -1306,21 +1409,28 @@ int synthe_lte_spectrum(synthe_ctx* ctx, long n_depths, long n_wl, const double*
       !flux_total || !flux_cont)
     return fail(SYNTHE_EINVAL, "synthe_lte_spectrum: NULL array");
   hipStream_t st = (hipStream_t)stream;
-  size_t need = align_up((size_t)2 * 3 * kNxtau * n_wl * sizeof(float)) + align_up((size_t)n_wl);
-  int rc = ensure_ws(ctx, need);
+  if (n_wl > 0x7fffffffL) return fail(SYNTHE_ERANGE, "synthe_lte_spectrum: more than 2^31 wavelengths");
+  size_t o_flags = align_up((size_t)2 * 3 * kNxtau * n_wl * sizeof(float));
+  size_t o_list = o_flags + align_up((size_t)n_wl);
+  size_t o_cnt = o_list + align_up((size_t)n_wl * sizeof(int));
+  int rc = ensure_ws(ctx, o_cnt + 256);
   if (rc) return rc;
   float* ws = (float*)ctx->ws;
-  uint8_t* flags = (uint8_t*)ctx->ws + align_up((size_t)2 * 3 * kNxtau * n_wl * sizeof(float));
+  uint8_t* flags = (uint8_t*)ctx->ws + o_flags;
+  int* list = (int*)((char*)ctx->ws + o_list);
+  int* list_n = (int*)((char*)ctx->ws + o_cnt);
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, list_n);
+  HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(k_rt_prep, dim3(blocks_for(n_wl, kRtBlock)), dim3(kRtBlock), 0, st, (int)n_depths,
                      n_wl, wl, temperature, rhox, cont_abs, cont_scat, line_abs, line_scat, line_source,
-                     ctx->tb.xtau, ws, flags);
+                     ctx->tb.xtau, ws, flags, list, list_n);
   HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(k_rt_iter, dim3(blocks_for(n_wl, kRtLanes)), dim3(kRtLanes), 0, st, n_wl, flags, ws,
                      flux_total, flux_cont, iters, ctx->tb);
   HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(k_lte, dim3(blocks_for(n_wl, kRtLanes)), dim3(kRtLanes), 0, st, (int)n_depths, n_wl,
-                     flags, wl, temperature, rhox, cont_abs, cont_scat, line_abs, line_scat, line_source,
-                     flux_total, flux_cont, iters, ctx->tb);
+                     list, list_n, wl, temperature, rhox, cont_abs, cont_scat, line_abs, line_scat,
+                     line_source, flux_total, flux_cont, iters, ctx->tb);
   HIP_TRY(hipGetLastError());
   return SYNTHE_OK;
 }
@@ -1338,6 +1448,17 @@ int synthe_josh_flux_batch(synthe_ctx* ctx, long n, long n_cols, const double* a
   hipLaunchKernelGGL(k_josh_batch, dim3(blocks_for(n_cols, kRtLanes)), dim3(kRtLanes), 0,
                      (hipStream_t)stream, (int)n, n_cols, acont, scont, aline, sline, sigmac, sigmal, rhox,
                      flux, iters, ctx->tb);
+  HIP_TRY(hipGetLastError());
+  return SYNTHE_OK;
+}
+
+int synthe_diag_div_pos(long n, const double* x, const double* b, double* q_fast, double* q_ieee,
+                        synthe_stream stream) {
+  if (n < 0 || (n > 0 && (!x || !b || !q_fast || !q_ieee)))
+    return fail(SYNTHE_EINVAL, "synthe_diag_div_pos: bad arguments");
+  if (n == 0) return SYNTHE_OK;
+  hipLaunchKernelGGL(k_check_div, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, (hipStream_t)stream, n, x, b,
+                     q_fast, q_ieee);
   HIP_TRY(hipGetLastError());
   return SYNTHE_OK;
 }

@@ -57,8 +57,9 @@ def build_depth_continuum(atmosphere: AtmosphereModel, wavelength_grid: np.ndarr
     ca = torch.empty((D, W), dtype=torch.float64, device="cuda")
     cs = torch.empty((D, W), dtype=torch.float64, device="cuda")
     ctx = N.context()
+    d_wl = _dev(wl)  # device tensors stay referenced until the launch is enqueued
     N.check("synthe_continuum", ctx.lib.synthe_continuum(
-        ctx.handle, D, W, N.ptr(_dev(wl)), int(d.wledge.shape[0]), N.ptr(d.wledge),
+        ctx.handle, D, W, N.ptr(d_wl), int(d.wledge.shape[0]), N.ptr(d.wledge),
         N.ptr(d.half_edge), N.ptr(d.delta_edge), N.ptr(d.coeff_abs), N.ptr(d.coeff_scat),
         int(d.coeff_log10), N.ptr(ca), N.ptr(cs), N.stream_handle()))
     return _host(ca), _host(cs), None, None
@@ -91,10 +92,10 @@ def compute_transp(catalog: LineCatalog, populations, atmosphere: AtmosphereMode
         ls, ds = lines.struct(), d.struct()
         import ctypes
 
+        d_km = _dev(continuum_absorption)
         N.check("synthe_transp", ctx.lib.synthe_transp(
-            ctx.handle, ctypes.byref(ls), ctypes.byref(ds), wl.size,
-            N.ptr(_dev(continuum_absorption)), float(cutoff), float(microturb_kms), N.ptr(tr),
-            N.ptr(va), N.stream_handle()))
+            ctx.handle, ctypes.byref(ls), ctypes.byref(ds), wl.size, N.ptr(d_km), float(cutoff),
+            float(microturb_kms), N.ptr(tr), N.ptr(va), N.stream_handle()))
     return _host(tr), _host(va).astype(bool), soa.center_t.copy()
 
 
@@ -133,11 +134,11 @@ def compute_asynth_from_transp(transp: np.ndarray, catalog: LineCatalog, atmosph
 
     ctx = N.context()
     ls, ds = lines.struct(), d.struct()
+    d_wl, d_tr, d_va, d_km = _dev(wl), _dev(tr), _dev(va, np.uint8), _dev(kapmin)
     N.check("synthe_asynth", ctx.lib.synthe_asynth(
-        ctx.handle, ctypes.byref(ls), ctypes.byref(ds), W, N.ptr(_dev(wl)), N.ptr(_dev(tr)),
-        N.ptr(_dev(va, np.uint8)), N.ptr(_dev(kapmin)), float(cutoff) if use_cutoff else 0.0,
-        N.ptr(lines.corder), N.ptr(lines.cstart), N.ptr(lines.worder), N.ptr(out), None,
-        N.stream_handle()))
+        ctx.handle, ctypes.byref(ls), ctypes.byref(ds), W, N.ptr(d_wl), N.ptr(d_tr), N.ptr(d_va),
+        N.ptr(d_km), float(cutoff) if use_cutoff else 0.0, N.ptr(lines.corder),
+        N.ptr(lines.cstart), N.ptr(lines.worder), N.ptr(out), None, N.stream_handle()))
     return _host(out)
 
 
@@ -163,9 +164,9 @@ def source_function(wavelength_grid: np.ndarray, temperature: np.ndarray):
     D, W = T.size, wl.size
     outs = [torch.empty((D, W), dtype=torch.float64, device="cuda") for _ in range(4)]
     ctx = N.context()
+    d_wl, d_t = _dev(wl), _dev(T)
     N.check("synthe_source_function", ctx.lib.synthe_source_function(
-        ctx.handle, D, W, N.ptr(_dev(wl)), N.ptr(_dev(T)), *[N.ptr(o) for o in outs],
-        N.stream_handle()))
+        ctx.handle, D, W, N.ptr(d_wl), N.ptr(d_t), *[N.ptr(o) for o in outs], N.stream_handle()))
     return tuple(_host(o) for o in outs)
 
 
@@ -188,9 +189,10 @@ def solve_lte_spectrum(wavelength_nm: np.ndarray, temperature: np.ndarray, colum
     ctx = N.context()
     arrs = [_dev(a) for a in (cont_abs, cont_scat, line_opacity, line_scattering)]
     ls = _dev(line_source) if line_source is not None else None
+    d_wl, d_t, d_m = _dev(wl), _dev(temperature), _dev(column_mass)
     N.check("synthe_lte_spectrum", ctx.lib.synthe_lte_spectrum(
-        ctx.handle, D, W, N.ptr(_dev(wl)), N.ptr(_dev(temperature)), N.ptr(_dev(column_mass)),
-        *[N.ptr(a) for a in arrs], N.ptr(ls), N.ptr(ft), N.ptr(fc), None, N.stream_handle()))
+        ctx.handle, D, W, N.ptr(d_wl), N.ptr(d_t), N.ptr(d_m), *[N.ptr(a) for a in arrs],
+        N.ptr(ls), N.ptr(ft), N.ptr(fc), None, N.stream_handle()))
     return _host(ft), _host(fc)
 
 

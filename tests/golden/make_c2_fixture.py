@@ -1,0 +1,48 @@
+"""Write tests/golden/c2_oracle_sample.npz: the oracle's full-window spectrum, sampled.
+
+Configuration: Teff 4000 K, log g 5.0, 72 layers, 1,000,000 seeded lines over
+300-1800 nm, R = 300000 (W = 537,529).  The oracle (C restatement of the
+reference hot path) runs the whole spectrum (~200 s on 8 cores); the fixture
+keeps every 97th wavelength's flux and continuum, plus the exact counts
+(line x depth x lambda evaluations, wing pairs, JOSH iterations).
+
+    python tests/golden/make_c2_fixture.py
+"""
+
+from __future__ import annotations
+
+import os
+import sys
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "oracle"))
+
+STRIDE = 97
+
+
+def main() -> None:
+    import oracle
+    from kurucz_py_joss_submission_amd import synthetic
+
+    oracle.build()
+    oracle.set_threads(os.cpu_count() or 1)
+    wk = synthetic.make_workload("c2", 4000.0, 5.0, 1_000_000, 300.0, 1800.0, seed=1)
+    r = oracle.synthesize(wk.atmosphere, wk.catalog, 300.0, 1800.0, 300000.0, keep_stages=True)
+    idx = np.arange(0, r.wavelength.size, STRIDE)
+    np.savez_compressed(
+        Path(__file__).with_name("c2_oracle_sample.npz"),
+        index=idx, wavelength=r.wavelength[idx], flux=r.flux[idx], continuum=r.continuum[idx],
+        iters=r.stages["iters"][idx], n_wavelengths=r.wavelength.size,
+        e_line=r.stats["e_line"], wing_pairs=r.stats["wing_pairs"],
+        sum_maxstep=r.stats["sum_maxstep"], max_maxstep=r.stats["max_maxstep"],
+        josh_iters=r.stats["josh_iters"], flux_sum=float(np.sum(r.flux)),
+        cont_sum=float(np.sum(r.continuum)))
+    print("e_line", r.stats["e_line"], "W", r.wavelength.size)
+
+
+if __name__ == "__main__":
+    main()

@@ -136,3 +136,25 @@ def test_josh_batch_parity(gpu, oracle):
             continue
         assert abs(got[c] - f) <= 1e-12 * max(abs(f), 1e-300), (c, got[c], f)
     assert n401 > 0
+
+
+def test_far_wing_division_bitwise(gpu):
+    """The far-wing quotient routine equals IEEE division bit for bit over the
+    operand ranges of X / k^2 (X = P(N10DOP) N10DOP^2, k <= 10^6)."""
+    import torch
+    from kurucz_py_joss_submission_amd import _native as N
+
+    rng = np.random.default_rng(11)
+    n = 1 << 21
+    x = 10.0 ** rng.uniform(-250, 250, n)
+    k = rng.integers(1, 1_000_001, n).astype(np.float64)
+    b = k * k
+    b[:1000] = np.arange(1, 1001, dtype=np.float64) ** 2
+    dx, db = torch.from_numpy(x).cuda(), torch.from_numpy(b).cuda()
+    qf = torch.empty_like(dx)
+    qi = torch.empty_like(dx)
+    N.check("synthe_diag_div_pos", N.load_library().synthe_diag_div_pos(
+        n, dx.data_ptr(), db.data_ptr(), qf.data_ptr(), qi.data_ptr(), N.stream_handle()))
+    qf, qi = qf.cpu().numpy(), qi.cpu().numpy()
+    assert np.array_equal(qf, qi)
+    assert np.array_equal(qi, x / b)  # and IEEE on the GPU equals numpy's
