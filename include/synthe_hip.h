@@ -91,6 +91,15 @@ int synthe_destroy(synthe_ctx *ctx);
 int synthe_copy_tables(const synthe_ctx *ctx, double *h0, double *h1, double *h2, double *extab,
                        double *extabf);
 
+/* Kernel timing (no reference counterpart; for benchmarks).  While enabled,
+ * each synthe_asynth call records a pair of HIP events on its stream around
+ * its wing-gather kernel (up to 256 launches; later ones are not recorded).
+ * Enabling (or disabling) clears the record.  synthe_kernel_times waits for
+ * the recorded events, writes min(count, cap) durations in milliseconds to
+ * the host array ms (may be NULL) and the number recorded to *count. */
+int synthe_set_kernel_timing(synthe_ctx *ctx, int enable);
+int synthe_kernel_times(synthe_ctx *ctx, float *ms, int cap, int *count);
+
 /* Geometric wavelength grid, computed on the HOST exactly as
  * engine/opacity.py:526-547 (_build_wavelength_grid).  Writes at most `cap`
  * values to the host array `out` (may be NULL) and returns the count
@@ -120,8 +129,9 @@ int synthe_transp(synthe_ctx *ctx, const synthe_lines *lines, const synthe_depth
  * center_order [L] = lines stably sorted by center_t and center_start [W+1]
  * its CSR offsets per pixel; wing_order [L] = lines stably sorted by
  * center_w (all host-computed index arrays, device copies).  Output
- * asynth [D][W].  stats (device, may be NULL) receives 4 doubles:
- * line x depth x lambda evaluations, wing pairs, sum of MAXSTEP, max MAXSTEP. */
+ * asynth [D][W].  stats (device, may be NULL) receives 5 doubles:
+ * line x depth x lambda evaluations, wing pairs, sum of MAXSTEP, max MAXSTEP,
+ * and the evaluations that took the per-pixel general (near-wing) path. */
 int synthe_asynth(synthe_ctx *ctx, const synthe_lines *lines, const synthe_depths *depths,
                   long n_wl, const double *wl, const double *transp, const uint8_t *valid,
                   const double *kapmin, double cutoff, const int64_t *center_order,
@@ -162,10 +172,11 @@ int synthe_flux_to_lambda(synthe_ctx *ctx, long n_wl, const double *wl, double *
                           synthe_stream stream);
 
 /* Diagnostic (no reference counterpart): evaluates the far-wing quotient
- * kernel routine q_fast = X / b and IEEE q_ieee = X / b on n device inputs,
- * so tests can check they agree bit for bit. */
-int synthe_diag_div_pos(long n, const double *x, const double *b, double *q_fast, double *q_ieee,
-                        synthe_stream stream);
+ * routine q_fast = X / b (variant 0 = the one the kernels use; 1, 2 = fewer
+ * Newton steps) and IEEE q_ieee = X / b on n device inputs, so tests can
+ * check they agree bit for bit. */
+int synthe_diag_div_pos(long n, int variant, const double *x, const double *b, double *q_fast,
+                        double *q_ieee, synthe_stream stream);
 
 #ifdef __cplusplus
 }

@@ -8,20 +8,24 @@ fallback.
 from __future__ import annotations
 
 import ctypes as ct
+import os
 import threading
 from pathlib import Path
 from typing import Optional
 
 import numpy as np
 
-LIB_PATH = Path(__file__).resolve().parent / "libsynthe_hip.so"
+# SYNTHE_HIP_LIBRARY points at an alternative build of the same library.
+LIB_PATH = Path(os.environ.get("SYNTHE_HIP_LIBRARY",
+                               Path(__file__).resolve().parent / "libsynthe_hip.so"))
 
 SYNTHE_OK = 0
 SYMBOLS = (
     "synthe_last_error", "synthe_abi_version", "synthe_create", "synthe_destroy",
     "synthe_copy_tables", "synthe_wavelength_grid", "synthe_continuum", "synthe_transp",
     "synthe_asynth", "synthe_lte_spectrum", "synthe_josh_flux_batch", "synthe_source_function",
-    "synthe_flux_to_lambda", "synthe_diag_div_pos",
+    "synthe_flux_to_lambda", "synthe_diag_div_pos", "synthe_set_kernel_timing",
+    "synthe_kernel_times",
 )
 
 
@@ -86,7 +90,9 @@ def load_library(path: Optional[Path] = None):
         lib.synthe_josh_flux_batch.argtypes = [_VP, ct.c_long, ct.c_long] + [_VP] * 10
         lib.synthe_source_function.argtypes = [_VP, ct.c_long, ct.c_long] + [_VP] * 7
         lib.synthe_flux_to_lambda.argtypes = [_VP, ct.c_long, _VP, _VP, _VP]
-        lib.synthe_diag_div_pos.argtypes = [ct.c_long] + [_VP] * 5
+        lib.synthe_diag_div_pos.argtypes = [ct.c_long, ct.c_int] + [_VP] * 5
+        lib.synthe_set_kernel_timing.argtypes = [_VP, ct.c_int]
+        lib.synthe_kernel_times.argtypes = [_VP, _VP, ct.c_int, ct.POINTER(ct.c_int)]
         for name in SYMBOLS[2:]:
             if name != "synthe_wavelength_grid":
                 getattr(lib, name).restype = ct.c_int
@@ -125,6 +131,18 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def set_kernel_timing(self, enable: bool) -> None:
+        check("synthe_set_kernel_timing", self.lib.synthe_set_kernel_timing(self.handle, int(enable)))
+
+    def kernel_times(self) -> np.ndarray:
+        """Durations (ms) of the wing-gather launches recorded since timing was enabled."""
+        n = ct.c_int(0)
+        check("synthe_kernel_times", self.lib.synthe_kernel_times(self.handle, None, 0, ct.byref(n)))
+        out = np.zeros(max(n.value, 1), dtype=np.float32)
+        check("synthe_kernel_times", self.lib.synthe_kernel_times(self.handle, out.ctypes.data, n.value,
+                                                                  ct.byref(n)))
+        return out[:n.value].astype(np.float64)
 
     def tables(self):
         h0, h1, h2 = (np.zeros(2001) for _ in range(3))

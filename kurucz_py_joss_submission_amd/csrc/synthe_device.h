@@ -120,6 +120,16 @@ __device__ __forceinline__ double div_pos(double x, double b) {
   return __builtin_fma(rem, r, q);
 }
 
+// The same quotient with one Newton step, for callers that guarantee the
+// operand ranges above; measured bit-identical to IEEE division on the GPU
+// over X in [1e-250, 1e250] and b = k^2, k <= 10^6 (synthe_diag_div_pos).
+__device__ __forceinline__ double div_pos_inrange(double x, double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  r = __builtin_fma(r, __builtin_fma(-b, r, 1.0), r);
+  double q = x * r;
+  return __builtin_fma(__builtin_fma(-b, q, x), r, q);
+}
+
 __device__ __forceinline__ double wing_value(const WingParams& p, long k, const double* h0t,
                                              const double* h1t, const double* h2t) {
   if (p.use_far && k > p.n10dop) return div_pos(p.x_far, (double)k * (double)k);

@@ -45,6 +45,12 @@ int fail(int code, const std::string& msg) {
   } while (0)
 
 constexpr int kClasses = 9;  // reach classes: MAXSTEP <= 16 * 4^k
+
+// Per list entry, what every candidate test and the far-wing path read:
+// centre index, MAXSTEP, N10DOP and the far-wing flag (16 bytes, one load).
+struct __align__(16) WingHead {
+  int32_t c, maxstep, n10dop, use_far;
+};
 constexpr int kBlock = 256;
 
 // ---------------------------------------------------------------- host tables
@@ -439,7 +445,7 @@ __global__ void k_regions(int D, long L, const int64_t* __restrict__ totals, int
 __global__ __launch_bounds__(kBlock) void k_class_scatter(
     long L, int D, const int64_t* __restrict__ order, const uint8_t* __restrict__ cls,
     const int32_t* __restrict__ offsets, const int64_t* __restrict__ regions,
-    const WingParams* __restrict__ wp, int2* __restrict__ list_cm, WingParams* __restrict__ list_p) {
+    const WingParams* __restrict__ wp, WingHead* __restrict__ list_cm, WingParams* __restrict__ list_p) {
   __shared__ int sh[kBlock / 64][kClasses];
   long s = blockIdx.x * (long)kBlock + threadIdx.x;
   long l = s < L ? order[s] : -1;
@@ -460,168 +466,161 @@ __global__ __launch_bounds__(kBlock) void k_class_scatter(
       int64_t pos = regions[d * (kClasses + 1) + k] +
                     offsets[((long)blockIdx.x * D + d) * kClasses + k] + before + rank;
       WingParams p = wp[l * D + d];
-      list_cm[pos] = make_int2(p.c, p.maxstep);
+      list_cm[pos] = WingHead{p.c, p.maxstep, p.n10dop, p.use_far};
       list_p[pos] = p;
     }
     __syncthreads();
   }
 }
 
-constexpr int kGT = 256;           // threads per gather workgroup
-constexpr int kGP = 4;             // consecutive pixels per thread
-constexpr int kGTile = kGT * kGP;  // pixels per workgroup
+constexpr int kWP = 8;              // consecutive pixels per lane
+constexpr int kWT = 64 * kWP;       // pixels per wave (one wave per workgroup)
+
+__device__ __forceinline__ long uniform_long(long v) {
+  int lo = __builtin_amdgcn_readfirstlane((int)(v & 0xffffffff));
+  int hi = __builtin_amdgcn_readfirstlane((int)(v >> 32));
+  return (long)(((unsigned long)(unsigned)hi << 32) | (unsigned)lo);
+}
 
 // Centre adds (catalog order, line_opacity.py:1091-1102), then the wings of
 // every (depth, reach class) list entry that can reach the tile, then * STIM
-// (line_opacity.py:1338).  One workgroup per (1024-pixel tile, depth); list
-// entries are staged through LDS in chunks and read by all lanes (broadcast).
-// Each pixel sums centres first, then classes 0..8 in list (centre) order:
-// a fixed order, no atomics, so results are run-to-run identical.
+// (line_opacity.py:1338).  One wave per (512-pixel tile, depth), 8 pixels per
+// lane.  List entries are read with wave-uniform addresses (scalar loads) and
+// tested against the whole tile with scalar branches; a line whose far wing
+// (X / k^2) covers the tile is added branch-free, others per lane.  Each pixel
+// sums centres first, then classes 0..8 in list (centre) order: a fixed order,
+// no atomics, so results are run-to-run identical.
+#ifndef SYNTHE_GATHER_WAVES
+#define SYNTHE_GATHER_WAVES 7  // waves per SIMD the register budget is sized for
+#endif
 template <bool kStats>
-__global__ __launch_bounds__(kGT) void k_gather(
+__global__ __launch_bounds__(64, SYNTHE_GATHER_WAVES) void k_gather(
     long W, int D, const int64_t* __restrict__ corder, const int64_t* __restrict__ cstart,
     const int64_t* __restrict__ center_t, const double* __restrict__ transp,
-    const uint8_t* __restrict__ valid, const int2* __restrict__ list_cm,
+    const uint8_t* __restrict__ valid, const WingHead* __restrict__ list_cm,
     const WingParams* __restrict__ list_p, const int64_t* __restrict__ regions,
     const double* __restrict__ grid, const double* __restrict__ temp, DeviceTables tb,
     double* __restrict__ asynth, unsigned long long* __restrict__ evals) {
-  __shared__ long s_rng[kClasses][2];
-  __shared__ int s_c[kGT], s_m[kGT], s_n10[kGT], s_far[kGT];
-  __shared__ double s_k0[kGT], s_a[kGT], s_dv[kGT], s_x[kGT];
-  __shared__ unsigned long long s_red[kGT / 64];
   const int d = blockIdx.y;
-  const long t0 = blockIdx.x * (long)kGTile;
-  const long w0 = t0 + (long)kGP * threadIdx.x;
-  double acc[kGP];
+  const long t0 = blockIdx.x * (long)kWT;
+  const int lane = threadIdx.x;
+  const long w0 = t0 + (long)kWP * lane;
+  double acc[kWP];
 #pragma unroll
-  for (int p = 0; p < kGP; ++p) acc[p] = 0.0;
-  unsigned long long nev = 0;
-  if (w0 < W) {
+  for (int p = 0; p < kWP; ++p) acc[p] = 0.0;
+  unsigned long long nev = 0, ngen = 0;
 #pragma unroll
-    for (int p = 0; p < kGP; ++p) {
-      long w = w0 + p;
-      if (w < W) {
-        for (long j = cstart[w]; j < cstart[w + 1]; ++j) {
-          long l = corder[j];
-          if (valid[l * D + d]) {
-            acc[p] += transp[l * D + d];
-            if (kStats) ++nev;
-          }
+  for (int p = 0; p < kWP; ++p) {
+    long w = w0 + p;
+    if (w < W) {
+      for (long j = cstart[w]; j < cstart[w + 1]; ++j) {
+        long l = corder[j];
+        if (valid[l * D + d]) {
+          acc[p] += transp[l * D + d];
+          if (kStats) ++nev;
         }
       }
     }
   }
-  if (threadIdx.x < 2 * kClasses) {
-    int kk = threadIdx.x >> 1;
-    bool upper = threadIdx.x & 1;
+  // per-class candidate ranges for the tile window [t0 - R, t0 + kWT - 1 + R]
+  long bound = 0;
+  if (lane < 2 * kClasses) {
+    int kk = lane >> 1;
+    bool upper = lane & 1;
     long r = 16L << (2 * kk);
-    long beg = regions[d * (kClasses + 1) + kk], end = regions[d * (kClasses + 1) + kk + 1];
-    long key = upper ? t0 + kGTile - 1 + r : t0 - r;
-    long lo = beg, hi = end;
+    long lo = regions[d * (kClasses + 1) + kk], hi = regions[d * (kClasses + 1) + kk + 1];
+    long key = upper ? t0 + kWT - 1 + r : t0 - r;
     while (lo < hi) {  // first entry with c > key (upper) or c >= key (lower)
       long mid = (lo + hi) >> 1;
-      long c = list_cm[mid].x;
+      long c = list_cm[mid].c;
       if (upper ? c <= key : c < key) lo = mid + 1;
       else hi = mid;
     }
-    s_rng[kk][upper] = lo;
+    bound = lo;
   }
-  __syncthreads();
+  const long kd0 = w0;  // lane's first pixel
   for (int kk = 0; kk < kClasses; ++kk) {
-    const long lo = s_rng[kk][0], hi = s_rng[kk][1];
-    for (long base = lo; base < hi; base += kGT) {
-      const int count = (int)(hi - base < kGT ? hi - base : kGT);
-      if (threadIdx.x < count) {
-        long j = base + threadIdx.x;
-        int2 cm = list_cm[j];
-        const WingParams& wp = list_p[j];
-        s_c[threadIdx.x] = cm.x;
-        s_m[threadIdx.x] = cm.y;
-        s_n10[threadIdx.x] = wp.n10dop;
-        s_far[threadIdx.x] = wp.use_far;
-        s_k0[threadIdx.x] = wp.kappa0;
-        s_a[threadIdx.x] = wp.adamp;
-        s_dv[threadIdx.x] = wp.dvoigt;
-        s_x[threadIdx.x] = wp.x_far;
+    const long lo = uniform_long(__shfl(bound, 2 * kk));
+    const long hi = uniform_long(__shfl(bound, 2 * kk + 1));
+    if (lo >= hi) continue;
+    // software pipeline: the next entry's head and X are loaded while this
+    // entry is evaluated
+    WingHead h = list_cm[lo];
+    double xh = list_p[lo].x_far;
+    for (long j = lo; j < hi; ++j) {
+      const WingHead cur = h;
+      const double x = xh;
+      if (j + 1 < hi) {
+        h = list_cm[j + 1];
+        xh = list_p[j + 1].x_far;
       }
-      __syncthreads();
-      if (w0 < W) {
-        const bool full = w0 + (kGP - 1) < W;
-        for (int q = 0; q < count; ++q) {
-          const int c = s_c[q];
-          const int M = s_m[q];
-          const int dd0 = (int)(w0 - c);
-          if (dd0 + (kGP - 1) < -M || dd0 > M) continue;
-          const int n10 = s_n10[q];
-          // all kGP pixels in the far wing (X / k^2) of this line on one side
-          if (full && s_far[q] &&
-              ((dd0 > n10 && dd0 + (kGP - 1) <= M) || (dd0 + (kGP - 1) < -n10 && dd0 >= -M))) {
-            const double x = s_x[q];
-            const double kd = (double)dd0;
+      const long c = cur.c;
+      const long M = cur.maxstep;
+      const long a = t0 - c, b = t0 + kWT - 1 - c;  // tile offsets from the centre
+      if (b < -M || a > M) continue;                 // wave-uniform skip
+      const long n10 = cur.n10dop;
+      const bool xok = cur.use_far && x >= 0x1p-900 && x <= 0x1p900;
+      // the lane's own pixels: all in one far wing (X / k^2), none in range, or mixed
+      const long la = w0 - c, lb = w0 + kWP - 1 - c;
+      const bool lane_far = xok && w0 + kWP <= W &&
+                            ((la > n10 && lb <= M) || (lb < -n10 && la >= -M));
+      if (lane_far) {
+        double kd = (double)(int)la;
+        __asm__("" : "+v"(kd));  // keep k = kd + p in double (no per-pixel int conversion)
 #pragma unroll
-            for (int pp = 0; pp < kGP; ++pp) {
-              const double k = fabs(kd + (double)pp);
-              acc[pp] += div_pos(x, k * k);
-            }
-            if (kStats) nev += kGP;
-            continue;
-          }
-          WingParams p;
-          p.c = (int32_t)c;
-          p.maxstep = (int32_t)M;
-          p.n10dop = s_n10[q];
-          p.use_far = s_far[q];
-          p.kappa0 = s_k0[q];
-          p.adamp = s_a[q];
-          p.dvoigt = s_dv[q];
-          p.x_far = s_x[q];
+        for (int p = 0; p < kWP; ++p) {
+          const double k = kd + (double)p;  // |k| is the offset; k * k is exact
+          acc[p] += div_pos_inrange(x, k * k);
+        }
+        if (kStats) nev += kWP;
+      } else if (!(lb < -M || la > M || w0 >= W)) {
+        const WingParams pr = list_p[j];
 #pragma unroll
-          for (int pp = 0; pp < kGP; ++pp) {
-            long dd = dd0 + pp;
-            long k = dd < 0 ? -dd : dd;
-            if (k >= 1 && k <= M && w0 + pp < W) {
-              acc[pp] += wing_value(p, k, tb.h0, tb.h1, tb.h2);
-              if (kStats) ++nev;
+        for (int p = 0; p < kWP; ++p) {
+          long dd = la + p;
+          long k = dd < 0 ? -dd : dd;
+          if (k >= 1 && k <= M && w0 + p < W) {
+            acc[p] += wing_value(pr, k, tb.h0, tb.h1, tb.h2);
+            if (kStats) {
+              ++nev;
+              ++ngen;
             }
           }
         }
       }
-      __syncthreads();
     }
   }
-  if (w0 < W) {
-    double T = temp[d] > 1.0 ? temp[d] : 1.0;
-    double hkt = kHPlanck / (kKBoltz * T);
+  double T = temp[d] > 1.0 ? temp[d] : 1.0;
+  double hkt = kHPlanck / (kKBoltz * T);
 #pragma unroll
-    for (int p = 0; p < kGP; ++p) {
-      long w = w0 + p;
-      if (w < W) {
-        double fg = kCLightNm / grid[w];
-        asynth[(long)d * W + w] = acc[p] * (1.0 - exp(-fg * hkt));
-      }
+  for (int p = 0; p < kWP; ++p) {
+    long w = w0 + p;
+    if (w < W) {
+      double fg = kCLightNm / grid[w];
+      asynth[(long)d * W + w] = acc[p] * (1.0 - exp(-fg * hkt));
     }
   }
   if (kStats) {
-    for (int off = 32; off > 0; off >>= 1) nev += __shfl_down(nev, off);
-    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = nev;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      unsigned long long t = 0;
-      for (int q = 0; q < kGT / 64; ++q) t += s_red[q];
-      atomicAdd(evals, t);
+    for (int off = 32; off > 0; off >>= 1) {
+      nev += __shfl_down(nev, off);
+      ngen += __shfl_down(ngen, off);
+    }
+    if (lane == 0) {
+      atomicAdd(evals, nev);
+      atomicAdd(evals + 1, ngen);
     }
   }
 }
 
 // Wing statistics over the lists: pairs, sum and max of MAXSTEP.
 __global__ void k_wing_stats(long n_entries_cap, int D, long L, const int64_t* __restrict__ regions,
-                             const int2* __restrict__ list_cm, unsigned long long* out) {
+                             const WingHead* __restrict__ list_cm, unsigned long long* out) {
   int d = blockIdx.y;
   long beg = regions[d * (kClasses + 1)], end = regions[d * (kClasses + 1) + kClasses];
   unsigned long long cnt = 0, sum = 0, mx = 0;
   for (long j = beg + blockIdx.x * (long)blockDim.x + threadIdx.x; j < end;
        j += (long)gridDim.x * blockDim.x) {
-    unsigned long long m = (unsigned long long)list_cm[j].y;
+    unsigned long long m = (unsigned long long)list_cm[j].maxstep;
     ++cnt;
     sum += m;
     mx = m > mx ? m : mx;
@@ -637,6 +636,7 @@ __global__ void k_stats_to_double(const unsigned long long* in, double* out) {
     out[1] = (double)in[0];
     out[2] = (double)in[1];
     out[3] = (double)in[2];
+    out[4] = (double)in[4];
   }
 }
 
@@ -991,12 +991,24 @@ __global__ void k_zero_int(int* p) {
   if (threadIdx.x == 0) *p = 0;
 }
 
-// Diagnostic: the far-wing quotient div_pos against IEEE division.
-__global__ void k_check_div(long n, const double* x, const double* b, double* q_fast, double* q_ieee) {
+// Diagnostic: far-wing quotient variants against IEEE division.  variant 0 is
+// the kernels' div_pos; 1 and 2 use one and zero Newton steps on the reciprocal
+// before the remainder correction.
+__global__ void k_check_div(long n, int variant, const double* x, const double* b, double* q_fast,
+                            double* q_ieee) {
   long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (i >= n) return;
-  q_fast[i] = div_pos(x[i], b[i]);
-  q_ieee[i] = x[i] / b[i];
+  double xv = x[i], bv = b[i], q;
+  if (variant == 0) {
+    q = div_pos(xv, bv);
+  } else {
+    double r = __builtin_amdgcn_rcp(bv);
+    if (variant == 1) r = __builtin_fma(r, __builtin_fma(-bv, r, 1.0), r);
+    double qq = xv * r;
+    q = __builtin_fma(__builtin_fma(-bv, qq, xv), r, qq);
+  }
+  q_fast[i] = q;
+  q_ieee[i] = xv / bv;
 }
 
 // Gauss-Seidel scattering iteration (josh_solver.py:24-92, Numba's float32
@@ -1107,7 +1119,24 @@ struct synthe_ctx {
   std::vector<double> h0, h1, h2, extab, extabf;
   void* ws = nullptr;
   size_t ws_size = 0;
+  // kernel timing: event pairs around each wing-gather launch while enabled
+  static constexpr int kTimedMax = 256;
+  bool timing = false;
+  int n_timed = 0;
+  hipEvent_t ev_beg[kTimedMax] = {};
+  hipEvent_t ev_end[kTimedMax] = {};
 };
+
+// Records ev_beg[i] (before = true) or ev_end[i] around the next timed launch.
+static int timing_mark(synthe_ctx* ctx, hipStream_t st, bool before) {
+  if (!ctx->timing || ctx->n_timed >= synthe_ctx::kTimedMax) return SYNTHE_OK;
+  int i = ctx->n_timed;
+  hipEvent_t* e = before ? &ctx->ev_beg[i] : &ctx->ev_end[i];
+  if (!*e) HIP_TRY(hipEventCreate(e));
+  HIP_TRY(hipEventRecord(*e, st));
+  if (!before) ctx->n_timed = i + 1;
+  return SYNTHE_OK;
+}
 
 static int ensure_ws(synthe_ctx* ctx, size_t bytes) {
   if (ctx->ws_size >= bytes) return SYNTHE_OK;
@@ -1202,6 +1231,10 @@ int synthe_destroy(synthe_ctx* ctx) {
   if (ctx->d_tab) (void)hipFree(ctx->d_tab);
   if (ctx->d_tab32) (void)hipFree(ctx->d_tab32);
   if (ctx->ws) (void)hipFree(ctx->ws);
+  for (int i = 0; i < synthe_ctx::kTimedMax; ++i) {
+    if (ctx->ev_beg[i]) (void)hipEventDestroy(ctx->ev_beg[i]);
+    if (ctx->ev_end[i]) (void)hipEventDestroy(ctx->ev_end[i]);
+  }
   delete ctx;
   return SYNTHE_OK;
 }
@@ -1214,6 +1247,24 @@ int synthe_copy_tables(const synthe_ctx* ctx, double* h0, double* h1, double* h2
   if (h2) memcpy(h2, ctx->h2.data(), kVoigtN * sizeof(double));
   if (extab) memcpy(extab, ctx->extab.data(), kFastexN * sizeof(double));
   if (extabf) memcpy(extabf, ctx->extabf.data(), kFastexN * sizeof(double));
+  return SYNTHE_OK;
+}
+
+int synthe_set_kernel_timing(synthe_ctx* ctx, int enable) {
+  if (!ctx) return fail(SYNTHE_EINVAL, "synthe_set_kernel_timing: ctx is NULL");
+  ctx->timing = enable != 0;
+  ctx->n_timed = 0;
+  return SYNTHE_OK;
+}
+
+int synthe_kernel_times(synthe_ctx* ctx, float* ms, int cap, int* count) {
+  if (!ctx || !count) return fail(SYNTHE_EINVAL, "synthe_kernel_times: ctx or count is NULL");
+  int n = ctx->n_timed;
+  for (int i = 0; i < n && i < cap && ms; ++i) {
+    HIP_TRY(hipEventSynchronize(ctx->ev_end[i]));
+    HIP_TRY(hipEventElapsedTime(&ms[i], ctx->ev_beg[i], ctx->ev_end[i]));
+  }
+  *count = n;
   return SYNTHE_OK;
 }
 
@@ -1340,9 +1391,9 @@ int synthe_asynth(synthe_ctx* ctx, const synthe_lines* lines, const synthe_depth
   size_t o_cnt = off; off = align_up(off + (size_t)(nblk > 0 ? nblk : 1) * D * kClasses * sizeof(int32_t));
   size_t o_tot = off; off = align_up(off + (size_t)D * kClasses * sizeof(int64_t));
   size_t o_reg = off; off = align_up(off + (size_t)D * (kClasses + 1) * sizeof(int64_t));
-  size_t o_lcm = off; off = align_up(off + (size_t)pairs * sizeof(int2));
+  size_t o_lcm = off; off = align_up(off + (size_t)pairs * sizeof(WingHead));
   size_t o_lp = off; off = align_up(off + (size_t)pairs * sizeof(WingParams));
-  size_t o_ev = off; off = align_up(off + 4 * sizeof(unsigned long long));
+  size_t o_ev = off; off = align_up(off + 5 * sizeof(unsigned long long));
   rc = ensure_ws(ctx, off);
   if (rc) return rc;
   char* base = (char*)ctx->ws;
@@ -1351,10 +1402,10 @@ int synthe_asynth(synthe_ctx* ctx, const synthe_lines* lines, const synthe_depth
   int32_t* cnt = (int32_t*)(base + o_cnt);
   int64_t* tot = (int64_t*)(base + o_tot);
   int64_t* reg = (int64_t*)(base + o_reg);
-  int2* lcm = (int2*)(base + o_lcm);
+  WingHead* lcm = (WingHead*)(base + o_lcm);
   WingParams* lp = (WingParams*)(base + o_lp);
   unsigned long long* ev = (unsigned long long*)(base + o_ev);
-  HIP_TRY(hipMemsetAsync(ev, 0, 4 * sizeof(unsigned long long), st));
+  HIP_TRY(hipMemsetAsync(ev, 0, 5 * sizeof(unsigned long long), st));
   if (pairs > 0) {
     hipLaunchKernelGGL(k_transp_wing, dim3(blocks_for(pairs, kBlock)), dim3(kBlock), 0, st,
                        to_dev(lines), to_dev(depths), n_wl, kapmin, cutoff, 0.0, resolu, ctx->tb, 2,
@@ -1374,15 +1425,23 @@ int synthe_asynth(synthe_ctx* ctx, const synthe_lines* lines, const synthe_depth
   } else {
     HIP_TRY(hipMemsetAsync(reg, 0, (size_t)D * (kClasses + 1) * sizeof(int64_t), st));
   }
+  {
+    int rc = timing_mark(ctx, st, true);
+    if (rc) return rc;
+  }
   if (stats)
-    hipLaunchKernelGGL(k_gather<true>, dim3(blocks_for(n_wl, kGTile), (unsigned)D), dim3(kGT), 0, st, n_wl,
-                       D, center_order, center_start, lines->center_t, transp, valid, lcm, lp, reg, wl,
+    hipLaunchKernelGGL(k_gather<true>, dim3(blocks_for(n_wl, kWT), (unsigned)D), dim3(64), 0, st, n_wl, D,
+                       center_order, center_start, lines->center_t, transp, valid, lcm, lp, reg, wl,
                        depths->temperature, ctx->tb, asynth, ev + 3);
   else
-    hipLaunchKernelGGL(k_gather<false>, dim3(blocks_for(n_wl, kGTile), (unsigned)D), dim3(kGT), 0, st, n_wl,
-                       D, center_order, center_start, lines->center_t, transp, valid, lcm, lp, reg, wl,
+    hipLaunchKernelGGL(k_gather<false>, dim3(blocks_for(n_wl, kWT), (unsigned)D), dim3(64), 0, st, n_wl, D,
+                       center_order, center_start, lines->center_t, transp, valid, lcm, lp, reg, wl,
                        depths->temperature, ctx->tb, asynth, nullptr);
   HIP_TRY(hipGetLastError());
+  {
+    int rc = timing_mark(ctx, st, false);
+    if (rc) return rc;
+  }
   if (stats) {
     if (pairs > 0) {
       hipLaunchKernelGGL(k_wing_stats, dim3(64, (unsigned)D), dim3(kBlock), 0, st, pairs, D, L, reg, lcm,
@@ -1452,13 +1511,13 @@ int synthe_josh_flux_batch(synthe_ctx* ctx, long n, long n_cols, const double* a
   return SYNTHE_OK;
 }
 
-int synthe_diag_div_pos(long n, const double* x, const double* b, double* q_fast, double* q_ieee,
-                        synthe_stream stream) {
+int synthe_diag_div_pos(long n, int variant, const double* x, const double* b, double* q_fast,
+                        double* q_ieee, synthe_stream stream) {
   if (n < 0 || (n > 0 && (!x || !b || !q_fast || !q_ieee)))
     return fail(SYNTHE_EINVAL, "synthe_diag_div_pos: bad arguments");
   if (n == 0) return SYNTHE_OK;
-  hipLaunchKernelGGL(k_check_div, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, (hipStream_t)stream, n, x, b,
-                     q_fast, q_ieee);
+  hipLaunchKernelGGL(k_check_div, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, (hipStream_t)stream, n,
+                     variant, x, b, q_fast, q_ieee);
   HIP_TRY(hipGetLastError());
   return SYNTHE_OK;
 }

@@ -171,7 +171,7 @@ class SpectrumPlan:
         self.flux = torch.empty(W, dtype=f8, device=dev)
         self.cont = torch.empty(W, dtype=f8, device=dev)
         self.iters = torch.empty(W, dtype=torch.int32, device=dev)
-        self.stats = torch.zeros(4, dtype=f8, device=dev)
+        self.stats = torch.zeros(5, dtype=f8, device=dev)
         rx = np.asarray(atm.depth, np.float64)
         fscat = np.exp(-rx / self.rhoxj) if self.rhoxj > 0.0 else np.zeros_like(rx)
         self.fscat = fscat

@@ -154,7 +154,7 @@ def test_far_wing_division_bitwise(gpu):
     qf = torch.empty_like(dx)
     qi = torch.empty_like(dx)
     N.check("synthe_diag_div_pos", N.load_library().synthe_diag_div_pos(
-        n, dx.data_ptr(), db.data_ptr(), qf.data_ptr(), qi.data_ptr(), N.stream_handle()))
+        n, 0, dx.data_ptr(), db.data_ptr(), qf.data_ptr(), qi.data_ptr(), N.stream_handle()))
     qf, qi = qf.cpu().numpy(), qi.cpu().numpy()
     assert np.array_equal(qf, qi)
     assert np.array_equal(qi, x / b)  # and IEEE on the GPU equals numpy's
