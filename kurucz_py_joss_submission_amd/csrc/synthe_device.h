@@ -88,15 +88,30 @@ struct WingParams {
   int32_t c, maxstep, n10dop, use_far;
 };
 
+// int(tabi) after the reference's k sequential adds tabi = 0.5; tabi += tabstep
+// (physics/line_opacity.py:468-490).  The closed form 0.5 + k * tabstep is
+// within (k/2 + 1) * 2^-52 * t of the sequential sum, so whenever t is farther
+// than (k + 4) * 2^-52 * t from an integer both truncate to the same index;
+// otherwise the sequential sum is formed.
+__device__ __forceinline__ long tabi_index(double tabstep, long k) {
+  double t = 0.5 + (double)k * tabstep;
+  if (tabstep >= 0.0 && t < 0x1p40) {
+    double fl = floor(t);
+    double f = t - fl;
+    double eps = (double)(k + 4) * t * 0x1p-52;
+    if (f > eps && 1.0 - f > eps) return (long)fl;
+  }
+  double s = 0.5;
+  for (long i = 0; i < k; ++i) s += tabstep;
+  return (long)s;
+}
+
 __device__ __forceinline__ double near_wing(const WingParams& p, long k, const double* h0t,
                                             const double* h1t, const double* h2t) {
   double x = (double)k * p.dvoigt;
   if (p.adamp < 0.2) {
-    double tabstep = 200.0 * p.dvoigt;
-    double tabi = 0.5;
-    for (long i = 0; i < k; ++i) tabi += tabstep;
     if (x > 10.0) return p.kappa0 * (0.5642 * p.adamp / (x * x));
-    long idx = (long)tabi;
+    long idx = tabi_index(200.0 * p.dvoigt, k);
     idx = idx < 0 ? 0 : (idx > kVoigtN - 1 ? kVoigtN - 1 : idx);
     return p.kappa0 * (h0t[idx] + p.adamp * h1t[idx]);
   }

@@ -44,7 +44,10 @@ int fail(int code, const std::string& msg) {
     if (e_ != hipSuccess) return fail(SYNTHE_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-constexpr int kClasses = 9;  // reach classes: MAXSTEP <= 16 * 4^k
+// Reach classes: class k holds pairs with 16 * 2^(k-1) < MAXSTEP <= 16 * 2^k
+// (class 0: MAXSTEP <= 16); the last class reaches 16 * 2^16 > 10^6 steps.
+constexpr int kClassShift = 1;
+constexpr int kClasses = 17;
 
 // Per list entry, what every candidate test and the far-wing path read:
 // centre index, MAXSTEP, N10DOP and the far-wing flag (16 bytes, one load).
@@ -151,7 +154,7 @@ __device__ __forceinline__ int reach_class(int m) {
   int k = 0;
   long r = 16;
   while (m > r && k < kClasses - 1) {
-    r <<= 2;
+    r <<= kClassShift;
     ++k;
   }
   return k;
@@ -473,8 +476,32 @@ __global__ __launch_bounds__(kBlock) void k_class_scatter(
   }
 }
 
-constexpr int kWP = 8;              // consecutive pixels per lane
-constexpr int kWT = 64 * kWP;       // pixels per wave (one wave per workgroup)
+// Per depth: the largest Voigt-zone reach over the list entries, i.e.
+// N10DOP for pairs with a far wing and MAXSTEP (<= N10DOP) for the others.
+// near_reach[] must be zero on entry; the result is a max, so it does not
+// depend on the order of the atomics.
+__global__ __launch_bounds__(kBlock) void k_near_reach(int D, const int64_t* __restrict__ regions,
+                                                       const WingHead* __restrict__ list_cm,
+                                                       int32_t* __restrict__ near_reach) {
+  const int d = blockIdx.y;
+  const long beg = regions[d * (kClasses + 1)], end = regions[d * (kClasses + 1) + kClasses];
+  int m = 0;
+  for (long j = beg + blockIdx.x * (long)kBlock + threadIdx.x; j < end; j += (long)gridDim.x * kBlock) {
+    const WingHead h = list_cm[j];
+    const int r = h.use_far ? h.n10dop : h.maxstep;
+    m = r > m ? r : m;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    int o = __shfl_down(m, off);
+    m = o > m ? o : m;
+  }
+  if ((threadIdx.x & 63) == 0 && m > 0) atomicMax(near_reach + d, m);
+}
+
+__global__ void k_zero_i32(int n, int32_t* p) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
 
 __device__ __forceinline__ long uniform_long(long v) {
   int lo = __builtin_amdgcn_readfirstlane((int)(v & 0xffffffff));
@@ -482,33 +509,113 @@ __device__ __forceinline__ long uniform_long(long v) {
   return (long)(((unsigned long)(unsigned)hi << 32) | (unsigned)lo);
 }
 
-// Centre adds (catalog order, line_opacity.py:1091-1102), then the wings of
-// every (depth, reach class) list entry that can reach the tile, then * STIM
-// (line_opacity.py:1338).  One wave per (512-pixel tile, depth), 8 pixels per
-// lane.  List entries are read with wave-uniform addresses (scalar loads) and
-// tested against the whole tile with scalar branches; a line whose far wing
-// (X / k^2) covers the tile is added branch-free, others per lane.  Each pixel
-// sums centres first, then classes 0..8 in list (centre) order: a fixed order,
-// no atomics, so results are run-to-run identical.
+// For each class kk < kClasses, lanes 2kk and 2kk+1 find the list window of
+// entries whose centre lies in [lo_px - r_kk, hi_px + r_kk], r_kk = min(reach
+// of class kk, cap); the result is read back with window_bounds().
+__device__ __forceinline__ long class_window_search(int lane, int d, long lo_px, long hi_px, long cap,
+                                                    const int64_t* __restrict__ regions,
+                                                    const WingHead* __restrict__ list_cm) {
+  long bound = 0;
+  if (lane < 2 * kClasses) {
+    int kk = lane >> 1;
+    bool upper = lane & 1;
+    long r = 16L << (kClassShift * kk);
+    r = r < cap ? r : cap;
+    long lo = regions[d * (kClasses + 1) + kk], hi = regions[d * (kClasses + 1) + kk + 1];
+    long key = upper ? hi_px + r : lo_px - r;
+    while (lo < hi) {  // first entry with c > key (upper) or c >= key (lower)
+      long mid = (lo + hi) >> 1;
+      long c = list_cm[mid].c;
+      if (upper ? c <= key : c < key) lo = mid + 1;
+      else hi = mid;
+    }
+    bound = lo;
+  }
+  return bound;
+}
+
+// Voigt zone of every wing: pixels 1 <= |k| <= N10DOP of pairs with a far
+// wing and 1 <= |k| <= MAXSTEP of the others, valued as
+// physics/line_opacity.py:410-560 (near_wing).  One lane per pixel, one wave
+// per 64 pixels of one depth; entries are walked with wave-uniform loads in
+// class-then-centre order, so each pixel's sum has a fixed order.  Writes the
+// pre-STIM sum to out [D][W] (0 where no wing reaches), which k_gather
+// continues from.
+template <bool kStats>
+__global__ __launch_bounds__(256) void k_near(long W, int D, const WingHead* __restrict__ list_cm,
+                                              const WingParams* __restrict__ list_p,
+                                              const int64_t* __restrict__ regions,
+                                              const int32_t* __restrict__ near_reach, DeviceTables tb,
+                                              double* __restrict__ out, unsigned long long* __restrict__ evals) {
+  const int d = blockIdx.y;
+  const int lane = threadIdx.x & 63;
+  const long v0 = ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
+  if (v0 >= W) return;  // whole wave
+  const long w = v0 + lane;
+  const long bound = class_window_search(lane, d, v0, v0 + 63, (long)near_reach[d], regions, list_cm);
+  double acc = 0.0;
+  unsigned long long nev = 0;
+  for (int kk = 0; kk < kClasses; ++kk) {
+    const long lo = uniform_long(__shfl(bound, 2 * kk));
+    const long hi = uniform_long(__shfl(bound, 2 * kk + 1));
+    for (long j = lo; j < hi; ++j) {
+      const WingHead h = list_cm[j];
+      const long c = h.c;
+      const long lim = h.use_far ? h.n10dop : h.maxstep;
+      if (v0 + 63 < c - lim || v0 > c + lim) continue;  // wave-uniform
+      const long k = w - c;
+      const long ka = k < 0 ? -k : k;
+      if (ka >= 1 && ka <= lim && w < W) {
+        acc += near_wing(list_p[j], ka, tb.h0, tb.h1, tb.h2);
+        if (kStats) ++nev;
+      }
+    }
+  }
+  if (w < W) out[(long)d * W + w] = acc;
+  if (kStats) {
+    for (int off = 32; off > 0; off >>= 1) nev += __shfl_down(nev, off);
+    if (lane == 0) {
+      atomicAdd(evals, nev);
+      atomicAdd(evals + 1, nev);
+    }
+  }
+}
+
+constexpr int kWP = 8;              // consecutive pixels per lane
+constexpr int kWT = 64 * kWP;       // pixels per wave (one wave per workgroup)
+
 #ifndef SYNTHE_GATHER_WAVES
 #define SYNTHE_GATHER_WAVES 7  // waves per SIMD the register budget is sized for
 #endif
+
+// Continues each pixel's sum from the Voigt-zone sums of k_near (in asynth),
+// adds the line centres (catalog order, line_opacity.py:1091-1102), then the
+// far wings X / k^2 (N10DOP < |k| <= MAXSTEP, line_opacity.py:562-617) of
+// every list entry that reaches the tile, and multiplies by STIM
+// (line_opacity.py:1338).  One wave per (512-pixel tile, depth), 8 consecutive
+// pixels per lane.  Entries are read with wave-uniform loads and classified
+// against the whole tile with scalar branches: where the tile lies wholly in
+// one far wing every lane adds X / k^2 branch-free; on the tiles holding a
+// wing's N10DOP or MAXSTEP edge, pixels outside the far zone add +0.  Each
+// pixel's sum has a fixed order (Voigt zones, centres, far wings by class and
+// centre), no atomics, so results are run-to-run identical.
 template <bool kStats>
 __global__ __launch_bounds__(64, SYNTHE_GATHER_WAVES) void k_gather(
     long W, int D, const int64_t* __restrict__ corder, const int64_t* __restrict__ cstart,
     const int64_t* __restrict__ center_t, const double* __restrict__ transp,
     const uint8_t* __restrict__ valid, const WingHead* __restrict__ list_cm,
     const WingParams* __restrict__ list_p, const int64_t* __restrict__ regions,
-    const double* __restrict__ grid, const double* __restrict__ temp, DeviceTables tb,
+    const double* __restrict__ grid, const double* __restrict__ temp,
     double* __restrict__ asynth, unsigned long long* __restrict__ evals) {
   const int d = blockIdx.y;
   const long t0 = blockIdx.x * (long)kWT;
   const int lane = threadIdx.x;
   const long w0 = t0 + (long)kWP * lane;
+  double* row = asynth + (long)d * W;
   double acc[kWP];
 #pragma unroll
-  for (int p = 0; p < kWP; ++p) acc[p] = 0.0;
-  unsigned long long nev = 0, ngen = 0;
+  for (int p = 0; p < kWP; ++p) acc[p] = w0 + p < W ? row[w0 + p] : 0.0;
+  unsigned long long nev = 0;
 #pragma unroll
   for (int p = 0; p < kWP; ++p) {
     long w = w0 + p;
@@ -522,22 +629,8 @@ __global__ __launch_bounds__(64, SYNTHE_GATHER_WAVES) void k_gather(
       }
     }
   }
-  // per-class candidate ranges for the tile window [t0 - R, t0 + kWT - 1 + R]
-  long bound = 0;
-  if (lane < 2 * kClasses) {
-    int kk = lane >> 1;
-    bool upper = lane & 1;
-    long r = 16L << (2 * kk);
-    long lo = regions[d * (kClasses + 1) + kk], hi = regions[d * (kClasses + 1) + kk + 1];
-    long key = upper ? t0 + kWT - 1 + r : t0 - r;
-    while (lo < hi) {  // first entry with c > key (upper) or c >= key (lower)
-      long mid = (lo + hi) >> 1;
-      long c = list_cm[mid].c;
-      if (upper ? c <= key : c < key) lo = mid + 1;
-      else hi = mid;
-    }
-    bound = lo;
-  }
+  const long t1 = t0 + kWT - 1;
+  const long bound = class_window_search(lane, d, t0, t1, 1L << 40, regions, list_cm);
   const long kd0 = w0;  // lane's first pixel
   for (int kk = 0; kk < kClasses; ++kk) {
     const long lo = uniform_long(__shfl(bound, 2 * kk));
@@ -556,35 +649,46 @@ __global__ __launch_bounds__(64, SYNTHE_GATHER_WAVES) void k_gather(
       }
       const long c = cur.c;
       const long M = cur.maxstep;
-      const long a = t0 - c, b = t0 + kWT - 1 - c;  // tile offsets from the centre
-      if (b < -M || a > M) continue;                 // wave-uniform skip
+      const long a = t0 - c, b = t1 - c;  // tile offsets from the centre
+      if (!cur.use_far || b < -M || a > M) continue;  // no far wing here
       const long n10 = cur.n10dop;
-      const bool xok = cur.use_far && x >= 0x1p-900 && x <= 0x1p900;
-      // the lane's own pixels: all in one far wing (X / k^2), none in range, or mixed
-      const long la = w0 - c, lb = w0 + kWP - 1 - c;
-      const bool lane_far = xok && w0 + kWP <= W &&
-                            ((la > n10 && lb <= M) || (lb < -n10 && la >= -M));
-      if (lane_far) {
-        double kd = (double)(int)la;
-        __asm__("" : "+v"(kd));  // keep k = kd + p in double (no per-pixel int conversion)
+      double kd = (double)(int)(kd0 - c);
+      __asm__("" : "+v"(kd));  // keep k = kd + p in double (no per-pixel int conversion)
+      const bool interior = (a > n10 && b <= M) || (b < -n10 && a >= -M);
+      const bool xok = x >= 0x1p-900 && x <= 0x1p900;
+      if (interior && xok) {
 #pragma unroll
         for (int p = 0; p < kWP; ++p) {
           const double k = kd + (double)p;  // |k| is the offset; k * k is exact
           acc[p] += div_pos_inrange(x, k * k);
         }
-        if (kStats) nev += kWP;
-      } else if (!(lb < -M || la > M || w0 >= W)) {
-        const WingParams pr = list_p[j];
+        if (kStats) {
+          long n = W - w0 < kWP ? W - w0 : kWP;
+          nev += n > 0 ? n : 0;
+        }
+      } else {
+        const double n10d = (double)(int)n10, md = (double)(int)M;
+        if (xok) {
 #pragma unroll
-        for (int p = 0; p < kWP; ++p) {
-          long dd = la + p;
-          long k = dd < 0 ? -dd : dd;
-          if (k >= 1 && k <= M && w0 + p < W) {
-            acc[p] += wing_value(pr, k, tb.h0, tb.h1, tb.h2);
-            if (kStats) {
-              ++nev;
-              ++ngen;
-            }
+          for (int p = 0; p < kWP; ++p) {
+            const double k = kd + (double)p;
+            const double ka = fabs(k);
+            const double xp = (ka > n10d && ka <= md) ? x : 0.0;
+            // 0 / max(k^2, 1) adds +0, also at the centre pixel (k = 0)
+            acc[p] += div_pos_inrange(xp, fmax(k * k, 1.0));
+          }
+        } else {
+          for (int p = 0; p < kWP; ++p) {
+            const double k = kd + (double)p;
+            const double ka = fabs(k);
+            if (ka > n10d && ka <= md) acc[p] += x / (k * k);
+          }
+        }
+        if (kStats) {
+          for (int p = 0; p < kWP; ++p) {
+            long ka = kd0 - c + p;
+            ka = ka < 0 ? -ka : ka;
+            if (ka > n10 && ka <= M && w0 + p < W) ++nev;
           }
         }
       }
@@ -597,18 +701,12 @@ __global__ __launch_bounds__(64, SYNTHE_GATHER_WAVES) void k_gather(
     long w = w0 + p;
     if (w < W) {
       double fg = kCLightNm / grid[w];
-      asynth[(long)d * W + w] = acc[p] * (1.0 - exp(-fg * hkt));
+      row[w] = acc[p] * (1.0 - exp(-fg * hkt));
     }
   }
   if (kStats) {
-    for (int off = 32; off > 0; off >>= 1) {
-      nev += __shfl_down(nev, off);
-      ngen += __shfl_down(ngen, off);
-    }
-    if (lane == 0) {
-      atomicAdd(evals, nev);
-      atomicAdd(evals + 1, ngen);
-    }
+    for (int off = 32; off > 0; off >>= 1) nev += __shfl_down(nev, off);
+    if (lane == 0) atomicAdd(evals, nev);
   }
 }
 
@@ -1394,6 +1492,7 @@ int synthe_asynth(synthe_ctx* ctx, const synthe_lines* lines, const synthe_depth
   size_t o_lcm = off; off = align_up(off + (size_t)pairs * sizeof(WingHead));
   size_t o_lp = off; off = align_up(off + (size_t)pairs * sizeof(WingParams));
   size_t o_ev = off; off = align_up(off + 5 * sizeof(unsigned long long));
+  size_t o_nr = off; off = align_up(off + (size_t)D * sizeof(int32_t));
   rc = ensure_ws(ctx, off);
   if (rc) return rc;
   char* base = (char*)ctx->ws;
@@ -1405,6 +1504,7 @@ int synthe_asynth(synthe_ctx* ctx, const synthe_lines* lines, const synthe_depth
   WingHead* lcm = (WingHead*)(base + o_lcm);
   WingParams* lp = (WingParams*)(base + o_lp);
   unsigned long long* ev = (unsigned long long*)(base + o_ev);
+  int32_t* near_reach = (int32_t*)(base + o_nr);
   HIP_TRY(hipMemsetAsync(ev, 0, 5 * sizeof(unsigned long long), st));
   if (pairs > 0) {
     hipLaunchKernelGGL(k_transp_wing, dim3(blocks_for(pairs, kBlock)), dim3(kBlock), 0, st,
@@ -1425,18 +1525,32 @@ int synthe_asynth(synthe_ctx* ctx, const synthe_lines* lines, const synthe_depth
   } else {
     HIP_TRY(hipMemsetAsync(reg, 0, (size_t)D * (kClasses + 1) * sizeof(int64_t), st));
   }
+  hipLaunchKernelGGL(k_zero_i32, dim3(blocks_for(D, 64)), dim3(64), 0, st, D, near_reach);
+  HIP_TRY(hipGetLastError());
+  if (pairs > 0) {
+    hipLaunchKernelGGL(k_near_reach, dim3(64, (unsigned)D), dim3(kBlock), 0, st, D, reg, lcm, near_reach);
+    HIP_TRY(hipGetLastError());
+  }
+  const dim3 g_near((unsigned)blocks_for(n_wl, 256), (unsigned)D);
+  if (stats)
+    hipLaunchKernelGGL(k_near<true>, g_near, dim3(256), 0, st, n_wl, D, lcm, lp, reg, near_reach, ctx->tb,
+                       asynth, ev + 3);
+  else
+    hipLaunchKernelGGL(k_near<false>, g_near, dim3(256), 0, st, n_wl, D, lcm, lp, reg, near_reach, ctx->tb,
+                       asynth, nullptr);
+  HIP_TRY(hipGetLastError());
   {
     int rc = timing_mark(ctx, st, true);
     if (rc) return rc;
   }
+  const dim3 g_gather((unsigned)blocks_for(n_wl, kWT), (unsigned)D);
   if (stats)
-    hipLaunchKernelGGL(k_gather<true>, dim3(blocks_for(n_wl, kWT), (unsigned)D), dim3(64), 0, st, n_wl, D,
-                       center_order, center_start, lines->center_t, transp, valid, lcm, lp, reg, wl,
-                       depths->temperature, ctx->tb, asynth, ev + 3);
+    hipLaunchKernelGGL(k_gather<true>, g_gather, dim3(64), 0, st, n_wl, D, center_order, center_start,
+                       lines->center_t, transp, valid, lcm, lp, reg, wl, depths->temperature, asynth, ev + 3);
   else
-    hipLaunchKernelGGL(k_gather<false>, dim3(blocks_for(n_wl, kWT), (unsigned)D), dim3(64), 0, st, n_wl, D,
-                       center_order, center_start, lines->center_t, transp, valid, lcm, lp, reg, wl,
-                       depths->temperature, ctx->tb, asynth, nullptr);
+    hipLaunchKernelGGL(k_gather<false>, g_gather, dim3(64), 0, st, n_wl, D, center_order, center_start,
+                       lines->center_t, transp, valid, lcm, lp, reg, wl, depths->temperature, asynth,
+                       nullptr);
   HIP_TRY(hipGetLastError());
   {
     int rc = timing_mark(ctx, st, false);

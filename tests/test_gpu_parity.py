@@ -63,6 +63,7 @@ def test_stage_parity_small(gpu, oracle, small_workload):
     assert np.array_equal(v, st["valid"])
     assert _rel(plan.transp.cpu().numpy(), st["transp"]) < 1e-14
     asy = plan.asynth.cpu().numpy()
+    assert np.isfinite(asy).all() and np.isfinite(st["asynth"]).all()
     assert _rel(asy, st["asynth"], floor=1e-300) < 1e-12
     stats = plan.stats.cpu().numpy()
     assert np.array_equal(plan.iters.cpu().numpy(), ref.stages["iters"])
@@ -98,6 +99,8 @@ def test_determinism(gpu, small_workload):
     plan.run()
     a = (plan.asynth.clone(), plan.flux.clone())
     plan.run()
+    assert bool((plan.asynth == a[0]).all()) and bool((plan.flux == a[1]).all())
+    plan.run(stats=True)  # the instrumented kernels give the same bits
     assert bool((plan.asynth == a[0]).all()) and bool((plan.flux == a[1]).all())
 
 
