@@ -93,6 +93,46 @@ def cpu_baseline(wk, cfg, budget_s: float = 20.0):
     }
 
 
+def fp32_form(plan, args, e_line):
+    """The same spectrum with single-precision far-wing terms: timed like the
+    plain form, and its flux compared with the FP64 result under the robust
+    mask |F| > 1e-3 median(C) (reference tools/compare_spectra.py:128-135)."""
+    import torch
+
+    torch.cuda.synchronize()
+    f64 = plan.flux.cpu().numpy()
+    c64 = plan.cont.cpu().numpy()
+    plan.precision = "fp32"
+    try:
+        for _ in range(args.warmup):
+            plan.run()
+        torch.cuda.synchronize()
+        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        plan.ctx.set_kernel_timing(True)
+        start.record()
+        for _ in range(args.steps):
+            plan.run()
+        end.record()
+        torch.cuda.synchronize()
+        g = plan.ctx.kernel_times()
+        plan.ctx.set_kernel_timing(False)
+        ms = start.elapsed_time(end) / args.steps
+        f32 = plan.flux.cpu().numpy()
+        c32 = plan.cont.cpu().numpy()
+    finally:
+        plan.precision = "fp64"
+    floor = 1e-3 * float(np.median(c64))
+    mask = np.abs(f64) > floor
+    err = float(np.max(np.abs(f32[mask] - f64[mask]) / np.abs(f64[mask]))) if mask.any() else 0.0
+    err_c = float(np.max(np.abs(c32 - c64) / np.maximum(np.abs(c64), 1e-300)))
+    log(f"[bench] fp32 profiles: {ms:.3f} ms per spectrum, max rel flux err {err:.3g}")
+    return {"value": e_line / (ms * 1e-3), "unit": "evals/s", "ms_per_step": ms,
+            "k_gather_ms": float(np.mean(g)) if len(g) else None,
+            "max_rel_flux_err_vs_fp64": err, "max_rel_cont_err_vs_fp64": err_c,
+            "note": "far-wing X/k^2 terms at single precision (hardware reciprocal), FP64 accumulation, "
+                    "FP64 populations, continuum, Voigt-zone values and RT"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -171,6 +211,7 @@ def main() -> int:
     plan.ctx.set_kernel_timing(False)
     ms = start.elapsed_time(end) / args.steps
     log(f"[bench] {args.steps} steps, {ms:.3f} ms per spectrum; k_gather {np.round(gather_ms, 3).tolist()} ms")
+    variants = {"fp32_profiles": fp32_form(plan, args, e_line)}
 
     # dominant kernel: k_gather (centre adds + wing sums + STIM), one launch per
     # spectrum.  Algorithmic flops per evaluation: 3 for the far-wing value
@@ -215,6 +256,7 @@ def main() -> int:
                    "n_lines": len(plan.soa), "e_line": e_line, "spectrum_ms": ms,
                    "stage_ms": stages},
         "roofline": roof,
+        "variants": variants,
     }
     if not args.no_cpu_baseline:
         t0 = time.perf_counter()

@@ -91,6 +91,16 @@ int synthe_destroy(synthe_ctx *ctx);
 int synthe_copy_tables(const synthe_ctx *ctx, double *h0, double *h1, double *h2, double *extab,
                        double *extabf);
 
+/* Profile precision of synthe_asynth's far-wing terms X / k^2 (no reference
+ * counterpart; the reference is FP64 throughout).  SYNTHE_PREC_FP64 (the
+ * default): each term within 32 ulp of the IEEE quotient.  SYNTHE_PREC_FP32:
+ * each term to single precision (~2^-22 relative) through the hardware
+ * reciprocal.  Both accumulate in FP64; populations, continuum, Voigt-zone
+ * values and the RT stay FP64.  Applies to later calls on the context. */
+#define SYNTHE_PREC_FP64 0
+#define SYNTHE_PREC_FP32 1
+int synthe_set_profile_precision(synthe_ctx *ctx, int precision);
+
 /* Kernel timing (no reference counterpart; for benchmarks).  While enabled,
  * each synthe_asynth call records a pair of HIP events on its stream around
  * its wing-gather kernel (up to 256 launches; later ones are not recorded).
@@ -171,10 +181,11 @@ int synthe_source_function(synthe_ctx *ctx, long n_depths, long n_wl, const doub
 int synthe_flux_to_lambda(synthe_ctx *ctx, long n_wl, const double *wl, double *flux,
                           synthe_stream stream);
 
-/* Diagnostic (no reference counterpart): evaluates the far-wing quotient
- * routine q_fast = X / b (variant 0 = the one the kernels use; 1, 2 = fewer
- * Newton steps) and IEEE q_ieee = X / b on n device inputs, so tests can
- * check they agree bit for bit. */
+/* Diagnostic (no reference counterpart): evaluates a far-wing quotient
+ * routine q_fast = X / b and IEEE q_ieee = X / b on n device inputs, so tests
+ * can compare them.  variant 0 = the correctly rounded reciprocal sequence;
+ * 1, 2 = it with one or no Newton step before the remainder correction;
+ * 3 = X * rcp(b); 4 = X * (rcp(b) after one Newton step), the wing gather's. */
 int synthe_diag_div_pos(long n, int variant, const double *x, const double *b, double *q_fast,
                         double *q_ieee, synthe_stream stream);
 

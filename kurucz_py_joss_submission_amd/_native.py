@@ -25,8 +25,11 @@ SYMBOLS = (
     "synthe_copy_tables", "synthe_wavelength_grid", "synthe_continuum", "synthe_transp",
     "synthe_asynth", "synthe_lte_spectrum", "synthe_josh_flux_batch", "synthe_source_function",
     "synthe_flux_to_lambda", "synthe_diag_div_pos", "synthe_set_kernel_timing",
-    "synthe_kernel_times",
+    "synthe_kernel_times", "synthe_set_profile_precision",
 )
+
+SYNTHE_PREC_FP64 = 0
+SYNTHE_PREC_FP32 = 1
 
 
 class NativeUnavailable(RuntimeError):
@@ -92,6 +95,7 @@ def load_library(path: Optional[Path] = None):
         lib.synthe_flux_to_lambda.argtypes = [_VP, ct.c_long, _VP, _VP, _VP]
         lib.synthe_diag_div_pos.argtypes = [ct.c_long, ct.c_int] + [_VP] * 5
         lib.synthe_set_kernel_timing.argtypes = [_VP, ct.c_int]
+        lib.synthe_set_profile_precision.argtypes = [_VP, ct.c_int]
         lib.synthe_kernel_times.argtypes = [_VP, _VP, ct.c_int, ct.POINTER(ct.c_int)]
         for name in SYMBOLS[2:]:
             if name != "synthe_wavelength_grid":
@@ -131,6 +135,11 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def set_profile_precision(self, precision: str) -> None:
+        """'fp64' (default) or 'fp32' far-wing profile terms for later asynth calls."""
+        code = {"fp64": SYNTHE_PREC_FP64, "fp32": SYNTHE_PREC_FP32}[precision]
+        check("synthe_set_profile_precision", self.lib.synthe_set_profile_precision(self.handle, code))
 
     def set_kernel_timing(self, enable: bool) -> None:
         check("synthe_set_kernel_timing", self.lib.synthe_set_kernel_timing(self.handle, int(enable)))

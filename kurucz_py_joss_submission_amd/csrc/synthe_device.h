@@ -145,6 +145,20 @@ __device__ __forceinline__ double div_pos_inrange(double x, double b) {
   return __builtin_fma(__builtin_fma(-b, q, x), r, q);
 }
 
+// x / b within 32 ulp (measured <= 19 on gfx950 for b = k^2 <= 10^12):
+// hardware reciprocal (a ~23-bit approximation), one Newton step, product.
+__device__ __forceinline__ double div_fast(double x, double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  r = __builtin_fma(r, __builtin_fma(-b, r, 1.0), r);
+  return x * r;
+}
+
+// x / b to single-precision accuracy (~2^-22 relative): the hardware
+// reciprocal alone.
+__device__ __forceinline__ double div_single(double x, double b) {
+  return x * __builtin_amdgcn_rcp(b);
+}
+
 __device__ __forceinline__ double wing_value(const WingParams& p, long k, const double* h0t,
                                              const double* h1t, const double* h2t) {
   if (p.use_far && k > p.n10dop) return div_pos(p.x_far, (double)k * (double)k);

@@ -581,8 +581,23 @@ __global__ __launch_bounds__(256) void k_near(long W, int D, const WingHead* __r
   }
 }
 
-constexpr int kWP = 8;              // consecutive pixels per lane
+#ifndef SYNTHE_GATHER_PIXELS
+#define SYNTHE_GATHER_PIXELS 8
+#endif
+constexpr int kWP = SYNTHE_GATHER_PIXELS;  // consecutive pixels per lane
 constexpr int kWT = 64 * kWP;       // pixels per wave (one wave per workgroup)
+
+// Profile precision of the far-wing term X / k^2 (accumulation is FP64 in
+// both): kPrecFP64 within 32 ulp of the IEEE quotient, kPrecFP32 at single
+// precision.
+constexpr int kPrecFP64 = 0;
+constexpr int kPrecFP32 = 1;
+
+template <int kPrec>
+__device__ __forceinline__ double far_quotient(double x, double b) {
+  if constexpr (kPrec == kPrecFP32) return div_single(x, b);
+  else return div_fast(x, b);
+}
 
 #ifndef SYNTHE_GATHER_WAVES
 #define SYNTHE_GATHER_WAVES 7  // waves per SIMD the register budget is sized for
@@ -599,7 +614,7 @@ constexpr int kWT = 64 * kWP;       // pixels per wave (one wave per workgroup)
 // wing's N10DOP or MAXSTEP edge, pixels outside the far zone add +0.  Each
 // pixel's sum has a fixed order (Voigt zones, centres, far wings by class and
 // centre), no atomics, so results are run-to-run identical.
-template <bool kStats>
+template <bool kStats, int kPrec>
 __global__ __launch_bounds__(64, SYNTHE_GATHER_WAVES) void k_gather(
     long W, int D, const int64_t* __restrict__ corder, const int64_t* __restrict__ cstart,
     const int64_t* __restrict__ center_t, const double* __restrict__ transp,
@@ -660,7 +675,7 @@ __global__ __launch_bounds__(64, SYNTHE_GATHER_WAVES) void k_gather(
 #pragma unroll
         for (int p = 0; p < kWP; ++p) {
           const double k = kd + (double)p;  // |k| is the offset; k * k is exact
-          acc[p] += div_pos_inrange(x, k * k);
+          acc[p] += far_quotient<kPrec>(x, k * k);
         }
         if (kStats) {
           long n = W - w0 < kWP ? W - w0 : kWP;
@@ -675,7 +690,7 @@ __global__ __launch_bounds__(64, SYNTHE_GATHER_WAVES) void k_gather(
             const double ka = fabs(k);
             const double xp = (ka > n10d && ka <= md) ? x : 0.0;
             // 0 / max(k^2, 1) adds +0, also at the centre pixel (k = 0)
-            acc[p] += div_pos_inrange(xp, fmax(k * k, 1.0));
+            acc[p] += far_quotient<kPrec>(xp, fmax(k * k, 1.0));
           }
         } else {
           for (int p = 0; p < kWP; ++p) {
@@ -1099,6 +1114,12 @@ __global__ void k_check_div(long n, int variant, const double* x, const double* 
   double xv = x[i], bv = b[i], q;
   if (variant == 0) {
     q = div_pos(xv, bv);
+  } else if (variant == 3) {
+    q = xv * __builtin_amdgcn_rcp(bv);
+  } else if (variant == 4) {
+    double r = __builtin_amdgcn_rcp(bv);
+    r = __builtin_fma(r, __builtin_fma(-bv, r, 1.0), r);
+    q = xv * r;
   } else {
     double r = __builtin_amdgcn_rcp(bv);
     if (variant == 1) r = __builtin_fma(r, __builtin_fma(-bv, r, 1.0), r);
@@ -1219,6 +1240,7 @@ struct synthe_ctx {
   size_t ws_size = 0;
   // kernel timing: event pairs around each wing-gather launch while enabled
   static constexpr int kTimedMax = 256;
+  int precision = SYNTHE_PREC_FP64;  // profile precision of synthe_asynth
   bool timing = false;
   int n_timed = 0;
   hipEvent_t ev_beg[kTimedMax] = {};
@@ -1345,6 +1367,14 @@ int synthe_copy_tables(const synthe_ctx* ctx, double* h0, double* h1, double* h2
   if (h2) memcpy(h2, ctx->h2.data(), kVoigtN * sizeof(double));
   if (extab) memcpy(extab, ctx->extab.data(), kFastexN * sizeof(double));
   if (extabf) memcpy(extabf, ctx->extabf.data(), kFastexN * sizeof(double));
+  return SYNTHE_OK;
+}
+
+int synthe_set_profile_precision(synthe_ctx* ctx, int precision) {
+  if (!ctx) return fail(SYNTHE_EINVAL, "synthe_set_profile_precision: ctx is NULL");
+  if (precision != SYNTHE_PREC_FP64 && precision != SYNTHE_PREC_FP32)
+    return fail(SYNTHE_EINVAL, "synthe_set_profile_precision: unknown precision " + std::to_string(precision));
+  ctx->precision = precision;
   return SYNTHE_OK;
 }
 
@@ -1544,13 +1574,10 @@ int synthe_asynth(synthe_ctx* ctx, const synthe_lines* lines, const synthe_depth
     if (rc) return rc;
   }
   const dim3 g_gather((unsigned)blocks_for(n_wl, kWT), (unsigned)D);
-  if (stats)
-    hipLaunchKernelGGL(k_gather<true>, g_gather, dim3(64), 0, st, n_wl, D, center_order, center_start,
-                       lines->center_t, transp, valid, lcm, lp, reg, wl, depths->temperature, asynth, ev + 3);
-  else
-    hipLaunchKernelGGL(k_gather<false>, g_gather, dim3(64), 0, st, n_wl, D, center_order, center_start,
-                       lines->center_t, transp, valid, lcm, lp, reg, wl, depths->temperature, asynth,
-                       nullptr);
+  auto gather = stats ? (ctx->precision == SYNTHE_PREC_FP32 ? k_gather<true, kPrecFP32> : k_gather<true, kPrecFP64>)
+                      : (ctx->precision == SYNTHE_PREC_FP32 ? k_gather<false, kPrecFP32> : k_gather<false, kPrecFP64>);
+  hipLaunchKernelGGL(gather, g_gather, dim3(64), 0, st, n_wl, D, center_order, center_start, lines->center_t,
+                     transp, valid, lcm, lp, reg, wl, depths->temperature, asynth, stats ? ev + 3 : nullptr);
   HIP_TRY(hipGetLastError());
   {
     int rc = timing_mark(ctx, st, false);

@@ -141,9 +141,12 @@ class SpectrumPlan:
 
     def __init__(self, atm: AtmosphereModel, catalog: LineCatalog, wl_start: float, wl_end: float,
                  resolution: float, cutoff: float = 1e-3, microturb_kms: float = 0.0,
-                 rhoxj: float = 0.0, grid: Optional[np.ndarray] = None):
+                 rhoxj: float = 0.0, grid: Optional[np.ndarray] = None, precision: str = "fp64"):
         torch = _torch()
         self.ctx = N.context()
+        if precision not in ("fp64", "fp32"):
+            raise ValueError(f"precision must be 'fp64' or 'fp32', not {precision!r}")
+        self.precision = precision
         self.grid = N.wavelength_grid(wl_start, wl_end, resolution) if grid is None else np.asarray(grid)
         self.W = int(self.grid.size)
         self.D = atm.layers
@@ -205,6 +208,7 @@ class SpectrumPlan:
     def run_asynth(self, stream=None, stats: bool = False):
         s = stream if stream is not None else N.stream_handle()
         ls, ds = self.lines.struct(), self.depth.struct()
+        self.ctx.set_profile_precision(self.precision)
         N.check("synthe_asynth", self.ctx.lib.synthe_asynth(
             self.ctx.handle, ct_byref(ls), ct_byref(ds), self.W, N.ptr(self.d_grid),
             N.ptr(self.transp), N.ptr(self.valid), N.ptr(self.kapmin), self.cutoff,

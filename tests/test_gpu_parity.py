@@ -161,3 +161,49 @@ def test_far_wing_division_bitwise(gpu):
     qf, qi = qf.cpu().numpy(), qi.cpu().numpy()
     assert np.array_equal(qf, qi)
     assert np.array_equal(qi, x / b)  # and IEEE on the GPU equals numpy's
+
+
+def test_far_wing_fast_quotient_ulps(gpu):
+    """The gather's far-wing quotient X * rcp'(k^2) (hardware reciprocal plus
+    one Newton step) is within 32 ulp (7e-15 relative) of IEEE X / k^2 over
+    the operand ranges of the far wing.  v_rcp_f64 alone is a ~23-bit
+    approximation (measured up to 4e8 ulp), which the Newton step squares."""
+    import torch
+    from kurucz_py_joss_submission_amd import _native as N
+
+    rng = np.random.default_rng(12)
+    n = 1 << 21
+    x = 10.0 ** rng.uniform(-250, 250, n)
+    k = rng.integers(1, 1_000_001, n).astype(np.float64)
+    b = k * k
+    b[:1000] = np.arange(1, 1001, dtype=np.float64) ** 2
+    dx, db = torch.from_numpy(x).cuda(), torch.from_numpy(b).cuda()
+    ulps = {}
+    for variant in (3, 4):
+        qf = torch.empty_like(dx)
+        qi = torch.empty_like(dx)
+        N.check("synthe_diag_div_pos", N.load_library().synthe_diag_div_pos(
+            n, variant, dx.data_ptr(), db.data_ptr(), qf.data_ptr(), qi.data_ptr(), N.stream_handle()))
+        qf, qi = qf.cpu().numpy(), qi.cpu().numpy()
+        ulps[variant] = float(np.max(np.abs(qf - qi) / np.spacing(qi)))
+    print("max ulp error: raw rcp", ulps[3], "rcp + Newton", ulps[4])
+    assert ulps[3] < 2.0 ** 31
+    assert ulps[4] <= 32
+
+
+def test_fp32_profiles_flux_error(gpu, oracle, c1_workload):
+    """Single-precision far-wing terms (FP64 accumulation, populations and RT):
+    the same evaluation set, and flux within 1e-4 of the FP64 oracle under
+    the robust mask |F| > 1e-3 median(C) (tools/compare_spectra.py:128-135)."""
+    wk = c1_workload
+    plan = _plan(wk, precision="fp32")
+    plan.run(stats=True)
+    ref = oracle.synthesize(wk.atmosphere, wk.catalog, wk.wl_start, wk.wl_end, wk.resolution)
+    flux, cont = plan.flux.cpu().numpy(), plan.cont.cpu().numpy()
+    assert np.isfinite(plan.asynth.cpu().numpy()).all()
+    assert plan.stats.cpu().numpy()[0] == ref.stats["e_line"]
+    assert _rel(cont, ref.continuum) < FLUX_RTOL
+    floor = 1e-3 * np.median(ref.continuum)
+    err = _rel(flux, ref.flux, floor=floor)
+    print("fp32 profiles: max relative flux error vs FP64 oracle", err)
+    assert err < 1e-4
