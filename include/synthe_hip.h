@@ -189,6 +189,50 @@ int synthe_flux_to_lambda(synthe_ctx *ctx, long n_wl, const double *wl, double *
 int synthe_diag_div_pos(long n, int variant, const double *x, const double *b, double *q_fast,
                         double *q_ieee, synthe_stream stream);
 
+/* ------------------------------------------------------------------ batches
+ * Many spectra (atmospheres of different depth counts, different windows)
+ * in one replayable unit.  No reference counterpart: the reference runs one
+ * spectrum per process (engine/opacity.py:2084 run_synthesis); a batch runs
+ * that forward pass (continuum, TRANSP, ASYNTH, LTE RT, F_lambda) for every
+ * instance, in LTE with no line scattering, with one persistent launch of the
+ * wing gather over all instances' (depth, tile) items.
+ *
+ * Inputs are device arrays owned by the caller and must stay valid and
+ * unchanged in shape for the batch's life (their contents may change between
+ * runs).  Outputs flux and cont [n_wl] receive F_lambda of the total and the
+ * continuum-only solve.  The library allocates one arena for everything
+ * else at create time; synthe_batch_run allocates nothing. */
+typedef struct {
+  synthe_lines lines;      /* this instance's line SoA (its own centre indices) */
+  synthe_depths depths;    /* n_depths <= SYNTHE_MAX_DEPTH */
+  long n_wl;
+  const double *wl;        /* [n_wl] grid */
+  long n_edges;            /* continuum coefficients as synthe_continuum */
+  const double *wledge, *half_edge, *delta_edge, *coeff_abs, *coeff_scat;
+  int coeff_log10;
+  const int64_t *center_order, *center_start, *wing_order; /* as synthe_asynth */
+  double cutoff;
+  double *flux, *cont;     /* outputs [n_wl], F_lambda */
+} synthe_instance;
+
+typedef struct synthe_batch synthe_batch;
+
+#define SYNTHE_BATCH_AUTOTUNE 1 /* time the gather's tile / grid candidates on this batch, keep the fastest */
+#define SYNTHE_BATCH_GRAPH 2    /* capture the whole batch into a hipGraph replayed by synthe_batch_run */
+
+/* Plan a batch of n instances (host array).  Uses the context's profile
+ * precision at create time.  Synchronous. */
+int synthe_batch_create(synthe_ctx *ctx, int n, const synthe_instance *inst, int flags, synthe_batch **out);
+/* Run the batch on stream (graph replay when created with SYNTHE_BATCH_GRAPH). */
+int synthe_batch_run(synthe_batch *b, synthe_stream stream);
+/* The gather's chosen pixels per lane (tile = 64 x that) and persistent grid
+ * (workgroups of 64), the autotuner's candidate times (ms, tile-major:
+ * 4, 8, 16 pixels x 2, 4, 8 waves per SIMD; up to tune_cap) and the arena
+ * size.  Any pointer may be NULL. */
+int synthe_batch_info(const synthe_batch *b, int *pixels_per_lane, int *gather_blocks, double *tune_ms,
+                      int tune_cap, int *n_tuned, double *arena_bytes);
+int synthe_batch_destroy(synthe_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

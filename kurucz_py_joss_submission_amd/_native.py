@@ -25,8 +25,12 @@ SYMBOLS = (
     "synthe_copy_tables", "synthe_wavelength_grid", "synthe_continuum", "synthe_transp",
     "synthe_asynth", "synthe_lte_spectrum", "synthe_josh_flux_batch", "synthe_source_function",
     "synthe_flux_to_lambda", "synthe_diag_div_pos", "synthe_set_kernel_timing",
-    "synthe_kernel_times", "synthe_set_profile_precision",
+    "synthe_kernel_times", "synthe_set_profile_precision", "synthe_batch_create", "synthe_batch_run",
+    "synthe_batch_info", "synthe_batch_destroy",
 )
+
+SYNTHE_BATCH_AUTOTUNE = 1
+SYNTHE_BATCH_GRAPH = 2
 
 SYNTHE_PREC_FP64 = 0
 SYNTHE_PREC_FP32 = 1
@@ -58,6 +62,16 @@ class Depths(ct.Structure):
     _fields_ = [("n_depths", ct.c_long), ("temperature", _VP), ("rhox", _VP), ("rho", _VP),
                 ("xne", _VP), ("txnxn", _VP), ("hckt", _VP), ("pop", _VP), ("dop", _VP),
                 ("n_ion", ct.c_long), ("n_elem", ct.c_long)]
+
+
+class Instance(ct.Structure):
+    """synthe_instance: one spectrum of a batch (include/synthe_hip.h)."""
+
+    _fields_ = [("lines", Lines), ("depths", Depths), ("n_wl", ct.c_long), ("wl", _VP),
+                ("n_edges", ct.c_long), ("wledge", _VP), ("half_edge", _VP), ("delta_edge", _VP),
+                ("coeff_abs", _VP), ("coeff_scat", _VP), ("coeff_log10", ct.c_int),
+                ("center_order", _VP), ("center_start", _VP), ("wing_order", _VP),
+                ("cutoff", ct.c_double), ("flux", _VP), ("cont", _VP)]
 
 
 _lib = None
@@ -96,6 +110,11 @@ def load_library(path: Optional[Path] = None):
         lib.synthe_diag_div_pos.argtypes = [ct.c_long, ct.c_int] + [_VP] * 5
         lib.synthe_set_kernel_timing.argtypes = [_VP, ct.c_int]
         lib.synthe_set_profile_precision.argtypes = [_VP, ct.c_int]
+        lib.synthe_batch_create.argtypes = [_VP, ct.c_int, ct.POINTER(Instance), ct.c_int, ct.POINTER(_VP)]
+        lib.synthe_batch_run.argtypes = [_VP, _VP]
+        lib.synthe_batch_info.argtypes = [_VP, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int), _VP, ct.c_int,
+                                          ct.POINTER(ct.c_int), ct.POINTER(ct.c_double)]
+        lib.synthe_batch_destroy.argtypes = [_VP]
         lib.synthe_kernel_times.argtypes = [_VP, _VP, ct.c_int, ct.POINTER(ct.c_int)]
         for name in SYMBOLS[2:]:
             if name != "synthe_wavelength_grid":

@@ -19,6 +19,7 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -238,7 +239,8 @@ __global__ void k_source(int D, long W, const double* __restrict__ wl,
 // pair per thread.  mode bit 0: compute transp/valid (else read transp_in /
 // valid_in); bit 1: compute wing parameters and the reach class.
 __global__ void k_transp_wing(LinesDev ln, DepthsDev dp, long W, const double* __restrict__ kapmin,
-                              double cutoff, double micro, double resolu, DeviceTables tb, int mode,
+                              double cutoff, double micro, const double* __restrict__ wl, DeviceTables tb,
+                              int mode,
                               double* transp, uint8_t* valid, const double* transp_in,
                               const uint8_t* valid_in, WingParams* wp, uint8_t* cls) {
   long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -293,6 +295,12 @@ __global__ void k_transp_wing(LinesDev ln, DepthsDev dp, long W, const double* _
     ok = valid_in[i] != 0;
   }
   if (!(mode & 2)) return;
+  // resolu from the grid's first ratio (line_opacity.py:233-238)
+  double resolu = 300000.0;
+  if (wl && W > 1) {
+    double ratio = wl[1] / wl[0];
+    if (ratio > 1.0) resolu = 1.0 / (ratio - 1.0);
+  }
   uint8_t k_out = 0xFF;
   long c = ln.center_w[l];
   int ei = ln.elem[l], nel = ln.ion[l];
@@ -498,6 +506,17 @@ __global__ __launch_bounds__(kBlock) void k_near_reach(int D, const int64_t* __r
   if ((threadIdx.x & 63) == 0 && m > 0) atomicMax(near_reach + d, m);
 }
 
+__global__ void k_add2(long n, const double* __restrict__ a, const double* __restrict__ b,
+                       double* __restrict__ out) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n) out[i] = a[i] + b[i];
+}
+
+__global__ void k_fill_u64(long n, unsigned long long* p) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
 __global__ void k_zero_i32(int n, int32_t* p) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
@@ -587,6 +606,22 @@ __global__ __launch_bounds__(256) void k_near(long W, int D, const WingHead* __r
 constexpr int kWP = SYNTHE_GATHER_PIXELS;  // consecutive pixels per lane
 constexpr int kWT = 64 * kWP;       // pixels per wave (one wave per workgroup)
 
+// Inputs and output of the wing gather for one spectrum (device pointers).
+struct GatherArgs {
+  long W;
+  int D;
+  const int64_t* corder;
+  const int64_t* cstart;
+  const double* transp;
+  const uint8_t* valid;
+  const WingHead* list_cm;
+  const WingParams* list_p;
+  const int64_t* regions;
+  const double* grid;
+  const double* temp;
+  double* asynth;
+};
+
 // Profile precision of the far-wing term X / k^2 (accumulation is FP64 in
 // both): kPrecFP64 within 32 ulp of the IEEE quotient, kPrecFP32 at single
 // precision.
@@ -614,17 +649,22 @@ __device__ __forceinline__ double far_quotient(double x, double b) {
 // wing's N10DOP or MAXSTEP edge, pixels outside the far zone add +0.  Each
 // pixel's sum has a fixed order (Voigt zones, centres, far wings by class and
 // centre), no atomics, so results are run-to-run identical.
-template <bool kStats, int kPrec>
-__global__ __launch_bounds__(64, SYNTHE_GATHER_WAVES) void k_gather(
-    long W, int D, const int64_t* __restrict__ corder, const int64_t* __restrict__ cstart,
-    const int64_t* __restrict__ center_t, const double* __restrict__ transp,
-    const uint8_t* __restrict__ valid, const WingHead* __restrict__ list_cm,
-    const WingParams* __restrict__ list_p, const int64_t* __restrict__ regions,
-    const double* __restrict__ grid, const double* __restrict__ temp,
-    double* __restrict__ asynth, unsigned long long* __restrict__ evals) {
-  const int d = blockIdx.y;
-  const long t0 = blockIdx.x * (long)kWT;
-  const int lane = threadIdx.x;
+template <bool kStats, int kPrec, int kWP>
+__device__ __forceinline__ void gather_tile(const GatherArgs& g, int d, long t0, int lane,
+                                            unsigned long long* __restrict__ evals) {
+  constexpr int kWT = 64 * kWP;
+  const long W = g.W;
+  const int D = g.D;
+  const int64_t* __restrict__ corder = g.corder;
+  const int64_t* __restrict__ cstart = g.cstart;
+  const double* __restrict__ transp = g.transp;
+  const uint8_t* __restrict__ valid = g.valid;
+  const WingHead* __restrict__ list_cm = g.list_cm;
+  const WingParams* __restrict__ list_p = g.list_p;
+  const int64_t* __restrict__ regions = g.regions;
+  const double* __restrict__ grid = g.grid;
+  const double* __restrict__ temp = g.temp;
+  double* __restrict__ asynth = g.asynth;
   const long w0 = t0 + (long)kWP * lane;
   double* row = asynth + (long)d * W;
   double acc[kWP];
@@ -722,6 +762,45 @@ __global__ __launch_bounds__(64, SYNTHE_GATHER_WAVES) void k_gather(
   if (kStats) {
     for (int off = 32; off > 0; off >>= 1) nev += __shfl_down(nev, off);
     if (lane == 0) atomicAdd(evals, nev);
+  }
+}
+
+
+template <bool kStats, int kPrec>
+__global__ __launch_bounds__(64, SYNTHE_GATHER_WAVES) void k_gather(GatherArgs g,
+                                                                    unsigned long long* __restrict__ evals) {
+  gather_tile<kStats, kPrec, kWP>(g, blockIdx.y, blockIdx.x * (long)kWT, threadIdx.x, evals);
+}
+
+// Persistent form over many spectra: each wave takes (instance, depth, tile)
+// items from a queue until none is left.  items[i] .. items[i+1] are
+// instance order[i]'s items, depth-major; every item writes only its own
+// tile's pixels, so the result does not depend on which wave takes it.
+template <int kPrec, int kWPt>
+__global__ __launch_bounds__(64, kWPt == 16 ? 5 : 8) void k_gather_batch(const GatherArgs* __restrict__ inst,
+                                                     const int32_t* __restrict__ order,
+                                                     const int64_t* __restrict__ items, int n_inst,
+                                                     unsigned long long* __restrict__ counter) {
+  constexpr long kTile = 64L * kWPt;
+  const int lane = threadIdx.x;
+  const long n_items = items[n_inst];
+  for (;;) {
+    unsigned long long it = 0;
+    if (lane == 0) it = atomicAdd(counter, 1ull);
+    const long item = uniform_long((long)__shfl(it, 0));
+    if (item >= n_items) break;
+    int lo = 0, hi = n_inst - 1;  // last i with items[i] <= item
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (items[mid] <= item) lo = mid;
+      else hi = mid - 1;
+    }
+    const GatherArgs& g = inst[order[lo]];
+    const long local = item - items[lo];
+    const long ntiles = (g.W + kTile - 1) / kTile;
+    const int d = (int)(local / ntiles);
+    const long t = local - (long)d * ntiles;
+    gather_tile<false, kPrec, kWPt>(g, d, t * kTile, lane, nullptr);
   }
 }
 
@@ -1277,6 +1356,184 @@ static int ensure_ws(synthe_ctx* ctx, size_t bytes) {
 
 static size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
 
+
+static int check_lines_depths(const synthe_lines* ln, const synthe_depths* dp, const char* who) {
+  if (!ln || !dp) return fail(SYNTHE_EINVAL, std::string(who) + ": lines/depths is NULL");
+  if (ln->n_lines < 0 || dp->n_depths <= 0 || dp->n_depths > 65535 || dp->n_ion <= 0 || dp->n_elem <= 0)
+    return fail(SYNTHE_EINVAL, std::string(who) + ": bad sizes");
+  if (ln->n_lines > 0 &&
+      (!ln->wl || !ln->gf || !ln->cgf || !ln->gamma_rad || !ln->gamma_stark || !ln->gamma_vdw ||
+       !ln->elo || !ln->elem || !ln->ion || !ln->ltype || !ln->process || !ln->center_t ||
+       !ln->center_w))
+    return fail(SYNTHE_EINVAL, std::string(who) + ": NULL line array");
+  if (!dp->temperature || !dp->rhox || !dp->rho || !dp->xne || !dp->txnxn || !dp->hckt || !dp->pop ||
+      !dp->dop)
+    return fail(SYNTHE_EINVAL, std::string(who) + ": NULL depth array");
+  return SYNTHE_OK;
+}
+
+static LinesDev to_dev(const synthe_lines* l) {
+  return LinesDev{l->n_lines, l->wl,   l->gf,     l->cgf,   l->gamma_rad, l->gamma_stark, l->gamma_vdw,
+                  l->elo,     l->elem, l->ion,    l->ltype, l->process,   l->center_t,    l->center_w};
+}
+
+static DepthsDev to_dev(const synthe_depths* d) {
+  return DepthsDev{(int)d->n_depths, d->temperature, d->rhox, d->rho, d->xne, d->txnxn, d->hckt,
+                   d->pop,           d->dop,         (int)d->n_ion, (int)d->n_elem};
+}
+
+// ------------------------------------------------------------ ASYNTH pieces
+// Workspace of one ASYNTH call (offsets from a 256-byte aligned base).
+struct AsynthLayout {
+  size_t o_wp, o_cls, o_cnt, o_tot, o_reg, o_lcm, o_lp, o_ev, o_nr, total;
+};
+
+static AsynthLayout asynth_layout(long L, int D) {
+  AsynthLayout y{};
+  const long pairs = L * (long)D;
+  const long nblk = (L + kBlock - 1) / kBlock;
+  size_t off = 0;
+  y.o_wp = off; off = align_up(off + (size_t)pairs * sizeof(WingParams));
+  y.o_cls = off; off = align_up(off + (size_t)pairs);
+  y.o_cnt = off; off = align_up(off + (size_t)(nblk > 0 ? nblk : 1) * D * kClasses * sizeof(int32_t));
+  y.o_tot = off; off = align_up(off + (size_t)D * kClasses * sizeof(int64_t));
+  y.o_reg = off; off = align_up(off + (size_t)D * (kClasses + 1) * sizeof(int64_t));
+  y.o_lcm = off; off = align_up(off + (size_t)pairs * sizeof(WingHead));
+  y.o_lp = off; off = align_up(off + (size_t)pairs * sizeof(WingParams));
+  y.o_ev = off; off = align_up(off + 5 * sizeof(unsigned long long));
+  y.o_nr = off; off = align_up(off + (size_t)D * sizeof(int32_t));
+  y.total = off;
+  return y;
+}
+
+struct AsynthBufs {
+  WingParams* wp;
+  uint8_t* cls;
+  int32_t* cnt;
+  int64_t* tot;
+  int64_t* reg;
+  WingHead* lcm;
+  WingParams* lp;
+  unsigned long long* ev;
+  int32_t* near_reach;
+};
+
+static AsynthBufs asynth_bufs(char* base, const AsynthLayout& y) {
+  return AsynthBufs{(WingParams*)(base + y.o_wp), (uint8_t*)(base + y.o_cls), (int32_t*)(base + y.o_cnt),
+                    (int64_t*)(base + y.o_tot),   (int64_t*)(base + y.o_reg), (WingHead*)(base + y.o_lcm),
+                    (WingParams*)(base + y.o_lp), (unsigned long long*)(base + y.o_ev),
+                    (int32_t*)(base + y.o_nr)};
+}
+
+static GatherArgs gather_args(const synthe_lines* lines, const synthe_depths* depths, long n_wl, const double* wl,
+                              const double* transp, const uint8_t* valid, const int64_t* center_order,
+                              const int64_t* center_start, double* asynth, const AsynthBufs& b) {
+  (void)lines;
+  return GatherArgs{n_wl,   (int)depths->n_depths, center_order, center_start, transp, valid, b.lcm, b.lp,
+                    b.reg, wl,                     depths->temperature, asynth};
+}
+
+static int check_asynth_args(const synthe_lines* lines, const synthe_depths* depths, long n_wl, const double* wl,
+                             const double* transp, const uint8_t* valid, const double* kapmin,
+                             const int64_t* center_order, const int64_t* center_start,
+                             const int64_t* wing_order, const double* asynth, const char* who) {
+  int rc = check_lines_depths(lines, depths, who);
+  if (rc) return rc;
+  if (n_wl <= 0 || !wl || !asynth || !center_start)
+    return fail(SYNTHE_EINVAL, std::string(who) + ": wl/asynth/center_start required and n_wl > 0");
+  if (lines->n_lines > 0 && (!transp || !valid || !kapmin || !center_order || !wing_order))
+    return fail(SYNTHE_EINVAL, std::string(who) + ": NULL transp/valid/kapmin/order array");
+  return SYNTHE_OK;
+}
+
+// Everything of ASYNTH before the gather: wing set-up, class lists, Voigt
+// zones (into asynth).  Kernel launches only (no allocation, copy, memset or
+// synchronisation), so it can be captured into a graph.
+static int asynth_prepare(const synthe_ctx* ctx, const synthe_lines* lines, const synthe_depths* depths,
+                          long n_wl, const double* wl, const double* transp, const uint8_t* valid,
+                          const double* kapmin, double cutoff, const int64_t* wing_order, double* asynth,
+                          bool stats, const AsynthBufs& b, hipStream_t st) {
+  const long L = lines->n_lines;
+  const int D = (int)depths->n_depths;
+  const long pairs = L * (long)D;
+  const long nblk = (L + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, 5L, b.ev);
+  HIP_TRY(hipGetLastError());
+  if (pairs > 0) {
+    hipLaunchKernelGGL(k_transp_wing, dim3(blocks_for(pairs, kBlock)), dim3(kBlock), 0, st, to_dev(lines),
+                       to_dev(depths), n_wl, kapmin, cutoff, 0.0, wl, ctx->tb, 2, nullptr, nullptr, transp, valid,
+                       b.wp, b.cls);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_class_count, dim3((unsigned)nblk), dim3(kBlock), 0, st, L, D, wing_order, b.cls, b.cnt);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_class_scan, dim3((unsigned)(D * kClasses)), dim3(kBlock), 0, st, nblk, D, b.cnt, b.tot);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_regions, dim3(blocks_for(D, 64)), dim3(64), 0, st, D, L, b.tot, b.reg);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_class_scatter, dim3((unsigned)nblk), dim3(kBlock), 0, st, L, D, wing_order, b.cls, b.cnt,
+                       b.reg, b.wp, b.lcm, b.lp);
+    HIP_TRY(hipGetLastError());
+  } else {
+    const long n = (long)D * (kClasses + 1);
+    hipLaunchKernelGGL(k_fill_u64, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, (unsigned long long*)b.reg);
+    HIP_TRY(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_zero_i32, dim3(blocks_for(D, 64)), dim3(64), 0, st, D, b.near_reach);
+  HIP_TRY(hipGetLastError());
+  if (pairs > 0) {
+    hipLaunchKernelGGL(k_near_reach, dim3(64, (unsigned)D), dim3(kBlock), 0, st, D, b.reg, b.lcm, b.near_reach);
+    HIP_TRY(hipGetLastError());
+  }
+  const dim3 g_near((unsigned)blocks_for(n_wl, 256), (unsigned)D);
+  if (stats)
+    hipLaunchKernelGGL(k_near<true>, g_near, dim3(256), 0, st, n_wl, D, b.lcm, b.lp, b.reg, b.near_reach, ctx->tb,
+                       asynth, b.ev + 3);
+  else
+    hipLaunchKernelGGL(k_near<false>, g_near, dim3(256), 0, st, n_wl, D, b.lcm, b.lp, b.reg, b.near_reach,
+                       ctx->tb, asynth, nullptr);
+  HIP_TRY(hipGetLastError());
+  return SYNTHE_OK;
+}
+
+// ---------------------------------------------------------------- RT pieces
+struct RtLayout {
+  size_t o_flags, o_list, o_cnt, total;
+};
+
+static RtLayout rt_layout(long n_wl) {
+  RtLayout y{};
+  y.o_flags = align_up((size_t)2 * 3 * kNxtau * n_wl * sizeof(float));
+  y.o_list = y.o_flags + align_up((size_t)n_wl);
+  y.o_cnt = y.o_list + align_up((size_t)n_wl * sizeof(int));
+  y.total = y.o_cnt + 256;
+  return y;
+}
+
+// Both JOSH solves for every wavelength; kernel launches only (capturable).
+static int rt_launch(const synthe_ctx* ctx, long n_depths, long n_wl, const double* wl, const double* temperature,
+                     const double* rhox, const double* cont_abs, const double* cont_scat, const double* line_abs,
+                     const double* line_scat, const double* line_source, double* flux_total, double* flux_cont,
+                     int32_t* iters, char* base, const RtLayout& y, hipStream_t st) {
+  float* ws = (float*)base;
+  uint8_t* flags = (uint8_t*)base + y.o_flags;
+  int* list = (int*)(base + y.o_list);
+  int* list_n = (int*)(base + y.o_cnt);
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, list_n);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_rt_prep, dim3(blocks_for(n_wl, kRtBlock)), dim3(kRtBlock), 0, st, (int)n_depths, n_wl, wl,
+                     temperature, rhox, cont_abs, cont_scat, line_abs, line_scat, line_source, ctx->tb.xtau, ws,
+                     flags, list, list_n);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_rt_iter, dim3(blocks_for(n_wl, kRtLanes)), dim3(kRtLanes), 0, st, n_wl, flags, ws,
+                     flux_total, flux_cont, iters, ctx->tb);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_lte, dim3(blocks_for(n_wl, kRtLanes)), dim3(kRtLanes), 0, st, (int)n_depths, n_wl, list,
+                     list_n, wl, temperature, rhox, cont_abs, cont_scat, line_abs, line_scat, line_source,
+                     flux_total, flux_cont, iters, ctx->tb);
+  HIP_TRY(hipGetLastError());
+  return SYNTHE_OK;
+}
+
 extern "C" {
 
 const char* synthe_last_error(void) { return g_err.c_str(); }
@@ -1444,30 +1701,6 @@ int synthe_source_function(synthe_ctx* ctx, long n_depths, long n_wl, const doub
   return SYNTHE_OK;
 }
 
-static int check_lines_depths(const synthe_lines* ln, const synthe_depths* dp, const char* who) {
-  if (!ln || !dp) return fail(SYNTHE_EINVAL, std::string(who) + ": lines/depths is NULL");
-  if (ln->n_lines < 0 || dp->n_depths <= 0 || dp->n_depths > 65535 || dp->n_ion <= 0 || dp->n_elem <= 0)
-    return fail(SYNTHE_EINVAL, std::string(who) + ": bad sizes");
-  if (ln->n_lines > 0 &&
-      (!ln->wl || !ln->gf || !ln->cgf || !ln->gamma_rad || !ln->gamma_stark || !ln->gamma_vdw ||
-       !ln->elo || !ln->elem || !ln->ion || !ln->ltype || !ln->process || !ln->center_t ||
-       !ln->center_w))
-    return fail(SYNTHE_EINVAL, std::string(who) + ": NULL line array");
-  if (!dp->temperature || !dp->rhox || !dp->rho || !dp->xne || !dp->txnxn || !dp->hckt || !dp->pop ||
-      !dp->dop)
-    return fail(SYNTHE_EINVAL, std::string(who) + ": NULL depth array");
-  return SYNTHE_OK;
-}
-
-static LinesDev to_dev(const synthe_lines* l) {
-  return LinesDev{l->n_lines, l->wl,   l->gf,     l->cgf,   l->gamma_rad, l->gamma_stark, l->gamma_vdw,
-                  l->elo,     l->elem, l->ion,    l->ltype, l->process,   l->center_t,    l->center_w};
-}
-
-static DepthsDev to_dev(const synthe_depths* d) {
-  return DepthsDev{(int)d->n_depths, d->temperature, d->rhox, d->rho, d->xne, d->txnxn, d->hckt,
-                   d->pop,           d->dop,         (int)d->n_ion, (int)d->n_elem};
-}
 
 int synthe_transp(synthe_ctx* ctx, const synthe_lines* lines, const synthe_depths* depths, long n_wl,
                   const double* kapmin, double cutoff, double microturb_kms, double* transp,
@@ -1482,7 +1715,7 @@ int synthe_transp(synthe_ctx* ctx, const synthe_lines* lines, const synthe_depth
   double micro = microturb_kms > 0.0 ? microturb_kms / 299792.458 : 0.0;
   hipLaunchKernelGGL(k_transp_wing, dim3(blocks_for(pairs, kBlock)), dim3(kBlock), 0,
                      (hipStream_t)stream, to_dev(lines), to_dev(depths), n_wl, kapmin, cutoff, micro,
-                     300000.0, ctx->tb, 1, transp, valid, nullptr, nullptr, nullptr, nullptr);
+                     nullptr, ctx->tb, 1, transp, valid, nullptr, nullptr, nullptr, nullptr);
   HIP_TRY(hipGetLastError());
   return SYNTHE_OK;
 }
@@ -1492,104 +1725,37 @@ int synthe_asynth(synthe_ctx* ctx, const synthe_lines* lines, const synthe_depth
                   double cutoff, const int64_t* center_order, const int64_t* center_start,
                   const int64_t* wing_order, double* asynth, double* stats, synthe_stream stream) {
   if (!ctx) return fail(SYNTHE_EINVAL, "synthe_asynth: ctx is NULL");
-  int rc = check_lines_depths(lines, depths, "synthe_asynth");
+  int rc = check_asynth_args(lines, depths, n_wl, wl, transp, valid, kapmin, center_order, center_start,
+                             wing_order, asynth, "synthe_asynth");
   if (rc) return rc;
-  if (n_wl <= 0 || !wl || !asynth || !center_start)
-    return fail(SYNTHE_EINVAL, "synthe_asynth: wl/asynth/center_start required and n_wl > 0");
-  long L = lines->n_lines;
-  int D = (int)depths->n_depths;
-  if (L > 0 && (!transp || !valid || !kapmin || !center_order || !wing_order))
-    return fail(SYNTHE_EINVAL, "synthe_asynth: NULL transp/valid/kapmin/order array");
   hipStream_t st = (hipStream_t)stream;
-  // resolu from the grid (line_opacity.py:233-238) is computed on the host
-  // from the first two grid points.
-  double g01[2] = {0.0, 0.0};
-  double resolu = 300000.0;
-  if (n_wl > 1) {
-    HIP_TRY(hipMemcpyAsync(g01, wl, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    double ratio = g01[1] / g01[0];
-    if (ratio > 1.0) resolu = 1.0 / (ratio - 1.0);
-  }
-  long pairs = L * (long)D;
-  long nblk = (L + kBlock - 1) / kBlock;
-  size_t off = 0;
-  size_t o_wp = off; off = align_up(off + (size_t)pairs * sizeof(WingParams));
-  size_t o_cls = off; off = align_up(off + (size_t)pairs);
-  size_t o_cnt = off; off = align_up(off + (size_t)(nblk > 0 ? nblk : 1) * D * kClasses * sizeof(int32_t));
-  size_t o_tot = off; off = align_up(off + (size_t)D * kClasses * sizeof(int64_t));
-  size_t o_reg = off; off = align_up(off + (size_t)D * (kClasses + 1) * sizeof(int64_t));
-  size_t o_lcm = off; off = align_up(off + (size_t)pairs * sizeof(WingHead));
-  size_t o_lp = off; off = align_up(off + (size_t)pairs * sizeof(WingParams));
-  size_t o_ev = off; off = align_up(off + 5 * sizeof(unsigned long long));
-  size_t o_nr = off; off = align_up(off + (size_t)D * sizeof(int32_t));
-  rc = ensure_ws(ctx, off);
+  const long L = lines->n_lines;
+  const int D = (int)depths->n_depths;
+  const AsynthLayout lay = asynth_layout(L, D);
+  rc = ensure_ws(ctx, lay.total);
   if (rc) return rc;
-  char* base = (char*)ctx->ws;
-  WingParams* wp = (WingParams*)(base + o_wp);
-  uint8_t* cls = (uint8_t*)(base + o_cls);
-  int32_t* cnt = (int32_t*)(base + o_cnt);
-  int64_t* tot = (int64_t*)(base + o_tot);
-  int64_t* reg = (int64_t*)(base + o_reg);
-  WingHead* lcm = (WingHead*)(base + o_lcm);
-  WingParams* lp = (WingParams*)(base + o_lp);
-  unsigned long long* ev = (unsigned long long*)(base + o_ev);
-  int32_t* near_reach = (int32_t*)(base + o_nr);
-  HIP_TRY(hipMemsetAsync(ev, 0, 5 * sizeof(unsigned long long), st));
-  if (pairs > 0) {
-    hipLaunchKernelGGL(k_transp_wing, dim3(blocks_for(pairs, kBlock)), dim3(kBlock), 0, st,
-                       to_dev(lines), to_dev(depths), n_wl, kapmin, cutoff, 0.0, resolu, ctx->tb, 2,
-                       nullptr, nullptr, transp, valid, wp, cls);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_class_count, dim3((unsigned)nblk), dim3(kBlock), 0, st, L, D, wing_order, cls,
-                       cnt);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_class_scan, dim3((unsigned)(D * kClasses)), dim3(kBlock), 0, st, nblk, D, cnt,
-                       tot);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_regions, dim3(blocks_for(D, 64)), dim3(64), 0, st, D, L, tot, reg);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_class_scatter, dim3((unsigned)nblk), dim3(kBlock), 0, st, L, D, wing_order, cls,
-                       cnt, reg, wp, lcm, lp);
-    HIP_TRY(hipGetLastError());
-  } else {
-    HIP_TRY(hipMemsetAsync(reg, 0, (size_t)D * (kClasses + 1) * sizeof(int64_t), st));
-  }
-  hipLaunchKernelGGL(k_zero_i32, dim3(blocks_for(D, 64)), dim3(64), 0, st, D, near_reach);
-  HIP_TRY(hipGetLastError());
-  if (pairs > 0) {
-    hipLaunchKernelGGL(k_near_reach, dim3(64, (unsigned)D), dim3(kBlock), 0, st, D, reg, lcm, near_reach);
-    HIP_TRY(hipGetLastError());
-  }
-  const dim3 g_near((unsigned)blocks_for(n_wl, 256), (unsigned)D);
-  if (stats)
-    hipLaunchKernelGGL(k_near<true>, g_near, dim3(256), 0, st, n_wl, D, lcm, lp, reg, near_reach, ctx->tb,
-                       asynth, ev + 3);
-  else
-    hipLaunchKernelGGL(k_near<false>, g_near, dim3(256), 0, st, n_wl, D, lcm, lp, reg, near_reach, ctx->tb,
-                       asynth, nullptr);
-  HIP_TRY(hipGetLastError());
-  {
-    int rc = timing_mark(ctx, st, true);
-    if (rc) return rc;
-  }
+  const AsynthBufs bufs = asynth_bufs((char*)ctx->ws, lay);
+  rc = asynth_prepare(ctx, lines, depths, n_wl, wl, transp, valid, kapmin, cutoff, wing_order, asynth,
+                      stats != nullptr, bufs, st);
+  if (rc) return rc;
+  rc = timing_mark(ctx, st, true);
+  if (rc) return rc;
   const dim3 g_gather((unsigned)blocks_for(n_wl, kWT), (unsigned)D);
   auto gather = stats ? (ctx->precision == SYNTHE_PREC_FP32 ? k_gather<true, kPrecFP32> : k_gather<true, kPrecFP64>)
                       : (ctx->precision == SYNTHE_PREC_FP32 ? k_gather<false, kPrecFP32> : k_gather<false, kPrecFP64>);
-  hipLaunchKernelGGL(gather, g_gather, dim3(64), 0, st, n_wl, D, center_order, center_start, lines->center_t,
-                     transp, valid, lcm, lp, reg, wl, depths->temperature, asynth, stats ? ev + 3 : nullptr);
+  const GatherArgs ga = gather_args(lines, depths, n_wl, wl, transp, valid, center_order, center_start, asynth,
+                                    bufs);
+  hipLaunchKernelGGL(gather, g_gather, dim3(64), 0, st, ga, stats ? bufs.ev + 3 : nullptr);
   HIP_TRY(hipGetLastError());
-  {
-    int rc = timing_mark(ctx, st, false);
-    if (rc) return rc;
-  }
+  rc = timing_mark(ctx, st, false);
+  if (rc) return rc;
   if (stats) {
-    if (pairs > 0) {
-      hipLaunchKernelGGL(k_wing_stats, dim3(64, (unsigned)D), dim3(kBlock), 0, st, pairs, D, L, reg, lcm,
-                         ev);
+    if (L * (long)D > 0) {
+      hipLaunchKernelGGL(k_wing_stats, dim3(64, (unsigned)D), dim3(kBlock), 0, st, L * (long)D, D, L, bufs.reg,
+                         bufs.lcm, bufs.ev);
       HIP_TRY(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_stats_to_double, dim3(1), dim3(64), 0, st, ev, stats);
+    hipLaunchKernelGGL(k_stats_to_double, dim3(1), dim3(64), 0, st, bufs.ev, stats);
     HIP_TRY(hipGetLastError());
   }
   return SYNTHE_OK;
@@ -1608,31 +1774,12 @@ int synthe_lte_spectrum(synthe_ctx* ctx, long n_depths, long n_wl, const double*
   if (!wl || !temperature || !rhox || !cont_abs || !cont_scat || !line_abs || !line_scat ||
       !flux_total || !flux_cont)
     return fail(SYNTHE_EINVAL, "synthe_lte_spectrum: NULL array");
-  hipStream_t st = (hipStream_t)stream;
   if (n_wl > 0x7fffffffL) return fail(SYNTHE_ERANGE, "synthe_lte_spectrum: more than 2^31 wavelengths");
-  size_t o_flags = align_up((size_t)2 * 3 * kNxtau * n_wl * sizeof(float));
-  size_t o_list = o_flags + align_up((size_t)n_wl);
-  size_t o_cnt = o_list + align_up((size_t)n_wl * sizeof(int));
-  int rc = ensure_ws(ctx, o_cnt + 256);
+  const RtLayout y = rt_layout(n_wl);
+  int rc = ensure_ws(ctx, y.total);
   if (rc) return rc;
-  float* ws = (float*)ctx->ws;
-  uint8_t* flags = (uint8_t*)ctx->ws + o_flags;
-  int* list = (int*)((char*)ctx->ws + o_list);
-  int* list_n = (int*)((char*)ctx->ws + o_cnt);
-  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, list_n);
-  HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(k_rt_prep, dim3(blocks_for(n_wl, kRtBlock)), dim3(kRtBlock), 0, st, (int)n_depths,
-                     n_wl, wl, temperature, rhox, cont_abs, cont_scat, line_abs, line_scat, line_source,
-                     ctx->tb.xtau, ws, flags, list, list_n);
-  HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(k_rt_iter, dim3(blocks_for(n_wl, kRtLanes)), dim3(kRtLanes), 0, st, n_wl, flags, ws,
-                     flux_total, flux_cont, iters, ctx->tb);
-  HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(k_lte, dim3(blocks_for(n_wl, kRtLanes)), dim3(kRtLanes), 0, st, (int)n_depths, n_wl,
-                     list, list_n, wl, temperature, rhox, cont_abs, cont_scat, line_abs, line_scat,
-                     line_source, flux_total, flux_cont, iters, ctx->tb);
-  HIP_TRY(hipGetLastError());
-  return SYNTHE_OK;
+  return rt_launch(ctx, n_depths, n_wl, wl, temperature, rhox, cont_abs, cont_scat, line_abs, line_scat,
+                   line_source, flux_total, flux_cont, iters, (char*)ctx->ws, y, (hipStream_t)stream);
 }
 
 int synthe_josh_flux_batch(synthe_ctx* ctx, long n, long n_cols, const double* acont,
@@ -1670,6 +1817,361 @@ int synthe_flux_to_lambda(synthe_ctx* ctx, long n_wl, const double* wl, double* 
   hipLaunchKernelGGL(k_flux_to_lambda, dim3(blocks_for(n_wl, kBlock)), dim3(kBlock), 0, (hipStream_t)stream,
                      n_wl, wl, flux);
   HIP_TRY(hipGetLastError());
+  return SYNTHE_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ batches
+namespace {
+
+struct BatchInst {
+  synthe_instance in;  // the caller's pointers and sizes
+  size_t o_cabs, o_cscat, o_kapmin, o_slinec, o_asynth, o_transp, o_valid, o_asy, o_rt;
+  AsynthLayout alay;
+  RtLayout rlay;
+  double cost;  // ordering key for the persistent gather (larger first)
+};
+
+constexpr int kBatchStreams = 4;
+constexpr int kTilePixels[3] = {4, 8, 16};  // pixels per lane of the gather candidates
+constexpr int kWavesPerSimd[3] = {2, 4, 8};
+
+}  // namespace
+
+struct synthe_batch {
+  synthe_ctx* ctx = nullptr;
+  std::vector<BatchInst> inst;
+  char* arena = nullptr;
+  size_t arena_size = 0;
+  double* zeros = nullptr;  // [max D*W] line scattering (rhoxj = 0)
+  GatherArgs* d_args = nullptr;
+  int32_t* d_order = nullptr;
+  int64_t* d_items = nullptr;
+  unsigned long long* d_counter = nullptr;
+  std::vector<int32_t> order;
+  int kwp = 8;
+  int gather_blocks = 4096;
+  int n_cu = 256;
+  int precision = SYNTHE_PREC_FP64;
+  std::vector<double> tune_ms;  // [3 tiles][3 wave counts], ms, or empty
+  hipStream_t streams[kBatchStreams] = {};
+  hipEvent_t ev_fork = nullptr, ev_mid = nullptr;
+  hipEvent_t ev_join[kBatchStreams] = {};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+// Items of the persistent gather for tiles of 64 * kwp pixels, instances in
+// b.order; uploads the prefix table.
+int batch_items(synthe_batch* b, int kwp) {
+  const long tile = 64L * kwp;
+  std::vector<int64_t> items(b->inst.size() + 1, 0);
+  for (size_t i = 0; i < b->order.size(); ++i) {
+    const synthe_instance& in = b->inst[b->order[i]].in;
+    long nt = (in.n_wl + tile - 1) / tile;
+    items[i + 1] = items[i] + nt * in.depths.n_depths;
+  }
+  HIP_TRY(hipMemcpy(b->d_items, items.data(), items.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+  return SYNTHE_OK;
+}
+
+int launch_gather_batch(synthe_batch* b, int kwp, int blocks, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, 1L, b->d_counter);
+  HIP_TRY(hipGetLastError());
+  const int n = (int)b->inst.size();
+  const bool f32 = b->precision == SYNTHE_PREC_FP32;
+#define SYNTHE_GB(P, K)                                                                                     \
+  hipLaunchKernelGGL((k_gather_batch<P, K>), dim3(blocks), dim3(64), 0, st, b->d_args, b->d_order, b->d_items, \
+                     n, b->d_counter)
+  if (kwp == 4) { if (f32) SYNTHE_GB(kPrecFP32, 4); else SYNTHE_GB(kPrecFP64, 4); }
+  else if (kwp == 16) { if (f32) SYNTHE_GB(kPrecFP32, 16); else SYNTHE_GB(kPrecFP64, 16); }
+  else { if (f32) SYNTHE_GB(kPrecFP32, 8); else SYNTHE_GB(kPrecFP64, 8); }
+#undef SYNTHE_GB
+  HIP_TRY(hipGetLastError());
+  return SYNTHE_OK;
+}
+
+// Per-instance stages before the gather: continuum, KAPMIN, source function,
+// TRANSP, wing set-up, class lists, Voigt zones.
+int batch_prepare_one(synthe_batch* b, BatchInst& bi, hipStream_t st) {
+  synthe_ctx* ctx = b->ctx;
+  const synthe_instance& in = bi.in;
+  const long D = in.depths.n_depths, W = in.n_wl, L = in.lines.n_lines;
+  char* base = b->arena;
+  double* cabs = (double*)(base + bi.o_cabs);
+  double* cscat = (double*)(base + bi.o_cscat);
+  double* kapmin = (double*)(base + bi.o_kapmin);
+  double* slinec = (double*)(base + bi.o_slinec);
+  double* asy = (double*)(base + bi.o_asynth);
+  double* transp = (double*)(base + bi.o_transp);
+  uint8_t* valid = (uint8_t*)(base + bi.o_valid);
+  int rc = synthe_continuum(ctx, D, W, in.wl, in.n_edges, in.wledge, in.half_edge, in.delta_edge, in.coeff_abs,
+                            in.coeff_scat, in.coeff_log10, cabs, cscat, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_add2, dim3(blocks_for(D * W, kBlock)), dim3(kBlock), 0, st, D * W, cabs, cscat, kapmin);
+  HIP_TRY(hipGetLastError());
+  rc = synthe_source_function(ctx, D, W, in.wl, in.depths.temperature, nullptr, nullptr, nullptr, slinec, st);
+  if (rc) return rc;
+  if (L > 0) {
+    rc = synthe_transp(ctx, &in.lines, &in.depths, W, kapmin, in.cutoff, 0.0, transp, valid, st);
+    if (rc) return rc;
+  }
+  const AsynthBufs bufs = asynth_bufs(base + bi.o_asy, bi.alay);
+  return asynth_prepare(ctx, &in.lines, &in.depths, W, in.wl, transp, valid, kapmin, in.cutoff, in.wing_order, asy,
+                        false, bufs, st);
+}
+
+int batch_finish_one(synthe_batch* b, BatchInst& bi, hipStream_t st) {
+  const synthe_instance& in = bi.in;
+  char* base = b->arena;
+  int rc = rt_launch(b->ctx, in.depths.n_depths, in.n_wl, in.wl, in.depths.temperature, in.depths.rhox,
+                     (double*)(base + bi.o_cabs), (double*)(base + bi.o_cscat), (double*)(base + bi.o_asynth),
+                     b->zeros, (double*)(base + bi.o_slinec), in.flux, in.cont, nullptr, base + bi.o_rt, bi.rlay, st);
+  if (rc) return rc;
+  rc = synthe_flux_to_lambda(b->ctx, in.n_wl, in.wl, in.flux, st);
+  if (rc) return rc;
+  return synthe_flux_to_lambda(b->ctx, in.n_wl, in.wl, in.cont, st);
+}
+
+int fork_streams(synthe_batch* b, hipEvent_t ev, hipStream_t s0) {
+  HIP_TRY(hipEventRecord(ev, s0));
+  for (int k = 0; k < kBatchStreams; ++k) HIP_TRY(hipStreamWaitEvent(b->streams[k], ev, 0));
+  return SYNTHE_OK;
+}
+
+int join_streams(synthe_batch* b, hipStream_t s0) {
+  for (int k = 0; k < kBatchStreams; ++k) {
+    HIP_TRY(hipEventRecord(b->ev_join[k], b->streams[k]));
+    HIP_TRY(hipStreamWaitEvent(s0, b->ev_join[k], 0));
+  }
+  return SYNTHE_OK;
+}
+
+// The whole batch: prepare stages fanned out over the batch's streams, one
+// persistent gather, RT fanned out again.  Launches and event records only.
+int batch_enqueue(synthe_batch* b, hipStream_t s0, bool prepare, bool gather, bool finish) {
+  const int n = (int)b->inst.size();
+  int rc;
+  if (prepare) {
+    if ((rc = fork_streams(b, b->ev_fork, s0))) return rc;
+    for (int i = 0; i < n; ++i)
+      if ((rc = batch_prepare_one(b, b->inst[b->order[i]], b->streams[i % kBatchStreams]))) return rc;
+    if ((rc = join_streams(b, s0))) return rc;
+  }
+  if (gather && (rc = launch_gather_batch(b, b->kwp, b->gather_blocks, s0))) return rc;
+  if (finish) {
+    if ((rc = fork_streams(b, b->ev_mid, s0))) return rc;
+    for (int i = 0; i < n; ++i)
+      if ((rc = batch_finish_one(b, b->inst[b->order[i]], b->streams[i % kBatchStreams]))) return rc;
+    if ((rc = join_streams(b, s0))) return rc;
+  }
+  return SYNTHE_OK;
+}
+
+void batch_free(synthe_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (int k = 0; k < kBatchStreams; ++k) {
+    if (b->streams[k]) (void)hipStreamDestroy(b->streams[k]);
+    if (b->ev_join[k]) (void)hipEventDestroy(b->ev_join[k]);
+  }
+  if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+  if (b->ev_mid) (void)hipEventDestroy(b->ev_mid);
+  if (b->arena) (void)hipFree(b->arena);
+  delete b;
+}
+
+}  // namespace
+
+extern "C" {
+
+int synthe_batch_create(synthe_ctx* ctx, int n, const synthe_instance* inst, int flags, synthe_batch** out) {
+  if (!ctx || !out || n <= 0 || !inst) return fail(SYNTHE_EINVAL, "synthe_batch_create: bad arguments");
+  *out = nullptr;
+  synthe_batch* b = new synthe_batch();
+  b->ctx = ctx;
+  b->precision = ctx->precision;
+  size_t off = 0, zeros_n = 1;
+  for (int i = 0; i < n; ++i) {
+    const synthe_instance& in = inst[i];
+    const char* who = "synthe_batch_create";
+    int rc = check_asynth_args(&in.lines, &in.depths, in.n_wl, in.wl, (const double*)1, (const uint8_t*)1,
+                               (const double*)1, in.center_order, in.center_start, in.wing_order, in.flux, who);
+    if (!rc && (in.depths.n_depths > kMaxDepth || in.n_wl > 0x7fffffffL))
+      rc = fail(SYNTHE_ERANGE, "synthe_batch_create: instance exceeds SYNTHE_MAX_DEPTH or 2^31 wavelengths");
+    if (!rc && (!in.cont || !in.wledge || !in.coeff_abs || !in.coeff_scat || in.n_edges < 2 ||
+                !in.depths.rhox || !in.depths.temperature))
+      rc = fail(SYNTHE_EINVAL, "synthe_batch_create: instance " + std::to_string(i) + " lacks an input or output");
+    if (rc) {
+      batch_free(b);
+      return rc;
+    }
+    BatchInst bi{};
+    bi.in = in;
+    const long D = in.depths.n_depths, W = in.n_wl, L = in.lines.n_lines;
+    const size_t dw = (size_t)D * W * sizeof(double);
+    bi.o_cabs = off; off = align_up(off + dw);
+    bi.o_cscat = off; off = align_up(off + dw);
+    bi.o_kapmin = off; off = align_up(off + dw);
+    bi.o_slinec = off; off = align_up(off + dw);
+    bi.o_asynth = off; off = align_up(off + dw);
+    bi.o_transp = off; off = align_up(off + (size_t)L * D * sizeof(double));
+    bi.o_valid = off; off = align_up(off + (size_t)L * D);
+    bi.alay = asynth_layout(L, (int)D);
+    bi.o_asy = off; off = align_up(off + bi.alay.total);
+    bi.rlay = rt_layout(W);
+    bi.o_rt = off; off = align_up(off + bi.rlay.total);
+    bi.cost = (double)L * (double)D + (double)W * (double)D;
+    zeros_n = std::max(zeros_n, (size_t)D * W);
+    b->inst.push_back(bi);
+  }
+  const size_t o_zero = off; off = align_up(off + zeros_n * sizeof(double));
+  const size_t o_args = off; off = align_up(off + n * sizeof(GatherArgs));
+  const size_t o_order = off; off = align_up(off + n * sizeof(int32_t));
+  const size_t o_items = off; off = align_up(off + (n + 1) * sizeof(int64_t));
+  const size_t o_counter = off; off = align_up(off + sizeof(unsigned long long));
+  if (hipMalloc(&b->arena, off) != hipSuccess) {
+    b->arena = nullptr;
+    batch_free(b);
+    return fail(SYNTHE_ENOMEM, "synthe_batch_create: arena of " + std::to_string(off) + " bytes failed");
+  }
+  b->arena_size = off;
+  b->zeros = (double*)(b->arena + o_zero);
+  b->d_args = (GatherArgs*)(b->arena + o_args);
+  b->d_order = (int32_t*)(b->arena + o_order);
+  b->d_items = (int64_t*)(b->arena + o_items);
+  b->d_counter = (unsigned long long*)(b->arena + o_counter);
+  int rc = SYNTHE_OK;
+  auto bail = [&](int code) {
+    batch_free(b);
+    return code;
+  };
+  // instance order: largest estimated work first
+  b->order.resize(n);
+  for (int i = 0; i < n; ++i) b->order[i] = i;
+  std::stable_sort(b->order.begin(), b->order.end(),
+                   [&](int x, int y) { return b->inst[x].cost > b->inst[y].cost; });
+  std::vector<GatherArgs> args(n);
+  for (int i = 0; i < n; ++i) {
+    BatchInst& bi = b->inst[i];
+    const synthe_instance& in = bi.in;
+    args[i] = gather_args(&in.lines, &in.depths, in.n_wl, in.wl, (double*)(b->arena + bi.o_transp),
+                          (uint8_t*)(b->arena + bi.o_valid), in.center_order, in.center_start,
+                          (double*)(b->arena + bi.o_asynth), asynth_bufs(b->arena + bi.o_asy, bi.alay));
+  }
+  if (hipMemcpy(b->d_args, args.data(), n * sizeof(GatherArgs), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->d_order, b->order.data(), n * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(b->zeros, 0, zeros_n * sizeof(double)) != hipSuccess)
+    return bail(fail(SYNTHE_EHIP, "synthe_batch_create: upload failed"));
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+    return bail(fail(SYNTHE_EHIP, "synthe_batch_create: device query failed"));
+  b->n_cu = prop.multiProcessorCount;
+  b->gather_blocks = b->n_cu * 4 * 4;
+  for (int k = 0; k < kBatchStreams; ++k) {
+    if (hipStreamCreateWithFlags(&b->streams[k], hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->ev_join[k], hipEventDisableTiming) != hipSuccess)
+      return bail(fail(SYNTHE_EHIP, "synthe_batch_create: stream creation failed"));
+  }
+  if (hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_mid, hipEventDisableTiming) != hipSuccess)
+    return bail(fail(SYNTHE_EHIP, "synthe_batch_create: event creation failed"));
+  hipStream_t s0 = b->streams[0];
+  if (flags & SYNTHE_BATCH_AUTOTUNE) {
+    // Time every (tile, persistent grid) candidate of the gather on this
+    // batch, after the stages it depends on, and keep the fastest.
+    if ((rc = batch_enqueue(b, s0, true, false, false))) return bail(rc);
+    hipEvent_t t0, t1;
+    if (hipEventCreate(&t0) != hipSuccess || hipEventCreate(&t1) != hipSuccess)
+      return bail(fail(SYNTHE_EHIP, "synthe_batch_create: event creation failed"));
+    double best = 1e300;
+    for (int a = 0; a < 3 && !rc; ++a) {
+      if ((rc = batch_items(b, kTilePixels[a]))) break;
+      for (int c = 0; c < 3 && !rc; ++c) {
+        const int blocks = b->n_cu * 4 * kWavesPerSimd[c];
+        float ms = 0.f, best_c = 1e30f;
+        for (int rep = 0; rep < 2 && !rc; ++rep) {
+          if (hipEventRecord(t0, s0) != hipSuccess) rc = fail(SYNTHE_EHIP, "event record failed");
+          if (!rc) rc = launch_gather_batch(b, kTilePixels[a], blocks, s0);
+          if (!rc && (hipEventRecord(t1, s0) != hipSuccess || hipEventSynchronize(t1) != hipSuccess ||
+                      hipEventElapsedTime(&ms, t0, t1) != hipSuccess))
+            rc = fail(SYNTHE_EHIP, "synthe_batch_create: timing failed");
+          best_c = std::min(best_c, ms);
+        }
+        b->tune_ms.push_back(best_c);
+        if (!rc && best_c < best) {
+          best = best_c;
+          b->kwp = kTilePixels[a];
+          b->gather_blocks = blocks;
+        }
+      }
+    }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (rc) return bail(rc);
+  }
+  if ((rc = batch_items(b, b->kwp))) return bail(rc);
+  if (flags & SYNTHE_BATCH_GRAPH) {
+    if (hipStreamSynchronize(s0) != hipSuccess)
+      return bail(fail(SYNTHE_EHIP, "synthe_batch_create: synchronisation failed"));
+    hipStream_t cap;
+    if (hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) != hipSuccess)
+      return bail(fail(SYNTHE_EHIP, "synthe_batch_create: stream creation failed"));
+    if (hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      (void)hipStreamDestroy(cap);
+      return bail(fail(SYNTHE_EHIP, "synthe_batch_create: graph capture failed to start"));
+    }
+    rc = batch_enqueue(b, cap, true, true, true);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(cap, &g);
+    (void)hipStreamDestroy(cap);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return bail(rc);
+    }
+    if (e != hipSuccess || !g) return bail(fail(SYNTHE_EHIP, "synthe_batch_create: graph capture failed"));
+    b->graph = g;
+    if (hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+      b->exec = nullptr;
+      return bail(fail(SYNTHE_EHIP, "synthe_batch_create: graph instantiation failed"));
+    }
+  }
+  if (hipStreamSynchronize(s0) != hipSuccess)
+    return bail(fail(SYNTHE_EHIP, "synthe_batch_create: synchronisation failed"));
+  *out = b;
+  return SYNTHE_OK;
+}
+
+int synthe_batch_run(synthe_batch* b, synthe_stream stream) {
+  if (!b) return fail(SYNTHE_EINVAL, "synthe_batch_run: batch is NULL");
+  hipStream_t st = (hipStream_t)stream;
+  if (b->exec) {
+    HIP_TRY(hipGraphLaunch(b->exec, st));
+    return SYNTHE_OK;
+  }
+  return batch_enqueue(b, st, true, true, true);
+}
+
+int synthe_batch_info(const synthe_batch* b, int* pixels_per_lane, int* gather_blocks, double* tune_ms,
+                      int tune_cap, int* n_tuned, double* arena_bytes) {
+  if (!b) return fail(SYNTHE_EINVAL, "synthe_batch_info: batch is NULL");
+  if (pixels_per_lane) *pixels_per_lane = b->kwp;
+  if (gather_blocks) *gather_blocks = b->gather_blocks;
+  if (n_tuned) *n_tuned = (int)b->tune_ms.size();
+  for (int i = 0; tune_ms && i < tune_cap && i < (int)b->tune_ms.size(); ++i) tune_ms[i] = b->tune_ms[i];
+  if (arena_bytes) *arena_bytes = (double)b->arena_size;
+  return SYNTHE_OK;
+}
+
+int synthe_batch_destroy(synthe_batch* b) {
+  if (b) {
+    (void)hipDeviceSynchronize();
+    batch_free(b);
+  }
   return SYNTHE_OK;
 }
 

@@ -9,7 +9,7 @@ current torch stream.  PyTorch is used only to own device memory and the stream.
 from __future__ import annotations
 
 from dataclasses import dataclass
-from typing import Dict, Optional
+from typing import Dict, List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -252,3 +252,92 @@ def ct_byref(s):
     import ctypes
 
     return ctypes.byref(s)
+
+
+class BatchPlan:
+    """Many spectra -- atmospheres of different depth counts, different
+    wavelength windows -- planned once and run as one unit on the GPU.
+
+    ``instances`` holds (atmosphere, catalog, wl_start, wl_end) tuples; each
+    catalog is DELLIM-filtered to its window here.  Every instance runs the
+    same forward pass as SpectrumPlan (LTE, no line scattering) with the same
+    kernels, so results equal per-instance SpectrumPlan results bit for bit;
+    the wing gather of all instances is one persistent launch.  ``autotune``
+    times the gather's tile and grid candidates on this batch and keeps the
+    fastest; ``graph`` captures the whole batch into a hipGraph that run()
+    replays.  Inputs stay resident in HBM for the plan's life.
+    """
+
+    def __init__(self, instances: Sequence[Tuple[AtmosphereModel, LineCatalog, float, float]],
+                 resolution: float = 300000.0, cutoff: float = 1e-3, precision: str = "fp64",
+                 autotune: bool = True, graph: bool = True):
+        import ctypes
+
+        from .lines import filter_by_range
+
+        torch = _torch()
+        self.ctx = N.context()
+        if precision not in ("fp64", "fp32"):
+            raise ValueError(f"precision must be 'fp64' or 'fp32', not {precision!r}")
+        self.precision = precision
+        self.items: List[dict] = []
+        arr = (N.Instance * len(instances))()
+        for i, (atm, catalog, wl0, wl1) in enumerate(instances):
+            grid = N.wavelength_grid(wl0, wl1, resolution)
+            depth = upload_atmosphere(atm)
+            cat = filter_by_range(catalog, wl0, wl1)
+            soa = build_line_soa(cat, grid, depth.n_elem, grid_origin=float(grid[0]) if grid.size else None)
+            lines = upload_lines(soa, grid.size)
+            d_grid = to_device(grid, np.float64)
+            flux = torch.empty(grid.size, dtype=torch.float64, device="cuda")
+            cont = torch.empty(grid.size, dtype=torch.float64, device="cuda")
+            p = N.ptr
+            arr[i] = N.Instance(lines.struct(), depth.struct(), int(grid.size), p(d_grid),
+                                int(depth.wledge.shape[0]), p(depth.wledge), p(depth.half_edge),
+                                p(depth.delta_edge), p(depth.coeff_abs), p(depth.coeff_scat),
+                                int(depth.coeff_log10), p(lines.corder), p(lines.cstart), p(lines.worder),
+                                float(cutoff), p(flux), p(cont))
+            self.items.append(dict(grid=grid, depth=depth, soa=soa, lines=lines, d_grid=d_grid, flux=flux,
+                                   cont=cont, wl_start=wl0, wl_end=wl1))
+        self._arr = arr
+        flags = (N.SYNTHE_BATCH_AUTOTUNE if autotune else 0) | (N.SYNTHE_BATCH_GRAPH if graph else 0)
+        self.ctx.set_profile_precision(precision)
+        torch.cuda.synchronize()
+        h = ctypes.c_void_p()
+        N.check("synthe_batch_create", self.ctx.lib.synthe_batch_create(
+            self.ctx.handle, len(instances), arr, flags, ctypes.byref(h)))
+        self.handle = h
+
+    def __len__(self) -> int:
+        return len(self.items)
+
+    def run(self, stream=None) -> None:
+        s = stream if stream is not None else N.stream_handle()
+        N.check("synthe_batch_run", self.ctx.lib.synthe_batch_run(self.handle, s))
+
+    def info(self) -> dict:
+        import ctypes
+
+        kwp, blocks, nt = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        arena = ctypes.c_double()
+        tune = np.zeros(16)
+        N.check("synthe_batch_info", self.ctx.lib.synthe_batch_info(
+            self.handle, ctypes.byref(kwp), ctypes.byref(blocks), tune.ctypes.data, 16, ctypes.byref(nt),
+            ctypes.byref(arena)))
+        return dict(pixels_per_lane=kwp.value, tile=64 * kwp.value, gather_workgroups=blocks.value,
+                    autotune_ms=tune[:nt.value].tolist(), arena_bytes=arena.value)
+
+    def results(self) -> List[dict]:
+        return [dict(wavelength=it["grid"].copy(), flux=it["flux"].cpu().numpy(),
+                     continuum=it["cont"].cpu().numpy()) for it in self.items]
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self.ctx.lib.synthe_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
