@@ -133,6 +133,77 @@ def fp32_form(plan, args, e_line):
                     "FP64 populations, continuum, Voigt-zone values and RT"}
 
 
+def c3_instances(wk, n: int = 256, seed: int = 3):
+    """256 atmospheres: Teff ~ U(3500, 40000) K, log g ~ U(0, 5), layers ~ U{64..80},
+    [M/H] ~ U(-2, 0.5); windows start ~ U(300, 1700) nm, width ~ U(5, 50) nm;
+    all drawing on the shared 1,000,000-line list of the full-window workload."""
+    from kurucz_py_joss_submission_amd import synthetic as S
+
+    rng = np.random.default_rng(seed)
+    out, meta = [], []
+    for _ in range(n):
+        teff = float(rng.uniform(3500.0, 40000.0))
+        logg = float(rng.uniform(0.0, 5.0))
+        nl = int(rng.integers(64, 81))
+        mh = float(rng.uniform(-2.0, 0.5))
+        w0 = float(rng.uniform(300.0, 1700.0))
+        w1 = w0 + float(rng.uniform(5.0, 50.0))
+        out.append((S.build_bundle(S.make_atm(teff, logg, nl, mh)), wk.catalog, w0, w1))
+        meta.append((teff, logg, nl, mh, w0, w1))
+    return out, meta
+
+
+def bench_batched(args, wk) -> dict:
+    """The batched configuration: per-instance evaluation counts and the
+    one-spectrum-at-a-time time from SpectrumPlan, then the batch (autotuned
+    persistent gather, hipGraph replay) timed over the steps."""
+    import torch
+
+    from kurucz_py_joss_submission_amd import lines as L
+    from kurucz_py_joss_submission_amd.device import BatchPlan, SpectrumPlan
+
+    t0 = time.perf_counter()
+    inst, meta = c3_instances(wk)
+    log(f"[bench] c3: {len(inst)} atmospheres in {time.perf_counter() - t0:.1f} s")
+    e_total, seq_ms, n_wl, n_lines = 0.0, 0.0, 0, 0
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for atm, cat, w0, w1 in inst:
+        plan = SpectrumPlan(atm, L.filter_by_range(cat, w0, w1), w0, w1, 300000.0)
+        plan.run(stats=True)
+        ev0.record()
+        plan.run()
+        ev1.record()
+        torch.cuda.synchronize()
+        seq_ms += ev0.elapsed_time(ev1)
+        e_total += float(plan.stats[0].item())
+        n_wl += plan.W
+        n_lines += len(plan.soa)
+        del plan
+    log(f"[bench] c3: one at a time {seq_ms:.1f} ms, E_line {e_total:.4g}, sum W {n_wl}, sum L {n_lines}")
+    t0 = time.perf_counter()
+    bp = BatchPlan(inst, autotune=True, graph=True)
+    info = bp.info()
+    log(f"[bench] c3: batch planned in {time.perf_counter() - t0:.1f} s: {info}")
+    for _ in range(max(args.warmup, 1)):
+        bp.run()
+    torch.cuda.synchronize()
+    ev0.record()
+    for _ in range(args.steps):
+        bp.run()
+    ev1.record()
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / args.steps
+    bp.close()
+    log(f"[bench] c3: batch {ms:.2f} ms per run of {len(inst)} spectra")
+    return {"value": e_total / (ms * 1e-3), "unit": "evals/s", "ms_per_batch": ms,
+            "spectra": len(inst), "sum_wavelengths": n_wl, "sum_lines": n_lines, "e_line": e_total,
+            "one_at_a_time_ms": seq_ms, "speedup_vs_one_at_a_time": seq_ms / ms,
+            "gather_tile": info["tile"], "gather_workgroups": info["gather_workgroups"],
+            "autotune_ms": info["autotune_ms"], "arena_gb": info["arena_bytes"] / 1e9,
+            "workload": "256 atmospheres, Teff U(3500,40000) K, log g U(0,5), 64-80 layers, [M/H] U(-2,0.5), "
+                        "windows U(300,1700) nm + U(5,50) nm, shared 1,000,000-line list, R=300000"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,6 +211,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 256-atmosphere batched variant")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain device copy of 1 GiB (3 times), to scale HBM counters")
     args = ap.parse_args()
@@ -212,6 +284,11 @@ def main() -> int:
     ms = start.elapsed_time(end) / args.steps
     log(f"[bench] {args.steps} steps, {ms:.3f} ms per spectrum; k_gather {np.round(gather_ms, 3).tolist()} ms")
     variants = {"fp32_profiles": fp32_form(plan, args, e_line)}
+    shape = {"n_depths": plan.D, "n_wavelengths": plan.W, "n_lines": len(plan.soa)}
+    if args.config == "c2" and not args.no_batched:
+        del plan
+        torch.cuda.synchronize()
+        variants["batched_c3"] = bench_batched(args, wk)
 
     # dominant kernel: k_gather (centre adds + wing sums + STIM), one launch per
     # spectrum.  Algorithmic flops per evaluation: 3 for the far-wing value
@@ -252,8 +329,7 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded atmosphere bundle and GFALL line list)",
-        "config": {"workload": cfg["desc"], "n_depths": plan.D, "n_wavelengths": plan.W,
-                   "n_lines": len(plan.soa), "e_line": e_line, "spectrum_ms": ms,
+        "config": {"workload": cfg["desc"], **shape, "e_line": e_line, "spectrum_ms": ms,
                    "stage_ms": stages},
         "roofline": roof,
         "variants": variants,

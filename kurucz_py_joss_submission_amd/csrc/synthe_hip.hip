@@ -999,25 +999,56 @@ constexpr int kRtLanes = 64;
 // form with per-lane depth arrays: runs only for the wavelengths the streaming
 // kernels list (masked or non-increasing depths, fewer than 4 depths, or a
 // label-401 solve), packed densely into waves.  list == NULL runs every one.
-__global__ __launch_bounds__(kRtLanes) void k_lte(int D, long W, const int* __restrict__ list,
-                                                  const int* __restrict__ list_n,
-                                                  const double* __restrict__ wl,
-                                                  const double* __restrict__ temp,
-                                                  const double* __restrict__ rhox,
-                                                  const double* __restrict__ ca,
-                                                  const double* __restrict__ cs,
-                                                  const double* __restrict__ la,
-                                                  const double* __restrict__ ls,
-                                                  const double* __restrict__ src,
-                                                  double* __restrict__ ft, double* __restrict__ fc,
-                                                  int32_t* __restrict__ iters, DeviceTables tb) {
-  __shared__ float s_xs[kNxtau][kRtLanes];
-  long w = blockIdx.x * (long)kRtLanes + threadIdx.x;
-  if (list) {
-    if (w >= *list_n) return;
-    w = list[w];
+// RT inputs and outputs of one spectrum within a set of spectra that share
+// a global wavelength index g = g0 + w (w < W).  to_lambda: store F_lambda
+// (engine/opacity.py:3852-3861) instead of F_nu.
+struct RtArgs {
+  long g0, W;
+  int D, to_lambda;
+  const double *wl, *temp, *rhox, *ca, *cs, *la, *ls, *src;
+  double *ft, *fc;
+  int32_t* iters;
+};
+
+// Index of the spectrum holding global wavelength g (rt[] ascending in g0).
+__device__ __forceinline__ int rt_find(const RtArgs* __restrict__ rt, int n, long g) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (rt[mid].g0 <= g) lo = mid;
+    else hi = mid - 1;
   }
-  if (w >= W) return;
+  return lo;
+}
+
+__device__ __forceinline__ void rt_store(const RtArgs& a, long w, double ftot, double fcont, int it) {
+  if (a.to_lambda) {
+    double w2 = a.wl[w] * a.wl[w];
+    double f = kCLightNm / (w2 > 1e-40 ? w2 : 1e-40);
+    ftot = ftot * f;
+    fcont = fcont * f;
+  }
+  a.ft[w] = ftot;
+  a.fc[w] = fcont;
+  if (a.iters) a.iters[w] = it;
+}
+
+__global__ void k_store_rtargs(RtArgs a, RtArgs* dst) {
+  if (threadIdx.x == 0) *dst = a;
+}
+
+__global__ __launch_bounds__(kRtLanes) void k_lte(const RtArgs* __restrict__ rt, int n_rt,
+                                                  const int64_t* __restrict__ list,
+                                                  const int* __restrict__ list_n, DeviceTables tb) {
+  __shared__ float s_xs[kNxtau][kRtLanes];
+  const long e = blockIdx.x * (long)kRtLanes + threadIdx.x;
+  if (e >= *list_n) return;
+  const long g = list[e];
+  const RtArgs& a = rt[rt_find(rt, n_rt, g)];
+  const long w = g - a.g0, W = a.W;
+  const int D = a.D;
+  const double *wl = a.wl, *temp = a.temp, *rhox = a.rhox, *ca = a.ca, *cs = a.cs, *la = a.la, *ls = a.ls,
+               *src = a.src;
   int lane = threadIdx.x;
   const double maxr4 = 3.4e38;
   int idx[kMaxDepth];
@@ -1030,9 +1061,7 @@ __global__ __launch_bounds__(kRtLanes) void k_lte(int D, long W, const int* __re
     if (okd && r > 0) idx[n++] = d;
   }
   if (n == 0) {
-    ft[w] = 0.0;
-    fc[w] = 0.0;
-    if (iters) iters[w] = 0;
+    rt_store(a, w, 0.0, 0.0, 0);
     return;
   }
   if (n > 1 && !(rhox[idx[0]] < rhox[idx[n - 1]])) {
@@ -1065,9 +1094,7 @@ __global__ __launch_bounds__(kRtLanes) void k_lte(int D, long W, const int* __re
   SolveIn tin{ca + w, nullptr, la + w, nullptr, cs + w, ls + w, rhox, W, pl, sl, false};
   double ftot = josh_solve<kRtLanes>(tin, idx, n, tb, s_xs, lane, abtot, alpha, snubar, rho, taunu,
                                      pa, pb, pc, &it_t, &b401t);
-  ft[w] = ftot;
-  fc[w] = fcont;
-  if (iters) iters[w] = it_c + it_t + (b401t ? 1000 : 0);
+  rt_store(a, w, ftot, fcont, it_c + it_t + (b401t ? 1000 : 0));
 }
 
 // ----------------------------------------------------- streaming RT kernels
@@ -1130,19 +1157,17 @@ constexpr int kRtBlock = 256;
 
 // Fast-path set-up of both JOSH solves per wavelength: validity check, then
 // the streaming INTEG + MAP1.  flags[w]: 1 = needs the general kernel.
-__global__ __launch_bounds__(kRtBlock) void k_rt_prep(int D, long W, const double* __restrict__ wl,
-                                                      const double* __restrict__ temp,
-                                                      const double* __restrict__ rhox,
-                                                      const double* __restrict__ ca,
-                                                      const double* __restrict__ cs,
-                                                      const double* __restrict__ la,
-                                                      const double* __restrict__ ls,
-                                                      const double* __restrict__ src,
+__global__ __launch_bounds__(kRtBlock, 3) void k_rt_prep(const RtArgs* __restrict__ rt, int n_rt, long G,
                                                       const double* __restrict__ xtau, float* ws,
                                                       uint8_t* __restrict__ flags,
-                                                      int* __restrict__ list, int* __restrict__ list_n) {
-  long w = blockIdx.x * (long)kRtBlock + threadIdx.x;
-  if (w >= W) return;
+                                                      int64_t* __restrict__ list, int* __restrict__ list_n) {
+  const long g = blockIdx.x * (long)kRtBlock + threadIdx.x;
+  if (g >= G) return;
+  const RtArgs& a = rt[rt_find(rt, n_rt, g)];
+  const long w = g - a.g0, W = a.W;
+  const int D = a.D;
+  const double *wl = a.wl, *temp = a.temp, *rhox = a.rhox, *ca = a.ca, *cs = a.cs, *la = a.la, *ls = a.ls,
+               *src = a.src;
   const double maxr4 = 3.4e38;
   bool ok = D >= 4 && rhox[0] < rhox[D - 1];
   for (int d = 0; d < D && ok; ++d) {
@@ -1152,8 +1177,8 @@ __global__ __launch_bounds__(kRtBlock) void k_rt_prep(int D, long W, const doubl
          a1 < maxr4 && a2 < maxr4 && a3 < maxr4 && a4 < maxr4;
   }
   if (!ok) {
-    flags[w] = 1;
-    list[atomicAdd(list_n, 1)] = (int)w;  // order is irrelevant: one slot per wavelength
+    flags[g] = 1;
+    list[atomicAdd(list_n, 1)] = g;  // order is irrelevant: one slot per wavelength
     return;
   }
   double wv = wl[w];
@@ -1164,8 +1189,8 @@ __global__ __launch_bounds__(kRtBlock) void k_rt_prep(int D, long W, const doubl
     ColSrc in{ca + w, cs + w, la + w, ls + w, src ? src + w : nullptr, temp, rhox, W, freq, f3, s == 1};
     double f0, x0, sb0, al0;
     in.get(0, &f0, &x0, &sb0, &al0);
-    float* base = ws + (long)s * 3 * kNxtau * W + w;
-    ColOut out{base, base + (long)kNxtau * W, base + 2L * kNxtau * W, W, xtau, 0.0,
+    float* base = ws + (long)s * 3 * kNxtau * G + g;
+    ColOut out{base, base + (long)kNxtau * G, base + 2L * kNxtau * G, G, xtau, 0.0,
                sb0 > kEps38 ? sb0 : kEps38, al0 < 0.0 ? 0.0 : (al0 > 1.0 ? 1.0 : al0)};
     out.tau0 = f0 * x0;
     if (out.tau0 > xtau[kNxtau - 1]) {
@@ -1175,8 +1200,8 @@ __global__ __launch_bounds__(kRtBlock) void k_rt_prep(int D, long W, const doubl
     double tau0;
     rt::integ_map1_stream(D, xtau, in, out, &tau0);
   }
-  flags[w] = fl;
-  if (fl) list[atomicAdd(list_n, 1)] = (int)w;
+  flags[g] = fl;
+  if (fl) list[atomicAdd(list_n, 1)] = g;
 }
 
 __global__ void k_zero_int(int* p) {
@@ -1211,31 +1236,30 @@ __global__ void k_check_div(long n, int variant, const double* x, const double* 
 
 // Gauss-Seidel scattering iteration (josh_solver.py:24-92, Numba's float32
 // arrays with float64 accumulation) and flux = CH . XS, both solves.
-__global__ __launch_bounds__(kRtLanes) void k_rt_iter(long W, const uint8_t* __restrict__ flags,
-                                                      const float* __restrict__ ws,
-                                                      double* __restrict__ ft,
-                                                      double* __restrict__ fc,
-                                                      int32_t* __restrict__ iters, DeviceTables tb) {
+__global__ __launch_bounds__(kRtLanes) void k_rt_iter(const RtArgs* __restrict__ rt, int n_rt, long G,
+                                                      const uint8_t* __restrict__ flags,
+                                                      const float* __restrict__ ws, DeviceTables tb) {
   __shared__ float s_xs[kNxtau][kRtLanes];
   __shared__ float s_cj[kNxtau * kNxtau + kNxtau];  // COEFJ rows, then its diagonal
   for (int i = threadIdx.x; i < kNxtau * kNxtau; i += kRtLanes) s_cj[i] = tb.coefj32[i];
   if (threadIdx.x < kNxtau) s_cj[kNxtau * kNxtau + threadIdx.x] = tb.diag32[threadIdx.x];
   __syncthreads();
-  long w = blockIdx.x * (long)kRtLanes + threadIdx.x;
-  if (w >= W) return;
-  if (flags[w] != 0) return;
+  const long g = blockIdx.x * (long)kRtLanes + threadIdx.x;
+  if (g >= G) return;
+  if (flags[g] != 0) return;
   const int lane = threadIdx.x;
   const double tol = (double)(float)1.0e-5;
   const double eps = (double)(float)kEps38;
   int it_sum = 0;
+  double fsol[2];
   for (int s = 0; s < 2; ++s) {
-    const float* base = ws + (long)s * 3 * kNxtau * W + w;
+    const float* base = ws + (long)s * 3 * kNxtau * G + g;
     float xal[kNxtau], xsb[kNxtau];
 #pragma unroll
     for (int k = 0; k < kNxtau; ++k) {
-      s_xs[k][lane] = base[(long)k * W];
-      xal[k] = base[(long)(kNxtau + k) * W];
-      xsb[k] = base[(long)(2 * kNxtau + k) * W];
+      s_xs[k][lane] = base[(long)k * G];
+      xal[k] = base[(long)(kNxtau + k) * G];
+      xsb[k] = base[(long)(2 * kNxtau + k) * G];
     }
     int it = kNxtau;
     for (int sweep = 0; sweep < kNxtau; ++sweep) {
@@ -1266,10 +1290,11 @@ __global__ __launch_bounds__(kRtLanes) void k_rt_iter(long W, const uint8_t* __r
     }
     double flux = 0.0;
     for (int k = 0; k < kNxtau; ++k) flux += tb.ch[k] * (double)s_xs[k][lane];
-    (s == 0 ? fc : ft)[w] = flux;
+    fsol[s] = flux;
     it_sum += it;
   }
-  if (iters) iters[w] = it_sum;
+  const RtArgs& a = rt[rt_find(rt, n_rt, g)];
+  rt_store(a, g - a.g0, fsol[1], fsol[0], it_sum);
 }
 
 // solve_josh_flux over independent columns (physics/josh_solver.py:367-767).
@@ -1497,41 +1522,52 @@ static int asynth_prepare(const synthe_ctx* ctx, const synthe_lines* lines, cons
 
 // ---------------------------------------------------------------- RT pieces
 struct RtLayout {
-  size_t o_flags, o_list, o_cnt, total;
+  size_t o_flags, o_list, o_cnt, o_args, total;
 };
 
-static RtLayout rt_layout(long n_wl) {
+// Workspace of the RT over G wavelengths of n_rt spectra.
+static RtLayout rt_layout(long G, int n_rt = 1) {
   RtLayout y{};
-  y.o_flags = align_up((size_t)2 * 3 * kNxtau * n_wl * sizeof(float));
-  y.o_list = y.o_flags + align_up((size_t)n_wl);
-  y.o_cnt = y.o_list + align_up((size_t)n_wl * sizeof(int));
-  y.total = y.o_cnt + 256;
+  y.o_flags = align_up((size_t)2 * 3 * kNxtau * G * sizeof(float));
+  y.o_list = y.o_flags + align_up((size_t)G);
+  y.o_cnt = y.o_list + align_up((size_t)G * sizeof(int64_t));
+  y.o_args = y.o_cnt + 256;
+  y.total = y.o_args + align_up((size_t)n_rt * sizeof(RtArgs));
   return y;
 }
 
-// Both JOSH solves for every wavelength; kernel launches only (capturable).
+// Both JOSH solves for every wavelength of n_rt spectra (rt: device array in
+// the workspace, filled by the caller); kernel launches only (capturable).
+static int rt_launch_set(const synthe_ctx* ctx, const RtArgs* rt, int n_rt, long G, char* base, const RtLayout& y,
+                         hipStream_t st) {
+  float* ws = (float*)base;
+  uint8_t* flags = (uint8_t*)base + y.o_flags;
+  int64_t* list = (int64_t*)(base + y.o_list);
+  int* list_n = (int*)(base + y.o_cnt);
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, list_n);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_rt_prep, dim3(blocks_for(G, kRtBlock)), dim3(kRtBlock), 0, st, rt, n_rt, G, ctx->tb.xtau, ws,
+                     flags, list, list_n);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_rt_iter, dim3(blocks_for(G, kRtLanes)), dim3(kRtLanes), 0, st, rt, n_rt, G, flags, ws,
+                     ctx->tb);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_lte, dim3(blocks_for(G, kRtLanes)), dim3(kRtLanes), 0, st, rt, n_rt, list, list_n, ctx->tb);
+  HIP_TRY(hipGetLastError());
+  return SYNTHE_OK;
+}
+
+// One spectrum: its RtArgs are stored into the workspace by a kernel.
 static int rt_launch(const synthe_ctx* ctx, long n_depths, long n_wl, const double* wl, const double* temperature,
                      const double* rhox, const double* cont_abs, const double* cont_scat, const double* line_abs,
                      const double* line_scat, const double* line_source, double* flux_total, double* flux_cont,
                      int32_t* iters, char* base, const RtLayout& y, hipStream_t st) {
-  float* ws = (float*)base;
-  uint8_t* flags = (uint8_t*)base + y.o_flags;
-  int* list = (int*)(base + y.o_list);
-  int* list_n = (int*)(base + y.o_cnt);
-  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, list_n);
+  RtArgs a{0, n_wl, (int)n_depths, 0, wl, temperature, rhox, cont_abs, cont_scat, line_abs, line_scat,
+           line_source, flux_total, flux_cont, iters};
+  RtArgs* d_args = (RtArgs*)(base + y.o_args);
+  hipLaunchKernelGGL(k_store_rtargs, dim3(1), dim3(64), 0, st, a, d_args);
   HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(k_rt_prep, dim3(blocks_for(n_wl, kRtBlock)), dim3(kRtBlock), 0, st, (int)n_depths, n_wl, wl,
-                     temperature, rhox, cont_abs, cont_scat, line_abs, line_scat, line_source, ctx->tb.xtau, ws,
-                     flags, list, list_n);
-  HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(k_rt_iter, dim3(blocks_for(n_wl, kRtLanes)), dim3(kRtLanes), 0, st, n_wl, flags, ws,
-                     flux_total, flux_cont, iters, ctx->tb);
-  HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(k_lte, dim3(blocks_for(n_wl, kRtLanes)), dim3(kRtLanes), 0, st, (int)n_depths, n_wl, list,
-                     list_n, wl, temperature, rhox, cont_abs, cont_scat, line_abs, line_scat, line_source,
-                     flux_total, flux_cont, iters, ctx->tb);
-  HIP_TRY(hipGetLastError());
-  return SYNTHE_OK;
+  return rt_launch_set(ctx, d_args, 1, n_wl, base, y, st);
 }
 
 extern "C" {
@@ -1827,9 +1863,8 @@ namespace {
 
 struct BatchInst {
   synthe_instance in;  // the caller's pointers and sizes
-  size_t o_cabs, o_cscat, o_kapmin, o_slinec, o_asynth, o_transp, o_valid, o_asy, o_rt;
+  size_t o_cabs, o_cscat, o_kapmin, o_slinec, o_asynth, o_transp, o_valid, o_asy;
   AsynthLayout alay;
-  RtLayout rlay;
   double cost;  // ordering key for the persistent gather (larger first)
 };
 
@@ -1845,6 +1880,10 @@ struct synthe_batch {
   char* arena = nullptr;
   size_t arena_size = 0;
   double* zeros = nullptr;  // [max D*W] line scattering (rhoxj = 0)
+  char* rt_base = nullptr;  // RT workspace over all instances' wavelengths
+  RtLayout rlay{};
+  long rt_g = 0;            // total wavelengths
+  RtArgs* d_rt = nullptr;   // [n], ascending g0 (in b->order)
   GatherArgs* d_args = nullptr;
   int32_t* d_order = nullptr;
   int64_t* d_items = nullptr;
@@ -1924,18 +1963,6 @@ int batch_prepare_one(synthe_batch* b, BatchInst& bi, hipStream_t st) {
                         false, bufs, st);
 }
 
-int batch_finish_one(synthe_batch* b, BatchInst& bi, hipStream_t st) {
-  const synthe_instance& in = bi.in;
-  char* base = b->arena;
-  int rc = rt_launch(b->ctx, in.depths.n_depths, in.n_wl, in.wl, in.depths.temperature, in.depths.rhox,
-                     (double*)(base + bi.o_cabs), (double*)(base + bi.o_cscat), (double*)(base + bi.o_asynth),
-                     b->zeros, (double*)(base + bi.o_slinec), in.flux, in.cont, nullptr, base + bi.o_rt, bi.rlay, st);
-  if (rc) return rc;
-  rc = synthe_flux_to_lambda(b->ctx, in.n_wl, in.wl, in.flux, st);
-  if (rc) return rc;
-  return synthe_flux_to_lambda(b->ctx, in.n_wl, in.wl, in.cont, st);
-}
-
 int fork_streams(synthe_batch* b, hipEvent_t ev, hipStream_t s0) {
   HIP_TRY(hipEventRecord(ev, s0));
   for (int k = 0; k < kBatchStreams; ++k) HIP_TRY(hipStreamWaitEvent(b->streams[k], ev, 0));
@@ -1951,7 +1978,8 @@ int join_streams(synthe_batch* b, hipStream_t s0) {
 }
 
 // The whole batch: prepare stages fanned out over the batch's streams, one
-// persistent gather, RT fanned out again.  Launches and event records only.
+// persistent gather over all instances, one RT over all instances'
+// wavelengths (F_lambda stored directly).  Launches and event records only.
 int batch_enqueue(synthe_batch* b, hipStream_t s0, bool prepare, bool gather, bool finish) {
   const int n = (int)b->inst.size();
   int rc;
@@ -1962,12 +1990,7 @@ int batch_enqueue(synthe_batch* b, hipStream_t s0, bool prepare, bool gather, bo
     if ((rc = join_streams(b, s0))) return rc;
   }
   if (gather && (rc = launch_gather_batch(b, b->kwp, b->gather_blocks, s0))) return rc;
-  if (finish) {
-    if ((rc = fork_streams(b, b->ev_mid, s0))) return rc;
-    for (int i = 0; i < n; ++i)
-      if ((rc = batch_finish_one(b, b->inst[b->order[i]], b->streams[i % kBatchStreams]))) return rc;
-    if ((rc = join_streams(b, s0))) return rc;
-  }
+  if (finish && (rc = rt_launch_set(b->ctx, b->d_rt, n, b->rt_g, b->rt_base, b->rlay, s0))) return rc;
   return SYNTHE_OK;
 }
 
@@ -2022,13 +2045,14 @@ int synthe_batch_create(synthe_ctx* ctx, int n, const synthe_instance* inst, int
     bi.o_valid = off; off = align_up(off + (size_t)L * D);
     bi.alay = asynth_layout(L, (int)D);
     bi.o_asy = off; off = align_up(off + bi.alay.total);
-    bi.rlay = rt_layout(W);
-    bi.o_rt = off; off = align_up(off + bi.rlay.total);
+    b->rt_g += W;
     bi.cost = (double)L * (double)D + (double)W * (double)D;
     zeros_n = std::max(zeros_n, (size_t)D * W);
     b->inst.push_back(bi);
   }
   const size_t o_zero = off; off = align_up(off + zeros_n * sizeof(double));
+  b->rlay = rt_layout(b->rt_g, n);
+  const size_t o_rt = off; off = align_up(off + b->rlay.total);
   const size_t o_args = off; off = align_up(off + n * sizeof(GatherArgs));
   const size_t o_order = off; off = align_up(off + n * sizeof(int32_t));
   const size_t o_items = off; off = align_up(off + (n + 1) * sizeof(int64_t));
@@ -2040,6 +2064,8 @@ int synthe_batch_create(synthe_ctx* ctx, int n, const synthe_instance* inst, int
   }
   b->arena_size = off;
   b->zeros = (double*)(b->arena + o_zero);
+  b->rt_base = b->arena + o_rt;
+  b->d_rt = (RtArgs*)(b->rt_base + b->rlay.o_args);
   b->d_args = (GatherArgs*)(b->arena + o_args);
   b->d_order = (int32_t*)(b->arena + o_order);
   b->d_items = (int64_t*)(b->arena + o_items);
@@ -2062,7 +2088,19 @@ int synthe_batch_create(synthe_ctx* ctx, int n, const synthe_instance* inst, int
                           (uint8_t*)(b->arena + bi.o_valid), in.center_order, in.center_start,
                           (double*)(b->arena + bi.o_asynth), asynth_bufs(b->arena + bi.o_asy, bi.alay));
   }
-  if (hipMemcpy(b->d_args, args.data(), n * sizeof(GatherArgs), hipMemcpyHostToDevice) != hipSuccess ||
+  std::vector<RtArgs> rts(n);
+  long g0 = 0;
+  for (int i = 0; i < n; ++i) {
+    const BatchInst& bi = b->inst[b->order[i]];
+    const synthe_instance& in = bi.in;
+    rts[i] = RtArgs{g0, in.n_wl, (int)in.depths.n_depths, 1, in.wl, in.depths.temperature, in.depths.rhox,
+                    (const double*)(b->arena + bi.o_cabs), (const double*)(b->arena + bi.o_cscat),
+                    (const double*)(b->arena + bi.o_asynth), b->zeros, (const double*)(b->arena + bi.o_slinec),
+                    in.flux, in.cont, nullptr};
+    g0 += in.n_wl;
+  }
+  if (hipMemcpy(b->d_rt, rts.data(), n * sizeof(RtArgs), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->d_args, args.data(), n * sizeof(GatherArgs), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(b->d_order, b->order.data(), n * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemset(b->zeros, 0, zeros_n * sizeof(double)) != hipSuccess)
     return bail(fail(SYNTHE_EHIP, "synthe_batch_create: upload failed"));
