@@ -204,6 +204,52 @@ def bench_batched(args, wk) -> dict:
                         "windows U(300,1700) nm + U(5,50) nm, shared 1,000,000-line list, R=300000"}
 
 
+def bench_adjoint(args) -> dict:
+    """The adjoint configuration: Teff 5750 K, log g 4.44, 72 layers, 50,000
+    lines over 400-420 nm.  Times the differentiable forward (FP32 far-wing
+    profiles, FP64 RT) and its backward (gradients of sum(w F) with respect to
+    every line's log gf and log gamma_vdW, the 72 layer temperatures and the
+    20 abundances) beside the plain FP64 forward of the same spectrum."""
+    import torch
+
+    from kurucz_py_joss_submission_amd import synthetic as S
+    from kurucz_py_joss_submission_amd.adjoint import SpectrumModel
+    from kurucz_py_joss_submission_amd.device import SpectrumPlan
+
+    cfg = CONFIGS["c4"]
+    wk = make_workload(cfg)
+    cols = S.make_atm(cfg["teff"], cfg["logg"], 72, 0.0)
+    m = SpectrumModel(cols, wk.catalog, cfg["wl0"], cfg["wl1"], precision="fp32")
+    plan = SpectrumPlan(S.build_bundle(cols), wk.catalog, cfg["wl0"], cfg["wl1"], 300000.0)
+    w = torch.rand(m.W, dtype=torch.float64, device="cuda", generator=torch.Generator("cuda").manual_seed(1))
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    f_ms, b_ms, p_ms = [], [], []
+    for it in range(max(args.steps, 1) + 1):
+        T, da, lgf, lgw = m.base_parameters()
+        for t in (T, da, lgf, lgw):
+            t.requires_grad_(True)
+        torch.cuda.synchronize()
+        ev[0].record()
+        flux = m(T, da, lgf, lgw)
+        loss = (w * flux).sum()
+        ev[1].record()
+        loss.backward()
+        ev[2].record()
+        plan.run()
+        ev[3].record()
+        torch.cuda.synchronize()
+        if it > 0:  # the first pass warms up
+            f_ms.append(ev[0].elapsed_time(ev[1]))
+            b_ms.append(ev[1].elapsed_time(ev[2]))
+            p_ms.append(ev[2].elapsed_time(ev[3]))
+    fm, bm, pm = float(np.median(f_ms)), float(np.median(b_ms)), float(np.median(p_ms))
+    log(f"[bench] adjoint c4: forward {fm:.2f} ms, backward {bm:.2f} ms, plain FP64 forward {pm:.2f} ms")
+    return {"forward_ms": fm, "backward_ms": bm, "forward_plus_adjoint_ms": fm + bm,
+            "plain_fp64_forward_ms": pm, "n_lines": m.L, "n_wavelengths": m.W,
+            "gradients": {"log_gf": m.L, "log_gamma_vdw": m.L, "temperature": 72, "abundances": 20},
+            "workload": cfg["desc"]}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -212,6 +258,7 @@ def main() -> int:
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-atmosphere batched variant")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint variant")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain device copy of 1 GiB (3 times), to scale HBM counters")
     args = ap.parse_args()
@@ -289,6 +336,8 @@ def main() -> int:
         del plan
         torch.cuda.synchronize()
         variants["batched_c3"] = bench_batched(args, wk)
+    if args.config == "c2" and not args.no_adjoint:
+        variants["adjoint_c4"] = bench_adjoint(args)
 
     # dominant kernel: k_gather (centre adds + wing sums + STIM), one launch per
     # spectrum.  Algorithmic flops per evaluation: 3 for the far-wing value

@@ -233,6 +233,38 @@ int synthe_batch_info(const synthe_batch *b, int *pixels_per_lane, int *gather_b
                       int tune_cap, int *n_tuned, double *arena_bytes);
 int synthe_batch_destroy(synthe_batch *b);
 
+/* ------------------------------------------------------------------ adjoint
+ * Reverse-mode derivatives of the forward pass for the autograd wrapper (no
+ * reference counterpart: the reference has no derivatives).  Branches -- the
+ * KAPMIN gates, MAXSTEP, N10DOP, table indices, label 401 -- are held at their
+ * forward values.
+ *
+ * synthe_asynth_vjp: g_asynth [D][W] = dL/d(asynth) of a synthe_asynth call
+ * with the same inputs (ltype-0 lines).  Writes per (line, depth) pair [L][D]:
+ * s_kappa0 = dL/d(kappa0), s_adamp = dL/d(a) and s_dopple = dL/d(Doppler
+ * width / c) through the profile shape (kappa0 = cgf N/U FASTEX / (rho dop),
+ * a = gamma / dop; the caller chains those) and, if kappa0 is not NULL,
+ * kappa0 [L][D] itself (0 for pairs TRANSP rejected).  Recomputes the wing set-up and
+ * every profile value; accumulates with floating-point atomics, so the last
+ * bits can vary run to run.
+ *
+ * synthe_lte_vjp: v_flux [W] = dL/d(flux_total) (per Hz) of a
+ * synthe_lte_spectrum call with the same inputs; writes dL/d(cont_abs,
+ * cont_scat, line_abs) [D][W], dL/d(B) [D][W] for the Planck function at T(d)
+ * (continuum source, and line source when line_source is NULL) and
+ * dL/d(line_source) [D][W] (outputs may be NULL).  The scattering fixed point
+ * is differentiated implicitly at FP64 convergence.  At most 96 depths. */
+int synthe_asynth_vjp(synthe_ctx *ctx, const synthe_lines *lines, const synthe_depths *depths, long n_wl,
+                      const double *wl, const double *transp, const uint8_t *valid, const double *kapmin,
+                      double cutoff, const int64_t *center_order, const int64_t *center_start,
+                      const int64_t *wing_order, const double *g_asynth, double *s_kappa0, double *s_adamp,
+                      double *s_dopple, double *kappa0, synthe_stream stream);
+int synthe_lte_vjp(synthe_ctx *ctx, long n_depths, long n_wl, const double *wl, const double *temperature,
+                   const double *rhox, const double *cont_abs, const double *cont_scat, const double *line_abs,
+                   const double *line_scat, const double *line_source, const double *v_flux, double *g_cont_abs,
+                   double *g_cont_scat, double *g_line_abs, double *g_planck, double *g_line_source,
+                   synthe_stream stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -26,7 +26,7 @@ SYMBOLS = (
     "synthe_asynth", "synthe_lte_spectrum", "synthe_josh_flux_batch", "synthe_source_function",
     "synthe_flux_to_lambda", "synthe_diag_div_pos", "synthe_set_kernel_timing",
     "synthe_kernel_times", "synthe_set_profile_precision", "synthe_batch_create", "synthe_batch_run",
-    "synthe_batch_info", "synthe_batch_destroy",
+    "synthe_batch_info", "synthe_batch_destroy", "synthe_asynth_vjp", "synthe_lte_vjp",
 )
 
 SYNTHE_BATCH_AUTOTUNE = 1
@@ -115,6 +115,9 @@ def load_library(path: Optional[Path] = None):
         lib.synthe_batch_info.argtypes = [_VP, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int), _VP, ct.c_int,
                                           ct.POINTER(ct.c_int), ct.POINTER(ct.c_double)]
         lib.synthe_batch_destroy.argtypes = [_VP]
+        lib.synthe_asynth_vjp.argtypes = [_VP, ct.POINTER(Lines), ct.POINTER(Depths), ct.c_long, _VP, _VP, _VP,
+                                          _VP, ct.c_double, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]
+        lib.synthe_lte_vjp.argtypes = [_VP, ct.c_long, ct.c_long] + [_VP] * 15
         lib.synthe_kernel_times.argtypes = [_VP, _VP, ct.c_int, ct.POINTER(ct.c_int)]
         for name in SYMBOLS[2:]:
             if name != "synthe_wavelength_grid":

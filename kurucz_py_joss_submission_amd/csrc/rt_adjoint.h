@@ -17,7 +17,9 @@
 //    differentiated in forward mode with dual numbers, one input at a time,
 //    and contracted with those sensitivities;
 //  - the label-401 branch is differentiated in forward mode through its
-//    iteration (same iteration count as the values take).
+//    iteration, run (values and derivatives) to a relative change of 1e-12
+//    instead of the reference's 1e-5, so that both branches differentiate
+//    the converged solve.
 // Branch choices (label 401, the surface set XTAU < TAUNU(0), clamps) are
 // held fixed, as their derivative is zero almost everywhere.
 //
@@ -249,12 +251,14 @@ SYNTHE_HD void map1_t(const T* xold, const T* fold, int nold, const double* xnew
 
 // Label-401 branch (physics/josh_solver.py:700-727): returns HNU(0) after the
 // iteration; snu, hnu, jnu, tmp are work arrays of n.
+// The reference stops at a summed relative change below 1e-5 within 51
+// iterations; the adjoint's smooth model iterates to tol (max_iter).
 template <class T>
 SYNTHE_HD T josh401_t(int n, const T* taunu, const T* snubar, const T* alpha, T* snu, T* hnu, T* jnu, T* tmp,
-                      int* iters) {
+                      int* iters, double tol = 1.0e-5, int max_iter = kXt) {
   for (int j = 0; j < n; ++j) snu[j] = snubar[j];
   int it = 0;
-  for (int l = 0; l < kXt; ++l) {
+  for (int l = 0; l < max_iter; ++l) {
     deriv_t(taunu, snu, n, hnu);
     for (int j = 0; j < n; ++j) hnu[j] = hnu[j] / 3.0;
     deriv_t(taunu, hnu, n, jnu);
@@ -267,11 +271,14 @@ SYNTHE_HD T josh401_t(int n, const T* taunu, const T* snubar, const T* alpha, T*
       snu[j] = sn;
     }
     it = l + 1;
-    if (err < 1.0e-5) break;
+    if (err < tol) break;
   }
   if (iters) *iters = it;
   return hnu[0];
 }
+
+constexpr double kTol401 = 1.0e-12;  // label-401 convergence of the smooth model
+constexpr int kMax401 = 300;
 
 struct JoshTab {
   const double* xtau;    // [51]
@@ -347,7 +354,7 @@ SYNTHE_HD double total_flux64(const Column<kMax>& col, const JoshTab& tb, Work<k
   integ_t(col.abtot, col.rho, n, w.taunu, w.pa, w.pb, w.pc);
   if (w.taunu[0] > tb.xtau[kXt - 1]) {
     *took401 = 1;
-    return josh401_t(n, w.taunu, col.snubar, col.alpha, w.pa, w.pb, w.pc, w.xs, nullptr);
+    return josh401_t(n, w.taunu, col.snubar, col.alpha, w.pa, w.pb, w.pc, w.xs, nullptr, kTol401, kMax401);
   }
   *took401 = 0;
   map1_t(w.taunu, col.snubar, n, tb.xtau, w.xsbar);
@@ -388,7 +395,7 @@ SYNTHE_HD double total_flux_vjp(const Column<kMax>& col, const JoshTab& tb, Work
           w.d_sb[i] = Dual(col.snubar[i], q == 2 && i == j ? 1.0 : 0.0);
         }
         integ_t(w.d_in, col.rho, n, w.d_tau, w.d_a, w.d_b, w.d_c);
-        Dual f = josh401_t(n, w.d_tau, w.d_sb, w.d_al, w.d_s, w.d_h, w.d_j, w.d_t, nullptr);
+        Dual f = josh401_t(n, w.d_tau, w.d_sb, w.d_al, w.d_s, w.d_h, w.d_j, w.d_t, nullptr, kTol401, kMax401);
         g[j] = v * f.d;
       }
     }

@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "_tables_gen.h"
+#include "rt_adjoint.h"
 #include "rt_core.h"
 #include "synthe_device.h"
 #include "synthe_hip.h"
@@ -456,7 +457,8 @@ __global__ void k_regions(int D, long L, const int64_t* __restrict__ totals, int
 __global__ __launch_bounds__(kBlock) void k_class_scatter(
     long L, int D, const int64_t* __restrict__ order, const uint8_t* __restrict__ cls,
     const int32_t* __restrict__ offsets, const int64_t* __restrict__ regions,
-    const WingParams* __restrict__ wp, WingHead* __restrict__ list_cm, WingParams* __restrict__ list_p) {
+    const WingParams* __restrict__ wp, WingHead* __restrict__ list_cm, WingParams* __restrict__ list_p,
+    int32_t* __restrict__ list_l) {
   __shared__ int sh[kBlock / 64][kClasses];
   long s = blockIdx.x * (long)kBlock + threadIdx.x;
   long l = s < L ? order[s] : -1;
@@ -479,6 +481,7 @@ __global__ __launch_bounds__(kBlock) void k_class_scatter(
       WingParams p = wp[l * D + d];
       list_cm[pos] = WingHead{p.c, p.maxstep, p.n10dop, p.use_far};
       list_p[pos] = p;
+      if (list_l) list_l[pos] = (int32_t)l;
     }
     __syncthreads();
   }
@@ -1407,13 +1410,325 @@ static DepthsDev to_dev(const synthe_depths* d) {
                    d->pop,           d->dop,         (int)d->n_ion, (int)d->n_elem};
 }
 
+
+// ------------------------------------------------------------------ adjoint
+// Reverse-mode derivatives of the synthesis for the autograd wrapper.  The
+// forward pass is not taped: the backward recomputes the wing set-up, the
+// lists and every profile value it needs.
+
+constexpr int kAdjMaxDepth = 96;  // depth limit of the RT vjp (private memory per lane)
+
+// Voigt-zone profile H at offset k (near_wing's formula, times kappa0 there)
+// and its partial derivatives in the damping a and in x = k * dvoigt: exact
+// for the table and asymptotic forms (the table index held fixed; dH/dx uses
+// d/dx exp(-x^2) and the centred difference of H1), central differences of
+// voigt() for a >= 0.2.
+__device__ void near_wing_partials(const WingParams& p, long k, const DeviceTables& tb, double* H, double* dHda,
+                                   double* dHdx) {
+  const double x = (double)k * p.dvoigt;
+  const double a = p.adamp;
+  if (a < 0.2) {
+    if (x > 10.0) {
+      *H = 0.5642 * a / (x * x);
+      *dHda = 0.5642 / (x * x);
+      *dHdx = -2.0 * 0.5642 * a / (x * x * x);
+      return;
+    }
+    long idx = tabi_index(200.0 * p.dvoigt, k);
+    idx = idx < 0 ? 0 : (idx > kVoigtN - 1 ? kVoigtN - 1 : idx);
+    *H = tb.h0[idx] + a * tb.h1[idx];
+    *dHda = tb.h1[idx];
+    const long i0 = idx > 0 ? idx - 1 : 0, i1 = idx < kVoigtN - 1 ? idx + 1 : kVoigtN - 1;
+    const double dh1 = (tb.h1[i1] - tb.h1[i0]) * 200.0 / (double)(i1 - i0);
+    *dHdx = -2.0 * x * exp(-x * x) + a * dh1;
+    return;
+  }
+  *H = voigt(x, a, tb.h0, tb.h1, tb.h2);
+  const double ea = 1e-6 * (a > 1e-3 ? a : 1e-3), ex = 1e-6 * (x > 1e-3 ? x : 1e-3);
+  *dHda = (voigt(x, a + ea, tb.h0, tb.h1, tb.h2) - voigt(x, a - ea, tb.h0, tb.h1, tb.h2)) / (2.0 * ea);
+  *dHdx = (voigt(x + ex, a, tb.h0, tb.h1, tb.h2) - voigt(x > ex ? x - ex : 0.0, a, tb.h0, tb.h1, tb.h2)) /
+          (x + ex - (x > ex ? x - ex : 0.0));
+}
+
+__device__ __forceinline__ double stim_at(const double* wl, const double* temp, int d, long w) {
+  const double T = temp[d] > 1.0 ? temp[d] : 1.0;
+  const double hkt = kHPlanck / (kKBoltz * T);
+  return 1.0 - exp(-(kCLightNm / wl[w]) * hkt);
+}
+
+// u[d][w] = dL/d(asynth) * STIM: the weight of each pre-STIM contribution.
+__global__ void k_adj_weights(int D, long W, const double* __restrict__ wl, const double* __restrict__ temp,
+                              const double* __restrict__ g_asy, double* __restrict__ u) {
+  const long w = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const int d = blockIdx.y;
+  if (w >= W) return;
+  u[(long)d * W + w] = g_asy[(long)d * W + w] * stim_at(wl, temp, d, w);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+  return v;
+}
+
+// Line centres (line_opacity.py:1091-1102): transp = kappa0 * vc(a) at the
+// centre pixel, vc = 1 - 1.128 a or H(0, a) (line_opacity.py:123-184).
+__global__ void k_adj_centre(LinesDev ln, DepthsDev dp, long W, const double* __restrict__ transp,
+                             const uint8_t* __restrict__ valid, const double* __restrict__ u, DeviceTables tb,
+                             double* __restrict__ s_k0, double* __restrict__ s_a, double* __restrict__ k0_out) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long L = ln.n;
+  const int D = dp.n;
+  if (i >= L * D) return;
+  if (!valid[i]) return;
+  const long l = i / D;
+  const int d = (int)(i - l * D);
+  const long c = ln.center_t[l];
+  if (ln.ltype[l] != 0) return;
+  const int ei = ln.elem[l], nel = ln.ion[l];
+  const long pi = ((long)d * dp.nion + (nel - 1)) * dp.nelem + ei;
+  const double dv = dp.dop[pi];
+  const double gt = ln.grad[l] + ln.gstark[l] * dp.xne[d] + ln.gvdw[l] * dp.txnxn[d];
+  const double a = gt / dv;
+  double vc, dvc;
+  if (a < 0.2) {
+    vc = 1.0 - 1.128 * a;
+    dvc = -1.128;
+  } else {
+    vc = voigt(0.0, a, tb.h0, tb.h1, tb.h2);
+    const double ea = 1e-6 * a;
+    dvc = (voigt(0.0, a + ea, tb.h0, tb.h1, tb.h2) - voigt(0.0, a - ea, tb.h0, tb.h1, tb.h2)) / (2.0 * ea);
+  }
+  const double post = transp[i] / vc;
+  if (k0_out) k0_out[i] = post;
+  if (c < 0 || c >= W) return;
+  const double uc = u[(long)d * W + c];
+  s_k0[i] += uc * vc;
+  s_a[i] += uc * post * dvc;
+}
+
+// Voigt zones (the k_near set): per list entry, sums over its pixels of
+// u * dc/dkappa0, u * dc/da, u * dc/ddopple with c = kappa0 H(k dvoigt, a).
+__global__ __launch_bounds__(256) void k_near_adj(long W, int D, const WingHead* __restrict__ list_cm,
+                                                  const WingParams* __restrict__ list_p,
+                                                  const int32_t* __restrict__ list_l,
+                                                  const int64_t* __restrict__ regions,
+                                                  const int32_t* __restrict__ near_reach, DeviceTables tb,
+                                                  const double* __restrict__ u, const double* __restrict__ wl,
+                                                  double* __restrict__ s_k0, double* __restrict__ s_a,
+                                                  double* __restrict__ s_dv) {
+  const int d = blockIdx.y;
+  const int lane = threadIdx.x & 63;
+  const long v0 = ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
+  if (v0 >= W) return;
+  const long w = v0 + lane;
+  const double resolu = W > 1 && wl[1] / wl[0] > 1.0 ? 1.0 / (wl[1] / wl[0] - 1.0) : 300000.0;
+  const double uw = w < W ? u[(long)d * W + w] : 0.0;
+  const long bound = class_window_search(lane, d, v0, v0 + 63, (long)near_reach[d], regions, list_cm);
+  for (int kk = 0; kk < kClasses; ++kk) {
+    const long lo = uniform_long(__shfl(bound, 2 * kk));
+    const long hi = uniform_long(__shfl(bound, 2 * kk + 1));
+    for (long j = lo; j < hi; ++j) {
+      const WingHead h = list_cm[j];
+      const long c = h.c;
+      const long lim = h.use_far ? h.n10dop : h.maxstep;
+      if (v0 + 63 < c - lim || v0 > c + lim) continue;  // wave-uniform
+      const long k = w - c;
+      const long ka = k < 0 ? -k : k;
+      double c0 = 0.0, ca = 0.0, cd = 0.0;
+      if (ka >= 1 && ka <= lim && w < W) {
+        const WingParams pr = list_p[j];
+        double H, dHa, dHx;
+        near_wing_partials(pr, ka, tb, &H, &dHa, &dHx);
+        const double x = (double)ka * pr.dvoigt;
+        c0 = uw * H;
+        ca = uw * pr.kappa0 * dHa;
+        cd = uw * pr.kappa0 * dHx * (-x * pr.dvoigt * resolu);
+      }
+      c0 = wave_sum(c0);
+      ca = wave_sum(ca);
+      cd = wave_sum(cd);
+      if (lane == 0) {
+        const long pair = (long)list_l[j] * D + d;
+        atomicAdd(s_k0 + pair, c0);
+        atomicAdd(s_a + pair, ca);
+        atomicAdd(s_dv + pair, cd);
+      }
+    }
+  }
+}
+
+// Far wings: per list entry, Q = sum over its far pixels (N10DOP < |k| <=
+// MAXSTEP) of u / k^2; one wave per (512-pixel tile, depth) as k_gather.
+__global__ __launch_bounds__(64) void k_far_adj(GatherArgs g, const double* __restrict__ u,
+                                                double* __restrict__ q_entry) {
+  constexpr int kP = 8;
+  constexpr long kT = 64L * kP;
+  const int d = blockIdx.y;
+  const long t0 = blockIdx.x * kT, t1 = t0 + kT - 1;
+  const int lane = threadIdx.x;
+  const long W = g.W;
+  const long w0 = t0 + (long)kP * lane;
+  double uu[kP];
+#pragma unroll
+  for (int p = 0; p < kP; ++p) uu[p] = w0 + p < W ? u[(long)d * W + w0 + p] : 0.0;
+  const long bound = class_window_search(lane, d, t0, t1, 1L << 40, g.regions, g.list_cm);
+  for (int kk = 0; kk < kClasses; ++kk) {
+    const long lo = uniform_long(__shfl(bound, 2 * kk));
+    const long hi = uniform_long(__shfl(bound, 2 * kk + 1));
+    for (long j = lo; j < hi; ++j) {
+      const WingHead h = g.list_cm[j];
+      const long c = h.c, M = h.maxstep;
+      if (!h.use_far || t1 - c < -M || t0 - c > M) continue;
+      const double n10d = (double)h.n10dop, md = (double)M;
+      const double kd = (double)(w0 - c);
+      double q = 0.0;
+#pragma unroll
+      for (int p = 0; p < kP; ++p) {
+        const double k = kd + (double)p;
+        const double ka = fabs(k);
+        const double up = (ka > n10d && ka <= md) ? uu[p] : 0.0;
+        q += div_fast(up, fmax(k * k, 1.0));
+      }
+      q = wave_sum(q);
+      if (lane == 0) atomicAdd(q_entry + j, q);
+    }
+  }
+}
+
+// Far-wing sums to the entry's pair: X = kappa0 H(x10, a) N10DOP^2, x10 =
+// N10DOP dvoigt (line_opacity.py:562-617).
+__global__ void k_far_finish(int D, const int64_t* __restrict__ regions, const WingParams* __restrict__ list_p,
+                             const int32_t* __restrict__ list_l, const double* __restrict__ q_entry,
+                             DeviceTables tb, const double* __restrict__ wl, long W, double* __restrict__ s_k0,
+                             double* __restrict__ s_a, double* __restrict__ s_dv) {
+  const int d = blockIdx.y;
+  const long beg = regions[d * (kClasses + 1)], end = regions[d * (kClasses + 1) + kClasses];
+  const double resolu = W > 1 && wl[1] / wl[0] > 1.0 ? 1.0 / (wl[1] / wl[0] - 1.0) : 300000.0;
+  for (long j = beg + blockIdx.x * (long)blockDim.x + threadIdx.x; j < end; j += (long)gridDim.x * blockDim.x) {
+    const double q = q_entry[j];
+    if (q == 0.0) continue;
+    const WingParams pr = list_p[j];
+    if (!pr.use_far) continue;
+    const long n10 = pr.n10dop;
+    double H, dHa, dHx;
+    near_wing_partials(pr, n10, tb, &H, &dHa, &dHx);
+    const double n2 = (double)n10 * (double)n10;
+    const double x10 = (double)n10 * pr.dvoigt;
+    const long pair = (long)list_l[j] * D + d;
+    atomicAdd(s_k0 + pair, q * H * n2);
+    atomicAdd(s_a + pair, q * pr.kappa0 * n2 * dHa);
+    atomicAdd(s_dv + pair, q * pr.kappa0 * n2 * dHx * (-x10 * pr.dvoigt * resolu));
+  }
+}
+
+// RT vjp per wavelength (total-flux solve, physics/josh_solver.py:367-767
+// through engine/radiative.py:42-218 masks): v[w] = dL/dF_nu(w); outputs
+// dL/d(cont_abs, cont_scat, line_abs) [D][W], dL/dB (the Planck function at
+// T(d) used as the continuum source) and dL/d(line_source), 0 on masked depths.
+__global__ __launch_bounds__(64) void k_rt_vjp(int D, long W, const double* __restrict__ wl,
+                                               const double* __restrict__ temp, const double* __restrict__ rhox,
+                                               const double* __restrict__ ca, const double* __restrict__ cs,
+                                               const double* __restrict__ la, const double* __restrict__ ls,
+                                               const double* __restrict__ src, const double* __restrict__ v,
+                                               DeviceTables tb, double* __restrict__ g_ca, double* __restrict__ g_cs,
+                                               double* __restrict__ g_la, double* __restrict__ g_b,
+                                               double* __restrict__ g_src) {
+  const long w = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (w >= W) return;
+  for (int d = 0; d < D; ++d) {
+    const long i = (long)d * W + w;
+    if (g_ca) g_ca[i] = 0.0;
+    if (g_cs) g_cs[i] = 0.0;
+    if (g_la) g_la[i] = 0.0;
+    if (g_b) g_b[i] = 0.0;
+    if (g_src) g_src[i] = 0.0;
+  }
+  const double vw = v[w];
+  if (vw == 0.0) return;
+  const double maxr4 = 3.4e38;
+  int idx[kAdjMaxDepth];
+  int n = 0;
+  for (int d = 0; d < D && n < kAdjMaxDepth; ++d) {
+    const long i = (long)d * W + w;
+    const double r = rhox[d], a1 = ca[i], a2 = cs[i], a3 = la[i], a4 = ls[i];
+    const bool okd = r >= 0.0 && isfinite(r) && isfinite(a1) && isfinite(a2) && isfinite(a3) && isfinite(a4) &&
+                     a1 < maxr4 && a2 < maxr4 && a3 < maxr4 && a4 < maxr4;
+    if (okd && r > 0) idx[n++] = d;
+  }
+  if (n < 4) return;
+  if (!(rhox[idx[0]] < rhox[idx[n - 1]])) {
+    for (int j = 0; j < n / 2; ++j) {
+      int t = idx[j];
+      idx[j] = idx[n - 1 - j];
+      idx[n - 1 - j] = t;
+    }
+  }
+  const double wv = wl[w];
+  const double freq = kCLightNm / (wv > 1e-12 ? wv : 1e-12);
+  const double f3 = pow(freq / 1.0e15, 3.0);
+  adj::Column<kAdjMaxDepth> col;
+  adj::Work<kAdjMaxDepth> wk;
+  double bc[kAdjMaxDepth], sl[kAdjMaxDepth];
+  col.n = n;
+  for (int j = 0; j < n; ++j) {
+    const int d = idx[j];
+    const long i = (long)d * W + w;
+    const double B = planck_one(freq, f3, temp[d]);
+    double S = B;
+    if (src) {
+      const double s0 = src[i];
+      S = isfinite(s0) ? s0 : B;
+    }
+    bc[j] = B;
+    sl[j] = S;
+    const double ac = ca[i], al = la[i], sc = cs[i], sgl = ls[i];
+    double ab = ac + al + sc + sgl;
+    ab = ab > kEps38 ? ab : kEps38;
+    double alpha = ab > 0.0 ? (sc + sgl) / ab : 0.0;
+    alpha = alpha < 0.0 ? 0.0 : (alpha > 1.0 ? 1.0 : alpha);
+    const double dn = ac + al;
+    col.rho[j] = rhox[d];
+    col.abtot[j] = ab;
+    col.alpha[j] = alpha;
+    col.snubar[j] = dn > 0 ? (ac * B + al * S) / dn : B;
+  }
+  const adj::JoshTab jt{tb.xtau, tb.ch, tb.coefj32};
+  double gab[kAdjMaxDepth], gal[kAdjMaxDepth], gsb[kAdjMaxDepth];
+  adj::total_flux_vjp(col, jt, wk, vw, gab, gal, gsb);
+  for (int j = 0; j < n; ++j) {
+    const int d = idx[j];
+    const long i = (long)d * W + w;
+    const double ac = ca[i], al = la[i];
+    const double ab = col.abtot[j], alpha = col.alpha[j], sb = col.snubar[j];
+    const double dn = ac + al;
+    const double galpha = (alpha > 0.0 && alpha < 1.0) ? gal[j] : 0.0;
+    const double da_dab = -alpha / ab;  // d alpha / d(ac or al)
+    double gsb_ac = 0.0, gsb_al = 0.0, gB = gsb[j], gS = 0.0;
+    if (dn > 0) {
+      gsb_ac = gsb[j] * (bc[j] - sb) / dn;
+      gsb_al = gsb[j] * (sl[j] - sb) / dn;
+      gB = gsb[j] * ac / dn;
+      gS = gsb[j] * al / dn;
+    }
+    if (g_ca) g_ca[i] = gab[j] + galpha * da_dab + gsb_ac;
+    if (g_la) g_la[i] = gab[j] + galpha * da_dab + gsb_al;
+    if (g_cs) g_cs[i] = gab[j] + galpha * (1.0 - alpha) / ab;
+    if (src) {
+      if (g_b) g_b[i] = gB;
+      if (g_src) g_src[i] = gS;
+    } else if (g_b) {
+      g_b[i] = gB + gS;
+    }
+  }
+}
+
 // ------------------------------------------------------------ ASYNTH pieces
 // Workspace of one ASYNTH call (offsets from a 256-byte aligned base).
 struct AsynthLayout {
-  size_t o_wp, o_cls, o_cnt, o_tot, o_reg, o_lcm, o_lp, o_ev, o_nr, total;
+  size_t o_wp, o_cls, o_cnt, o_tot, o_reg, o_lcm, o_lp, o_ev, o_nr, o_ll, total;
 };
 
-static AsynthLayout asynth_layout(long L, int D) {
+static AsynthLayout asynth_layout(long L, int D, bool line_index = false) {
   AsynthLayout y{};
   const long pairs = L * (long)D;
   const long nblk = (L + kBlock - 1) / kBlock;
@@ -1427,6 +1742,7 @@ static AsynthLayout asynth_layout(long L, int D) {
   y.o_lp = off; off = align_up(off + (size_t)pairs * sizeof(WingParams));
   y.o_ev = off; off = align_up(off + 5 * sizeof(unsigned long long));
   y.o_nr = off; off = align_up(off + (size_t)D * sizeof(int32_t));
+  y.o_ll = line_index ? off : 0; off = line_index ? align_up(off + (size_t)pairs * sizeof(int32_t)) : off;
   y.total = off;
   return y;
 }
@@ -1441,13 +1757,14 @@ struct AsynthBufs {
   WingParams* lp;
   unsigned long long* ev;
   int32_t* near_reach;
+  int32_t* list_l;  // line of each list entry, or NULL
 };
 
 static AsynthBufs asynth_bufs(char* base, const AsynthLayout& y) {
   return AsynthBufs{(WingParams*)(base + y.o_wp), (uint8_t*)(base + y.o_cls), (int32_t*)(base + y.o_cnt),
                     (int64_t*)(base + y.o_tot),   (int64_t*)(base + y.o_reg), (WingHead*)(base + y.o_lcm),
                     (WingParams*)(base + y.o_lp), (unsigned long long*)(base + y.o_ev),
-                    (int32_t*)(base + y.o_nr)};
+                    (int32_t*)(base + y.o_nr), y.o_ll ? (int32_t*)(base + y.o_ll) : nullptr};
 }
 
 static GatherArgs gather_args(const synthe_lines* lines, const synthe_depths* depths, long n_wl, const double* wl,
@@ -1477,7 +1794,7 @@ static int check_asynth_args(const synthe_lines* lines, const synthe_depths* dep
 static int asynth_prepare(const synthe_ctx* ctx, const synthe_lines* lines, const synthe_depths* depths,
                           long n_wl, const double* wl, const double* transp, const uint8_t* valid,
                           const double* kapmin, double cutoff, const int64_t* wing_order, double* asynth,
-                          bool stats, const AsynthBufs& b, hipStream_t st) {
+                          bool stats, const AsynthBufs& b, hipStream_t st, bool near_values = true) {
   const long L = lines->n_lines;
   const int D = (int)depths->n_depths;
   const long pairs = L * (long)D;
@@ -1496,7 +1813,7 @@ static int asynth_prepare(const synthe_ctx* ctx, const synthe_lines* lines, cons
     hipLaunchKernelGGL(k_regions, dim3(blocks_for(D, 64)), dim3(64), 0, st, D, L, b.tot, b.reg);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_class_scatter, dim3((unsigned)nblk), dim3(kBlock), 0, st, L, D, wing_order, b.cls, b.cnt,
-                       b.reg, b.wp, b.lcm, b.lp);
+                       b.reg, b.wp, b.lcm, b.lp, b.list_l);
     HIP_TRY(hipGetLastError());
   } else {
     const long n = (long)D * (kClasses + 1);
@@ -1509,6 +1826,7 @@ static int asynth_prepare(const synthe_ctx* ctx, const synthe_lines* lines, cons
     hipLaunchKernelGGL(k_near_reach, dim3(64, (unsigned)D), dim3(kBlock), 0, st, D, b.reg, b.lcm, b.near_reach);
     HIP_TRY(hipGetLastError());
   }
+  if (!near_values) return SYNTHE_OK;
   const dim3 g_near((unsigned)blocks_for(n_wl, 256), (unsigned)D);
   if (stats)
     hipLaunchKernelGGL(k_near<true>, g_near, dim3(256), 0, st, n_wl, D, b.lcm, b.lp, b.reg, b.near_reach, ctx->tb,
@@ -2210,6 +2528,82 @@ int synthe_batch_destroy(synthe_batch* b) {
     (void)hipDeviceSynchronize();
     batch_free(b);
   }
+  return SYNTHE_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ adjoint ABI
+extern "C" {
+
+int synthe_asynth_vjp(synthe_ctx* ctx, const synthe_lines* lines, const synthe_depths* depths, long n_wl,
+                      const double* wl, const double* transp, const uint8_t* valid, const double* kapmin,
+                      double cutoff, const int64_t* center_order, const int64_t* center_start,
+                      const int64_t* wing_order, const double* g_asynth, double* s_kappa0, double* s_adamp,
+                      double* s_dopple, double* kappa0, synthe_stream stream) {
+  if (!ctx) return fail(SYNTHE_EINVAL, "synthe_asynth_vjp: ctx is NULL");
+  int rc = check_asynth_args(lines, depths, n_wl, wl, transp, valid, kapmin, center_order, center_start, wing_order,
+                             g_asynth, "synthe_asynth_vjp");
+  if (rc) return rc;
+  if (!s_kappa0 || !s_adamp || !s_dopple) return fail(SYNTHE_EINVAL, "synthe_asynth_vjp: NULL output");
+  hipStream_t st = (hipStream_t)stream;
+  const long L = lines->n_lines;
+  const int D = (int)depths->n_depths;
+  const long pairs = L * (long)D;
+  const AsynthLayout lay = asynth_layout(L, D, true);
+  const size_t o_u = lay.total;
+  const size_t o_q = align_up(o_u + (size_t)D * n_wl * sizeof(double));
+  const size_t total = align_up(o_q + (size_t)(pairs > 0 ? pairs : 1) * sizeof(double));
+  rc = ensure_ws(ctx, total);
+  if (rc) return rc;
+  char* base = (char*)ctx->ws;
+  const AsynthBufs b = asynth_bufs(base, lay);
+  double* u = (double*)(base + o_u);
+  double* q = (double*)(base + o_q);
+  for (double* z : {s_kappa0, s_adamp, s_dopple, q, kappa0}) {
+    if (!z) continue;
+    hipLaunchKernelGGL(k_fill_u64, dim3(blocks_for(pairs > 0 ? pairs : 1, kBlock)), dim3(kBlock), 0, st,
+                       pairs > 0 ? pairs : 1L, (unsigned long long*)z);
+    HIP_TRY(hipGetLastError());
+  }
+  if (pairs == 0) return SYNTHE_OK;
+  rc = asynth_prepare(ctx, lines, depths, n_wl, wl, transp, valid, kapmin, cutoff, wing_order, nullptr, false, b,
+                      st, false);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_adj_weights, dim3(blocks_for(n_wl, kBlock), (unsigned)D), dim3(kBlock), 0, st, D, n_wl, wl,
+                     depths->temperature, g_asynth, u);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_adj_centre, dim3(blocks_for(pairs, kBlock)), dim3(kBlock), 0, st, to_dev(lines),
+                     to_dev(depths), n_wl, transp, valid, u, ctx->tb, s_kappa0, s_adamp, kappa0);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_near_adj, dim3(blocks_for(n_wl, 256), (unsigned)D), dim3(256), 0, st, n_wl, D, b.lcm, b.lp,
+                     b.list_l, b.reg, b.near_reach, ctx->tb, u, wl, s_kappa0, s_adamp, s_dopple);
+  HIP_TRY(hipGetLastError());
+  const GatherArgs ga = gather_args(lines, depths, n_wl, wl, transp, valid, center_order, center_start, nullptr, b);
+  hipLaunchKernelGGL(k_far_adj, dim3(blocks_for(n_wl, 512), (unsigned)D), dim3(64), 0, st, ga, u, q);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_far_finish, dim3(64, (unsigned)D), dim3(kBlock), 0, st, D, b.reg, b.lp, b.list_l, q, ctx->tb,
+                     wl, n_wl, s_kappa0, s_adamp, s_dopple);
+  HIP_TRY(hipGetLastError());
+  return SYNTHE_OK;
+}
+
+int synthe_lte_vjp(synthe_ctx* ctx, long n_depths, long n_wl, const double* wl, const double* temperature,
+                   const double* rhox, const double* cont_abs, const double* cont_scat, const double* line_abs,
+                   const double* line_scat, const double* line_source, const double* v_flux, double* g_cont_abs,
+                   double* g_cont_scat, double* g_line_abs, double* g_planck, double* g_line_source,
+                   synthe_stream stream) {
+  if (!ctx) return fail(SYNTHE_EINVAL, "synthe_lte_vjp: ctx is NULL");
+  if (n_depths <= 0 || n_wl < 0) return fail(SYNTHE_EINVAL, "synthe_lte_vjp: bad sizes");
+  if (n_depths > kAdjMaxDepth)
+    return fail(SYNTHE_ERANGE, "synthe_lte_vjp: more than " + std::to_string(kAdjMaxDepth) + " depths");
+  if (n_wl == 0) return SYNTHE_OK;
+  if (!wl || !temperature || !rhox || !cont_abs || !cont_scat || !line_abs || !line_scat || !v_flux)
+    return fail(SYNTHE_EINVAL, "synthe_lte_vjp: NULL input");
+  hipLaunchKernelGGL(k_rt_vjp, dim3(blocks_for(n_wl, 64)), dim3(64), 0, (hipStream_t)stream, (int)n_depths, n_wl, wl,
+                     temperature, rhox, cont_abs, cont_scat, line_abs, line_scat, line_source, v_flux, ctx->tb,
+                     g_cont_abs, g_cont_scat, g_line_abs, g_planck, g_line_source);
+  HIP_TRY(hipGetLastError());
   return SYNTHE_OK;
 }
 

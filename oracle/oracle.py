@@ -177,6 +177,11 @@ def josh_iterate(xs, xalpha, xsbar_mod, max_iter=51):
     return xs, int(it)
 
 
+def set_josh_fp64(on: bool, tol: float = 1e-13) -> None:
+    """Test-only smooth mode: FP64 scattering sweep and label-401 iteration to tol."""
+    lib().orc_set_josh_fp64(ct.c_int(int(on)), ct.c_double(tol))
+
+
 def solve_josh_flux(acont, scont, aline, sline, sigmac, sigmal, rhox):
     tables()
     arrs = [_c(a, np.float64) for a in (acont, scont, aline, sline, sigmac, sigmal, rhox)]

@@ -822,6 +822,37 @@ int orc_map1_josh(const double *xold, const double *fold, int nold, const double
 
 /* physics/josh_solver.py:24-92 (_josh_iteration_kernel) as Numba types it
  * when driven from 655-680: float32 arrays, float64 accumulator. */
+/* Test-only smooth mode (orc_set_josh_fp64): the scattering sweep in FP64
+ * on the same float32 COEFJ values and the label-401 iteration, both to a
+ * relative change below tol.  The reference's float32 sweep stops at 1e-5,
+ * which makes finite differences of the flux noisy; this mode is the FP64
+ * reference the adjoint's finite-difference checks use.  Off by default. */
+static int g_josh_fp64 = 0;
+static double g_josh_tol = 1e-13;
+
+void orc_set_josh_fp64(int on, double tol) {
+    g_josh_fp64 = on;
+    g_josh_tol = tol;
+}
+
+static int josh_iterate64(double *xs, const double *xa, const double *xb) {
+    for (int k = 0; k < NXTAU; ++k) xs[k] = xb[k];
+    for (int it = 0; it < 400; ++it) {
+        double err = 0.0;
+        for (int k = NXTAU - 1; k >= 0; --k) {
+            double cx = 0.0;
+            for (int m = 0; m < NXTAU; ++m) cx += (double)g_coefj32[k * NXTAU + m] * xs[m];
+            double ckk = (double)g_coefj32[k * NXTAU + k];
+            double nv = (xb[k] * (1.0 - xa[k]) + xa[k] * (cx - ckk * xs[k])) / (1.0 - xa[k] * ckk);
+            double e = fabs(nv - xs[k]) / (fabs(nv) > EPS38 ? fabs(nv) : EPS38);
+            if (e > err) err = e;
+            xs[k] = nv;
+        }
+        if (err < g_josh_tol) return it + 1;
+    }
+    return 400;
+}
+
 int orc_josh_iterate(float *xs, const float *xalpha, const float *xsbar_mod, int max_iter) {
     float diag[NXTAU];
     for (int k = 0; k < NXTAU; ++k) diag[k] = 1.0f - xalpha[k] * g_diag32[k];
@@ -912,7 +943,9 @@ double orc_solve_josh_flux(long n, const double *acont, const double *scont, con
         /* label-401 branch (josh_solver.py:700-727) */
         for (long j = 0; j < n; ++j) snu[j] = snubar[j];
         int it = 0;
-        for (int l = 0; l < NXTAU; ++l) {
+        const int max401 = g_josh_fp64 ? 300 : NXTAU;
+        const double tol401 = g_josh_fp64 ? g_josh_tol * 10.0 : 1.0e-5;
+        for (int l = 0; l < max401; ++l) {
             orc_deriv(taunu, snu, n, hnu);
             for (long j = 0; j < n; ++j) hnu[j] = hnu[j] / 3.0;
             orc_deriv(taunu, hnu, n, jnu);
@@ -925,7 +958,7 @@ double orc_solve_josh_flux(long n, const double *acont, const double *scont, con
             }
             double err = np_pairwise_sum(tmp, n);
             it = l + 1;
-            if (err < 1.0e-5) break;
+            if (err < tol401) break;
         }
         flux = hnu[0];
         if (info) { info[0] = it; info[1] = 1; }
@@ -944,9 +977,23 @@ double orc_solve_josh_flux(long n, const double *acont, const double *scont, con
             xal32[k] = (float)xa;
             xsb32[k] = (float)(xb * (1.0 - xa));
         }
-        int it = orc_josh_iterate(xs32, xal32, xsb32, NXTAU);
+        int it;
         flux = 0.0;
-        for (int k = 0; k < NXTAU; ++k) flux += g_ch[k] * (double)xs32[k];
+        if (g_josh_fp64) {
+            double xa64[NXTAU], xb64[NXTAU], xs64[NXTAU];
+            for (int k = 0; k < NXTAU; ++k) {
+                double xb = xsbar[k] > EPS38 ? xsbar[k] : EPS38;
+                double xa = xalpha[k] < 0.0 ? 0.0 : (xalpha[k] > 1.0 ? 1.0 : xalpha[k]);
+                if (g_xtau[k] < taunu[0]) { xb = s0; xa = a0; }
+                xa64[k] = xa;
+                xb64[k] = xb;
+            }
+            it = josh_iterate64(xs64, xa64, xb64);
+            for (int k = 0; k < NXTAU; ++k) flux += g_ch[k] * xs64[k];
+        } else {
+            it = orc_josh_iterate(xs32, xal32, xsb32, NXTAU);
+            for (int k = 0; k < NXTAU; ++k) flux += g_ch[k] * (double)xs32[k];
+        }
         if (info) { info[0] = it; info[1] = 0; }
     }
     free(buf);

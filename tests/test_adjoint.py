@@ -1,0 +1,198 @@
+"""Adjoint of the synthesis (torch.autograd.Function over the HIP forward and
+hand-written HIP backward) against the FP64 reference gradient: central
+finite differences of the oracle with its JOSH solve converged in FP64
+(steps: 1e-4 in log gf and log gamma_vdW, 1e-3 dex in abundances; T in its
+own tests).  The FP32-profile forward's gradient is checked against the FP64
+one.  Workload: Teff 5750 K, log g 4.44, 72 layers, 400-400.6 nm."""
+
+from __future__ import annotations
+
+import copy
+import tempfile
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+WL0, WL1 = 400.0, 400.6
+
+
+@pytest.fixture(scope="module")
+def adj_case():
+    from kurucz_py_joss_submission_amd import lines as L
+    from kurucz_py_joss_submission_amd import synthetic as S
+
+    cols = S.make_atm(5750.0, 4.44, 72, 0.0)
+    d = S.synthetic_lines(600, WL0 - 0.5, WL1 + 0.5, seed=21)
+    with tempfile.TemporaryDirectory() as td:
+        gf = Path(td) / "lines.gfall"
+        S.write_line_list(gf, d)
+        cat = L.filter_by_range(L.load_catalog(gf), WL0, WL1)
+    return cols, cat
+
+
+def _loss_weights(W, seed=4):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(0.5, 1.5, W)
+
+
+def _model_grads(cols, cat, precision, cutoff=1e-3):
+    import torch
+
+    from kurucz_py_joss_submission_amd.adjoint import SpectrumModel
+
+    m = SpectrumModel(cols, cat, WL0, WL1, precision=precision, cutoff=cutoff)
+    T, da, lgf, lgw = m.base_parameters()
+    for t in (T, da, lgf, lgw):
+        t.requires_grad_(True)
+    flux = m(T, da, lgf, lgw)
+    w = torch.tensor(_loss_weights(m.W), dtype=torch.float64, device="cuda")
+    loss = (w * flux).sum()
+    loss.backward()
+    torch.cuda.synchronize()
+    return m, flux.detach().cpu().numpy(), {k: t.grad.detach().cpu().numpy()
+                                             for k, t in (("T", T), ("A", da), ("gf", lgf), ("gw", lgw))}
+
+
+def _oracle_loss(oracle, cols, cat, w, abund=None, cutoff=1e-3):
+    from kurucz_py_joss_submission_amd import synthetic as S
+
+    atm = S.build_bundle(cols, abund)
+    r = oracle.synthesize(atm, cat, WL0, WL1, 300000.0, cutoff=cutoff, keep_stages=False)
+    return float(np.sum(w * r.flux))
+
+
+def _cols_with_T(cols, d, factor):
+    from kurucz_py_joss_submission_amd import synthetic as S
+
+    c = copy.deepcopy(cols)
+    c.temperature = c.temperature.copy()
+    c.temperature[d] *= factor
+    c.abross = S._kappa_rosseland(c.pressure, c.temperature)
+    return c
+
+
+def _cat_scaled(cat, field, l, factor):
+    c = copy.deepcopy(cat)
+    arr = getattr(c, field).copy()
+    arr[l] *= factor
+    setattr(c, field, arr)
+    return c
+
+
+@pytest.mark.gpu
+def test_adjoint_forward_matches_spectrum_plan(gpu, adj_case):
+    import torch
+
+    from kurucz_py_joss_submission_amd import synthetic as S
+    from kurucz_py_joss_submission_amd.device import SpectrumPlan
+
+    cols, cat = adj_case
+    _, flux, _ = _model_grads(cols, cat, "fp64")
+    plan = SpectrumPlan(S.build_bundle(cols), cat, WL0, WL1, 300000.0)
+    plan.run()
+    torch.cuda.synchronize()
+    ref = plan.flux.cpu().numpy()
+    assert np.max(np.abs(flux - ref) / np.abs(ref)) < 1e-10
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cutoff", [0.0, 1e-3])
+def test_adjoint_against_fp64_reference_gradient(gpu, oracle, adj_case, cutoff):
+    cols, cat = adj_case
+    m, flux, g = _model_grads(cols, cat, "fp64", cutoff)
+    w = _loss_weights(m.W)
+    loss = lambda c=cols, k=cat, a=None: _oracle_loss(oracle, c, k, w, a, cutoff)  # noqa: E731
+    oracle.set_josh_fp64(True, 1e-13)
+    try:
+        checks = []
+        # log gf and log gamma_vdW of the most sensitive lines (catalog order = SoA order)
+        for key, field, n in (("gf", "gf", 4), ("gw", "gamma_vdw", 3)):
+            for l in np.argsort(-np.abs(g[key]))[:n]:
+                h = 1e-4
+                up = loss(k=_cat_scaled(cat, field, l, 10 ** h))
+                dn = loss(k=_cat_scaled(cat, field, l, 10 ** -h))
+                checks.append((f"{key}[{l}]", g[key][l], (up - dn) / (2 * h), np.max(np.abs(g[key]))))
+        from kurucz_py_joss_submission_amd import synthetic as S
+
+        for z in (26, 28):
+            iz = list(S.ABUNDANCE_Z).index(z)
+            h = 1e-3
+            up = loss(a={z: h})
+            dn = loss(a={z: -h})
+            checks.append((f"A[{z}]", g["A"][iz], (up - dn) / (2 * h), np.max(np.abs(g["A"]))))
+    finally:
+        oracle.set_josh_fp64(False)
+    for name, adj, fd, scale in checks:
+        print(f"cutoff {cutoff:g} {name}: adjoint {adj:.6e} fp64-fd {fd:.6e} "
+              f"rel {abs(adj - fd) / max(abs(fd), 1e-300):.2e}")
+    for name, adj, fd, scale in checks:
+        assert abs(adj - fd) <= 2e-3 * max(abs(fd), 1e-3 * scale), name
+
+
+@pytest.mark.gpu
+def test_fp32_profile_gradient_against_fp64(gpu, adj_case):
+    cols, cat = adj_case
+    _, f64, g64 = _model_grads(cols, cat, "fp64")
+    _, f32, g32 = _model_grads(cols, cat, "fp32")
+    assert np.max(np.abs(f32 - f64) / np.abs(f64)) < 1e-4
+    for k in ("T", "A", "gf", "gw"):
+        scale = np.max(np.abs(g64[k]))
+        err = np.max(np.abs(g32[k] - g64[k])) / scale
+        print(f"{k}: max |g32 - g64| / max |g64| = {err:.2e}")
+        assert err < 1e-4, k
+
+
+def _t_checks(oracle, cols, cat, w, g, layers, h, tol):
+    """Central differences in T (relative step h) against the adjoint."""
+    scale = np.max(np.abs(g["T"]))
+    for d in layers:
+        T0 = cols.temperature[d]
+        up = _oracle_loss(oracle, _cols_with_T(cols, d, 1 + h), cat, w)
+        dn = _oracle_loss(oracle, _cols_with_T(cols, d, 1 - h), cat, w)
+        fd = (up - dn) / (2 * h * T0)
+        print(f"T[{d}] adjoint {g['T'][d]:.6e} fp64-fd({h:g}) {fd:.6e} "
+              f"rel {abs(g['T'][d] - fd) / max(abs(fd), 1e-300):.2e}")
+        assert abs(g["T"][d] - fd) <= tol * abs(fd) + 1e-4 * scale, d
+
+
+# The forward is piecewise smooth in T: the continuum opacity goes as T^6,
+# so a small T change moves the TAUNU nodes across XTAU points and switches
+# MAP1's bracket (physics/josh_solver.py:220-353), and PARCOE/MAP1 blend
+# parabolas with weights |c'| / (|c'| + |c|) that kink where a curvature c
+# changes sign; FASTEX (steps of 1e-3 in E HCKT) and the Voigt tables
+# (steps of 0.005 in x) are step functions.  One-sided slopes at steps of
+# 1e-4 therefore differ by several per cent; the adjoint differentiates the
+# smooth pieces, and central differences over 2e-2 (averaging the kinks and
+# table steps) agree with it to a few per cent.
+
+
+@pytest.mark.gpu
+def test_temperature_gradient_continuum_and_planck(gpu, oracle, adj_case):
+    """No lines: T enters through the continuum, the Planck source and the RT.
+    Checked at the surface and at the layers where the continuum forms, with
+    a 1e-4 step (the shallow layers' sensitivities are small and dominated by
+    the kinks; the deep layers' continuum, ~T^6, is too nonlinear for 2e-2)."""
+    cols, cat = adj_case
+    empty = cat.subset(np.array([], dtype=np.int64))
+    m, flux, g = _model_grads(cols, empty, "fp64")
+    w = _loss_weights(m.W)
+    oracle.set_josh_fp64(True, 1e-13)
+    try:
+        _t_checks(oracle, cols, empty, w, g, (0, 49, 56, 63), 1e-4, 0.05)
+    finally:
+        oracle.set_josh_fp64(False)
+
+
+@pytest.mark.gpu
+def test_temperature_gradient_with_lines(gpu, oracle, adj_case):
+    """All T dependences: populations, Doppler widths, HCKT, TXNXN, STIM,
+    continuum and Planck."""
+    cols, cat = adj_case
+    m, flux, g = _model_grads(cols, cat, "fp64")
+    w = _loss_weights(m.W)
+    oracle.set_josh_fp64(True, 1e-13)
+    try:
+        _t_checks(oracle, cols, cat, w, g, (5, 10, 15, 20, 25, 30, 35, 40), 2e-2, 0.05)
+    finally:
+        oracle.set_josh_fp64(False)
