@@ -224,11 +224,23 @@ class SpectrumModel:
         dl = np.where(dl == 0.0, 1e-20, dl)
         cw = np.stack([(x - h) * (x - right) / dl, (left - x) * (x - right) * 2.0 / dl,
                        (x - left) * (x - h) / dl], axis=1)
-        self.cont_edge = torch.tensor(e, dtype=torch.int64, device="cuda")
         self.cont_w = torch.tensor(cw, dtype=torch.float64, device="cuda")
+        onehot = np.zeros((self.W, S.WLEDGE.size - 1))
+        onehot[np.arange(self.W), e] = 1.0
+        self.cont_onehot = torch.tensor(onehot, dtype=torch.float64, device="cuda")  # pixel -> edge interval
         valid = (soa.process != 0) & (soa.elem >= 0)
-        self.l_ion = torch.tensor(np.where(valid, soa.ion - 1, 0), dtype=torch.int64, device="cuda")
-        self.l_el = torch.tensor(np.where(valid, soa.elem, 0), dtype=torch.int64, device="cuda")
+        ion = np.where(valid, soa.ion - 1, 0)
+        el = np.where(valid, soa.elem, 0)
+        self.l_ion = torch.tensor(ion, dtype=torch.int64, device="cuda")
+        self.l_el = torch.tensor(el, dtype=torch.int64, device="cuda")
+        # species (ion, element) of the lines: one-hot [L][n_species] for the per-species sums
+        key = ion * S.N_ELEMENTS + el
+        self.species, sid = np.unique(key, return_inverse=True)
+        oh = np.zeros((L, self.species.size))
+        oh[np.arange(L), sid] = 1.0
+        self.species_onehot = torch.tensor(oh, dtype=torch.float64, device="cuda")
+        self.species_ion = torch.tensor(self.species // S.N_ELEMENTS, dtype=torch.int64, device="cuda")
+        self.species_el = torch.tensor(self.species % S.N_ELEMENTS, dtype=torch.int64, device="cuda")
 
     def base_parameters(self):
         """(T, dlog_abund, log_gf, log_gvdw) of the unperturbed model."""
@@ -365,7 +377,8 @@ def _make_function():
                 ok = (val > 0.0) & (val < 3.4e38)
                 gl = torch.where(ok, src_g * val * LN10, torch.zeros_like(val))  # dL/d(log10 kappa) per pixel
                 contrib = gl[:, :, None] * m.cont_w[None, :, :]  # [D][W][3]
-                out.index_add_(1, m.cont_edge, contrib)
+                # sum over each edge interval's pixels: [D][3][W] @ [W][E-1]
+                out += torch.matmul(contrib.permute(0, 2, 1), m.cont_onehot).permute(0, 2, 1)
             # line opacity
             g_pop = torch.zeros_like(pop)
             g_dop = torch.zeros_like(dop)
@@ -401,10 +414,11 @@ def _make_function():
                 gdv = -gk / dv - a / dv * s_a + s_dv
                 g_xne = ((m.t["gamma_stark"][:, None] / dv) * s_a).sum(dim=0)
                 g_txnxn = ((gvdw[:, None] / dv) * s_a).sum(dim=0)
-                flat = (torch.arange(D, device="cuda")[None, :] * (S.N_ION * S.N_ELEMENTS)
-                        + (ion * S.N_ELEMENTS + el)[:, None])
-                g_pop.view(-1).index_add_(0, flat.reshape(-1), gpp.reshape(-1))
-                g_dop.view(-1).index_add_(0, flat.reshape(-1), gdv.reshape(-1))
+                # per-species sums over lines: [n_species][L] @ [L][D]
+                sp_pop = torch.matmul(m.species_onehot.T, gpp)
+                sp_dop = torch.matmul(m.species_onehot.T, gdv)
+                g_pop[:, m.species_ion, m.species_el] = sp_pop.T
+                g_dop[:, m.species_ion, m.species_el] = sp_dop.T
             return (g_pop, g_dop, g_rho, g_xne, g_txnxn, g_hckt, g_T, g_cabs, g_cscat, g_lgf, g_lgw, None)
 
     return FluxFunction

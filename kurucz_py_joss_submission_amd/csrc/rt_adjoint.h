@@ -377,10 +377,13 @@ SYNTHE_HD double total_flux64(const Column<kMax>& col, const JoshTab& tb, Work<k
   return f;
 }
 
-// g_abtot, g_alpha, g_snubar [n] = v * dF/d(input); returns F.
+// g_abtot, g_alpha, g_snubar [n] = v * dF/d(input) for the inputs j = part,
+// part + nparts, ... (all of them by default; others are left untouched);
+// returns F.
 template <int kMax>
 SYNTHE_HD double total_flux_vjp(const Column<kMax>& col, const JoshTab& tb, Work<kMax>& w, double v,
-                                double* g_abtot, double* g_alpha, double* g_snubar) {
+                                double* g_abtot, double* g_alpha, double* g_snubar, int part = 0,
+                                int nparts = 1) {
   const int n = col.n;
   int took401 = 0;
   const double F = total_flux64(col, tb, w, 1e-13, &took401);
@@ -388,7 +391,7 @@ SYNTHE_HD double total_flux_vjp(const Column<kMax>& col, const JoshTab& tb, Work
     // forward mode through INTEG and the label-401 iteration, one input at a time
     for (int q = 0; q < 3; ++q) {
       double* g = q == 0 ? g_abtot : (q == 1 ? g_alpha : g_snubar);
-      for (int j = 0; j < n; ++j) {
+      for (int j = part; j < n; j += nparts) {
         for (int i = 0; i < n; ++i) {
           w.d_in[i] = Dual(col.abtot[i], q == 0 && i == j ? 1.0 : 0.0);
           w.d_al[i] = Dual(col.alpha[i], q == 1 && i == j ? 1.0 : 0.0);
@@ -424,7 +427,7 @@ SYNTHE_HD double total_flux_vjp(const Column<kMax>& col, const JoshTab& tb, Work
   if (col.alpha[0] < 0.0 || col.alpha[0] > 1.0) ga0 = 0.0;
   // snubar and alpha enter MAP1 as values on fixed nodes
   for (int i = 0; i < n; ++i) w.d_tau[i] = Dual(w.taunu[i]);
-  for (int j = 0; j < n; ++j) {
+  for (int j = part; j < n; j += nparts) {
     for (int i = 0; i < n; ++i) w.d_f[i] = Dual(col.snubar[i], i == j ? 1.0 : 0.0);
     map1_t(w.d_tau, w.d_f, n, tb.xtau, w.d_out);
     double s = j == 0 ? gs0 : 0.0;
@@ -437,7 +440,7 @@ SYNTHE_HD double total_flux_vjp(const Column<kMax>& col, const JoshTab& tb, Work
     g_alpha[j] = v * s;
   }
   // abtot moves the nodes TAUNU
-  for (int j = 0; j < n; ++j) {
+  for (int j = part; j < n; j += nparts) {
     for (int i = 0; i < n; ++i) w.d_in[i] = Dual(col.abtot[i], i == j ? 1.0 : 0.0);
     integ_t(w.d_in, col.rho, n, w.d_tau, w.d_a, w.d_b, w.d_c);
     for (int i = 0; i < n; ++i) w.d_f[i] = Dual(col.snubar[i]);

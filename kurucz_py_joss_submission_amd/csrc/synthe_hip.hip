@@ -1625,6 +1625,8 @@ __global__ void k_far_finish(int D, const int64_t* __restrict__ regions, const W
 // through engine/radiative.py:42-218 masks): v[w] = dL/dF_nu(w); outputs
 // dL/d(cont_abs, cont_scat, line_abs) [D][W], dL/dB (the Planck function at
 // T(d) used as the continuum source) and dL/d(line_source), 0 on masked depths.
+constexpr int kVjpParts = 8;  // lanes sharing one wavelength's inputs (depth j = part mod 8)
+
 __global__ __launch_bounds__(64) void k_rt_vjp(int D, long W, const double* __restrict__ wl,
                                                const double* __restrict__ temp, const double* __restrict__ rhox,
                                                const double* __restrict__ ca, const double* __restrict__ cs,
@@ -1633,16 +1635,11 @@ __global__ __launch_bounds__(64) void k_rt_vjp(int D, long W, const double* __re
                                                DeviceTables tb, double* __restrict__ g_ca, double* __restrict__ g_cs,
                                                double* __restrict__ g_la, double* __restrict__ g_b,
                                                double* __restrict__ g_src) {
-  const long w = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (w >= W) return;
-  for (int d = 0; d < D; ++d) {
-    const long i = (long)d * W + w;
-    if (g_ca) g_ca[i] = 0.0;
-    if (g_cs) g_cs[i] = 0.0;
-    if (g_la) g_la[i] = 0.0;
-    if (g_b) g_b[i] = 0.0;
-    if (g_src) g_src[i] = 0.0;
-  }
+  // lane -> (part, wavelength); outputs were zeroed by the caller
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= W * kVjpParts) return;
+  const int part = (int)(t / W);
+  const long w = t - (long)part * W;
   const double vw = v[w];
   if (vw == 0.0) return;
   const double maxr4 = 3.4e38;
@@ -1694,8 +1691,8 @@ __global__ __launch_bounds__(64) void k_rt_vjp(int D, long W, const double* __re
   }
   const adj::JoshTab jt{tb.xtau, tb.ch, tb.coefj32};
   double gab[kAdjMaxDepth], gal[kAdjMaxDepth], gsb[kAdjMaxDepth];
-  adj::total_flux_vjp(col, jt, wk, vw, gab, gal, gsb);
-  for (int j = 0; j < n; ++j) {
+  adj::total_flux_vjp(col, jt, wk, vw, gab, gal, gsb, part, kVjpParts);
+  for (int j = part; j < n; j += kVjpParts) {
     const int d = idx[j];
     const long i = (long)d * W + w;
     const double ac = ca[i], al = la[i];
@@ -2600,7 +2597,14 @@ int synthe_lte_vjp(synthe_ctx* ctx, long n_depths, long n_wl, const double* wl, 
   if (n_wl == 0) return SYNTHE_OK;
   if (!wl || !temperature || !rhox || !cont_abs || !cont_scat || !line_abs || !line_scat || !v_flux)
     return fail(SYNTHE_EINVAL, "synthe_lte_vjp: NULL input");
-  hipLaunchKernelGGL(k_rt_vjp, dim3(blocks_for(n_wl, 64)), dim3(64), 0, (hipStream_t)stream, (int)n_depths, n_wl, wl,
+  const long dw = n_depths * n_wl;
+  for (double* z : {g_cont_abs, g_cont_scat, g_line_abs, g_planck, g_line_source}) {
+    if (!z) continue;
+    hipLaunchKernelGGL(k_fill_u64, dim3(blocks_for(dw, kBlock)), dim3(kBlock), 0, (hipStream_t)stream, dw,
+                       (unsigned long long*)z);
+    HIP_TRY(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_rt_vjp, dim3(blocks_for(n_wl * kVjpParts, 64)), dim3(64), 0, (hipStream_t)stream, (int)n_depths, n_wl, wl,
                      temperature, rhox, cont_abs, cont_scat, line_abs, line_scat, line_source, v_flux, ctx->tb,
                      g_cont_abs, g_cont_scat, g_line_abs, g_planck, g_line_source);
   HIP_TRY(hipGetLastError());
