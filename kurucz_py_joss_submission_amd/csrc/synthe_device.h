@@ -159,6 +159,41 @@ __device__ __forceinline__ double div_single(double x, double b) {
   return x * __builtin_amdgcn_rcp(b);
 }
 
+// out[p] = x / (kd + p0 + p)^2 for p < C with one reciprocal for the C
+// quotients (Montgomery's simultaneous inversion).  With q_p = (kd + p0 + p)^2
+// and running products P_p = q_0 ... q_p, y = x / P_{C-1}; then for p = C-1
+// down to 1, x / q_p = y * P_{p-1} and y <- y * q_p = x / P_{p-1}; finally
+// x / q_0 = y.  The rounding of each P_p cancels in P_{p-1} / P_p, so a
+// quotient carries the reciprocal's error plus two roundings per later pixel.
+// kNewton = 2 gives the reciprocal to within an ulp (x / q within 16 ulp for
+// C = 8), kNewton = 0 leaves the hardware's ~2^-22.  Every k must satisfy
+// 1 <= |k| < 2^31 (so P_{C-1} <= 2^496 for C <= 8) and x must lie in
+// [2^-500, 2^1023]: then no product leaves the normal range.
+template <int C, int kNewton>
+__device__ __forceinline__ void far_chunk(double x, double kd, int p0, double* __restrict__ out) {
+  static_assert(C >= 1 && C <= 8, "chunk of 1..8 pixels");
+  double q[C], P[C];
+#pragma unroll
+  for (int p = 0; p < C; ++p) {
+    const double k = kd + (double)(p0 + p);
+    q[p] = k * k;
+    P[p] = p == 0 ? q[0] : P[p - 1] * q[p];
+  }
+  double r = __builtin_amdgcn_rcp(P[C - 1]);
+#pragma unroll
+  for (int i = 0; i < kNewton; ++i) r = __builtin_fma(r, __builtin_fma(-P[C - 1], r, 1.0), r);
+  double y = x * r;
+#pragma unroll
+  for (int p = C - 1; p > 0; --p) {
+    out[p] = y * P[p - 1];
+    y *= q[p];
+  }
+  out[0] = y;
+}
+
+// Smallest X for which far_chunk's products stay normal.
+constexpr double kChunkMinX = 0x1p-500;
+
 __device__ __forceinline__ double wing_value(const WingParams& p, long k, const double* h0t,
                                              const double* h1t, const double* h2t) {
   if (p.use_far && k > p.n10dop) return div_pos(p.x_far, (double)k * (double)k);

@@ -52,10 +52,13 @@ constexpr int kClassShift = 1;
 constexpr int kClasses = 17;
 
 // Per list entry, what every candidate test and the far-wing path read:
-// centre index, MAXSTEP, N10DOP and the far-wing flag (16 bytes, one load).
+// centre index, MAXSTEP, N10DOP and the far-wing flags (16 bytes, one load).
+// use_far is nonzero for a pair with a far wing; kFarChunk is set besides
+// when its X lies in far_chunk's range.
 struct __align__(16) WingHead {
   int32_t c, maxstep, n10dop, use_far;
 };
+constexpr int32_t kFarChunk = 2;
 constexpr int kBlock = 256;
 
 // ---------------------------------------------------------------- host tables
@@ -479,7 +482,8 @@ __global__ __launch_bounds__(kBlock) void k_class_scatter(
       int64_t pos = regions[d * (kClasses + 1) + k] +
                     offsets[((long)blockIdx.x * D + d) * kClasses + k] + before + rank;
       WingParams p = wp[l * D + d];
-      list_cm[pos] = WingHead{p.c, p.maxstep, p.n10dop, p.use_far};
+      const bool chunk = p.x_far >= kChunkMinX && p.x_far <= 0x1p1023;
+      list_cm[pos] = WingHead{p.c, p.maxstep, p.n10dop, p.use_far ? 1 | (chunk ? kFarChunk : 0) : 0};
       list_p[pos] = p;
       if (list_l) list_l[pos] = (int32_t)l;
     }
@@ -626,8 +630,8 @@ struct GatherArgs {
 };
 
 // Profile precision of the far-wing term X / k^2 (accumulation is FP64 in
-// both): kPrecFP64 within 32 ulp of the IEEE quotient, kPrecFP32 at single
-// precision.
+// both): kPrecFP64 within 32 ulp of the IEEE quotient (2^-1074 absolute where
+// the quotient is subnormal), kPrecFP32 at single precision.
 constexpr int kPrecFP64 = 0;
 constexpr int kPrecFP32 = 1;
 
@@ -636,6 +640,10 @@ __device__ __forceinline__ double far_quotient(double x, double b) {
   if constexpr (kPrec == kPrecFP32) return div_single(x, b);
   else return div_fast(x, b);
 }
+
+#ifndef SYNTHE_FAR_CHUNK
+#define SYNTHE_FAR_CHUNK 8  // far-wing quotients per reciprocal (1: one each)
+#endif
 
 #ifndef SYNTHE_GATHER_WAVES
 #define SYNTHE_GATHER_WAVES 7  // waves per SIMD the register budget is sized for
@@ -648,8 +656,9 @@ __device__ __forceinline__ double far_quotient(double x, double b) {
 // (line_opacity.py:1338).  One wave per (512-pixel tile, depth), 8 consecutive
 // pixels per lane.  Entries are read with wave-uniform loads and classified
 // against the whole tile with scalar branches: where the tile lies wholly in
-// one far wing every lane adds X / k^2 branch-free; on the tiles holding a
-// wing's N10DOP or MAXSTEP edge, pixels outside the far zone add +0.  Each
+// one far wing every lane adds X / k^2 branch-free, its 8 pixels sharing one
+// reciprocal (far_chunk); on the tiles holding a wing's N10DOP or MAXSTEP
+// edge, pixels outside the far zone add +0.  Each
 // pixel's sum has a fixed order (Voigt zones, centres, far wings by class and
 // centre), no atomics, so results are run-to-run identical.
 template <bool kStats, int kPrec, int kWP>
@@ -687,69 +696,76 @@ __device__ __forceinline__ void gather_tile(const GatherArgs& g, int d, long t0,
       }
     }
   }
-  const long t1 = t0 + kWT - 1;
-  const long bound = class_window_search(lane, d, t0, t1, 1L << 40, regions, list_cm);
-  const long kd0 = w0;  // lane's first pixel
+  // Pixel indices, centres and offsets fit in 32 bits (W <= 2^30, centres
+  // within 10^6 of the grid), so the per-entry tests are scalar 32-bit ones.
+  const int t0i = (int)t0, t1i = (int)t0 + kWT - 1;
+  const long bound = class_window_search(lane, d, t0, t0 + kWT - 1, 1L << 40, regions, list_cm);
+  const int kd0 = (int)w0;  // lane's first pixel
+  constexpr int kChunk = SYNTHE_FAR_CHUNK < kWP ? SYNTHE_FAR_CHUNK : kWP;
+  static_assert(kWP % kChunk == 0, "pixels per lane must be a multiple of the chunk");
   for (int kk = 0; kk < kClasses; ++kk) {
     const long lo = uniform_long(__shfl(bound, 2 * kk));
     const long hi = uniform_long(__shfl(bound, 2 * kk + 1));
     if (lo >= hi) continue;
+    const WingHead* __restrict__ hp = list_cm + lo;
+    const WingParams* __restrict__ pp = list_p + lo;
+    const int n = (int)(hi - lo);  // entries of one depth: fewer than the lines
     // software pipeline: the next entry's head and X are loaded while this
     // entry is evaluated
-    WingHead h = list_cm[lo];
-    double xh = list_p[lo].x_far;
-    for (long j = lo; j < hi; ++j) {
+    WingHead h = hp[0];
+    double xh = pp[0].x_far;
+    for (int j = 0; j < n; ++j) {
       const WingHead cur = h;
       const double x = xh;
-      if (j + 1 < hi) {
-        h = list_cm[j + 1];
-        xh = list_p[j + 1].x_far;
+      if (j + 1 < n) {
+        h = hp[j + 1];
+        xh = pp[j + 1].x_far;
       }
-      const long c = cur.c;
-      const long M = cur.maxstep;
-      const long a = t0 - c, b = t1 - c;  // tile offsets from the centre
+      const int c = cur.c, M = cur.maxstep, n10 = cur.n10dop;
+      const int a = t0i - c, b = t1i - c;  // tile offsets from the centre
       if (!cur.use_far || b < -M || a > M) continue;  // no far wing here
-      const long n10 = cur.n10dop;
-      double kd = (double)(int)(kd0 - c);
+      double kd = (double)(kd0 - c);
       __asm__("" : "+v"(kd));  // keep k = kd + p in double (no per-pixel int conversion)
       const bool interior = (a > n10 && b <= M) || (b < -n10 && a >= -M);
-      const bool xok = x >= 0x1p-900 && x <= 0x1p900;
-      if (interior && xok) {
+      double add[kWP];  // both paths end in one accumulation
+      if (interior && (kChunk == 1 || (cur.use_far & kFarChunk))) {
+        // |k| >= N10DOP + 1 >= 1 on every pixel of the tile; k * k is exact
+        if constexpr (kChunk == 1) {
 #pragma unroll
-        for (int p = 0; p < kWP; ++p) {
-          const double k = kd + (double)p;  // |k| is the offset; k * k is exact
-          acc[p] += far_quotient<kPrec>(x, k * k);
+          for (int p = 0; p < kWP; ++p) {
+            const double k = kd + (double)p;
+            add[p] = far_quotient<kPrec>(x, k * k);
+          }
+        } else {
+#pragma unroll
+          for (int p0 = 0; p0 < kWP; p0 += kChunk)
+            far_chunk<kChunk, kPrec == kPrecFP32 ? 0 : 2>(x, kd, p0, add + p0);
         }
         if (kStats) {
           long n = W - w0 < kWP ? W - w0 : kWP;
           nev += n > 0 ? n : 0;
         }
       } else {
-        const double n10d = (double)(int)n10, md = (double)(int)M;
-        if (xok) {
+        // N10DOP < |k| <= MAXSTEP tested on k^2 (exact integers below 2^53)
+        const double n10q = (double)n10 * (double)n10, mq = (double)M * (double)M;
 #pragma unroll
-          for (int p = 0; p < kWP; ++p) {
-            const double k = kd + (double)p;
-            const double ka = fabs(k);
-            const double xp = (ka > n10d && ka <= md) ? x : 0.0;
-            // 0 / max(k^2, 1) adds +0, also at the centre pixel (k = 0)
-            acc[p] += far_quotient<kPrec>(xp, fmax(k * k, 1.0));
-          }
-        } else {
-          for (int p = 0; p < kWP; ++p) {
-            const double k = kd + (double)p;
-            const double ka = fabs(k);
-            if (ka > n10d && ka <= md) acc[p] += x / (k * k);
-          }
+        for (int p = 0; p < kWP; ++p) {
+          const double k = kd + (double)p;
+          const double q = k * k;
+          const double xp = (q > n10q && q <= mq) ? x : 0.0;
+          // 0 / max(k^2, 1) adds +0, also at the centre pixel (k = 0)
+          add[p] = far_quotient<kPrec>(xp, fmax(q, 1.0));
         }
         if (kStats) {
           for (int p = 0; p < kWP; ++p) {
-            long ka = kd0 - c + p;
+            int ka = kd0 - c + p;
             ka = ka < 0 ? -ka : ka;
             if (ka > n10 && ka <= M && w0 + p < W) ++nev;
           }
         }
       }
+#pragma unroll
+      for (int p = 0; p < kWP; ++p) acc[p] += add[p];
     }
   }
   double T = temp[d] > 1.0 ? temp[d] : 1.0;
@@ -770,7 +786,7 @@ __device__ __forceinline__ void gather_tile(const GatherArgs& g, int d, long t0,
 
 
 template <bool kStats, int kPrec>
-__global__ __launch_bounds__(64, SYNTHE_GATHER_WAVES) void k_gather(GatherArgs g,
+__global__ __launch_bounds__(64, kStats ? 6 : SYNTHE_GATHER_WAVES) void k_gather(GatherArgs g,
                                                                     unsigned long long* __restrict__ evals) {
   gather_tile<kStats, kPrec, kWP>(g, blockIdx.y, blockIdx.x * (long)kWT, threadIdx.x, evals);
 }
@@ -780,7 +796,7 @@ __global__ __launch_bounds__(64, SYNTHE_GATHER_WAVES) void k_gather(GatherArgs g
 // instance order[i]'s items, depth-major; every item writes only its own
 // tile's pixels, so the result does not depend on which wave takes it.
 template <int kPrec, int kWPt>
-__global__ __launch_bounds__(64, kWPt == 16 ? 5 : 8) void k_gather_batch(const GatherArgs* __restrict__ inst,
+__global__ __launch_bounds__(64, kWPt == 16 ? 4 : (kWPt == 8 ? 6 : 7)) void k_gather_batch(const GatherArgs* __restrict__ inst,
                                                      const int32_t* __restrict__ order,
                                                      const int64_t* __restrict__ items, int n_inst,
                                                      unsigned long long* __restrict__ counter) {
@@ -805,6 +821,28 @@ __global__ __launch_bounds__(64, kWPt == 16 ? 5 : 8) void k_gather_batch(const G
     const long t = local - (long)d * ntiles;
     gather_tile<false, kPrec, kWPt>(g, d, t * kTile, lane, nullptr);
   }
+}
+
+// The same items with one workgroup each (item = blockIdx.x), in the queue's
+// order: no loop around the tile, so nothing of it stays live between items.
+template <int kPrec, int kWPt>
+__global__ __launch_bounds__(64, kWPt == 16 ? 4 : (kWPt == 8 ? SYNTHE_GATHER_WAVES : 8)) void k_gather_items(
+    const GatherArgs* __restrict__ inst, const int32_t* __restrict__ order, const int64_t* __restrict__ items,
+    int n_inst) {
+  constexpr long kTile = 64L * kWPt;
+  const long item = blockIdx.x;
+  int lo = 0, hi = n_inst - 1;  // last i with items[i] <= item
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (items[mid] <= item) lo = mid;
+    else hi = mid - 1;
+  }
+  const GatherArgs& g = inst[order[lo]];
+  const long local = item - items[lo];
+  const long ntiles = (g.W + kTile - 1) / kTile;
+  const int d = (int)(local / ntiles);
+  const long t = local - (long)d * ntiles;
+  gather_tile<false, kPrec, kWPt>(g, d, t * kTile, threadIdx.x, nullptr);
 }
 
 // Wing statistics over the lists: pairs, sum and max of MAXSTEP.
@@ -1219,6 +1257,19 @@ __global__ void k_check_div(long n, int variant, const double* x, const double* 
   long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (i >= n) return;
   double xv = x[i], bv = b[i], q;
+  if (variant == 5 || variant == 6) {
+    // b holds k; each aligned group of 8 holds consecutive k (far_chunk)
+    const int p = (int)(i & 7);
+    double out[8];
+    if (variant == 5) far_chunk<8, 2>(xv, bv - (double)p, 0, out);
+    else far_chunk<8, 0>(xv, bv - (double)p, 0, out);
+    q = out[0];
+#pragma unroll
+    for (int j = 1; j < 8; ++j) q = j == p ? out[j] : q;
+    q_fast[i] = q;
+    q_ieee[i] = xv / (bv * bv);
+    return;
+  }
   if (variant == 0) {
     q = div_pos(xv, bv);
   } else if (variant == 3) {
@@ -1780,6 +1831,8 @@ static int check_asynth_args(const synthe_lines* lines, const synthe_depths* dep
   if (rc) return rc;
   if (n_wl <= 0 || !wl || !asynth || !center_start)
     return fail(SYNTHE_EINVAL, std::string(who) + ": wl/asynth/center_start required and n_wl > 0");
+  if (n_wl > (1L << 30))
+    return fail(SYNTHE_EINVAL, std::string(who) + ": n_wl above 2^30 pixels");
   if (lines->n_lines > 0 && (!transp || !valid || !kapmin || !center_order || !wing_order))
     return fail(SYNTHE_EINVAL, std::string(who) + ": NULL transp/valid/kapmin/order array");
   return SYNTHE_OK;
@@ -2185,7 +2238,8 @@ struct BatchInst {
 
 constexpr int kBatchStreams = 4;
 constexpr int kTilePixels[3] = {4, 8, 16};  // pixels per lane of the gather candidates
-constexpr int kWavesPerSimd[3] = {2, 4, 8};
+// persistent grids of the gather candidates; 0: one workgroup per item
+constexpr int kWavesPerSimd[4] = {2, 4, 8, 0};
 
 }  // namespace
 
@@ -2208,7 +2262,8 @@ struct synthe_batch {
   int gather_blocks = 4096;
   int n_cu = 256;
   int precision = SYNTHE_PREC_FP64;
-  std::vector<double> tune_ms;  // [3 tiles][3 wave counts], ms, or empty
+  long n_items = 0;              // gather items of the current tile
+  std::vector<double> tune_ms;  // [3 tiles][4 grids], ms, or empty
   hipStream_t streams[kBatchStreams] = {};
   hipEvent_t ev_fork = nullptr, ev_mid = nullptr;
   hipEvent_t ev_join[kBatchStreams] = {};
@@ -2228,18 +2283,29 @@ int batch_items(synthe_batch* b, int kwp) {
     long nt = (in.n_wl + tile - 1) / tile;
     items[i + 1] = items[i] + nt * in.depths.n_depths;
   }
+  if (items.back() > INT32_MAX) return fail(SYNTHE_EINVAL, "synthe_batch_create: too many gather tiles");
+  b->n_items = items.back();
   HIP_TRY(hipMemcpy(b->d_items, items.data(), items.size() * sizeof(int64_t), hipMemcpyHostToDevice));
   return SYNTHE_OK;
 }
 
+// blocks > 0: persistent grid of that many workgroups; 0: one per item.
 int launch_gather_batch(synthe_batch* b, int kwp, int blocks, hipStream_t st) {
-  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, 1L, b->d_counter);
-  HIP_TRY(hipGetLastError());
   const int n = (int)b->inst.size();
   const bool f32 = b->precision == SYNTHE_PREC_FP32;
-#define SYNTHE_GB(P, K)                                                                                     \
-  hipLaunchKernelGGL((k_gather_batch<P, K>), dim3(blocks), dim3(64), 0, st, b->d_args, b->d_order, b->d_items, \
-                     n, b->d_counter)
+  if (blocks > 0) {
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, 1L, b->d_counter);
+    HIP_TRY(hipGetLastError());
+  } else if (b->n_items == 0) {
+    return SYNTHE_OK;
+  }
+#define SYNTHE_GB(P, K)                                                                                       \
+  if (blocks > 0)                                                                                             \
+    hipLaunchKernelGGL((k_gather_batch<P, K>), dim3(blocks), dim3(64), 0, st, b->d_args, b->d_order,         \
+                       b->d_items, n, b->d_counter);                                                         \
+  else                                                                                                        \
+    hipLaunchKernelGGL((k_gather_items<P, K>), dim3((unsigned)b->n_items), dim3(64), 0, st, b->d_args,       \
+                       b->d_order, b->d_items, n)
   if (kwp == 4) { if (f32) SYNTHE_GB(kPrecFP32, 4); else SYNTHE_GB(kPrecFP64, 4); }
   else if (kwp == 16) { if (f32) SYNTHE_GB(kPrecFP32, 16); else SYNTHE_GB(kPrecFP64, 16); }
   else { if (f32) SYNTHE_GB(kPrecFP32, 8); else SYNTHE_GB(kPrecFP64, 8); }
@@ -2444,7 +2510,7 @@ int synthe_batch_create(synthe_ctx* ctx, int n, const synthe_instance* inst, int
     double best = 1e300;
     for (int a = 0; a < 3 && !rc; ++a) {
       if ((rc = batch_items(b, kTilePixels[a]))) break;
-      for (int c = 0; c < 3 && !rc; ++c) {
+      for (int c = 0; c < 4 && !rc; ++c) {
         const int blocks = b->n_cu * 4 * kWavesPerSimd[c];
         float ms = 0.f, best_c = 1e30f;
         for (int rep = 0; rep < 2 && !rc; ++rep) {

@@ -191,6 +191,42 @@ def test_far_wing_fast_quotient_ulps(gpu):
     assert ulps[4] <= 32
 
 
+def test_far_wing_shared_reciprocal_ulps(gpu):
+    """The interior tiles' form: 8 consecutive k share one reciprocal of the
+    product of their k^2 (running products, reciprocal, then one product per
+    pixel back).  With two Newton steps every X / k^2 is within 16 ulp of the
+    IEEE quotient; without them, within 2^-20 relative.  Operands: |k| from 1
+    to 10^6 of either sign, X down to 2^-500 (the form's lower limit)."""
+    import torch
+    from kurucz_py_joss_submission_amd import _native as N
+
+    rng = np.random.default_rng(13)
+    groups = 1 << 18
+    m = rng.integers(1, 1_000_000 - 8, groups)
+    m[:64] = 1  # k = 1..8 and -8..-1
+    sign = np.where(rng.random(groups) < 0.5, 1, -1)
+    sign[:32] = 1
+    base = np.where(sign > 0, m, -(m + 7))
+    k = (base[:, None] + np.arange(8)[None, :]).reshape(-1).astype(np.float64)
+    n = k.size
+    x = 10.0 ** rng.uniform(-150, 300, n)
+    x[:4096] = 2.0 ** rng.uniform(-500, -490, 4096)
+    dx, dk = torch.from_numpy(x).cuda(), torch.from_numpy(k).cuda()
+    err = {}
+    for variant in (5, 6):
+        qf = torch.empty_like(dx)
+        qi = torch.empty_like(dx)
+        N.check("synthe_diag_div_pos", N.load_library().synthe_diag_div_pos(
+            n, variant, dx.data_ptr(), dk.data_ptr(), qf.data_ptr(), qi.data_ptr(), N.stream_handle()))
+        qf, qi = qf.cpu().numpy(), qi.cpu().numpy()
+        assert np.all(np.isfinite(qf)) and np.all(qi > 0)
+        err[variant] = (float(np.max(np.abs(qf - qi) / np.spacing(qi))),
+                        float(np.max(np.abs(qf - qi) / qi)))
+    print("shared reciprocal: max ulp (2 Newton)", err[5][0], "max rel (none)", err[6][1])
+    assert err[5][0] <= 16
+    assert err[6][1] <= 2.0 ** -20
+
+
 def test_fp32_profiles_flux_error(gpu, oracle, c1_workload):
     """Single-precision far-wing terms (FP64 accumulation, populations and RT):
     the same evaluation set, and flux within 1e-4 of the FP64 oracle under
