@@ -186,7 +186,7 @@ int synthe_flux_to_lambda(synthe_ctx *ctx, long n_wl, const double *wl, double *
  * can compare them.  variant 0 = the correctly rounded reciprocal sequence;
  * 1, 2 = it with one or no Newton step before the remainder correction;
  * 3 = X * rcp(b); 4 = X * (rcp(b) after one Newton step), the wing gather's
- * edge tiles; 5, 6 = the shared-reciprocal form of its interior tiles with two
+ * edge tiles; 5, 6 = the shared-reciprocal form of its interior tiles with one
  * or no Newton steps, where b holds k (q = X / k^2) and each aligned group of
  * 8 inputs holds consecutive k of one sign, |k| >= 1. */
 int synthe_diag_div_pos(long n, int variant, const double *x, const double *b, double *q_fast,
@@ -228,11 +228,11 @@ typedef struct synthe_batch synthe_batch;
 int synthe_batch_create(synthe_ctx *ctx, int n, const synthe_instance *inst, int flags, synthe_batch **out);
 /* Run the batch on stream (graph replay when created with SYNTHE_BATCH_GRAPH). */
 int synthe_batch_run(synthe_batch *b, synthe_stream stream);
-/* The gather's chosen pixels per lane (tile = 64 x that) and persistent grid
- * (workgroups of 64; 0 = one workgroup per tile and depth), the autotuner's
- * candidate times (ms, tile-major: 4, 8, 16 pixels x persistent grids of 2,
- * 4, 8 waves per SIMD and one workgroup per tile; up to tune_cap) and the
- * arena size.  Any pointer may be NULL. */
+/* The gather's pixels per lane (tile = 64 x that, as synthe_asynth's) and
+ * chosen grid (persistent, in workgroups of 64; 0 = one workgroup per tile and
+ * depth), the autotuner's candidate times (ms: persistent grids of 2, 4, 8
+ * waves per SIMD, then one workgroup per tile; up to tune_cap) and the arena
+ * size.  Any pointer may be NULL. */
 int synthe_batch_info(const synthe_batch *b, int *pixels_per_lane, int *gather_blocks, double *tune_ms,
                       int tune_cap, int *n_tuned, double *arena_bytes);
 int synthe_batch_destroy(synthe_batch *b);

@@ -166,16 +166,48 @@ __device__ __forceinline__ double div_single(double x, double b) {
 // x / q_0 = y.  The rounding of each P_p cancels in P_{p-1} / P_p, so a
 // quotient carries the reciprocal's error plus two roundings per later pixel.
 // kNewton = 2 gives the reciprocal to within an ulp (x / q within 16 ulp for
-// C = 8), kNewton = 0 leaves the hardware's ~2^-22.  Every k must satisfy
-// 1 <= |k| < 2^31 (so P_{C-1} <= 2^496 for C <= 8) and x must lie in
-// [2^-500, 2^1023]: then no product leaves the normal range.
+// C = 8), kNewton = 1 squares the hardware reciprocal's ~2^-23 error (x / q
+// within 32 ulp, as div_fast), kNewton = 0 leaves it (~2^-22).  kMask:
+// out[p] = +0 unless lo_k < |k| <= hi_k, and q_p = max(k^2, 1) so the centre
+// pixel k = 0 is harmless.  Every |k| must be < 2^31, |k| >= 1 unless kMask (so
+// P_{C-1} <= 2^496 for C <= 8), and x must be 0 or lie in [2^-500, 2^1023]
+// (or be infinite or NaN, which propagate): then no product leaves the normal
+// range.
+template <int C, int kNewton, bool kMask>
+__device__ __forceinline__ void far_chunk(double x, double kd, int p0, double lo_k, double hi_k,
+                                          double* __restrict__ out) {
+  static_assert(C >= 1 && C <= 8, "chunk of 1..8 pixels");
+  double q[C], P[C];
+  bool keep[C];
+#pragma unroll
+  for (int p = 0; p < C; ++p) {
+    const double k = p0 + p == 0 ? kd : kd + (double)(p0 + p);
+    const double kq = k * k;  // exact: integers below 2^62
+    keep[p] = !kMask || (fabs(k) > lo_k && fabs(k) <= hi_k);
+    q[p] = kMask ? fmax(kq, 1.0) : kq;
+    P[p] = p == 0 ? q[0] : P[p - 1] * q[p];
+  }
+  double r = __builtin_amdgcn_rcp(P[C - 1]);
+#pragma unroll
+  for (int i = 0; i < kNewton; ++i) r = __builtin_fma(r, __builtin_fma(-P[C - 1], r, 1.0), r);
+  double y = x * r;
+#pragma unroll
+  for (int p = C - 1; p > 0; --p) {
+    const double v = y * P[p - 1];
+    out[p] = keep[p] ? v : 0.0;
+    y *= q[p];
+  }
+  out[0] = keep[0] ? y : 0.0;
+}
+
+// far_chunk without a mask, each quotient added to acc[p] with one rounding.
 template <int C, int kNewton>
-__device__ __forceinline__ void far_chunk(double x, double kd, int p0, double* __restrict__ out) {
+__device__ __forceinline__ void far_chunk_acc(double x, double kd, int p0, double* __restrict__ acc) {
   static_assert(C >= 1 && C <= 8, "chunk of 1..8 pixels");
   double q[C], P[C];
 #pragma unroll
   for (int p = 0; p < C; ++p) {
-    const double k = kd + (double)(p0 + p);
+    const double k = p0 + p == 0 ? kd : kd + (double)(p0 + p);
     q[p] = k * k;
     P[p] = p == 0 ? q[0] : P[p - 1] * q[p];
   }
@@ -185,11 +217,16 @@ __device__ __forceinline__ void far_chunk(double x, double kd, int p0, double* _
   double y = x * r;
 #pragma unroll
   for (int p = C - 1; p > 0; --p) {
-    out[p] = y * P[p - 1];
+    acc[p] = __builtin_fma(y, P[p - 1], acc[p]);
     y *= q[p];
   }
-  out[0] = y;
+  acc[0] += y;
 }
+
+// Newton steps on the shared reciprocal of the FP64 far-wing form: one step
+// squares the hardware reciprocal's ~2^-23 error, which leaves each quotient
+// within 32 ulp of IEEE division (as div_fast).
+constexpr int kFarNewton = 1;
 
 // Smallest X for which far_chunk's products stay normal.
 constexpr double kChunkMinX = 0x1p-500;

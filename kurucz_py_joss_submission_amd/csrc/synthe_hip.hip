@@ -51,14 +51,22 @@ int fail(int code, const std::string& msg) {
 constexpr int kClassShift = 1;
 constexpr int kClasses = 17;
 
-// Per list entry, what every candidate test and the far-wing path read:
-// centre index, MAXSTEP, N10DOP and the far-wing flags (16 bytes, one load).
-// use_far is nonzero for a pair with a far wing; kFarChunk is set besides
-// when its X lies in far_chunk's range.
-struct __align__(16) WingHead {
-  int32_t c, maxstep, n10dop, use_far;
+// Per list entry, what every candidate test and the far-wing path read (32
+// bytes, one scalar load):
+//   lo, hi    pixels c - MAXSTEP and c + MAXSTEP of a far wing; lo > hi (an
+//             empty range) for a pair without one;
+//   nlo, nhi  the Voigt zone c - r .. c + r, r = N10DOP with a far wing and
+//             MAXSTEP without;
+//   xs        the far wing's X, negated when it lies below far_chunk's range
+//             (those wings take one quotient per pixel);
+//   cd        the centre c as a double.
+// Pixel indices fit in 32 bits: n_wl <= 2^30 and centres within 10^6 pixels
+// of the grid.
+struct __align__(32) WingHead {
+  int32_t lo, hi, nlo, nhi;
+  double xs, cd;
 };
-constexpr int32_t kFarChunk = 2;
+__device__ __forceinline__ int head_c(const WingHead& h) { return (int)h.cd; }
 constexpr int kBlock = 256;
 
 // ---------------------------------------------------------------- host tables
@@ -482,8 +490,16 @@ __global__ __launch_bounds__(kBlock) void k_class_scatter(
       int64_t pos = regions[d * (kClasses + 1) + k] +
                     offsets[((long)blockIdx.x * D + d) * kClasses + k] + before + rank;
       WingParams p = wp[l * D + d];
-      const bool chunk = p.x_far >= kChunkMinX && p.x_far <= 0x1p1023;
-      list_cm[pos] = WingHead{p.c, p.maxstep, p.n10dop, p.use_far ? 1 | (chunk ? kFarChunk : 0) : 0};
+      const int r = p.use_far ? p.n10dop : p.maxstep;
+      const bool chunk = !(p.x_far > 0.0 && p.x_far < kChunkMinX);  // X >= 0
+      WingHead h;
+      h.lo = p.use_far ? p.c - p.maxstep : INT32_MAX;
+      h.hi = p.use_far ? p.c + p.maxstep : INT32_MIN;
+      h.nlo = p.c - r;
+      h.nhi = p.c + r;
+      h.xs = p.use_far ? (chunk ? p.x_far : -p.x_far) : 0.0;
+      h.cd = (double)p.c;
+      list_cm[pos] = h;
       list_p[pos] = p;
       if (list_l) list_l[pos] = (int32_t)l;
     }
@@ -503,7 +519,7 @@ __global__ __launch_bounds__(kBlock) void k_near_reach(int D, const int64_t* __r
   int m = 0;
   for (long j = beg + blockIdx.x * (long)kBlock + threadIdx.x; j < end; j += (long)gridDim.x * kBlock) {
     const WingHead h = list_cm[j];
-    const int r = h.use_far ? h.n10dop : h.maxstep;
+    const int r = h.nhi - head_c(h);
     m = r > m ? r : m;
   }
   for (int off = 32; off > 0; off >>= 1) {
@@ -551,7 +567,7 @@ __device__ __forceinline__ long class_window_search(int lane, int d, long lo_px,
     long key = upper ? hi_px + r : lo_px - r;
     while (lo < hi) {  // first entry with c > key (upper) or c >= key (lower)
       long mid = (lo + hi) >> 1;
-      long c = list_cm[mid].c;
+      long c = head_c(list_cm[mid]);
       if (upper ? c <= key : c < key) lo = mid + 1;
       else hi = mid;
     }
@@ -586,9 +602,9 @@ __global__ __launch_bounds__(256) void k_near(long W, int D, const WingHead* __r
     const long hi = uniform_long(__shfl(bound, 2 * kk + 1));
     for (long j = lo; j < hi; ++j) {
       const WingHead h = list_cm[j];
-      const long c = h.c;
-      const long lim = h.use_far ? h.n10dop : h.maxstep;
-      if (v0 + 63 < c - lim || v0 > c + lim) continue;  // wave-uniform
+      if (v0 + 63 < h.nlo || v0 > h.nhi) continue;  // wave-uniform
+      const long c = head_c(h);
+      const long lim = h.nhi - c;
       const long k = w - c;
       const long ka = k < 0 ? -k : k;
       if (ka >= 1 && ka <= lim && w < W) {
@@ -641,12 +657,8 @@ __device__ __forceinline__ double far_quotient(double x, double b) {
   else return div_fast(x, b);
 }
 
-#ifndef SYNTHE_FAR_CHUNK
-#define SYNTHE_FAR_CHUNK 8  // far-wing quotients per reciprocal (1: one each)
-#endif
-
 #ifndef SYNTHE_GATHER_WAVES
-#define SYNTHE_GATHER_WAVES 7  // waves per SIMD the register budget is sized for
+#define SYNTHE_GATHER_WAVES 6  // waves per SIMD the register budget is sized for
 #endif
 
 // Continues each pixel's sum from the Voigt-zone sums of k_near (in asynth),
@@ -700,74 +712,71 @@ __device__ __forceinline__ void gather_tile(const GatherArgs& g, int d, long t0,
   // within 10^6 of the grid), so the per-entry tests are scalar 32-bit ones.
   const int t0i = (int)t0, t1i = (int)t0 + kWT - 1;
   const long bound = class_window_search(lane, d, t0, t0 + kWT - 1, 1L << 40, regions, list_cm);
-  const int kd0 = (int)w0;  // lane's first pixel
-  constexpr int kChunk = SYNTHE_FAR_CHUNK < kWP ? SYNTHE_FAR_CHUNK : kWP;
-  static_assert(kWP % kChunk == 0, "pixels per lane must be a multiple of the chunk");
+  const double kdl = (double)(int)w0;  // lane's first pixel
+  // pixels per shared reciprocal: the largest divisor of kWP up to 8 (4 on
+  // edge tiles, which need more registers)
+  constexpr int kChunk = kWP % 8 == 0 ? 8 : (kWP % 6 == 0 ? 6 : (kWP % 4 == 0 ? 4 : 1));
+#ifndef SYNTHE_EDGE_CHUNK
+#define SYNTHE_EDGE_CHUNK 4
+#endif
+  constexpr int kChunkEdge = kWP % SYNTHE_EDGE_CHUNK == 0 ? SYNTHE_EDGE_CHUNK : (kWP % 2 == 0 ? 2 : 1);
+  constexpr int kNewton = kPrec == kPrecFP32 ? 0 : kFarNewton;
+  double edge[kWP];  // edge tiles' terms, kept apart so each sum updates in place
+#pragma unroll
+  for (int p = 0; p < kWP; ++p) edge[p] = 0.0;
   for (int kk = 0; kk < kClasses; ++kk) {
     const long lo = uniform_long(__shfl(bound, 2 * kk));
     const long hi = uniform_long(__shfl(bound, 2 * kk + 1));
-    if (lo >= hi) continue;
     const WingHead* __restrict__ hp = list_cm + lo;
-    const WingParams* __restrict__ pp = list_p + lo;
     const int n = (int)(hi - lo);  // entries of one depth: fewer than the lines
-    // software pipeline: the next entry's head and X are loaded while this
-    // entry is evaluated
-    WingHead h = hp[0];
-    double xh = pp[0].x_far;
     for (int j = 0; j < n; ++j) {
-      const WingHead cur = h;
-      const double x = xh;
-      if (j + 1 < n) {
-        h = hp[j + 1];
-        xh = pp[j + 1].x_far;
-      }
-      const int c = cur.c, M = cur.maxstep, n10 = cur.n10dop;
-      const int a = t0i - c, b = t1i - c;  // tile offsets from the centre
-      if (!cur.use_far || b < -M || a > M) continue;  // no far wing here
-      double kd = (double)(kd0 - c);
+      const WingHead h = hp[j];
+      if (t1i < h.lo || t0i > h.hi) continue;  // no far wing on this tile
+      double kd = kdl - h.cd;  // offset of the lane's first pixel from the centre
       __asm__("" : "+v"(kd));  // keep k = kd + p in double (no per-pixel int conversion)
-      const bool interior = (a > n10 && b <= M) || (b < -n10 && a >= -M);
-      double add[kWP];  // both paths end in one accumulation
-      if (interior && (kChunk == 1 || (cur.use_far & kFarChunk))) {
-        // |k| >= N10DOP + 1 >= 1 on every pixel of the tile; k * k is exact
-        if constexpr (kChunk == 1) {
+      // the tile lies wholly in the far zone N10DOP < |k| <= MAXSTEP
+      const bool interior = (t0i > h.nhi && t1i <= h.hi) || (t1i < h.nlo && t0i >= h.lo);
+      const bool chunk = __double_as_longlong(h.xs) >= 0;
+      const int n10 = (h.nhi - h.nlo) >> 1, M = (h.hi - h.lo) >> 1;
+      if (chunk && interior) {  // |k| >= N10DOP + 1 >= 1 on every pixel of the tile
+#pragma unroll
+        for (int p0 = 0; p0 < kWP; p0 += kChunk)
+          far_chunk_acc<kChunk, kNewton>(h.xs, kd, p0, acc + p0);
+      } else {
+        double add[kWP];
+        if (chunk) {
+          const double n10d = (double)n10, md = (double)M;
+#pragma unroll
+          for (int p0 = 0; p0 < kWP; p0 += kChunkEdge)
+            far_chunk<kChunkEdge, kNewton, true>(h.xs, kd, p0, n10d, md, add + p0);
+        } else {  // X below 2^-500: one quotient per pixel
+          const double n10d = (double)n10, md = (double)M, x = -h.xs;
 #pragma unroll
           for (int p = 0; p < kWP; ++p) {
             const double k = kd + (double)p;
-            add[p] = far_quotient<kPrec>(x, k * k);
+            const double xp = (fabs(k) > n10d && fabs(k) <= md) ? x : 0.0;
+            add[p] = far_quotient<kPrec>(xp, fmax(k * k, 1.0));
           }
+        }
+#pragma unroll
+        for (int p = 0; p < kWP; ++p) edge[p] += add[p];
+      }
+      if (kStats) {
+        if (interior) {
+          long nv = W - w0 < kWP ? W - w0 : kWP;
+          nev += nv > 0 ? nv : 0;
         } else {
-#pragma unroll
-          for (int p0 = 0; p0 < kWP; p0 += kChunk)
-            far_chunk<kChunk, kPrec == kPrecFP32 ? 0 : 2>(x, kd, p0, add + p0);
-        }
-        if (kStats) {
-          long n = W - w0 < kWP ? W - w0 : kWP;
-          nev += n > 0 ? n : 0;
-        }
-      } else {
-        // N10DOP < |k| <= MAXSTEP tested on k^2 (exact integers below 2^53)
-        const double n10q = (double)n10 * (double)n10, mq = (double)M * (double)M;
-#pragma unroll
-        for (int p = 0; p < kWP; ++p) {
-          const double k = kd + (double)p;
-          const double q = k * k;
-          const double xp = (q > n10q && q <= mq) ? x : 0.0;
-          // 0 / max(k^2, 1) adds +0, also at the centre pixel (k = 0)
-          add[p] = far_quotient<kPrec>(xp, fmax(q, 1.0));
-        }
-        if (kStats) {
+          const int c = h.lo + M;
           for (int p = 0; p < kWP; ++p) {
-            int ka = kd0 - c + p;
-            ka = ka < 0 ? -ka : ka;
+            const int ka = abs((int)w0 - c + p);
             if (ka > n10 && ka <= M && w0 + p < W) ++nev;
           }
         }
       }
-#pragma unroll
-      for (int p = 0; p < kWP; ++p) acc[p] += add[p];
     }
   }
+#pragma unroll
+  for (int p = 0; p < kWP; ++p) acc[p] += edge[p];
   double T = temp[d] > 1.0 ? temp[d] : 1.0;
   double hkt = kHPlanck / (kKBoltz * T);
 #pragma unroll
@@ -853,7 +862,8 @@ __global__ void k_wing_stats(long n_entries_cap, int D, long L, const int64_t* _
   unsigned long long cnt = 0, sum = 0, mx = 0;
   for (long j = beg + blockIdx.x * (long)blockDim.x + threadIdx.x; j < end;
        j += (long)gridDim.x * blockDim.x) {
-    unsigned long long m = (unsigned long long)list_cm[j].maxstep;
+    const WingHead h = list_cm[j];
+    unsigned long long m = (unsigned long long)(h.lo <= h.hi ? h.hi - h.lo : h.nhi - h.nlo) / 2;
     ++cnt;
     sum += m;
     mx = m > mx ? m : mx;
@@ -1261,8 +1271,8 @@ __global__ void k_check_div(long n, int variant, const double* x, const double* 
     // b holds k; each aligned group of 8 holds consecutive k (far_chunk)
     const int p = (int)(i & 7);
     double out[8];
-    if (variant == 5) far_chunk<8, 2>(xv, bv - (double)p, 0, out);
-    else far_chunk<8, 0>(xv, bv - (double)p, 0, out);
+    if (variant == 5) far_chunk<8, kFarNewton, false>(xv, bv - (double)p, 0, 0.0, 0.0, out);
+    else far_chunk<8, 0, false>(xv, bv - (double)p, 0, 0.0, 0.0, out);
     q = out[0];
 #pragma unroll
     for (int j = 1; j < 8; ++j) q = j == p ? out[j] : q;
@@ -1580,9 +1590,9 @@ __global__ __launch_bounds__(256) void k_near_adj(long W, int D, const WingHead*
     const long hi = uniform_long(__shfl(bound, 2 * kk + 1));
     for (long j = lo; j < hi; ++j) {
       const WingHead h = list_cm[j];
-      const long c = h.c;
-      const long lim = h.use_far ? h.n10dop : h.maxstep;
-      if (v0 + 63 < c - lim || v0 > c + lim) continue;  // wave-uniform
+      if (v0 + 63 < h.nlo || v0 > h.nhi) continue;  // wave-uniform
+      const long c = head_c(h);
+      const long lim = h.nhi - c;
       const long k = w - c;
       const long ka = k < 0 ? -k : k;
       double c0 = 0.0, ca = 0.0, cd = 0.0;
@@ -1628,9 +1638,9 @@ __global__ __launch_bounds__(64) void k_far_adj(GatherArgs g, const double* __re
     const long hi = uniform_long(__shfl(bound, 2 * kk + 1));
     for (long j = lo; j < hi; ++j) {
       const WingHead h = g.list_cm[j];
-      const long c = h.c, M = h.maxstep;
-      if (!h.use_far || t1 - c < -M || t0 - c > M) continue;
-      const double n10d = (double)h.n10dop, md = (double)M;
+      if (t1 < h.lo || t0 > h.hi) continue;  // no far wing here
+      const long c = head_c(h), M = h.hi - c;
+      const double n10d = (double)(h.nhi - c), md = (double)M;
       const double kd = (double)(w0 - c);
       double q = 0.0;
 #pragma unroll
@@ -2237,7 +2247,12 @@ struct BatchInst {
 };
 
 constexpr int kBatchStreams = 4;
-constexpr int kTilePixels[3] = {4, 8, 16};  // pixels per lane of the gather candidates
+// Pixels per lane of the batch gather: the single-spectrum tile.  A far-wing
+// sum depends on the tile (which wings have an edge in it, and the groups of 8
+// pixels sharing a reciprocal), so the batch keeps the single-spectrum tile
+// and its spectra are bit-identical to single ones; the autotuner picks the
+// grid.
+constexpr int kTilePixels[1] = {kWP};
 // persistent grids of the gather candidates; 0: one workgroup per item
 constexpr int kWavesPerSimd[4] = {2, 4, 8, 0};
 
@@ -2258,12 +2273,12 @@ struct synthe_batch {
   int64_t* d_items = nullptr;
   unsigned long long* d_counter = nullptr;
   std::vector<int32_t> order;
-  int kwp = 8;
+  int kwp = kWP;
   int gather_blocks = 4096;
   int n_cu = 256;
   int precision = SYNTHE_PREC_FP64;
   long n_items = 0;              // gather items of the current tile
-  std::vector<double> tune_ms;  // [3 tiles][4 grids], ms, or empty
+  std::vector<double> tune_ms;  // [tiles][4 grids], ms, or empty
   hipStream_t streams[kBatchStreams] = {};
   hipEvent_t ev_fork = nullptr, ev_mid = nullptr;
   hipEvent_t ev_join[kBatchStreams] = {};
@@ -2306,9 +2321,9 @@ int launch_gather_batch(synthe_batch* b, int kwp, int blocks, hipStream_t st) {
   else                                                                                                        \
     hipLaunchKernelGGL((k_gather_items<P, K>), dim3((unsigned)b->n_items), dim3(64), 0, st, b->d_args,       \
                        b->d_order, b->d_items, n)
-  if (kwp == 4) { if (f32) SYNTHE_GB(kPrecFP32, 4); else SYNTHE_GB(kPrecFP64, 4); }
-  else if (kwp == 16) { if (f32) SYNTHE_GB(kPrecFP32, 16); else SYNTHE_GB(kPrecFP64, 16); }
-  else { if (f32) SYNTHE_GB(kPrecFP32, 8); else SYNTHE_GB(kPrecFP64, 8); }
+  if (kwp != kWP) return fail(SYNTHE_EINVAL, "synthe_batch: unsupported gather tile");
+  if (f32) SYNTHE_GB(kPrecFP32, kWP);
+  else SYNTHE_GB(kPrecFP64, kWP);
 #undef SYNTHE_GB
   HIP_TRY(hipGetLastError());
   return SYNTHE_OK;
@@ -2508,7 +2523,7 @@ int synthe_batch_create(synthe_ctx* ctx, int n, const synthe_instance* inst, int
     if (hipEventCreate(&t0) != hipSuccess || hipEventCreate(&t1) != hipSuccess)
       return bail(fail(SYNTHE_EHIP, "synthe_batch_create: event creation failed"));
     double best = 1e300;
-    for (int a = 0; a < 3 && !rc; ++a) {
+    for (int a = 0; a < (int)(sizeof(kTilePixels) / sizeof(int)) && !rc; ++a) {
       if ((rc = batch_items(b, kTilePixels[a]))) break;
       for (int c = 0; c < 4 && !rc; ++c) {
         const int blocks = b->n_cu * 4 * kWavesPerSimd[c];

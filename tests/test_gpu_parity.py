@@ -194,8 +194,8 @@ def test_far_wing_fast_quotient_ulps(gpu):
 def test_far_wing_shared_reciprocal_ulps(gpu):
     """The interior tiles' form: 8 consecutive k share one reciprocal of the
     product of their k^2 (running products, reciprocal, then one product per
-    pixel back).  With two Newton steps every X / k^2 is within 16 ulp of the
-    IEEE quotient; without them, within 2^-20 relative.  Operands: |k| from 1
+    pixel back).  With the FP64 form's one Newton step every X / k^2 is within
+    32 ulp of the IEEE quotient; without it, within 2^-20 relative.  Operands: |k| from 1
     to 10^6 of either sign, X down to 2^-500 (the form's lower limit)."""
     import torch
     from kurucz_py_joss_submission_amd import _native as N
@@ -222,8 +222,8 @@ def test_far_wing_shared_reciprocal_ulps(gpu):
         assert np.all(np.isfinite(qf)) and np.all(qi > 0)
         err[variant] = (float(np.max(np.abs(qf - qi) / np.spacing(qi))),
                         float(np.max(np.abs(qf - qi) / qi)))
-    print("shared reciprocal: max ulp (2 Newton)", err[5][0], "max rel (none)", err[6][1])
-    assert err[5][0] <= 16
+    print("shared reciprocal: max ulp (1 Newton)", err[5][0], "max rel (none)", err[6][1])
+    assert err[5][0] <= 32
     assert err[6][1] <= 2.0 ** -20
 
 
