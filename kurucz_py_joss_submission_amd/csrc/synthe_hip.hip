@@ -174,6 +174,8 @@ __device__ __forceinline__ int reach_class(int m) {
 }
 
 // physics/continuum.py:73-303, coefficient path, one thread per wavelength.
+constexpr int kContinuumDepths = 8;  // depths per thread (grid rows cover the rest)
+
 __global__ void k_continuum(int D, long W, const double* __restrict__ wl, int E,
                             const double* __restrict__ wledge, const double* __restrict__ half_in,
                             const double* __restrict__ delta_in, const double* __restrict__ cabs,
@@ -203,7 +205,8 @@ __global__ void k_continuum(int D, long W, const double* __restrict__ wl, int E,
   double c1 = (x - h) * (x - right) / dl;
   double c2 = (left - x) * (x - right) * 2.0 / dl;
   double c3 = (x - left) * (x - h) / dl;
-  for (int d = 0; d < D; ++d) {
+  const int d1 = min(D, (int)(blockIdx.y + 1) * kContinuumDepths);
+  for (int d = blockIdx.y * kContinuumDepths; d < d1; ++d) {
     const double* ca = cabs + ((long)d * (E - 1) + e) * 3;
     const double* cs = cscat + ((long)d * (E - 1) + e) * 3;
     double la = ca[0] * c1 + ca[1] * c2 + ca[2] * c3;
@@ -409,19 +412,17 @@ __global__ __launch_bounds__(kBlock) void k_class_count(long L, int D, const int
   long s = blockIdx.x * (long)kBlock + threadIdx.x;
   long l = s < L ? order[s] : -1;
   int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int d = 0; d < D; ++d) {
-    int k = l >= 0 ? cls[l * D + d] : 0xFF;
-    for (int kk = 0; kk < kClasses; ++kk) {
-      unsigned long long m = __ballot(k == kk);
-      if (lane == 0) sh[wave][kk] = __popcll(m);
-    }
-    __syncthreads();
-    if (threadIdx.x < kClasses) {
-      int t = 0;
-      for (int q = 0; q < kBlock / 64; ++q) t += sh[q][threadIdx.x];
-      counts[((long)blockIdx.x * D + d) * kClasses + threadIdx.x] = t;
-    }
-    __syncthreads();
+  const int d = blockIdx.y;  // depths are independent: one per workgroup row
+  int k = l >= 0 ? cls[l * D + d] : 0xFF;
+  for (int kk = 0; kk < kClasses; ++kk) {
+    unsigned long long m = __ballot(k == kk);
+    if (lane == 0) sh[wave][kk] = __popcll(m);
+  }
+  __syncthreads();
+  if (threadIdx.x < kClasses) {
+    int t = 0;
+    for (int q = 0; q < kBlock / 64; ++q) t += sh[q][threadIdx.x];
+    counts[((long)blockIdx.x * D + d) * kClasses + threadIdx.x] = t;
   }
 }
 
@@ -475,35 +476,33 @@ __global__ __launch_bounds__(kBlock) void k_class_scatter(
   long l = s < L ? order[s] : -1;
   int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   unsigned long long ltmask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  for (int d = 0; d < D; ++d) {
-    int k = l >= 0 ? cls[l * D + d] : 0xFF;
-    int rank = 0;
-    for (int kk = 0; kk < kClasses; ++kk) {
-      unsigned long long m = __ballot(k == kk);
-      if (lane == 0) sh[wave][kk] = __popcll(m);
-      if (k == kk) rank = __popcll(m & ltmask);
-    }
-    __syncthreads();
-    if (k != 0xFF) {
-      int before = 0;
-      for (int q = 0; q < wave; ++q) before += sh[q][k];
-      int64_t pos = regions[d * (kClasses + 1) + k] +
-                    offsets[((long)blockIdx.x * D + d) * kClasses + k] + before + rank;
-      WingParams p = wp[l * D + d];
-      const int r = p.use_far ? p.n10dop : p.maxstep;
-      const bool chunk = !(p.x_far > 0.0 && p.x_far < kChunkMinX);  // X >= 0
-      WingHead h;
-      h.lo = p.use_far ? p.c - p.maxstep : INT32_MAX;
-      h.hi = p.use_far ? p.c + p.maxstep : INT32_MIN;
-      h.nlo = p.c - r;
-      h.nhi = p.c + r;
-      h.xs = p.use_far ? (chunk ? p.x_far : -p.x_far) : 0.0;
-      h.cd = (double)p.c;
-      list_cm[pos] = h;
-      list_p[pos] = p;
-      if (list_l) list_l[pos] = (int32_t)l;
-    }
-    __syncthreads();
+  const int d = blockIdx.y;  // depths are independent: one per workgroup row
+  int k = l >= 0 ? cls[l * D + d] : 0xFF;
+  int rank = 0;
+  for (int kk = 0; kk < kClasses; ++kk) {
+    unsigned long long m = __ballot(k == kk);
+    if (lane == 0) sh[wave][kk] = __popcll(m);
+    if (k == kk) rank = __popcll(m & ltmask);
+  }
+  __syncthreads();
+  if (k != 0xFF) {
+    int before = 0;
+    for (int q = 0; q < wave; ++q) before += sh[q][k];
+    int64_t pos = regions[d * (kClasses + 1) + k] + offsets[((long)blockIdx.x * D + d) * kClasses + k] + before +
+                  rank;
+    WingParams p = wp[l * D + d];
+    const int r = p.use_far ? p.n10dop : p.maxstep;
+    const bool chunk = !(p.x_far > 0.0 && p.x_far < kChunkMinX);  // X >= 0
+    WingHead h;
+    h.lo = p.use_far ? p.c - p.maxstep : INT32_MAX;
+    h.hi = p.use_far ? p.c + p.maxstep : INT32_MIN;
+    h.nlo = p.c - r;
+    h.nhi = p.c + r;
+    h.xs = p.use_far ? (chunk ? p.x_far : -p.x_far) : 0.0;
+    h.cd = (double)p.c;
+    list_cm[pos] = h;
+    list_p[pos] = p;
+    if (list_l) list_l[pos] = (int32_t)l;
   }
 }
 
@@ -1866,13 +1865,15 @@ static int asynth_prepare(const synthe_ctx* ctx, const synthe_lines* lines, cons
                        to_dev(depths), n_wl, kapmin, cutoff, 0.0, wl, ctx->tb, 2, nullptr, nullptr, transp, valid,
                        b.wp, b.cls);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_class_count, dim3((unsigned)nblk), dim3(kBlock), 0, st, L, D, wing_order, b.cls, b.cnt);
+    hipLaunchKernelGGL(k_class_count, dim3((unsigned)nblk, (unsigned)D), dim3(kBlock), 0, st, L, D, wing_order,
+                       b.cls, b.cnt);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_class_scan, dim3((unsigned)(D * kClasses)), dim3(kBlock), 0, st, nblk, D, b.cnt, b.tot);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_regions, dim3(blocks_for(D, 64)), dim3(64), 0, st, D, L, b.tot, b.reg);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_class_scatter, dim3((unsigned)nblk), dim3(kBlock), 0, st, L, D, wing_order, b.cls, b.cnt,
+    hipLaunchKernelGGL(k_class_scatter, dim3((unsigned)nblk, (unsigned)D), dim3(kBlock), 0, st, L, D, wing_order,
+                       b.cls, b.cnt,
                        b.reg, b.wp, b.lcm, b.lp, b.list_l);
     HIP_TRY(hipGetLastError());
   } else {
@@ -2095,7 +2096,8 @@ int synthe_continuum(synthe_ctx* ctx, long n_depths, long n_wl, const double* wl
   if (n_wl == 0) return SYNTHE_OK;
   if (!wl || !wledge || !coeff_abs || !coeff_scat || !cont_abs || !cont_scat)
     return fail(SYNTHE_EINVAL, "synthe_continuum: NULL array");
-  hipLaunchKernelGGL(k_continuum, dim3(blocks_for(n_wl, kBlock)), dim3(kBlock), 0, (hipStream_t)stream,
+  const unsigned rows = (unsigned)((n_depths + kContinuumDepths - 1) / kContinuumDepths);
+  hipLaunchKernelGGL(k_continuum, dim3(blocks_for(n_wl, kBlock), rows), dim3(kBlock), 0, (hipStream_t)stream,
                      (int)n_depths, n_wl, wl, (int)n_edges, wledge, half_edge, delta_edge, coeff_abs,
                      coeff_scat, coeff_log10, cont_abs, cont_scat);
   HIP_TRY(hipGetLastError());
