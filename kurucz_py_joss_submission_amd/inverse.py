@@ -1,0 +1,348 @@
+"""Abundance recovery from many spectra: Gauss-Newton over batched syntheses.
+
+The reference has no inverse solver; this module is the batched-inverse form
+of the hot path.  Each target spectrum comes from its own atmosphere with its
+own 20 abundance offsets (dex); the unknowns are those offsets.  Every
+Gauss-Newton iteration evaluates all spectra at the current offsets and at one
+forward-difference step per abundance by replaying one BatchPlan graph (all
+spectra per launch), takes a least-squares step per spectrum from the SVD of
+its Jacobian (Levenberg-damped when a step fails), and keeps the step when it
+lowers that spectrum's misfit.
+
+Abundances enter the forward pass only through the atmosphere bundle
+(synthetic.build_bundle): populations ~ A_z / sum(A), the mean molecular
+weight (so rho), N(H I) and N(He I) (continuum scattering, TXNXN).  With the
+columns' electron density held fixed, as build_bundle does, those are closed
+forms in the offsets; AbundanceResponse evaluates them on the GPU for a batch
+of offsets and the plan's device arrays are overwritten in place before each
+replay, so no bundle is rebuilt on the host.
+"""
+
+from __future__ import annotations
+
+import tempfile
+import time
+from dataclasses import dataclass
+from pathlib import Path
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from . import lines as L
+from . import synthetic as S
+from .constants import AMU, K_BOLTZ
+
+# elements with lines in the synthetic lists (Fe I/II, Ti I, Cr I, Ca I, Ni I)
+LINE_ELEMENTS = (20, 22, 24, 26, 28)
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+class AbundanceResponse:
+    """The abundance-dependent arrays of synthetic.build_bundle for a batch of
+    offsets delta[B][20] (dex, in synthetic.ABUNDANCE_Z order): populations
+    [B][D][N_ION][N_ELEMENTS], mass density [B][D], TXNXN [B][D] and the
+    log10 continuum scattering coefficients [B][D][E-1][3].  Temperature, the
+    electron density, Doppler widths and the absorption coefficients do not
+    depend on the abundances."""
+
+    def __init__(self, cols: S.AtmColumns, device: str = "cuda"):
+        torch = _torch()
+        f8 = torch.float64
+        temp = cols.temperature.astype(np.float64)
+        ne = cols.xne.astype(np.float64)
+        p = cols.pressure.astype(np.float64)
+        D = temp.size
+        self.D = D
+        zs = S.ABUNDANCE_Z
+        # populations per nucleus of each element: fraction in stage s / U_s
+        unit = np.zeros((D, S.N_ION, len(zs)))
+        fr_h1 = fr_he1 = None
+        for i, z in enumerate(zs):
+            fr = S._saha_fractions(z, temp, ne)
+            if z == 1:
+                fr_h1 = fr[0]
+            if z == 2:
+                fr_he1 = fr[0]
+            for s in range(min(fr.shape[0], S.N_ION)):
+                unit[:, s, i] = fr[s] / S._partition(z, s, temp)
+        ab0 = S.abundances(cols.mh)
+        nnuc = p / (K_BOLTZ * temp) - ne
+        # continuum scattering at the edge nodes: sigma = (ne sT + N(H I) sR(lam)) / rho
+        E = S.WLEDGE.size
+        half = 0.5 * (S.WLEDGE[:-1] + S.WLEDGE[1:])
+        sr = np.zeros((E - 1, 3))
+        for e in range(E - 1):
+            for j, lam in enumerate((S.WLEDGE[e], half[e], S.WLEDGE[e + 1])):
+                lam_a = lam * 10.0
+                sr[e, j] = 5.799e-13 / lam_a ** 4 + 1.422e-6 / lam_a ** 6 + 2.784 / lam_a ** 8
+        t = lambda a: torch.tensor(np.ascontiguousarray(a), dtype=f8, device=device)  # noqa: E731
+        self.unit = t(unit)
+        self.nnuc = t(nnuc)
+        self.fr_h1, self.fr_he1 = t(fr_h1), t(fr_he1)
+        self.ne_st = t(ne * S.SIGMA_THOMSON)
+        self.sr = t(sr)
+        self.tfac = t((np.maximum(temp, 1.0) / 10_000.0) ** 0.3)
+        self.ab0 = t([ab0[z] for z in zs])
+        self.mass = t([S.ELEMENTS[z]["mass"] for z in zs])
+        self.zidx = torch.tensor([z - 1 for z in zs], dtype=torch.int64, device=device)
+        self.device = device
+
+    def arrays(self, delta) -> Dict[str, object]:
+        torch = _torch()
+        delta = torch.as_tensor(delta, dtype=torch.float64, device=self.device).reshape(-1, self.ab0.numel())
+        B = delta.shape[0]
+        ab = self.ab0[None, :] * torch.pow(10.0, delta)
+        asum = ab.sum(dim=1, keepdim=True)
+        frac = ab / asum                                          # [B][20]
+        nz = frac[:, None, :] * self.nnuc[None, :, None]          # [B][D][20]
+        pop = torch.zeros((B, self.D, S.N_ION, S.N_ELEMENTS), dtype=torch.float64, device=self.device)
+        pop[:, :, :, self.zidx] = nz[:, :, None, :] * self.unit[None]
+        mu = (ab * self.mass[None, :]).sum(dim=1, keepdim=True) / asum
+        rho = self.nnuc[None, :] * mu * AMU                       # [B][D]
+        nh1 = nz[:, :, 0] * self.fr_h1[None, :]
+        he1 = nz[:, :, 1] * self.fr_he1[None, :]
+        sig = (self.ne_st[None, :, None, None] + nh1[:, :, None, None] * self.sr[None, None]) / rho[:, :, None, None]
+        cscat = torch.log10(torch.clamp(sig, min=1e-30))
+        txnxn = (nh1 + 0.42 * he1) * self.tfac[None, :]
+        return dict(pop=pop, rho=rho, txnxn=txnxn, coeff_scat=cscat)
+
+
+@dataclass
+class Target:
+    cols: S.AtmColumns
+    catalog: L.LineCatalog
+    wl_start: float
+    wl_end: float
+    delta_true: np.ndarray  # [20] dex
+    flux: np.ndarray        # F_lambda of the FP64 plain path at delta_true
+
+
+def window_for(wl_start: float, n_wl: int, resolution: float) -> float:
+    """End of a window starting at wl_start whose grid has n_wl points."""
+    from . import _native as N
+
+    r = np.log1p(1.0 / resolution)
+    w1 = wl_start * np.exp((n_wl - 0.5) * r)
+    for _ in range(8):
+        m = N.wavelength_grid(wl_start, w1, resolution).size
+        if m == n_wl:
+            return float(w1)
+        w1 *= np.exp((n_wl - m) * r)
+    raise RuntimeError("could not size the window")
+
+
+def make_targets(n: int = 64, n_lines: int = 30_000, n_wl: int = 5_000, seed: int = 5,
+                 sigma_dex: float = 0.2, wl_range=(400.0, 700.0), resolution: float = 300000.0,
+                 cutoff: float = 1e-3) -> List[Target]:
+    """n atmospheres drawn as the batched configuration (Teff U(3500, 40000) K,
+    log g U(0, 5), 64-80 layers, [M/H] U(-2, 0.5)), each with a window of n_wl
+    points starting in wl_range, its own n_lines-line list, and abundance
+    offsets N(0, sigma_dex) for the 19 elements other than hydrogen (the
+    reference: a common offset of all 20 leaves every spectrum unchanged);
+    the target flux is the FP64 plain path (SpectrumPlan) at those offsets."""
+    from .device import SpectrumPlan
+
+    rng = np.random.default_rng(seed)
+    out = []
+    with tempfile.TemporaryDirectory() as td:
+        for i in range(n):
+            teff = float(rng.uniform(3500.0, 40000.0))
+            logg = float(rng.uniform(0.0, 5.0))
+            nl = int(rng.integers(64, 81))
+            mh = float(rng.uniform(-2.0, 0.5))
+            w0 = float(rng.uniform(wl_range[0], wl_range[1]))
+            w1 = window_for(w0, n_wl, resolution)
+            delta = rng.normal(0.0, sigma_dex, len(S.ABUNDANCE_Z))
+            delta[S.ABUNDANCE_Z.index(1)] = 0.0  # hydrogen is the reference (log eps = 12)
+            cols = S.make_atm(teff, logg, nl, mh)
+            d = S.synthetic_lines(n_lines, w0, w1, seed=1000 + seed * 7919 + i)
+            path = Path(td) / f"lines_{i}.gfall"
+            S.write_line_list(path, d)
+            cat = L.filter_by_range(L.load_catalog(path), w0, w1)
+            atm = S.build_bundle(cols, dict(zip(S.ABUNDANCE_Z, delta.tolist())))
+            plan = SpectrumPlan(atm, cat, w0, w1, resolution, cutoff=cutoff)
+            plan.run()
+            out.append(Target(cols, cat, w0, w1, delta, plan.flux.cpu().numpy()))
+            del plan
+    return out
+
+
+class InverseProblem:
+    """Batched Gauss-Newton recovery of each target's 20 abundance offsets.
+
+    One BatchPlan holds every target's spectrum; evaluate(delta) writes the
+    bundle arrays of delta[n][20] into the plan's device arrays and replays
+    the graph.  The residual of spectrum i is (F - F_target) / median(F_target)."""
+
+    def __init__(self, targets: Sequence[Target], resolution: float = 300000.0, cutoff: float = 1e-3,
+                 autotune: bool = True):
+        from .device import BatchPlan
+
+        torch = _torch()
+        self.targets = list(targets)
+        self.n = len(self.targets)
+        self.nz = len(S.ABUNDANCE_Z)
+        self.h_index = S.ABUNDANCE_Z.index(1)
+        inst = [(S.build_bundle(t.cols), t.catalog, t.wl_start, t.wl_end) for t in self.targets]
+        self.plan = BatchPlan(inst, resolution=resolution, cutoff=cutoff, precision="fp64", autotune=autotune,
+                              graph=True)
+        self.resp = [AbundanceResponse(t.cols) for t in self.targets]
+        self.f_target = [torch.tensor(t.flux, dtype=torch.float64, device="cuda") for t in self.targets]
+        # residuals in units of each target's median flux (saturated line
+        # cores would dominate relative residuals)
+        self.f_scale = [float(np.median(t.flux)) for t in self.targets]
+        self.evaluations = 0
+
+    def evaluate(self, delta: np.ndarray) -> List[object]:
+        """Fluxes (torch, F_lambda) of every spectrum at offsets delta[n][20]."""
+        torch = _torch()
+        for i, (it, r) in enumerate(zip(self.plan.items, self.resp)):
+            a = r.arrays(delta[i])
+            dp = it["depth"]
+            dp.pop.copy_(a["pop"][0])
+            dp.rho.copy_(a["rho"][0])
+            dp.txnxn.copy_(a["txnxn"][0])
+            dp.coeff_scat.copy_(a["coeff_scat"][0])
+        self.plan.run()
+        self.evaluations += 1
+        torch.cuda.synchronize()
+        return [it["flux"].clone() for it in self.plan.items]
+
+    def residuals(self, flux) -> List[object]:
+        return [(f - t) / s for f, t, s in zip(flux, self.f_target, self.f_scale)]
+
+    def solve(self, max_iter: int = 12, step: float = 1e-3, tol: float = 1e-6, rcond: float = 1e-6,
+              cfloor: float = 1e-3, max_tries: int = 6, max_step: float = 0.5, bound: float = 2.0) -> dict:
+        """Gauss-Newton from zero offsets, with Levenberg damping on failure.
+
+        Per iteration: a forward-difference Jacobian (one batched evaluation
+        per abundance) and, per spectrum, the SVD of its columns; the step is
+        -V diag(s / (s^2 + mu s_0^2)) U^T r (mu = 0 first: plain Gauss-Newton).
+        A spectrum keeps its step if the misfit drops (mu / 10), else
+        mu = max(10 mu, 1e-6) and a new trial with the same Jacobian.
+        Hydrogen is the reference and stays at zero; columns below cfloor
+        times the largest (elements without lines reach the spectrum only
+        through sum(A) and the mean molecular weight) are held at zero.  A
+        step is shortened to max_step dex, and offsets stay within +-bound."""
+        torch = _torch()
+        n, nz = self.n, self.nz
+        delta = np.zeros((n, nz))
+        t0 = time.perf_counter()
+        res = self.residuals(self.evaluate(delta))
+        cost = np.array([float((r * r).sum()) for r in res])
+        history = [float(cost.sum())]
+        mu = np.zeros(n)
+        active = cost > 0
+        sens = np.zeros((n, nz))
+        it = 0
+        for it in range(1, max_iter + 1):
+            cols = []
+            for k in range(nz):
+                if k == self.h_index:
+                    cols.append(None)
+                    continue
+                dk = delta.copy()
+                dk[:, k] += step
+                rk = self.residuals(self.evaluate(dk))
+                cols.append([(rk[i] - res[i]) / step for i in range(n)])
+            svd = []
+            for i in range(n):
+                idx = [k for k in range(nz) if cols[k] is not None]
+                J = torch.stack([cols[k][i] for k in idx], dim=1)              # [W][nz - 1]
+                c = torch.linalg.vector_norm(J, dim=0)
+                live = (c > cfloor * c.max()).cpu().numpy()
+                sens[i, idx] = c.cpu().numpy()
+                idx = [k for k, lv in zip(idx, live) if lv]
+                if not idx:
+                    svd.append(None)
+                    continue
+                u, s_, vh = torch.linalg.svd(J[:, torch.tensor(np.nonzero(live)[0], device=J.device)],
+                                             full_matrices=False)
+                svd.append((idx, s_.cpu().numpy(), vh.T.cpu().numpy(), (u.T @ res[i]).cpu().numpy()))
+            improved = np.zeros(n, bool)
+            for _ in range(max_tries):
+                dstep = np.zeros((n, nz))
+                for i in np.nonzero(active & ~improved)[0]:
+                    if svd[i] is None:
+                        continue
+                    idx, s_, V, Ur = svd[i]
+                    keep = s_ > rcond * s_[0]
+                    f = np.zeros_like(s_)
+                    f[keep] = s_[keep] / (s_[keep] ** 2 + mu[i] * s_[0] ** 2)
+                    dstep[i, idx] = -(V @ (f * Ur))
+                    big = float(np.abs(dstep[i]).max())
+                    if big > max_step:
+                        dstep[i] *= max_step / big
+                trial = np.clip(delta + dstep, -bound, bound)
+                dstep = trial - delta
+                rt = self.residuals(self.evaluate(trial))
+                ct = np.array([float((r * r).sum()) for r in rt])
+                for i in np.nonzero(active & ~improved)[0]:
+                    if ct[i] < cost[i]:
+                        improved[i] = True
+                        delta[i] = trial[i]
+                        res[i] = rt[i]
+                        cost[i] = ct[i]
+                        mu[i] = mu[i] / 10.0 if mu[i] > 1e-9 else 0.0
+                        if float(np.abs(dstep[i]).max()) < tol:
+                            active[i] = False
+                    else:
+                        mu[i] = max(10.0 * mu[i], 1e-6)
+                if (improved | ~active).all():
+                    break
+            active &= improved  # a spectrum no trial improves has converged as far as it can
+            history.append(float(cost.sum()))
+            if not active.any():
+                break
+        elapsed = time.perf_counter() - t0
+        return dict(delta=delta, iterations=it, seconds=elapsed, evaluations=self.evaluations,
+                    cost_history=history, cost=cost.copy(), sensitivity=sens)
+
+
+def report(targets: Sequence[Target], delta: np.ndarray, sensitivity: np.ndarray,
+           min_sensitivity: float = 2e-3) -> dict:
+    """Abundance errors (dex).  A spectrum constrains a line element (Fe, Ti,
+    Cr, Ca, Ni) when its residual moves by at least min_sensitivity (L2, in
+    units of the median flux) per dex of that element -- 0.05 dex then changes the misfit
+    by 2.5e-3 min_sensitivity^2 or more; neutral-metal lines in hot
+    atmospheres are too weak for that.  Errors are given for the constrained
+    and the unconstrained pairs, and over all 19 offsets (hydrogen is the
+    reference element)."""
+    truth = np.stack([t.delta_true for t in targets])
+    err = np.abs(delta - truth)
+    li = [S.ABUNDANCE_Z.index(z) for z in LINE_ELEMENTS]
+    fit = sensitivity[:, li] >= min_sensitivity
+    e_fit, e_not = err[:, li][fit], err[:, li][~fit]
+    free = [k for k in range(len(S.ABUNDANCE_Z)) if S.ABUNDANCE_Z[k] != 1]
+    out = dict(line_pairs=int(fit.size), line_pairs_constrained=int(fit.sum()),
+               max_err_constrained_line_dex=float(e_fit.max()) if e_fit.size else None,
+               median_err_constrained_line_dex=float(np.median(e_fit)) if e_fit.size else None,
+               rms_err_unconstrained_line_dex=float(np.sqrt((e_not ** 2).mean())) if e_not.size else None,
+               rms_err_all_19_dex=float(np.sqrt((err[:, free] ** 2).mean())),
+               rms_true_offset_dex=float(np.sqrt((truth[:, free] ** 2).mean())))
+    return out
+
+
+def judge(targets: Sequence[Target], delta: np.ndarray, indices: Optional[Sequence[int]] = None,
+          resolution: float = 300000.0, cutoff: float = 1e-3) -> float:
+    """Max relative difference between each target and the FP64 plain path
+    (SpectrumPlan from a host-built bundle) at the recovered offsets, over the
+    pixels with |F| > 1e-3 median(F) (the robust mask of the reference's
+    tools/compare_spectra.py:128-135)."""
+    from .device import SpectrumPlan
+
+    worst = 0.0
+    for i in (range(len(targets)) if indices is None else indices):
+        t = targets[i]
+        atm = S.build_bundle(t.cols, dict(zip(S.ABUNDANCE_Z, np.asarray(delta[i]).tolist())))
+        plan = SpectrumPlan(atm, t.catalog, t.wl_start, t.wl_end, resolution, cutoff=cutoff)
+        plan.run()
+        f = plan.flux.cpu().numpy()
+        m = np.abs(t.flux) > 1e-3 * np.median(np.abs(t.flux))
+        worst = max(worst, float(np.max(np.abs(f[m] - t.flux[m]) / np.abs(t.flux[m]))))
+    return worst

@@ -1,0 +1,63 @@
+"""Host logic of the batched inverse problem: the closed-form abundance
+response (populations, mass density, TXNXN, continuum scattering) against the
+host bundle builder at random offsets, and the window sizing."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+
+@pytest.mark.parametrize("teff,logg,nl,mh,seed", [(5750.0, 4.44, 72, 0.0, 1), (7200.0, 3.1, 70, -0.7, 2),
+                                                  (3900.0, 1.2, 64, 0.4, 3), (24000.0, 4.0, 80, -1.8, 4)])
+def test_abundance_response_matches_bundle(teff, logg, nl, mh, seed):
+    from kurucz_py_joss_submission_amd import synthetic as S
+    from kurucz_py_joss_submission_amd.inverse import AbundanceResponse
+
+    cols = S.make_atm(teff, logg, nl, mh)
+    rng = np.random.default_rng(seed)
+    deltas = rng.normal(0.0, 0.3, (3, len(S.ABUNDANCE_Z)))
+    deltas[0] = 0.0
+    got = AbundanceResponse(cols, device="cpu").arrays(deltas)
+    for b, d in enumerate(deltas):
+        ref = S.build_bundle(cols, dict(zip(S.ABUNDANCE_Z, d.tolist())))
+        for key, want in (("pop", ref.population_per_ion), ("rho", ref.mass_density),
+                          ("txnxn", ref.txnxn_or_default()), ("coeff_scat", ref.continuum_scat_coeff)):
+            x = got[key][b].numpy()
+            assert x.shape == want.shape, key
+            assert np.array_equal(x != 0, want != 0), key
+            if key == "coeff_scat":  # log10 values: compare absolutely (1e-12 dex)
+                assert np.max(np.abs(x - want)) < 1e-12, key
+                continue
+            m = want != 0
+            assert np.max(np.abs(x[m] - want[m]) / np.abs(want[m])) < 1e-13, key
+
+
+def test_response_is_linear_in_abundance_for_line_species():
+    """A metal's populations scale with its abundance over sum(A): doubling
+    iron's abundance doubles every iron population up to that normalisation."""
+    from kurucz_py_joss_submission_amd import synthetic as S
+    from kurucz_py_joss_submission_amd.inverse import AbundanceResponse
+
+    cols = S.make_atm(5750.0, 4.44, 72, 0.0)
+    r = AbundanceResponse(cols, device="cpu")
+    d = np.zeros((2, len(S.ABUNDANCE_Z)))
+    d[1, S.ABUNDANCE_Z.index(26)] = np.log10(2.0)
+    a = r.arrays(d)
+    ab = S.abundances(0.0)
+    asum0 = sum(ab.values())
+    asum1 = asum0 + ab[26]
+    num, den = a["pop"][1, :, :, 25].numpy(), a["pop"][0, :, :, 25].numpy()
+    m = den > 0
+    ratio = num[m] / den[m]
+    assert m.sum() >= 72
+    assert np.allclose(ratio, 2.0 * asum0 / asum1, rtol=1e-13, atol=0.0)
+
+
+def test_window_has_requested_points():
+    from kurucz_py_joss_submission_amd import _native as N
+    from kurucz_py_joss_submission_amd.inverse import window_for
+
+    for w0 in (400.0, 523.7, 699.9):
+        w1 = window_for(w0, 5000, 300000.0)
+        assert N.wavelength_grid(w0, w1, 300000.0).size == 5000
