@@ -577,41 +577,101 @@ __device__ __forceinline__ long class_window_search(int lane, int d, long lo_px,
 
 // Voigt zone of every wing: pixels 1 <= |k| <= N10DOP of pairs with a far
 // wing and 1 <= |k| <= MAXSTEP of the others, valued as
-// physics/line_opacity.py:410-560 (near_wing).  One lane per pixel, one wave
-// per 64 pixels of one depth; entries are walked with wave-uniform loads in
-// class-then-centre order, so each pixel's sum has a fixed order.  Writes the
-// pre-STIM sum to out [D][W] (0 where no wing reaches), which k_gather
-// continues from.
+// physics/line_opacity.py:410-560 (near_wing).  One wave per 64 pixels of one
+// depth; entries are walked with wave-uniform loads in class-then-centre
+// order.  A zone covers a few tens of the 64 pixels, so evaluating it lane per
+// pixel would leave most lanes idle: instead each (entry, pixel) pair is
+// queued in LDS, with the pixel's next slot, and the queue is evaluated 64
+// pairs at a time when a pixel runs out of slots (and at the end); each pixel
+// then adds its slots in order.  The sum order is the entry order, the same
+// as adding entry by entry.  Writes the pre-STIM sum to out [D][W] (0 where no
+// wing reaches), which k_gather continues from.
+constexpr int kNearSlots = 6;                  // queued values per pixel
+constexpr int kNearQueue = kNearSlots * 64;    // pairs per wave between flushes
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// A queued pair: bits 0-31 the entry's index within its depth's lists, 32-51
+// |k| (<= MAXSTEP <= 10^6 < 2^20), 52-57 the pixel's lane, 58-61 its slot.
+__device__ __forceinline__ uint64_t near_record(long j_rel, long ka, int lane, int slot) {
+  return (uint64_t)(uint32_t)j_rel | ((uint64_t)ka << 32) | ((uint64_t)lane << 52) | ((uint64_t)slot << 58);
+}
+
+// Evaluates the qn queued pairs into vals (64 at a time), then adds each
+// lane's cnt slots to acc in slot order and empties the queue.
+__device__ __forceinline__ void near_flush(int lane, int qn, int& cnt, double& acc, const uint64_t* queue,
+                                           double (*vals)[64], const WingParams* __restrict__ list_p,
+                                           const DeviceTables& tb) {
+  wave_lds_sync();
+  for (int q = lane; q < qn; q += 64) {
+    const uint64_t r = queue[q];
+    const long j_rel = (long)(r & 0xffffffffull);
+    const long ka = (long)((r >> 32) & 0xfffffull);
+    const int pl = (int)((r >> 52) & 63);
+    const int sl = (int)((r >> 58) & 15);
+    vals[sl][pl] = near_wing(list_p[j_rel], ka, tb.h0, tb.h1, tb.h2);
+  }
+  wave_lds_sync();
+  for (int sl = 0; sl < cnt; ++sl) acc += vals[sl][lane];
+  wave_lds_sync();
+  cnt = 0;
+}
+
 template <bool kStats>
 __global__ __launch_bounds__(256) void k_near(long W, int D, const WingHead* __restrict__ list_cm,
                                               const WingParams* __restrict__ list_p,
                                               const int64_t* __restrict__ regions,
                                               const int32_t* __restrict__ near_reach, DeviceTables tb,
                                               double* __restrict__ out, unsigned long long* __restrict__ evals) {
+  __shared__ uint64_t s_queue[4][kNearQueue];
+  __shared__ double s_vals[4][kNearSlots][64];
   const int d = blockIdx.y;
   const int lane = threadIdx.x & 63;
-  const long v0 = ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
-  if (v0 >= W) return;  // whole wave
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const long v0 = ((long)blockIdx.x * 4 + wv) * 64;
+  if (v0 >= W) return;  // whole wave; the waves of a block share no LDS
   const long w = v0 + lane;
+  // pixel indices and centres fit 32 bits (n_wl <= 2^30): scalar entry tests
+  const int v0i = (int)v0, wi = v0i + lane;
+  const bool on_grid = w < W;
   const long bound = class_window_search(lane, d, v0, v0 + 63, (long)near_reach[d], regions, list_cm);
+  const long base = regions[(long)d * (kClasses + 1)];  // the depth's lists start here
+  const WingParams* __restrict__ lp = list_p + base;
+  uint64_t* queue = s_queue[wv];
+  double(*vals)[64] = s_vals[wv];
   double acc = 0.0;
+  int cnt = 0;  // this pixel's queued slots
+  int qn = 0;   // queued pairs (wave-uniform)
   unsigned long long nev = 0;
+  const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   for (int kk = 0; kk < kClasses; ++kk) {
     const long lo = uniform_long(__shfl(bound, 2 * kk));
     const long hi = uniform_long(__shfl(bound, 2 * kk + 1));
     for (long j = lo; j < hi; ++j) {
       const WingHead h = list_cm[j];
-      if (v0 + 63 < h.nlo || v0 > h.nhi) continue;  // wave-uniform
-      const long c = head_c(h);
-      const long lim = h.nhi - c;
-      const long k = w - c;
-      const long ka = k < 0 ? -k : k;
-      if (ka >= 1 && ka <= lim && w < W) {
-        acc += near_wing(list_p[j], ka, tb.h0, tb.h1, tb.h2);
+      if (v0i + 63 < h.nlo || v0i > h.nhi) continue;  // wave-uniform
+      const int lim = (h.nhi - h.nlo) >> 1;          // the zone's half-width
+      const int c = h.nlo + lim;
+      const int ka = abs(wi - c);
+      const bool act = (unsigned)(ka - 1) < (unsigned)lim && on_grid;  // 1 <= |k| <= lim
+      if (__ballot(act && cnt == kNearSlots)) {  // a pixel is out of slots
+        near_flush(lane, qn, cnt, acc, queue, vals, lp, tb);
+        qn = 0;
+      }
+      const unsigned long long m = __ballot(act);
+      if (act) {
+        queue[qn + __popcll(m & below)] = near_record(j - base, ka, lane, cnt);
+        ++cnt;
         if (kStats) ++nev;
       }
+      qn += __popcll(m);
     }
   }
+  near_flush(lane, qn, cnt, acc, queue, vals, lp, tb);
   if (w < W) out[(long)d * W + w] = acc;
   if (kStats) {
     for (int off = 32; off > 0; off >>= 1) nev += __shfl_down(nev, off);
