@@ -404,6 +404,8 @@ __global__ void k_transp_wing(LinesDev ln, DepthsDev dp, long W, const double* _
   cls[i] = k_out;
 }
 
+constexpr int kListDepths = 8;  // depths per workgroup row of the class count and scatter
+
 // Per (block of 256 centre-sorted lines, depth): pair counts per reach class.
 __global__ __launch_bounds__(kBlock) void k_class_count(long L, int D, const int64_t* __restrict__ order,
                                                         const uint8_t* __restrict__ cls,
@@ -412,17 +414,21 @@ __global__ __launch_bounds__(kBlock) void k_class_count(long L, int D, const int
   long s = blockIdx.x * (long)kBlock + threadIdx.x;
   long l = s < L ? order[s] : -1;
   int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int d = blockIdx.y;  // depths are independent: one per workgroup row
-  int k = l >= 0 ? cls[l * D + d] : 0xFF;
-  for (int kk = 0; kk < kClasses; ++kk) {
-    unsigned long long m = __ballot(k == kk);
-    if (lane == 0) sh[wave][kk] = __popcll(m);
-  }
-  __syncthreads();
-  if (threadIdx.x < kClasses) {
-    int t = 0;
-    for (int q = 0; q < kBlock / 64; ++q) t += sh[q][threadIdx.x];
-    counts[((long)blockIdx.x * D + d) * kClasses + threadIdx.x] = t;
+  // depths are independent: each workgroup row takes kListDepths of them
+  const int d1 = min(D, (int)(blockIdx.y + 1) * kListDepths);
+  for (int d = blockIdx.y * kListDepths; d < d1; ++d) {
+    int k = l >= 0 ? cls[l * D + d] : 0xFF;
+    for (int kk = 0; kk < kClasses; ++kk) {
+      unsigned long long m = __ballot(k == kk);
+      if (lane == 0) sh[wave][kk] = __popcll(m);
+    }
+    __syncthreads();
+    if (threadIdx.x < kClasses) {
+      int t = 0;
+      for (int q = 0; q < kBlock / 64; ++q) t += sh[q][threadIdx.x];
+      counts[((long)blockIdx.x * D + d) * kClasses + threadIdx.x] = t;
+    }
+    __syncthreads();
   }
 }
 
@@ -476,33 +482,37 @@ __global__ __launch_bounds__(kBlock) void k_class_scatter(
   long l = s < L ? order[s] : -1;
   int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   unsigned long long ltmask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const int d = blockIdx.y;  // depths are independent: one per workgroup row
-  int k = l >= 0 ? cls[l * D + d] : 0xFF;
-  int rank = 0;
-  for (int kk = 0; kk < kClasses; ++kk) {
-    unsigned long long m = __ballot(k == kk);
-    if (lane == 0) sh[wave][kk] = __popcll(m);
-    if (k == kk) rank = __popcll(m & ltmask);
-  }
-  __syncthreads();
-  if (k != 0xFF) {
-    int before = 0;
-    for (int q = 0; q < wave; ++q) before += sh[q][k];
-    int64_t pos = regions[d * (kClasses + 1) + k] + offsets[((long)blockIdx.x * D + d) * kClasses + k] + before +
-                  rank;
-    WingParams p = wp[l * D + d];
-    const int r = p.use_far ? p.n10dop : p.maxstep;
-    const bool chunk = !(p.x_far > 0.0 && p.x_far < kChunkMinX);  // X >= 0
-    WingHead h;
-    h.lo = p.use_far ? p.c - p.maxstep : INT32_MAX;
-    h.hi = p.use_far ? p.c + p.maxstep : INT32_MIN;
-    h.nlo = p.c - r;
-    h.nhi = p.c + r;
-    h.xs = p.use_far ? (chunk ? p.x_far : -p.x_far) : 0.0;
-    h.cd = (double)p.c;
-    list_cm[pos] = h;
-    list_p[pos] = p;
-    if (list_l) list_l[pos] = (int32_t)l;
+  // depths are independent: each workgroup row takes kListDepths of them
+  const int d1 = min(D, (int)(blockIdx.y + 1) * kListDepths);
+  for (int d = blockIdx.y * kListDepths; d < d1; ++d) {
+    int k = l >= 0 ? cls[l * D + d] : 0xFF;
+    int rank = 0;
+    for (int kk = 0; kk < kClasses; ++kk) {
+      unsigned long long m = __ballot(k == kk);
+      if (lane == 0) sh[wave][kk] = __popcll(m);
+      if (k == kk) rank = __popcll(m & ltmask);
+    }
+    __syncthreads();
+    if (k != 0xFF) {
+      int before = 0;
+      for (int q = 0; q < wave; ++q) before += sh[q][k];
+      int64_t pos = regions[d * (kClasses + 1) + k] + offsets[((long)blockIdx.x * D + d) * kClasses + k] +
+                    before + rank;
+      WingParams p = wp[l * D + d];
+      const int r = p.use_far ? p.n10dop : p.maxstep;
+      const bool chunk = !(p.x_far > 0.0 && p.x_far < kChunkMinX);  // X >= 0
+      WingHead h;
+      h.lo = p.use_far ? p.c - p.maxstep : INT32_MAX;
+      h.hi = p.use_far ? p.c + p.maxstep : INT32_MIN;
+      h.nlo = p.c - r;
+      h.nhi = p.c + r;
+      h.xs = p.use_far ? (chunk ? p.x_far : -p.x_far) : 0.0;
+      h.cd = (double)p.c;
+      list_cm[pos] = h;
+      list_p[pos] = p;
+      if (list_l) list_l[pos] = (int32_t)l;
+    }
+    __syncthreads();
   }
 }
 
@@ -1925,14 +1935,15 @@ static int asynth_prepare(const synthe_ctx* ctx, const synthe_lines* lines, cons
                        to_dev(depths), n_wl, kapmin, cutoff, 0.0, wl, ctx->tb, 2, nullptr, nullptr, transp, valid,
                        b.wp, b.cls);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_class_count, dim3((unsigned)nblk, (unsigned)D), dim3(kBlock), 0, st, L, D, wing_order,
+    const unsigned rows = (unsigned)((D + kListDepths - 1) / kListDepths);
+    hipLaunchKernelGGL(k_class_count, dim3((unsigned)nblk, rows), dim3(kBlock), 0, st, L, D, wing_order,
                        b.cls, b.cnt);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_class_scan, dim3((unsigned)(D * kClasses)), dim3(kBlock), 0, st, nblk, D, b.cnt, b.tot);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_regions, dim3(blocks_for(D, 64)), dim3(64), 0, st, D, L, b.tot, b.reg);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_class_scatter, dim3((unsigned)nblk, (unsigned)D), dim3(kBlock), 0, st, L, D, wing_order,
+    hipLaunchKernelGGL(k_class_scatter, dim3((unsigned)nblk, rows), dim3(kBlock), 0, st, L, D, wing_order,
                        b.cls, b.cnt,
                        b.reg, b.wp, b.lcm, b.lp, b.list_l);
     HIP_TRY(hipGetLastError());
