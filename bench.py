@@ -294,22 +294,30 @@ def bench_inverse(args) -> dict:
     w = torch.rand(m.W, dtype=torch.float64, device="cuda", generator=torch.Generator("cuda").manual_seed(3))
     (w * m(T, da, lgf, lgw)).sum().backward()
     g_adj = da.grad.cpu().numpy()
-    worst = 0.0
+    g_fd = {}
     for z in INV.LINE_ELEMENTS:
         k = S.ABUNDANCE_Z.index(z)
         up, dn = sol["delta"].copy(), sol["delta"].copy()
         up[:, k] += 1e-3
         dn[:, k] -= 1e-3
-        g_fd = float((w * (prob.evaluate(up)[i0] - prob.evaluate(dn)[i0])).sum()) / 2e-3
-        worst = max(worst, abs(g_adj[k] - g_fd) / max(abs(g_fd), 1e-300))
+        g_fd[z] = float((w * (prob.evaluate(up)[i0] - prob.evaluate(dn)[i0])).sum()) / 2e-3
+    # against the spectrum's largest line-element gradient; per element where
+    # the element constrains the spectrum (sensitivity >= 2e-3 of the median
+    # flux per dex: below that the differences see the forward's steps)
+    big = max(abs(v) for v in g_fd.values())
+    worst = max(abs(g_adj[S.ABUNDANCE_Z.index(z)] - v) for z, v in g_fd.items()) / max(big, 1e-300)
+    per_el = {int(z): abs(g_adj[S.ABUNDANCE_Z.index(z)] - v) / max(abs(v), 1e-300) for z, v in g_fd.items()
+              if sol["sensitivity"][i0, S.ABUNDANCE_Z.index(z)] >= 2e-3}
     log(f"[bench] c5: {sol['iterations']} iterations, {sol['evaluations']} batched evaluations, "
-        f"{sol['seconds']:.2f} s; {rep}; judge {judge:.2e}; adjoint vs batched gradient {worst:.2e}")
+        f"{sol['seconds']:.2f} s; {rep}; judge {judge:.2e}; adjoint vs batched gradient {worst:.2e} "
+        f"(constrained elements {per_el})")
     prob.plan.close()
     return {"seconds_to_converge": sol["seconds"], "iterations": sol["iterations"],
             "batched_evaluations": sol["evaluations"], "ms_per_batched_evaluation": batch_ms,
             "cost_history": sol["cost_history"], **rep,
             "max_rel_flux_err_vs_fp64_plain_path": judge,
-            "max_rel_adjoint_vs_batched_gradient": worst, "gradient_check_spectrum": i0,
+            "max_err_adjoint_vs_batched_gradient_rel_largest": worst,
+            "rel_err_adjoint_vs_batched_gradient_constrained": per_el, "gradient_check_spectrum": i0,
             "setup_s": {"targets": t_targets, "plan": t_plan},
             "workload": "64 targets: atmospheres as the batched configuration, 5,000-point windows in "
                         "400-700 nm, 30,000 lines each, 20 abundance offsets N(0, 0.2) dex; Gauss-Newton "

@@ -257,7 +257,9 @@ int synthe_batch_destroy(synthe_batch *b);
  * cont_scat, line_abs) [D][W], dL/d(B) [D][W] for the Planck function at T(d)
  * (continuum source, and line source when line_source is NULL) and
  * dL/d(line_source) [D][W] (outputs may be NULL).  The scattering fixed point
- * is differentiated implicitly at FP64 convergence.  At most 96 depths. */
+ * and the label-401 iteration are differentiated implicitly at FP64
+ * convergence, O(depths) per wavelength.  At most 96 depths; uses the
+ * context's workspace (one int per wavelength: the processing order). */
 int synthe_asynth_vjp(synthe_ctx *ctx, const synthe_lines *lines, const synthe_depths *depths, long n_wl,
                       const double *wl, const double *transp, const uint8_t *valid, const double *kapmin,
                       double cutoff, const int64_t *center_order, const int64_t *center_start,

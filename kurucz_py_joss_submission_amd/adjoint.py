@@ -16,8 +16,9 @@ HIP library runs the same forward pass as SpectrumPlan (continuum, TRANSP,
 ASYNTH at the chosen profile precision, LTE JOSH, F_lambda).
 
 Backward (no tape of the forward): synthe_lte_vjp differentiates both JOSH
-branches per wavelength (implicit adjoint of the scattering fixed point,
-dual numbers through INTEG/MAP1/DERIV), synthe_asynth_vjp recomputes every
+branches per wavelength in O(depths) (implicit adjoints of the scattering
+fixed point and of the label-401 iteration, colour-seeded dual-number
+Jacobians of PARCOE/MAP1/DERIV, INTEG reversed), synthe_asynth_vjp recomputes every
 profile value and gives per (line, depth) pair dL/dkappa0, dL/da and
 dL/d(Doppler width) through the profile shape; this module chains those to
 the bundle tensors (then torch autograd takes the layer model to T and the
@@ -93,34 +94,49 @@ class LayerModel:
                 sray[e, j] = 5.799e-13 / la ** 4 + 1.422e-6 / la ** 6 + 2.784 / la ** 8
         self.kfac = torch.tensor(kfac, dtype=f8, device=device)
         self.sray = torch.tensor(sray, dtype=f8, device=device)
+        # every element at once: stages 0..ns-1 padded to the largest count
+        nz, ns = len(self.Z), 1 + max(len(S.ELEMENTS[z]["chi"]) for z in self.Z)
+        chi = np.zeros((nz, ns - 1))
+        u0 = np.ones((nz, ns))
+        up = np.zeros((nz, ns))
+        live = np.zeros((nz, ns), bool)
+        for iz, z in enumerate(self.Z):
+            c = S.ELEMENTS[z]["chi"]
+            chi[iz, :len(c)] = c
+            live[iz, :len(c) + 1] = True
+            for st, (a, b) in enumerate(S.ELEMENTS[z]["u"][:ns]):
+                u0[iz, st], up[iz, st] = a, b
+        self.chi = torch.tensor(chi, dtype=f8, device=device)[:, :, None]
+        self.u0 = torch.tensor(u0, dtype=f8, device=device)[:, :, None]
+        self.up = torch.tensor(up, dtype=f8, device=device)[:, :, None]
+        self.live = torch.tensor(live, device=device)[:, :, None]
+        self.charge = torch.arange(ns, dtype=f8, device=device)[None, :, None]
+        # (element, stage) pairs stored in the bundle: stage < N_ION
+        zi, si = np.nonzero(live[:, :S.N_ION])
+        self.pair_z = torch.tensor(zi, dtype=torch.int64, device=device)
+        self.pair_s = torch.tensor(si, dtype=torch.int64, device=device)
+        self.pair_el = torch.tensor(np.asarray(self.Z)[zi] - 1, dtype=torch.int64, device=device)
+        self.iz_h, self.iz_he = self.Z.index(1), self.Z.index(2)
 
-    def _partition(self, z, stage, T):
-        us = S.ELEMENTS[z]["u"]
-        u0, p = us[stage] if stage < len(us) else (1.0, 0.0)
-        return u0 * (T / 5000.0) ** p
+    def _partitions(self, T):
+        """U [element][stage][depth] (u0 (T / 5000)^p; 1 past an element's table)."""
+        return self.u0 * (T / 5000.0)[None, None, :] ** self.up
 
-    def _fractions(self, z, T, ne):
+    def _fractions(self, T, ne, U):
+        """Stage fractions [element][stage][depth] (0 past an element's last stage):
+        synthetic._saha_fractions for all elements at once."""
         torch = _torch()
-        chi = S.ELEMENTS[z]["chi"]
-        logr = [torch.zeros_like(T)]
-        for i, x in enumerate(chi):
-            ratio = (S.SAHA_CONST * T ** 1.5 / ne * (self._partition(z, i + 1, T) / self._partition(z, i, T))
-                     * torch.exp(-x / (S.K_EV * T)))
-            logr.append(logr[-1] + torch.log(ratio))
-        lr = torch.stack(logr)
-        lr = lr - lr.max(dim=0).values
+        ratio = (S.SAHA_CONST * T ** 1.5 / ne)[None, None, :] * (U[:, 1:] / U[:, :-1]) * torch.exp(
+            -self.chi / (S.K_EV * T)[None, None, :])
+        logr = torch.cat([torch.zeros_like(U[:, :1]), torch.cumsum(torch.log(ratio), dim=1)], dim=1)
+        logr = torch.where(self.live, logr, torch.full_like(logr, -math.inf))
+        lr = logr - logr.max(dim=1, keepdim=True).values
         w = torch.exp(lr)
-        return w / w.sum(dim=0)
+        return w / w.sum(dim=1, keepdim=True)
 
     def _supply(self, T, ne, ab, ntot):
-        asum = ab.sum()
-        nnuc = ntot - ne
-        sup = 0.0
-        for iz, z in enumerate(self.Z):
-            fr = self._fractions(z, T, ne)
-            charge = _torch().arange(fr.shape[0], dtype=fr.dtype, device=fr.device)[:, None]
-            sup = sup + ab[iz] / asum * nnuc * (charge * fr).sum(dim=0)
-        return sup
+        fr = self._fractions(T, ne, self._partitions(T))
+        return ((ab / ab.sum())[:, None] * (self.charge * fr).sum(dim=1)).sum(dim=0) * (ntot - ne)
 
     def electron_density(self, T, ab):
         torch = _torch()
@@ -153,22 +169,15 @@ class LayerModel:
         mu = (ab * self.mass).sum() / asum
         rho = nnuc * mu * AMU
         D = T.shape[0]
+        U = self._partitions(T)
+        n = (ab / asum)[:, None, None] * nnuc[None, None, :] * self._fractions(T, ne, U)  # [el][stage][D]
+        nh1, he1 = n[self.iz_h, 0], n[self.iz_he, 0]
+        dopz = torch.sqrt(2.0 * K_BOLTZ * T[None, :] / (self.mass[:, None] * AMU)
+                          + (self.vt * self.vt)[None, :]) / C_LIGHT_CM  # [el][D]
         pop = torch.zeros((D, S.N_ION, S.N_ELEMENTS), dtype=T.dtype, device=T.device)
         dop = torch.zeros_like(pop)
-        nh1 = he1 = None
-        for iz, z in enumerate(self.Z):
-            fr = self._fractions(z, T, ne)
-            nz = ab[iz] / asum * nnuc
-            for s in range(fr.shape[0]):
-                n_s = nz * fr[s]
-                if z == 1 and s == 0:
-                    nh1 = n_s
-                if z == 2 and s == 0:
-                    he1 = n_s
-                if s < S.N_ION:
-                    pop[:, s, z - 1] = n_s / self._partition(z, s, T)
-                    dop[:, s, z - 1] = torch.sqrt(2.0 * K_BOLTZ * T / (S.ELEMENTS[z]["mass"] * AMU)
-                                                  + self.vt * self.vt) / C_LIGHT_CM
+        pop[:, self.pair_s, self.pair_el] = (n[self.pair_z, self.pair_s] / U[self.pair_z, self.pair_s]).T
+        dop[:, self.pair_s, self.pair_el] = dopz[self.pair_z].T
         hckt = H_PLANCK * C_LIGHT_CM / (K_BOLTZ * T)
         txnxn = (nh1 + 0.42 * he1) * (torch.clamp(T, min=1.0) / 10_000.0) ** 0.3
         kr = 0.25 * (self.P / 1.0e5) ** 0.6 * (T / 5800.0) ** 6.0 + 2.0e-4
@@ -225,9 +234,11 @@ class SpectrumModel:
         cw = np.stack([(x - h) * (x - right) / dl, (left - x) * (x - right) * 2.0 / dl,
                        (x - left) * (x - h) / dl], axis=1)
         self.cont_w = torch.tensor(cw, dtype=torch.float64, device="cuda")
-        onehot = np.zeros((self.W, S.WLEDGE.size - 1))
-        onehot[np.arange(self.W), e] = 1.0
-        self.cont_onehot = torch.tensor(onehot, dtype=torch.float64, device="cuda")  # pixel -> edge interval
+        # pixels of one edge interval are contiguous (the grid ascends): (interval, first, end)
+        cuts = np.flatnonzero(np.diff(e)) + 1
+        starts = np.concatenate([[0], cuts]) if self.W else np.zeros(0, np.int64)
+        ends = np.concatenate([cuts, [self.W]]) if self.W else np.zeros(0, np.int64)
+        self.cont_segs = [(int(e[a]), int(a), int(b)) for a, b in zip(starts, ends)]
         valid = (soa.process != 0) & (soa.elem >= 0)
         ion = np.where(valid, soa.ion - 1, 0)
         el = np.where(valid, soa.elem, 0)
@@ -236,9 +247,11 @@ class SpectrumModel:
         # species (ion, element) of the lines: one-hot [L][n_species] for the per-species sums
         key = ion * S.N_ELEMENTS + el
         self.species, sid = np.unique(key, return_inverse=True)
-        oh = np.zeros((L, self.species.size))
-        oh[np.arange(L), sid] = 1.0
-        self.species_onehot = torch.tensor(oh, dtype=torch.float64, device="cuda")
+        # lines ordered by species, so each species' lines are one slice
+        order = np.argsort(sid, kind="stable")
+        self.species_order = torch.tensor(order, dtype=torch.int64, device="cuda")
+        bounds = np.searchsorted(sid[order], np.arange(self.species.size + 1))
+        self.species_slices = [(int(a), int(b)) for a, b in zip(bounds[:-1], bounds[1:])]
         self.species_ion = torch.tensor(self.species // S.N_ELEMENTS, dtype=torch.int64, device="cuda")
         self.species_el = torch.tensor(self.species % S.N_ELEMENTS, dtype=torch.int64, device="cuda")
 
@@ -376,9 +389,9 @@ def _make_function():
             for src_g, val, out in ((g_ca, ca, g_cabs), (g_cs, cs, g_cscat)):
                 ok = (val > 0.0) & (val < 3.4e38)
                 gl = torch.where(ok, src_g * val * LN10, torch.zeros_like(val))  # dL/d(log10 kappa) per pixel
-                contrib = gl[:, :, None] * m.cont_w[None, :, :]  # [D][W][3]
-                # sum over each edge interval's pixels: [D][3][W] @ [W][E-1]
-                out += torch.matmul(contrib.permute(0, 2, 1), m.cont_onehot).permute(0, 2, 1)
+                # sum over each edge interval's pixels (a contiguous slice)
+                for e, w0, w1 in m.cont_segs:
+                    out[:, e, :] += (gl[:, w0:w1, None] * m.cont_w[None, w0:w1, :]).sum(dim=1)
             # line opacity
             g_pop = torch.zeros_like(pop)
             g_dop = torch.zeros_like(dop)
@@ -414,9 +427,11 @@ def _make_function():
                 gdv = -gk / dv - a / dv * s_a + s_dv
                 g_xne = ((m.t["gamma_stark"][:, None] / dv) * s_a).sum(dim=0)
                 g_txnxn = ((gvdw[:, None] / dv) * s_a).sum(dim=0)
-                # per-species sums over lines: [n_species][L] @ [L][D]
-                sp_pop = torch.matmul(m.species_onehot.T, gpp)
-                sp_dop = torch.matmul(m.species_onehot.T, gdv)
+                # per-species sums over lines (slices of the species order)
+                gpp_s = gpp.index_select(0, m.species_order)
+                gdv_s = gdv.index_select(0, m.species_order)
+                sp_pop = torch.stack([gpp_s[a:b].sum(dim=0) for a, b in m.species_slices])
+                sp_dop = torch.stack([gdv_s[a:b].sum(dim=0) for a, b in m.species_slices])
                 g_pop[:, m.species_ion, m.species_el] = sp_pop.T
                 g_dop[:, m.species_ion, m.species_el] = sp_dop.T
             return (g_pop, g_dop, g_rho, g_xne, g_txnxn, g_hckt, g_T, g_cabs, g_cscat, g_lgf, g_lgw, None)

@@ -8,20 +8,26 @@
 // XALPHA (COEFJ . XS) and flux = CH . XS -- or, when TAUNU(0) > XTAU(51),
 // the label-401 DERIV iteration on the depth grid with flux = HNU(0).
 //
-// Derivatives:
+// Derivatives (reverse mode, O(n) per column):
 //  - the scattering fixed point is differentiated implicitly: with M = I -
 //    diag(XA) C, the adjoint lam solves M^T lam = CH (a transposed sweep
 //    iteration in FP64), giving dF/dXB_k = lam_k (1 - XA_k) and dF/dXA_k =
 //    lam_k ((C XS)_k - XB_k) at the FP64 fixed point XS;
-//  - INTEG, PARCOE and MAP1 (whose smoothing weights depend on the data) are
-//    differentiated in forward mode with dual numbers, one input at a time,
-//    and contracted with those sensitivities;
-//  - the label-401 branch is differentiated in forward mode through its
-//    iteration, run (values and derivatives) to a relative change of 1e-12
-//    instead of the reference's 1e-5, so that both branches differentiate
-//    the converged solve.
-// Branch choices (label 401, the surface set XTAU < TAUNU(0), clamps) are
-// held fixed, as their derivative is zero almost everywhere.
+//  - PARCOE and MAP1 (whose smoothing weights depend on the data) are local:
+//    an output depends on at most 4 consecutive inputs.  Their Jacobians are
+//    therefore exact from 4 dual-number passes each, seeding every input
+//    j = r mod 4 in pass r (no output sees two seeded inputs), and are
+//    contracted with the sensitivities; INTEG's running sum is reversed as a
+//    suffix sum;
+//  - the label-401 branch is differentiated implicitly at its converged
+//    iterate S = PHI(S) = (1 - alpha) snubar + alpha (J(S) + S), where J is
+//    two DERIVs: PHI's Jacobians in S and TAUNU are 5-banded (5 dual passes
+//    each) and the adjoint solves (I - dPHI/dS)^T lam = dF/dS by the same
+//    (transposed) iteration.  The value iteration runs to a relative change
+//    of 1e-12 instead of the reference's 1e-5, so that both branches
+//    differentiate the converged solve.
+// Branch choices (label 401, the surface set XTAU < TAUNU(0), clamps, MAP1
+// brackets) are held fixed, as their derivative is zero almost everywhere.
 //
 // The header compiles as HIP device code and as plain C++ (for CPU checks).
 #pragma once
@@ -38,7 +44,7 @@ constexpr double kEps = 1.0e-38;
 
 struct Dual {
   double v, d;
-  SYNTHE_HD Dual() : v(0.0), d(0.0) {}
+  Dual() = default;  // trivial, so work arrays of duals cost nothing to create
   SYNTHE_HD Dual(double x) : v(x), d(0.0) {}
   SYNTHE_HD Dual(double x, double y) : v(x), d(y) {}
 };
@@ -172,8 +178,10 @@ SYNTHE_HD void deriv_t(const T* x, const T* f, int n, T* dfdx) {
 }
 
 // physics/josh_solver.py:220-353 (_map1_kernel) onto the 51 XTAU points.
+// base[k] (optional): the 1-based bracket index L at which fnew[k]'s parabola
+// was formed; fnew[k] depends on xold and fold at 0-based L-3 .. L only.
 template <class T>
-SYNTHE_HD void map1_t(const T* xold, const T* fold, int nold, const double* xnew, T* fnew) {
+SYNTHE_HD void map1_t(const T* xold, const T* fold, int nold, const double* xnew, T* fnew, int* base = nullptr) {
 #define XO(i) xold[(i)-1]
 #define FO(i) fold[(i)-1]
   for (int k = 0; k < kXt; ++k) fnew[k] = T(0.0);
@@ -244,6 +252,7 @@ SYNTHE_HD void map1_t(const T* xold, const T* fold, int nold, const double* xnew
       }
     }
     fnew[k - 1] = a + (b + c * xk) * xk;
+    if (base) base[k - 1] = ll;
   }
 #undef XO
 #undef FO
@@ -286,10 +295,19 @@ struct JoshTab {
   const float* coefj32;  // [51][51], the float32 COEFJ of the sweep
 };
 
+// A vector of XTAU values with a stride: a private array (stride 1) or one
+// lane's column of a workgroup array interleaved by lane (stride = lanes).
+struct Vec {
+  double* p;
+  int s;
+  SYNTHE_HD double& operator[](int k) const { return p[k * s]; }
+};
+
 // FP64 scattering fixed point XS = XB (1 - XA) + XA (C XS) by Gauss-Seidel
 // sweeps (descending k, as physics/josh_solver.py:24-92) to relative change
-// tol; returns the sweep count.
-SYNTHE_HD int fixed_point64(const JoshTab& tb, const double* xb, const double* xa, double* xs, double tol,
+// tol; returns the sweep count.  V: double* or Vec.
+template <class V>
+SYNTHE_HD int fixed_point64(const JoshTab& tb, const double* xb, const V& xa, const V& xs, double tol,
                             int max_sweeps) {
   for (int k = 0; k < kXt; ++k) xs[k] = xb[k];
   for (int it = 0; it < max_sweeps; ++it) {
@@ -309,7 +327,8 @@ SYNTHE_HD int fixed_point64(const JoshTab& tb, const double* xb, const double* x
 }
 
 // Adjoint of the fixed point: lam = CH + C^T (XA o lam), Gauss-Seidel.
-SYNTHE_HD int adjoint64(const JoshTab& tb, const double* xa, double* lam, double tol, int max_sweeps) {
+template <class V>
+SYNTHE_HD int adjoint64(const JoshTab& tb, const V& xa, const V& lam, double tol, int max_sweeps) {
   for (int k = 0; k < kXt; ++k) lam[k] = tb.ch[k];
   for (int it = 0; it < max_sweeps; ++it) {
     double err = 0.0;
@@ -335,13 +354,33 @@ struct Column {
   double rho[kMax], abtot[kMax], alpha[kMax], snubar[kMax];
 };
 
-// Work space of the solve (caller-provided, e.g. private memory).
+// Work space of the solve (caller-provided, e.g. private memory); the two
+// branches' arrays share storage, and d0..d3 are the dual-number arrays of
+// whichever pass runs.
 template <int kMax>
 struct Work {
-  double taunu[kMax], pa[kMax], pb[kMax], pc[kMax];
-  double xsbar[kXt], xalpha[kXt], xb[kXt], xa[kXt], xs[kXt], lam[kXt], gxb[kXt], gxa[kXt];
-  Dual d_in[kMax], d_tau[kMax], d_a[kMax], d_b[kMax], d_c[kMax], d_f[kMax], d_out[kXt], d_out2[kXt];
-  Dual d_s[kMax], d_h[kMax], d_j[kMax], d_t[kMax], d_al[kMax], d_sb[kMax];
+  double taunu[kMax], pa[kMax], pb[kMax], pc[kMax], g_tau[kMax];
+  struct Xtau {
+    double xsbar[kXt], xalpha[kXt], xb[kXt], xa[kXt], xs[kXt], lam[kXt], gxb[kXt], gxa[kXt];
+    int base[kXt];  // MAP1 bracket of each XTAU point
+    Dual out1[kXt], out2[kXt];
+  };
+  struct Label401 {
+    double lam[kMax], tmp[kMax], band[kMax][5];
+  };
+  union {
+    Xtau x;
+    Label401 s;
+  };
+  Dual d0[kMax], d1[kMax], d2[kMax], d3[kMax];
+  // the fixed point's vectors XS, XA and its adjoint: x's own arrays after
+  // bind_private(), or storage the caller binds (e.g. workgroup memory)
+  Vec xs, xa, lam;
+  SYNTHE_HD void bind_private() {
+    xs = Vec{x.xs, 1};
+    xa = Vec{x.xa, 1};
+    lam = Vec{x.lam, 1};
+  }
 };
 
 // FP64 total flux of the column with the fixed point solved to tol (the
@@ -354,102 +393,188 @@ SYNTHE_HD double total_flux64(const Column<kMax>& col, const JoshTab& tb, Work<k
   integ_t(col.abtot, col.rho, n, w.taunu, w.pa, w.pb, w.pc);
   if (w.taunu[0] > tb.xtau[kXt - 1]) {
     *took401 = 1;
-    return josh401_t(n, w.taunu, col.snubar, col.alpha, w.pa, w.pb, w.pc, w.xs, nullptr, kTol401, kMax401);
+    return josh401_t(n, w.taunu, col.snubar, col.alpha, w.pa, w.pb, w.pc, w.g_tau, nullptr, kTol401, kMax401);
   }
   *took401 = 0;
-  map1_t(w.taunu, col.snubar, n, tb.xtau, w.xsbar);
-  map1_t(w.taunu, col.alpha, n, tb.xtau, w.xalpha);
+  typename Work<kMax>::Xtau& x = w.x;
+  map1_t(w.taunu, col.snubar, n, tb.xtau, x.xsbar, x.base);
+  map1_t(w.taunu, col.alpha, n, tb.xtau, x.xalpha);
   const double s0 = col.snubar[0] > kEps ? col.snubar[0] : kEps;
   const double a0 = col.alpha[0] < 0.0 ? 0.0 : (col.alpha[0] > 1.0 ? 1.0 : col.alpha[0]);
   for (int k = 0; k < kXt; ++k) {
-    double xb = w.xsbar[k] > kEps ? w.xsbar[k] : kEps;
-    double xa = w.xalpha[k] < 0.0 ? 0.0 : (w.xalpha[k] > 1.0 ? 1.0 : w.xalpha[k]);
+    double xb = x.xsbar[k] > kEps ? x.xsbar[k] : kEps;
+    double xa = x.xalpha[k] < 0.0 ? 0.0 : (x.xalpha[k] > 1.0 ? 1.0 : x.xalpha[k]);
     if (tb.xtau[k] < w.taunu[0]) {
       xb = s0;
       xa = a0;
     }
-    w.xb[k] = xb;
+    x.xb[k] = xb;
     w.xa[k] = xa;
   }
-  fixed_point64(tb, w.xb, w.xa, w.xs, tol, 400);
+  fixed_point64(tb, x.xb, w.xa, w.xs, tol, 400);
   double f = 0.0;
   for (int k = 0; k < kXt; ++k) f += tb.ch[k] * w.xs[k];
   return f;
 }
 
-// g_abtot, g_alpha, g_snubar [n] = v * dF/d(input) for the inputs j = part,
-// part + nparts, ... (all of them by default; others are left untouched);
-// returns F.
+// The index j = r (mod period) in [lo, lo + period).
+SYNTHE_HD int seeded_in(int lo, int r, int period) { return lo + ((r - lo) % period + period) % period; }
+
+// g_abtot[j] += dF/dabtot[j] given g_tau[m] = dF/dTAUNU[m] (overwritten by
+// its suffix sums).  TAUNU[0] = abtot[0] rho[0] and TAUNU[i+1] = TAUNU[i] +
+// u_i, where u_i depends on abtot[i-1 .. i+2] through PARCOE.
+template <int kMax>
+SYNTHE_HD void integ_vjp(const Column<kMax>& col, Work<kMax>& w, double* g_abtot) {
+  const int n = col.n;
+  for (int i = n - 2; i >= 0; --i) w.g_tau[i] += w.g_tau[i + 1];
+  g_abtot[0] += w.g_tau[0] * col.rho[0];
+  for (int r = 0; r < 4; ++r) {
+    for (int i = 0; i < n; ++i) w.d0[i] = Dual(col.abtot[i], (i & 3) == r ? 1.0 : 0.0);
+    parcoe_t(w.d0, col.rho, n, w.d1, w.d2, w.d3);
+    for (int i = 0; i + 1 < n; ++i) {
+      const int j = seeded_in(i - 1, r, 4);
+      if (j < 0 || j >= n) continue;
+      const double r0 = col.rho[i], r1 = col.rho[i + 1];
+      // the derivative part of INTEG's increment (pa + pb (r0 + r1) / 2 + pc (...) / 3) (r1 - r0)
+      const double dterm = w.d1[i].d + 0.5 * w.d2[i].d * (r1 + r0) +
+                           (w.d3[i].d / 3.0) * ((r1 + r0) * r1 + r0 * r0);
+      g_abtot[j] += w.g_tau[i + 1] * dterm * (r1 - r0);
+    }
+  }
+}
+
+// One label-401 update PHI(S) = (1 - alpha) snubar + alpha (J(S) + S) with
+// J = DERIV(TAUNU, DERIV(TAUNU, S) / 3) in dual numbers, TAUNU in d0 and S
+// in d1: d3 <- J(S).
+template <int kMax>
+SYNTHE_HD void josh401_j(int n, Work<kMax>& w) {
+  deriv_t(w.d0, w.d1, n, w.d2);
+  for (int j = 0; j < n; ++j) w.d2[j] = w.d2[j] / 3.0;
+  deriv_t(w.d0, w.d2, n, w.d3);
+}
+
+// g_abtot, g_alpha, g_snubar [n] = v * dF/d(input) for every input; returns F.
 template <int kMax>
 SYNTHE_HD double total_flux_vjp(const Column<kMax>& col, const JoshTab& tb, Work<kMax>& w, double v,
-                                double* g_abtot, double* g_alpha, double* g_snubar, int part = 0,
-                                int nparts = 1) {
+                                double* g_abtot, double* g_alpha, double* g_snubar) {
   const int n = col.n;
   int took401 = 0;
   const double F = total_flux64(col, tb, w, 1e-13, &took401);
+  for (int i = 0; i < n; ++i) g_abtot[i] = g_alpha[i] = g_snubar[i] = w.g_tau[i] = 0.0;
   if (took401) {
-    // forward mode through INTEG and the label-401 iteration, one input at a time
-    for (int q = 0; q < 3; ++q) {
-      double* g = q == 0 ? g_abtot : (q == 1 ? g_alpha : g_snubar);
-      for (int j = part; j < n; j += nparts) {
-        for (int i = 0; i < n; ++i) {
-          w.d_in[i] = Dual(col.abtot[i], q == 0 && i == j ? 1.0 : 0.0);
-          w.d_al[i] = Dual(col.alpha[i], q == 1 && i == j ? 1.0 : 0.0);
-          w.d_sb[i] = Dual(col.snubar[i], q == 2 && i == j ? 1.0 : 0.0);
-        }
-        integ_t(w.d_in, col.rho, n, w.d_tau, w.d_a, w.d_b, w.d_c);
-        Dual f = josh401_t(n, w.d_tau, w.d_sb, w.d_al, w.d_s, w.d_h, w.d_j, w.d_t, nullptr, kTol401, kMax401);
-        g[j] = v * f.d;
+    // converged iterate S (w.pa); F = (S[1] - S[0]) / (TAUNU[1] - TAUNU[0]) / 3
+    const double* S = w.pa;
+    const double dt = w.taunu[1] - w.taunu[0];
+    const double dS = S[1] - S[0];
+    w.g_tau[0] = dS / (3.0 * dt * dt);
+    w.g_tau[1] = -dS / (3.0 * dt * dt);
+    // band[j][o] = dPHI_j / dS[j - 2 + o]
+    for (int j = 0; j < n; ++j) {
+      for (int o = 0; o < 5; ++o) w.s.band[j][o] = 0.0;
+      w.d0[j] = Dual(w.taunu[j]);
+    }
+    for (int r = 0; r < 5; ++r) {
+      for (int i = 0; i < n; ++i) w.d1[i] = Dual(S[i], i % 5 == r ? 1.0 : 0.0);
+      josh401_j(n, w);
+      for (int j = 0; j < n; ++j) {
+        const int i = seeded_in(j - 2, r, 5);
+        if (i >= 0 && i < n) w.s.band[j][i - j + 2] = col.alpha[j] * (w.d3[j].d + w.d1[j].d);
       }
     }
-    return F;
-  }
-  // sensitivities of F to the fixed point's inputs XB, XA
-  adjoint64(tb, w.xa, w.lam, 1e-13, 400);
-  double gs0 = 0.0, ga0 = 0.0;
-  for (int k = 0; k < kXt; ++k) {
-    double cx = 0.0;
-    for (int m = 0; m < kXt; ++m) cx += (double)tb.coefj32[k * kXt + m] * w.xs[m];
-    double gxb = w.lam[k] * (1.0 - w.xa[k]);
-    double gxa = w.lam[k] * (cx - w.xb[k]);
-    if (tb.xtau[k] < w.taunu[0]) {
-      gs0 += gxb;
-      ga0 += gxa;
-      gxb = gxa = 0.0;
-    } else {
-      if (!(w.xsbar[k] > kEps)) gxb = 0.0;
-      if (w.xalpha[k] < 0.0 || w.xalpha[k] > 1.0) gxa = 0.0;
+    // dPHI_j / dalpha_j = J(S)_j + S_j - snubar_j (values of the last pass)
+    for (int j = 0; j < n; ++j) w.s.tmp[j] = w.d3[j].v + S[j] - col.snubar[j];
+    for (int j = 0; j < n; ++j) g_alpha[j] = w.s.tmp[j];
+    // lam = dF/dS + (dPHI/dS)^T lam (the forward's iteration transposed)
+    for (int i = 0; i < n; ++i) w.s.lam[i] = 0.0;
+    w.s.lam[0] = -1.0 / (3.0 * dt);
+    w.s.lam[1] = 1.0 / (3.0 * dt);
+    for (int it = 0; it < 4 * kMax401; ++it) {
+      double err = 0.0;
+      for (int i = 0; i < n; ++i) {
+        double s = i == 0 ? -1.0 / (3.0 * dt) : (i == 1 ? 1.0 / (3.0 * dt) : 0.0);
+        for (int j = i - 2; j <= i + 2; ++j)
+          if (j >= 0 && j < n) s += w.s.band[j][i - j + 2] * w.s.lam[j];
+        w.s.tmp[i] = s;
+      }
+      for (int i = 0; i < n; ++i) {
+        const double den = fabs(w.s.tmp[i]) > kEps ? fabs(w.s.tmp[i]) : kEps;
+        err += fabs(w.s.tmp[i] - w.s.lam[i]) / den;
+        w.s.lam[i] = w.s.tmp[i];
+      }
+      if (err < 1e-13) break;
     }
-    w.gxb[k] = gxb;
-    w.gxa[k] = gxa;
+    for (int j = 0; j < n; ++j) {
+      g_snubar[j] = w.s.lam[j] * (1.0 - col.alpha[j]);
+      g_alpha[j] *= w.s.lam[j];
+    }
+    // TAUNU enters PHI through both DERIVs (5-banded)
+    for (int i = 0; i < n; ++i) w.d1[i] = Dual(S[i]);
+    for (int r = 0; r < 5; ++r) {
+      for (int i = 0; i < n; ++i) w.d0[i] = Dual(w.taunu[i], i % 5 == r ? 1.0 : 0.0);
+      josh401_j(n, w);
+      for (int j = 0; j < n; ++j) {
+        const int i = seeded_in(j - 2, r, 5);
+        if (i >= 0 && i < n) w.g_tau[i] += w.s.lam[j] * col.alpha[j] * w.d3[j].d;
+      }
+    }
+  } else {
+    // sensitivities of F to the fixed point's inputs XB, XA
+    adjoint64(tb, w.xa, w.lam, 1e-13, 400);
+    double gs0 = 0.0, ga0 = 0.0;
+    for (int k = 0; k < kXt; ++k) {
+      double cx = 0.0;
+      for (int m = 0; m < kXt; ++m) cx += (double)tb.coefj32[k * kXt + m] * w.xs[m];
+      double gxb = w.lam[k] * (1.0 - w.xa[k]);
+      double gxa = w.lam[k] * (cx - w.x.xb[k]);
+      if (tb.xtau[k] < w.taunu[0]) {
+        gs0 += gxb;
+        ga0 += gxa;
+        gxb = gxa = 0.0;
+      } else {
+        if (!(w.x.xsbar[k] > kEps)) gxb = 0.0;
+        if (w.x.xalpha[k] < 0.0 || w.x.xalpha[k] > 1.0) gxa = 0.0;
+      }
+      w.x.gxb[k] = gxb;
+      w.x.gxa[k] = gxa;
+    }
+    if (!(col.snubar[0] > kEps)) gs0 = 0.0;
+    if (col.alpha[0] < 0.0 || col.alpha[0] > 1.0) ga0 = 0.0;
+    g_snubar[0] = gs0;
+    g_alpha[0] = ga0;
+    // MAP1 of snubar and alpha on the nodes TAUNU: values, then nodes
+    for (int r = 0; r < 4; ++r) {
+      for (int i = 0; i < n; ++i) {
+        w.d0[i] = Dual(w.taunu[i]);
+        w.d1[i] = Dual(col.snubar[i], (i & 3) == r ? 1.0 : 0.0);
+      }
+      map1_t(w.d0, w.d1, n, tb.xtau, w.x.out1);
+      for (int i = 0; i < n; ++i) w.d1[i] = Dual(col.alpha[i], (i & 3) == r ? 1.0 : 0.0);
+      map1_t(w.d0, w.d1, n, tb.xtau, w.x.out2);
+      for (int k = 0; k < kXt; ++k) {
+        const int j = seeded_in(w.x.base[k] - 3, r, 4);
+        if (j < 0 || j >= n) continue;
+        g_snubar[j] += w.x.gxb[k] * w.x.out1[k].d;
+        g_alpha[j] += w.x.gxa[k] * w.x.out2[k].d;
+      }
+      for (int i = 0; i < n; ++i) {
+        w.d0[i] = Dual(w.taunu[i], (i & 3) == r ? 1.0 : 0.0);
+        w.d1[i] = Dual(col.snubar[i]);
+      }
+      map1_t(w.d0, w.d1, n, tb.xtau, w.x.out1);
+      for (int i = 0; i < n; ++i) w.d1[i] = Dual(col.alpha[i]);
+      map1_t(w.d0, w.d1, n, tb.xtau, w.x.out2);
+      for (int k = 0; k < kXt; ++k) {
+        const int j = seeded_in(w.x.base[k] - 3, r, 4);
+        if (j < 0 || j >= n) continue;
+        w.g_tau[j] += w.x.gxb[k] * w.x.out1[k].d + w.x.gxa[k] * w.x.out2[k].d;
+      }
+    }
   }
-  if (!(col.snubar[0] > kEps)) gs0 = 0.0;
-  if (col.alpha[0] < 0.0 || col.alpha[0] > 1.0) ga0 = 0.0;
-  // snubar and alpha enter MAP1 as values on fixed nodes
-  for (int i = 0; i < n; ++i) w.d_tau[i] = Dual(w.taunu[i]);
-  for (int j = part; j < n; j += nparts) {
-    for (int i = 0; i < n; ++i) w.d_f[i] = Dual(col.snubar[i], i == j ? 1.0 : 0.0);
-    map1_t(w.d_tau, w.d_f, n, tb.xtau, w.d_out);
-    double s = j == 0 ? gs0 : 0.0;
-    for (int k = 0; k < kXt; ++k) s += w.gxb[k] * w.d_out[k].d;
-    g_snubar[j] = v * s;
-    for (int i = 0; i < n; ++i) w.d_f[i] = Dual(col.alpha[i], i == j ? 1.0 : 0.0);
-    map1_t(w.d_tau, w.d_f, n, tb.xtau, w.d_out);
-    s = j == 0 ? ga0 : 0.0;
-    for (int k = 0; k < kXt; ++k) s += w.gxa[k] * w.d_out[k].d;
-    g_alpha[j] = v * s;
-  }
-  // abtot moves the nodes TAUNU
-  for (int j = part; j < n; j += nparts) {
-    for (int i = 0; i < n; ++i) w.d_in[i] = Dual(col.abtot[i], i == j ? 1.0 : 0.0);
-    integ_t(w.d_in, col.rho, n, w.d_tau, w.d_a, w.d_b, w.d_c);
-    for (int i = 0; i < n; ++i) w.d_f[i] = Dual(col.snubar[i]);
-    map1_t(w.d_tau, w.d_f, n, tb.xtau, w.d_out);
-    for (int i = 0; i < n; ++i) w.d_f[i] = Dual(col.alpha[i]);
-    map1_t(w.d_tau, w.d_f, n, tb.xtau, w.d_out2);
-    double s = 0.0;
-    for (int k = 0; k < kXt; ++k) s += w.gxb[k] * w.d_out[k].d + w.gxa[k] * w.d_out2[k].d;
-    g_abtot[j] = v * s;
+  integ_vjp(col, w, g_abtot);
+  for (int i = 0; i < n; ++i) {
+    g_abtot[i] *= v;
+    g_alpha[i] *= v;
+    g_snubar[i] *= v;
   }
   return F;
 }

@@ -1755,25 +1755,19 @@ __global__ void k_far_finish(int D, const int64_t* __restrict__ regions, const W
 // through engine/radiative.py:42-218 masks): v[w] = dL/dF_nu(w); outputs
 // dL/d(cont_abs, cont_scat, line_abs) [D][W], dL/dB (the Planck function at
 // T(d) used as the continuum source) and dL/d(line_source), 0 on masked depths.
-constexpr int kVjpParts = 8;  // lanes sharing one wavelength's inputs (depth j = part mod 8)
+// One lane per wavelength (the vjp is O(depths) per column).  The 51x51
+// loops of the scattering fixed point and its adjoint read COEFJ and the
+// vectors XS, XA, lam for every product: COEFJ is staged in LDS once per
+// workgroup and the three vectors live in LDS interleaved by lane (88 KB per
+// 64-lane workgroup), the rest in private memory.
+constexpr int kVjpBlock = 64;
 
-__global__ __launch_bounds__(64) void k_rt_vjp(int D, long W, const double* __restrict__ wl,
-                                               const double* __restrict__ temp, const double* __restrict__ rhox,
-                                               const double* __restrict__ ca, const double* __restrict__ cs,
-                                               const double* __restrict__ la, const double* __restrict__ ls,
-                                               const double* __restrict__ src, const double* __restrict__ v,
-                                               DeviceTables tb, double* __restrict__ g_ca, double* __restrict__ g_cs,
-                                               double* __restrict__ g_la, double* __restrict__ g_b,
-                                               double* __restrict__ g_src) {
-  // lane -> (part, wavelength); outputs were zeroed by the caller
-  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (t >= W * kVjpParts) return;
-  const int part = (int)(t / W);
-  const long w = t - (long)part * W;
-  const double vw = v[w];
-  if (vw == 0.0) return;
+// The depths of wavelength w that enter the solve, in increasing rho (the
+// masks of engine/radiative.py:42-218): writes idx[0..n), returns n.
+__device__ int vjp_depths(int D, long W, long w, const double* __restrict__ rhox, const double* __restrict__ ca,
+                          const double* __restrict__ cs, const double* __restrict__ la,
+                          const double* __restrict__ ls, int* idx) {
   const double maxr4 = 3.4e38;
-  int idx[kAdjMaxDepth];
   int n = 0;
   for (int d = 0; d < D && n < kAdjMaxDepth; ++d) {
     const long i = (long)d * W + w;
@@ -1782,32 +1776,85 @@ __global__ __launch_bounds__(64) void k_rt_vjp(int D, long W, const double* __re
                      a1 < maxr4 && a2 < maxr4 && a3 < maxr4 && a4 < maxr4;
     if (okd && r > 0) idx[n++] = d;
   }
-  if (n < 4) return;
-  if (!(rhox[idx[0]] < rhox[idx[n - 1]])) {
+  if (n >= 4 && !(rhox[idx[0]] < rhox[idx[n - 1]])) {
     for (int j = 0; j < n / 2; ++j) {
       int t = idx[j];
       idx[j] = idx[n - 1 - j];
       idx[n - 1 - j] = t;
     }
   }
+  return n;
+}
+
+// Processing order of the vjp: wavelengths whose solve takes the XTAU branch
+// (or none) first, label-401 ones (TAUNU(0) = abtot rho at the top above
+// XTAU(51)) last, so that a wave runs one branch.  Only the order depends on
+// the atomics; each wavelength's result does not.
+__global__ void k_vjp_order(int D, long W, const double* __restrict__ rhox, const double* __restrict__ ca,
+                            const double* __restrict__ cs, const double* __restrict__ la,
+                            const double* __restrict__ ls, const double* __restrict__ v, const double* xtau,
+                            int* __restrict__ counts, int32_t* __restrict__ order) {
+  const long w = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (w >= W) return;
+  bool b401 = false;
+  if (v[w] != 0.0) {
+    int idx[kAdjMaxDepth];
+    const int n = vjp_depths(D, W, w, rhox, ca, cs, la, ls, idx);
+    if (n >= 4) {
+      const long i = (long)idx[0] * W + w;
+      double ab = ca[i] + la[i] + cs[i] + ls[i];
+      ab = ab > kEps38 ? ab : kEps38;
+      b401 = ab * rhox[idx[0]] > xtau[adj::kXt - 1];
+    }
+  }
+  if (b401)
+    order[W - 1 - atomicAdd(&counts[1], 1)] = (int32_t)w;
+  else
+    order[atomicAdd(&counts[0], 1)] = (int32_t)w;
+}
+
+__global__ __launch_bounds__(kVjpBlock) void k_rt_vjp(int D, long W, const int32_t* __restrict__ order,
+                                                      const double* __restrict__ wl,
+                                                      const double* __restrict__ temp, const double* __restrict__ rhox,
+                                                      const double* __restrict__ ca, const double* __restrict__ cs,
+                                                      const double* __restrict__ la, const double* __restrict__ ls,
+                                                      const double* __restrict__ src, const double* __restrict__ v,
+                                                      DeviceTables tb, double* __restrict__ g_ca,
+                                                      double* __restrict__ g_cs, double* __restrict__ g_la,
+                                                      double* __restrict__ g_b, double* __restrict__ g_src) {
+  // outputs were zeroed by the caller
+  __shared__ float s_cj[adj::kXt * adj::kXt];
+  __shared__ double s_vec[3][adj::kXt * kVjpBlock];
+  for (int i = threadIdx.x; i < adj::kXt * adj::kXt; i += kVjpBlock) s_cj[i] = tb.coefj32[i];
+  __syncthreads();  // the only barrier: lanes leave independently below
+  const long t = blockIdx.x * (long)kVjpBlock + threadIdx.x;
+  if (t >= W) return;
+  const long w = order[t];
+  const double vw = v[w];
+  if (vw == 0.0) return;
+  int idx[kAdjMaxDepth];
+  const int n = vjp_depths(D, W, w, rhox, ca, cs, la, ls, idx);
+  if (n < 4) return;
   const double wv = wl[w];
   const double freq = kCLightNm / (wv > 1e-12 ? wv : 1e-12);
   const double f3 = pow(freq / 1.0e15, 3.0);
   adj::Column<kAdjMaxDepth> col;
   adj::Work<kAdjMaxDepth> wk;
-  double bc[kAdjMaxDepth], sl[kAdjMaxDepth];
-  col.n = n;
-  for (int j = 0; j < n; ++j) {
-    const int d = idx[j];
-    const long i = (long)d * W + w;
-    const double B = planck_one(freq, f3, temp[d]);
-    double S = B;
+  // the continuum source B(T(d)) and the line source S at depth d
+  auto sources = [&](int d, long i, double& B, double& S) {
+    B = planck_one(freq, f3, temp[d]);
+    S = B;
     if (src) {
       const double s0 = src[i];
       S = isfinite(s0) ? s0 : B;
     }
-    bc[j] = B;
-    sl[j] = S;
+  };
+  col.n = n;
+  for (int j = 0; j < n; ++j) {
+    const int d = idx[j];
+    const long i = (long)d * W + w;
+    double B, S;
+    sources(d, i, B, S);
     const double ac = ca[i], al = la[i], sc = cs[i], sgl = ls[i];
     double ab = ac + al + sc + sgl;
     ab = ab > kEps38 ? ab : kEps38;
@@ -1819,10 +1866,13 @@ __global__ __launch_bounds__(64) void k_rt_vjp(int D, long W, const double* __re
     col.alpha[j] = alpha;
     col.snubar[j] = dn > 0 ? (ac * B + al * S) / dn : B;
   }
-  const adj::JoshTab jt{tb.xtau, tb.ch, tb.coefj32};
+  const adj::JoshTab jt{tb.xtau, tb.ch, s_cj};
+  wk.xs = adj::Vec{&s_vec[0][threadIdx.x], kVjpBlock};
+  wk.xa = adj::Vec{&s_vec[1][threadIdx.x], kVjpBlock};
+  wk.lam = adj::Vec{&s_vec[2][threadIdx.x], kVjpBlock};
   double gab[kAdjMaxDepth], gal[kAdjMaxDepth], gsb[kAdjMaxDepth];
-  adj::total_flux_vjp(col, jt, wk, vw, gab, gal, gsb, part, kVjpParts);
-  for (int j = part; j < n; j += kVjpParts) {
+  adj::total_flux_vjp(col, jt, wk, vw, gab, gal, gsb);
+  for (int j = 0; j < n; ++j) {
     const int d = idx[j];
     const long i = (long)d * W + w;
     const double ac = ca[i], al = la[i];
@@ -1832,8 +1882,10 @@ __global__ __launch_bounds__(64) void k_rt_vjp(int D, long W, const double* __re
     const double da_dab = -alpha / ab;  // d alpha / d(ac or al)
     double gsb_ac = 0.0, gsb_al = 0.0, gB = gsb[j], gS = 0.0;
     if (dn > 0) {
-      gsb_ac = gsb[j] * (bc[j] - sb) / dn;
-      gsb_al = gsb[j] * (sl[j] - sb) / dn;
+      double B, S;
+      sources(d, i, B, S);
+      gsb_ac = gsb[j] * (B - sb) / dn;
+      gsb_al = gsb[j] * (S - sb) / dn;
       gB = gsb[j] * ac / dn;
       gS = gsb[j] * al / dn;
     }
@@ -2749,6 +2801,7 @@ int synthe_lte_vjp(synthe_ctx* ctx, long n_depths, long n_wl, const double* wl, 
   if (n_depths > kAdjMaxDepth)
     return fail(SYNTHE_ERANGE, "synthe_lte_vjp: more than " + std::to_string(kAdjMaxDepth) + " depths");
   if (n_wl == 0) return SYNTHE_OK;
+  if (n_wl > 0x7fffffffL) return fail(SYNTHE_ERANGE, "synthe_lte_vjp: more than 2^31 wavelengths");
   if (!wl || !temperature || !rhox || !cont_abs || !cont_scat || !line_abs || !line_scat || !v_flux)
     return fail(SYNTHE_EINVAL, "synthe_lte_vjp: NULL input");
   const long dw = n_depths * n_wl;
@@ -2758,9 +2811,18 @@ int synthe_lte_vjp(synthe_ctx* ctx, long n_depths, long n_wl, const double* wl, 
                        (unsigned long long*)z);
     HIP_TRY(hipGetLastError());
   }
-  hipLaunchKernelGGL(k_rt_vjp, dim3(blocks_for(n_wl * kVjpParts, 64)), dim3(64), 0, (hipStream_t)stream, (int)n_depths, n_wl, wl,
-                     temperature, rhox, cont_abs, cont_scat, line_abs, line_scat, line_source, v_flux, ctx->tb,
-                     g_cont_abs, g_cont_scat, g_line_abs, g_planck, g_line_source);
+  int rc = ensure_ws(ctx, align_up((size_t)n_wl * sizeof(int32_t)) + 256);
+  if (rc) return rc;
+  int32_t* order = (int32_t*)ctx->ws;
+  int* counts = (int*)((char*)ctx->ws + align_up((size_t)n_wl * sizeof(int32_t)));
+  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, (hipStream_t)stream, 2, counts);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_vjp_order, dim3(blocks_for(n_wl, 256)), dim3(256), 0, (hipStream_t)stream, (int)n_depths,
+                     n_wl, rhox, cont_abs, cont_scat, line_abs, line_scat, v_flux, ctx->tb.xtau, counts, order);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_rt_vjp, dim3(blocks_for(n_wl, kVjpBlock)), dim3(kVjpBlock), 0, (hipStream_t)stream,
+                     (int)n_depths, n_wl, order, wl, temperature, rhox, cont_abs, cont_scat, line_abs, line_scat,
+                     line_source, v_flux, ctx->tb, g_cont_abs, g_cont_scat, g_line_abs, g_planck, g_line_source);
   HIP_TRY(hipGetLastError());
   return SYNTHE_OK;
 }

@@ -107,3 +107,30 @@ def test_vjp_against_central_differences(adj, seed, thick):
     for q, (fd, gq, arr) in enumerate(zip(fds, g, (ab, al, sb))):
         err = np.max(np.abs((gq - fd) * arr)) / scale
         assert err < 2e-5, (q, err)
+
+
+@pytest.mark.parametrize("thick", [False, True])
+def test_reverse_vjp_matches_forward_mode_by_inputs(adj, thick):
+    """The O(n) vjp (colour-seeded Jacobians of PARCOE/MAP1, implicit label-401
+    adjoint) against forward-mode dual numbers seeded one input at a time
+    (O(n^2), the definition).  The XTAU branch contracts the same partial
+    derivatives (equal to rounding); the label-401 branch compares the implicit
+    derivative with the derivative of the iteration run to 1e-12."""
+    lib, tabs = adj
+    vp = ct.c_void_p
+    lib.adj_vjp_by_inputs.restype = ct.c_double
+    lib.adj_vjp_by_inputs.argtypes = [ct.c_int] + [vp] * 7 + [ct.c_double] + [vp] * 3
+    tol = 1e-8 if thick else 1e-13
+    for seed in range(8):
+        rng = np.random.default_rng(500 + seed)
+        n = int(rng.integers(5, 96))
+        rho, ab, al, sb, _, _ = _column(rng, n, thick)
+        g = [np.zeros(n) for _ in range(3)]
+        h = [np.zeros(n) for _ in range(3)]
+        f1 = lib.adj_vjp(n, _p(rho), _p(ab), _p(al), _p(sb), *map(_p, tabs), 0.7, *map(_p, g))
+        f2 = lib.adj_vjp_by_inputs(n, _p(rho), _p(ab), _p(al), _p(sb), *map(_p, tabs), 0.7, *map(_p, h))
+        assert f1 == f2
+        scale = max(np.max(np.abs(b * x)) for b, x in zip(h, (ab, al, sb)))
+        for q, (a, b, x) in enumerate(zip(g, h, (ab, al, sb))):
+            err = np.max(np.abs((a - b) * x)) / scale
+            assert err < tol, (seed, n, q, err)
