@@ -303,14 +303,82 @@ struct Vec {
   SYNTHE_HD double& operator[](int k) const { return p[k * s]; }
 };
 
+constexpr int kGsSweeps = 60;      // Gauss-Seidel sweeps before the Krylov finish
+constexpr int kKrylovIters = 200;  // BiCGSTAB iterations at most
+constexpr double kKrylovTol = 1e-15;  // relative residual of the Krylov finish
+
+// y = A v for the fixed point's matrix A = I - diag(XA) C, or for its
+// transpose I - C^T diag(XA) (the adjoint's).
+template <class VA, class VV>
+SYNTHE_HD void fp_matvec(const JoshTab& tb, const VA& xa, const VV& v, double* y, bool transpose) {
+  for (int k = 0; k < kXt; ++k) {
+    double s = 0.0;
+    if (transpose) {
+      for (int m = 0; m < kXt; ++m) s += (double)tb.coefj32[m * kXt + k] * xa[m] * v[m];
+      y[k] = v[k] - s;
+    } else {
+      for (int m = 0; m < kXt; ++m) s += (double)tb.coefj32[k * kXt + m] * v[m];
+      y[k] = v[k] - xa[k] * s;
+    }
+  }
+}
+
+SYNTHE_HD double dot51(const double* a, const double* b) {
+  double s = 0.0;
+  for (int k = 0; k < kXt; ++k) s += a[k] * b[k];
+  return s;
+}
+
+// BiCGSTAB (van der Vorst 1992) for A x = b, A as fp_matvec, from the
+// iterate in x; b is given in kr[0].  kr: 6 vectors of kXt (b/r, r0, p, v,
+// s, t).  Stops at |r| <= kKrylovTol |b|; returns the iteration count.
+template <class VA, class VX>
+SYNTHE_HD int bicgstab51(const JoshTab& tb, const VA& xa, const VX& x, bool transpose, double (*kr)[kXt]) {
+  double *r = kr[0], *r0 = kr[1], *p = kr[2], *v = kr[3], *sv = kr[4], *t = kr[5];
+  const double bn = sqrt(dot51(r, r));
+  fp_matvec(tb, xa, x, v, transpose);
+  for (int k = 0; k < kXt; ++k) {
+    r[k] -= v[k];
+    r0[k] = r[k];
+    p[k] = v[k] = 0.0;
+  }
+  double rho = 1.0, alpha = 1.0, omega = 1.0;
+  int it = 0;
+  while (it < kKrylovIters && sqrt(dot51(r, r)) > kKrylovTol * bn) {
+    ++it;
+    const double rho1 = dot51(r0, r);
+    if (rho1 == 0.0 || omega == 0.0) break;
+    const double beta = (rho1 / rho) * (alpha / omega);
+    for (int k = 0; k < kXt; ++k) p[k] = r[k] + beta * (p[k] - omega * v[k]);
+    fp_matvec(tb, xa, p, v, transpose);
+    const double r0v = dot51(r0, v);
+    if (r0v == 0.0) break;
+    alpha = rho1 / r0v;
+    for (int k = 0; k < kXt; ++k) sv[k] = r[k] - alpha * v[k];
+    fp_matvec(tb, xa, sv, t, transpose);
+    const double tt = dot51(t, t);
+    omega = tt > 0.0 ? dot51(t, sv) / tt : 0.0;
+    for (int k = 0; k < kXt; ++k) {
+      x[k] = x[k] + alpha * p[k] + omega * sv[k];
+      r[k] = sv[k] - omega * t[k];
+    }
+    rho = rho1;
+  }
+  return it;
+}
+
 // FP64 scattering fixed point XS = XB (1 - XA) + XA (C XS) by Gauss-Seidel
 // sweeps (descending k, as physics/josh_solver.py:24-92) to relative change
-// tol; returns the sweep count.  V: double* or Vec.
+// tol.  Strong scattering (XA near 1) makes the sweeps contract slowly (by
+// about max XA each): after kGsSweeps, if kr (6 x kXt scratch) is given, the
+// solve is finished by BiCGSTAB.  Returns sweeps + Krylov iterations.
+// V: double* or Vec.
 template <class V>
 SYNTHE_HD int fixed_point64(const JoshTab& tb, const double* xb, const V& xa, const V& xs, double tol,
-                            int max_sweeps) {
+                            int max_sweeps, double (*kr)[kXt] = nullptr) {
   for (int k = 0; k < kXt; ++k) xs[k] = xb[k];
-  for (int it = 0; it < max_sweeps; ++it) {
+  const int gs = kr && max_sweeps > kGsSweeps ? kGsSweeps : max_sweeps;
+  for (int it = 0; it < gs; ++it) {
     double err = 0.0;
     for (int k = kXt - 1; k >= 0; --k) {
       double cx = 0.0;
@@ -323,14 +391,19 @@ SYNTHE_HD int fixed_point64(const JoshTab& tb, const double* xb, const V& xa, co
     }
     if (err < tol) return it + 1;
   }
-  return max_sweeps;
+  if (gs == max_sweeps) return max_sweeps;
+  for (int k = 0; k < kXt; ++k) kr[0][k] = xb[k] * (1.0 - xa[k]);
+  return gs + bicgstab51(tb, xa, xs, false, kr);
 }
 
-// Adjoint of the fixed point: lam = CH + C^T (XA o lam), Gauss-Seidel.
+// Adjoint of the fixed point: lam = CH + C^T (XA o lam), Gauss-Seidel, with
+// the same Krylov finish.
 template <class V>
-SYNTHE_HD int adjoint64(const JoshTab& tb, const V& xa, const V& lam, double tol, int max_sweeps) {
+SYNTHE_HD int adjoint64(const JoshTab& tb, const V& xa, const V& lam, double tol, int max_sweeps,
+                        double (*kr)[kXt] = nullptr) {
   for (int k = 0; k < kXt; ++k) lam[k] = tb.ch[k];
-  for (int it = 0; it < max_sweeps; ++it) {
+  const int gs = kr && max_sweeps > kGsSweeps ? kGsSweeps : max_sweeps;
+  for (int it = 0; it < gs; ++it) {
     double err = 0.0;
     for (int m = 0; m < kXt; ++m) {
       double s = 0.0;
@@ -344,7 +417,9 @@ SYNTHE_HD int adjoint64(const JoshTab& tb, const V& xa, const V& lam, double tol
     }
     if (err < tol) return it + 1;
   }
-  return max_sweeps;
+  if (gs == max_sweeps) return max_sweeps;
+  for (int k = 0; k < kXt; ++k) kr[0][k] = tb.ch[k];
+  return gs + bicgstab51(tb, xa, lam, true, kr);
 }
 
 // Per-depth inputs of one total-flux solve in increasing rho order (n >= 4).
@@ -364,6 +439,7 @@ struct Work {
     double xsbar[kXt], xalpha[kXt], xb[kXt], xa[kXt], xs[kXt], lam[kXt], gxb[kXt], gxa[kXt];
     int base[kXt];  // MAP1 bracket of each XTAU point
     Dual out1[kXt], out2[kXt];
+    double kr[6][kXt];  // Krylov vectors of the fixed point's strong-scattering finish
   };
   struct Label401 {
     double lam[kMax], tmp[kMax], band[kMax][5];
@@ -411,7 +487,7 @@ SYNTHE_HD double total_flux64(const Column<kMax>& col, const JoshTab& tb, Work<k
     x.xb[k] = xb;
     w.xa[k] = xa;
   }
-  fixed_point64(tb, x.xb, w.xa, w.xs, tol, 400);
+  fixed_point64(tb, x.xb, w.xa, w.xs, tol, 400, x.kr);
   double f = 0.0;
   for (int k = 0; k < kXt; ++k) f += tb.ch[k] * w.xs[k];
   return f;
@@ -519,7 +595,7 @@ SYNTHE_HD double total_flux_vjp(const Column<kMax>& col, const JoshTab& tb, Work
     }
   } else {
     // sensitivities of F to the fixed point's inputs XB, XA
-    adjoint64(tb, w.xa, w.lam, 1e-13, 400);
+    adjoint64(tb, w.xa, w.lam, 1e-13, 400, w.x.kr);
     double gs0 = 0.0, ga0 = 0.0;
     for (int k = 0; k < kXt; ++k) {
       double cx = 0.0;

@@ -76,7 +76,7 @@ double adj_vjp_by_inputs(int n, const double* rho, const double* abtot, const do
     }
     return F;
   }
-  A::adjoint64(tb, w.xa, w.lam, 1e-13, 400);
+  A::adjoint64(tb, w.xa, w.lam, 1e-13, 400, w.x.kr);
   double gs0 = 0.0, ga0 = 0.0;
   for (int k = 0; k < A::kXt; ++k) {
     double cx = 0.0;

@@ -134,3 +134,38 @@ def test_reverse_vjp_matches_forward_mode_by_inputs(adj, thick):
         for q, (a, b, x) in enumerate(zip(g, h, (ab, al, sb))):
             err = np.max(np.abs((a - b) * x)) / scale
             assert err < tol, (seed, n, q, err)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_vjp_strong_scattering(adj, seed):
+    """Albedo 0.995-0.9995 at every depth (electron scattering in hot,
+    low-gravity atmospheres): Gauss-Seidel contracts by ~0.999 per sweep, and
+    the Krylov finish must bring the fixed point and its adjoint to
+    convergence, so that the vjp equals differences of the converged model."""
+    lib, tabs = adj
+    rng = np.random.default_rng(700 + seed)
+    n = 60
+    rho = np.logspace(-4, 1.5, n)
+    acont = 0.3 * (rho / rho[0]) ** 0.5
+    albedo = 1 - 10 ** rng.uniform(-3.3, -2.3, n)
+    sig = acont * albedo / (1 - albedo)
+    ab, al = acont + sig, sig / (acont + sig)
+    sb = 1e-5 * (1 + (rho / rho[-1]) ** 0.25 * 4)
+    f_loose, _ = _flux(lib, tabs, rho, ab, al, sb, tol=1e-10)
+    f_tight, t401 = _flux(lib, tabs, rho, ab, al, sb, tol=1e-15)
+    assert t401 == 0 and abs(f_loose / f_tight - 1) < 1e-12
+    g = [np.zeros(n) for _ in range(3)]
+    lib.adj_vjp(n, _p(rho), _p(ab), _p(al), _p(sb), *map(_p, tabs), 1.0, *map(_p, g))
+    fds = []
+    for q, arr in enumerate((ab, al, sb)):
+        fd = np.zeros(n)
+        for j in range(n):
+            h = 1e-7 * arr[j]
+            args_u, args_d = [ab.copy(), al.copy(), sb.copy()], [ab.copy(), al.copy(), sb.copy()]
+            args_u[q][j] += h
+            args_d[q][j] -= h
+            fd[j] = (_flux(lib, tabs, rho, *args_u)[0] - _flux(lib, tabs, rho, *args_d)[0]) / (2 * h)
+        fds.append(fd)
+    scale = max(np.max(np.abs(fd * arr)) for fd, arr in zip(fds, (ab, al, sb)))
+    for q, (fd, gq, arr) in enumerate(zip(fds, g, (ab, al, sb))):
+        assert np.max(np.abs((gq - fd) * arr)) / scale < 1e-6, q
