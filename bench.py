@@ -334,6 +334,8 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-atmosphere batched variant")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint variant")
     ap.add_argument("--no-inverse", action="store_true", help="skip the batched inverse-problem variant")
+    ap.add_argument("--variant", choices=("batched", "adjoint", "inverse"),
+                    help="run only this variant and print its figures as JSON (for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain device copy of 1 GiB (3 times), to scale HBM counters")
     args = ap.parse_args()
@@ -352,6 +354,13 @@ def main() -> int:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU layout "
               "is a design note, not a measured path", file=sys.stderr)
         return 2
+    if args.variant:
+        if args.variant == "batched":
+            res = bench_batched(args, make_workload(CONFIGS["c2"]))
+        else:
+            res = (bench_adjoint if args.variant == "adjoint" else bench_inverse)(args)
+        print(json.dumps({"variant": args.variant, **res}), flush=True)
+        return 0
     import torch
 
     from kurucz_py_joss_submission_amd.device import SpectrumPlan
