@@ -10,6 +10,8 @@ file, or a cached .npz of a parsed catalog.
 
 from __future__ import annotations
 
+import os
+
 import logging
 import time
 from dataclasses import dataclass, field
@@ -127,11 +129,22 @@ def _load_atmosphere(cfg: SynthesisConfig) -> AtmosphereModel:
 
 
 def _load_catalog(cfg: SynthesisConfig) -> L.LineCatalog:
+    """The atomic catalog of the run: a saved .npz catalog, or a GFALL file
+    through the parsed cache (io/lines/compiler.py:394-412; PY_DISABLE_PARSED_CACHE=1
+    parses every time), DELLIM-filtered to the window when line_filter is set."""
+    from . import parsed_cache as PC
+
     p = Path(cfg.line_data.atomic_catalog)
-    cat = L.LineCatalog.load_npz(p) if p.suffix == ".npz" else L.load_catalog(p)
+    w0, w1 = cfg.wavelength_grid.start, cfg.wavelength_grid.end
+    if p.suffix == ".npz":
+        cat = L.LineCatalog.load_npz(p)
+        return L.filter_by_range(cat, w0, w1) if cfg.line_filter else cat
+    if os.environ.get("PY_DISABLE_PARSED_CACHE", "0") == "1":
+        cat = L.load_catalog(p)
+        return L.filter_by_range(cat, w0, w1) if cfg.line_filter else cat
     if cfg.line_filter:
-        cat = L.filter_by_range(cat, cfg.wavelength_grid.start, cfg.wavelength_grid.end)
-    return cat
+        return PC.load_or_build_parsed_catalog_window(p, w0, w1)
+    return PC.load_or_build_parsed_catalog(p)
 
 
 def run_synthesis(cfg: SynthesisConfig) -> SynthResult:
