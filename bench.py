@@ -294,20 +294,25 @@ def bench_inverse(args) -> dict:
     w = torch.rand(m.W, dtype=torch.float64, device="cuda", generator=torch.Generator("cuda").manual_seed(3))
     (w * m(T, da, lgf, lgw)).sum().backward()
     g_adj = da.grad.cpu().numpy()
-    g_fd = {}
+    def central(k, h):
+        up, dn = sol["delta"].copy(), sol["delta"].copy()
+        up[:, k] += h
+        dn[:, k] -= h
+        return float((w * (prob.evaluate(up)[i0] - prob.evaluate(dn)[i0])).sum()) / (2 * h)
+
+    g_fd, g_fd3 = {}, {}
     for z in INV.LINE_ELEMENTS:
         k = S.ABUNDANCE_Z.index(z)
-        up, dn = sol["delta"].copy(), sol["delta"].copy()
-        up[:, k] += 1e-3
-        dn[:, k] -= 1e-3
-        g_fd[z] = float((w * (prob.evaluate(up)[i0] - prob.evaluate(dn)[i0])).sum()) / 2e-3
+        g_fd[z], g_fd3[z] = central(k, 1e-3), central(k, 3e-3)
     # against the spectrum's largest line-element gradient; per element where
     # the element constrains the spectrum (sensitivity >= 2e-3 of the median
-    # flux per dex: below that the differences see the forward's steps)
+    # flux per dex), beside the spread of the differences themselves between
+    # steps of 1e-3 and 3e-3 dex (the forward's gates and table steps)
     big = max(abs(v) for v in g_fd.values())
     worst = max(abs(g_adj[S.ABUNDANCE_Z.index(z)] - v) for z, v in g_fd.items()) / max(big, 1e-300)
-    per_el = {int(z): abs(g_adj[S.ABUNDANCE_Z.index(z)] - v) / max(abs(v), 1e-300) for z, v in g_fd.items()
-              if sol["sensitivity"][i0, S.ABUNDANCE_Z.index(z)] >= 2e-3}
+    per_el = {int(z): {"adjoint_vs_diff": abs(g_adj[S.ABUNDANCE_Z.index(z)] - v) / max(abs(v), 1e-300),
+                       "diff_step_spread": abs(g_fd3[z] - v) / max(abs(v), 1e-300)}
+              for z, v in g_fd.items() if sol["sensitivity"][i0, S.ABUNDANCE_Z.index(z)] >= 2e-3}
     log(f"[bench] c5: {sol['iterations']} iterations, {sol['evaluations']} batched evaluations, "
         f"{sol['seconds']:.2f} s; {rep}; judge {judge:.2e}; adjoint vs batched gradient {worst:.2e} "
         f"(constrained elements {per_el})")

@@ -270,7 +270,12 @@ class BatchPlan:
 
     def __init__(self, instances: Sequence[Tuple[AtmosphereModel, LineCatalog, float, float]],
                  resolution: float = 300000.0, cutoff: float = 1e-3, precision: str = "fp64",
-                 autotune: bool = True, graph: bool = True):
+                 autotune: bool = True, graph: bool = True,
+                 depth_arrays: Optional[Sequence[DepthArrays]] = None):
+        """depth_arrays (optional): the device arrays of each instance's
+        atmosphere, as upload_atmosphere makes them, for callers that keep
+        them elsewhere (e.g. views into one batched tensor they rewrite
+        between replays); by default each atmosphere is uploaded here."""
         import ctypes
 
         from .lines import filter_by_range
@@ -284,7 +289,7 @@ class BatchPlan:
         arr = (N.Instance * len(instances))()
         for i, (atm, catalog, wl0, wl1) in enumerate(instances):
             grid = N.wavelength_grid(wl0, wl1, resolution)
-            depth = upload_atmosphere(atm)
+            depth = upload_atmosphere(atm) if depth_arrays is None else depth_arrays[i]
             cat = filter_by_range(catalog, wl0, wl1)
             soa = build_line_soa(cat, grid, depth.n_elem, grid_origin=float(grid[0]) if grid.size else None)
             lines = upload_lines(soa, grid.size)

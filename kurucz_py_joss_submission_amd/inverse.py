@@ -112,6 +112,59 @@ class AbundanceResponse:
         return dict(pop=pop, rho=rho, txnxn=txnxn, coeff_scat=cscat)
 
 
+class BatchResponse:
+    """AbundanceResponse of many atmospheres at once, written in place.
+
+    The per-atmosphere constants are padded to the deepest atmosphere
+    ([atmosphere][depth], padding rows hold harmless values) and
+    ``write(delta)`` evaluates the same closed forms as
+    AbundanceResponse.arrays, element for element, for every atmosphere in one
+    set of batched operations, into caller-owned tensors pop [n][Dmax][ion][Z],
+    rho / txnxn [n][Dmax] and coeff_scat [n][Dmax][E-1][3] (rows past an
+    atmosphere's depth count are not meaningful)."""
+
+    def __init__(self, responses: Sequence[AbundanceResponse]):
+        torch = _torch()
+        self.n = len(responses)
+        self.dmax = max(r.D for r in responses)
+        dev = responses[0].device
+
+        def pad(get, fill=0.0):
+            first = get(responses[0])
+            out = torch.full((self.n, self.dmax) + tuple(first.shape[1:]), fill, dtype=torch.float64, device=dev)
+            for i, r in enumerate(responses):
+                out[i, :r.D] = get(r)
+            return out
+
+        self.unit = pad(lambda r: r.unit)
+        self.nnuc = pad(lambda r: r.nnuc, 1.0)
+        self.fr_h1 = pad(lambda r: r.fr_h1)
+        self.fr_he1 = pad(lambda r: r.fr_he1)
+        self.ne_st = pad(lambda r: r.ne_st)
+        self.tfac = pad(lambda r: r.tfac)
+        self.ab0 = torch.stack([r.ab0 for r in responses])      # [n][20]
+        self.mass = responses[0].mass
+        self.sr = responses[0].sr
+        self.zidx = responses[0].zidx
+
+    def write(self, delta, pop, rho, txnxn, coeff_scat) -> None:
+        torch = _torch()
+        delta = torch.as_tensor(delta, dtype=torch.float64, device=self.ab0.device).reshape(self.n, -1)
+        ab = self.ab0 * torch.pow(10.0, delta)
+        asum = ab.sum(dim=1, keepdim=True)
+        frac = ab / asum                                            # [n][20]
+        nz = frac[:, None, :] * self.nnuc[:, :, None]               # [n][D][20]
+        pop[:, :, :, self.zidx] = nz[:, :, None, :] * self.unit
+        mu = (ab * self.mass[None, :]).sum(dim=1, keepdim=True) / asum
+        r = self.nnuc * mu * AMU                                    # [n][D]
+        rho.copy_(r)
+        nh1 = nz[:, :, 0] * self.fr_h1
+        he1 = nz[:, :, 1] * self.fr_he1
+        sig = (self.ne_st[:, :, None, None] + nh1[:, :, None, None] * self.sr[None, None]) / r[:, :, None, None]
+        torch.log10(torch.clamp(sig, min=1e-30), out=coeff_scat)
+        torch.mul(nh1 + 0.42 * he1, self.tfac, out=txnxn)
+
+
 @dataclass
 class Target:
     cols: S.AtmColumns
@@ -189,10 +242,30 @@ class InverseProblem:
         self.nz = len(S.ABUNDANCE_Z)
         self.h_index = S.ABUNDANCE_Z.index(1)
         self.line_mask = np.array([z in LINE_ELEMENTS for z in S.ABUNDANCE_Z])
+        from .device import upload_atmosphere
+
         inst = [(S.build_bundle(t.cols), t.catalog, t.wl_start, t.wl_end) for t in self.targets]
-        self.plan = BatchPlan(inst, resolution=resolution, cutoff=cutoff, precision="fp64", autotune=autotune,
-                              graph=True)
         self.resp = [AbundanceResponse(t.cols) for t in self.targets]
+        self.batch_resp = BatchResponse(self.resp)
+        # the abundance-dependent arrays of every spectrum are views into one
+        # padded tensor each, rewritten by one batched evaluation per replay
+        depths = [upload_atmosphere(atm) for atm, _, _, _ in inst]
+        n, dmax = self.n, self.batch_resp.dmax
+        p0, c0 = depths[0].pop, depths[0].coeff_scat
+        f8 = torch.float64
+        self.pop = torch.zeros((n, dmax) + tuple(p0.shape[1:]), dtype=f8, device="cuda")
+        self.rho = torch.ones((n, dmax), dtype=f8, device="cuda")
+        self.txnxn = torch.zeros((n, dmax), dtype=f8, device="cuda")
+        self.coeff_scat = torch.zeros((n, dmax) + tuple(c0.shape[1:]), dtype=f8, device="cuda")
+        for i, dp in enumerate(depths):
+            D = dp.n
+            for name, big in (("pop", self.pop), ("rho", self.rho), ("txnxn", self.txnxn),
+                              ("coeff_scat", self.coeff_scat)):
+                view = big[i, :D]
+                view.copy_(getattr(dp, name))
+                setattr(dp, name, view)
+        self.plan = BatchPlan(inst, resolution=resolution, cutoff=cutoff, precision="fp64", autotune=autotune,
+                              graph=True, depth_arrays=depths)
         self.f_target = [torch.tensor(t.flux, dtype=torch.float64, device="cuda") for t in self.targets]
         # residuals in units of each target's median flux (saturated line
         # cores would dominate relative residuals)
@@ -202,13 +275,7 @@ class InverseProblem:
     def evaluate(self, delta: np.ndarray) -> List[object]:
         """Fluxes (torch, F_lambda) of every spectrum at offsets delta[n][20]."""
         torch = _torch()
-        for i, (it, r) in enumerate(zip(self.plan.items, self.resp)):
-            a = r.arrays(delta[i])
-            dp = it["depth"]
-            dp.pop.copy_(a["pop"][0])
-            dp.rho.copy_(a["rho"][0])
-            dp.txnxn.copy_(a["txnxn"][0])
-            dp.coeff_scat.copy_(a["coeff_scat"][0])
+        self.batch_resp.write(delta, self.pop, self.rho, self.txnxn, self.coeff_scat)
         self.plan.run()
         self.evaluations += 1
         torch.cuda.synchronize()
@@ -216,6 +283,12 @@ class InverseProblem:
 
     def residuals(self, flux) -> List[object]:
         return [(f - t) / s for f, t, s in zip(flux, self.f_target, self.f_scale)]
+
+    @staticmethod
+    def _costs(res) -> np.ndarray:
+        """Sum of squared residuals per spectrum (one device sync)."""
+        torch = _torch()
+        return torch.stack([(r * r).sum() for r in res]).cpu().numpy()
 
     def solve(self, max_iter: int = 12, step: float = 1e-3, tol: float = 1e-6, rcond: float = 1e-6,
               cfloor: float = 1e-3, line_floor: float = 2e-3, max_tries: int = 6, max_step: float = 0.5,
@@ -239,7 +312,7 @@ class InverseProblem:
         delta = np.zeros((n, nz))
         t0 = time.perf_counter()
         res = self.residuals(self.evaluate(delta))
-        cost = np.array([float((r * r).sum()) for r in res])
+        cost = self._costs(res)
         history = [float(cost.sum())]
         mu = np.zeros(n)
         active = cost > 0
@@ -287,7 +360,7 @@ class InverseProblem:
                 trial = np.clip(delta + dstep, -bound, bound)
                 dstep = trial - delta
                 rt = self.residuals(self.evaluate(trial))
-                ct = np.array([float((r * r).sum()) for r in rt])
+                ct = self._costs(rt)
                 for i in np.nonzero(active & ~improved)[0]:
                     if ct[i] < cost[i]:
                         improved[i] = True

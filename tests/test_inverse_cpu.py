@@ -61,3 +61,32 @@ def test_window_has_requested_points():
     for w0 in (400.0, 523.7, 699.9):
         w1 = window_for(w0, 5000, 300000.0)
         assert N.wavelength_grid(w0, w1, 300000.0).size == 5000
+
+
+def test_batch_response_equals_per_atmosphere_response():
+    """BatchResponse (all atmospheres at once, padded to the deepest) writes,
+    for each atmosphere's depths, exactly what that atmosphere's
+    AbundanceResponse gives."""
+    import torch
+
+    from kurucz_py_joss_submission_amd import synthetic as S
+    from kurucz_py_joss_submission_amd.inverse import AbundanceResponse, BatchResponse
+
+    atm = [(5750.0, 4.44, 72, 0.0), (24000.0, 4.0, 80, -1.8), (3900.0, 1.2, 64, 0.4)]
+    resp = [AbundanceResponse(S.make_atm(*a), device="cpu") for a in atm]
+    br = BatchResponse(resp)
+    n, dmax = len(atm), br.dmax
+    rng = np.random.default_rng(9)
+    delta = rng.normal(0.0, 0.3, (n, len(S.ABUNDANCE_Z)))
+    pop = torch.zeros((n, dmax, S.N_ION, S.N_ELEMENTS), dtype=torch.float64)
+    rho = torch.zeros((n, dmax), dtype=torch.float64)
+    txnxn = torch.zeros_like(rho)
+    cscat = torch.zeros((n, dmax, S.WLEDGE.size - 1, 3), dtype=torch.float64)
+    br.write(delta, pop, rho, txnxn, cscat)
+    for i, r in enumerate(resp):
+        ref = r.arrays(delta[i])
+        D = r.D
+        for key, got in (("pop", pop), ("rho", rho), ("txnxn", txnxn), ("coeff_scat", cscat)):
+            x, want = got[i, :D].numpy(), ref[key][0].numpy()
+            assert x.shape == want.shape, key
+            np.testing.assert_array_max_ulp(x, want, maxulp=2)
