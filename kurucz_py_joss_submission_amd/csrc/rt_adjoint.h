@@ -496,26 +496,35 @@ SYNTHE_HD double total_flux64(const Column<kMax>& col, const JoshTab& tb, Work<k
 // The index j = r (mod period) in [lo, lo + period).
 SYNTHE_HD int seeded_in(int lo, int r, int period) { return lo + ((r - lo) % period + period) % period; }
 
-// g_abtot[j] += dF/dabtot[j] given g_tau[m] = dF/dTAUNU[m] (overwritten by
-// its suffix sums).  TAUNU[0] = abtot[0] rho[0] and TAUNU[i+1] = TAUNU[i] +
-// u_i, where u_i depends on abtot[i-1 .. i+2] through PARCOE.
+// The vjp in phases.  Each colour pass r writes only the entries j = r
+// (mod 4, or mod 5 for the label-401 Jacobians), so the passes of one column
+// can run in any order or concurrently; total_flux_vjp below runs them in
+// sequence.  Outputs g_* are accumulated unscaled (dF/d input); the caller
+// zeroes them first.
+
+// Suffix sums of g_tau: g_tau[i] <- sum_{m >= i} dF/dTAUNU[m].
 template <int kMax>
-SYNTHE_HD void integ_vjp(const Column<kMax>& col, Work<kMax>& w, double* g_abtot) {
-  const int n = col.n;
+SYNTHE_HD void tau_suffix(int n, Work<kMax>& w) {
   for (int i = n - 2; i >= 0; --i) w.g_tau[i] += w.g_tau[i + 1];
-  g_abtot[0] += w.g_tau[0] * col.rho[0];
-  for (int r = 0; r < 4; ++r) {
-    for (int i = 0; i < n; ++i) w.d0[i] = Dual(col.abtot[i], (i & 3) == r ? 1.0 : 0.0);
-    parcoe_t(w.d0, col.rho, n, w.d1, w.d2, w.d3);
-    for (int i = 0; i + 1 < n; ++i) {
-      const int j = seeded_in(i - 1, r, 4);
-      if (j < 0 || j >= n) continue;
-      const double r0 = col.rho[i], r1 = col.rho[i + 1];
-      // the derivative part of INTEG's increment (pa + pb (r0 + r1) / 2 + pc (...) / 3) (r1 - r0)
-      const double dterm = w.d1[i].d + 0.5 * w.d2[i].d * (r1 + r0) +
-                           (w.d3[i].d / 3.0) * ((r1 + r0) * r1 + r0 * r0);
-      g_abtot[j] += w.g_tau[i + 1] * dterm * (r1 - r0);
-    }
+}
+
+// INTEG, colour r: g_abtot[j] += dF/dabtot[j] for j = r (mod 4), given the
+// suffix sums in w.g_tau.  TAUNU[0] = abtot[0] rho[0] and TAUNU[i+1] =
+// TAUNU[i] + u_i, where u_i depends on abtot[i-1 .. i+2] through PARCOE.
+template <int kMax>
+SYNTHE_HD void integ_pass(const Column<kMax>& col, Work<kMax>& w, int r, double* g_abtot) {
+  const int n = col.n;
+  if (r == 0) g_abtot[0] += w.g_tau[0] * col.rho[0];
+  for (int i = 0; i < n; ++i) w.d0[i] = Dual(col.abtot[i], (i & 3) == r ? 1.0 : 0.0);
+  parcoe_t(w.d0, col.rho, n, w.d1, w.d2, w.d3);
+  for (int i = 0; i + 1 < n; ++i) {
+    const int j = seeded_in(i - 1, r, 4);
+    if (j < 0 || j >= n) continue;
+    const double r0 = col.rho[i], r1 = col.rho[i + 1];
+    // the derivative part of INTEG's increment (pa + pb (r0 + r1) / 2 + pc (...) / 3) (r1 - r0)
+    const double dterm = w.d1[i].d + 0.5 * w.d2[i].d * (r1 + r0) +
+                         (w.d3[i].d / 3.0) * ((r1 + r0) * r1 + r0 * r0);
+    g_abtot[j] += w.g_tau[i + 1] * dterm * (r1 - r0);
   }
 }
 
@@ -529,6 +538,142 @@ SYNTHE_HD void josh401_j(int n, Work<kMax>& w) {
   deriv_t(w.d0, w.d2, n, w.d3);
 }
 
+// XTAU branch, after total_flux64: the fixed point's adjoint and the
+// sensitivities gxb, gxa of F to MAP1's outputs (into w.x); returns the
+// sensitivities to snubar[0] and alpha[0] through the surface clamp.
+template <int kMax>
+SYNTHE_HD void xtau_adjoint(const Column<kMax>& col, const JoshTab& tb, Work<kMax>& w, double* gs0,
+                            double* ga0) {
+  adjoint64(tb, w.xa, w.lam, 1e-13, 400, w.x.kr);
+  double s0 = 0.0, a0 = 0.0;
+  for (int k = 0; k < kXt; ++k) {
+    double cx = 0.0;
+    for (int m = 0; m < kXt; ++m) cx += (double)tb.coefj32[k * kXt + m] * w.xs[m];
+    double gxb = w.lam[k] * (1.0 - w.xa[k]);
+    double gxa = w.lam[k] * (cx - w.x.xb[k]);
+    if (tb.xtau[k] < w.taunu[0]) {
+      s0 += gxb;
+      a0 += gxa;
+      gxb = gxa = 0.0;
+    } else {
+      if (!(w.x.xsbar[k] > kEps)) gxb = 0.0;
+      if (w.x.xalpha[k] < 0.0 || w.x.xalpha[k] > 1.0) gxa = 0.0;
+    }
+    w.x.gxb[k] = gxb;
+    w.x.gxa[k] = gxa;
+  }
+  *gs0 = col.snubar[0] > kEps ? s0 : 0.0;
+  *ga0 = (col.alpha[0] < 0.0 || col.alpha[0] > 1.0) ? 0.0 : a0;
+}
+
+// XTAU branch, colour r: MAP1 of snubar and alpha on the nodes TAUNU, values
+// then nodes (needs w.taunu, w.x.base, gxb, gxa): g_snubar, g_alpha, g_tau
+// entries j = r (mod 4).
+template <int kMax>
+SYNTHE_HD void xtau_pass(const Column<kMax>& col, const JoshTab& tb, Work<kMax>& w, int r, double* g_snubar,
+                         double* g_alpha, double* g_tau) {
+  const int n = col.n;
+  for (int i = 0; i < n; ++i) {
+    w.d0[i] = Dual(w.taunu[i]);
+    w.d1[i] = Dual(col.snubar[i], (i & 3) == r ? 1.0 : 0.0);
+  }
+  map1_t(w.d0, w.d1, n, tb.xtau, w.x.out1);
+  for (int i = 0; i < n; ++i) w.d1[i] = Dual(col.alpha[i], (i & 3) == r ? 1.0 : 0.0);
+  map1_t(w.d0, w.d1, n, tb.xtau, w.x.out2);
+  for (int k = 0; k < kXt; ++k) {
+    const int j = seeded_in(w.x.base[k] - 3, r, 4);
+    if (j < 0 || j >= n) continue;
+    g_snubar[j] += w.x.gxb[k] * w.x.out1[k].d;
+    g_alpha[j] += w.x.gxa[k] * w.x.out2[k].d;
+  }
+  for (int i = 0; i < n; ++i) {
+    w.d0[i] = Dual(w.taunu[i], (i & 3) == r ? 1.0 : 0.0);
+    w.d1[i] = Dual(col.snubar[i]);
+  }
+  map1_t(w.d0, w.d1, n, tb.xtau, w.x.out1);
+  for (int i = 0; i < n; ++i) w.d1[i] = Dual(col.alpha[i]);
+  map1_t(w.d0, w.d1, n, tb.xtau, w.x.out2);
+  for (int k = 0; k < kXt; ++k) {
+    const int j = seeded_in(w.x.base[k] - 3, r, 4);
+    if (j < 0 || j >= n) continue;
+    g_tau[j] += w.x.gxb[k] * w.x.out1[k].d + w.x.gxa[k] * w.x.out2[k].d;
+  }
+}
+
+// Label-401 branch, colour r of 5: the band entries band[j][i - j + 2] =
+// dPHI_j / dS[i] for i = r (mod 5), at the converged iterate S (w.pa).
+template <int kMax>
+SYNTHE_HD void l401_band(const Column<kMax>& col, Work<kMax>& w, int r, double (*band)[5]) {
+  const int n = col.n;
+  for (int i = 0; i < n; ++i) {
+    w.d0[i] = Dual(w.taunu[i]);
+    w.d1[i] = Dual(w.pa[i], i % 5 == r ? 1.0 : 0.0);
+  }
+  josh401_j(n, w);
+  for (int j = 0; j < n; ++j) {
+    const int i = seeded_in(j - 2, r, 5);
+    if (i >= 0 && i < n) band[j][i - j + 2] = col.alpha[j] * (w.d3[j].d + w.d1[j].d);
+  }
+}
+
+// Label-401 branch, given the whole band (zero outside the column): the
+// adjoint lam = dF/dS + (dPHI/dS)^T lam (the forward's iteration transposed,
+// into w.s.lam), g_snubar and g_alpha for every j, and the direct TAUNU terms
+// of F = (S[1] - S[0]) / (TAUNU[1] - TAUNU[0]) / 3 into g_tau.
+template <int kMax>
+SYNTHE_HD void l401_adjoint(const Column<kMax>& col, Work<kMax>& w, const double (*band)[5], double* g_snubar,
+                            double* g_alpha, double* g_tau) {
+  const int n = col.n;
+  const double* S = w.pa;
+  const double dt = w.taunu[1] - w.taunu[0];
+  const double dS = S[1] - S[0];
+  g_tau[0] += dS / (3.0 * dt * dt);
+  g_tau[1] += -dS / (3.0 * dt * dt);
+  // J(S) in double (the value parts of the dual passes)
+  for (int i = 0; i < n; ++i) w.s.tmp[i] = 0.0;
+  deriv_t(w.taunu, S, n, w.pb);
+  for (int j = 0; j < n; ++j) w.pb[j] = w.pb[j] / 3.0;
+  deriv_t(w.taunu, w.pb, n, w.pc);
+  for (int i = 0; i < n; ++i) w.s.lam[i] = 0.0;
+  w.s.lam[0] = -1.0 / (3.0 * dt);
+  w.s.lam[1] = 1.0 / (3.0 * dt);
+  for (int it = 0; it < 4 * kMax401; ++it) {
+    double err = 0.0;
+    for (int i = 0; i < n; ++i) {
+      double s = i == 0 ? -1.0 / (3.0 * dt) : (i == 1 ? 1.0 / (3.0 * dt) : 0.0);
+      for (int j = i - 2; j <= i + 2; ++j)
+        if (j >= 0 && j < n) s += band[j][i - j + 2] * w.s.lam[j];
+      w.s.tmp[i] = s;
+    }
+    for (int i = 0; i < n; ++i) {
+      const double den = fabs(w.s.tmp[i]) > kEps ? fabs(w.s.tmp[i]) : kEps;
+      err += fabs(w.s.tmp[i] - w.s.lam[i]) / den;
+      w.s.lam[i] = w.s.tmp[i];
+    }
+    if (err < 1e-13) break;
+  }
+  for (int j = 0; j < n; ++j) {
+    g_snubar[j] += w.s.lam[j] * (1.0 - col.alpha[j]);
+    g_alpha[j] += (w.pc[j] + S[j] - col.snubar[j]) * w.s.lam[j];  // dPHI_j/dalpha_j lam_j
+  }
+}
+
+// Label-401 branch, colour r of 5: TAUNU enters PHI through both DERIVs
+// (5-banded): g_tau entries i = r (mod 5), given lam (w.s.lam) and S.
+template <int kMax>
+SYNTHE_HD void l401_tau(const Column<kMax>& col, Work<kMax>& w, int r, double* g_tau) {
+  const int n = col.n;
+  for (int i = 0; i < n; ++i) {
+    w.d0[i] = Dual(w.taunu[i], i % 5 == r ? 1.0 : 0.0);
+    w.d1[i] = Dual(w.pa[i]);
+  }
+  josh401_j(n, w);
+  for (int j = 0; j < n; ++j) {
+    const int i = seeded_in(j - 2, r, 5);
+    if (i >= 0 && i < n) g_tau[i] += w.s.lam[j] * col.alpha[j] * w.d3[j].d;
+  }
+}
+
 // g_abtot, g_alpha, g_snubar [n] = v * dF/d(input) for every input; returns F.
 template <int kMax>
 SYNTHE_HD double total_flux_vjp(const Column<kMax>& col, const JoshTab& tb, Work<kMax>& w, double v,
@@ -538,115 +683,17 @@ SYNTHE_HD double total_flux_vjp(const Column<kMax>& col, const JoshTab& tb, Work
   const double F = total_flux64(col, tb, w, 1e-13, &took401);
   for (int i = 0; i < n; ++i) g_abtot[i] = g_alpha[i] = g_snubar[i] = w.g_tau[i] = 0.0;
   if (took401) {
-    // converged iterate S (w.pa); F = (S[1] - S[0]) / (TAUNU[1] - TAUNU[0]) / 3
-    const double* S = w.pa;
-    const double dt = w.taunu[1] - w.taunu[0];
-    const double dS = S[1] - S[0];
-    w.g_tau[0] = dS / (3.0 * dt * dt);
-    w.g_tau[1] = -dS / (3.0 * dt * dt);
-    // band[j][o] = dPHI_j / dS[j - 2 + o]
-    for (int j = 0; j < n; ++j) {
+    for (int j = 0; j < n; ++j)
       for (int o = 0; o < 5; ++o) w.s.band[j][o] = 0.0;
-      w.d0[j] = Dual(w.taunu[j]);
-    }
-    for (int r = 0; r < 5; ++r) {
-      for (int i = 0; i < n; ++i) w.d1[i] = Dual(S[i], i % 5 == r ? 1.0 : 0.0);
-      josh401_j(n, w);
-      for (int j = 0; j < n; ++j) {
-        const int i = seeded_in(j - 2, r, 5);
-        if (i >= 0 && i < n) w.s.band[j][i - j + 2] = col.alpha[j] * (w.d3[j].d + w.d1[j].d);
-      }
-    }
-    // dPHI_j / dalpha_j = J(S)_j + S_j - snubar_j (values of the last pass)
-    for (int j = 0; j < n; ++j) w.s.tmp[j] = w.d3[j].v + S[j] - col.snubar[j];
-    for (int j = 0; j < n; ++j) g_alpha[j] = w.s.tmp[j];
-    // lam = dF/dS + (dPHI/dS)^T lam (the forward's iteration transposed)
-    for (int i = 0; i < n; ++i) w.s.lam[i] = 0.0;
-    w.s.lam[0] = -1.0 / (3.0 * dt);
-    w.s.lam[1] = 1.0 / (3.0 * dt);
-    for (int it = 0; it < 4 * kMax401; ++it) {
-      double err = 0.0;
-      for (int i = 0; i < n; ++i) {
-        double s = i == 0 ? -1.0 / (3.0 * dt) : (i == 1 ? 1.0 / (3.0 * dt) : 0.0);
-        for (int j = i - 2; j <= i + 2; ++j)
-          if (j >= 0 && j < n) s += w.s.band[j][i - j + 2] * w.s.lam[j];
-        w.s.tmp[i] = s;
-      }
-      for (int i = 0; i < n; ++i) {
-        const double den = fabs(w.s.tmp[i]) > kEps ? fabs(w.s.tmp[i]) : kEps;
-        err += fabs(w.s.tmp[i] - w.s.lam[i]) / den;
-        w.s.lam[i] = w.s.tmp[i];
-      }
-      if (err < 1e-13) break;
-    }
-    for (int j = 0; j < n; ++j) {
-      g_snubar[j] = w.s.lam[j] * (1.0 - col.alpha[j]);
-      g_alpha[j] *= w.s.lam[j];
-    }
-    // TAUNU enters PHI through both DERIVs (5-banded)
-    for (int i = 0; i < n; ++i) w.d1[i] = Dual(S[i]);
-    for (int r = 0; r < 5; ++r) {
-      for (int i = 0; i < n; ++i) w.d0[i] = Dual(w.taunu[i], i % 5 == r ? 1.0 : 0.0);
-      josh401_j(n, w);
-      for (int j = 0; j < n; ++j) {
-        const int i = seeded_in(j - 2, r, 5);
-        if (i >= 0 && i < n) w.g_tau[i] += w.s.lam[j] * col.alpha[j] * w.d3[j].d;
-      }
-    }
+    for (int r = 0; r < 5; ++r) l401_band(col, w, r, w.s.band);
+    l401_adjoint(col, w, w.s.band, g_snubar, g_alpha, w.g_tau);
+    for (int r = 0; r < 5; ++r) l401_tau(col, w, r, w.g_tau);
   } else {
-    // sensitivities of F to the fixed point's inputs XB, XA
-    adjoint64(tb, w.xa, w.lam, 1e-13, 400, w.x.kr);
-    double gs0 = 0.0, ga0 = 0.0;
-    for (int k = 0; k < kXt; ++k) {
-      double cx = 0.0;
-      for (int m = 0; m < kXt; ++m) cx += (double)tb.coefj32[k * kXt + m] * w.xs[m];
-      double gxb = w.lam[k] * (1.0 - w.xa[k]);
-      double gxa = w.lam[k] * (cx - w.x.xb[k]);
-      if (tb.xtau[k] < w.taunu[0]) {
-        gs0 += gxb;
-        ga0 += gxa;
-        gxb = gxa = 0.0;
-      } else {
-        if (!(w.x.xsbar[k] > kEps)) gxb = 0.0;
-        if (w.x.xalpha[k] < 0.0 || w.x.xalpha[k] > 1.0) gxa = 0.0;
-      }
-      w.x.gxb[k] = gxb;
-      w.x.gxa[k] = gxa;
-    }
-    if (!(col.snubar[0] > kEps)) gs0 = 0.0;
-    if (col.alpha[0] < 0.0 || col.alpha[0] > 1.0) ga0 = 0.0;
-    g_snubar[0] = gs0;
-    g_alpha[0] = ga0;
-    // MAP1 of snubar and alpha on the nodes TAUNU: values, then nodes
-    for (int r = 0; r < 4; ++r) {
-      for (int i = 0; i < n; ++i) {
-        w.d0[i] = Dual(w.taunu[i]);
-        w.d1[i] = Dual(col.snubar[i], (i & 3) == r ? 1.0 : 0.0);
-      }
-      map1_t(w.d0, w.d1, n, tb.xtau, w.x.out1);
-      for (int i = 0; i < n; ++i) w.d1[i] = Dual(col.alpha[i], (i & 3) == r ? 1.0 : 0.0);
-      map1_t(w.d0, w.d1, n, tb.xtau, w.x.out2);
-      for (int k = 0; k < kXt; ++k) {
-        const int j = seeded_in(w.x.base[k] - 3, r, 4);
-        if (j < 0 || j >= n) continue;
-        g_snubar[j] += w.x.gxb[k] * w.x.out1[k].d;
-        g_alpha[j] += w.x.gxa[k] * w.x.out2[k].d;
-      }
-      for (int i = 0; i < n; ++i) {
-        w.d0[i] = Dual(w.taunu[i], (i & 3) == r ? 1.0 : 0.0);
-        w.d1[i] = Dual(col.snubar[i]);
-      }
-      map1_t(w.d0, w.d1, n, tb.xtau, w.x.out1);
-      for (int i = 0; i < n; ++i) w.d1[i] = Dual(col.alpha[i]);
-      map1_t(w.d0, w.d1, n, tb.xtau, w.x.out2);
-      for (int k = 0; k < kXt; ++k) {
-        const int j = seeded_in(w.x.base[k] - 3, r, 4);
-        if (j < 0 || j >= n) continue;
-        w.g_tau[j] += w.x.gxb[k] * w.x.out1[k].d + w.x.gxa[k] * w.x.out2[k].d;
-      }
-    }
+    xtau_adjoint(col, tb, w, &g_snubar[0], &g_alpha[0]);
+    for (int r = 0; r < 4; ++r) xtau_pass(col, tb, w, r, g_snubar, g_alpha, w.g_tau);
   }
-  integ_vjp(col, w, g_abtot);
+  tau_suffix(n, w);
+  for (int r = 0; r < 4; ++r) integ_pass(col, w, r, g_abtot);
   for (int i = 0; i < n; ++i) {
     g_abtot[i] *= v;
     g_alpha[i] *= v;

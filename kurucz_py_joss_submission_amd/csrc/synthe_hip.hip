@@ -1813,48 +1813,84 @@ __global__ void k_vjp_order(int D, long W, const double* __restrict__ rhox, cons
     order[atomicAdd(&counts[0], 1)] = (int32_t)w;
 }
 
-__global__ __launch_bounds__(kVjpBlock) void k_rt_vjp(int D, long W, const int32_t* __restrict__ order,
-                                                      const double* __restrict__ wl,
-                                                      const double* __restrict__ temp, const double* __restrict__ rhox,
-                                                      const double* __restrict__ ca, const double* __restrict__ cs,
-                                                      const double* __restrict__ la, const double* __restrict__ ls,
-                                                      const double* __restrict__ src, const double* __restrict__ v,
-                                                      DeviceTables tb, double* __restrict__ g_ca,
-                                                      double* __restrict__ g_cs, double* __restrict__ g_la,
-                                                      double* __restrict__ g_b, double* __restrict__ g_src) {
-  // outputs were zeroed by the caller
+// The vjp of one wavelength in stages, each with the parallelism it has (the
+// per-column solve is one long dependent chain; the colour passes of
+// rt_adjoint.h are independent and write disjoint entries):
+//   k_vjp_solve   one lane per wavelength: the column, the FP64 solve and, in
+//                 the XTAU branch, the fixed point's adjoint;
+//   k_vjp_pass    one lane per (wavelength, colour of 5): the XTAU MAP1
+//                 passes (colours 0-3) or the label-401 band passes;
+//   k_vjp_adj401  one lane per label-401 wavelength: its adjoint iteration;
+//   k_vjp_tau401  one lane per (label-401 wavelength, colour of 5): TAUNU;
+//   k_vjp_finish  one lane per (wavelength, colour of 4): INTEG/PARCOE and the
+//                 chain to the outputs for depths j = colour (mod 4).
+// A wavelength's state lives in its record between stages; each entry is
+// owned by one lane of a stage, which reads, updates and writes it back, so
+// every sum runs in the order adj::total_flux_vjp runs it.
+struct VjpRecord {
+  int n, flag;  // depths in the solve; 0 XTAU branch, 1 label 401, -1 no work
+  double vw;
+  int idx[kAdjMaxDepth];
+  int base[adj::kXt];
+  double rho[kAdjMaxDepth], abtot[kAdjMaxDepth], alpha[kAdjMaxDepth], snubar[kAdjMaxDepth];
+  double taunu[kAdjMaxDepth], S[kAdjMaxDepth], lam[kAdjMaxDepth];
+  double gxb[adj::kXt], gxa[adj::kXt];
+  double band[kAdjMaxDepth][5];
+  double g_tau[kAdjMaxDepth], g_snubar[kAdjMaxDepth], g_alpha[kAdjMaxDepth];
+};
+
+using VjpColumn = adj::Column<kAdjMaxDepth>;
+using VjpWork = adj::Work<kAdjMaxDepth>;
+
+__device__ __forceinline__ void vjp_load(const VjpRecord& R, VjpColumn& col, VjpWork& wk) {
+  col.n = R.n;
+  for (int j = 0; j < R.n; ++j) {
+    col.rho[j] = R.rho[j];
+    col.abtot[j] = R.abtot[j];
+    col.alpha[j] = R.alpha[j];
+    col.snubar[j] = R.snubar[j];
+    wk.taunu[j] = R.taunu[j];
+  }
+}
+
+__global__ __launch_bounds__(kVjpBlock) void k_vjp_solve(int D, long W, const int32_t* __restrict__ order,
+                                                         const double* __restrict__ wl,
+                                                         const double* __restrict__ temp,
+                                                         const double* __restrict__ rhox,
+                                                         const double* __restrict__ ca, const double* __restrict__ cs,
+                                                         const double* __restrict__ la, const double* __restrict__ ls,
+                                                         const double* __restrict__ src, const double* __restrict__ v,
+                                                         DeviceTables tb, VjpRecord* __restrict__ recs) {
   __shared__ float s_cj[adj::kXt * adj::kXt];
   __shared__ double s_vec[3][adj::kXt * kVjpBlock];
   for (int i = threadIdx.x; i < adj::kXt * adj::kXt; i += kVjpBlock) s_cj[i] = tb.coefj32[i];
   __syncthreads();  // the only barrier: lanes leave independently below
   const long t = blockIdx.x * (long)kVjpBlock + threadIdx.x;
   if (t >= W) return;
+  VjpRecord& R = recs[t];
   const long w = order[t];
   const double vw = v[w];
-  if (vw == 0.0) return;
   int idx[kAdjMaxDepth];
-  const int n = vjp_depths(D, W, w, rhox, ca, cs, la, ls, idx);
-  if (n < 4) return;
+  const int n = vw != 0.0 ? vjp_depths(D, W, w, rhox, ca, cs, la, ls, idx) : 0;
+  R.flag = -1;
+  R.n = n;
+  R.vw = vw;
+  if (vw == 0.0 || n < 4) return;
   const double wv = wl[w];
   const double freq = kCLightNm / (wv > 1e-12 ? wv : 1e-12);
   const double f3 = pow(freq / 1.0e15, 3.0);
-  adj::Column<kAdjMaxDepth> col;
-  adj::Work<kAdjMaxDepth> wk;
-  // the continuum source B(T(d)) and the line source S at depth d
-  auto sources = [&](int d, long i, double& B, double& S) {
-    B = planck_one(freq, f3, temp[d]);
-    S = B;
-    if (src) {
-      const double s0 = src[i];
-      S = isfinite(s0) ? s0 : B;
-    }
-  };
+  VjpColumn col;
+  VjpWork wk;
   col.n = n;
   for (int j = 0; j < n; ++j) {
     const int d = idx[j];
     const long i = (long)d * W + w;
-    double B, S;
-    sources(d, i, B, S);
+    const double B = planck_one(freq, f3, temp[d]);
+    double S = B;
+    if (src) {
+      const double s0 = src[i];
+      S = isfinite(s0) ? s0 : B;
+    }
     const double ac = ca[i], al = la[i], sc = cs[i], sgl = ls[i];
     double ab = ac + al + sc + sgl;
     ab = ab > kEps38 ? ab : kEps38;
@@ -1865,33 +1901,179 @@ __global__ __launch_bounds__(kVjpBlock) void k_rt_vjp(int D, long W, const int32
     col.abtot[j] = ab;
     col.alpha[j] = alpha;
     col.snubar[j] = dn > 0 ? (ac * B + al * S) / dn : B;
+    R.idx[j] = d;
+    R.rho[j] = col.rho[j];
+    R.abtot[j] = ab;
+    R.alpha[j] = alpha;
+    R.snubar[j] = col.snubar[j];
   }
   const adj::JoshTab jt{tb.xtau, tb.ch, s_cj};
   wk.xs = adj::Vec{&s_vec[0][threadIdx.x], kVjpBlock};
   wk.xa = adj::Vec{&s_vec[1][threadIdx.x], kVjpBlock};
   wk.lam = adj::Vec{&s_vec[2][threadIdx.x], kVjpBlock};
-  double gab[kAdjMaxDepth], gal[kAdjMaxDepth], gsb[kAdjMaxDepth];
-  adj::total_flux_vjp(col, jt, wk, vw, gab, gal, gsb);
+  int took401 = 0;
+  adj::total_flux64(col, jt, wk, 1e-13, &took401);
   for (int j = 0; j < n; ++j) {
-    const int d = idx[j];
+    R.taunu[j] = wk.taunu[j];
+    R.g_tau[j] = R.g_snubar[j] = R.g_alpha[j] = 0.0;
+  }
+  if (took401) {
+    for (int j = 0; j < n; ++j) {
+      R.S[j] = wk.pa[j];
+      for (int o = 0; o < 5; ++o) R.band[j][o] = 0.0;
+    }
+  } else {
+    double gs0, ga0;
+    adj::xtau_adjoint(col, jt, wk, &gs0, &ga0);
+    R.g_snubar[0] = gs0;
+    R.g_alpha[0] = ga0;
+    for (int k = 0; k < adj::kXt; ++k) {
+      R.base[k] = wk.x.base[k];
+      R.gxb[k] = wk.x.gxb[k];
+      R.gxa[k] = wk.x.gxa[k];
+    }
+  }
+  R.flag = took401;
+}
+
+__global__ __launch_bounds__(64) void k_vjp_pass(long W, DeviceTables tb, VjpRecord* __restrict__ recs) {
+  const long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (u >= W * 5) return;
+  VjpRecord& R = recs[u / 5];
+  const int r = (int)(u % 5);
+  const int flag = R.flag;
+  if (flag < 0 || (flag == 0 && r == 4)) return;
+  VjpColumn col;
+  VjpWork wk;
+  vjp_load(R, col, wk);
+  const int n = col.n;
+  if (flag == 0) {
+    for (int k = 0; k < adj::kXt; ++k) {
+      wk.x.base[k] = R.base[k];
+      wk.x.gxb[k] = R.gxb[k];
+      wk.x.gxa[k] = R.gxa[k];
+    }
+    double gs[kAdjMaxDepth], ga[kAdjMaxDepth], gt[kAdjMaxDepth];
+    for (int j = r; j < n; j += 4) {
+      gs[j] = R.g_snubar[j];
+      ga[j] = R.g_alpha[j];
+      gt[j] = R.g_tau[j];
+    }
+    const adj::JoshTab jt{tb.xtau, tb.ch, tb.coefj32};
+    adj::xtau_pass(col, jt, wk, r, gs, ga, gt);
+    for (int j = r; j < n; j += 4) {
+      R.g_snubar[j] = gs[j];
+      R.g_alpha[j] = ga[j];
+      R.g_tau[j] = gt[j];
+    }
+  } else {
+    for (int j = 0; j < n; ++j) wk.pa[j] = R.S[j];
+    adj::l401_band(col, wk, r, wk.s.band);
+    for (int j = 0; j < n; ++j) {
+      const int i = adj::seeded_in(j - 2, r, 5);
+      if (i >= 0 && i < n) R.band[j][i - j + 2] = wk.s.band[j][i - j + 2];
+    }
+  }
+}
+
+__global__ __launch_bounds__(64) void k_vjp_adj401(long W, VjpRecord* __restrict__ recs) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= W) return;
+  VjpRecord& R = recs[t];
+  if (R.flag != 1) return;
+  VjpColumn col;
+  VjpWork wk;
+  vjp_load(R, col, wk);
+  const int n = col.n;
+  double gs[kAdjMaxDepth], ga[kAdjMaxDepth], gt[kAdjMaxDepth];
+  for (int j = 0; j < n; ++j) {
+    wk.pa[j] = R.S[j];
+    gs[j] = R.g_snubar[j];
+    ga[j] = R.g_alpha[j];
+    gt[j] = R.g_tau[j];
+  }
+  adj::l401_adjoint(col, wk, R.band, gs, ga, gt);
+  for (int j = 0; j < n; ++j) {
+    R.g_snubar[j] = gs[j];
+    R.g_alpha[j] = ga[j];
+    R.g_tau[j] = gt[j];
+    R.lam[j] = wk.s.lam[j];
+  }
+}
+
+__global__ __launch_bounds__(64) void k_vjp_tau401(long W, VjpRecord* __restrict__ recs) {
+  const long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (u >= W * 5) return;
+  VjpRecord& R = recs[u / 5];
+  const int r = (int)(u % 5);
+  if (R.flag != 1) return;
+  VjpColumn col;
+  VjpWork wk;
+  vjp_load(R, col, wk);
+  const int n = col.n;
+  double gt[kAdjMaxDepth];
+  for (int j = 0; j < n; ++j) {
+    wk.pa[j] = R.S[j];
+    wk.s.lam[j] = R.lam[j];
+  }
+  for (int i = r; i < n; i += 5) gt[i] = R.g_tau[i];
+  adj::l401_tau(col, wk, r, gt);
+  for (int i = r; i < n; i += 5) R.g_tau[i] = gt[i];
+}
+
+__global__ __launch_bounds__(64) void k_vjp_finish(int D, long W, const int32_t* __restrict__ order,
+                                                   const double* __restrict__ wl, const double* __restrict__ temp,
+                                                   const double* __restrict__ ca, const double* __restrict__ la,
+                                                   const double* __restrict__ src,
+                                                   const VjpRecord* __restrict__ recs, double* __restrict__ g_ca,
+                                                   double* __restrict__ g_cs, double* __restrict__ g_la,
+                                                   double* __restrict__ g_b, double* __restrict__ g_src) {
+  const long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (u >= W * 4) return;
+  const long t = u / 4;
+  const int r = (int)(u % 4);
+  const VjpRecord& R = recs[t];
+  if (R.flag < 0) return;
+  VjpColumn col;
+  VjpWork wk;
+  vjp_load(R, col, wk);
+  const int n = col.n;
+  for (int j = 0; j < n; ++j) wk.g_tau[j] = R.g_tau[j];
+  adj::tau_suffix(n, wk);
+  double gab[kAdjMaxDepth];
+  for (int j = r; j < n; j += 4) gab[j] = 0.0;
+  adj::integ_pass(col, wk, r, gab);
+  const long w = order[t];
+  const double vw = R.vw;
+  const double wv = wl[w];
+  const double freq = kCLightNm / (wv > 1e-12 ? wv : 1e-12);
+  const double f3 = pow(freq / 1.0e15, 3.0);
+  for (int j = r; j < n; j += 4) {
+    const int d = R.idx[j];
     const long i = (long)d * W + w;
+    const double g_ab = gab[j] * vw, g_al = R.g_alpha[j] * vw, g_sb = R.g_snubar[j] * vw;
     const double ac = ca[i], al = la[i];
     const double ab = col.abtot[j], alpha = col.alpha[j], sb = col.snubar[j];
     const double dn = ac + al;
-    const double galpha = (alpha > 0.0 && alpha < 1.0) ? gal[j] : 0.0;
+    const double galpha = (alpha > 0.0 && alpha < 1.0) ? g_al : 0.0;
     const double da_dab = -alpha / ab;  // d alpha / d(ac or al)
-    double gsb_ac = 0.0, gsb_al = 0.0, gB = gsb[j], gS = 0.0;
+    double gsb_ac = 0.0, gsb_al = 0.0, gB = g_sb, gS = 0.0;
     if (dn > 0) {
-      double B, S;
-      sources(d, i, B, S);
-      gsb_ac = gsb[j] * (B - sb) / dn;
-      gsb_al = gsb[j] * (S - sb) / dn;
-      gB = gsb[j] * ac / dn;
-      gS = gsb[j] * al / dn;
+      // the continuum source B(T(d)) and the line source S at depth d
+      const double B = planck_one(freq, f3, temp[d]);
+      double S = B;
+      if (src) {
+        const double s0 = src[i];
+        S = isfinite(s0) ? s0 : B;
+      }
+      gsb_ac = g_sb * (B - sb) / dn;
+      gsb_al = g_sb * (S - sb) / dn;
+      gB = g_sb * ac / dn;
+      gS = g_sb * al / dn;
     }
-    if (g_ca) g_ca[i] = gab[j] + galpha * da_dab + gsb_ac;
-    if (g_la) g_la[i] = gab[j] + galpha * da_dab + gsb_al;
-    if (g_cs) g_cs[i] = gab[j] + galpha * (1.0 - alpha) / ab;
+    if (g_ca) g_ca[i] = g_ab + galpha * da_dab + gsb_ac;
+    if (g_la) g_la[i] = g_ab + galpha * da_dab + gsb_al;
+    if (g_cs) g_cs[i] = g_ab + galpha * (1.0 - alpha) / ab;
     if (src) {
       if (g_b) g_b[i] = gB;
       if (g_src) g_src[i] = gS;
@@ -2811,18 +2993,32 @@ int synthe_lte_vjp(synthe_ctx* ctx, long n_depths, long n_wl, const double* wl, 
                        (unsigned long long*)z);
     HIP_TRY(hipGetLastError());
   }
-  int rc = ensure_ws(ctx, align_up((size_t)n_wl * sizeof(int32_t)) + 256);
+  const size_t o_cnt = align_up((size_t)n_wl * sizeof(int32_t));
+  const size_t o_rec = o_cnt + 256;
+  int rc = ensure_ws(ctx, o_rec + (size_t)n_wl * sizeof(VjpRecord));
   if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
   int32_t* order = (int32_t*)ctx->ws;
-  int* counts = (int*)((char*)ctx->ws + align_up((size_t)n_wl * sizeof(int32_t)));
-  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, (hipStream_t)stream, 2, counts);
+  int* counts = (int*)((char*)ctx->ws + o_cnt);
+  VjpRecord* recs = (VjpRecord*)((char*)ctx->ws + o_rec);
+  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, 2, counts);
   HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(k_vjp_order, dim3(blocks_for(n_wl, 256)), dim3(256), 0, (hipStream_t)stream, (int)n_depths,
-                     n_wl, rhox, cont_abs, cont_scat, line_abs, line_scat, v_flux, ctx->tb.xtau, counts, order);
+  hipLaunchKernelGGL(k_vjp_order, dim3(blocks_for(n_wl, 256)), dim3(256), 0, st, (int)n_depths, n_wl, rhox,
+                     cont_abs, cont_scat, line_abs, line_scat, v_flux, ctx->tb.xtau, counts, order);
   HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(k_rt_vjp, dim3(blocks_for(n_wl, kVjpBlock)), dim3(kVjpBlock), 0, (hipStream_t)stream,
-                     (int)n_depths, n_wl, order, wl, temperature, rhox, cont_abs, cont_scat, line_abs, line_scat,
-                     line_source, v_flux, ctx->tb, g_cont_abs, g_cont_scat, g_line_abs, g_planck, g_line_source);
+  hipLaunchKernelGGL(k_vjp_solve, dim3(blocks_for(n_wl, kVjpBlock)), dim3(kVjpBlock), 0, st, (int)n_depths,
+                     n_wl, order, wl, temperature, rhox, cont_abs, cont_scat, line_abs, line_scat, line_source,
+                     v_flux, ctx->tb, recs);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_vjp_pass, dim3(blocks_for(n_wl * 5, 64)), dim3(64), 0, st, n_wl, ctx->tb, recs);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_vjp_adj401, dim3(blocks_for(n_wl, 64)), dim3(64), 0, st, n_wl, recs);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_vjp_tau401, dim3(blocks_for(n_wl * 5, 64)), dim3(64), 0, st, n_wl, recs);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_vjp_finish, dim3(blocks_for(n_wl * 4, 64)), dim3(64), 0, st, (int)n_depths, n_wl, order,
+                     wl, temperature, cont_abs, line_abs, line_source, recs, g_cont_abs, g_cont_scat,
+                     g_line_abs, g_planck, g_line_source);
   HIP_TRY(hipGetLastError());
   return SYNTHE_OK;
 }
