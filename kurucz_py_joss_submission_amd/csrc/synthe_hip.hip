@@ -1711,14 +1711,27 @@ __global__ __launch_bounds__(64) void k_far_adj(GatherArgs g, const double* __re
       const long c = head_c(h), M = h.hi - c;
       const double n10d = (double)(h.nhi - c), md = (double)M;
       const double kd = (double)(w0 - c);
-      double q = 0.0;
+      // sum_p up_p / k_p^2 over the lane's 8 pixels with one shared
+      // reciprocal (the far_chunk scheme): P_p = prod_{i <= p} qk_i, 1/qk_p =
+      // P_{p-1} / P_p; qk = max(k^2, 1) keeps masked pixels (up = 0) finite
+      double up[kP], qk[kP], P[kP];
 #pragma unroll
       for (int p = 0; p < kP; ++p) {
         const double k = kd + (double)p;
         const double ka = fabs(k);
-        const double up = (ka > n10d && ka <= md) ? uu[p] : 0.0;
-        q += div_fast(up, fmax(k * k, 1.0));
+        up[p] = (ka > n10d && ka <= md) ? uu[p] : 0.0;
+        qk[p] = fmax(k * k, 1.0);
+        P[p] = p == 0 ? qk[0] : P[p - 1] * qk[p];
       }
+      double y = __builtin_amdgcn_rcp(P[kP - 1]);
+      y = __builtin_fma(y, __builtin_fma(-P[kP - 1], y, 1.0), y);
+      double q = 0.0;
+#pragma unroll
+      for (int p = kP - 1; p > 0; --p) {
+        q = __builtin_fma(up[p], y * P[p - 1], q);
+        y *= qk[p];
+      }
+      q = __builtin_fma(up[0], y, q);
       q = wave_sum(q);
       if (lane == 0) atomicAdd(q_entry + j, q);
     }
