@@ -1,0 +1,125 @@
+"""The RT vector-Jacobian product on the GPU (synthe_lte_vjp: the staged
+k_vjp_* kernels) against its CPU build (csrc/rt_adjoint.h total_flux_vjp,
+compiled with g++), column by column: ordinary columns, label-401 columns
+(optically thick at the top), columns scattering at albedo 0.995-0.9995 at
+every depth (the Krylov finish), a masked depth and a zero weight."""
+
+from __future__ import annotations
+
+import ctypes as ct
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+SRC = ROOT / "tests" / "native" / "rt_adj_check.cpp"
+INC = ROOT / "kurucz_py_joss_submission_amd" / "csrc"
+TABLES = ROOT / "kurucz_py_joss_submission_amd" / "data" / "kurucz_tables.npz"
+
+
+@pytest.fixture(scope="module")
+def cpu_vjp(tmp_path_factory):
+    out = tmp_path_factory.mktemp("adjgpu") / "libadj.so"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", f"-I{INC}", str(SRC),
+                    "-o", str(out)], check=True)
+    lib = ct.CDLL(str(out))
+    vp = ct.c_void_p
+    lib.adj_vjp.restype = ct.c_double
+    lib.adj_vjp.argtypes = [ct.c_int] + [vp] * 7 + [ct.c_double] + [vp] * 3
+    z = np.load(TABLES)
+    tabs = tuple(np.ascontiguousarray(z[k], t) for k, t in (("xtau", np.float64), ("ch", np.float64),
+                                                            ("coefj", np.float32)))
+    return lib, tabs
+
+
+def _columns(D, W, rng):
+    rhox = np.logspace(-4.3, 1.4, D)
+    T = 3800.0 + 7000.0 * (rhox / rhox[-1]) ** 0.3
+    ca, cs, la, ls = (np.zeros((D, W)) for _ in range(4))
+    for w in range(W):
+        kind = w % 4
+        k0 = 10 ** rng.uniform(-1.5, 0.5) * (1e7 if kind == 1 else 1.0)
+        acont = k0 * (rhox / rhox[0]) ** rng.uniform(0.3, 0.8) * (1 + 0.05 * rng.standard_normal(D))
+        if kind == 2:
+            albedo = 1 - 10 ** rng.uniform(-3.3, -2.3, D)
+        else:
+            albedo = 10 ** rng.uniform(-3, -0.3, D)
+        ca[:, w] = acont
+        cs[:, w] = acont * albedo / (1 - albedo)
+        la[:, w] = acont * 10 ** rng.uniform(-2, 1.5) * np.exp(-((np.arange(D) - D / 2) / (D / 3)) ** 2)
+    ca[10, 7] = np.nan  # one depth of one column leaves the solve
+    return rhox, T, ca, cs, la, ls
+
+
+def _chain(ac, al, ab, alpha, sb, B, S, gab, gal, gsb):
+    """dL/d(cont_abs, cont_scat, line_abs, B, S) of one depth from dL/d(abtot,
+    alpha, snubar), through abtot = ac + al + sc + sl, alpha = (sc + sl) / abtot
+    and snubar = (ac B + al S) / (ac + al)."""
+    dn = ac + al
+    galpha = gal if 0.0 < alpha < 1.0 else 0.0
+    dadab = -alpha / ab
+    gsb_ac = gsb * (B - sb) / dn if dn > 0 else 0.0
+    gsb_al = gsb * (S - sb) / dn if dn > 0 else 0.0
+    gB = gsb * ac / dn if dn > 0 else gsb
+    gS = gsb * al / dn if dn > 0 else 0.0
+    return (gab + galpha * dadab + gsb_ac, gab + galpha * (1.0 - alpha) / ab, gab + galpha * dadab + gsb_al, gB, gS)
+
+
+@pytest.mark.gpu
+def test_lte_vjp_matches_cpu_reverse_mode(gpu, oracle, cpu_vjp):
+    import torch
+
+    from kurucz_py_joss_submission_amd import _native as N
+
+    lib, tabs = cpu_vjp
+    rng = np.random.default_rng(2024)
+    D, W = 56, 96
+    rhox, T, ca, cs, la, ls = _columns(D, W, rng)
+    wl = np.linspace(400.0, 401.0, W)
+    bnu = oracle.freq_quantities(wl, T)[2]
+    src = bnu * (1 + 0.2 * rng.uniform(-1, 1, (D, W)))
+    v = rng.uniform(0.5, 1.5, W)
+    v[5] = 0.0
+    dev = lambda a: torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda")  # noqa: E731
+    d_in = [dev(x) for x in (wl, T, rhox, ca, cs, la, ls, src, v)]
+    outs = [torch.full((D, W), np.nan, dtype=torch.float64, device="cuda") for _ in range(5)]
+    ctx = gpu
+    N.check("synthe_lte_vjp", ctx.lib.synthe_lte_vjp(ctx.handle, D, W, *[N.ptr(t) for t in d_in],
+                                                     *[N.ptr(t) for t in outs], N.stream_handle()))
+    torch.cuda.synchronize()
+    got = [o.cpu().numpy() for o in outs]
+    want = [np.zeros((D, W)) for _ in range(5)]
+    n401 = 0
+    for w in range(W):
+        if v[w] == 0.0:
+            continue
+        cols = np.stack([ca[:, w], cs[:, w], la[:, w], ls[:, w]])
+        ok = np.all(np.isfinite(cols), axis=0) & np.all(cols < 3.4e38, axis=0) & (rhox > 0)
+        idx = np.nonzero(ok)[0]
+        ac, sc, al, sl = (x[idx] for x in cols)
+        B, S = bnu[idx, w], src[idx, w]
+        ab = np.maximum(ac + al + sc + sl, 1e-38)
+        alpha = np.clip((sc + sl) / ab, 0.0, 1.0)
+        dn = ac + al
+        sb = np.where(dn > 0, (ac * B + al * S) / np.where(dn > 0, dn, 1.0), B)
+        n = idx.size
+        g = [np.zeros(n) for _ in range(3)]
+        args = [np.ascontiguousarray(x) for x in (rhox[idx], ab, alpha, sb)]
+        lib.adj_vjp(n, *[a.ctypes.data for a in args], *[t.ctypes.data for t in tabs], float(v[w]),
+                    *[x.ctypes.data for x in g])
+        n401 += int(ab[0] * rhox[idx[0]] > tabs[0][-1])
+        for j, d in enumerate(idx):
+            for q, val in enumerate(_chain(ac[j], al[j], ab[j], alpha[j], sb[j], B[j], S[j],
+                                           g[0][j], g[1][j], g[2][j])):
+                want[q][d, w] = val
+    assert n401 >= W // 4 - 1  # the thick columns took the label-401 branch
+    assert np.count_nonzero(want[0]) > 0.8 * D * (W - 1)  # gradients at (almost) every depth
+    for q, name in enumerate(("cont_abs", "cont_scat", "line_abs", "planck", "line_source")):
+        scale = np.max(np.abs(want[q]), axis=0)
+        scale = np.where(scale > 0, scale, 1.0)
+        err = np.max(np.abs(got[q] - want[q]) / scale[None, :])
+        assert np.all(np.isfinite(got[q])), name
+        assert err < 1e-9, (name, err)
+    assert np.all(got[0][10, 7] == 0.0) and np.all(got[0][:, 5] == 0.0)
