@@ -1817,11 +1817,11 @@ __global__ void k_far_finish(int D, const int64_t* __restrict__ regions, const W
 // through engine/radiative.py:42-218 masks): v[w] = dL/dF_nu(w); outputs
 // dL/d(cont_abs, cont_scat, line_abs) [D][W], dL/dB (the Planck function at
 // T(d) used as the continuum source) and dL/d(line_source), 0 on masked depths.
-// One lane per wavelength (the vjp is O(depths) per column).  The 51x51
-// loops of the scattering fixed point and its adjoint read COEFJ and the
-// vectors XS, XA, lam for every product: COEFJ is staged in LDS once per
-// workgroup and the three vectors live in LDS interleaved by lane (88 KB per
-// 64-lane workgroup), the rest in private memory.
+// O(depths) per column, in the stages below.  In k_vjp_solve the 51x51 loops
+// of the scattering fixed point and its adjoint read COEFJ and the vectors
+// XS, XA, lam for every product: COEFJ is staged in LDS once per workgroup and
+// the three vectors live in LDS interleaved by lane (88 KB per 64-lane
+// workgroup), the rest in private memory.
 constexpr int kVjpBlock = 64;
 
 // The depths of wavelength w that enter the solve, in increasing rho (the
