@@ -259,7 +259,8 @@ int synthe_batch_destroy(synthe_batch *b);
  * dL/d(line_source) [D][W] (outputs may be NULL).  The scattering fixed point
  * and the label-401 iteration are differentiated implicitly at FP64
  * convergence, O(depths) per wavelength.  At most 96 depths; uses the
- * context's workspace (one int per wavelength: the processing order). */
+ * context's workspace, about 13 KB per wavelength (the per-wavelength state
+ * between its kernel stages; 0.19 GB for 14,637 wavelengths). */
 int synthe_asynth_vjp(synthe_ctx *ctx, const synthe_lines *lines, const synthe_depths *depths, long n_wl,
                       const double *wl, const double *transp, const uint8_t *valid, const double *kapmin,
                       double cutoff, const int64_t *center_order, const int64_t *center_start,
