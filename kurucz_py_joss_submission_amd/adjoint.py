@@ -193,7 +193,10 @@ class SpectrumModel:
 
     def __init__(self, columns: S.AtmColumns, catalog: LineCatalog, wl_start: float, wl_end: float,
                  resolution: float = 300000.0, cutoff: float = 1e-3, precision: str = "fp32",
-                 electron_density: str = "columns"):
+                 electron_density: str = "columns", rhoxj: float = 0.0):
+        """rhoxj > 0: line scattering as in SpectrumPlan (engine/opacity.py:2407-2420):
+        the line opacity splits into absorption asynth (1 - f) and scattering
+        asynth f, f = exp(-rhox / rhoxj)."""
         torch = _torch()
         self.ctx = N.context()
         self.layers = LayerModel(columns, electron_density=electron_density)
@@ -221,6 +224,10 @@ class SpectrumModel:
         self.worder = to_device(np.argsort(soa.center_w, kind="stable").astype(np.int64))
         self.d_grid = to_device(self.grid, np.float64)
         self.rhox = to_device(self.layers.rhox, np.float64)
+        self.rhoxj = float(rhoxj)
+        fscat = np.exp(-self.layers.rhox / self.rhoxj) if self.rhoxj > 0.0 else np.zeros_like(self.layers.rhox)
+        self.fscat_col = to_device(fscat[:, None], np.float64)
+        self.one_minus_fscat = to_device((1.0 - fscat)[:, None], np.float64)
         half = 0.5 * (S.WLEDGE[:-1] + S.WLEDGE[1:])
         self.wledge = to_device(S.WLEDGE.copy())
         self.half = to_device(half)
@@ -268,6 +275,13 @@ class SpectrumModel:
                              b["cscat"], log_gf, log_gvdw, self)
 
     # ------------------------------------------------------------ internals
+    def _line_split(self, asy):
+        """(line absorption, line scattering) [D][W] from asynth, as SpectrumPlan."""
+        torch = _torch()
+        if self.rhoxj <= 0.0:
+            return asy, torch.zeros_like(asy)
+        return asy * self.one_minus_fscat, asy * self.fscat_col
+
     def _structs(self, T, rho, xne, txnxn, hckt, pop, dop, cgf, gvdw):
         p = N.ptr
         t = self.t
@@ -335,11 +349,11 @@ def _make_function():
                                                            p(m.corder), p(m.cstart), p(m.worder), p(asy), None, s))
             else:
                 asy.zero_()
-            zeros = torch.zeros_like(ca)
+            la, ls = m._line_split(asy)
             fnu = torch.empty(W, dtype=f8, device="cuda")
             fc = torch.empty_like(fnu)
             N.check("synthe_lte_spectrum", lib.synthe_lte_spectrum(h, D, W, p(m.d_grid), p(T), p(m.rhox), p(ca),
-                                                                   p(cs), p(asy), p(zeros), p(slinec), p(fnu),
+                                                                   p(cs), p(la), p(ls), p(slinec), p(fnu),
                                                                    p(fc), None, s))
             flux = fnu.clone()
             N.check("synthe_flux_to_lambda", lib.synthe_flux_to_lambda(h, W, p(m.d_grid), p(flux), s))
@@ -364,10 +378,14 @@ def _make_function():
             g_la = torch.empty_like(ca)
             g_b = torch.empty_like(ca)
             g_src = torch.empty_like(ca)
-            zeros = torch.zeros_like(ca)
-            N.check("synthe_lte_vjp", lib.synthe_lte_vjp(h, D, W, p(wl), p(T), p(m.rhox), p(ca), p(cs), p(asy),
-                                                         p(zeros), p(slinec), p(v), p(g_ca), p(g_cs), p(g_la),
+            la, ls = m._line_split(asy)
+            N.check("synthe_lte_vjp", lib.synthe_lte_vjp(h, D, W, p(wl), p(T), p(m.rhox), p(ca), p(cs), p(la),
+                                                         p(ls), p(slinec), p(v), p(g_ca), p(g_cs), p(g_la),
                                                          p(g_b), p(g_src), s))
+            # dL/dasynth: line scattering enters the solve as continuum
+            # scattering does (abtot and alpha, not snubar), so dL/d(line_scat)
+            # is the cont_scat output
+            g_line = g_la if m.rhoxj <= 0.0 else g_la * m.one_minus_fscat + g_cs * m.fscat_col
             # stimulated emission and Planck: direct T dependences
             Tc = torch.clamp(T, min=1.0)[:, None]
             hnu_k = (H_PLANCK / K_BOLTZ) * (C_NM / wl)[None, :]
@@ -375,7 +393,7 @@ def _make_function():
             ex = torch.exp(-xx)
             stim = 1.0 - ex
             dstim_dT = -ex * xx / Tc
-            g_T = (g_la * (asy / stim) * dstim_dT).sum(dim=1)
+            g_T = (g_line * (asy / stim) * dstim_dT).sum(dim=1)
             Tt = T[:, None]
             xb = hnu_k / Tt
             eb = torch.exp(-xb)
@@ -407,7 +425,7 @@ def _make_function():
                 s_dv = torch.empty_like(s_k0)
                 k0 = torch.empty_like(s_k0)
                 lines, depths = m._structs(T, rho, xne, txnxn, hckt, pop, dop, cgf, gvdw)
-                g_asy = g_la.contiguous()
+                g_asy = g_line.contiguous()
                 N.check("synthe_asynth_vjp", lib.synthe_asynth_vjp(
                     h, ctypes.byref(lines), ctypes.byref(depths), W, p(wl), p(transp), p(valid), p(kapmin),
                     m.cutoff, p(m.corder), p(m.cstart), p(m.worder), p(g_asy), p(s_k0), p(s_a), p(s_dv), p(k0), s))

@@ -36,12 +36,12 @@ def _loss_weights(W, seed=4):
     return rng.uniform(0.5, 1.5, W)
 
 
-def _model_grads(cols, cat, precision, cutoff=1e-3):
+def _model_grads(cols, cat, precision, cutoff=1e-3, rhoxj=0.0):
     import torch
 
     from kurucz_py_joss_submission_amd.adjoint import SpectrumModel
 
-    m = SpectrumModel(cols, cat, WL0, WL1, precision=precision, cutoff=cutoff)
+    m = SpectrumModel(cols, cat, WL0, WL1, precision=precision, cutoff=cutoff, rhoxj=rhoxj)
     T, da, lgf, lgw = m.base_parameters()
     for t in (T, da, lgf, lgw):
         t.requires_grad_(True)
@@ -54,11 +54,11 @@ def _model_grads(cols, cat, precision, cutoff=1e-3):
                                              for k, t in (("T", T), ("A", da), ("gf", lgf), ("gw", lgw))}
 
 
-def _oracle_loss(oracle, cols, cat, w, abund=None, cutoff=1e-3):
+def _oracle_loss(oracle, cols, cat, w, abund=None, cutoff=1e-3, rhoxj=0.0):
     from kurucz_py_joss_submission_amd import synthetic as S
 
     atm = S.build_bundle(cols, abund)
-    r = oracle.synthesize(atm, cat, WL0, WL1, 300000.0, cutoff=cutoff, keep_stages=False)
+    r = oracle.synthesize(atm, cat, WL0, WL1, 300000.0, cutoff=cutoff, rhoxj=rhoxj, keep_stages=False)
     return float(np.sum(w * r.flux))
 
 
@@ -202,3 +202,46 @@ def test_temperature_gradient_with_lines(gpu, oracle, adj_case):
         _t_checks(oracle, cols, cat, w, g, (5, 10, 15, 20, 25, 30, 35, 40), 2e-2, 0.05)
     finally:
         oracle.set_josh_fp64(False)
+
+
+@pytest.mark.gpu
+def test_adjoint_with_line_scattering(gpu, oracle, adj_case):
+    """rhoxj > 0 (line opacity split into absorption asynth (1 - f) and
+    scattering asynth f, f = exp(-rhox / rhoxj), engine/opacity.py:2407-2420):
+    the forward equals SpectrumPlan's with the same rhoxj, and the gradients
+    equal FP64 central differences of the oracle with that split; they differ
+    from the rhoxj = 0 gradients, so the split is exercised."""
+    import torch
+
+    from kurucz_py_joss_submission_amd import synthetic as S
+    from kurucz_py_joss_submission_amd.device import SpectrumPlan
+
+    cols, cat = adj_case
+    rhoxj = 0.1  # scattering above rhox ~ 0.1 g cm^-2, absorption below
+    m, flux, g = _model_grads(cols, cat, "fp64", 0.0, rhoxj)
+    plan = SpectrumPlan(S.build_bundle(cols), cat, WL0, WL1, 300000.0, cutoff=0.0, rhoxj=rhoxj)
+    plan.run()
+    torch.cuda.synchronize()
+    ref = plan.flux.cpu().numpy()
+    rel = np.abs(flux - ref) / np.abs(ref)
+    assert np.mean(rel > 1e-10) < 0.02 and np.max(rel) < 1e-7
+    _, _, g0 = _model_grads(cols, cat, "fp64", 0.0, 0.0)
+    w = _loss_weights(m.W)
+    loss = lambda k=cat, a=None: _oracle_loss(oracle, cols, k, w, a, 0.0, rhoxj)  # noqa: E731
+    oracle.set_josh_fp64(True, 1e-13)
+    try:
+        checks = []
+        for l in np.argsort(-np.abs(g["gf"]))[:3]:
+            h = 1e-4
+            up = loss(k=_cat_scaled(cat, "gf", l, 10 ** h))
+            dn = loss(k=_cat_scaled(cat, "gf", l, 10 ** -h))
+            checks.append((f"gf[{l}]", g["gf"][l], (up - dn) / (2 * h), g0["gf"][l]))
+        iz = list(S.ABUNDANCE_Z).index(26)
+        up, dn = loss(a={26: 1e-3}), loss(a={26: -1e-3})
+        checks.append(("A[26]", g["A"][iz], (up - dn) / 2e-3, g0["A"][iz]))
+    finally:
+        oracle.set_josh_fp64(False)
+    for name, adj, fd, adj0 in checks:
+        print(f"rhoxj {rhoxj}: {name}: adjoint {adj:.6e} fp64-fd {fd:.6e} (rhoxj 0: {adj0:.6e})")
+        assert abs(adj - fd) <= 2e-3 * abs(fd), name
+    assert max(abs(a - a0) / abs(a0) for _, a, _, a0 in checks) > 1e-2
