@@ -170,11 +170,19 @@ int synthe_josh_flux_batch(synthe_ctx *ctx, long n, long n_cols, const double *a
                            double *flux, int32_t *iters, synthe_stream stream);
 
 /* engine/opacity.py:2215-2228 (frequency quantities) and
- * physics/bfudge.py:312-406 in LTE (BFUDGE = 1): slinec [D][W] =
+ * physics/bfudge.py:72-168 in LTE (BFUDGE = 1): slinec [D][W] =
  * BNU * STIM / (1 - EHVKT).  Any output may be NULL. */
 int synthe_source_function(synthe_ctx *ctx, long n_depths, long n_wl, const double *wl,
                            const double *temperature, double *bnu, double *stim, double *ehvkt,
                            double *slinec, synthe_stream stream);
+
+/* physics/bfudge.py:72-168 (compute_bfudge_and_slinec) with a departure
+ * column: as synthe_source_function, with slinec = BNU * STIM / (BFUDGE(d) -
+ * EHVKT) for bfudge [D] (device; NULL = 1, the LTE case).  BFUDGE itself is
+ * BHYD^PH1 (BC1/BC2)^PC1 (BSI1/BSI2)^PSI1 per depth (host: physics.bfudge_column). */
+int synthe_source_function_bfudge(synthe_ctx *ctx, long n_depths, long n_wl, const double *wl,
+                                  const double *temperature, const double *bfudge, double *bnu, double *stim,
+                                  double *ehvkt, double *slinec, synthe_stream stream);
 
 /* engine/opacity.py:3852-3861: flux[W] *= 2.99792458e17 / max(wl^2, 1e-40),
  * in place. */

@@ -24,6 +24,7 @@ SYMBOLS = (
     "synthe_last_error", "synthe_abi_version", "synthe_create", "synthe_destroy",
     "synthe_copy_tables", "synthe_wavelength_grid", "synthe_continuum", "synthe_transp",
     "synthe_asynth", "synthe_lte_spectrum", "synthe_josh_flux_batch", "synthe_source_function",
+    "synthe_source_function_bfudge",
     "synthe_flux_to_lambda", "synthe_diag_div_pos", "synthe_set_kernel_timing",
     "synthe_kernel_times", "synthe_set_profile_precision", "synthe_batch_create", "synthe_batch_run",
     "synthe_batch_info", "synthe_batch_destroy", "synthe_asynth_vjp", "synthe_lte_vjp",
@@ -106,6 +107,7 @@ def load_library(path: Optional[Path] = None):
         lib.synthe_lte_spectrum.argtypes = [_VP, ct.c_long, ct.c_long] + [_VP] * 12
         lib.synthe_josh_flux_batch.argtypes = [_VP, ct.c_long, ct.c_long] + [_VP] * 10
         lib.synthe_source_function.argtypes = [_VP, ct.c_long, ct.c_long] + [_VP] * 7
+        lib.synthe_source_function_bfudge.argtypes = [_VP, ct.c_long, ct.c_long] + [_VP] * 8
         lib.synthe_flux_to_lambda.argtypes = [_VP, ct.c_long, _VP, _VP, _VP]
         lib.synthe_diag_div_pos.argtypes = [ct.c_long, ct.c_int] + [_VP] * 5
         lib.synthe_set_kernel_timing.argtypes = [_VP, ct.c_int]

@@ -61,6 +61,12 @@ class AtmosphereModel:
     continuum_scat_coeff: Optional[np.ndarray] = None
     continuum_coeff_log10: bool = True
     metadata: Optional[Dict[str, float]] = None
+    # NLTE departure coefficients of the BFUDGE column (io/atmosphere.py:56-60), [D] or [D][k]
+    bhyd: Optional[np.ndarray] = None
+    bc1: Optional[np.ndarray] = None
+    bc2: Optional[np.ndarray] = None
+    bsi1: Optional[np.ndarray] = None
+    bsi2: Optional[np.ndarray] = None
 
     @property
     def layers(self) -> int:
@@ -145,4 +151,9 @@ def load_cached(path: Path) -> AtmosphereModel:
             continuum_scat_coeff=pick("cont_scat_coeff", required=False),
             continuum_coeff_log10=bool(data["cont_coeff_log10"]) if "cont_coeff_log10" in data else False,
             metadata=meta,
+            bhyd=pick("bhyd", required=False),
+            bc1=pick("bc1", required=False),
+            bc2=pick("bc2", required=False),
+            bsi1=pick("bsi1", required=False),
+            bsi2=pick("bsi2", required=False),
         )

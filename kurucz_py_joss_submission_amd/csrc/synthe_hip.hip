@@ -229,10 +229,10 @@ __global__ void k_continuum(int D, long W, const double* __restrict__ wl, int E,
   }
 }
 
-// engine/opacity.py:2215-2228 and physics/bfudge.py:312-406 (LTE).
+// engine/opacity.py:2215-2228 and physics/bfudge.py:72-168 (LTE).
 __global__ void k_source(int D, long W, const double* __restrict__ wl,
-                         const double* __restrict__ temp, double* bnu_o, double* stim_o,
-                         double* ehvkt_o, double* slinec_o) {
+                         const double* __restrict__ temp, const double* __restrict__ bfudge, double* bnu_o,
+                         double* stim_o, double* ehvkt_o, double* slinec_o) {
   long w = blockIdx.x * (long)blockDim.x + threadIdx.x;
   int d = blockIdx.y;
   if (w >= W) return;
@@ -246,7 +246,9 @@ __global__ void k_source(int D, long W, const double* __restrict__ wl,
   if (bnu_o) bnu_o[i] = bnu;
   if (stim_o) stim_o[i] = stim;
   if (ehvkt_o) ehvkt_o[i] = ehvkt;
-  if (slinec_o) slinec_o[i] = (bnu * stim) / (1.0 - ehvkt);
+  // physics/bfudge.py:72-168: SLINEC = BNU STIM / (BFUDGE - EHVKT), BFUDGE = 1 in LTE
+  const double bf = bfudge ? bfudge[d] : 1.0;
+  if (slinec_o) slinec_o[i] = (bnu * stim) / (bf - ehvkt);
 }
 
 // TRANSP (physics/line_opacity.py:56-184) and the ASYNTH wing setup
@@ -2471,17 +2473,25 @@ int synthe_continuum(synthe_ctx* ctx, long n_depths, long n_wl, const double* wl
   return SYNTHE_OK;
 }
 
-int synthe_source_function(synthe_ctx* ctx, long n_depths, long n_wl, const double* wl,
-                           const double* temperature, double* bnu, double* stim, double* ehvkt,
-                           double* slinec, synthe_stream stream) {
+int synthe_source_function_bfudge(synthe_ctx* ctx, long n_depths, long n_wl, const double* wl,
+                                  const double* temperature, const double* bfudge, double* bnu, double* stim,
+                                  double* ehvkt, double* slinec, synthe_stream stream) {
   if (!ctx || !wl || !temperature) return fail(SYNTHE_EINVAL, "synthe_source_function: NULL argument");
   if (n_depths <= 0 || n_wl < 0 || n_depths > 65535)
     return fail(SYNTHE_EINVAL, "synthe_source_function: bad sizes");
   if (n_wl == 0) return SYNTHE_OK;
   hipLaunchKernelGGL(k_source, dim3(blocks_for(n_wl, kBlock), (unsigned)n_depths), dim3(kBlock), 0,
-                     (hipStream_t)stream, (int)n_depths, n_wl, wl, temperature, bnu, stim, ehvkt, slinec);
+                     (hipStream_t)stream, (int)n_depths, n_wl, wl, temperature, bfudge, bnu, stim, ehvkt,
+                     slinec);
   HIP_TRY(hipGetLastError());
   return SYNTHE_OK;
+}
+
+int synthe_source_function(synthe_ctx* ctx, long n_depths, long n_wl, const double* wl,
+                           const double* temperature, double* bnu, double* stim, double* ehvkt,
+                           double* slinec, synthe_stream stream) {
+  return synthe_source_function_bfudge(ctx, n_depths, n_wl, wl, temperature, nullptr, bnu, stim, ehvkt, slinec,
+                                       stream);
 }
 
 

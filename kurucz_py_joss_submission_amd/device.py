@@ -141,7 +141,8 @@ class SpectrumPlan:
 
     def __init__(self, atm: AtmosphereModel, catalog: LineCatalog, wl_start: float, wl_end: float,
                  resolution: float, cutoff: float = 1e-3, microturb_kms: float = 0.0,
-                 rhoxj: float = 0.0, grid: Optional[np.ndarray] = None, precision: str = "fp64"):
+                 rhoxj: float = 0.0, grid: Optional[np.ndarray] = None, precision: str = "fp64",
+                 bfudge: Optional[np.ndarray] = None):
         torch = _torch()
         self.ctx = N.context()
         if precision not in ("fp64", "fp32"):
@@ -183,6 +184,14 @@ class SpectrumPlan:
         self.line_abs = self.asynth
         self.line_scat = torch.zeros((D, W), dtype=f8, device=dev) if self.rhoxj <= 0.0 else \
             torch.empty((D, W), dtype=f8, device=dev)
+        # BFUDGE [D] (physics.bfudge_column; None = 1, LTE) for the line source
+        if bfudge is not None:
+            bf = np.asarray(bfudge, np.float64).reshape(-1)
+            if bf.size != D:
+                raise ValueError(f"bfudge has {bf.size} depths, the atmosphere {D}")
+            self.bfudge = to_device(bf, np.float64)
+        else:
+            self.bfudge = None
 
     # ---------------------------------------------------------------- stages
     def run_continuum(self, stream=None):
@@ -194,9 +203,9 @@ class SpectrumPlan:
             int(d.coeff_log10), N.ptr(self.cont_abs), N.ptr(self.cont_scat), s))
         self.kapmin.copy_(self.cont_abs)
         self.kapmin.add_(self.cont_scat)
-        N.check("synthe_source_function", self.ctx.lib.synthe_source_function(
-            self.ctx.handle, self.D, self.W, N.ptr(self.d_grid), N.ptr(d.temperature), None, None,
-            None, N.ptr(self.slinec), s))
+        N.check("synthe_source_function", self.ctx.lib.synthe_source_function_bfudge(
+            self.ctx.handle, self.D, self.W, N.ptr(self.d_grid), N.ptr(d.temperature), N.ptr(self.bfudge),
+            None, None, None, N.ptr(self.slinec), s))
 
     def run_transp(self, stream=None):
         s = stream if stream is not None else N.stream_handle()

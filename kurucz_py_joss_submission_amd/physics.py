@@ -11,7 +11,7 @@ the reference can switch module imports without other changes:
 * solve_josh_flux                physics/josh_solver.py:367-767
 * solve_lte_frequency            engine/radiative.py:42-218
 * solve_lte_spectrum             engine/radiative.py:282-494
-* compute_bfudge_and_slinec      physics/bfudge.py:312-406 (LTE: BFUDGE = 1)
+* compute_bfudge_and_slinec      physics/bfudge.py:72-168
 
 The Boltzmann factors and TXNXN that the reference takes from a Populations
 object (physics/populations.py:131-240) are computed on the device from the
@@ -142,15 +142,52 @@ def compute_asynth_from_transp(transp: np.ndarray, catalog: LineCatalog, atmosph
     return _host(out)
 
 
+def _departure_factor(data, layers: int) -> np.ndarray:
+    """physics/bfudge.py:26-69 (_prep_factor with one column): a missing table
+    is ones; a [D] or [D][k] table gives its first column; another row count
+    is a ValueError."""
+    if data is None:
+        return np.ones(layers, dtype=np.float64)
+    arr = np.asarray(data, dtype=np.float64)
+    if arr.ndim == 1:
+        arr = arr[:, None]
+    if arr.ndim != 2:
+        raise ValueError("Expected 2-D array for BFUDGE inputs")
+    if arr.shape[0] != layers:
+        raise ValueError(f"BFUDGE helper expected {layers} layers, received {arr.shape[0]}")
+    if arr.shape[1] == 0:
+        return np.ones(layers, dtype=np.float64)
+    return arr[:, 0]
+
+
+def bfudge_column(atmosphere: AtmosphereModel, params, layers: Optional[int] = None) -> np.ndarray:
+    """BFUDGE [D] = BHYD^PH1 (BC1/BC2)^PC1 (BSI1/BSI2)^PSI1 (physics/bfudge.py:
+    108-115, spectrv.for 173-174), each ratio and base floored at 1e-30;
+    exactly 1 in LTE (PH1 = PC1 = PSI1 = 0).  params: a spectrv.SpectrvParams
+    (None = LTE)."""
+    n = int(layers if layers is not None else atmosphere.layers)
+    ph1, pc1, psi1 = ((0.0, 0.0, 0.0) if params is None else
+                      (float(params.ph1), float(params.pc1), float(params.psi1)))
+    bhyd = _departure_factor(getattr(atmosphere, "bhyd", None), n)
+    bc1 = _departure_factor(getattr(atmosphere, "bc1", None), n)
+    bc2 = _departure_factor(getattr(atmosphere, "bc2", None), n)
+    bsi1 = _departure_factor(getattr(atmosphere, "bsi1", None), n)
+    bsi2 = _departure_factor(getattr(atmosphere, "bsi2", None), n)
+    with np.errstate(over="ignore", under="ignore", divide="ignore", invalid="ignore"):
+        return (np.power(np.maximum(bhyd, 1e-30), ph1)
+                * np.power(np.maximum(bc1 / np.maximum(bc2, 1e-30), 1e-30), pc1)
+                * np.power(np.maximum(bsi1 / np.maximum(bsi2, 1e-30), 1e-30), psi1))
+
+
 def compute_bfudge_and_slinec(atmosphere: AtmosphereModel, params, bnu: np.ndarray, stim: np.ndarray,
                               ehvkt: np.ndarray, *, clip=None):
-    """physics/bfudge.py:312-406 in LTE (PH1 = PC1 = PSI1 = 0): BFUDGE = 1,
-    SLINEC = BNU * STIM / (BFUDGE - EHVKT).  Elementwise on the host inputs."""
-    for name in ("ph1", "pc1", "psi1"):
-        if params is not None and float(getattr(params, name, 0.0)) != 0.0:
-            raise NotImplementedError("NLTE departure exponents (PH1/PC1/PSI1 != 0) are out of scope")
+    """physics/bfudge.py:72-168: (BFUDGE [D], SLINEC [D][W] = BNU STIM / (BFUDGE -
+    EHVKT)), elementwise on the caller's host arrays as the reference
+    computes it (no clamping; NaN/inf propagate; clip is accepted and
+    ignored, as there).  The device path computes SLINEC inside the source
+    function kernel (synthe_source_function_bfudge) from the same BFUDGE."""
     layers = bnu.shape[0]
-    bfudge = np.ones(layers, dtype=np.float64)
+    bfudge = bfudge_column(atmosphere, params, layers)
     with np.errstate(over="ignore", under="ignore", divide="ignore", invalid="ignore"):
         slinec = (bnu * stim) / (bfudge[:, None] - ehvkt)
     return bfudge, slinec

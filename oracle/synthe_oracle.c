@@ -316,7 +316,7 @@ void orc_continuum(long D, long W, const double *wl, long E, const double *wledg
 
 /* --------------------------------------------------- frequency quantities
  * engine/opacity.py:2215-2228 (freq, ehvkt, stim, bnu) and
- * physics/bfudge.py:312-406 in LTE (BFUDGE = 1): slinec = bnu*stim/(1-ehvkt).
+ * physics/bfudge.py:72-168 in LTE (BFUDGE = 1): slinec = bnu*stim/(1-ehvkt).
  * Writes [D][W] arrays; any output pointer may be NULL. */
 void orc_freq_quantities(long D, long W, const double *wl, const double *temp,
                          double *ehvkt_o, double *stim_o, double *bnu_o, double *slinec_o) {
