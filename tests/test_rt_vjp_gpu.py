@@ -115,7 +115,8 @@ def test_lte_vjp_matches_cpu_reverse_mode(gpu, oracle, cpu_vjp):
                                            g[0][j], g[1][j], g[2][j])):
                 want[q][d, w] = val
     assert n401 >= W // 4 - 1  # the thick columns took the label-401 branch
-    assert np.count_nonzero(want[0]) > 0.8 * D * (W - 1)  # gradients at (almost) every depth
+    # gradients at most depths (layers below the deepest XTAU point have none)
+    assert np.count_nonzero(want[0]) > 0.5 * D * (W - 1)
     for q, name in enumerate(("cont_abs", "cont_scat", "line_abs", "planck", "line_source")):
         scale = np.max(np.abs(want[q]), axis=0)
         scale = np.where(scale > 0, scale, 1.0)
