@@ -243,3 +243,24 @@ def test_fp32_profiles_flux_error(gpu, oracle, c1_workload):
     err = _rel(flux, ref.flux, floor=floor)
     print("fp32 profiles: max relative flux error vs FP64 oracle", err)
     assert err < 1e-4
+
+
+@pytest.mark.parametrize("teff,logg,nl,mh,wl0", [(3800.0, 4.8, 72, 0.3, 420.0), (9500.0, 4.0, 64, -0.5, 390.0),
+                                                 (20000.0, 2.0, 80, 0.0, 450.0), (35000.0, 4.0, 72, -1.0, 600.0)])
+def test_flux_parity_regimes(gpu, oracle, teff, logg, nl, mh, wl0):
+    """Cool dwarf, A star, B giant (electron scattering, label-401 cores),
+    hot O/B star: flux and continuum against the oracle at the plain-path bar
+    (<= 1e-6), and the JOSH iteration counts."""
+    from kurucz_py_joss_submission_amd import synthetic as S
+
+    wk = S.make_workload(f"regime{int(teff)}", teff, logg, 3000, wl0, wl0 + 3.0, seed=11, n_layers=nl, mh=mh)
+    plan = _plan(wk)
+    plan.run()
+    ref = oracle.synthesize(wk.atmosphere, wk.catalog, wk.wl_start, wk.wl_end, wk.resolution, keep_stages=True)
+    flux = plan.flux.cpu().numpy()
+    cont = plan.cont.cpu().numpy()
+    assert _rel(cont, ref.continuum) < FLUX_RTOL
+    mask = np.abs(ref.flux) > 1e-3 * np.median(ref.continuum)
+    assert _rel(flux[mask], ref.flux[mask]) < FLUX_RTOL
+    assert _rel(flux, ref.flux, floor=1e-3 * np.median(ref.continuum)) < FLUX_RTOL
+    assert np.mean(plan.iters.cpu().numpy() == ref.stages["iters"]) > 0.995
