@@ -176,14 +176,15 @@ __device__ __forceinline__ int reach_class(int m) {
 // physics/continuum.py:73-303, coefficient path, one thread per wavelength.
 constexpr int kContinuumDepths = 8;  // depths per thread (grid rows cover the rest)
 
-__global__ void k_continuum(int D, long W, const double* __restrict__ wl, int E,
-                            const double* __restrict__ wledge, const double* __restrict__ half_in,
-                            const double* __restrict__ delta_in, const double* __restrict__ cabs,
-                            const double* __restrict__ cscat, int log10flag,
-                            double* __restrict__ out_abs, double* __restrict__ out_scat) {
-  long w = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (w >= W) return;
-  double x = wl[w];
+// The edge interval of wavelength x and its three quadratic weights.
+struct ContWeights {
+  int e;
+  double c1, c2, c3;
+};
+
+__device__ __forceinline__ ContWeights continuum_weights(double x, int E, const double* __restrict__ wledge,
+                                                         const double* __restrict__ half_in,
+                                                         const double* __restrict__ delta_in) {
   double ax = fabs(x);
   int lo = 0, hi = E;
   while (lo < hi) {
@@ -202,31 +203,70 @@ __global__ void k_continuum(int D, long W, const double* __restrict__ wl, int E,
     dl = 0.5 * dd * dd;
   }
   if (dl == 0.0) dl = 1e-20;
-  double c1 = (x - h) * (x - right) / dl;
-  double c2 = (left - x) * (x - right) * 2.0 / dl;
-  double c3 = (x - left) * (x - h) / dl;
+  ContWeights c;
+  c.e = e;
+  c.c1 = (x - h) * (x - right) / dl;
+  c.c2 = (left - x) * (x - right) * 2.0 / dl;
+  c.c3 = (x - left) * (x - h) / dl;
+  return c;
+}
+
+// Continuum absorption and scattering at depth d.
+__device__ __forceinline__ void continuum_value(const ContWeights& c, int d, int E, const double* __restrict__ cabs,
+                                                const double* __restrict__ cscat, int log10flag, double* va_o,
+                                                double* vs_o) {
+  const double* ca = cabs + ((long)d * (E - 1) + c.e) * 3;
+  const double* cs = cscat + ((long)d * (E - 1) + c.e) * 3;
+  double la = ca[0] * c.c1 + ca[1] * c.c2 + ca[2] * c.c3;
+  double ls = cs[0] * c.c1 + cs[1] * c.c2 + cs[2] * c.c3;
+  double va = la, vs = ls;
+  if (log10flag) {
+    if (la > 38.0) la = 38.0;
+    if (ls > 38.0) ls = 38.0;
+    va = pow(10.0, la);
+    vs = pow(10.0, ls);
+    if (va < 0.0) va = 0.0;
+    if (va > 3.4e38) va = 3.4e38;
+    if (vs < 0.0) vs = 0.0;
+    if (vs > 3.4e38) vs = 3.4e38;
+    if (!isfinite(va)) va = 3.4e38;
+    if (!isfinite(vs)) vs = 3.4e38;
+  }
+  *va_o = va;
+  *vs_o = vs;
+}
+
+__global__ void k_continuum(int D, long W, const double* __restrict__ wl, int E,
+                            const double* __restrict__ wledge, const double* __restrict__ half_in,
+                            const double* __restrict__ delta_in, const double* __restrict__ cabs,
+                            const double* __restrict__ cscat, int log10flag,
+                            double* __restrict__ out_abs, double* __restrict__ out_scat) {
+  long w = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (w >= W) return;
+  const ContWeights c = continuum_weights(wl[w], E, wledge, half_in, delta_in);
   const int d1 = min(D, (int)(blockIdx.y + 1) * kContinuumDepths);
   for (int d = blockIdx.y * kContinuumDepths; d < d1; ++d) {
-    const double* ca = cabs + ((long)d * (E - 1) + e) * 3;
-    const double* cs = cscat + ((long)d * (E - 1) + e) * 3;
-    double la = ca[0] * c1 + ca[1] * c2 + ca[2] * c3;
-    double ls = cs[0] * c1 + cs[1] * c2 + cs[2] * c3;
-    double va = la, vs = ls;
-    if (log10flag) {
-      if (la > 38.0) la = 38.0;
-      if (ls > 38.0) ls = 38.0;
-      va = pow(10.0, la);
-      vs = pow(10.0, ls);
-      if (va < 0.0) va = 0.0;
-      if (va > 3.4e38) va = 3.4e38;
-      if (vs < 0.0) vs = 0.0;
-      if (vs > 3.4e38) vs = 3.4e38;
-      if (!isfinite(va)) va = 3.4e38;
-      if (!isfinite(vs)) vs = 3.4e38;
-    }
+    double va, vs;
+    continuum_value(c, d, E, cabs, cscat, log10flag, &va, &vs);
     out_abs[(long)d * W + w] = va;
     out_scat[(long)d * W + w] = vs;
   }
+}
+
+// engine/opacity.py:2215-2228: EHVKT, STIM and BNU at one (depth, wavelength).
+struct Planck {
+  double bnu, stim, ehvkt;
+};
+
+__device__ __forceinline__ Planck planck_terms(double wl_nm, double temp) {
+  double hkt = kHPlanck / (kKBoltz * temp);
+  double freq = kCLightCm / (wl_nm * kNmToCm);
+  Planck p;
+  p.ehvkt = exp(-freq * hkt);
+  p.stim = 1.0 - p.ehvkt;
+  double freq15 = freq / 1.0e15;
+  p.bnu = 1.47439e-02 * pow(freq15, 3.0) * p.ehvkt / p.stim;
+  return p;
 }
 
 // engine/opacity.py:2215-2228 and physics/bfudge.py:72-168 (LTE).
@@ -236,19 +276,55 @@ __global__ void k_source(int D, long W, const double* __restrict__ wl,
   long w = blockIdx.x * (long)blockDim.x + threadIdx.x;
   int d = blockIdx.y;
   if (w >= W) return;
-  double hkt = kHPlanck / (kKBoltz * temp[d]);
-  double freq = kCLightCm / (wl[w] * kNmToCm);
-  double ehvkt = exp(-freq * hkt);
-  double stim = 1.0 - ehvkt;
-  double freq15 = freq / 1.0e15;
-  double bnu = 1.47439e-02 * pow(freq15, 3.0) * ehvkt / stim;
+  const Planck p = planck_terms(wl[w], temp[d]);
   long i = (long)d * W + w;
-  if (bnu_o) bnu_o[i] = bnu;
-  if (stim_o) stim_o[i] = stim;
-  if (ehvkt_o) ehvkt_o[i] = ehvkt;
+  if (bnu_o) bnu_o[i] = p.bnu;
+  if (stim_o) stim_o[i] = p.stim;
+  if (ehvkt_o) ehvkt_o[i] = p.ehvkt;
   // physics/bfudge.py:72-168: SLINEC = BNU STIM / (BFUDGE - EHVKT), BFUDGE = 1 in LTE
   const double bf = bfudge ? bfudge[d] : 1.0;
-  if (slinec_o) slinec_o[i] = (bnu * stim) / (bf - ehvkt);
+  if (slinec_o) slinec_o[i] = (p.bnu * p.stim) / (bf - p.ehvkt);
+}
+
+// One spectrum of a batch for k_cont_src_batch.
+struct ContSrcArgs {
+  long W;
+  long blk0;  // first x-block of this spectrum in the flat grid
+  int D, E, log10flag;
+  const double *wl, *wledge, *half, *delta, *cabs, *cscat, *temp;
+  double *out_abs, *out_scat, *kapmin, *slinec;
+};
+
+// Continuum, KAPMIN = abs + scat and the LTE line source of every spectrum of
+// a batch in one launch: x-blocks are laid end to end over the spectra
+// (a[i].blk0 ascending, a[0].blk0 = 0), grid rows cover kContinuumDepths
+// depths each.  Each value is the expression of k_continuum, the KAPMIN sum
+// and k_source, so batched spectra equal single ones bit for bit.
+__global__ void k_cont_src_batch(const ContSrcArgs* __restrict__ a, int n) {
+  int lo = 0, hi = n;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (a[mid].blk0 <= (long)blockIdx.x) lo = mid;
+    else hi = mid;
+  }
+  const ContSrcArgs& A = a[lo];
+  const long W = A.W;
+  const long w = ((long)blockIdx.x - A.blk0) * blockDim.x + threadIdx.x;
+  const int d0 = blockIdx.y * kContinuumDepths;
+  if (w >= W || d0 >= A.D) return;
+  const double x = A.wl[w];
+  const ContWeights c = continuum_weights(x, A.E, A.wledge, A.half, A.delta);
+  const int d1 = min(A.D, d0 + kContinuumDepths);
+  for (int d = d0; d < d1; ++d) {
+    double va, vs;
+    continuum_value(c, d, A.E, A.cabs, A.cscat, A.log10flag, &va, &vs);
+    const long i = (long)d * W + w;
+    A.out_abs[i] = va;
+    A.out_scat[i] = vs;
+    A.kapmin[i] = va + vs;
+    const Planck p = planck_terms(x, A.temp[d]);
+    A.slinec[i] = (p.bnu * p.stim) / (1.0 - p.ehvkt);
+  }
 }
 
 // TRANSP (physics/line_opacity.py:56-184) and the ASYNTH wing setup
@@ -538,12 +614,6 @@ __global__ __launch_bounds__(kBlock) void k_near_reach(int D, const int64_t* __r
     m = o > m ? o : m;
   }
   if ((threadIdx.x & 63) == 0 && m > 0) atomicMax(near_reach + d, m);
-}
-
-__global__ void k_add2(long n, const double* __restrict__ a, const double* __restrict__ b,
-                       double* __restrict__ out) {
-  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (i < n) out[i] = a[i] + b[i];
 }
 
 __global__ void k_fill_u64(long n, unsigned long long* p) {
@@ -2648,6 +2718,9 @@ struct synthe_batch {
   long rt_g = 0;            // total wavelengths
   RtArgs* d_rt = nullptr;   // [n], ascending g0 (in b->order)
   GatherArgs* d_args = nullptr;
+  ContSrcArgs* d_cs = nullptr;  // [n] continuum/source stage, in instance order
+  long cs_blocks = 0;           // x-blocks of that stage over all spectra
+  unsigned cs_rows = 0;         // its depth rows
   int32_t* d_order = nullptr;
   int64_t* d_items = nullptr;
   unsigned long long* d_counter = nullptr;
@@ -2708,27 +2781,27 @@ int launch_gather_batch(synthe_batch* b, int kwp, int blocks, hipStream_t st) {
   return SYNTHE_OK;
 }
 
-// Per-instance stages before the gather: continuum, KAPMIN, source function,
-// TRANSP, wing set-up, class lists, Voigt zones.
+// Continuum, KAPMIN and the line source of every instance: one launch.
+int batch_cont_src(synthe_batch* b, hipStream_t st) {
+  if (b->cs_blocks == 0) return SYNTHE_OK;
+  hipLaunchKernelGGL(k_cont_src_batch, dim3((unsigned)b->cs_blocks, b->cs_rows), dim3(kBlock), 0, st, b->d_cs,
+                     (int)b->inst.size());
+  HIP_TRY(hipGetLastError());
+  return SYNTHE_OK;
+}
+
+// Per-instance stages before the gather that follow batch_cont_src: TRANSP,
+// wing set-up, class lists, Voigt zones.
 int batch_prepare_one(synthe_batch* b, BatchInst& bi, hipStream_t st) {
   synthe_ctx* ctx = b->ctx;
   const synthe_instance& in = bi.in;
-  const long D = in.depths.n_depths, W = in.n_wl, L = in.lines.n_lines;
+  const long W = in.n_wl, L = in.lines.n_lines;
   char* base = b->arena;
-  double* cabs = (double*)(base + bi.o_cabs);
-  double* cscat = (double*)(base + bi.o_cscat);
   double* kapmin = (double*)(base + bi.o_kapmin);
-  double* slinec = (double*)(base + bi.o_slinec);
   double* asy = (double*)(base + bi.o_asynth);
   double* transp = (double*)(base + bi.o_transp);
   uint8_t* valid = (uint8_t*)(base + bi.o_valid);
-  int rc = synthe_continuum(ctx, D, W, in.wl, in.n_edges, in.wledge, in.half_edge, in.delta_edge, in.coeff_abs,
-                            in.coeff_scat, in.coeff_log10, cabs, cscat, st);
-  if (rc) return rc;
-  hipLaunchKernelGGL(k_add2, dim3(blocks_for(D * W, kBlock)), dim3(kBlock), 0, st, D * W, cabs, cscat, kapmin);
-  HIP_TRY(hipGetLastError());
-  rc = synthe_source_function(ctx, D, W, in.wl, in.depths.temperature, nullptr, nullptr, nullptr, slinec, st);
-  if (rc) return rc;
+  int rc;
   if (L > 0) {
     rc = synthe_transp(ctx, &in.lines, &in.depths, W, kapmin, in.cutoff, 0.0, transp, valid, st);
     if (rc) return rc;
@@ -2759,6 +2832,7 @@ int batch_enqueue(synthe_batch* b, hipStream_t s0, bool prepare, bool gather, bo
   const int n = (int)b->inst.size();
   int rc;
   if (prepare) {
+    if ((rc = batch_cont_src(b, s0))) return rc;
     if ((rc = fork_streams(b, b->ev_fork, s0))) return rc;
     for (int i = 0; i < n; ++i)
       if ((rc = batch_prepare_one(b, b->inst[b->order[i]], b->streams[i % kBatchStreams]))) return rc;
@@ -2829,6 +2903,7 @@ int synthe_batch_create(synthe_ctx* ctx, int n, const synthe_instance* inst, int
   b->rlay = rt_layout(b->rt_g, n);
   const size_t o_rt = off; off = align_up(off + b->rlay.total);
   const size_t o_args = off; off = align_up(off + n * sizeof(GatherArgs));
+  const size_t o_cs = off; off = align_up(off + n * sizeof(ContSrcArgs));
   const size_t o_order = off; off = align_up(off + n * sizeof(int32_t));
   const size_t o_items = off; off = align_up(off + (n + 1) * sizeof(int64_t));
   const size_t o_counter = off; off = align_up(off + sizeof(unsigned long long));
@@ -2842,6 +2917,7 @@ int synthe_batch_create(synthe_ctx* ctx, int n, const synthe_instance* inst, int
   b->rt_base = b->arena + o_rt;
   b->d_rt = (RtArgs*)(b->rt_base + b->rlay.o_args);
   b->d_args = (GatherArgs*)(b->arena + o_args);
+  b->d_cs = (ContSrcArgs*)(b->arena + o_cs);
   b->d_order = (int32_t*)(b->arena + o_order);
   b->d_items = (int64_t*)(b->arena + o_items);
   b->d_counter = (unsigned long long*)(b->arena + o_counter);
@@ -2863,6 +2939,19 @@ int synthe_batch_create(synthe_ctx* ctx, int n, const synthe_instance* inst, int
                           (uint8_t*)(b->arena + bi.o_valid), in.center_order, in.center_start,
                           (double*)(b->arena + bi.o_asynth), asynth_bufs(b->arena + bi.o_asy, bi.alay));
   }
+  std::vector<ContSrcArgs> cs(n);
+  for (int i = 0; i < n; ++i) {
+    const BatchInst& bi = b->inst[i];
+    const synthe_instance& in = bi.in;
+    const int D = (int)in.depths.n_depths;
+    cs[i] = ContSrcArgs{in.n_wl, b->cs_blocks, D, (int)in.n_edges, in.coeff_log10, in.wl, in.wledge, in.half_edge,
+                        in.delta_edge, in.coeff_abs, in.coeff_scat, in.depths.temperature,
+                        (double*)(b->arena + bi.o_cabs), (double*)(b->arena + bi.o_cscat),
+                        (double*)(b->arena + bi.o_kapmin), (double*)(b->arena + bi.o_slinec)};
+    b->cs_blocks += blocks_for(in.n_wl, kBlock);
+    b->cs_rows = std::max(b->cs_rows, (unsigned)((D + kContinuumDepths - 1) / kContinuumDepths));
+  }
+  if (b->cs_blocks > 0x7fffffffL) return bail(fail(SYNTHE_ERANGE, "synthe_batch_create: too many wavelengths"));
   std::vector<RtArgs> rts(n);
   long g0 = 0;
   for (int i = 0; i < n; ++i) {
@@ -2876,6 +2965,7 @@ int synthe_batch_create(synthe_ctx* ctx, int n, const synthe_instance* inst, int
   }
   if (hipMemcpy(b->d_rt, rts.data(), n * sizeof(RtArgs), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(b->d_args, args.data(), n * sizeof(GatherArgs), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->d_cs, cs.data(), n * sizeof(ContSrcArgs), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(b->d_order, b->order.data(), n * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemset(b->zeros, 0, zeros_n * sizeof(double)) != hipSuccess)
     return bail(fail(SYNTHE_EHIP, "synthe_batch_create: upload failed"));
