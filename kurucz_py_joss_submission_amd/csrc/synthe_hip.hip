@@ -668,8 +668,12 @@ __device__ __forceinline__ long class_window_search(int lane, int d, long lo_px,
 // then adds its slots in order.  The sum order is the entry order, the same
 // as adding entry by entry.  Writes the pre-STIM sum to out [D][W] (0 where no
 // wing reaches), which k_gather continues from.
-constexpr int kNearSlots = 6;                  // queued values per pixel
+#ifndef SYNTHE_NEAR_SLOTS
+#define SYNTHE_NEAR_SLOTS 6
+#endif
+constexpr int kNearSlots = SYNTHE_NEAR_SLOTS;  // queued values per pixel
 constexpr int kNearQueue = kNearSlots * 64;    // pairs per wave between flushes
+static_assert(kNearSlots >= 1 && kNearSlots <= 16, "a queue record holds the slot in 4 bits");
 
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
