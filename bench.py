@@ -282,7 +282,7 @@ def bench_inverse(args) -> dict:
     prob.evaluations = 0
     sol = prob.solve(max_iter=20)
     rep = INV.report(targets, sol["delta"], sol["sensitivity"])
-    judge = INV.judge(targets, sol["delta"], indices=range(8))
+    check = INV.fp64_plain_check(targets, sol["delta"], indices=range(8))
     # adjoint cross-check at the solution, on the spectrum whose lines
     # constrain the line elements most: d(sum w F)/d(offsets)
     li = [S.ABUNDANCE_Z.index(z) for z in INV.LINE_ELEMENTS]
@@ -314,13 +314,13 @@ def bench_inverse(args) -> dict:
                        "diff_step_spread": abs(g_fd3[z] - v) / max(abs(v), 1e-300)}
               for z, v in g_fd.items() if sol["sensitivity"][i0, S.ABUNDANCE_Z.index(z)] >= 2e-3}
     log(f"[bench] c5: {sol['iterations']} iterations, {sol['evaluations']} batched evaluations, "
-        f"{sol['seconds']:.2f} s; {rep}; judge {judge:.2e}; adjoint vs batched gradient {worst:.2e} "
+        f"{sol['seconds']:.2f} s; {rep}; FP64 plain path {check:.2e}; adjoint vs batched gradient {worst:.2e} "
         f"(constrained elements {per_el})")
     prob.plan.close()
     return {"seconds_to_converge": sol["seconds"], "iterations": sol["iterations"],
             "batched_evaluations": sol["evaluations"], "ms_per_batched_evaluation": batch_ms,
             "cost_history": sol["cost_history"], **rep,
-            "max_rel_flux_err_vs_fp64_plain_path": judge,
+            "max_rel_flux_err_vs_fp64_plain_path": check,
             "max_err_adjoint_vs_batched_gradient_rel_largest": worst,
             "rel_err_adjoint_vs_batched_gradient_constrained": per_el, "gradient_check_spectrum": i0,
             "setup_s": {"targets": t_targets, "plan": t_plan},

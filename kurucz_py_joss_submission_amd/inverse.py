@@ -420,8 +420,8 @@ def report(targets: Sequence[Target], delta: np.ndarray, sensitivity: np.ndarray
     return out
 
 
-def judge(targets: Sequence[Target], delta: np.ndarray, indices: Optional[Sequence[int]] = None,
-          resolution: float = 300000.0, cutoff: float = 1e-3) -> float:
+def fp64_plain_check(targets: Sequence[Target], delta: np.ndarray, indices: Optional[Sequence[int]] = None,
+                     resolution: float = 300000.0, cutoff: float = 1e-3) -> float:
     """Max relative difference between each target and the FP64 plain path
     (SpectrumPlan from a host-built bundle) at the recovered offsets, over the
     pixels with |F| > 1e-3 median(F) (the robust mask of the reference's

@@ -39,10 +39,10 @@ def test_gauss_newton_recovers_line_elements(small_problem):
 
 @pytest.mark.gpu
 def test_recovered_spectra_under_fp64_plain_path(small_problem):
-    from kurucz_py_joss_submission_amd.inverse import judge
+    from kurucz_py_joss_submission_amd.inverse import fp64_plain_check
 
     targets, prob, sol = small_problem
-    assert judge(targets, sol["delta"], indices=[0, 1]) < 1e-4
+    assert fp64_plain_check(targets, sol["delta"], indices=[0, 1]) < 1e-4
 
 
 @pytest.mark.gpu
