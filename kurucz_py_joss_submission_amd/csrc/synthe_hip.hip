@@ -561,9 +561,23 @@ __global__ __launch_bounds__(kBlock) void k_class_scatter(
   int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   unsigned long long ltmask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   // depths are independent: each workgroup row takes kListDepths of them
-  const int d1 = min(D, (int)(blockIdx.y + 1) * kListDepths);
-  for (int d = blockIdx.y * kListDepths; d < d1; ++d) {
-    int k = l >= 0 ? cls[l * D + d] : 0xFF;
+  const int d0 = blockIdx.y * kListDepths;
+  const int d1 = min(D, d0 + kListDepths);
+  // The lane's classes and wing parameters for the row's depths, loaded
+  // before the scatter so that each line's consecutive depths are fetched
+  // together rather than once per depth iteration, after L2 has moved on.
+  uint8_t kc[kListDepths];
+  WingParams pr[kListDepths];
+#pragma unroll
+  for (int t = 0; t < kListDepths; ++t) kc[t] = (l >= 0 && d0 + t < d1) ? cls[l * D + d0 + t] : (uint8_t)0xFF;
+#pragma unroll
+  for (int t = 0; t < kListDepths; ++t)
+    if (kc[t] != 0xFF) pr[t] = wp[l * D + d0 + t];
+#pragma unroll
+  for (int t = 0; t < kListDepths; ++t) {
+    const int d = d0 + t;
+    if (d >= d1) break;  // uniform over the workgroup
+    int k = kc[t];
     int rank = 0;
     for (int kk = 0; kk < kClasses; ++kk) {
       unsigned long long m = __ballot(k == kk);
@@ -576,7 +590,7 @@ __global__ __launch_bounds__(kBlock) void k_class_scatter(
       for (int q = 0; q < wave; ++q) before += sh[q][k];
       int64_t pos = regions[d * (kClasses + 1) + k] + offsets[((long)blockIdx.x * D + d) * kClasses + k] +
                     before + rank;
-      WingParams p = wp[l * D + d];
+      const WingParams p = pr[t];
       const int r = p.use_far ? p.n10dop : p.maxstep;
       const bool chunk = !(p.x_far > 0.0 && p.x_far < kChunkMinX);  // X >= 0
       WingHead h;
