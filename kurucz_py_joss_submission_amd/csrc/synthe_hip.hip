@@ -563,21 +563,8 @@ __global__ __launch_bounds__(kBlock) void k_class_scatter(
   // depths are independent: each workgroup row takes kListDepths of them
   const int d0 = blockIdx.y * kListDepths;
   const int d1 = min(D, d0 + kListDepths);
-  // The lane's classes and wing parameters for the row's depths, loaded
-  // before the scatter so that each line's consecutive depths are fetched
-  // together rather than once per depth iteration, after L2 has moved on.
-  uint8_t kc[kListDepths];
-  WingParams pr[kListDepths];
-#pragma unroll
-  for (int t = 0; t < kListDepths; ++t) kc[t] = (l >= 0 && d0 + t < d1) ? cls[l * D + d0 + t] : (uint8_t)0xFF;
-#pragma unroll
-  for (int t = 0; t < kListDepths; ++t)
-    if (kc[t] != 0xFF) pr[t] = wp[l * D + d0 + t];
-#pragma unroll
-  for (int t = 0; t < kListDepths; ++t) {
-    const int d = d0 + t;
-    if (d >= d1) break;  // uniform over the workgroup
-    int k = kc[t];
+  for (int d = d0; d < d1; ++d) {
+    int k = l >= 0 ? cls[l * D + d] : 0xFF;
     int rank = 0;
     for (int kk = 0; kk < kClasses; ++kk) {
       unsigned long long m = __ballot(k == kk);
@@ -590,7 +577,7 @@ __global__ __launch_bounds__(kBlock) void k_class_scatter(
       for (int q = 0; q < wave; ++q) before += sh[q][k];
       int64_t pos = regions[d * (kClasses + 1) + k] + offsets[((long)blockIdx.x * D + d) * kClasses + k] +
                     before + rank;
-      const WingParams p = pr[t];
+      const WingParams p = wp[l * D + d];
       const int r = p.use_far ? p.n10dop : p.maxstep;
       const bool chunk = !(p.x_far > 0.0 && p.x_far < kChunkMinX);  // X >= 0
       WingHead h;
@@ -2309,7 +2296,8 @@ static int check_asynth_args(const synthe_lines* lines, const synthe_depths* dep
 static int asynth_prepare(const synthe_ctx* ctx, const synthe_lines* lines, const synthe_depths* depths,
                           long n_wl, const double* wl, const double* transp, const uint8_t* valid,
                           const double* kapmin, double cutoff, const int64_t* wing_order, double* asynth,
-                          bool stats, const AsynthBufs& b, hipStream_t st, bool near_values = true) {
+                          bool stats, const AsynthBufs& b, hipStream_t st, bool near_values = true,
+                          double* transp_out = nullptr, uint8_t* valid_out = nullptr) {
   const long L = lines->n_lines;
   const int D = (int)depths->n_depths;
   const long pairs = L * (long)D;
@@ -2317,9 +2305,16 @@ static int asynth_prepare(const synthe_ctx* ctx, const synthe_lines* lines, cons
   hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, 5L, b.ev);
   HIP_TRY(hipGetLastError());
   if (pairs > 0) {
-    hipLaunchKernelGGL(k_transp_wing, dim3(blocks_for(pairs, kBlock)), dim3(kBlock), 0, st, to_dev(lines),
-                       to_dev(depths), n_wl, kapmin, cutoff, 0.0, wl, ctx->tb, 2, nullptr, nullptr, transp, valid,
-                       b.wp, b.cls);
+    // transp_out: TRANSP and valid are computed here in the same pass (mode 3,
+    // no microturbulence) instead of being read from an earlier mode-1 pass
+    if (transp_out)
+      hipLaunchKernelGGL(k_transp_wing, dim3(blocks_for(pairs, kBlock)), dim3(kBlock), 0, st, to_dev(lines),
+                         to_dev(depths), n_wl, kapmin, cutoff, 0.0, wl, ctx->tb, 3, transp_out, valid_out, nullptr,
+                         nullptr, b.wp, b.cls);
+    else
+      hipLaunchKernelGGL(k_transp_wing, dim3(blocks_for(pairs, kBlock)), dim3(kBlock), 0, st, to_dev(lines),
+                         to_dev(depths), n_wl, kapmin, cutoff, 0.0, wl, ctx->tb, 2, nullptr, nullptr, transp, valid,
+                         b.wp, b.cls);
     HIP_TRY(hipGetLastError());
     const unsigned rows = (unsigned)((D + kListDepths - 1) / kListDepths);
     hipLaunchKernelGGL(k_class_count, dim3((unsigned)nblk, rows), dim3(kBlock), 0, st, L, D, wing_order,
@@ -2808,25 +2803,20 @@ int batch_cont_src(synthe_batch* b, hipStream_t st) {
   return SYNTHE_OK;
 }
 
-// Per-instance stages before the gather that follow batch_cont_src: TRANSP,
-// wing set-up, class lists, Voigt zones.
+// Per-instance stages before the gather that follow batch_cont_src: TRANSP
+// with the wing set-up (one pass), class lists, Voigt zones.
 int batch_prepare_one(synthe_batch* b, BatchInst& bi, hipStream_t st) {
   synthe_ctx* ctx = b->ctx;
   const synthe_instance& in = bi.in;
-  const long W = in.n_wl, L = in.lines.n_lines;
+  const long W = in.n_wl;
   char* base = b->arena;
   double* kapmin = (double*)(base + bi.o_kapmin);
   double* asy = (double*)(base + bi.o_asynth);
   double* transp = (double*)(base + bi.o_transp);
   uint8_t* valid = (uint8_t*)(base + bi.o_valid);
-  int rc;
-  if (L > 0) {
-    rc = synthe_transp(ctx, &in.lines, &in.depths, W, kapmin, in.cutoff, 0.0, transp, valid, st);
-    if (rc) return rc;
-  }
   const AsynthBufs bufs = asynth_bufs(base + bi.o_asy, bi.alay);
   return asynth_prepare(ctx, &in.lines, &in.depths, W, in.wl, transp, valid, kapmin, in.cutoff, in.wing_order, asy,
-                        false, bufs, st);
+                        false, bufs, st, true, transp, valid);
 }
 
 int fork_streams(synthe_batch* b, hipEvent_t ev, hipStream_t s0) {
