@@ -485,6 +485,16 @@ __global__ void k_transp_wing(LinesDev ln, DepthsDev dp, long W, const double* _
 constexpr int kListDepths = 8;  // depths per workgroup row of the class count and scatter
 
 // Per (block of 256 centre-sorted lines, depth): pair counts per reach class.
+// A lane's reach classes for the depths [d0, d1) of its line l (0xFF past the
+// row, or for l < 0), read before the depth loop of the class count and
+// scatter: read once per depth, a line's bytes were fetched again after L2
+// had evicted them.
+__device__ __forceinline__ void load_row_classes(long l, int D, int d0, int d1, const uint8_t* __restrict__ cls,
+                                                 uint8_t (&kc)[kListDepths]) {
+#pragma unroll
+  for (int t = 0; t < kListDepths; ++t) kc[t] = (l >= 0 && d0 + t < d1) ? cls[l * D + d0 + t] : (uint8_t)0xFF;
+}
+
 __global__ __launch_bounds__(kBlock) void k_class_count(long L, int D, const int64_t* __restrict__ order,
                                                         const uint8_t* __restrict__ cls,
                                                         int32_t* __restrict__ counts) {
@@ -493,9 +503,15 @@ __global__ __launch_bounds__(kBlock) void k_class_count(long L, int D, const int
   long l = s < L ? order[s] : -1;
   int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   // depths are independent: each workgroup row takes kListDepths of them
-  const int d1 = min(D, (int)(blockIdx.y + 1) * kListDepths);
-  for (int d = blockIdx.y * kListDepths; d < d1; ++d) {
-    int k = l >= 0 ? cls[l * D + d] : 0xFF;
+  const int d0 = blockIdx.y * kListDepths;
+  const int d1 = min(D, d0 + kListDepths);
+  uint8_t kc[kListDepths];
+  load_row_classes(l, D, d0, d1, cls, kc);
+#pragma unroll
+  for (int t = 0; t < kListDepths; ++t) {
+    const int d = d0 + t;
+    if (d >= d1) break;  // uniform over the workgroup
+    int k = kc[t];
     for (int kk = 0; kk < kClasses; ++kk) {
       unsigned long long m = __ballot(k == kk);
       if (lane == 0) sh[wave][kk] = __popcll(m);
@@ -563,8 +579,13 @@ __global__ __launch_bounds__(kBlock) void k_class_scatter(
   // depths are independent: each workgroup row takes kListDepths of them
   const int d0 = blockIdx.y * kListDepths;
   const int d1 = min(D, d0 + kListDepths);
-  for (int d = d0; d < d1; ++d) {
-    int k = l >= 0 ? cls[l * D + d] : 0xFF;
+  uint8_t kc[kListDepths];
+  load_row_classes(l, D, d0, d1, cls, kc);
+#pragma unroll
+  for (int t = 0; t < kListDepths; ++t) {
+    const int d = d0 + t;
+    if (d >= d1) break;  // uniform over the workgroup
+    int k = kc[t];
     int rank = 0;
     for (int kk = 0; kk < kClasses; ++kk) {
       unsigned long long m = __ballot(k == kk);
