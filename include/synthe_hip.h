@@ -38,6 +38,7 @@ extern "C" {
 #define SYNTHE_ERANGE (-4) /* input exceeds a compiled limit (e.g. depth count) */
 
 #define SYNTHE_MAX_DEPTH 128 /* largest depth count the RT kernels accept */
+#define SYNTHE_NOALLOC_MAX_WL 65536 /* largest n_wl of synthe_lte_spectrum_noalloc */
 
 typedef struct synthe_ctx synthe_ctx;
 typedef void *synthe_stream; /* hipStream_t */
@@ -160,6 +161,22 @@ int synthe_lte_spectrum(synthe_ctx *ctx, long n_depths, long n_wl, const double 
                         const double *cont_scat, const double *line_abs, const double *line_scat,
                         const double *line_source, double *flux_total, double *flux_cont,
                         int32_t *iters, synthe_stream stream);
+
+/* No counterpart in the reference: synthe_lte_spectrum for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, kernels and
+ * results as synthe_lte_spectrum, but the RT workspace is a static device
+ * array of the library (77 MiB per GPU, held until the process ends) instead
+ * of the context's growable workspace: the call makes no hipMalloc, hipFree
+ * or other allocation.  n_wl above SYNTHE_NOALLOC_MAX_WL returns
+ * SYNTHE_ERANGE.  Every call on one GPU shares that array: calls that overlap
+ * on one GPU (other streams, other contexts, other host threads) race. */
+int synthe_lte_spectrum_noalloc(synthe_ctx *ctx, long n_depths, long n_wl, const double *wl,
+                                const double *temperature, const double *rhox,
+                                const double *cont_abs, const double *cont_scat,
+                                const double *line_abs, const double *line_scat,
+                                const double *line_source, double *flux_total, double *flux_cont,
+                                int32_t *iters, synthe_stream stream);
 
 /* physics/josh_solver.py:367-767 (solve_josh_flux) over n_cols independent
  * columns: each input is [n][n_cols] (column c at j*n_cols + c); rhox [n]

@@ -20,6 +20,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1573,6 +1574,7 @@ struct synthe_ctx {
   std::vector<double> h0, h1, h2, extab, extabf;
   void* ws = nullptr;
   size_t ws_size = 0;
+  void* scratch = nullptr;  // device address of the static RT scratch on this device
   // kernel timing: event pairs around each wing-gather launch while enabled
   static constexpr int kTimedMax = 256;
   int precision = SYNTHE_PREC_FP64;  // profile precision of synthe_asynth
@@ -1610,7 +1612,7 @@ static int ensure_ws(synthe_ctx* ctx, size_t bytes) {
   return SYNTHE_OK;
 }
 
-static size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
+static constexpr size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
 
 
 static int check_lines_depths(const synthe_lines* ln, const synthe_depths* dp, const char* who) {
@@ -2378,7 +2380,7 @@ struct RtLayout {
 };
 
 // Workspace of the RT over G wavelengths of n_rt spectra.
-static RtLayout rt_layout(long G, int n_rt = 1) {
+static constexpr RtLayout rt_layout(long G, int n_rt = 1) {
   RtLayout y{};
   y.o_flags = align_up((size_t)2 * 3 * kNxtau * G * sizeof(float));
   y.o_list = y.o_flags + align_up((size_t)G);
@@ -2420,6 +2422,35 @@ static int rt_launch(const synthe_ctx* ctx, long n_depths, long n_wl, const doub
   hipLaunchKernelGGL(k_store_rtargs, dim3(1), dim3(64), 0, st, a, d_args);
   HIP_TRY(hipGetLastError());
   return rt_launch_set(ctx, d_args, 1, n_wl, base, y, st);
+}
+
+// ------------------------------------------------- allocation-free RT scratch
+// The RT workspace of synthe_lte_spectrum_noalloc for up to
+// SYNTHE_NOALLOC_MAX_WL wavelengths (77 MiB), in a file-scope device array:
+// every GPU the library runs on holds its own copy until the process ends.
+// In host code `scratch` names a host-side stand-in, not device memory:
+// kernels get the device address from hipGetSymbolAddress, taken once per
+// device in synthe_create and kept in g_scratch_dev.
+constexpr size_t kNoAllocScratchBytes = rt_layout(SYNTHE_NOALLOC_MAX_WL).total;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on `device` (the current device).
+static int scratch_address(int device, void** out) {
+  if (device < 0 || device >= kMaxDevices) return fail(SYNTHE_ERANGE, "device ordinal above the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[device]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(SYNTHE_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_dev[device] = p;
+  }
+  *out = g_scratch_dev[device];
+  return SYNTHE_OK;
 }
 
 extern "C" {
@@ -2487,6 +2518,10 @@ int synthe_create(synthe_ctx** out) {
   ctx->tb.ch = ctx->tb.xtau + kNxtau;
   ctx->tb.coefj32 = ctx->d_tab32;
   ctx->tb.diag32 = ctx->d_tab32 + kNxtau * kNxtau;
+  if (int rc = scratch_address(ctx->device, &ctx->scratch)) {
+    synthe_destroy(ctx);
+    return rc;
+  }
   *out = ctx;
   return SYNTHE_OK;
 }
@@ -2677,6 +2712,33 @@ int synthe_lte_spectrum(synthe_ctx* ctx, long n_depths, long n_wl, const double*
   if (rc) return rc;
   return rt_launch(ctx, n_depths, n_wl, wl, temperature, rhox, cont_abs, cont_scat, line_abs, line_scat,
                    line_source, flux_total, flux_cont, iters, (char*)ctx->ws, y, (hipStream_t)stream);
+}
+
+int synthe_lte_spectrum_noalloc(synthe_ctx* ctx, long n_depths, long n_wl, const double* wl,
+                                const double* temperature, const double* rhox, const double* cont_abs,
+                                const double* cont_scat, const double* line_abs, const double* line_scat,
+                                const double* line_source, double* flux_total, double* flux_cont,
+                                int32_t* iters, synthe_stream stream) {
+  if (!ctx) return fail(SYNTHE_EINVAL, "synthe_lte_spectrum_noalloc: ctx is NULL");
+  if (n_depths <= 0 || n_wl < 0) return fail(SYNTHE_EINVAL, "synthe_lte_spectrum_noalloc: bad sizes");
+  if (n_depths > kMaxDepth)
+    return fail(SYNTHE_ERANGE, "synthe_lte_spectrum_noalloc: more than SYNTHE_MAX_DEPTH depths");
+  if (n_wl > SYNTHE_NOALLOC_MAX_WL)
+    return fail(SYNTHE_ERANGE, "synthe_lte_spectrum_noalloc: " + std::to_string(n_wl) +
+                                   " wavelengths, above the static scratch's SYNTHE_NOALLOC_MAX_WL = " +
+                                   std::to_string((long)SYNTHE_NOALLOC_MAX_WL));
+  if (n_wl == 0) return SYNTHE_OK;
+  if (!wl || !temperature || !rhox || !cont_abs || !cont_scat || !line_abs || !line_scat ||
+      !flux_total || !flux_cont)
+    return fail(SYNTHE_EINVAL, "synthe_lte_spectrum_noalloc: NULL array");
+  if (!ctx->scratch) return fail(SYNTHE_EHIP, "synthe_lte_spectrum_noalloc: no scratch address");
+  const RtLayout y = rt_layout(n_wl);
+  if (y.total > kNoAllocScratchBytes)
+    return fail(SYNTHE_ERANGE, "synthe_lte_spectrum_noalloc: workspace exceeds the static scratch");
+  // every byte of the layout that a kernel reads is written earlier in the
+  // same call (the list counter by k_zero_int), so the scratch needs no clearing
+  return rt_launch(ctx, n_depths, n_wl, wl, temperature, rhox, cont_abs, cont_scat, line_abs, line_scat,
+                   line_source, flux_total, flux_cont, iters, (char*)ctx->scratch, y, (hipStream_t)stream);
 }
 
 int synthe_josh_flux_batch(synthe_ctx* ctx, long n, long n_cols, const double* acont,
