@@ -56,21 +56,42 @@ def make_workload(cfg):
                                    cfg["wl1"], seed=1, cache_dir=Path(cache) if cache else None)
 
 
+def host_cores():
+    """(cores this process may use, os.cpu_count(), the cgroup CPU quota in cores or None)."""
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    quota = None
+    try:
+        q, period = Path("/sys/fs/cgroup/cpu.max").read_text().split()[:2]
+        if q != "max":
+            quota = max(1, int(int(q) // int(period)))
+    except (OSError, ValueError):
+        pass
+    return avail, os.cpu_count(), quota
+
+
 def cpu_baseline(wk, cfg, budget_s: float = 20.0):
-    """Time the oracle (all host threads, capped at 16) on a sub-window of the workload."""
+    """Time the oracle on every host core this process may use (its CPU
+    affinity, bounded by the cgroup's CPU quota) on a sub-window of the
+    workload; the oracle reads the window's GFALL text with its own parser."""
+    import tempfile
+
     sys.path.insert(0, str(ROOT / "oracle"))
+    import catalog as oracle_catalog
     import oracle
 
-    threads = min(os.cpu_count() or 1, 16)
-    oracle.set_threads(threads)
-    from kurucz_py_joss_submission_amd import lines as L
+    from kurucz_py_joss_submission_amd import synthetic
 
+    avail, n_cpu, quota = host_cores()
+    threads = min(avail, quota) if quota else avail
+    oracle.set_threads(threads)
     lo = max(cfg["wl0"], 500.0)
     width = 2.0
     last = None
     while True:
         hi = min(lo + width, cfg["wl1"])
-        cat = L.filter_by_range(wk.catalog, lo, hi)
+        with tempfile.TemporaryDirectory() as td:
+            gfall = synthetic.write_workload_gfall(wk, Path(td) / "lines.gfall", window=(lo, hi))
+            cat = oracle_catalog.filter_by_range(oracle_catalog.read_gfall(gfall), lo, hi)
         t0 = time.perf_counter()
         r = oracle.synthesize(wk.atmosphere, cat, lo, hi, 300000.0, keep_stages=False)
         dt = time.perf_counter() - t0
@@ -83,6 +104,7 @@ def cpu_baseline(wk, cfg, budget_s: float = 20.0):
         "value": r.stats["e_line"] / dt,
         "unit": "evals/s",
         "cores": threads,
+        "host_cores": {"affinity": avail, "os_cpu_count": n_cpu, "cgroup_quota": quota},
         "kind": "port",
         "sample": (f"oracle (C restatement of the synthe_py hot path, FP64) on {lo:g}-{hi:g} nm of the "
                    f"same atmosphere and line list: W={r.wavelength.size}, {nl} lines, "

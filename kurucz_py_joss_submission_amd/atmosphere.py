@@ -73,25 +73,10 @@ class AtmosphereModel:
         return int(self.temperature.size)
 
     def hckt_or_default(self) -> np.ndarray:
-        """HCKT per depth; the reference falls back to 1.4388/T (populations.py:166-171)."""
-        if self.hckt is not None and self.hckt.size == self.layers:
-            return np.asarray(self.hckt, dtype=np.float64)
-        t = np.maximum(self.temperature.astype(np.float64), 1.0)
-        return 1.4388 / t
+        return depth_hckt(self)
 
     def txnxn_or_default(self) -> np.ndarray:
-        """TXNXN per depth (populations.py:190-199)."""
-        if self.txnxn is not None:
-            return np.asarray(self.txnxn, dtype=np.float64)
-        n = self.layers
-        out = np.empty(n, dtype=np.float64)
-        for i in range(n):
-            temp = max(float(self.temperature[i]), 1.0)
-            h = float(self.xnf_h[i]) if self.xnf_h is not None else 0.0
-            he = float(self.xnf_he1[i]) if self.xnf_he1 is not None else 0.0
-            h2 = float(self.xnf_h2[i]) if self.xnf_h2 is not None else 0.0
-            out[i] = (h + 0.42 * he + 0.85 * h2) * (temp / 10_000.0) ** 0.3
-        return out
+        return depth_txnxn(self)
 
     def to_npz_dict(self) -> Dict[str, np.ndarray]:
         d = {
@@ -116,6 +101,51 @@ class AtmosphereModel:
 
     def save(self, path: Path) -> None:
         np.savez(path, **self.to_npz_dict())
+
+
+def _opt(atm, name):
+    v = getattr(atm, name, None)
+    return None if v is None else np.asarray(v, dtype=np.float64)
+
+
+def depth_hckt(atm) -> np.ndarray:
+    """HCKT per depth of any atmosphere with the reference's fields: the
+    bundle's value where it has one, else 1.4388 / max(T, 1) (populations.py:162-171)."""
+    temp = np.maximum(np.asarray(atm.temperature, dtype=np.float64), 1.0)
+    hckt = _opt(atm, "hckt")
+    out = 1.4388 / temp
+    if hckt is not None:
+        k = min(hckt.size, temp.size)
+        out[:k] = hckt[:k]
+    return out
+
+
+def depth_txnxn(atm) -> np.ndarray:
+    """TXNXN per depth (populations.py:190-199): the bundle's column, else
+    (xnf_h + 0.42 xnf_he1 + 0.85 xnf_h2) (T / 10^4)^0.3."""
+    tx = _opt(atm, "txnxn")
+    if tx is not None:
+        return tx
+    temp = np.asarray(atm.temperature, dtype=np.float64)
+    cols = [_opt(atm, n) for n in ("xnf_h", "xnf_he1", "xnf_h2")]
+    out = np.empty(temp.size, dtype=np.float64)
+    for i in range(temp.size):  # scalar arithmetic, as the reference's per-depth loop
+        t = max(float(temp[i]), 1.0)
+        h, he, h2 = (float(c[i]) if c is not None else 0.0 for c in cols)
+        out[i] = (h + 0.42 * he + 0.85 * h2) * (t / 10_000.0) ** 0.3
+    return out
+
+
+def depth_rho(atm) -> np.ndarray:
+    """Mass density, ones when absent (physics/line_opacity.py:794-797)."""
+    v = _opt(atm, "mass_density")
+    return np.ones(atm.temperature.size) if v is None else v
+
+
+def depth_xne(atm) -> np.ndarray:
+    """Electron density, zeros when absent (physics/line_opacity.py:798-801)."""
+    v = _opt(atm, "electron_density")
+    return np.zeros(atm.temperature.size) if v is None else v
 
 
 def load_cached(path: Path) -> AtmosphereModel:

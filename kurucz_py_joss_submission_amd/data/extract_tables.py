@@ -1,11 +1,14 @@
 """Write ``kurucz_tables.npz``: the numeric tables the synthe hot path needs.
 
 The tables are Kurucz ATLAS/SYNTHE data, not code: the JOSH quadrature
-weights and Lambda-operator matrices (``synthe_py/physics/josh_tables.py``)
-and the Voigt H1 node table (``synthe_py/physics/tables.py:249-272``).
-They are read here as text with ``ast`` (nothing from the reference is
-imported or executed) and stored as float64 arrays, so the package and the
-GPU library carry the identical numbers without the reference present.
+weights and Lambda-operator matrices (``synthe_py/physics/josh_tables.py``),
+the Voigt H1 node table (``synthe_py/physics/tables.py:249-272``) and the
+POTION ionisation potentials in cm^-1 that the GFALL damping defaults use
+(``synthe_py/data/fortran_data.npz``, read by ``io/lines/atomic.py:21-37``).
+The source tables are read as text with ``ast`` and the archive with
+``numpy.load(allow_pickle=False)`` (nothing from the reference is imported
+or executed) and stored as float64 arrays, so the package and the GPU
+library carry the identical numbers without the reference present.
 
 Usage::
 
@@ -49,6 +52,8 @@ def main(ref_root: str = "/root/reference") -> None:
     root = Path(ref_root) / "synthe_py" / "physics"
     josh = _literal_assignments(root / "josh_tables.py")
     voigt = _literal_assignments(root / "tables.py")  # tabvi / tabh1 inside _voigt_reference_tables
+    with np.load(Path(ref_root) / "synthe_py" / "data" / "fortran_data.npz", allow_pickle=False) as z:
+        potion = np.asarray(z["potion"], dtype=np.float64)
     tables = {
         "xtau": josh["XTAU_GRID"],
         "ch": josh["CH_WEIGHTS"],
@@ -57,9 +62,11 @@ def main(ref_root: str = "/root/reference") -> None:
         "coefh": josh["COEFH_MATRIX"],
         "tabvi": voigt["tabvi"],
         "tabh1": voigt["tabh1"],
+        "potion": potion,
     }
     assert tables["xtau"].shape == (51,) and tables["coefj"].shape == (51, 51)
     assert tables["tabvi"].shape == (81,) and tables["tabh1"].shape == (81,)
+    assert tables["potion"].shape == (999,)
     np.savez(OUT, **tables)
     print(f"wrote {OUT}: " + ", ".join(f"{k}{v.shape}" for k, v in tables.items()))
 

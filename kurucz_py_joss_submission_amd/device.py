@@ -14,7 +14,7 @@ from typing import Dict, List, Optional, Sequence, Tuple
 import numpy as np
 
 from . import _native as N
-from .atmosphere import AtmosphereModel
+from .atmosphere import AtmosphereModel, depth_hckt, depth_rho, depth_txnxn, depth_xne
 from .lines import LineCatalog, LineSoA, build_line_soa
 
 
@@ -69,6 +69,12 @@ class DepthArrays:
 
 
 def upload_atmosphere(atm: AtmosphereModel) -> DepthArrays:
+    """Device copies of a bundle's per-depth arrays.  atm is this package's
+    AtmosphereModel or any object with the reference's fields
+    (io/atmosphere.py:16-79); missing optional columns take the reference's
+    fallbacks."""
+    if getattr(atm, "population_per_ion", None) is None or getattr(atm, "doppler_per_ion", None) is None:
+        raise ValueError("atmosphere bundle lacks population_per_ion / doppler_per_ion")
     if atm.population_per_ion is None or atm.doppler_per_ion is None:
         raise ValueError("atmosphere bundle lacks population_per_ion / doppler_per_ion")
     if atm.continuum_wledge is None or atm.continuum_abs_coeff is None or atm.continuum_scat_coeff is None:
@@ -89,10 +95,10 @@ def upload_atmosphere(atm: AtmosphereModel) -> DepthArrays:
     return DepthArrays(
         temperature=to_device(atm.temperature, f8),
         rhox=to_device(atm.depth, f8),
-        rho=to_device(atm.mass_density, f8),
-        xne=to_device(atm.electron_density, f8),
-        txnxn=to_device(atm.txnxn_or_default(), f8),
-        hckt=to_device(atm.hckt_or_default(), f8),
+        rho=to_device(depth_rho(atm), f8),
+        xne=to_device(depth_xne(atm), f8),
+        txnxn=to_device(depth_txnxn(atm), f8),
+        hckt=to_device(depth_hckt(atm), f8),
         pop=to_device(pop, f8),
         dop=to_device(atm.doppler_per_ion, f8),
         n_ion=int(pop.shape[1]),

@@ -7,19 +7,25 @@ Mirrors the reference's line semantics (synthe_py/io/lines/atomic.py):
 * gf = 10**(log gf + X1 + X2) (atomic.py:381-383, LineCatalog.from_records 275);
 * E_low = min(|E|, |EP|) in cm^-1 (atomic.py:389-391);
 * gamma = 10**G / (12.5664 * nu), nu = 2.99792458e17/lambda (atomic.py:495-504),
-  with G == 0 meaning "use the default" (radiative default 2.223e13/lambda^2,
-  atomic.py:466-469);
+  with G == 0 meaning "use the default": radiative 2.223e13/lambda^2
+  (atomic.py:466-469), Stark and van der Waals from the effective quantum
+  numbers of the levels against the POTION ionisation potentials
+  (atomic.py:59-199, applied at 476-492);
 * ion stage = round(frac(CODE)*100) + 1 (atomic.py:508-511);
+* records that are not fixed-width GFALL go to the whitespace reader
+  (atomic.py:594-672);
 * the DELLIM window filter (atomic.py:712-816).
 
 The parser is vectorised over fixed-width records (one numpy pass instead of a
 Python loop per line); every arithmetic step is the reference's, in the same
-order.  Stark/van der Waals defaults need the POTION ionisation table and are
-not implemented: records with GS == 0 or GW == 0 raise.
+order, with the same routine (numpy or the C library) where the two round
+differently.
 """
 
 from __future__ import annotations
 
+import math
+import os
 from dataclasses import dataclass, field
 from pathlib import Path
 from typing import Optional
@@ -35,6 +41,20 @@ from .constants import (
 
 # DELLIM margins in nm (atomic.py:693)
 _DELLIM = np.array([100.0, 30.0, 10.0, 3.0, 1.0, 0.3, 0.1], dtype=np.float64)
+# the GFALL reader names elements up to uranium; heavier codes become "Z<n>"
+# and no plain-line kernel takes them (io/lines/tfort.py:14-108)
+PARSER_MAX_Z = 92
+
+_POTION: Optional[np.ndarray] = None
+
+
+def potion() -> np.ndarray:
+    """Ionisation potentials in cm^-1 by POTION slot (synthe_py/data/fortran_data.npz)."""
+    global _POTION
+    if _POTION is None:
+        with np.load(Path(__file__).resolve().parent / "data" / "kurucz_tables.npz", allow_pickle=False) as z:
+            _POTION = np.asarray(z["potion"], dtype=np.float64)
+    return _POTION
 
 
 @dataclass
@@ -62,8 +82,15 @@ class LineCatalog:
 
     @property
     def elements(self) -> np.ndarray:
-        return np.array([ELEMENT_SYMBOLS[z] if 0 < z < len(ELEMENT_SYMBOLS) else f"Z{z}"
-                         for z in self.atomic_number], dtype=object)
+        """Element symbols as the reference's records carry them ("Z<n>" past uranium)."""
+        out = []
+        for i, z in enumerate(self.atomic_number.tolist()):
+            if 0 < z < len(ELEMENT_SYMBOLS):
+                out.append(ELEMENT_SYMBOLS[z])
+            else:
+                n = int(self.code[i] + 1e-6) if self.code is not None else z
+                out.append(f"Z{n}")
+        return np.array(out, dtype=object)
 
     def subset(self, idx: np.ndarray) -> "LineCatalog":
         kw = {}
@@ -88,64 +115,207 @@ def _field(block: np.ndarray, a: int, b: int) -> np.ndarray:
     return np.char.strip(raw)
 
 
-def _to_float(col: np.ndarray, default: float = 0.0) -> np.ndarray:
+def _floats(col: np.ndarray, default: float = 0.0):
+    """float() of each stripped field (blank -> default) and where it parsed."""
     out = np.full(col.shape, default, dtype=np.float64)
+    ok = np.ones(col.shape, dtype=bool)
     nz = col != b""
     if np.any(nz):
-        out[nz] = col[nz].astype(np.float64)
+        try:
+            out[nz] = col[nz].astype(np.float64)
+        except ValueError:
+            for i in np.nonzero(nz)[0]:
+                try:
+                    out[i] = float(col[i])
+                except ValueError:
+                    ok[i] = False
+    return out, ok
+
+
+def _ints(col: np.ndarray, default: int = 0) -> np.ndarray:
+    """int() of each field, default where blank or not an integer."""
+    uniq, inv = np.unique(col, return_inverse=True)
+    vals = np.empty(uniq.size, dtype=np.int64)
+    for k, u in enumerate(uniq.tolist()):
+        try:
+            vals[k] = int(u) if u.strip() else default
+        except ValueError:
+            vals[k] = default
+    return vals[inv.reshape(col.shape)]
+
+
+def _code_parts(code: np.ndarray):
+    nelem = (code + 1e-6).astype(np.int64)
+    icharge = ((code - nelem) * 100.0 + 0.1).astype(np.int64)
+    return nelem, icharge, icharge + 1
+
+
+def _potion_at(nelem: np.ndarray, icharge: np.ndarray):
+    """POTION value of each (element, charge) and whether the slot exists (atomic.py:40-56)."""
+    slot = np.where(nelem <= 30, nelem * (nelem + 1) // 2 + icharge, nelem * 5 + 341 + icharge) - 1
+    tab = potion()
+    inside = (slot >= 0) & (slot < tab.size)
+    return np.where(inside, tab[np.clip(slot, 0, tab.size - 1)], 0.0), inside
+
+
+def stark_default_log(code: np.ndarray, e_lower: np.ndarray, e_upper: np.ndarray) -> np.ndarray:
+    """log10 GAMMAS of records without one (atomic.py:59-106): an effective
+    principal quantum number of the upper level from its distance to the
+    ionisation limit, n*^2 = R Z_eff^2 / (IP - E_up) (25 when unbound),
+    GAMMAS = 1e-8 n*^5; molecules (CODE >= 100) take 1e-5."""
+    code = np.asarray(code, np.float64)
+    nelem, icharge, zeff = _code_parts(code)
+    eup = np.maximum(np.abs(e_lower), np.abs(e_upper))
+    ip, inside = _potion_at(nelem, icharge)
+    gap = ip - eup
+    n2 = np.full(code.shape, 25.0)
+    bound = inside & (gap > 0)
+    n2[bound] = 109737.31 * zeff[bound] * zeff[bound] / gap[bound]
+    out = np.empty(code.shape)
+    for i in range(code.size):  # the reference's scalar math, for the same roundings
+        out[i] = -5.0 if code[i] >= 100.0 else math.log10(1.0e-8 * n2[i] * n2[i] * math.sqrt(n2[i]))
     return out
 
 
-def load_catalog(path: Path) -> LineCatalog:
-    """Parse a GFALL fixed-width file (atomic.py:299-610, fixed-width path).
+def vdw_default_log(code: np.ndarray, e_lower: np.ndarray, e_upper: np.ndarray) -> np.ndarray:
+    """log10 GAMMAW of records without one (atomic.py:109-199): Unsold's
+    approximation 4.5e-9 (<r^2>_up - <r^2>_lo)^0.4 with hydrogenic
+    <r^2> = 2.5 (n*^2 / Z_eff)^2, n*^2 capped at 1000; the transition-metal
+    sequences 21-28 take (45 - NSEQ)/Z_eff; molecules take 1e-7 / Z_eff."""
+    code = np.asarray(code, np.float64)
+    nelem, icharge, zeff = _code_parts(code)
+    elo = np.minimum(np.abs(e_lower), np.abs(e_upper))
+    eup = np.maximum(np.abs(e_lower), np.abs(e_upper))
+    ip, _ = _potion_at(nelem, icharge)
+    out = np.empty(code.shape)
+    for i in range(code.size):
+        z = int(zeff[i])
+        if code[i] >= 100.0:
+            out[i] = math.log10(1.0e-7 / z)
+            continue
+        r2 = []
+        for level in (eup[i], elo[i]):
+            n2 = 25.0
+            if ip[i] > 0 and ip[i] - level > 0:
+                n2 = 109737.31 * z * z / (ip[i] - level)
+            n2 = min(n2, 1000.0)
+            r2.append(2.5 * (n2 / z) ** 2)
+        r_up, r_lo = r2
+        nseq = int(nelem[i]) - z + 1
+        if 20 < nseq < 29:
+            r_up, r_lo = (45.0 - nseq) / z, 0.0
+        if r_up < r_lo:
+            r_up = 2.0 * r_lo
+        d = r_up - r_lo
+        out[i] = math.log10(4.5e-9 * (d ** 0.4) if d > 0 else 1.0e-9)
+    return out
 
-    Records must be at least 98 characters wide; comment and blank lines are
-    skipped.  Raises ValueError for records the hot path cannot represent.
+
+def _pow10_or_zero(log_values: np.ndarray) -> np.ndarray:
+    # scalar pow, as the reference exponentiates each record's G
+    return np.array([10.0 ** v if v != 0.0 else 0.0 for v in log_values.tolist()], dtype=np.float64)
+
+
+def _whitespace_record(line: str):
+    """A record in the whitespace format (atomic.py:594-672), or None."""
+    parts = line.split()
+    if len(parts) < 7:
+        return None
+    try:
+        wl = float(parts[0])
+        sym = parts[1]
+        try:
+            stage = int(float(parts[2]))
+        except (ValueError, IndexError):
+            return None
+        vals = [float(parts[k]) for k in (3, 4, 5, 6)]
+        gw = float(parts[7]) if len(parts) > 7 else 0.0
+        nlo = nup = 0
+        if len(parts) > 9:
+            try:
+                nlo = int(float(parts[8]))
+                nup = int(float(parts[9]))
+            except ValueError:
+                pass
+        lt = {("H", 1): -1, ("He", 1): -3, ("He", 2): -6}.get((sym, stage), 0)
+        z = ELEMENT_Z.get(sym.strip().upper().replace(" ", ""), 0)
+        return dict(wl=wl, z=z, stage=stage, log_gf=vals[0], elo=vals[1],
+                    gr=vals[2], gs=vals[3], gw=gw, lt=lt, nlo=nlo, nup=nup)
+    except (ValueError, IndexError):
+        return None
+
+
+def load_catalog(path: Path) -> LineCatalog:
+    """Parse a GFALL file (atomic.py:299-686).
+
+    Fixed-width records (98 columns or more) are parsed column-wise; a record
+    that is shorter, or whose fixed-width fields do not parse, is read in the
+    whitespace format instead, and skipped if that fails too, as the
+    reference does.  Comment and blank lines are skipped.
     """
-    text = Path(path).read_bytes().splitlines()
-    recs = [ln for ln in text if ln.strip() and not ln.strip().startswith(b"#")]
+    with Path(path).open("r", encoding="ascii", errors="ignore") as fh:
+        text = fh.read().splitlines()
+    recs = [ln for ln in text if ln.strip() and not ln.strip().startswith("#")]
     if not recs:
         return _empty_catalog()
-    width = max(len(r) for r in recs)
-    if min(len(r) for r in recs) < 98:
-        raise ValueError("GFALL records shorter than 98 columns are not supported")
-    width = max(width, 160)
-    block = np.zeros((len(recs), width), dtype=np.uint8) + ord(" ")
-    for i, r in enumerate(recs):
-        block[i, : len(r)] = np.frombuffer(r, dtype=np.uint8)
+    n_rec = len(recs)
     lens = np.array([len(r) for r in recs])
+    width = max(int(lens.max()), 160)
+    fixed = lens >= 98
+    block = np.full((n_rec, width), ord(" "), dtype=np.uint8)
+    for i, r in enumerate(recs):
+        block[i, : len(r)] = np.frombuffer(r.encode("ascii"), dtype=np.uint8)
 
-    wl_stored = _to_float(_field(block, 0, 11))
-    log_gf_raw = _to_float(_field(block, 11, 18))
-    code = _to_float(_field(block, 18, 24))
-    e_val = _to_float(_field(block, 24, 36))
-    ep_val = _to_float(_field(block, 52, 64))
-    gr_log = _to_float(_field(block, 80, 86))
-    gs_log = _to_float(_field(block, 86, 92))
-    gw_log = _to_float(_field(block, 92, 98))
-    x1 = np.where(lens >= 115, _to_float(_field(block, 109, 115)), 0.0)
-    x2 = np.where(lens >= 124, _to_float(_field(block, 118, 124)), 0.0)
-    nblo = np.where(lens >= 106, _to_float(_field(block, 102, 104)), 0.0).astype(np.int16)
-    nbup = np.where(lens >= 106, _to_float(_field(block, 104, 106)), 0.0).astype(np.int16)
-    iso1 = np.where(lens >= 109, _to_float(_field(block, 106, 109)), 0.0).astype(np.int32)
-    cother1 = _field(block, 124, 134)
-    isoshift = np.where(lens >= 160, _to_float(_field(block, 154, 160)), 0.0)
-    cother2 = np.ascontiguousarray(block[:, 134:144]).view("S10").ravel()
+    wl_stored, ok0 = _floats(_field(block, 0, 11))
+    log_gf_raw, ok1 = _floats(_field(block, 11, 18))
+    code, ok2 = _floats(_field(block, 18, 24))
+    e_col = _field(block, 24, 36)
+    e_val, ok3 = _floats(e_col)
+    ep_val, ok4 = _floats(_field(block, 52, 64))
+    gr_log, ok5 = _floats(_field(block, 80, 86))
+    gs_log, ok6 = _floats(_field(block, 86, 92))
+    gw_log, ok7 = _floats(_field(block, 92, 98))
+    _, ok8 = _floats(_field(block, 36, 41))  # J values: parsed, a bad one fails the record
+    _, ok9 = _floats(_field(block, 64, 69))
+    good = fixed & ok0 & ok1 & ok2 & ok3 & ok4 & ok5 & ok6 & ok7 & ok8 & ok9
+    good &= (_field(block, 0, 11) != b"") & (_field(block, 11, 18) != b"") & (_field(block, 18, 24) != b"")
+    good &= e_col != b""
+
+    x1, okx1 = _floats(_field(block, 109, 115))
+    x2, okx2 = _floats(_field(block, 118, 124))
+    x1 = np.where((lens >= 115) & okx1, x1, 0.0)
+    x2 = np.where((lens >= 124) & okx2, x2, 0.0)
+    nblo_s, nbup_s = _field(block, 102, 104), _field(block, 104, 106)
+    nblo = np.where(lens >= 106, _ints(nblo_s), 0)
+    nbup = np.where(lens >= 106, _ints(nbup_s), 0)
+    # int() of NBUP failing after NBLO parsed keeps NBLO (atomic.py:524-531)
+    bad_lo = (nblo_s != b"") & (_ints(nblo_s, -99999) == -99999)
+    nbup = np.where(bad_lo, 0, nbup)
+    nblo = nblo.astype(np.int16)
+    nbup = nbup.astype(np.int16)
+    iso1 = np.where(lens >= 109, _ints(_field(block, 106, 109)), 0)
+    isoshift = np.where(lens >= 160, _ints(_field(block, 154, 160)), 0)
+    hyper = np.ascontiguousarray(block[:, 124:134]).view("S10").ravel()
+    has_hyper = (lens >= 134) & (np.char.strip(hyper) != b"")
+    eshift = np.zeros(n_rec)
+    eshiftp = np.zeros(n_rec)
+    if np.any(has_hyper):
+        # ISHIFT / ISHIFTP in mK; a field that is not an integer zeroes both (atomic.py:421-432)
+        s1 = _ints(np.ascontiguousarray(block[:, 124:129]).view("S5").ravel(), -99999)
+        s2 = _ints(np.ascontiguousarray(block[:, 129:134]).view("S5").ravel(), -99999)
+        both = has_hyper & (s1 != -99999) & (s2 != -99999)
+        eshift = np.where(both, s1 * 1.0e-3, 0.0)
+        eshiftp = np.where(both, s2 * 1.0e-3, 0.0)
+    tail_ok = (lens >= 144) & (np.char.strip(np.ascontiguousarray(block[:, 134:144]).view("S10").ravel()) != b"")
     line_size_chr = np.ascontiguousarray(block[:, 140:141]).view("S1").ravel()
     auto_tag = np.char.strip(np.ascontiguousarray(block[:, 141:144]).view("S3").ravel())
 
-    if np.any(cother1 != b""):
-        # hyperfine shifts: ishift/ishiftp in mK (atomic.py:421-432)
-        ishift = _to_float(_field(block, 124, 129))
-        ishiftp = _to_float(_field(block, 129, 134))
-        eshift = ishift * 1.0e-3
-        eshiftp = ishiftp * 1.0e-3
+    if os.environ.get("PY_APPLY_ISO_CORR", "1") != "0":
+        log_gf = log_gf_raw + (x1 + x2)
     else:
-        eshift = np.zeros_like(e_val)
-        eshiftp = np.zeros_like(e_val)
-
-    log_gf = log_gf_raw + (x1 + x2)
+        log_gf = log_gf_raw + 0.0
     excitation_cm = np.minimum(np.abs(e_val), np.abs(ep_val))
+    excitation_up = np.maximum(np.abs(e_val), np.abs(ep_val))
 
     wavelength = wl_stored.copy()
     energy_diff = np.abs((np.abs(ep_val) + eshiftp) - (np.abs(e_val) + eshift))
@@ -153,14 +323,16 @@ def load_catalog(path: Path) -> LineCatalog:
     dwliso = isoshift * 1.0e-4
     wavelength[pos] = 1.0e7 / energy_diff[pos] + dwliso[pos]
 
-    if np.any(gs_log == 0.0) or np.any(gw_log == 0.0):
-        raise ValueError("Stark / van der Waals defaults (GS or GW == 0) need the POTION "
-                         "table and are outside the hot path's line lists")
-    gamma_rad = np.array([10.0 ** v if v != 0.0 else 0.0 for v in gr_log], dtype=np.float64)
-    gamma_stark = np.array([10.0 ** v for v in gs_log], dtype=np.float64)
-    gamma_vdw = np.array([10.0 ** v for v in gw_log], dtype=np.float64)
+    gamma_rad = _pow10_or_zero(gr_log)
+    gamma_stark = _pow10_or_zero(gs_log)
+    gamma_vdw = _pow10_or_zero(gw_log)
     zero_r = gamma_rad == 0.0
-    gamma_rad[zero_r] = 2.223e13 / (wavelength[zero_r] ** 2)
+    with np.errstate(divide="ignore"):  # rows that failed to parse are dropped below
+        gamma_rad[zero_r] = 2.223e13 / (wavelength[zero_r] ** 2)
+    for gam, default in ((gamma_stark, stark_default_log), (gamma_vdw, vdw_default_log)):
+        zero = np.nonzero(good & (gam == 0.0))[0]
+        if zero.size:
+            gam[zero] = _pow10_or_zero(default(code[zero], excitation_cm[zero], excitation_up[zero]))
     frelin = C_LIGHT_NM / np.maximum(wavelength, 1e-12)
     gamma_rad = gamma_rad / (12.5664 * frelin)
     gamma_stark = gamma_stark / (12.5664 * frelin)
@@ -169,12 +341,12 @@ def load_catalog(path: Path) -> LineCatalog:
     nelem = (code + 1e-6).astype(np.int64)
     frac = code - nelem
     ion_stage = np.where(frac > 1e-6, np.round(frac * 100.0).astype(np.int64) + 1, 1).astype(np.int16)
-    z = np.where((nelem > 0) & (nelem < len(ELEMENT_SYMBOLS)), nelem, 0).astype(np.int32)
+    z = np.where((nelem > 0) & (nelem <= PARSER_MAX_Z), nelem, 0).astype(np.int32)
 
-    line_size = np.array([int(c) if c.isdigit() else 0 for c in line_size_chr.tolist()],
-                         dtype=np.int16)
-    line_type = np.zeros(len(recs), dtype=np.int8)
-    tags = auto_tag.tolist()
+    line_size = np.array([int(c) if c.isdigit() else 0 for c in line_size_chr.tolist()], dtype=np.int16)
+    line_size = np.where(tail_ok, line_size, 0).astype(np.int16)
+    line_type = np.zeros(n_rec, dtype=np.int8)
+    tags = np.where(tail_ok, auto_tag, b"").tolist()
     for i, t in enumerate(tags):
         if t == b"AUT":
             line_type[i] = 1
@@ -188,14 +360,12 @@ def load_catalog(path: Path) -> LineCatalog:
     line_type[plain & (nelem == 2) & (ion_stage == 1) & (iso1 == 3)] = -4
     line_type[plain & (nelem == 2) & (ion_stage == 1) & (iso1 != 3)] = -3
     line_type[plain & (nelem == 2) & (ion_stage == 2)] = -6
-    del cother2
 
-    gf = np.power(10.0, log_gf)
-    return LineCatalog(
+    cat = LineCatalog(
         wavelength=wavelength,
         index_wavelength=wavelength.copy(),
         log_gf=log_gf,
-        gf=gf,
+        gf=np.zeros(n_rec),
         excitation_energy=excitation_cm,
         gamma_rad=gamma_rad,
         gamma_stark=gamma_stark,
@@ -208,6 +378,41 @@ def load_catalog(path: Path) -> LineCatalog:
         code=code,
         line_size=line_size,
     )
+    if not np.all(good):
+        cat = _merge_whitespace(cat, recs, good)
+    # from_records exponentiates the whole catalog's log gf with numpy (atomic.py:275)
+    cat.gf = np.power(10.0, cat.log_gf)
+    return cat
+
+
+def _merge_whitespace(cat: LineCatalog, recs, good: np.ndarray) -> LineCatalog:
+    """Fixed-width records where good, whitespace-format ones elsewhere, file order."""
+    keep, extra = [], {}
+    for i in range(len(recs)):
+        if good[i]:
+            keep.append(i)
+        else:
+            r = _whitespace_record(recs[i])
+            if r is not None:
+                keep.append(i)
+                extra[i] = r
+    keep = np.array(keep, dtype=np.int64)
+    out = cat.subset(keep)
+    for j, i in enumerate(keep.tolist()):
+        r = extra.get(i)
+        if r is None:
+            continue
+        out.wavelength[j] = out.index_wavelength[j] = r["wl"]
+        out.log_gf[j] = r["log_gf"]
+        out.excitation_energy[j] = r["elo"]
+        out.gamma_rad[j], out.gamma_stark[j], out.gamma_vdw[j] = r["gr"], r["gs"], r["gw"]
+        out.atomic_number[j] = r["z"]
+        out.ion_stages[j] = r["stage"]
+        out.line_types[j] = r["lt"]
+        out.n_lower[j], out.n_upper[j] = r["nlo"], r["nup"]
+        out.code[j] = 0.0
+        out.line_size[j] = 0
+    return out
 
 
 def _empty_catalog() -> LineCatalog:
@@ -215,6 +420,52 @@ def _empty_catalog() -> LineCatalog:
     return LineCatalog(e, e, e, e, e, e, e, e, np.zeros(0, np.int32), np.zeros(0, np.int16),
                        np.zeros(0, np.int8), np.zeros(0, np.int16), np.zeros(0, np.int16), e,
                        np.zeros(0, np.int16))
+
+
+def as_line_catalog(catalog) -> LineCatalog:
+    """This package's LineCatalog for either catalog type.
+
+    A catalog of the reference's own type (io/lines/atomic.py:235-296) carries
+    element symbols, not atomic numbers, and keeps CODE and LINESIZE on its
+    records; its arrays are taken as they are, Z from the symbol as the engine
+    looks it up (engine/opacity.py:634-636)."""
+    if isinstance(catalog, LineCatalog):
+        return catalog
+    from .constants import element_atomic_number
+
+    recs = getattr(catalog, "records", None)
+    wl = np.asarray(catalog.wavelength, dtype=np.float64)
+    n = wl.size
+
+    def arr(name, dtype):
+        v = getattr(catalog, name, None)
+        return np.zeros(n, dtype) if v is None else np.asarray(v).astype(dtype)
+
+    def from_records(name, dtype):
+        if recs is None:
+            return np.zeros(n, dtype)
+        return np.array([getattr(r, name, 0) or 0 for r in recs], dtype=dtype)
+
+    idx = getattr(catalog, "index_wavelength", None)
+    z = np.array([element_atomic_number(e) or 0 for e in np.asarray(catalog.elements, dtype=object)],
+                 dtype=np.int32)
+    return LineCatalog(
+        wavelength=wl,
+        index_wavelength=wl.copy() if idx is None else np.asarray(idx, dtype=np.float64),
+        log_gf=arr("log_gf", np.float64),
+        gf=arr("gf", np.float64),
+        excitation_energy=arr("excitation_energy", np.float64),
+        gamma_rad=arr("gamma_rad", np.float64),
+        gamma_stark=arr("gamma_stark", np.float64),
+        gamma_vdw=arr("gamma_vdw", np.float64),
+        atomic_number=z,
+        ion_stages=arr("ion_stages", np.int16),
+        line_types=arr("line_types", np.int8),
+        n_lower=arr("n_lower", np.int16),
+        n_upper=arr("n_upper", np.int16),
+        code=from_records("code", np.float64),
+        line_size=from_records("line_size", np.int16),
+    )
 
 
 def line_margin(line_type: np.ndarray, atomic_number: np.ndarray, wl_min: float,
@@ -233,6 +484,7 @@ def line_margin(line_type: np.ndarray, atomic_number: np.ndarray, wl_min: float,
 
 def filter_by_range(catalog: LineCatalog, wl_min: float, wl_max: float) -> LineCatalog:
     """Keep lines inside [wl_min - margin, wl_max + margin] (atomic.py:743-816)."""
+    catalog = as_line_catalog(catalog)
     if len(catalog) == 0:
         return catalog
     margin = line_margin(catalog.line_types, catalog.atomic_number, wl_min, catalog.line_size,
@@ -336,7 +588,10 @@ def wing_center_indices(grid: np.ndarray, values: np.ndarray,
 
 def build_line_soa(catalog: LineCatalog, grid: np.ndarray, n_elements: int,
                    include_h_lines: bool = False, grid_origin: Optional[float] = None) -> LineSoA:
-    """Host prep of the per-line arrays (line_opacity.py:740-784, 1030-1059)."""
+    """Host prep of the per-line arrays (line_opacity.py:740-784, 1030-1059).
+
+    catalog is this package's LineCatalog or the reference's (as_line_catalog)."""
+    catalog = as_line_catalog(catalog)
     lt = catalog.line_types.astype(np.int32)
     nlo = np.abs(catalog.n_lower.astype(np.int64)) if catalog.n_lower is not None else 0
     nup = np.abs(catalog.n_upper.astype(np.int64)) if catalog.n_upper is not None else 0

@@ -18,7 +18,7 @@ from __future__ import annotations
 
 from dataclasses import dataclass
 from pathlib import Path
-from typing import Dict, Optional, Sequence
+from typing import Dict, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -314,6 +314,26 @@ class Workload:
     wl_end: float
     resolution: float
     name: str
+    line_spec: Optional[Tuple[int, float, float, int]] = None  # (n, lambda0, lambda1, seed) drawn
+
+
+def write_workload_gfall(wk: Workload, path: Path, window: Optional[Tuple[float, float]] = None,
+                         pad_nm: float = 2.0) -> Path:
+    """Write the GFALL text a workload's catalog was parsed from (the draw is
+    seeded, so it is redrawn).  With window, only lines drawn within pad_nm of
+    it are written: the DELLIM margin of these species' lines is 0.1 nm times
+    max(1, lambda0/500) (< 0.4 nm below 2000 nm) and a parsed wavelength is
+    within 1e-3 nm of the drawn one, so the window's filtered catalog is
+    unchanged."""
+    if wk.line_spec is None:
+        raise ValueError("workload has no line draw to write")
+    n, lw0, lw1, seed = wk.line_spec
+    d = synthetic_lines(n, lw0, lw1, seed)
+    if window is not None:
+        keep = (d["wl"] >= window[0] - pad_nm) & (d["wl"] <= window[1] + pad_nm)
+        d = {k: v[keep] for k, v in d.items()}
+    write_line_list(Path(path), d)
+    return Path(path)
 
 
 def make_workload(name: str, teff: float, logg: float, n_lines: int, wl0: float, wl1: float,
@@ -337,7 +357,7 @@ def make_workload(name: str, teff: float, logg: float, n_lines: int, wl0: float,
             from .atmosphere import load_cached
             cat = _lines.LineCatalog.load_npz(cat_npz)
             return Workload(load_cached(atm_npz), _lines.filter_by_range(cat, wl0, wl1),
-                            wl0, wl1, resolution, name)
+                            wl0, wl1, resolution, name, (n_lines, lw0, lw1, seed))
     atm = build_bundle(make_atm(teff, logg, n_layers, mh))
     d = synthetic_lines(n_lines, lw0, lw1, seed)
     import tempfile
@@ -348,4 +368,5 @@ def make_workload(name: str, teff: float, logg: float, n_lines: int, wl0: float,
     if cache is not None:
         atm.save(atm_npz)
         cat.save_npz(cat_npz)
-    return Workload(atm, _lines.filter_by_range(cat, wl0, wl1), wl0, wl1, resolution, name)
+    return Workload(atm, _lines.filter_by_range(cat, wl0, wl1), wl0, wl1, resolution, name,
+                    (n_lines, lw0, lw1, seed))

@@ -1,0 +1,396 @@
+"""GFALL line lists for the oracle (TEST INFRASTRUCTURE ONLY).
+
+The oracle's own restatement of the reference's line-list steps, from the
+text of a GFALL file to the per-line arrays the opacity functions of
+synthe_oracle.c read.  It shares no code with the package under test: only
+tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it, as
+the checker.
+
+* read_gfall        io/lines/atomic.py:299-686 (one record at a time, as the
+                    reference loops), LineCatalog.from_records 256-296
+* stark_default     io/lines/atomic.py:59-106
+* vdw_default       io/lines/atomic.py:109-199
+* filter_by_range   io/lines/atomic.py:697-816
+* line_soa          physics/line_opacity.py:700-784 (the TRANSP prep loop);
+                    centre indices from the C restatements of
+                    engine/opacity.py:550-592 and physics/line_opacity.py:1030-1059
+
+The ionisation potentials (POTION, cm^-1) are the reference's data table
+(synthe_py/data/fortran_data.npz), carried in the package's table archive.
+"""
+
+from __future__ import annotations
+
+import math
+import os
+from dataclasses import dataclass
+from pathlib import Path
+from typing import List, Optional
+
+import numpy as np
+
+_TABLES = Path(__file__).resolve().parents[1] / "kurucz_py_joss_submission_amd" / "data" / "kurucz_tables.npz"
+
+# element symbols of the GFALL reader (io/lines/tfort.py:14-108): Z = 0..92
+PARSER_SYMBOLS = (
+    "", "H", "He", "Li", "Be", "B", "C", "N", "O", "F", "Ne", "Na", "Mg", "Al", "Si", "P", "S", "Cl",
+    "Ar", "K", "Ca", "Sc", "Ti", "V", "Cr", "Mn", "Fe", "Co", "Ni", "Cu", "Zn", "Ga", "Ge", "As", "Se",
+    "Br", "Kr", "Rb", "Sr", "Y", "Zr", "Nb", "Mo", "Tc", "Ru", "Rh", "Pd", "Ag", "Cd", "In", "Sn", "Sb",
+    "Te", "I", "Xe", "Cs", "Ba", "La", "Ce", "Pr", "Nd", "Pm", "Sm", "Eu", "Gd", "Tb", "Dy", "Ho", "Er",
+    "Tm", "Yb", "Lu", "Hf", "Ta", "W", "Re", "Os", "Ir", "Pt", "Au", "Hg", "Tl", "Pb", "Bi", "Po", "At",
+    "Rn", "Fr", "Ra", "Ac", "Th", "Pa", "U",
+)
+# atomic numbers the engine looks symbols up in (engine/opacity.py:316-416): H..Es
+_ENGINE_Z = {s.upper(): z for z, s in enumerate(PARSER_SYMBOLS) if s}
+_ENGINE_Z.update({"NP": 93, "PU": 94, "AM": 95, "CM": 96, "BK": 97, "CF": 98, "ES": 99})
+
+DELLIM = (100.0, 30.0, 10.0, 3.0, 1.0, 0.3, 0.1)  # nm, rgfall's margins (atomic.py:693)
+CGF_FACTOR = 0.026538 / 1.77245
+C_NM = 2.99792458e17
+
+_potion: Optional[np.ndarray] = None
+
+
+def potion() -> np.ndarray:
+    global _potion
+    if _potion is None:
+        with np.load(_TABLES, allow_pickle=False) as z:
+            _potion = np.array(z["potion"], dtype=np.float64)
+    return _potion
+
+
+def potion_index(iz: int, icharge: int) -> int:
+    """0-based POTION slot of element iz in charge state icharge (atomic.py:40-56)."""
+    one_based = iz * (iz + 1) // 2 + icharge if iz <= 30 else iz * 5 + 341 + icharge
+    return one_based - 1
+
+
+def _code_parts(code: float):
+    nelem = int(code + 1e-6)
+    icharge = int((code - nelem) * 100.0 + 0.1)
+    return nelem, icharge, icharge + 1
+
+
+def stark_default(code: float, e_lower: float, e_upper: float) -> float:
+    """log10 GAMMAS when the record's GS is 0 (atomic.py:59-106)."""
+    if code >= 100.0:
+        return -5.0
+    nelem, icharge, zeff = _code_parts(code)
+    e_up = max(abs(e_lower), abs(e_upper))
+    n_eff_sq = 25.0
+    table = potion()
+    k = potion_index(nelem, icharge)
+    if 0 <= k < len(table):
+        gap = table[k] - e_up
+        if gap > 0:
+            n_eff_sq = 109737.31 * zeff * zeff / gap
+    return math.log10(1.0e-8 * n_eff_sq * n_eff_sq * math.sqrt(n_eff_sq))
+
+
+def vdw_default(code: float, e_lower: float, e_upper: float) -> float:
+    """log10 GAMMAW when the record's GW is 0 (atomic.py:109-199)."""
+    nelem, icharge, zeff = _code_parts(code)
+    if code >= 100.0:
+        return math.log10(1.0e-7 / zeff)
+    e_lo = min(abs(e_lower), abs(e_upper))
+    e_up = max(abs(e_lower), abs(e_upper))
+    table = potion()
+    k = potion_index(nelem, icharge)
+    ip = table[k] if 0 <= k < len(table) else 0.0
+
+    def mean_sq_radius(level: float) -> float:
+        n_eff_sq = 25.0
+        if ip > 0 and ip - level > 0:
+            n_eff_sq = 109737.31 * zeff * zeff / (ip - level)
+        n_eff_sq = min(n_eff_sq, 1000.0)
+        return 2.5 * (n_eff_sq / zeff) ** 2
+
+    r_up, r_lo = mean_sq_radius(e_up), mean_sq_radius(e_lo)
+    nseq = nelem - zeff + 1
+    if 20 < nseq < 29:
+        r_up, r_lo = (45.0 - nseq) / zeff, 0.0
+    if r_up < r_lo:
+        r_up = 2.0 * r_lo
+    diff = r_up - r_lo
+    return math.log10(4.5e-9 * diff ** 0.4 if diff > 0 else 1.0e-9)
+
+
+@dataclass
+class Record:
+    wavelength: float
+    index_wavelength: float
+    element: str
+    ion_stage: int
+    log_gf: float
+    excitation: float
+    gamma_rad: float
+    gamma_stark: float
+    gamma_vdw: float
+    line_type: int = 0
+    n_lower: int = 0
+    n_upper: int = 0
+    code: float = 0.0
+    line_size: int = 0
+
+
+def _int_or(text: str, default: int = 0) -> int:
+    try:
+        return int(text)
+    except ValueError:
+        return default
+
+
+def _fixed_width(line: str) -> Record:
+    """One GFALL record by columns (atomic.py:323-592); ValueError if malformed."""
+    n = len(line)
+    col = lambda a, b: line[a:b].strip()  # noqa: E731
+    wl_stored = float(col(0, 11))
+    log_gf = float(col(11, 18))
+    code = float(col(18, 24))
+    e = float(col(24, 36))
+    ep = float(col(52, 64)) if col(52, 64) else 0.0
+    xj, xjp = col(36, 41), col(64, 69)
+    glog = [float(col(a, a + 6)) if col(a, a + 6) else 0.0 for a in (80, 86, 92)]
+    x1 = x2 = 0.0
+    if n >= 115 and col(109, 115):
+        try:
+            x1 = float(col(109, 115))
+        except ValueError:
+            pass
+    if n >= 124 and col(118, 124):
+        try:
+            x2 = float(col(118, 124))
+        except ValueError:
+            pass
+    if os.environ.get("PY_APPLY_ISO_CORR", "1") != "0":
+        log_gf = log_gf + (x1 + x2)
+    else:
+        log_gf = log_gf + 0.0
+    e_lo, e_hi = min(abs(e), abs(ep)), max(abs(e), abs(ep))
+    g_rad, g_stark, g_vdw = (0.0 if v == 0.0 else 10.0 ** v for v in glog)
+
+    wavelength = wl_stored
+    shift_lo = shift_hi = 0.0
+    hyperfine = line[124:134] if n >= 134 else ""
+    if hyperfine.strip():
+        try:
+            shift_lo, shift_hi = int(hyperfine[0:5]) * 1.0e-3, int(hyperfine[5:10]) * 1.0e-3
+        except ValueError:
+            shift_lo = shift_hi = 0.0
+    gap = abs((abs(ep) + shift_hi) - (abs(e) + shift_lo))
+    if gap > 0.0:
+        iso = col(154, 160) if n >= 160 else ""
+        dwliso = _int_or(iso) * 1.0e-4 if iso else 0.0
+        wavelength = 1.0e7 / gap + dwliso
+
+    if g_rad == 0.0:
+        g_rad = 2.223e13 / (wavelength ** 2)
+    if g_stark == 0.0:
+        v = stark_default(code, e_lo, e_hi)
+        g_stark = 10.0 ** v if v != 0.0 else 0.0
+    if g_vdw == 0.0:
+        v = vdw_default(code, e_lo, e_hi)
+        g_vdw = 10.0 ** v if v != 0.0 else 0.0
+    nu = C_NM / max(wavelength, 1e-12)
+    g_rad, g_stark, g_vdw = (g / (12.5664 * nu) for g in (g_rad, g_stark, g_vdw))
+
+    nelem = int(code + 1e-6)
+    frac = code - nelem
+    stage = int(round(frac * 100.0)) + 1 if frac > 1e-6 else 1
+    element = PARSER_SYMBOLS[nelem] if 0 < nelem < len(PARSER_SYMBOLS) else f"Z{nelem}"
+
+    n_lo = n_up = 0
+    if n >= 106:
+        try:
+            n_lo = int(col(102, 104)) if col(102, 104) else 0
+            n_up = int(col(104, 106)) if col(104, 106) else 0
+        except ValueError:
+            pass
+    iso1 = _int_or(col(106, 109)) if n >= 109 and col(106, 109) else 0
+    size, tag = 0, ""
+    tail = line[134:144] if n >= 144 else ""
+    if tail.strip():
+        tag = tail[7:10].strip()
+        if tail[6:7].strip().isdigit():
+            size = int(tail[6:7])
+    if tag in ("AUT", "COR", "PRD"):
+        ltype = {"AUT": 1, "COR": 2, "PRD": 3}[tag]
+    elif nelem == 1 and stage == 1:
+        ltype = -2 if iso1 == 2 else -1
+    elif nelem == 2 and stage == 1:
+        ltype = -4 if iso1 == 3 else -3
+    elif nelem == 2 and stage == 2:
+        ltype = -6
+    else:
+        ltype = 0
+    # the J values are parsed last; a malformed one sends the record to the
+    # whitespace reader
+    float(xj) if xj else 0.0
+    float(xjp) if xjp else 0.0
+    return Record(wavelength, wavelength, element, stage, log_gf, e_lo, g_rad, g_stark, g_vdw, ltype,
+                  n_lo, n_up, code, size)
+
+
+def _whitespace(line: str) -> Optional[Record]:
+    """Fallback for records that are not fixed-width GFALL (atomic.py:594-672)."""
+    parts = line.split()
+    if len(parts) < 7:
+        return None
+    try:
+        wl = float(parts[0])
+        element = parts[1]
+        try:
+            stage = int(float(parts[2]))
+        except (ValueError, IndexError):
+            return None
+        log_gf, exc = float(parts[3]), float(parts[4])
+        g_rad, g_stark = float(parts[5]), float(parts[6])
+        g_vdw = float(parts[7]) if len(parts) > 7 else 0.0
+        n_lo = n_up = 0
+        if len(parts) > 9:
+            try:
+                n_lo = int(float(parts[8]))
+                n_up = int(float(parts[9]))
+            except ValueError:
+                pass
+        ltype = 0
+        if element == "H" and stage == 1:
+            ltype = -1
+        elif element == "He" and stage == 1:
+            ltype = -3
+        elif element == "He" and stage == 2:
+            ltype = -6
+        return Record(wl, wl, element, stage, log_gf, exc, g_rad, g_stark, g_vdw, ltype, n_lo, n_up, 0.0, 0)
+    except (ValueError, IndexError):
+        return None
+
+
+@dataclass
+class Catalog:
+    """The oracle's line catalog: one entry per kept record, file order."""
+
+    records: List[Record]
+    wavelength: np.ndarray
+    index_wavelength: np.ndarray
+    log_gf: np.ndarray
+    gf: np.ndarray
+    excitation: np.ndarray
+    gamma_rad: np.ndarray
+    gamma_stark: np.ndarray
+    gamma_vdw: np.ndarray
+
+    def __len__(self) -> int:
+        return len(self.records)
+
+    @classmethod
+    def of(cls, records: List[Record]) -> "Catalog":
+        """LineCatalog.from_records (atomic.py:256-296)."""
+        f = lambda name: np.array([getattr(r, name) for r in records], dtype=np.float64)  # noqa: E731
+        idx = np.array([r.index_wavelength if r.index_wavelength > 0.0 else r.wavelength for r in records],
+                       dtype=np.float64)
+        log_gf = f("log_gf")
+        return cls(list(records), f("wavelength"), idx, log_gf, np.power(10.0, log_gf), f("excitation"),
+                   f("gamma_rad"), f("gamma_stark"), f("gamma_vdw"))
+
+
+def read_lines(lines: List[str]) -> Catalog:
+    records = []
+    for line in lines:
+        if line.strip().startswith("#") or not line.strip():
+            continue
+        if len(line) >= 98:
+            try:
+                records.append(_fixed_width(line))
+                continue
+            except (ValueError, IndexError):
+                pass
+        rec = _whitespace(line)
+        if rec is not None:
+            records.append(rec)
+    return Catalog.of(records)
+
+
+def read_gfall(path) -> Catalog:
+    """load_catalog (atomic.py:299-686): ASCII text, undecodable bytes dropped."""
+    with Path(path).open("r", encoding="ascii", errors="ignore") as fh:
+        return read_lines(fh.read().splitlines())
+
+
+def line_margin(rec: Record, wl_min: float) -> float:
+    """DELLIM margin of one record in nm (atomic.py:697-740)."""
+    factor = 1.0 if wl_min <= 500.0 else wl_min / 500.0
+    size = int(rec.line_size) if rec.line_size > 0 else 0
+    lim = min(8 - size, 7)
+    if abs(float(rec.code) - 1.0) < 1.0e-6 or rec.line_type in (-1, -2) or rec.element in {"H", "D"}:
+        lim = 1
+    return DELLIM[lim - 1] * factor  # lim <= 0 wraps to the table's end, as Python indexing does
+
+
+def filter_by_range(cat: Catalog, wl_min: float, wl_max: float) -> Catalog:
+    """Records inside [wl_min - margin, wl_max + margin] (atomic.py:743-816)."""
+    if len(cat) == 0:
+        return cat
+    keep = []
+    for r in cat.records:
+        m = line_margin(r, wl_min)
+        if r.wavelength >= wl_min - m and r.wavelength <= wl_max + m:
+            keep.append(r)
+    return Catalog.of(keep)
+
+
+@dataclass
+class Soa:
+    """Per-line arrays in the layout synthe_oracle.c's TRANSP/ASYNTH read."""
+
+    wl: np.ndarray
+    gf: np.ndarray
+    cgf: np.ndarray
+    gamma_rad: np.ndarray
+    gamma_stark: np.ndarray
+    gamma_vdw: np.ndarray
+    elo: np.ndarray
+    elem: np.ndarray
+    ion: np.ndarray
+    ltype: np.ndarray
+    process: np.ndarray
+    center_t: np.ndarray
+    center_w: np.ndarray
+
+    def __len__(self) -> int:
+        return int(self.wl.size)
+
+
+def line_soa(cat: Catalog, grid: np.ndarray, n_elem: int, grid_origin: Optional[float] = None) -> Soa:
+    """The TRANSP prep loop (physics/line_opacity.py:700-784): which lines the
+    plain-line kernels take, and their arrays (zero for the others)."""
+    from oracle import nearest_grid_indices, wing_center_indices
+
+    L = len(cat)
+    include_h = os.getenv("PY_INCLUDE_H_LINES") == "1"
+    process = np.zeros(L, np.uint8)
+    elem = np.full(L, -1, np.int32)
+    ion = np.zeros(L, np.int32)
+    ltype = np.zeros(L, np.int32)
+    wl, gf, cgf, gr, gs, gw = (np.zeros(L) for _ in range(6))
+    for i, r in enumerate(cat.records):
+        sym = str(r.element).strip().upper()
+        lt = int(r.line_type or 0)
+        if not include_h and (lt == -1 or (sym in {"H", "HI", "H I"} and r.ion_stage == 1)):
+            continue
+        if not (lt != 2 and lt != 1 and lt <= 3 and abs(int(r.n_lower or 0)) + abs(int(r.n_upper or 0)) == 0):
+            continue
+        z = _ENGINE_Z.get(str(r.element).strip().upper().replace(" ", ""))
+        if z is None or z - 1 >= n_elem:
+            continue
+        process[i] = 1
+        elem[i] = z - 1
+        wl[i] = float(cat.index_wavelength[i])
+        gf[i] = float(cat.gf[i])
+        ion[i] = int(r.ion_stage)
+        ltype[i] = lt
+        gr[i], gs[i], gw[i] = (float(a[i]) for a in (cat.gamma_rad, cat.gamma_stark, cat.gamma_vdw))
+        cgf[i] = CGF_FACTOR * gf[i] / (C_NM / wl[i])
+    idx = np.ascontiguousarray(cat.index_wavelength, np.float64)
+    return Soa(wl, gf, cgf, gr, gs, gw, np.ascontiguousarray(cat.excitation, np.float64), elem, ion, ltype,
+               process, nearest_grid_indices(grid, idx),
+               wing_center_indices(grid, idx, grid_origin if grid_origin else float(grid[0])))

@@ -240,6 +240,41 @@ def freq_quantities(grid, temp):
     return e, s, b, sl
 
 
+def hckt(atm) -> np.ndarray:
+    """HCKT per depth: the bundle's, else 1.4388/max(T, 1) (physics/populations.py:162-171)."""
+    t = np.maximum(np.asarray(atm.temperature, np.float64), 1.0)
+    out = 1.4388 / t
+    h = getattr(atm, "hckt", None)
+    if h is not None:
+        h = np.asarray(h, np.float64)
+        out[: min(h.size, t.size)] = h[: min(h.size, t.size)]
+    return out
+
+
+def txnxn(atm) -> np.ndarray:
+    """TXNXN per depth: the bundle's, else (xnf_h + .42 xnf_he1 + .85 xnf_h2)(T/1e4)^.3
+    with T = max(T, 1) (physics/populations.py:190-199)."""
+    tx = getattr(atm, "txnxn", None)
+    if tx is not None:
+        return np.asarray(tx, np.float64)
+    out = []
+    for i, t in enumerate(np.asarray(atm.temperature, np.float64).tolist()):
+        col = [getattr(atm, n, None) for n in ("xnf_h", "xnf_he1", "xnf_h2")]
+        h, he, h2 = (float(c[i]) if c is not None else 0.0 for c in col)
+        out.append((h + 0.42 * he + 0.85 * h2) * (max(t, 1.0) / 10_000.0) ** 0.3)
+    return np.array(out, np.float64)
+
+
+def _rho(atm):
+    v = getattr(atm, "mass_density", None)  # ones when absent (physics/line_opacity.py:794)
+    return np.ones(atm.temperature.size) if v is None else _c(v, np.float64)
+
+
+def _xne(atm):
+    v = getattr(atm, "electron_density", None)  # zeros when absent (physics/line_opacity.py:798)
+    return np.zeros(atm.temperature.size) if v is None else _c(v, np.float64)
+
+
 def transp(soa, atm, grid, kapmin, cutoff=1e-3, microturb_kms=0.0):
     t = tables()
     L, D, W = len(soa), atm.layers, grid.size
@@ -251,9 +286,9 @@ def transp(soa, atm, grid, kapmin, cutoff=1e-3, microturb_kms=0.0):
         ct.c_long(L), ct.c_long(D), ct.c_long(W), p(soa.process, _U8), p(soa.elem, _I32),
         p(soa.ion, _I32), p(soa.ltype, _I32), p(soa.wl), p(soa.gf), p(soa.cgf),
         p(soa.gamma_rad), p(soa.gamma_stark), p(soa.gamma_vdw), p(soa.elo), p(soa.center_t, _I64),
-        p(_c(atm.hckt_or_default(), np.float64)), p(pop), p(dop), ct.c_long(pop.shape[1]),
-        ct.c_long(pop.shape[2]), p(_c(atm.mass_density, np.float64)),
-        p(_c(atm.electron_density, np.float64)), p(_c(atm.txnxn_or_default(), np.float64)),
+        p(hckt(atm)), p(pop), p(dop), ct.c_long(pop.shape[1]),
+        ct.c_long(pop.shape[2]), p(_rho(atm)),
+        p(_xne(atm)), p(txnxn(atm)),
         p(_c(kapmin, np.float64)), ct.c_double(cutoff), ct.c_double(microturb_kms),
         ct.byref(t.struct), p(out), p(valid, _U8))
     return out, valid.astype(bool)
@@ -271,8 +306,8 @@ def asynth(soa, atm, grid, transp_arr, valid, kapmin, cutoff=1e-3):
         p(soa.elem, _I32), p(soa.ion, _I32), p(soa.wl), p(soa.gamma_rad), p(soa.gamma_stark),
         p(soa.gamma_vdw), p(soa.center_t, _I64), p(soa.center_w, _I64),
         p(_c(transp_arr, np.float64)), p(_c(valid, np.uint8), _U8), p(pop), p(dop),
-        ct.c_long(pop.shape[1]), ct.c_long(pop.shape[2]), p(_c(atm.electron_density, np.float64)),
-        p(_c(atm.txnxn_or_default(), np.float64)), p(_c(atm.temperature, np.float64)),
+        ct.c_long(pop.shape[1]), ct.c_long(pop.shape[2]), p(_xne(atm)),
+        p(txnxn(atm)), p(_c(atm.temperature, np.float64)),
         p(_c(kapmin, np.float64)), ct.c_double(cutoff), ct.byref(t.struct), p(out), p(stats))
     return out, dict(e_line=stats[0], p_line=stats[1], wing_pairs=stats[2], sum_maxstep=stats[3],
                      max_maxstep=stats[4])
@@ -320,18 +355,28 @@ class OracleResult:
     timings: Dict[str, float] = field(default_factory=dict)
 
 
-def synthesize(atm, catalog, wl_start, wl_end, resolution, cutoff=1e-3, microturb_kms=0.0,
+def synthesize(atm, lines, wl_start, wl_end, resolution, cutoff=1e-3, microturb_kms=0.0,
                rhoxj=0.0, keep_stages=True, threads: Optional[int] = None) -> OracleResult:
-    """Full hot path on the CPU (engine/opacity.py:2084-3861 stage order)."""
-    import time
-    from kurucz_py_joss_submission_amd import lines as L_
+    """Full hot path on the CPU (engine/opacity.py:2084-3861 stage order).
 
+    lines: the oracle's own catalog (catalog.Catalog, from catalog.read_gfall)
+    or the path of a GFALL file, which is read and DELLIM-filtered to the
+    window here.  The line arrays come from the oracle's restatement of the
+    reference's prep (catalog.line_soa), not from the package under test."""
+    import time
+
+    import catalog as C_
+
+    if isinstance(lines, (str, os.PathLike)):
+        lines = C_.filter_by_range(C_.read_gfall(lines), wl_start, wl_end)
+    if not isinstance(lines, C_.Catalog):
+        raise TypeError("oracle.synthesize takes the oracle's own catalog (catalog.read_gfall) or a GFALL path")
     if threads:
         set_threads(threads)
     tm = {}
     t0 = time.perf_counter()
     grid = wavelength_grid(wl_start, wl_end, resolution)
-    soa = L_.build_line_soa(catalog, grid, atm.population_per_ion.shape[2], grid_origin=float(grid[0]))
+    soa = C_.line_soa(lines, grid, atm.population_per_ion.shape[2], grid_origin=float(grid[0]))
     tm["prep"] = time.perf_counter() - t0
     t0 = time.perf_counter()
     ca, cs = continuum(atm, grid)
@@ -360,7 +405,8 @@ def synthesize(atm, catalog, wl_start, wl_end, resolution, cutoff=1e-3, microtur
     stages = {}
     if keep_stages:
         stages = dict(cont_abs=ca, cont_scat=cs, transp=tr, valid=valid, asynth=asy, slinec=slinec,
-                      center_t=soa.center_t, center_w=soa.center_w, iters=lte_spectrum.last_iters)
+                      center_t=soa.center_t, center_w=soa.center_w, process=soa.process,
+                      iters=lte_spectrum.last_iters)
     return OracleResult(grid, flux, cont, ft, fc, stages, st, tm)
 
 
@@ -380,6 +426,6 @@ def wing_params(soa, atm, grid, transp_arr, valid, kapmin, cutoff=1e-3):
         p(soa.elem, _I32), p(soa.ion, _I32), p(soa.wl), p(soa.gamma_rad), p(soa.gamma_stark),
         p(soa.gamma_vdw), p(soa.center_w, _I64), p(_c(transp_arr, np.float64)),
         p(_c(valid, np.uint8), _U8), p(pop), p(dop), ct.c_long(pop.shape[1]), ct.c_long(pop.shape[2]),
-        p(_c(atm.electron_density, np.float64)), p(_c(atm.txnxn_or_default(), np.float64)),
+        p(_xne(atm)), p(txnxn(atm)),
         p(_c(kapmin, np.float64)), ct.c_double(cutoff), ct.byref(t.struct), p(od), p(oi, _I64))
     return od, oi

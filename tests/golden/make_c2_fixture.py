@@ -25,13 +25,17 @@ STRIDE = 97
 
 
 def main() -> None:
+    import tempfile
+
     import oracle
     from kurucz_py_joss_submission_amd import synthetic
 
     oracle.build()
     oracle.set_threads(os.cpu_count() or 1)
     wk = synthetic.make_workload("c2", 4000.0, 5.0, 1_000_000, 300.0, 1800.0, seed=1)
-    r = oracle.synthesize(wk.atmosphere, wk.catalog, 300.0, 1800.0, 300000.0, keep_stages=True)
+    with tempfile.TemporaryDirectory() as td:  # the oracle parses the GFALL text itself
+        gfall = synthetic.write_workload_gfall(wk, Path(td) / "lines.gfall")
+        r = oracle.synthesize(wk.atmosphere, gfall, 300.0, 1800.0, 300000.0, keep_stages=True)
     idx = np.arange(0, r.wavelength.size, STRIDE)
     np.savez_compressed(
         Path(__file__).with_name("c2_oracle_sample.npz"),

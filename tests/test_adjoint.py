@@ -19,6 +19,7 @@ WL0, WL1 = 400.0, 400.6
 
 @pytest.fixture(scope="module")
 def adj_case():
+    import catalog as C
     from kurucz_py_joss_submission_amd import lines as L
     from kurucz_py_joss_submission_amd import synthetic as S
 
@@ -28,7 +29,9 @@ def adj_case():
         gf = Path(td) / "lines.gfall"
         S.write_line_list(gf, d)
         cat = L.filter_by_range(L.load_catalog(gf), WL0, WL1)
-    return cols, cat
+        ocat = C.filter_by_range(C.read_gfall(gf), WL0, WL1)  # the oracle's own parse
+    assert len(cat) == len(ocat) and np.array_equal(cat.wavelength, ocat.wavelength)
+    return cols, cat, ocat
 
 
 def _loss_weights(W, seed=4):
@@ -87,7 +90,7 @@ def test_adjoint_forward_matches_spectrum_plan(gpu, adj_case):
     from kurucz_py_joss_submission_amd import synthetic as S
     from kurucz_py_joss_submission_amd.device import SpectrumPlan
 
-    cols, cat = adj_case
+    cols, cat, ocat = adj_case
     _, flux, _ = _model_grads(cols, cat, "fp64")
     plan = SpectrumPlan(S.build_bundle(cols), cat, WL0, WL1, 300000.0)
     plan.run()
@@ -105,10 +108,10 @@ def test_adjoint_forward_matches_spectrum_plan(gpu, adj_case):
 @pytest.mark.gpu
 @pytest.mark.parametrize("cutoff", [0.0, 1e-3])
 def test_adjoint_against_fp64_reference_gradient(gpu, oracle, adj_case, cutoff):
-    cols, cat = adj_case
+    cols, cat, ocat = adj_case
     m, flux, g = _model_grads(cols, cat, "fp64", cutoff)
     w = _loss_weights(m.W)
-    loss = lambda c=cols, k=cat, a=None: _oracle_loss(oracle, c, k, w, a, cutoff)  # noqa: E731
+    loss = lambda c=cols, k=ocat, a=None: _oracle_loss(oracle, c, k, w, a, cutoff)  # noqa: E731
     oracle.set_josh_fp64(True, 1e-13)
     try:
         checks = []
@@ -116,8 +119,8 @@ def test_adjoint_against_fp64_reference_gradient(gpu, oracle, adj_case, cutoff):
         for key, field, n in (("gf", "gf", 4), ("gw", "gamma_vdw", 3)):
             for l in np.argsort(-np.abs(g[key]))[:n]:
                 h = 1e-4
-                up = loss(k=_cat_scaled(cat, field, l, 10 ** h))
-                dn = loss(k=_cat_scaled(cat, field, l, 10 ** -h))
+                up = loss(k=_cat_scaled(ocat, field, l, 10 ** h))
+                dn = loss(k=_cat_scaled(ocat, field, l, 10 ** -h))
                 checks.append((f"{key}[{l}]", g[key][l], (up - dn) / (2 * h), np.max(np.abs(g[key]))))
         from kurucz_py_joss_submission_amd import synthetic as S
 
@@ -138,7 +141,7 @@ def test_adjoint_against_fp64_reference_gradient(gpu, oracle, adj_case, cutoff):
 
 @pytest.mark.gpu
 def test_fp32_profile_gradient_against_fp64(gpu, adj_case):
-    cols, cat = adj_case
+    cols, cat, ocat = adj_case
     _, f64, g64 = _model_grads(cols, cat, "fp64")
     _, f32, g32 = _model_grads(cols, cat, "fp32")
     assert np.max(np.abs(f32 - f64) / np.abs(f64)) < 1e-4
@@ -179,13 +182,15 @@ def test_temperature_gradient_continuum_and_planck(gpu, oracle, adj_case):
     Checked at the surface and at the layers where the continuum forms, with
     a 1e-4 step (the shallow layers' sensitivities are small and dominated by
     the kinks; the deep layers' continuum, ~T^6, is too nonlinear for 2e-2)."""
-    cols, cat = adj_case
+    import catalog as C
+
+    cols, cat, ocat = adj_case
     empty = cat.subset(np.array([], dtype=np.int64))
     m, flux, g = _model_grads(cols, empty, "fp64")
     w = _loss_weights(m.W)
     oracle.set_josh_fp64(True, 1e-13)
     try:
-        _t_checks(oracle, cols, empty, w, g, (0, 49, 56, 63), 1e-4, 0.05)
+        _t_checks(oracle, cols, C.Catalog.of([]), w, g, (0, 49, 56, 63), 1e-4, 0.05)
     finally:
         oracle.set_josh_fp64(False)
 
@@ -194,12 +199,12 @@ def test_temperature_gradient_continuum_and_planck(gpu, oracle, adj_case):
 def test_temperature_gradient_with_lines(gpu, oracle, adj_case):
     """All T dependences: populations, Doppler widths, HCKT, TXNXN, STIM,
     continuum and Planck."""
-    cols, cat = adj_case
+    cols, cat, ocat = adj_case
     m, flux, g = _model_grads(cols, cat, "fp64")
     w = _loss_weights(m.W)
     oracle.set_josh_fp64(True, 1e-13)
     try:
-        _t_checks(oracle, cols, cat, w, g, (5, 10, 15, 20, 25, 30, 35, 40), 2e-2, 0.05)
+        _t_checks(oracle, cols, ocat, w, g, (5, 10, 15, 20, 25, 30, 35, 40), 2e-2, 0.05)
     finally:
         oracle.set_josh_fp64(False)
 
@@ -216,7 +221,7 @@ def test_adjoint_with_line_scattering(gpu, oracle, adj_case):
     from kurucz_py_joss_submission_amd import synthetic as S
     from kurucz_py_joss_submission_amd.device import SpectrumPlan
 
-    cols, cat = adj_case
+    cols, cat, ocat = adj_case
     rhoxj = 0.1  # scattering above rhox ~ 0.1 g cm^-2, absorption below
     m, flux, g = _model_grads(cols, cat, "fp64", 0.0, rhoxj)
     plan = SpectrumPlan(S.build_bundle(cols), cat, WL0, WL1, 300000.0, cutoff=0.0, rhoxj=rhoxj)
@@ -227,14 +232,14 @@ def test_adjoint_with_line_scattering(gpu, oracle, adj_case):
     assert np.mean(rel > 1e-10) < 0.02 and np.max(rel) < 1e-7
     _, _, g0 = _model_grads(cols, cat, "fp64", 0.0, 0.0)
     w = _loss_weights(m.W)
-    loss = lambda k=cat, a=None: _oracle_loss(oracle, cols, k, w, a, 0.0, rhoxj)  # noqa: E731
+    loss = lambda k=ocat, a=None: _oracle_loss(oracle, cols, k, w, a, 0.0, rhoxj)  # noqa: E731
     oracle.set_josh_fp64(True, 1e-13)
     try:
         checks = []
         for l in np.argsort(-np.abs(g["gf"]))[:3]:
             h = 1e-4
-            up = loss(k=_cat_scaled(cat, "gf", l, 10 ** h))
-            dn = loss(k=_cat_scaled(cat, "gf", l, 10 ** -h))
+            up = loss(k=_cat_scaled(ocat, "gf", l, 10 ** h))
+            dn = loss(k=_cat_scaled(ocat, "gf", l, 10 ** -h))
             checks.append((f"gf[{l}]", g["gf"][l], (up - dn) / (2 * h), g0["gf"][l]))
         iz = list(S.ABUNDANCE_Z).index(26)
         up, dn = loss(a={26: 1e-3}), loss(a={26: -1e-3})

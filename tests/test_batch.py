@@ -16,8 +16,15 @@ INSTANCES = [  # Teff, log g, layers, [M/H], window (nm)
 ]
 
 
+class _Inputs(list):
+    """(atmosphere, catalog, wl0, wl1) per instance, plus the oracle's catalog."""
+
+    oracle_catalog = None
+
+
 @pytest.fixture(scope="module")
 def batch_inputs():
+    import catalog as C
     from kurucz_py_joss_submission_amd import lines as L
     from kurucz_py_joss_submission_amd import synthetic as S
 
@@ -29,7 +36,9 @@ def batch_inputs():
         gf = Path(td) / "lines.gfall"
         S.write_line_list(gf, d)
         cat = L.load_catalog(gf)
-    out = []
+        ocat = C.read_gfall(gf)  # the oracle's own parse of the same text
+    out = _Inputs()
+    out.oracle_catalog = ocat
     for teff, logg, nl, mh, w0, w1 in INSTANCES:
         atm = S.build_bundle(S.make_atm(teff, logg, nl, mh))
         out.append((atm, cat, w0, w1))
@@ -100,7 +109,9 @@ def test_batch_first_instance_against_oracle(gpu, oracle, batch_inputs):
     bp.run()
     torch.cuda.synchronize()
     r = bp.results()[0]
-    ref = oracle.synthesize(atm, L.filter_by_range(cat, w0, w1), w0, w1, 300000.0)
+    import catalog as C
+
+    ref = oracle.synthesize(atm, C.filter_by_range(batch_inputs.oracle_catalog, w0, w1), w0, w1, 300000.0)
     floor = 1e-3 * np.median(ref.continuum)
     assert np.max(np.abs(r["continuum"] - ref.continuum) / np.abs(ref.continuum)) < 1e-6
     assert np.max(np.abs(r["flux"] - ref.flux) / np.maximum(np.abs(ref.flux), floor)) < 1e-6

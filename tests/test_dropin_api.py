@@ -10,6 +10,8 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
+from conftest import oracle_catalog
+
 
 def _rel(a, b, floor=1e-300):
     return float(np.max(np.abs(np.asarray(a) - np.asarray(b)) / np.maximum(np.abs(np.asarray(b)), floor)))
@@ -36,7 +38,9 @@ def test_physics_dropins_match_oracle(gpu, oracle, small_workload):
     assert _rel(ca, oca) < 1e-14 and _rel(cs, ocs) < 1e-14
     km = oca + ocs
     tr, va, ci = physics.compute_transp(wk.catalog, None, atm, 1e-3, km, grid)
-    soa = L.build_line_soa(wk.catalog, grid, atm.population_per_ion.shape[2], grid_origin=float(grid[0]))
+    import catalog as C
+
+    soa = C.line_soa(oracle_catalog(wk), grid, atm.population_per_ion.shape[2], grid_origin=float(grid[0]))
     otr, ova = oracle.transp(soa, atm, grid, km)
     assert np.array_equal(va, ova) and _rel(tr, otr) < 1e-14
     assert np.array_equal(ci, oracle.nearest_grid_indices(grid, soa.wl))
@@ -82,8 +86,7 @@ def test_run_synthesis_cli(gpu, oracle, small_workload, tmp_path):
     rows = np.loadtxt(spec)
     from kurucz_py_joss_submission_amd import lines as L
 
-    cat = L.filter_by_range(L.load_catalog(lines_path), 500.0, 501.0)
-    ref = oracle.synthesize(wk.atmosphere, cat, 500.0, 501.0, 300000.0)
+    ref = oracle.synthesize(wk.atmosphere, lines_path, 500.0, 501.0, 300000.0)  # the oracle parses the file
     assert rows.shape == (ref.wavelength.size, 3)
     np.testing.assert_allclose(rows[:, 0], ref.wavelength, rtol=1e-9)
     np.testing.assert_allclose(rows[:, 1], ref.flux, rtol=1e-6)  # 7 significant digits in .spec

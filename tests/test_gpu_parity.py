@@ -12,6 +12,8 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
+from conftest import oracle_catalog
+
 pytestmark = pytest.mark.gpu
 
 FLUX_RTOL = 1e-6
@@ -53,10 +55,13 @@ def test_stage_parity_small(gpu, oracle, small_workload):
     import torch
 
     torch.cuda.synchronize()
-    ref = oracle.synthesize(wk.atmosphere, wk.catalog, wk.wl_start, wk.wl_end, wk.resolution)
+    ref = oracle.synthesize(wk.atmosphere, oracle_catalog(wk), wk.wl_start, wk.wl_end, wk.resolution)
     st = ref.stages
     assert np.array_equal(plan.grid, ref.wavelength)
+    # the line arrays: the package's parse and prep against the oracle's own
+    assert np.array_equal(plan.soa.process, st["process"])
     assert np.array_equal(plan.soa.center_t, st["center_t"])
+    assert np.array_equal(plan.soa.center_w, st["center_w"])
     assert _rel(plan.cont_abs.cpu().numpy(), st["cont_abs"]) < 1e-14
     assert _rel(plan.cont_scat.cpu().numpy(), st["cont_scat"]) < 1e-14
     v = plan.valid.cpu().numpy().astype(bool)
@@ -80,7 +85,7 @@ def test_flux_parity_config1(gpu, oracle, c1_workload):
     wk = c1_workload
     plan = _plan(wk)
     plan.run(stats=True)
-    ref = oracle.synthesize(wk.atmosphere, wk.catalog, wk.wl_start, wk.wl_end, wk.resolution,
+    ref = oracle.synthesize(wk.atmosphere, oracle_catalog(wk), wk.wl_start, wk.wl_end, wk.resolution,
                             keep_stages=True)
     assert plan.W == 5941
     flux = plan.flux.cpu().numpy()
@@ -234,7 +239,7 @@ def test_fp32_profiles_flux_error(gpu, oracle, c1_workload):
     wk = c1_workload
     plan = _plan(wk, precision="fp32")
     plan.run(stats=True)
-    ref = oracle.synthesize(wk.atmosphere, wk.catalog, wk.wl_start, wk.wl_end, wk.resolution)
+    ref = oracle.synthesize(wk.atmosphere, oracle_catalog(wk), wk.wl_start, wk.wl_end, wk.resolution)
     flux, cont = plan.flux.cpu().numpy(), plan.cont.cpu().numpy()
     assert np.isfinite(plan.asynth.cpu().numpy()).all()
     assert plan.stats.cpu().numpy()[0] == ref.stats["e_line"]
@@ -256,7 +261,7 @@ def test_flux_parity_regimes(gpu, oracle, teff, logg, nl, mh, wl0):
     wk = S.make_workload(f"regime{int(teff)}", teff, logg, 3000, wl0, wl0 + 3.0, seed=11, n_layers=nl, mh=mh)
     plan = _plan(wk)
     plan.run()
-    ref = oracle.synthesize(wk.atmosphere, wk.catalog, wk.wl_start, wk.wl_end, wk.resolution, keep_stages=True)
+    ref = oracle.synthesize(wk.atmosphere, oracle_catalog(wk), wk.wl_start, wk.wl_end, wk.resolution, keep_stages=True)
     flux = plan.flux.cpu().numpy()
     cont = plan.cont.cpu().numpy()
     assert _rel(cont, ref.continuum) < FLUX_RTOL

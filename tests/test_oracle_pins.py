@@ -272,19 +272,20 @@ def test_single_line_transp_and_wings(oracle, small_workload):
     """One line at one depth, by hand: kappa0 = cgf N/U / (rho dop) * FASTEX(E hckt),
     KAPCEN = kappa0 (1 - 1.128 a) for a < 0.2; wings symmetric, far wing X / k^2,
     MAXSTEP = int(sqrt(X / KAPMIN) + 1) (physics/line_opacity.py:123-184, 410-617)."""
-    from kurucz_py_joss_submission_amd import lines as L
+    import catalog as C
 
     wk = small_workload
     atm = wk.atmosphere
     g = oracle.wavelength_grid(500.0, 500.4, 3e5)
-    cat = wk.catalog
-    # strongest low-excitation Fe I line in the list, placed mid-grid
-    fe1 = np.nonzero((cat.atomic_number == 26) & (cat.ion_stages == 1))[0]
-    i = fe1[np.argmax(cat.log_gf[fe1] - 0.5 * cat.excitation_energy[fe1] / 8065.5)]
-    one = cat.subset(np.array([i]))
-    one.wavelength[:] = g[120] * (1 + 1e-8)
-    one.index_wavelength[:] = one.wavelength
-    soa = L.build_line_soa(one, g, 99, grid_origin=float(g[0]))
+    # a strong low-excitation Fe I line placed mid-grid (gamma values already
+    # normalised by 12.5664 nu, as the parser stores them)
+    lam = g[120] * (1 + 1e-8)
+    nu = 2.99792458e17 / lam
+    rec = C.Record(lam, lam, "Fe", 1, 0.5, 1000.0, 10 ** 8.0 / (12.5664 * nu), 10 ** -5.5 / (12.5664 * nu),
+                   10 ** -7.5 / (12.5664 * nu), code=26.0)
+    soa = C.line_soa(C.Catalog.of([rec]), g, 99, grid_origin=float(g[0]))
+    assert soa.process[0] == 1 and soa.elem[0] == 25 and soa.ion[0] == 1
+    assert soa.cgf[0] == 0.026538 / 1.77245 * 10 ** 0.5 / nu
     ca, cs = oracle.continuum(atm, g)
     km = ca + cs
     tr, va = oracle.transp(soa, atm, g, km)
@@ -296,7 +297,7 @@ def test_single_line_transp_and_wings(oracle, small_workload):
     k0 = soa.cgf[0] * (pv / (atm.mass_density[d] * dv)) * oracle.fastex(x)
     assert abs(oracle.fastex(x) / math.exp(-x) - 1) < 5.1e-4
     a = (soa.gamma_rad[0] + soa.gamma_stark[0] * atm.electron_density[d]
-         + soa.gamma_vdw[0] * atm.txnxn_or_default()[d]) / dv
+         + soa.gamma_vdw[0] * oracle.txnxn(atm)[d]) / dv
     assert a < 0.2
     assert abs(tr[0, d] / (k0 * (1 - 1.128 * a)) - 1) < 1e-14
     od, oi = oracle.wing_params(soa, atm, g, tr, va, km)
