@@ -26,6 +26,7 @@
 
 #include "_tables_gen.h"
 #include "rt_adjoint.h"
+#include "near_queue.h"
 #include "rt_core.h"
 #include "synthe_device.h"
 #include "synthe_hip.h"
@@ -420,6 +421,10 @@ __global__ void k_transp_wing(LinesDev ln, DepthsDev dp, long W, const double* _
       if (dw > 0.0) {
         double dopple = lw > 0.0 ? dw / lw : 1e-10;
         long n10dop = (long)(10.0 * dopple * resolu);
+        // MAXPROF bounds every wing; the Voigt zone past it is never added
+        // unless the reference's near-zone cut falls beyond it, which needs a
+        // grid finer than R ~ 10^5 / (v/c) (the Python layer rejects such grids)
+        if (n10dop > kMaxProfileSteps) n10dop = kMaxProfileSteps;
         double dvoigt = dopple > 0 ? 1.0 / (dopple * resolu) : 1.0;
         double tabstep = 200.0 * dvoigt;
         long cut = -1;
@@ -600,7 +605,8 @@ __global__ __launch_bounds__(kBlock) void k_class_scatter(
       int64_t pos = regions[d * (kClasses + 1) + k] + offsets[((long)blockIdx.x * D + d) * kClasses + k] +
                     before + rank;
       const WingParams p = wp[l * D + d];
-      const int r = p.use_far ? p.n10dop : p.maxstep;
+      // the zone ends at MAXSTEP when MAXPROF caps a far wing below N10DOP
+      const int r = p.use_far ? (p.n10dop < p.maxstep ? p.n10dop : p.maxstep) : p.maxstep;
       const bool chunk = !(p.x_far > 0.0 && p.x_far < kChunkMinX);  // X >= 0
       WingHead h;
       h.lo = p.use_far ? p.c - p.maxstep : INT32_MAX;
@@ -696,7 +702,8 @@ __device__ __forceinline__ long class_window_search(int lane, int d, long lo_px,
 #endif
 constexpr int kNearSlots = SYNTHE_NEAR_SLOTS;  // queued values per pixel
 constexpr int kNearQueue = kNearSlots * 64;    // pairs per wave between flushes
-static_assert(kNearSlots >= 1 && kNearSlots <= 16, "a queue record holds the slot in 4 bits");
+static_assert(kNearSlots >= 1 && kNearSlots <= 16, "slots per pixel: 1..16");
+static_assert(nq::kMaxZone == kMaxProfileSteps, "queue records hold offsets up to MAXPROF");
 
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -704,10 +711,9 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// A queued pair: bits 0-31 the entry's index within its depth's lists, 32-51
-// |k| (<= MAXSTEP <= 10^6 < 2^20), 52-57 the pixel's lane, 58-61 its slot.
+// A queued pair: its entry, |k|, the pixel's lane and slot (near_queue.h).
 __device__ __forceinline__ uint64_t near_record(long j_rel, long ka, int lane, int slot) {
-  return (uint64_t)(uint32_t)j_rel | ((uint64_t)ka << 32) | ((uint64_t)lane << 52) | ((uint64_t)slot << 58);
+  return nq::pack(j_rel, ka, lane, slot);
 }
 
 // Evaluates the qn queued pairs into vals (64 at a time), then adds each
@@ -717,12 +723,8 @@ __device__ __forceinline__ void near_flush(int lane, int qn, int& cnt, double& a
                                            const DeviceTables& tb) {
   wave_lds_sync();
   for (int q = lane; q < qn; q += 64) {
-    const uint64_t r = queue[q];
-    const long j_rel = (long)(r & 0xffffffffull);
-    const long ka = (long)((r >> 32) & 0xfffffull);
-    const int pl = (int)((r >> 52) & 63);
-    const int sl = (int)((r >> 58) & 15);
-    vals[sl][pl] = near_wing(list_p[j_rel], ka, tb.h0, tb.h1, tb.h2);
+    const nq::Record r = nq::unpack(queue[q]);
+    vals[r.tag][r.lane] = near_wing(list_p[r.j_rel], r.ka, tb.h0, tb.h1, tb.h2);
   }
   wave_lds_sync();
   for (int sl = 0; sl < cnt; ++sl) acc += vals[sl][lane];
@@ -1619,6 +1621,8 @@ static int check_lines_depths(const synthe_lines* ln, const synthe_depths* dp, c
   if (!ln || !dp) return fail(SYNTHE_EINVAL, std::string(who) + ": lines/depths is NULL");
   if (ln->n_lines < 0 || dp->n_depths <= 0 || dp->n_depths > 65535 || dp->n_ion <= 0 || dp->n_elem <= 0)
     return fail(SYNTHE_EINVAL, std::string(who) + ": bad sizes");
+  if (ln->n_lines > 0x7fffffffL)  // list entries of a depth are indexed in 32 bits
+    return fail(SYNTHE_ERANGE, std::string(who) + ": more than 2^31 lines");
   if (ln->n_lines > 0 &&
       (!ln->wl || !ln->gf || !ln->cgf || !ln->gamma_rad || !ln->gamma_stark || !ln->gamma_vdw ||
        !ln->elo || !ln->elem || !ln->ion || !ln->ltype || !ln->process || !ln->center_t ||
@@ -1743,7 +1747,7 @@ constexpr int kNearAdjQueue = 256;  // pairs per wave between flushes
 constexpr int kNearAdjRuns = 64;    // entries per flush (6 bits)
 
 __device__ __forceinline__ uint64_t near_adj_record(long j_rel, long ka, int lane, int run) {
-  return (uint64_t)(uint32_t)j_rel | ((uint64_t)ka << 32) | ((uint64_t)lane << 52) | ((uint64_t)run << 58);
+  return nq::pack(j_rel, ka, lane, run);
 }
 
 // Per entry (line, depth): dL/dkappa0, dL/da and dL/d(Doppler width) summed
@@ -1787,11 +1791,11 @@ __global__ __launch_bounds__(256) void k_near_adj(long W, int D, const WingHead*
     if (lane < nr) run[lane][0] = run[lane][1] = run[lane][2] = 0.0;
     wave_lds_sync();
     for (int q = lane; q < qn; q += 64) {
-      const uint64_t r = queue[q];
-      const long j_rel = (long)(r & 0xffffffffull);
-      const long ka = (long)((r >> 32) & 0xfffffull);
-      const int pl = (int)((r >> 52) & 63);
-      const int rr = (int)(r >> 58);
+      const nq::Record rec = nq::unpack(queue[q]);
+      const long j_rel = rec.j_rel;
+      const long ka = rec.ka;
+      const int pl = rec.lane;
+      const int rr = rec.tag;
       const WingParams pr = lp[j_rel];
       double H, dHa, dHx;
       near_wing_partials(pr, ka, tb, &H, &dHa, &dHx);

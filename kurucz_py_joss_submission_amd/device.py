@@ -30,6 +30,24 @@ def to_device(a: np.ndarray, dtype=None):
     return torch.from_numpy(arr).to("cuda", non_blocking=False)
 
 
+def check_wing_reach(grid: np.ndarray, doppler_per_ion) -> None:
+    """Reject a grid so fine that a Voigt zone, N10DOP = 10 (v/c) R
+    (physics/line_opacity.py:419-420), could pass MAXPROF = 10^6 pixels: the
+    kernels bound every zone there, where the reference's near-zone loop
+    would go on."""
+    from .constants import MAX_PROFILE_STEPS
+
+    g = np.asarray(grid, np.float64)
+    if g.size < 2 or not g[1] / g[0] > 1.0 or doppler_per_ion is None:
+        return
+    resolu = 1.0 / (g[1] / g[0] - 1.0)
+    dop = np.asarray(doppler_per_ion, np.float64)
+    worst = 10.0 * float(np.max(dop, initial=0.0)) * resolu
+    if worst > MAX_PROFILE_STEPS:
+        raise ValueError(f"grid resolving power {resolu:.4g} with Doppler widths up to {np.max(dop):.3g} c "
+                         f"gives Voigt zones of {worst:.4g} pixels, above MAXPROF = {MAX_PROFILE_STEPS}")
+
+
 def center_csr(center_t: np.ndarray, n_wl: int):
     """Lines stably sorted by centre index and per-pixel offsets (host prep)."""
     order = np.argsort(center_t, kind="stable").astype(np.int64)
@@ -157,6 +175,7 @@ class SpectrumPlan:
         self.grid = N.wavelength_grid(wl_start, wl_end, resolution) if grid is None else np.asarray(grid)
         self.W = int(self.grid.size)
         self.D = atm.layers
+        check_wing_reach(self.grid, atm.doppler_per_ion)
         self.cutoff = float(cutoff)
         self.microturb = float(microturb_kms)
         self.rhoxj = float(rhoxj)
@@ -304,6 +323,7 @@ class BatchPlan:
         arr = (N.Instance * len(instances))()
         for i, (atm, catalog, wl0, wl1) in enumerate(instances):
             grid = N.wavelength_grid(wl0, wl1, resolution)
+            check_wing_reach(grid, atm.doppler_per_ion)
             depth = upload_atmosphere(atm) if depth_arrays is None else depth_arrays[i]
             cat = filter_by_range(catalog, wl0, wl1)
             soa = build_line_soa(cat, grid, depth.n_elem, grid_origin=float(grid[0]) if grid.size else None)

@@ -114,11 +114,12 @@ def compute_asynth_from_transp(transp: np.ndarray, catalog: LineCatalog, atmosph
         raise NotImplementedError("full-grid KAPMIN (continuum_absorption_full) is not on the hot path")
     if metal_tables is not None:
         raise NotImplementedError("WCON/WTAIL merged-continuum tapers (fort.19 lines) are out of scope")
-    from .device import upload_atmosphere, upload_lines
+    from .device import check_wing_reach, upload_atmosphere, upload_lines
 
     torch = _t()
     wl = np.asarray(wavelength_grid, dtype=np.float64)
     D, W = atmosphere.layers, wl.size
+    check_wing_reach(wl, getattr(atmosphere, "doppler_per_ion", None))
     d = upload_atmosphere(atmosphere)
     soa = build_line_soa(catalog, wl, d.n_elem, grid_origin=grid_origin)
     lines = upload_lines(soa, W)

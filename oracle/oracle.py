@@ -232,6 +232,14 @@ def continuum(atm, grid):
     return out_a, out_s
 
 
+def flux_to_lambda(grid, flux_hz):
+    """F_nu per Hz -> F_lambda per nm (engine/opacity.py:3852-3861)."""
+    grid = _c(grid, np.float64)
+    out = _c(flux_hz, np.float64).copy()
+    lib().orc_flux_to_lambda(ct.c_long(grid.size), p(grid), p(out))
+    return out
+
+
 def freq_quantities(grid, temp):
     D, W = temp.size, grid.size
     e, s, b, sl = (np.zeros((D, W)) for _ in range(4))

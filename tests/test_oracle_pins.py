@@ -318,3 +318,171 @@ def test_single_line_transp_and_wings(oracle, small_workload):
         assert np.allclose(row[c + kk] * kk * kk, xfar, rtol=1e-14)
     else:
         assert ms <= n10
+
+
+# ------------------------------------------------- A5 Planck, A15 F_lambda
+def test_planck_closed_form_and_limits(oracle):
+    """engine/radiative.py:20-39: B_nu = 2 h nu^3 / c^2 / (e^{h nu / kT} - 1).
+    The reference's prefactor 1.47439e-2 (nu in units of 1e15 Hz) is 2h/c^2
+    with an older h: within 1e-4 of CODATA.  EHVKT = e^{-x}, STIM = 1 - e^{-x}
+    exactly, the LTE line source equals B, and the Wien (x >> 1) and
+    Rayleigh-Jeans (x << 1) limits hold."""
+    h, k, c = 6.62607015e-27, 1.380649e-16, 2.99792458e10
+    grid = np.array([100.0, 300.0, 500.0, 1000.0, 5000.0, 1e5, 1e7])  # nm
+    temp = np.array([3000.0, 5750.0, 40000.0])
+    e, st, b, sl = oracle.freq_quantities(grid, temp)
+    nu = c / (grid * 1e-7)
+    for d, T in enumerate(temp):
+        x = h * nu / (k * T)
+        np.testing.assert_allclose(e[d], np.exp(-x), rtol=1e-14)
+        np.testing.assert_allclose(st[d], -np.expm1(-x), rtol=0, atol=1.2e-16)  # 1 - e^-x as written
+        planck = 2 * h * nu ** 3 / c ** 2 / np.expm1(x)
+        np.testing.assert_allclose(b[d], planck, rtol=1e-4)
+        np.testing.assert_allclose(sl[d], b[d], rtol=4e-16)
+        wien = x > 30
+        np.testing.assert_allclose(b[d][wien], 2 * h * nu[wien] ** 3 / c ** 2 * np.exp(-x[wien]), rtol=1e-4)
+        rj = x < 1e-3
+        np.testing.assert_allclose(b[d][rj], 2 * nu[rj] ** 2 * k * T / c ** 2, rtol=2e-3)
+    assert np.any(h * nu / (k * temp[0]) > 30) and np.any(h * nu / (k * temp[-1]) < 1e-3)
+
+
+def test_flux_to_lambda_closed_form(oracle):
+    """engine/opacity.py:3852-3861: F_lambda = F_nu c / lambda^2, c in nm/s."""
+    grid = np.array([300.0, 500.0, 1800.0])
+    f = np.array([1.0e-5, 2.5e-5, 7.0e-6])
+    out = oracle.flux_to_lambda(grid, f)
+    np.testing.assert_allclose(out, f * 2.99792458e17 / grid ** 2, rtol=2e-16)
+    # 1 erg/s/cm^2/Hz at 500 nm is 1.1992e12 erg/s/cm^2/nm
+    assert oracle.flux_to_lambda(np.array([500.0]), np.array([1.0]))[0] == pytest.approx(1.19916983e12, rel=1e-8)
+
+
+# -------------------------------------------- A7 gates, A10 near-wing values
+def _one_line(C, lam, loggf, elo, log_gr, log_gs=-5.5, log_gw=-7.5):
+    nu = 2.99792458e17 / lam
+    g = lambda v: 10 ** v / (12.5664 * nu)  # noqa: E731
+    return C.Record(lam, lam, "Fe", 1, loggf, elo, g(log_gr), g(log_gs), g(log_gw), code=26.0)
+
+
+def test_transp_gates_at_their_thresholds(oracle, small_workload):
+    """physics/line_opacity.py:111-135: a pair is kept when the line-centre
+    opacity before the Boltzmann factor reaches KAPMIN = continuum(centre) *
+    cutoff (gate 1) and again after it (gate 2).  Lines placed 2% either side
+    of each threshold at one depth, by hand from the bundle's N/U and Doppler
+    width."""
+    import catalog as C
+
+    atm = small_workload.atmosphere
+    g = oracle.wavelength_grid(500.0, 500.4, 3e5)
+    ca, cs = oracle.continuum(atm, g)
+    km = ca + cs
+    d, c = 30, 120
+    lam = g[c] * (1 + 1e-8)
+    pv = atm.population_per_ion[d, 0, 25]
+    dv = atm.doppler_per_ion[d, 0, 25]
+    rho = atm.mass_density[d]
+    kapmin = km[d, c] * 1e-3
+    nu = 2.99792458e17 / lam
+    # cgf N/(rho dop) = kapmin (1 +- 2%) fixes gf; with E = 0 the Boltzmann factor is 1
+    gf_at = kapmin / (0.026538 / 1.77245 / nu * pv / (rho * dv))
+    elo_half = math.log(2.0) / oracle.hckt(atm)[d]  # FASTEX(x) = 1/2 (table value at ln 2 +- 5e-4)
+    recs = [_one_line(C, lam, math.log10(gf_at * 0.98), 0.0, 8.0),        # fails gate 1
+            _one_line(C, lam, math.log10(gf_at * 1.02), 0.0, 8.0),        # passes both
+            _one_line(C, lam, math.log10(gf_at * 1.9), elo_half, 8.0),    # x 1.9 x 1/2 < 1: fails gate 2
+            _one_line(C, lam, math.log10(gf_at * 2.1), elo_half, 8.0)]    # x 2.1 x 1/2 > 1: passes
+    soa = C.line_soa(C.Catalog.of(recs), g, 99, grid_origin=float(g[0]))
+    tr, va = oracle.transp(soa, atm, g, km)
+    assert va[:, d].tolist() == [False, True, False, True]
+    assert tr[0, d] == 0.0 and tr[2, d] == 0.0 and tr[1, d] > 0.0 and tr[3, d] > 0.0
+
+
+def test_near_wing_values(oracle, small_workload):
+    """The Voigt zone 1 <= k <= N10DOP (physics/line_opacity.py:464-490,
+    530-548): for a < 0.2, kappa0 (H0[i] + a H1[i]) with the table index
+    i = int(0.5 + k * 200 dvoigt) (the reference accumulates 200 dvoigt from
+    0.5, so pixels whose index sits within 1e-9 of a step are left out), or
+    kappa0 0.5642 a / x^2 past x = k dvoigt = 10; for a >= 0.2,
+    kappa0 H(a, x) with H = Re w(x + i a) (Faddeeva) within the Voigt
+    approximation's error."""
+    import catalog as C
+
+    atm = small_workload.atmosphere
+    g = oracle.wavelength_grid(500.0, 500.4, 3e5)
+    ca, cs = oracle.continuum(atm, g)
+    km = ca + cs
+    t = oracle.tables()
+    c, d = 120, 40
+    lam = g[c] * (1 + 1e-8)
+    resolu = 1.0 / (g[1] / g[0] - 1.0)
+    stim = 1 - np.exp(-2.99792458e17 / g * 6.62607015e-27 / (1.380649e-16 * atm.temperature[d]))
+    for log_gr, small_a in ((8.0, True), (12.5, False)):
+        rec = _one_line(C, lam, 0.5, 1000.0, log_gr)
+        soa = C.line_soa(C.Catalog.of([rec]), g, 99, grid_origin=float(g[0]))
+        tr, va = oracle.transp(soa, atm, g, km)
+        od, oi = oracle.wing_params(soa, atm, g, tr, va, km)
+        kappa0, a, dvoigt = od[0, d, 0], od[0, d, 1], od[0, d, 2]
+        n10 = int(oi[0, d, 0])
+        assert (a < 0.2) == small_a and n10 >= 10
+        assert dvoigt == pytest.approx(1.0 / (atm.doppler_per_ion[d, 0, 25] * resolu), rel=1e-12)
+        asy, _ = oracle.asynth(soa, atm, g, tr, va, km)
+        row = asy[d] / stim
+        checked = 0
+        for k in range(1, n10 + 1):
+            if c + k >= g.size:
+                break
+            x = k * dvoigt
+            if small_a:
+                pos = 0.5 + k * 200.0 * dvoigt
+                if abs(pos - round(pos)) < 1e-9:
+                    continue
+                i = min(int(pos), 2000)
+                want = kappa0 * (0.5642 * a / (x * x) if x > 10.0 else t.h0[i] + a * t.h1[i])
+                rtol = 1e-13
+            else:
+                want = kappa0 * wofz(x + 1j * a).real
+                rtol = 3e-3
+            for side in (c + k, c - k):
+                assert row[side] == pytest.approx(want, rel=rtol), (k, side, small_a)
+            checked += 1
+        assert checked >= 10
+
+
+# ---------------------------------------------------- A13 label 401, alpha > 0
+@pytest.mark.parametrize("fp64", [True, False])
+def test_josh_label401_with_scattering(oracle, fp64):
+    """tau(0) > 20 with alpha = sigma / (kappa + sigma) = 1/2: the label-401
+    iteration S <- (1 - alpha) Sbar + alpha (D(D(S)/3) + S), D = DERIV on the
+    tau grid (physics/josh_solver.py:700-727), ends at the fixed point of that
+    map, which scipy's root finder locates here independently (DERIV itself is
+    pinned separately); the flux is H(0) = D(S*)[0] / 3.  The source varies on
+    a scale of 3 in tau, so scattering moves the flux by 0.2%."""
+    from scipy.optimize import root
+
+    n = 40
+    rhox = np.logspace(np.log10(1.5e-5), -3.0, n)
+    ktot = 2.0e6
+    ka = ks = 0.5 * ktot
+    tau = ktot * rhox
+    assert tau[0] > 20.0
+    # the oracle integrates tau with INTEG (exact here: abtot is constant)
+    np.testing.assert_allclose(oracle.integ(rhox, np.full(n, ktot), ktot * rhox[0]), tau, rtol=1e-13)
+    sbar = 1.0e4 * (1.0 + 0.5 * np.exp(-(tau - tau[0]) / 3.0))
+    alpha = ks / ktot
+
+    def step(S):
+        return (1 - alpha) * sbar + alpha * (oracle.deriv(tau, oracle.deriv(tau, S) / 3.0) + S)
+
+    sol = root(lambda S: step(S) - S, sbar, method="hybr", tol=1e-14)
+    assert sol.success and np.max(np.abs(step(sol.x) - sol.x)) < 1e-9
+    h_fixed = oracle.deriv(tau, sol.x)[0] / 3.0
+    h_noscat = oracle.deriv(tau, sbar)[0] / 3.0
+    assert abs(h_fixed - h_noscat) > 1e-3 * abs(h_noscat)
+    z = np.zeros(n)
+    if fp64:
+        oracle.set_josh_fp64(True, 1e-13)
+    try:
+        f, it, b401 = oracle.solve_josh_flux(np.full(n, ka), sbar, z, sbar, np.full(n, ks), z, rhox)
+    finally:
+        oracle.set_josh_fp64(False)
+    assert b401 and it > 1
+    # the reference's tolerance: relative changes summing to < 1e-5
+    assert f == pytest.approx(h_fixed, rel=1e-11 if fp64 else 2e-5)
