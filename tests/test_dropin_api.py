@@ -93,3 +93,42 @@ def test_run_synthesis_cli(gpu, oracle, small_workload, tmp_path):
     np.testing.assert_allclose(rows[:, 2], ref.continuum, rtol=1e-6)
     z = np.load(diag)
     assert _rel(z["flux_total"], ref.flux) < 1e-6
+
+
+@pytest.mark.gpu
+def test_dropins_take_reference_shaped_objects(gpu, small_workload):
+    """compute_transp / compute_asynth_from_transp / build_depth_continuum on
+    objects carrying only the reference's fields: a LineCatalog with element
+    symbols and records (io/lines/atomic.py:235-296) and an AtmosphereModel
+    without this package's helper methods (io/atmosphere.py:16-79; txnxn
+    absent, so the xnf_* fallback of physics/populations.py:190-199 applies)."""
+    import types
+
+    from kurucz_py_joss_submission_amd import physics
+
+    wk = small_workload
+    atm, cat = wk.atmosphere, wk.catalog
+    fields = ("depth", "temperature", "gas_pressure", "electron_density", "mass_density", "turbulent_velocity",
+              "hckt", "txnxn", "xnf_h", "xnf_he1", "xnf_h2", "population_per_ion", "doppler_per_ion",
+              "continuum_wledge", "continuum_half_edge", "continuum_delta_edge", "continuum_abs_coeff",
+              "continuum_scat_coeff", "continuum_coeff_log10")
+    ref_atm = types.SimpleNamespace(**{f: getattr(atm, f) for f in fields}, metadata={}, xnf_he2=None)
+    ref_atm.layers = atm.layers
+    assert not hasattr(ref_atm, "txnxn_or_default") and ref_atm.txnxn is None
+    recs = [types.SimpleNamespace(code=float(c), line_size=int(s)) for c, s in zip(cat.code, cat.line_size)]
+    ref_cat = types.SimpleNamespace(
+        records=recs, wavelength=cat.wavelength, index_wavelength=cat.index_wavelength, log_gf=cat.log_gf,
+        gf=cat.gf, excitation_energy=cat.excitation_energy, gamma_rad=cat.gamma_rad,
+        gamma_stark=cat.gamma_stark, gamma_vdw=cat.gamma_vdw, elements=cat.elements, ion_stages=cat.ion_stages,
+        line_types=cat.line_types, n_lower=cat.n_lower, n_upper=cat.n_upper)
+    grid = physics.N.wavelength_grid(wk.wl_start, wk.wl_end, wk.resolution)
+    ca, cs, _, _ = physics.build_depth_continuum(ref_atm, grid)
+    ca0, cs0, _, _ = physics.build_depth_continuum(atm, grid)
+    assert np.array_equal(ca, ca0) and np.array_equal(cs, cs0)
+    km = ca + cs
+    tr, va, ci = physics.compute_transp(ref_cat, None, ref_atm, 1e-3, km, grid)
+    tr0, va0, ci0 = physics.compute_transp(cat, None, atm, 1e-3, km, grid)
+    assert np.array_equal(tr, tr0) and np.array_equal(va, va0) and np.array_equal(ci, ci0)
+    a = physics.compute_asynth_from_transp(tr, ref_cat, ref_atm, grid, va, None, 1e-3, km, grid_origin=grid[0])
+    a0 = physics.compute_asynth_from_transp(tr0, cat, atm, grid, va0, None, 1e-3, km, grid_origin=grid[0])
+    assert np.array_equal(a, a0)
