@@ -827,22 +827,66 @@ __device__ __forceinline__ double far_quotient(double x, double b) {
   else return div_fast(x, b);
 }
 
+// waves per SIMD the gather's register budget is sized for: the FP64 form's
+// 24 expansion moments need ~100 VGPRs, the FP32 form's 12 fewer
 #ifndef SYNTHE_GATHER_WAVES
-#define SYNTHE_GATHER_WAVES 6  // waves per SIMD the register budget is sized for
+#define SYNTHE_GATHER_WAVES 5
 #endif
+#ifndef SYNTHE_GATHER_WAVES32
+#define SYNTHE_GATHER_WAVES32 7
+#endif
+template <int kPrec>
+constexpr int gather_waves() {
+  return kPrec == 1 ? SYNTHE_GATHER_WAVES32 : SYNTHE_GATHER_WAVES;
+}
+
+// Local expansion of the far wings of entries far from a tile.  For an entry
+// whose far zone covers the whole tile, with the tile centre at distance
+// K = w_m - c from the line centre and pixel w = w_m + u, |u| <= h (half the
+// tile),
+//   X / (K + u)^2 = (X / K^2) sum_n (n + 1) (-u/K)^n.
+// With rho = h / |K|, the terms from n = T on sum to at most
+// (T + 1) rho^T (1 + rho)^2 / (1 - rho)^2 of the exact value.  Entries with
+// |K| >= kFarExpandRatio h take the expansion: rho <= 1/4.5, so T = 24 terms
+// leave 1.2e-14 relative per entry (FP64 form) and T = 12 leave 4.7e-7 (FP32
+// form).  Every entry's exact contribution X / k^2 is positive, so a pixel's
+// sum of expansions keeps the same relative bound.
+#ifndef SYNTHE_FAR_TERMS
+#define SYNTHE_FAR_TERMS 24
+#endif
+constexpr int kFarTerms = SYNTHE_FAR_TERMS;
+constexpr int kFarTerms32 = 12;
+constexpr double kFarExpandRatio = 4.5;
+
+__device__ __forceinline__ double wave_sum_butterfly(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;  // every lane holds the same sum (fixed pairing order)
+}
+
+// The tile lies wholly in the entry's far zone N10DOP < |k| <= MAXSTEP.
+__device__ __forceinline__ bool far_interior(const WingHead& h, int t0i, int t1i) {
+  return (t0i > h.nhi && t1i <= h.hi) || (t1i < h.nlo && t0i >= h.lo);
+}
 
 // Continues each pixel's sum from the Voigt-zone sums of k_near (in asynth),
 // adds the line centres (catalog order, line_opacity.py:1091-1102), then the
 // far wings X / k^2 (N10DOP < |k| <= MAXSTEP, line_opacity.py:562-617) of
 // every list entry that reaches the tile, and multiplies by STIM
-// (line_opacity.py:1338).  One wave per (512-pixel tile, depth), 8 consecutive
-// pixels per lane.  Entries are read with wave-uniform loads and classified
-// against the whole tile with scalar branches: where the tile lies wholly in
-// one far wing every lane adds X / k^2 branch-free, its 8 pixels sharing one
-// reciprocal (far_chunk); on the tiles holding a wing's N10DOP or MAXSTEP
-// edge, pixels outside the far zone add +0.  Each
-// pixel's sum has a fixed order (Voigt zones, centres, far wings by class and
-// centre), no atomics, so results are run-to-run identical.
+// (line_opacity.py:1338).  One wave per (64 kWP pixel tile, depth), kWP
+// consecutive pixels per lane.  Two passes over the class windows, each
+// taking 64 entries per step with one coalesced load of their heads:
+//  1. entries whose far zone covers the tile at |K| >= 4.5 h add their
+//     expansion moments (one entry per lane); one wave reduction, then each
+//     pixel evaluates the expansion (Horner in u/h);
+//  2. the other entries that reach the tile are taken one at a time, their
+//     heads broadcast from the lane that loaded them: where the tile lies in
+//     one far wing every lane adds X / k^2 branch-free, its pixels sharing one
+//     reciprocal (far_chunk); on the tiles holding a wing's N10DOP or MAXSTEP
+//     edge, pixels outside the far zone add +0.
+// Each pixel's sum has a fixed order (Voigt zones, centres, expansions, exact
+// far wings by class and centre), no atomics, so results are run-to-run
+// identical.
 template <bool kStats, int kPrec, int kWP>
 __device__ __forceinline__ void gather_tile(const GatherArgs& g, int d, long t0, int lane,
                                             unsigned long long* __restrict__ evals) {
@@ -854,7 +898,6 @@ __device__ __forceinline__ void gather_tile(const GatherArgs& g, int d, long t0,
   const double* __restrict__ transp = g.transp;
   const uint8_t* __restrict__ valid = g.valid;
   const WingHead* __restrict__ list_cm = g.list_cm;
-  const WingParams* __restrict__ list_p = g.list_p;
   const int64_t* __restrict__ regions = g.regions;
   const double* __restrict__ grid = g.grid;
   const double* __restrict__ temp = g.temp;
@@ -879,9 +922,65 @@ __device__ __forceinline__ void gather_tile(const GatherArgs& g, int d, long t0,
     }
   }
   // Pixel indices, centres and offsets fit in 32 bits (W <= 2^30, centres
-  // within 10^6 of the grid), so the per-entry tests are scalar 32-bit ones.
+  // within 10^6 of the grid), so the per-entry tests are 32-bit ones.
   const int t0i = (int)t0, t1i = (int)t0 + kWT - 1;
   const long bound = class_window_search(lane, d, t0, t0 + kWT - 1, 1L << 40, regions, list_cm);
+  const long npix = W - t0 < kWT ? W - t0 : kWT;  // tile pixels on the grid
+  constexpr double kHalf = 0.5 * kWT;             // a power of two: u / h is exact
+  const double wm = (double)t0 + 0.5 * (kWT - 1);
+  const double kmin = kFarExpandRatio * kHalf;
+  WingHead none;
+  none.lo = INT32_MAX;
+  none.hi = INT32_MIN;
+  none.nlo = INT32_MAX;
+  none.nhi = INT32_MIN;
+  none.xs = 0.0;
+  none.cd = 0.0;
+
+  // ---- pass 1: expansion moments m_n = sum X/K^2 (-h/K)^n
+  {
+    constexpr int kT = kPrec == kPrecFP32 ? kFarTerms32 : kFarTerms;
+    double mom[kT];
+#pragma unroll
+    for (int n = 0; n < kT; ++n) mom[n] = 0.0;
+    for (int kk = 0; kk < kClasses; ++kk) {
+      const long lo = uniform_long(__shfl(bound, 2 * kk));
+      const long hi = uniform_long(__shfl(bound, 2 * kk + 1));
+      for (long j0 = lo; j0 < hi; j0 += 64) {
+        const long j = j0 + lane;
+        const WingHead h = j < hi ? list_cm[j] : none;
+        const double K = wm - h.cd;
+        const bool expand = far_interior(h, t0i, t1i) && fabs(K) >= kmin && h.xs >= 0.0;
+        const double x = expand ? h.xs : 0.0;
+        const double kq = expand ? K : kmin;
+        double y = __builtin_amdgcn_rcp(kq);  // 1/K to an ulp: two Newton steps
+        y = __builtin_fma(y, __builtin_fma(-kq, y, 1.0), y);
+        y = __builtin_fma(y, __builtin_fma(-kq, y, 1.0), y);
+        double q = x * y * y;
+        const double rr = -kHalf * y;
+        mom[0] += q;
+#pragma unroll
+        for (int n = 1; n < kT; ++n) {
+          q *= rr;
+          mom[n] += q;
+        }
+        if (kStats && expand) nev += npix > 0 ? npix : 0;
+      }
+    }
+    double coef[kT];
+#pragma unroll
+    for (int n = 0; n < kT; ++n) coef[n] = (double)(n + 1) * wave_sum_butterfly(mom[n]);
+#pragma unroll
+    for (int p = 0; p < kWP; ++p) {
+      const double t = ((double)(w0 + p) - wm) / kHalf;
+      double s = coef[kT - 1];
+#pragma unroll
+      for (int n = kT - 2; n >= 0; --n) s = __builtin_fma(s, t, coef[n]);
+      acc[p] += s;
+    }
+  }
+
+  // ---- pass 2: the remaining entries, exactly
   const double kdl = (double)(int)w0;  // lane's first pixel
   // pixels per shared reciprocal: the largest divisor of kWP up to 8 (4 on
   // edge tiles, which need more registers)
@@ -897,49 +996,68 @@ __device__ __forceinline__ void gather_tile(const GatherArgs& g, int d, long t0,
   for (int kk = 0; kk < kClasses; ++kk) {
     const long lo = uniform_long(__shfl(bound, 2 * kk));
     const long hi = uniform_long(__shfl(bound, 2 * kk + 1));
-    const WingHead* __restrict__ hp = list_cm + lo;
-    const int n = (int)(hi - lo);  // entries of one depth: fewer than the lines
-    for (int j = 0; j < n; ++j) {
-      const WingHead h = hp[j];
-      if (t1i < h.lo || t0i > h.hi) continue;  // no far wing on this tile
-      double kd = kdl - h.cd;  // offset of the lane's first pixel from the centre
-      __asm__("" : "+v"(kd));  // keep k = kd + p in double (no per-pixel int conversion)
-      // the tile lies wholly in the far zone N10DOP < |k| <= MAXSTEP
-      const bool interior = (t0i > h.nhi && t1i <= h.hi) || (t1i < h.nlo && t0i >= h.lo);
-      const bool chunk = __double_as_longlong(h.xs) >= 0;
-      const int n10 = (h.nhi - h.nlo) >> 1, M = (h.hi - h.lo) >> 1;
-      if (chunk && interior) {  // |k| >= N10DOP + 1 >= 1 on every pixel of the tile
-#pragma unroll
-        for (int p0 = 0; p0 < kWP; p0 += kChunk)
-          far_chunk_acc<kChunk, kNewton>(h.xs, kd, p0, acc + p0);
-      } else {
-        double add[kWP];
-        if (chunk) {
-          const double n10d = (double)n10, md = (double)M;
-#pragma unroll
-          for (int p0 = 0; p0 < kWP; p0 += kChunkEdge)
-            far_chunk<kChunkEdge, kNewton, true>(h.xs, kd, p0, n10d, md, add + p0);
-        } else {  // X below 2^-500: one quotient per pixel
-          const double n10d = (double)n10, md = (double)M, x = -h.xs;
-#pragma unroll
-          for (int p = 0; p < kWP; ++p) {
-            const double k = kd + (double)p;
-            const double xp = (fabs(k) > n10d && fabs(k) <= md) ? x : 0.0;
-            add[p] = far_quotient<kPrec>(xp, fmax(k * k, 1.0));
-          }
+    for (long j0 = lo; j0 < hi; j0 += 64) {
+      const long jl = j0 + lane;
+      const WingHead hv = jl < hi ? list_cm[jl] : none;
+      const bool reach = !(t1i < hv.lo || t0i > hv.hi);
+      const bool expand = far_interior(hv, t0i, t1i) && fabs(wm - hv.cd) >= kmin && hv.xs >= 0.0;
+      unsigned long long todo = __ballot(reach && !expand);
+      while (todo) {
+        const int src = __builtin_ctzll(todo);
+        todo &= todo - 1;
+        WingHead h;  // the entry's head, from the lane that loaded it
+        h.lo = __builtin_amdgcn_readlane(hv.lo, src);
+        h.hi = __builtin_amdgcn_readlane(hv.hi, src);
+        h.nlo = __builtin_amdgcn_readlane(hv.nlo, src);
+        h.nhi = __builtin_amdgcn_readlane(hv.nhi, src);
+        {
+          const long long xb = __double_as_longlong(hv.xs);
+          const long long cb = __double_as_longlong(hv.cd);
+          const int xlo = __builtin_amdgcn_readlane((int)(xb & 0xffffffff), src);
+          const int xhi = __builtin_amdgcn_readlane((int)(xb >> 32), src);
+          const int clo = __builtin_amdgcn_readlane((int)(cb & 0xffffffff), src);
+          const int chi = __builtin_amdgcn_readlane((int)(cb >> 32), src);
+          h.xs = __longlong_as_double((long long)(((unsigned long long)(unsigned)xhi << 32) | (unsigned)xlo));
+          h.cd = __longlong_as_double((long long)(((unsigned long long)(unsigned)chi << 32) | (unsigned)clo));
         }
+        double kd = kdl - h.cd;  // offset of the lane's first pixel from the centre
+        __asm__("" : "+v"(kd));  // keep k = kd + p in double (no per-pixel int conversion)
+        const bool interior = far_interior(h, t0i, t1i);
+        const bool chunk = __double_as_longlong(h.xs) >= 0;
+        const int n10 = (h.nhi - h.nlo) >> 1, M = (h.hi - h.lo) >> 1;
+        if (chunk && interior) {  // |k| >= N10DOP + 1 >= 1 on every pixel of the tile
 #pragma unroll
-        for (int p = 0; p < kWP; ++p) edge[p] += add[p];
-      }
-      if (kStats) {
-        if (interior) {
-          long nv = W - w0 < kWP ? W - w0 : kWP;
-          nev += nv > 0 ? nv : 0;
+          for (int p0 = 0; p0 < kWP; p0 += kChunk)
+            far_chunk_acc<kChunk, kNewton>(h.xs, kd, p0, acc + p0);
         } else {
-          const int c = h.lo + M;
-          for (int p = 0; p < kWP; ++p) {
-            const int ka = abs((int)w0 - c + p);
-            if (ka > n10 && ka <= M && w0 + p < W) ++nev;
+          double add[kWP];
+          if (chunk) {
+            const double n10d = (double)n10, md = (double)M;
+#pragma unroll
+            for (int p0 = 0; p0 < kWP; p0 += kChunkEdge)
+              far_chunk<kChunkEdge, kNewton, true>(h.xs, kd, p0, n10d, md, add + p0);
+          } else {  // X below 2^-500: one quotient per pixel
+            const double n10d = (double)n10, md = (double)M, x = -h.xs;
+#pragma unroll
+            for (int p = 0; p < kWP; ++p) {
+              const double k = kd + (double)p;
+              const double xp = (fabs(k) > n10d && fabs(k) <= md) ? x : 0.0;
+              add[p] = far_quotient<kPrec>(xp, fmax(k * k, 1.0));
+            }
+          }
+#pragma unroll
+          for (int p = 0; p < kWP; ++p) edge[p] += add[p];
+        }
+        if (kStats) {
+          if (interior) {
+            long nv = W - w0 < kWP ? W - w0 : kWP;
+            nev += nv > 0 ? nv : 0;
+          } else {
+            const int c = h.lo + M;
+            for (int p = 0; p < kWP; ++p) {
+              const int ka = abs((int)w0 - c + p);
+              if (ka > n10 && ka <= M && w0 + p < W) ++nev;
+            }
           }
         }
       }
@@ -963,11 +1081,25 @@ __device__ __forceinline__ void gather_tile(const GatherArgs& g, int d, long t0,
   }
 }
 
+// XCD-aware order of the gather's (depth, tile) items: the hardware hands
+// workgroup L to XCD L mod 8, so item v(L) gives each XCD one contiguous run
+// of items (adjacent tiles of a depth), whose common list heads then stay in
+// that XCD's L2.  A bijection on [0, n).
+constexpr int kXcds = 8;
+__device__ __forceinline__ long xcd_item(long L, long n) {
+  const long q = n / kXcds, r = n % kXcds;
+  const long x = L % kXcds, i = L / kXcds;
+  return x * q + (x < r ? x : r) + i;
+}
+
 
 template <bool kStats, int kPrec>
-__global__ __launch_bounds__(64, kStats ? 6 : SYNTHE_GATHER_WAVES) void k_gather(GatherArgs g,
+__global__ __launch_bounds__(64, gather_waves<kPrec>()) void k_gather(GatherArgs g,
                                                                     unsigned long long* __restrict__ evals) {
-  gather_tile<kStats, kPrec, kWP>(g, blockIdx.y, blockIdx.x * (long)kWT, threadIdx.x, evals);
+  const long ntiles = gridDim.x;
+  const long v = xcd_item(blockIdx.y * ntiles + blockIdx.x, ntiles * gridDim.y);
+  const int d = (int)(v / ntiles);
+  gather_tile<kStats, kPrec, kWP>(g, d, (v - (long)d * ntiles) * kWT, threadIdx.x, evals);
 }
 
 // Persistent form over many spectra: each wave takes (instance, depth, tile)
@@ -975,7 +1107,7 @@ __global__ __launch_bounds__(64, kStats ? 6 : SYNTHE_GATHER_WAVES) void k_gather
 // instance order[i]'s items, depth-major; every item writes only its own
 // tile's pixels, so the result does not depend on which wave takes it.
 template <int kPrec, int kWPt>
-__global__ __launch_bounds__(64, kWPt == 16 ? 4 : (kWPt == 8 ? 6 : 7)) void k_gather_batch(const GatherArgs* __restrict__ inst,
+__global__ __launch_bounds__(64, gather_waves<kPrec>()) void k_gather_batch(const GatherArgs* __restrict__ inst,
                                                      const int32_t* __restrict__ order,
                                                      const int64_t* __restrict__ items, int n_inst,
                                                      unsigned long long* __restrict__ counter) {
@@ -1005,7 +1137,7 @@ __global__ __launch_bounds__(64, kWPt == 16 ? 4 : (kWPt == 8 ? 6 : 7)) void k_ga
 // The same items with one workgroup each (item = blockIdx.x), in the queue's
 // order: no loop around the tile, so nothing of it stays live between items.
 template <int kPrec, int kWPt>
-__global__ __launch_bounds__(64, kWPt == 16 ? 4 : (kWPt == 8 ? SYNTHE_GATHER_WAVES : 8)) void k_gather_items(
+__global__ __launch_bounds__(64, gather_waves<kPrec>()) void k_gather_items(
     const GatherArgs* __restrict__ inst, const int32_t* __restrict__ order, const int64_t* __restrict__ items,
     int n_inst) {
   constexpr long kTile = 64L * kWPt;

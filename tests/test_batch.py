@@ -66,7 +66,7 @@ def test_batch_graph_equals_single_plans(gpu, batch_inputs):
     ref = [_single(*x) for x in batch_inputs]
     bp = BatchPlan(batch_inputs, autotune=True, graph=True)
     info = bp.info()
-    assert info["pixels_per_lane"] == 8 and len(info["autotune_ms"]) == 4
+    assert info["pixels_per_lane"] in (2, 4, 8) and len(info["autotune_ms"]) == 4
     for replay in range(3):
         bp.run()
         torch.cuda.synchronize()
