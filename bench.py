@@ -413,14 +413,12 @@ def main() -> int:
     ev[0].record()
     plan.run_continuum()
     ev[1].record()
-    plan.run_transp()
+    plan.run_line_opacity()
     ev[2].record()
-    plan.run_asynth()
-    ev[3].record()
     plan.run_rt()
-    ev[4].record()
+    ev[3].record()
     torch.cuda.synchronize()
-    for i, k in enumerate(("continuum+source", "transp", "asynth", "rt")):
+    for i, k in enumerate(("continuum+source", "transp+asynth", "rt")):
         stages[k] = ev[i].elapsed_time(ev[i + 1])
     log(f"[bench] stages ms: {json.dumps({k: round(v, 3) for k, v in stages.items()})}")
 
@@ -472,7 +470,7 @@ def main() -> int:
         "traffic": None,
         "kernel_ms": g_ms,
         "flops_per_launch": flops,
-        "asynth_stage_ms": stages["asynth"],
+        "line_opacity_stage_ms": stages["transp+asynth"],
     }
     pmc = ROOT / "profiles" / f"traffic_{cfg['name']}.json"
     if pmc.exists():

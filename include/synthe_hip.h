@@ -149,6 +149,18 @@ int synthe_asynth(synthe_ctx *ctx, const synthe_lines *lines, const synthe_depth
                   const int64_t *center_start, const int64_t *wing_order, double *asynth,
                   double *stats, synthe_stream stream);
 
+/* physics/line_opacity.py:620-860 and 946-1340 in one call, as
+ * engine/opacity.py:2320-2363 runs compute_transp and then
+ * compute_asynth_from_transp: TRANSP and its valid mask are computed in the
+ * wing set-up's pass over the (line, depth) pairs and written to transp,
+ * valid [L][D] (outputs here), then ASYNTH as synthe_asynth.  No
+ * microturbulence (synthe_transp's microturb_kms = 0). */
+int synthe_transp_asynth(synthe_ctx *ctx, const synthe_lines *lines, const synthe_depths *depths,
+                         long n_wl, const double *wl, const double *kapmin, double cutoff,
+                         const int64_t *center_order, const int64_t *center_start,
+                         const int64_t *wing_order, double *transp, uint8_t *valid, double *asynth,
+                         double *stats, synthe_stream stream);
+
 /* engine/radiative.py:282-494 (solve_lte_spectrum) with
  * physics/josh_solver.py:367-767 (solve_josh_flux) per wavelength: the
  * continuum-only and the total JOSH solves.  All [D][W] except wl [W],

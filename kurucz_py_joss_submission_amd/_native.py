@@ -28,7 +28,7 @@ SYMBOLS = (
     "synthe_flux_to_lambda", "synthe_diag_div_pos", "synthe_set_kernel_timing",
     "synthe_kernel_times", "synthe_set_profile_precision", "synthe_batch_create", "synthe_batch_run",
     "synthe_batch_info", "synthe_batch_destroy", "synthe_asynth_vjp", "synthe_lte_vjp",
-    "synthe_lte_spectrum_noalloc",
+    "synthe_lte_spectrum_noalloc", "synthe_transp_asynth",
 )
 
 SYNTHE_BATCH_AUTOTUNE = 1
@@ -107,6 +107,8 @@ def load_library(path: Optional[Path] = None):
                                       ct.c_double, ct.c_double, _VP, _VP, _VP]
         lib.synthe_asynth.argtypes = [_VP, ct.POINTER(Lines), ct.POINTER(Depths), ct.c_long, _VP, _VP,
                                       _VP, _VP, ct.c_double, _VP, _VP, _VP, _VP, _VP, _VP]
+        lib.synthe_transp_asynth.argtypes = [_VP, ct.POINTER(Lines), ct.POINTER(Depths), ct.c_long, _VP, _VP,
+                                             ct.c_double, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]
         lib.synthe_lte_spectrum.argtypes = [_VP, ct.c_long, ct.c_long] + [_VP] * 12
         lib.synthe_lte_spectrum_noalloc.argtypes = [_VP, ct.c_long, ct.c_long] + [_VP] * 12
         lib.synthe_josh_flux_batch.argtypes = [_VP, ct.c_long, ct.c_long] + [_VP] * 10

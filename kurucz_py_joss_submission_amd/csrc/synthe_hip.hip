@@ -762,24 +762,37 @@ __global__ __launch_bounds__(256) void k_near(long W, int D, const WingHead* __r
   for (int kk = 0; kk < kClasses; ++kk) {
     const long lo = uniform_long(__shfl(bound, 2 * kk));
     const long hi = uniform_long(__shfl(bound, 2 * kk + 1));
-    for (long j = lo; j < hi; ++j) {
-      const WingHead h = list_cm[j];
-      if (v0i + 63 < h.nlo || v0i > h.nhi) continue;  // wave-uniform
-      const int lim = (h.nhi - h.nlo) >> 1;          // the zone's half-width
-      const int c = h.nlo + lim;
-      const int ka = abs(wi - c);
-      const bool act = (unsigned)(ka - 1) < (unsigned)lim && on_grid;  // 1 <= |k| <= lim
-      if (__ballot(act && cnt == kNearSlots)) {  // a pixel is out of slots
-        near_flush(lane, qn, cnt, acc, queue, vals, lp, tb);
-        qn = 0;
+    // 64 entries per step: one coalesced load of their zone bounds, a ballot
+    // of those whose zone meets the window, then each of those in list order
+    for (long j0 = lo; j0 < hi; j0 += 64) {
+      const long jl = j0 + lane;
+      int2 z = make_int2(INT32_MAX, INT32_MIN);
+      if (jl < hi) {
+        const WingHead hv = list_cm[jl];
+        z = make_int2(hv.nlo, hv.nhi);
       }
-      const unsigned long long m = __ballot(act);
-      if (act) {
-        queue[qn + __popcll(m & below)] = near_record(j - base, ka, lane, cnt);
-        ++cnt;
-        if (kStats) ++nev;
+      unsigned long long meet = __ballot(!(v0i + 63 < z.x || v0i > z.y));
+      while (meet) {
+        const int src = __builtin_ctzll(meet);
+        meet &= meet - 1;
+        const int nlo = __builtin_amdgcn_readlane(z.x, src);
+        const int nhi = __builtin_amdgcn_readlane(z.y, src);
+        const int lim = (nhi - nlo) >> 1;  // the zone's half-width
+        const int c = nlo + lim;
+        const int ka = abs(wi - c);
+        const bool act = (unsigned)(ka - 1) < (unsigned)lim && on_grid;  // 1 <= |k| <= lim
+        if (__ballot(act && cnt == kNearSlots)) {  // a pixel is out of slots
+          near_flush(lane, qn, cnt, acc, queue, vals, lp, tb);
+          qn = 0;
+        }
+        const unsigned long long m = __ballot(act);
+        if (act) {
+          queue[qn + __popcll(m & below)] = near_record(j0 + src - base, ka, lane, cnt);
+          ++cnt;
+          if (kStats) ++nev;
+        }
+        qn += __popcll(m);
       }
-      qn += __popcll(m);
     }
   }
   near_flush(lane, qn, cnt, acc, queue, vals, lp, tb);
@@ -2788,31 +2801,29 @@ int synthe_transp(synthe_ctx* ctx, const synthe_lines* lines, const synthe_depth
   return SYNTHE_OK;
 }
 
-int synthe_asynth(synthe_ctx* ctx, const synthe_lines* lines, const synthe_depths* depths, long n_wl,
-                  const double* wl, const double* transp, const uint8_t* valid, const double* kapmin,
-                  double cutoff, const int64_t* center_order, const int64_t* center_start,
-                  const int64_t* wing_order, double* asynth, double* stats, synthe_stream stream) {
-  if (!ctx) return fail(SYNTHE_EINVAL, "synthe_asynth: ctx is NULL");
-  int rc = check_asynth_args(lines, depths, n_wl, wl, transp, valid, kapmin, center_order, center_start,
-                             wing_order, asynth, "synthe_asynth");
-  if (rc) return rc;
-  hipStream_t st = (hipStream_t)stream;
+// synthe_asynth, or with transp_out the fused form: TRANSP and valid are
+// computed in the wing set-up's pass (k_transp_wing mode 3) and written out.
+static int asynth_call(synthe_ctx* ctx, const synthe_lines* lines, const synthe_depths* depths, long n_wl,
+                       const double* wl, const double* transp, const uint8_t* valid, const double* kapmin,
+                       double cutoff, const int64_t* center_order, const int64_t* center_start,
+                       const int64_t* wing_order, double* asynth, double* stats, hipStream_t st,
+                       double* transp_out, uint8_t* valid_out) {
   const long L = lines->n_lines;
   const int D = (int)depths->n_depths;
   const AsynthLayout lay = asynth_layout(L, D);
-  rc = ensure_ws(ctx, lay.total);
+  int rc = ensure_ws(ctx, lay.total);
   if (rc) return rc;
   const AsynthBufs bufs = asynth_bufs((char*)ctx->ws, lay);
   rc = asynth_prepare(ctx, lines, depths, n_wl, wl, transp, valid, kapmin, cutoff, wing_order, asynth,
-                      stats != nullptr, bufs, st);
+                      stats != nullptr, bufs, st, true, transp_out, valid_out);
   if (rc) return rc;
   rc = timing_mark(ctx, st, true);
   if (rc) return rc;
   const dim3 g_gather((unsigned)blocks_for(n_wl, kWT), (unsigned)D);
   auto gather = stats ? (ctx->precision == SYNTHE_PREC_FP32 ? k_gather<true, kPrecFP32> : k_gather<true, kPrecFP64>)
                       : (ctx->precision == SYNTHE_PREC_FP32 ? k_gather<false, kPrecFP32> : k_gather<false, kPrecFP64>);
-  const GatherArgs ga = gather_args(lines, depths, n_wl, wl, transp, valid, center_order, center_start, asynth,
-                                    bufs);
+  const GatherArgs ga = gather_args(lines, depths, n_wl, wl, transp_out ? transp_out : transp,
+                                    valid_out ? valid_out : valid, center_order, center_start, asynth, bufs);
   hipLaunchKernelGGL(gather, g_gather, dim3(64), 0, st, ga, stats ? bufs.ev + 3 : nullptr);
   HIP_TRY(hipGetLastError());
   rc = timing_mark(ctx, st, false);
@@ -2827,6 +2838,33 @@ int synthe_asynth(synthe_ctx* ctx, const synthe_lines* lines, const synthe_depth
     HIP_TRY(hipGetLastError());
   }
   return SYNTHE_OK;
+}
+
+int synthe_asynth(synthe_ctx* ctx, const synthe_lines* lines, const synthe_depths* depths, long n_wl,
+                  const double* wl, const double* transp, const uint8_t* valid, const double* kapmin,
+                  double cutoff, const int64_t* center_order, const int64_t* center_start,
+                  const int64_t* wing_order, double* asynth, double* stats, synthe_stream stream) {
+  if (!ctx) return fail(SYNTHE_EINVAL, "synthe_asynth: ctx is NULL");
+  int rc = check_asynth_args(lines, depths, n_wl, wl, transp, valid, kapmin, center_order, center_start,
+                             wing_order, asynth, "synthe_asynth");
+  if (rc) return rc;
+  return asynth_call(ctx, lines, depths, n_wl, wl, transp, valid, kapmin, cutoff, center_order, center_start,
+                     wing_order, asynth, stats, (hipStream_t)stream, nullptr, nullptr);
+}
+
+int synthe_transp_asynth(synthe_ctx* ctx, const synthe_lines* lines, const synthe_depths* depths, long n_wl,
+                         const double* wl, const double* kapmin, double cutoff, const int64_t* center_order,
+                         const int64_t* center_start, const int64_t* wing_order, double* transp, uint8_t* valid,
+                         double* asynth, double* stats, synthe_stream stream) {
+  if (!ctx) return fail(SYNTHE_EINVAL, "synthe_transp_asynth: ctx is NULL");
+  if (lines && lines->n_lines > 0 && (!transp || !valid))
+    return fail(SYNTHE_EINVAL, "synthe_transp_asynth: transp/valid outputs required");
+  // transp/valid are outputs here; the argument check wants non-NULL inputs
+  int rc = check_asynth_args(lines, depths, n_wl, wl, transp, valid, kapmin, center_order, center_start,
+                             wing_order, asynth, "synthe_transp_asynth");
+  if (rc) return rc;
+  return asynth_call(ctx, lines, depths, n_wl, wl, nullptr, nullptr, kapmin, cutoff, center_order, center_start,
+                     wing_order, asynth, stats, (hipStream_t)stream, transp, valid);
 }
 
 int synthe_lte_spectrum(synthe_ctx* ctx, long n_depths, long n_wl, const double* wl,

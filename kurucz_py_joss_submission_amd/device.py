@@ -252,6 +252,25 @@ class SpectrumPlan:
             self.line_scat.copy_(self.asynth * self.fscat_col)
             self.line_abs = self.asynth * self.one_minus_fscat
 
+    def run_line_opacity(self, stream=None, stats: bool = False):
+        """TRANSP and ASYNTH (engine/opacity.py:2320-2363): one fused call
+        (TRANSP in the wing set-up's pass) without microturbulence, else the
+        two drop-in stages."""
+        if self.microturb > 0.0:
+            self.run_transp(stream)
+            self.run_asynth(stream, stats)
+            return
+        s = stream if stream is not None else N.stream_handle()
+        ls, ds = self.lines.struct(), self.depth.struct()
+        self.ctx.set_profile_precision(self.precision)
+        N.check("synthe_transp_asynth", self.ctx.lib.synthe_transp_asynth(
+            self.ctx.handle, ct_byref(ls), ct_byref(ds), self.W, N.ptr(self.d_grid), N.ptr(self.kapmin),
+            self.cutoff, N.ptr(self.lines.corder), N.ptr(self.lines.cstart), N.ptr(self.lines.worder),
+            N.ptr(self.transp), N.ptr(self.valid), N.ptr(self.asynth), N.ptr(self.stats) if stats else None, s))
+        if self.rhoxj > 0.0:
+            self.line_scat.copy_(self.asynth * self.fscat_col)
+            self.line_abs = self.asynth * self.one_minus_fscat
+
     def run_rt(self, stream=None):
         s = stream if stream is not None else N.stream_handle()
         d = self.depth
@@ -270,8 +289,7 @@ class SpectrumPlan:
             return
         self.run_continuum()
         if len(self.soa) > 0:
-            self.run_transp()
-            self.run_asynth(stats=stats)
+            self.run_line_opacity(stats=stats)
         else:
             self.asynth.zero_()
         self.run_rt()

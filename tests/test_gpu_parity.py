@@ -269,3 +269,22 @@ def test_flux_parity_regimes(gpu, oracle, teff, logg, nl, mh, wl0):
     assert _rel(flux[mask], ref.flux[mask]) < FLUX_RTOL
     assert _rel(flux, ref.flux, floor=1e-3 * np.median(ref.continuum)) < FLUX_RTOL
     assert np.mean(plan.iters.cpu().numpy() == ref.stages["iters"]) > 0.995
+
+
+def test_fused_transp_asynth_equals_two_calls(gpu, c1_workload):
+    """synthe_transp_asynth (TRANSP in the wing set-up's pass) gives the bits
+    of synthe_transp followed by synthe_asynth."""
+    import torch
+
+    plan = _plan(c1_workload)
+    plan.run_continuum()
+    plan.run_line_opacity(stats=True)
+    torch.cuda.synchronize()
+    fused = [t.clone() for t in (plan.transp, plan.valid, plan.asynth, plan.stats)]
+    for t in (plan.transp, plan.asynth):
+        t.fill_(float("nan"))
+    plan.run_transp()
+    plan.run_asynth(stats=True)
+    torch.cuda.synchronize()
+    for a, b in zip(fused, (plan.transp, plan.valid, plan.asynth, plan.stats)):
+        assert bool((a == b).all())
