@@ -161,18 +161,7 @@ def c3_instances(wk, n: int = 256, seed: int = 3):
     all drawing on the shared 1,000,000-line list of the full-window workload."""
     from kurucz_py_joss_submission_amd import synthetic as S
 
-    rng = np.random.default_rng(seed)
-    out, meta = [], []
-    for _ in range(n):
-        teff = float(rng.uniform(3500.0, 40000.0))
-        logg = float(rng.uniform(0.0, 5.0))
-        nl = int(rng.integers(64, 81))
-        mh = float(rng.uniform(-2.0, 0.5))
-        w0 = float(rng.uniform(300.0, 1700.0))
-        w1 = w0 + float(rng.uniform(5.0, 50.0))
-        out.append((S.build_bundle(S.make_atm(teff, logg, nl, mh)), wk.catalog, w0, w1))
-        meta.append((teff, logg, nl, mh, w0, w1))
-    return out, meta
+    return S.batch_instances(wk.catalog, n, seed)
 
 
 def bench_batched(args, wk) -> dict:

@@ -370,3 +370,29 @@ def make_workload(name: str, teff: float, logg: float, n_lines: int, wl0: float,
         cat.save_npz(cat_npz)
     return Workload(atm, _lines.filter_by_range(cat, wl0, wl1), wl0, wl1, resolution, name,
                     (n_lines, lw0, lw1, seed))
+
+
+def batch_draws(n: int = 256, seed: int = 3):
+    """The batched configuration's draws: per instance (Teff ~ U(3500, 40000) K,
+    log g ~ U(0, 5), layers ~ U{64..80}, [M/H] ~ U(-2, 0.5), window start ~
+    U(300, 1700) nm, width ~ U(5, 50) nm)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        teff = float(rng.uniform(3500.0, 40000.0))
+        logg = float(rng.uniform(0.0, 5.0))
+        nl = int(rng.integers(64, 81))
+        mh = float(rng.uniform(-2.0, 0.5))
+        w0 = float(rng.uniform(300.0, 1700.0))
+        w1 = w0 + float(rng.uniform(5.0, 50.0))
+        out.append((teff, logg, nl, mh, w0, w1))
+    return out
+
+
+def batch_instances(catalog, n: int = 256, seed: int = 3):
+    """(atmosphere, catalog, wl0, wl1) per instance of the batched
+    configuration, all drawing on one shared line list, and the draws."""
+    draws = batch_draws(n, seed)
+    inst = [(build_bundle(make_atm(teff, logg, nl, mh)), catalog, w0, w1)
+            for teff, logg, nl, mh, w0, w1 in draws]
+    return inst, draws
