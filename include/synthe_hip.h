@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define SYNTHE_ABI_VERSION 1
+#define SYNTHE_ABI_VERSION 2
 
 #define SYNTHE_OK 0
 #define SYNTHE_EINVAL (-1) /* bad size, pointer or argument */
@@ -253,6 +253,8 @@ typedef struct {
   const int64_t *center_order, *center_start, *wing_order; /* as synthe_asynth */
   double cutoff;
   double *flux, *cont;     /* outputs [n_wl], F_lambda */
+  int32_t *iters;          /* output [n_wl] or NULL: as synthe_lte_spectrum's iters
+                            * (+1000 where the total solve took the label-401 branch) */
 } synthe_instance;
 
 typedef struct synthe_batch synthe_batch;

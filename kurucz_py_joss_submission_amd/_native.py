@@ -75,7 +75,7 @@ class Instance(ct.Structure):
                 ("n_edges", ct.c_long), ("wledge", _VP), ("half_edge", _VP), ("delta_edge", _VP),
                 ("coeff_abs", _VP), ("coeff_scat", _VP), ("coeff_log10", ct.c_int),
                 ("center_order", _VP), ("center_start", _VP), ("wing_order", _VP),
-                ("cutoff", ct.c_double), ("flux", _VP), ("cont", _VP)]
+                ("cutoff", ct.c_double), ("flux", _VP), ("cont", _VP), ("iters", _VP)]
 
 
 _lib = None

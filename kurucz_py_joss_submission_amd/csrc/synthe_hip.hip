@@ -3225,7 +3225,7 @@ int synthe_batch_create(synthe_ctx* ctx, int n, const synthe_instance* inst, int
     rts[i] = RtArgs{g0, in.n_wl, (int)in.depths.n_depths, 1, in.wl, in.depths.temperature, in.depths.rhox,
                     (const double*)(b->arena + bi.o_cabs), (const double*)(b->arena + bi.o_cscat),
                     (const double*)(b->arena + bi.o_asynth), b->zeros, (const double*)(b->arena + bi.o_slinec),
-                    in.flux, in.cont, nullptr};
+                    in.flux, in.cont, in.iters};
     g0 += in.n_wl;
   }
   if (hipMemcpy(b->d_rt, rts.data(), n * sizeof(RtArgs), hipMemcpyHostToDevice) != hipSuccess ||

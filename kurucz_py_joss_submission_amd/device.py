@@ -323,11 +323,13 @@ class BatchPlan:
     def __init__(self, instances: Sequence[Tuple[AtmosphereModel, LineCatalog, float, float]],
                  resolution: float = 300000.0, cutoff: float = 1e-3, precision: str = "fp64",
                  autotune: bool = True, graph: bool = True,
-                 depth_arrays: Optional[Sequence[DepthArrays]] = None):
+                 depth_arrays: Optional[Sequence[DepthArrays]] = None, iters: bool = False):
         """depth_arrays (optional): the device arrays of each instance's
         atmosphere, as upload_atmosphere makes them, for callers that keep
         them elsewhere (e.g. views into one batched tensor they rewrite
-        between replays); by default each atmosphere is uploaded here."""
+        between replays); by default each atmosphere is uploaded here.
+        iters: also keep each wavelength's JOSH iteration count (+1000 where
+        the total solve took the label-401 branch) in items[i]["iters"]."""
         import ctypes
 
         from .lines import filter_by_range
@@ -349,14 +351,15 @@ class BatchPlan:
             d_grid = to_device(grid, np.float64)
             flux = torch.empty(grid.size, dtype=torch.float64, device="cuda")
             cont = torch.empty(grid.size, dtype=torch.float64, device="cuda")
+            its = torch.zeros(grid.size, dtype=torch.int32, device="cuda") if iters else None
             p = N.ptr
             arr[i] = N.Instance(lines.struct(), depth.struct(), int(grid.size), p(d_grid),
                                 int(depth.wledge.shape[0]), p(depth.wledge), p(depth.half_edge),
                                 p(depth.delta_edge), p(depth.coeff_abs), p(depth.coeff_scat),
                                 int(depth.coeff_log10), p(lines.corder), p(lines.cstart), p(lines.worder),
-                                float(cutoff), p(flux), p(cont))
+                                float(cutoff), p(flux), p(cont), p(its))
             self.items.append(dict(grid=grid, depth=depth, soa=soa, lines=lines, d_grid=d_grid, flux=flux,
-                                   cont=cont, wl_start=wl0, wl_end=wl1))
+                                   cont=cont, iters=its, wl_start=wl0, wl_end=wl1))
         self._arr = arr
         flags = (N.SYNTHE_BATCH_AUTOTUNE if autotune else 0) | (N.SYNTHE_BATCH_GRAPH if graph else 0)
         self.ctx.set_profile_precision(precision)

@@ -173,6 +173,7 @@ class Target:
     wl_end: float
     delta_true: np.ndarray  # [20] dex
     flux: np.ndarray        # F_lambda of the FP64 plain path at delta_true
+    label401: np.ndarray    # [n_wl] bool: the total solve took the label-401 branch
 
 
 def window_for(wl_start: float, n_wl: int, resolution: float) -> float:
@@ -220,7 +221,8 @@ def make_targets(n: int = 64, n_lines: int = 30_000, n_wl: int = 5_000, seed: in
             atm = S.build_bundle(cols, dict(zip(S.ABUNDANCE_Z, delta.tolist())))
             plan = SpectrumPlan(atm, cat, w0, w1, resolution, cutoff=cutoff)
             plan.run()
-            out.append(Target(cols, cat, w0, w1, delta, plan.flux.cpu().numpy()))
+            out.append(Target(cols, cat, w0, w1, delta, plan.flux.cpu().numpy(),
+                              plan.iters.cpu().numpy() >= 1000))
             del plan
     return out
 
@@ -230,7 +232,13 @@ class InverseProblem:
 
     One BatchPlan holds every target's spectrum; evaluate(delta) writes the
     bundle arrays of delta[n][20] into the plan's device arrays and replays
-    the graph.  The residual of spectrum i is (F - F_target) / median(F_target)."""
+    the graph.  The residual of spectrum i is (F - F_target) / median(F_target)
+    over the pixels where neither the target nor the model takes the JOSH
+    label-401 branch (surface tau > 20, physics/josh_solver.py:700-727): that
+    branch's flux differs from the regular solve's by about the median flux,
+    so a saturated pixel crossing the switch jumps by that much for an
+    arbitrarily small abundance change, and those jumps -- not the smooth
+    response -- made up the whole misfit of the targets where they occur."""
 
     def __init__(self, targets: Sequence[Target], resolution: float = 300000.0, cutoff: float = 1e-3,
                  autotune: bool = True):
@@ -265,7 +273,9 @@ class InverseProblem:
                 view.copy_(getattr(dp, name))
                 setattr(dp, name, view)
         self.plan = BatchPlan(inst, resolution=resolution, cutoff=cutoff, precision="fp64", autotune=autotune,
-                              graph=True, depth_arrays=depths)
+                              graph=True, depth_arrays=depths, iters=True)
+        self.t_regular = [torch.tensor(~t.label401, device="cuda") for t in self.targets]
+        self.valid = None  # per spectrum: pixels regular in target and model at the last evaluation
         self.f_target = [torch.tensor(t.flux, dtype=torch.float64, device="cuda") for t in self.targets]
         # residuals in units of each target's median flux (saturated line
         # cores would dominate relative residuals)
@@ -273,22 +283,28 @@ class InverseProblem:
         self.evaluations = 0
 
     def evaluate(self, delta: np.ndarray) -> List[object]:
-        """Fluxes (torch, F_lambda) of every spectrum at offsets delta[n][20]."""
+        """Fluxes (torch, F_lambda) of every spectrum at offsets delta[n][20];
+        self.valid gets each spectrum's pixels regular in target and model."""
         torch = _torch()
         self.batch_resp.write(delta, self.pop, self.rho, self.txnxn, self.coeff_scat)
         self.plan.run()
         self.evaluations += 1
         torch.cuda.synchronize()
+        self.valid = [(it["iters"] < 1000) & tr for it, tr in zip(self.plan.items, self.t_regular)]
         return [it["flux"].clone() for it in self.plan.items]
 
     def residuals(self, flux) -> List[object]:
+        """Full residual vectors (F - F_target) / median(F_target); the
+        caller masks them with evaluate()'s valid pixels."""
         return [(f - t) / s for f, t, s in zip(flux, self.f_target, self.f_scale)]
 
     @staticmethod
-    def _costs(res) -> np.ndarray:
-        """Sum of squared residuals per spectrum (one device sync)."""
+    def _costs(res, masks=None) -> np.ndarray:
+        """Sum of squared residuals per spectrum over masks (one device sync)."""
         torch = _torch()
-        return torch.stack([(r * r).sum() for r in res]).cpu().numpy()
+        if masks is None:
+            return torch.stack([(r * r).sum() for r in res]).cpu().numpy()
+        return torch.stack([(r * r * m).sum() for r, m in zip(res, masks)]).cpu().numpy()
 
     def solve(self, max_iter: int = 12, step: float = 1e-3, tol: float = 1e-6, rcond: float = 1e-6,
               cfloor: float = 1e-3, line_floor: float = 2e-3, max_tries: int = 6, max_step: float = 0.5,
@@ -296,9 +312,11 @@ class InverseProblem:
         """Gauss-Newton from zero offsets, with Levenberg damping on failure.
 
         Per iteration: a forward-difference Jacobian (one batched evaluation
-        per abundance) and, per spectrum, the SVD of its columns; the step is
+        per abundance, over the pixels valid at the point and at every step)
+        and, per spectrum, the SVD of its columns; the step is
         -V diag(s / (s^2 + mu s_0^2)) U^T r (mu = 0 first: plain Gauss-Newton).
-        A spectrum keeps its step if the misfit drops (mu / 10), else
+        A spectrum keeps its step if the misfit over the pixels valid at both
+        points drops (mu / 10), else
         mu = max(10 mu, 1e-6) and a new trial with the same Jacobian.
         Hydrogen is the reference and stays at zero.  The line elements are
         fitted when their column clears line_floor (in median-flux units per
@@ -312,13 +330,16 @@ class InverseProblem:
         delta = np.zeros((n, nz))
         t0 = time.perf_counter()
         res = self.residuals(self.evaluate(delta))
-        cost = self._costs(res)
+        valid = list(self.valid)
+        cost = self._costs(res, valid)
         history = [float(cost.sum())]
         mu = np.zeros(n)
         active = cost > 0
         sens = np.zeros((n, nz))
         it = 0
         for it in range(1, max_iter + 1):
+            # Jacobian columns over the pixels valid at delta and at every step
+            common = [v.clone() for v in valid]
             cols = []
             for k in range(nz):
                 if k == self.h_index:
@@ -327,22 +348,25 @@ class InverseProblem:
                 dk = delta.copy()
                 dk[:, k] += step
                 rk = self.residuals(self.evaluate(dk))
+                for i in range(n):
+                    common[i] &= self.valid[i]
                 cols.append([(rk[i] - res[i]) / step for i in range(n)])
             svd = []
             for i in range(n):
+                m = common[i].to(torch.float64)
                 idx = [k for k in range(nz) if cols[k] is not None]
-                J = torch.stack([cols[k][i] for k in idx], dim=1)              # [W][nz - 1]
+                J = torch.stack([cols[k][i] for k in idx], dim=1) * m[:, None]  # [W][nz - 1]
                 c = torch.linalg.vector_norm(J, dim=0)
                 cn = c.cpu().numpy()
                 live = (cn > cfloor * cn.max()) | ((cn > line_floor) & self.line_mask[idx])
-                sens[i, idx] = c.cpu().numpy()
+                sens[i, idx] = cn
                 idx = [k for k, lv in zip(idx, live) if lv]
                 if not idx:
                     svd.append(None)
                     continue
                 u, s_, vh = torch.linalg.svd(J[:, torch.tensor(np.nonzero(live)[0], device=J.device)],
                                              full_matrices=False)
-                svd.append((idx, s_.cpu().numpy(), vh.T.cpu().numpy(), (u.T @ res[i]).cpu().numpy()))
+                svd.append((idx, s_.cpu().numpy(), vh.T.cpu().numpy(), (u.T @ (res[i] * m)).cpu().numpy()))
             improved = np.zeros(n, bool)
             for _ in range(max_tries):
                 dstep = np.zeros((n, nz))
@@ -360,13 +384,18 @@ class InverseProblem:
                 trial = np.clip(delta + dstep, -bound, bound)
                 dstep = trial - delta
                 rt = self.residuals(self.evaluate(trial))
-                ct = self._costs(rt)
+                vt = self.valid
+                both = [cm & v for cm, v in zip(common, vt)]  # compare on pixels valid at both points
+                ct = self._costs(rt, both)
+                c0 = self._costs(res, both)
+                cnew = self._costs(rt, vt)
                 for i in np.nonzero(active & ~improved)[0]:
-                    if ct[i] < cost[i]:
+                    if ct[i] < c0[i]:
                         improved[i] = True
                         delta[i] = trial[i]
                         res[i] = rt[i]
-                        cost[i] = ct[i]
+                        valid[i] = vt[i]
+                        cost[i] = cnew[i]
                         mu[i] = mu[i] / 10.0 if mu[i] > 1e-9 else 0.0
                         if float(np.abs(dstep[i]).max()) < tol:
                             active[i] = False
@@ -425,7 +454,8 @@ def fp64_plain_check(targets: Sequence[Target], delta: np.ndarray, indices: Opti
     """Max relative difference between each target and the FP64 plain path
     (SpectrumPlan from a host-built bundle) at the recovered offsets, over the
     pixels with |F| > 1e-3 median(F) (the robust mask of the reference's
-    tools/compare_spectra.py:128-135)."""
+    tools/compare_spectra.py:128-135) where neither takes the label-401
+    branch (the fit's pixels)."""
     from .device import SpectrumPlan
 
     worst = 0.0
@@ -435,6 +465,6 @@ def fp64_plain_check(targets: Sequence[Target], delta: np.ndarray, indices: Opti
         plan = SpectrumPlan(atm, t.catalog, t.wl_start, t.wl_end, resolution, cutoff=cutoff)
         plan.run()
         f = plan.flux.cpu().numpy()
-        m = np.abs(t.flux) > 1e-3 * np.median(np.abs(t.flux))
+        m = (np.abs(t.flux) > 1e-3 * np.median(np.abs(t.flux))) & ~t.label401 & (plan.iters.cpu().numpy() < 1000)
         worst = max(worst, float(np.max(np.abs(f[m] - t.flux[m]) / np.abs(t.flux[m]))))
     return worst

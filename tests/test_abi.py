@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
     assert set(declared_functions()) <= set(_native.SYMBOLS)
     L = _native.load_library()
-    assert L.synthe_abi_version() == 1
+    assert L.synthe_abi_version() == 2
 
 
 def test_host_grid_and_error_path():

@@ -31,7 +31,7 @@ def test_gauss_newton_recovers_line_elements(small_problem):
     targets, prob, sol = small_problem
     rep = report(targets, sol["delta"], sol["sensitivity"])
     print(sol["iterations"], sol["evaluations"], sol["cost"], rep)
-    c0 = np.array([float((r * r).sum()) for r in prob.residuals(prob.evaluate(np.zeros_like(sol["delta"])))])
+    c0 = prob._costs(prob.residuals(prob.evaluate(np.zeros_like(sol["delta"]))), prob.valid)
     assert np.all(sol["cost"] < 1e-6 * c0)
     assert rep["line_pairs_constrained"] >= 5
     assert rep["max_err_constrained_line_dex"] < 2e-3
