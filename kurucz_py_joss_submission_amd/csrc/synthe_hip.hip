@@ -841,12 +841,13 @@ __device__ __forceinline__ double far_quotient(double x, double b) {
 }
 
 // waves per SIMD the gather's register budget is sized for: the FP64 form's
-// 24 expansion moments need ~100 VGPRs, the FP32 form's 12 fewer
+// 24 expansion moments need ~100 VGPRs; the FP32 form fits more waves but
+// gains nothing from them (its tiles thrash L2 at 7 waves: 46 ms against 26)
 #ifndef SYNTHE_GATHER_WAVES
 #define SYNTHE_GATHER_WAVES 5
 #endif
 #ifndef SYNTHE_GATHER_WAVES32
-#define SYNTHE_GATHER_WAVES32 7
+#define SYNTHE_GATHER_WAVES32 5
 #endif
 template <int kPrec>
 constexpr int gather_waves() {
