@@ -893,14 +893,26 @@ __device__ __forceinline__ bool far_interior(const WingHead& h, int t0i, int t1i
 //  1. entries whose far zone covers the tile at |K| >= 4.5 h add their
 //     expansion moments (one entry per lane); one wave reduction, then each
 //     pixel evaluates the expansion (Horner in u/h);
-//  2. the other entries that reach the tile are taken one at a time, their
-//     heads broadcast from the lane that loaded them: where the tile lies in
+//     Pass 1 also queues, in LDS, the list positions of the other entries
+//     that reach the tile (about one in seven visits), in the order met;
+//  2. the queued entries are taken one at a time, 64 heads per gathered load,
+//     each broadcast from the lane that loaded it: where the tile lies in
 //     one far wing every lane adds X / k^2 branch-free, its pixels sharing one
 //     reciprocal (far_chunk); on the tiles holding a wing's N10DOP or MAXSTEP
-//     edge, pixels outside the far zone add +0.
+//     edge, pixels outside the far zone add +0.  If the queue fills, pass 1
+//     notes where, and pass 2 rescans the class windows from that step on, so
+//     the order of the exact terms is the same either way.
 // Each pixel's sum has a fixed order (Voigt zones, centres, expansions, exact
 // far wings by class and centre), no atomics, so results are run-to-run
 // identical.
+// List positions (relative to the depth's lists) of one tile's exactly
+// evaluated entries: 7 KiB of LDS per wave, so the 20 waves a CU holds at
+// five per SIMD fit in its 160 KiB.
+#ifndef SYNTHE_GATHER_QUEUE
+#define SYNTHE_GATHER_QUEUE 1792
+#endif
+constexpr int kGatherQueue = SYNTHE_GATHER_QUEUE;
+
 template <bool kStats, int kPrec, int kWP>
 __device__ __forceinline__ void gather_tile(const GatherArgs& g, int d, long t0, int lane,
                                             unsigned long long* __restrict__ evals) {
@@ -951,6 +963,13 @@ __device__ __forceinline__ void gather_tile(const GatherArgs& g, int d, long t0,
   none.xs = 0.0;
   none.cd = 0.0;
 
+  __shared__ uint32_t s_exact[kGatherQueue];
+  const long base = regions[(long)d * (kClasses + 1)];  // the depth's lists start here
+  const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  int qn = 0;                    // queued entries (wave-uniform)
+  int spill_kk = kClasses;       // the class and step at which the queue filled
+  long spill_j0 = 0;
+
   // ---- pass 1: expansion moments m_n = sum X/K^2 (-h/K)^n
   {
     constexpr int kT = kPrec == kPrecFP32 ? kFarTerms32 : kFarTerms;
@@ -965,6 +984,18 @@ __device__ __forceinline__ void gather_tile(const GatherArgs& g, int d, long t0,
         const WingHead h = j < hi ? list_cm[j] : none;
         const double K = wm - h.cd;
         const bool expand = far_interior(h, t0i, t1i) && fabs(K) >= kmin && h.xs >= 0.0;
+        const bool exact = !(t1i < h.lo || t0i > h.hi) && !expand;
+        const unsigned long long em = __ballot(exact);
+        if (em != 0ull && spill_kk == kClasses) {
+          const int ne = __popcll(em);
+          if (qn + ne <= kGatherQueue) {
+            if (exact) s_exact[qn + __popcll(em & below)] = (uint32_t)(j - base);
+            qn += ne;
+          } else {
+            spill_kk = kk;
+            spill_j0 = j0;
+          }
+        }
         const double x = expand ? h.xs : 0.0;
         const double kq = expand ? K : kmin;
         double y = __builtin_amdgcn_rcp(kq);  // 1/K to an ulp: two Newton steps
@@ -1007,76 +1038,86 @@ __device__ __forceinline__ void gather_tile(const GatherArgs& g, int d, long t0,
   double edge[kWP];  // edge tiles' terms, kept apart so each sum updates in place
 #pragma unroll
   for (int p = 0; p < kWP; ++p) edge[p] = 0.0;
-  for (int kk = 0; kk < kClasses; ++kk) {
-    const long lo = uniform_long(__shfl(bound, 2 * kk));
+  // every entry of `todo`, in lane order, with its head in lane src's hv
+  auto take = [&](const WingHead& hv, unsigned long long todo) {
+    while (todo) {
+      const int src = __builtin_ctzll(todo);
+      todo &= todo - 1;
+      WingHead h;  // the entry's head, from the lane that loaded it
+      h.lo = __builtin_amdgcn_readlane(hv.lo, src);
+      h.hi = __builtin_amdgcn_readlane(hv.hi, src);
+      h.nlo = __builtin_amdgcn_readlane(hv.nlo, src);
+      h.nhi = __builtin_amdgcn_readlane(hv.nhi, src);
+      {
+        const long long xb = __double_as_longlong(hv.xs);
+        const long long cb = __double_as_longlong(hv.cd);
+        const int xlo = __builtin_amdgcn_readlane((int)(xb & 0xffffffff), src);
+        const int xhi = __builtin_amdgcn_readlane((int)(xb >> 32), src);
+        const int clo = __builtin_amdgcn_readlane((int)(cb & 0xffffffff), src);
+        const int chi = __builtin_amdgcn_readlane((int)(cb >> 32), src);
+        h.xs = __longlong_as_double((long long)(((unsigned long long)(unsigned)xhi << 32) | (unsigned)xlo));
+        h.cd = __longlong_as_double((long long)(((unsigned long long)(unsigned)chi << 32) | (unsigned)clo));
+      }
+      double kd = kdl - h.cd;  // offset of the lane's first pixel from the centre
+      __asm__("" : "+v"(kd));  // keep k = kd + p in double (no per-pixel int conversion)
+      const bool interior = far_interior(h, t0i, t1i);
+      const bool chunk = __double_as_longlong(h.xs) >= 0;
+      const int n10 = (h.nhi - h.nlo) >> 1, M = (h.hi - h.lo) >> 1;
+      if (chunk && interior) {  // |k| >= N10DOP + 1 >= 1 on every pixel of the tile
+#pragma unroll
+        for (int p0 = 0; p0 < kWP; p0 += kChunk)
+          far_chunk_acc<kChunk, kNewton>(h.xs, kd, p0, acc + p0);
+      } else {
+        double add[kWP];
+        if (chunk) {
+          const double n10d = (double)n10, md = (double)M;
+#pragma unroll
+          for (int p0 = 0; p0 < kWP; p0 += kChunkEdge)
+            far_chunk<kChunkEdge, kNewton, true>(h.xs, kd, p0, n10d, md, add + p0);
+        } else {  // X below 2^-500: one quotient per pixel
+          const double n10d = (double)n10, md = (double)M, x = -h.xs;
+#pragma unroll
+          for (int p = 0; p < kWP; ++p) {
+            const double k = kd + (double)p;
+            const double xp = (fabs(k) > n10d && fabs(k) <= md) ? x : 0.0;
+            add[p] = far_quotient<kPrec>(xp, fmax(k * k, 1.0));
+          }
+        }
+#pragma unroll
+        for (int p = 0; p < kWP; ++p) edge[p] += add[p];
+      }
+      if (kStats) {
+        if (interior) {
+          long nv = W - w0 < kWP ? W - w0 : kWP;
+          nev += nv > 0 ? nv : 0;
+        } else {
+          const int c = h.lo + M;
+          for (int p = 0; p < kWP; ++p) {
+            const int ka = abs((int)w0 - c + p);
+            if (ka > n10 && ka <= M && w0 + p < W) ++nev;
+          }
+        }
+      }
+    }
+  };
+  __syncthreads();  // the queue's writes before its reads (one wave per workgroup)
+  for (int i0 = 0; i0 < qn; i0 += 64) {
+    const int i = i0 + lane;
+    const WingHead hv = i < qn ? list_cm[base + s_exact[i]] : none;
+    take(hv, __ballot(i < qn));
+  }
+  for (int kk = spill_kk; kk < kClasses; ++kk) {  // only when the queue filled
+    const long lo = kk == spill_kk ? spill_j0 : uniform_long(__shfl(bound, 2 * kk));
     const long hi = uniform_long(__shfl(bound, 2 * kk + 1));
     for (long j0 = lo; j0 < hi; j0 += 64) {
       const long jl = j0 + lane;
       const WingHead hv = jl < hi ? list_cm[jl] : none;
       const bool reach = !(t1i < hv.lo || t0i > hv.hi);
       const bool expand = far_interior(hv, t0i, t1i) && fabs(wm - hv.cd) >= kmin && hv.xs >= 0.0;
-      unsigned long long todo = __ballot(reach && !expand);
-      while (todo) {
-        const int src = __builtin_ctzll(todo);
-        todo &= todo - 1;
-        WingHead h;  // the entry's head, from the lane that loaded it
-        h.lo = __builtin_amdgcn_readlane(hv.lo, src);
-        h.hi = __builtin_amdgcn_readlane(hv.hi, src);
-        h.nlo = __builtin_amdgcn_readlane(hv.nlo, src);
-        h.nhi = __builtin_amdgcn_readlane(hv.nhi, src);
-        {
-          const long long xb = __double_as_longlong(hv.xs);
-          const long long cb = __double_as_longlong(hv.cd);
-          const int xlo = __builtin_amdgcn_readlane((int)(xb & 0xffffffff), src);
-          const int xhi = __builtin_amdgcn_readlane((int)(xb >> 32), src);
-          const int clo = __builtin_amdgcn_readlane((int)(cb & 0xffffffff), src);
-          const int chi = __builtin_amdgcn_readlane((int)(cb >> 32), src);
-          h.xs = __longlong_as_double((long long)(((unsigned long long)(unsigned)xhi << 32) | (unsigned)xlo));
-          h.cd = __longlong_as_double((long long)(((unsigned long long)(unsigned)chi << 32) | (unsigned)clo));
-        }
-        double kd = kdl - h.cd;  // offset of the lane's first pixel from the centre
-        __asm__("" : "+v"(kd));  // keep k = kd + p in double (no per-pixel int conversion)
-        const bool interior = far_interior(h, t0i, t1i);
-        const bool chunk = __double_as_longlong(h.xs) >= 0;
-        const int n10 = (h.nhi - h.nlo) >> 1, M = (h.hi - h.lo) >> 1;
-        if (chunk && interior) {  // |k| >= N10DOP + 1 >= 1 on every pixel of the tile
-#pragma unroll
-          for (int p0 = 0; p0 < kWP; p0 += kChunk)
-            far_chunk_acc<kChunk, kNewton>(h.xs, kd, p0, acc + p0);
-        } else {
-          double add[kWP];
-          if (chunk) {
-            const double n10d = (double)n10, md = (double)M;
-#pragma unroll
-            for (int p0 = 0; p0 < kWP; p0 += kChunkEdge)
-              far_chunk<kChunkEdge, kNewton, true>(h.xs, kd, p0, n10d, md, add + p0);
-          } else {  // X below 2^-500: one quotient per pixel
-            const double n10d = (double)n10, md = (double)M, x = -h.xs;
-#pragma unroll
-            for (int p = 0; p < kWP; ++p) {
-              const double k = kd + (double)p;
-              const double xp = (fabs(k) > n10d && fabs(k) <= md) ? x : 0.0;
-              add[p] = far_quotient<kPrec>(xp, fmax(k * k, 1.0));
-            }
-          }
-#pragma unroll
-          for (int p = 0; p < kWP; ++p) edge[p] += add[p];
-        }
-        if (kStats) {
-          if (interior) {
-            long nv = W - w0 < kWP ? W - w0 : kWP;
-            nev += nv > 0 ? nv : 0;
-          } else {
-            const int c = h.lo + M;
-            for (int p = 0; p < kWP; ++p) {
-              const int ka = abs((int)w0 - c + p);
-              if (ka > n10 && ka <= M && w0 + p < W) ++nev;
-            }
-          }
-        }
-      }
+      take(hv, __ballot(reach && !expand));
     }
   }
+  __syncthreads();  // the queue's reads before a later tile's writes
 #pragma unroll
   for (int p = 0; p < kWP; ++p) acc[p] += edge[p];
   double T = temp[d] > 1.0 ? temp[d] : 1.0;
