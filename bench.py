@@ -215,20 +215,22 @@ def bench_batched(args, wk) -> dict:
                         "windows U(300,1700) nm + U(5,50) nm, shared 1,000,000-line list, R=300000"}
 
 
-def bench_adjoint(args) -> dict:
-    """The adjoint configuration: Teff 5750 K, log g 4.44, 72 layers, 50,000
-    lines over 400-420 nm.  Times the differentiable forward (FP32 far-wing
-    profiles, FP64 RT) and its backward (gradients of sum(w F) with respect to
-    every line's log gf and log gamma_vdW, the 72 layer temperatures and the
-    20 abundances) beside the plain FP64 forward of the same spectrum."""
+def bench_adjoint(args, name: str = "c4", wk=None) -> dict:
+    """The adjoint configuration (c4): Teff 5750 K, log g 4.44, 72 layers,
+    50,000 lines over 400-420 nm; or the same on the full window (c2: 1,000,000
+    lines over 300-1800 nm, Teff 4000 K, log g 5.0).  Times the
+    differentiable forward (FP32 far-wing profiles, FP64 RT) and its backward
+    (gradients of sum(w F) with respect to every line's log gf and log
+    gamma_vdW, the 72 layer temperatures and the 20 abundances) beside the
+    plain FP64 forward of the same spectrum."""
     import torch
 
     from kurucz_py_joss_submission_amd import synthetic as S
     from kurucz_py_joss_submission_amd.adjoint import SpectrumModel
     from kurucz_py_joss_submission_amd.device import SpectrumPlan
 
-    cfg = CONFIGS["c4"]
-    wk = make_workload(cfg)
+    cfg = CONFIGS[name]
+    wk = wk if wk is not None else make_workload(cfg)
     cols = S.make_atm(cfg["teff"], cfg["logg"], 72, 0.0)
     m = SpectrumModel(cols, wk.catalog, cfg["wl0"], cfg["wl1"], precision="fp32")
     plan = SpectrumPlan(S.build_bundle(cols), wk.catalog, cfg["wl0"], cfg["wl1"], 300000.0)
@@ -254,10 +256,15 @@ def bench_adjoint(args) -> dict:
             b_ms.append(ev[1].elapsed_time(ev[2]))
             p_ms.append(ev[2].elapsed_time(ev[3]))
     fm, bm, pm = float(np.median(f_ms)), float(np.median(b_ms)), float(np.median(p_ms))
-    log(f"[bench] adjoint c4: forward {fm:.2f} ms, backward {bm:.2f} ms, plain FP64 forward {pm:.2f} ms")
+    m_L, m_W = m.L, m.W
+    log(f"[bench] adjoint {name}: forward {fm:.2f} ms, backward {bm:.2f} ms, plain FP64 forward {pm:.2f} ms")
+    del m, plan
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
     return {"forward_ms": fm, "backward_ms": bm, "forward_plus_adjoint_ms": fm + bm,
-            "plain_fp64_forward_ms": pm, "n_lines": m.L, "n_wavelengths": m.W,
-            "gradients": {"log_gf": m.L, "log_gamma_vdw": m.L, "temperature": 72, "abundances": 20},
+            "plain_fp64_forward_ms": pm, "forward_plus_adjoint_over_plain": (fm + bm) / pm,
+            "n_lines": m_L, "n_wavelengths": m_W,
+            "gradients": {"log_gf": m_L, "log_gamma_vdw": m_L, "temperature": 72, "abundances": 20},
             "workload": cfg["desc"]}
 
 
@@ -363,7 +370,7 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-atmosphere batched variant")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint variant")
     ap.add_argument("--no-inverse", action="store_true", help="skip the batched inverse-problem variant")
-    ap.add_argument("--variant", choices=("batched", "adjoint", "inverse"),
+    ap.add_argument("--variant", choices=("batched", "adjoint", "adjoint_c2", "inverse"),
                     help="run only this variant and print its figures as JSON (for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain device copy of 1 GiB (3 times), to scale HBM counters")
@@ -386,6 +393,8 @@ def main() -> int:
     if args.variant:
         if args.variant == "batched":
             res = bench_batched(args, make_workload(CONFIGS["c2"]))
+        elif args.variant == "adjoint_c2":
+            res = bench_adjoint(args, "c2")
         else:
             res = (bench_adjoint if args.variant == "adjoint" else bench_inverse)(args)
         print(json.dumps({"variant": args.variant, **res}), flush=True)
@@ -449,6 +458,7 @@ def main() -> int:
         variants["batched_c3"] = bench_batched(args, wk)
     if args.config == "c2" and not args.no_adjoint:
         variants["adjoint_c4"] = bench_adjoint(args)
+        variants["adjoint_c2"] = bench_adjoint(args, "c2", wk)
     if args.config == "c2" and not args.no_inverse:
         variants["inverse_c5"] = bench_inverse(args)
 
