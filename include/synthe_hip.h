@@ -161,6 +161,41 @@ int synthe_transp_asynth(synthe_ctx *ctx, const synthe_lines *lines, const synth
                          const int64_t *wing_order, double *transp, uint8_t *valid, double *asynth,
                          double *stats, synthe_stream stream);
 
+/* Hydrogen lines (HPROF4 profiles), engine/opacity.py:1706-1884
+ * (_compute_hydrogen_line_opacity) with :1547-1704 (_accumulate_hydrogen_profile)
+ * and physics/profiles/hydrogen.py:1541-1709 (hydrogen_line_profile); the
+ * result AHLINE joins ASYNTH as engine/opacity.py:2483-2487 adds it before the
+ * FSCAT split.  All host arrays. */
+typedef struct {
+  int64_t n_lines;          /* the catalog's line type -1/-2, ion stage 1 lines, catalog order */
+  const double *wavelength; /* [n] the record's wavelength, nm */
+  const int64_t *center;    /* [n] nearest grid index of the index wavelength; -1 / n_wl off the grid */
+  const double *gf;         /* [n] linear gf */
+  const double *elo;        /* [n] lower level energy, cm^-1 */
+  const int32_t *n_lower;   /* [n] NBLO as the catalog holds it */
+  const int32_t *n_upper;   /* [n] NBUP */
+} synthe_hlines;
+
+typedef struct {
+  int32_t n_depths;
+  const double *temperature, *electron_density, *mass_density, *hckt; /* [D] */
+  const double *pop_h1, *dop_h1;       /* [D] H I population and Doppler width (population_per_ion[:, 0, 0]) */
+  const double *xnf_he1, *xnf_h2;      /* [D] or NULL; the hydrogen state exists when either is given */
+  const double *xnfph;                 /* [D][2] or NULL (zeros) */
+  const double *dopph;                 /* [D] or NULL: thermal H + turbulent widths */
+  const double *turbulent_velocity;    /* [D] cm/s or NULL (for the dopph default) */
+} synthe_hdepths;
+
+/* Adds AHLINE (post-STIM, [D][W]) into asynth (device, [D][W]): per pixel
+ * the lines' terms in catalog order, summed from zero, then added.  cont
+ * (device [D][W]) is the total continuum (absorption + scattering) the
+ * reference's cutoffs compare with; wl (device [W]) the grid.  Returns
+ * SYNTHE_ERANGE (nothing added) if a line's n_upper + 2 exceeds the
+ * radiative-damping table (96 levels), where the reference raises. */
+int synthe_hydrogen_lines(synthe_ctx *ctx, const synthe_hlines *lines, const synthe_hdepths *depths,
+                          double microturb_kms, double cutoff, long n_wl, const double *wl,
+                          const double *cont, double *asynth, synthe_stream stream);
+
 /* engine/radiative.py:282-494 (solve_lte_spectrum) with
  * physics/josh_solver.py:367-767 (solve_josh_flux) per wavelength: the
  * continuum-only and the total JOSH solves.  All [D][W] except wl [W],

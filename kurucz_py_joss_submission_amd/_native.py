@@ -28,7 +28,7 @@ SYMBOLS = (
     "synthe_flux_to_lambda", "synthe_diag_div_pos", "synthe_set_kernel_timing",
     "synthe_kernel_times", "synthe_set_profile_precision", "synthe_batch_create", "synthe_batch_run",
     "synthe_batch_info", "synthe_batch_destroy", "synthe_asynth_vjp", "synthe_lte_vjp",
-    "synthe_lte_spectrum_noalloc", "synthe_transp_asynth",
+    "synthe_lte_spectrum_noalloc", "synthe_transp_asynth", "synthe_hydrogen_lines",
 )
 
 SYNTHE_BATCH_AUTOTUNE = 1
@@ -66,6 +66,19 @@ class Depths(ct.Structure):
     _fields_ = [("n_depths", ct.c_long), ("temperature", _VP), ("rhox", _VP), ("rho", _VP),
                 ("xne", _VP), ("txnxn", _VP), ("hckt", _VP), ("pop", _VP), ("dop", _VP),
                 ("n_ion", ct.c_long), ("n_elem", ct.c_long)]
+
+
+class HLines(ct.Structure):
+    """synthe_hlines (host arrays)."""
+    _fields_ = [("n_lines", ct.c_int64), ("wavelength", _VP), ("center", _VP), ("gf", _VP), ("elo", _VP),
+                ("n_lower", _VP), ("n_upper", _VP)]
+
+
+class HDepths(ct.Structure):
+    """synthe_hdepths (host arrays)."""
+    _fields_ = [("n_depths", ct.c_int32), ("temperature", _VP), ("electron_density", _VP), ("mass_density", _VP),
+                ("hckt", _VP), ("pop_h1", _VP), ("dop_h1", _VP), ("xnf_he1", _VP), ("xnf_h2", _VP),
+                ("xnfph", _VP), ("dopph", _VP), ("turbulent_velocity", _VP)]
 
 
 class Instance(ct.Structure):
@@ -109,6 +122,8 @@ def load_library(path: Optional[Path] = None):
                                       _VP, _VP, ct.c_double, _VP, _VP, _VP, _VP, _VP, _VP]
         lib.synthe_transp_asynth.argtypes = [_VP, ct.POINTER(Lines), ct.POINTER(Depths), ct.c_long, _VP, _VP,
                                              ct.c_double, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]
+        lib.synthe_hydrogen_lines.argtypes = [_VP, ct.POINTER(HLines), ct.POINTER(HDepths), ct.c_double,
+                                              ct.c_double, ct.c_long, _VP, _VP, _VP, _VP]
         lib.synthe_lte_spectrum.argtypes = [_VP, ct.c_long, ct.c_long] + [_VP] * 12
         lib.synthe_lte_spectrum_noalloc.argtypes = [_VP, ct.c_long, ct.c_long] + [_VP] * 12
         lib.synthe_josh_flux_batch.argtypes = [_VP, ct.c_long, ct.c_long] + [_VP] * 10

@@ -52,6 +52,8 @@ class AtmosphereModel:
     xnf_h: Optional[np.ndarray] = None
     xnf_he1: Optional[np.ndarray] = None
     xnf_h2: Optional[np.ndarray] = None
+    xnfph: Optional[np.ndarray] = None  # [D][2] H I and H II populations (hydrogen-line broadening)
+    dopph: Optional[np.ndarray] = None  # [D] hydrogen Doppler width / c
     population_per_ion: Optional[np.ndarray] = None
     doppler_per_ion: Optional[np.ndarray] = None
     continuum_wledge: Optional[np.ndarray] = None
@@ -86,7 +88,7 @@ class AtmosphereModel:
         }
         opt = {
             "hckt": self.hckt, "txnxn": self.txnxn, "xnf_h": self.xnf_h,
-            "xnf_he1": self.xnf_he1, "xnf_h2": self.xnf_h2,
+            "xnf_he1": self.xnf_he1, "xnf_h2": self.xnf_h2, "xnfph": self.xnfph, "dopph": self.dopph,
             "population_per_ion": self.population_per_ion,
             "doppler_per_ion": self.doppler_per_ion, "wledge": self.continuum_wledge,
             "half_edge": self.continuum_half_edge, "delta_edge": self.continuum_delta_edge,
@@ -148,6 +150,25 @@ def depth_xne(atm) -> np.ndarray:
     return np.zeros(atm.temperature.size) if v is None else v
 
 
+
+def _xnfph(xnfph, qxnfpel):
+    """XNFPH [D][2] (H I, H II) as io/atmosphere.py:157-161 reads it: the
+    archive's own, else the first two columns of the legacy QXNFPEL."""
+    if xnfph is not None:
+        return np.asarray(xnfph, np.float64)
+    if qxnfpel is not None and qxnfpel.ndim == 2 and qxnfpel.shape[1] >= 2:
+        return np.ascontiguousarray(qxnfpel[:, :2], np.float64)
+    return None
+
+
+def _dopph(dopph, qdopple):
+    """DOPPH [D] as io/atmosphere.py:163-167 reads it (QDOPPLE's first column)."""
+    if dopph is not None:
+        return np.asarray(dopph, np.float64)
+    if qdopple is not None and qdopple.ndim == 2:
+        return np.ascontiguousarray(qdopple[:, 0], np.float64)
+    return None
+
 def load_cached(path: Path) -> AtmosphereModel:
     """Load a bundle ``.npz`` (io/atmosphere.py:82-506, hot-path keys only)."""
     with np.load(path, allow_pickle=False) as data:
@@ -172,6 +193,8 @@ def load_cached(path: Path) -> AtmosphereModel:
             xnf_h=pick("xnf_h", "qxnf_h", required=False),
             xnf_he1=pick("xnf_he1", required=False),
             xnf_h2=pick("xnf_h2", "qxnf_h2", required=False),
+            xnfph=_xnfph(pick("xnfph", required=False), pick("qxnfpel", required=False)),
+            dopph=_dopph(pick("dopph", required=False), pick("qdopple", required=False)),
             population_per_ion=pick("population_per_ion", required=False),
             doppler_per_ion=pick("doppler_per_ion", required=False),
             continuum_wledge=pick("wledge", required=False),

@@ -16,10 +16,11 @@ NPZ = HERE.parent / "data" / "kurucz_tables.npz"
 OUT = HERE / "_tables_gen.h"
 
 
-def _arr(name: str, a: np.ndarray) -> str:
+def _arr(name: str, a: np.ndarray, device: bool = False) -> str:
     vals = ",\n  ".join(", ".join(f"{float(v):.17g}" for v in a.ravel()[i:i + 4])
                         for i in range(0, a.size, 4))
-    return f"static const double {name}[{a.size}] = {{\n  {vals}\n}};\n"
+    q = "static __device__ const double" if device else "static const double"
+    return f"{q} {name}[{a.size}] = {{\n  {vals}\n}};\n"
 
 
 def main(out: Path = OUT) -> Path:
@@ -29,6 +30,13 @@ def main(out: Path = OUT) -> Path:
     for key, name in (("xtau", "kXtau"), ("ch", "kCh"), ("coefj", "kCoefj"), ("tabvi", "kTabvi"),
                       ("tabh1", "kTabh1")):
         parts.append(_arr(name, np.asarray(t[key], dtype=np.float64)))
+    # hydrogen (HPROF4) tables, row-major as the archive holds them: host
+    # copies for the per-line constants, device copies for the per-pixel profile
+    for key in ("h_stalph", "h_stwtal", "h_istal", "h_lnghal", "h_stcomp", "h_stcpwt", "h_lncomp", "h_asum_lyman",
+                "h_asum", "h_y1wtm", "h_xknmtb", "conth", "ehyd_cm"):
+        parts.append(_arr("kH_" + key.removeprefix("h_"), np.asarray(t[key], dtype=np.float64)))
+    for key in ("h_propbm", "h_c", "h_d", "h_pp", "h_beta_grid", "h_cutoff_h2", "h_cutoff_h2_plus"):
+        parts.append(_arr("kD_" + key.removeprefix("h_"), np.asarray(t[key], dtype=np.float64), device=True))
     text = "\n".join(parts)
     if not out.exists() or out.read_text() != text:
         out.write_text(text)

@@ -29,6 +29,7 @@
 #include "near_queue.h"
 #include "rt_core.h"
 #include "synthe_device.h"
+#include "hydrogen.h"
 #include "synthe_hip.h"
 
 using namespace synthe;
@@ -1764,6 +1765,8 @@ struct synthe_ctx {
   void* ws = nullptr;
   size_t ws_size = 0;
   void* scratch = nullptr;  // device address of the static RT scratch on this device
+  void* hstage = nullptr;   // hydrogen-line staging (synthe_hydrogen_lines), freed by hstage_free
+  void (*hstage_free)(void*) = nullptr;
   // kernel timing: event pairs around each wing-gather launch while enabled
   static constexpr int kTimedMax = 256;
   int precision = SYNTHE_PREC_FP64;  // profile precision of synthe_asynth
@@ -2722,6 +2725,7 @@ int synthe_destroy(synthe_ctx* ctx) {
   if (ctx->d_tab) (void)hipFree(ctx->d_tab);
   if (ctx->d_tab32) (void)hipFree(ctx->d_tab32);
   if (ctx->ws) (void)hipFree(ctx->ws);
+  if (ctx->hstage_free) ctx->hstage_free(ctx->hstage);
   for (int i = 0; i < synthe_ctx::kTimedMax; ++i) {
     if (ctx->ev_beg[i]) (void)hipEventDestroy(ctx->ev_beg[i]);
     if (ctx->ev_end[i]) (void)hipEventDestroy(ctx->ev_end[i]);
@@ -2991,6 +2995,316 @@ int synthe_flux_to_lambda(synthe_ctx* ctx, long n_wl, const double* wl, double* 
   if (n_wl <= 0) return n_wl == 0 ? SYNTHE_OK : fail(SYNTHE_EINVAL, "synthe_flux_to_lambda: bad size");
   hipLaunchKernelGGL(k_flux_to_lambda, dim3(blocks_for(n_wl, kBlock)), dim3(kBlock), 0, (hipStream_t)stream,
                      n_wl, wl, flux);
+  HIP_TRY(hipGetLastError());
+  return SYNTHE_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ hydrogen lines
+namespace {
+
+// One hydrogen line's depth-independent quantities (engine/opacity.py:1768-1819)
+// and its three transitions: n_lower -> n_upper and the neighbours the
+// wing cut-offs compare with (n_upper - 2 on the red side, + 2 on the blue).
+struct HLine {
+  hyd::Trans main, minus2, plus2;
+  double wl, cgf, elo, wlminus1, wlminus2, wlplus1, wlplus2, redcut, bluecut, wshift, conth_val;
+  long c;
+  int simple;  // n_upper <= n_lower + 2: the plain walk (labels 620/630)
+};
+
+// A (line, depth) pair after the gates: its KAPPA0, WCON/WTAIL and how far
+// each wing's walk ran (the last offset it passed; 0 for none).
+struct HPair {
+  double kappa0, wcon, wtail;
+  int red_ext, blue_ext, center_add, active;
+};
+
+struct HDepthDev {
+  const double *pop, *dop, *rho, *hckt, *emerge_h, *state, *temp;
+};
+
+// The value a wing pixel adds, before the cut-off tests (opacity.py:1620-1700).
+__device__ __forceinline__ double h_value(const HLine& L, const HPair& pr, const hyd::Trans& t, double shift,
+                                          const double* st, double wave, double stim, bool taper,
+                                          const DeviceTables& tb) {
+  double v = pr.kappa0 * hyd::profile(t, st, wave - shift, tb.extab, tb.extabf) * stim;
+  if (taper && wave < pr.wtail) v *= (wave - pr.wcon) / (pr.wtail - pr.wcon);
+  return v;
+}
+
+// Gates, WCON/WTAIL and the extent of each wing's walk for one (line, depth)
+// pair (opacity.py:1820-1881 and 1547-1704).  One wave per pair; each walk
+// takes 64 offsets per step and stops at the first offset the reference
+// stops at, so the pixels it passes are exactly the reference's.
+__global__ __launch_bounds__(64) void k_h_extent(const HLine* __restrict__ lines, HDepthDev dp, int D, long W,
+                                                 const double* __restrict__ grid, const double* __restrict__ cont,
+                                                 double cutoff, DeviceTables tb, HPair* __restrict__ pairs) {
+  const int l = blockIdx.x, d = blockIdx.y, lane = threadIdx.x;
+  const HLine& L = lines[l];
+  HPair pr{0.0, 0.0, 0.0, 0, 0, 0, 0};
+  const double* st = dp.state + (long)d * hyd::kStateN;
+  const double pop_val = dp.pop[d], dop_val = dp.dop[d], r = dp.rho[d];
+  bool ok = !(pop_val <= 0.0 || dop_val <= 0.0 || r <= 0.0) && st[hyd::kHAS] != 0.0;
+  if (ok) {
+    const double xnfdop = pop_val / (r * dop_val);
+    const double boltz = fastex(L.elo * dp.hckt[d], tb.extab, tb.extabf);
+    const double kappa0_pre = L.cgf * xnfdop;
+    const long cc = L.c < 0 ? 0 : (L.c > W - 1 ? W - 1 : L.c);
+    const double kapmin = cont[(long)d * W + cc] * cutoff;
+    const double kappa0 = kappa0_pre * boltz;
+    ok = !(kappa0_pre < kapmin) && !(kappa0 < kapmin);
+    if (ok) {
+      double wmerge = 1.0e7 / (L.conth_val - dp.emerge_h[d]);
+      if (wmerge < 0.0) wmerge = L.wshift + L.wshift;
+      double wcon = L.wshift > wmerge ? L.wshift : wmerge;
+      double wtail = wcon > 0.0 ? 1.0e7 / (1.0e7 / wcon - 500.0) : wcon + wcon;
+      wcon = L.wshift + L.wshift < wcon ? L.wshift + L.wshift : wcon;
+      if (wtail < 0.0) wtail = wcon + wcon;
+      wtail = wcon + wcon < wtail ? wcon + wcon : wtail;
+      pr.kappa0 = kappa0;
+      pr.wcon = wcon;
+      pr.wtail = wtail;
+    }
+  }
+  if (!ok) {
+    if (lane == 0) pairs[(long)l * D + d] = pr;
+    return;
+  }
+  pr.active = 1;
+  const double temp = dp.temp[d];
+  const bool simple = L.simple != 0;
+  const bool taper = !simple && pr.wtail > pr.wcon;
+  const long c = L.c;
+  const long max_steps = c > W - c - 1 ? c : W - c - 1;
+  if (c >= 0 && c < W) {
+    const double wave = grid[c];
+    if (!(!simple && wave < pr.wcon)) {
+      const double v = h_value(L, pr, L.main, L.wl, st, wave, planck_terms(wave, temp).stim, taper, tb);
+      pr.center_add = v >= cont[(long)d * W + c] * cutoff;
+    }
+  }
+  // red wing
+  long ext = 0;
+  if (c < W) {
+    const long start = c < 0 ? (-c > 1 ? -c : 1) : 1;
+    for (long o0 = start;; o0 += 64) {
+      const long o = o0 + lane;
+      bool stop = o > max_steps;
+      if (!stop) {
+        const long idx = c + o;
+        if (idx >= W) {
+          stop = true;
+        } else {
+          const double wave = grid[idx];
+          const double cut = cont[(long)d * W + idx] * cutoff;
+          if (!simple) {
+            if (wave > L.wlminus1) {
+              stop = true;
+            } else if (!(wave < pr.wcon)) {
+              const double sv = planck_terms(wave, temp).stim;
+              double v = h_value(L, pr, L.main, L.wl, st, wave, sv, taper, tb);
+              if (wave > L.redcut && h_value(L, pr, L.minus2, L.wlminus2, st, wave, sv, taper, tb) >= v) v = 0.0;
+              stop = v <= 0.0 || v < cut;
+            }
+          } else {
+            const double v = h_value(L, pr, L.main, L.wl, st, wave, planck_terms(wave, temp).stim, false, tb);
+            stop = v <= 0.0 || v < cut;
+          }
+        }
+      }
+      const unsigned long long m = __ballot(stop);
+      if (m) {
+        ext = o0 + __builtin_ctzll(m) - 1;
+        break;
+      }
+    }
+  }
+  pr.red_ext = (int)ext;
+  // blue wing
+  ext = 0;
+  if (c >= 0) {
+    const long start = c >= W ? (c - (W - 1) > 1 ? c - (W - 1) : 1) : 1;
+    for (long o0 = start;; o0 += 64) {
+      const long o = o0 + lane;
+      bool stop = o > max_steps;
+      if (!stop) {
+        const long idx = c - o;
+        if (idx < 0) {
+          stop = true;
+        } else {
+          const double wave = grid[idx];
+          if (!simple && (wave < pr.wcon || wave < L.wlplus1)) {
+            stop = true;
+          } else {
+            const double sv = planck_terms(wave, temp).stim;
+            double v = h_value(L, pr, L.main, L.wl, st, wave, sv, taper, tb);
+            if (!simple && wave < L.bluecut && h_value(L, pr, L.plus2, L.wlplus2, st, wave, sv, taper, tb) >= v)
+              v = 0.0;
+            stop = v <= 0.0 || v < cont[(long)d * W + idx] * cutoff;
+          }
+        }
+      }
+      const unsigned long long m = __ballot(stop);
+      if (m) {
+        ext = o0 + __builtin_ctzll(m) - 1;
+        break;
+      }
+    }
+  }
+  pr.blue_ext = (int)ext;
+  if (lane == 0) pairs[(long)l * D + d] = pr;
+}
+
+// AHLINE per pixel: every active pair whose walk passed the pixel adds its
+// value, in catalog order, from zero; the sum joins ASYNTH.
+__global__ __launch_bounds__(256) void k_h_gather(const HLine* __restrict__ lines, const HPair* __restrict__ pairs,
+                                                  int NH, int D, long W, const double* __restrict__ grid,
+                                                  const double* __restrict__ temp, const double* __restrict__ state,
+                                                  DeviceTables tb, double* __restrict__ asynth) {
+  const long w = blockIdx.x * 256L + threadIdx.x;
+  const int d = blockIdx.y;
+  if (w >= W) return;
+  const double wave = grid[w];
+  const double sv = planck_terms(wave, temp[d]).stim;
+  const double* st = state + (long)d * hyd::kStateN;
+  double sum = 0.0;
+  for (int l = 0; l < NH; ++l) {
+    const HPair& pr = pairs[(long)l * D + d];
+    if (!pr.active) continue;
+    const HLine& L = lines[l];
+    const long o = w - L.c;
+    const bool simple = L.simple != 0;
+    const bool taper = !simple && pr.wtail > pr.wcon;
+    bool add;
+    if (o == 0) add = pr.center_add;
+    else if (o > 0) add = o <= pr.red_ext && !(!simple && wave < pr.wcon);
+    else add = -o <= pr.blue_ext;
+    if (add) sum += h_value(L, pr, L.main, L.wl, st, wave, sv, taper, tb);
+  }
+  asynth[(long)d * W + w] += sum;
+}
+
+// Host-staged inputs of one call, kept until the next call on the context.
+struct HydrogenStage {
+  std::vector<HLine> lines;
+  std::vector<double> depth;  // pop dop rho hckt emerge_h temp [D] each, then state [D][kStateN]
+  void* dev = nullptr;
+  size_t dev_size = 0;
+};
+void hstage_free(void* p) {
+  HydrogenStage* S = (HydrogenStage*)p;
+  if (S->dev) (void)hipFree(S->dev);
+  delete S;
+}
+
+HydrogenStage* hstage(synthe_ctx* ctx) {
+  if (!ctx->hstage) {
+    ctx->hstage = new HydrogenStage();
+    ctx->hstage_free = hstage_free;
+  }
+  return (HydrogenStage*)ctx->hstage;
+}
+
+}  // namespace
+
+extern "C" {
+
+int synthe_hydrogen_lines(synthe_ctx* ctx, const synthe_hlines* hl, const synthe_hdepths* hd, double microturb_kms,
+                          double cutoff, long n_wl, const double* wl, const double* cont, double* asynth,
+                          synthe_stream stream) {
+  const char* who = "synthe_hydrogen_lines";
+  if (!ctx || !hl || !hd || !wl || !cont || !asynth) return fail(SYNTHE_EINVAL, std::string(who) + ": NULL argument");
+  const long NH = hl->n_lines, D = hd->n_depths;
+  if (NH < 0 || D <= 0 || D > 65535 || n_wl <= 0 || n_wl > 0x7fffffffL)
+    return fail(SYNTHE_EINVAL, std::string(who) + ": bad sizes");
+  if (NH == 0) return SYNTHE_OK;
+  if (NH > 65535) return fail(SYNTHE_ERANGE, std::string(who) + ": more than 65535 hydrogen lines");
+  if (!hl->wavelength || !hl->center || !hl->gf || !hl->elo || !hl->n_lower || !hl->n_upper || !hd->temperature ||
+      !hd->electron_density || !hd->mass_density || !hd->hckt || !hd->pop_h1 || !hd->dop_h1)
+    return fail(SYNTHE_EINVAL, std::string(who) + ": NULL array");
+  hipStream_t st = (hipStream_t)stream;
+  HydrogenStage* S = hstage(ctx);
+  HIP_TRY(hipStreamSynchronize(st));  // the previous call's copies have left the staging vectors
+  S->lines.resize(NH);
+  for (long l = 0; l < NH; ++l) {  // engine/opacity.py:1768-1819
+    HLine& L = S->lines[l];
+    const int n_lower = hl->n_lower[l], n_upper = hl->n_upper[l];
+    const int nle = n_lower > 1 ? n_lower : 1;
+    const int nue = n_upper > n_lower + 1 ? n_upper : n_lower + 1;
+    const int um2 = nue - 2 > nle + 1 ? nue - 2 : nle + 1;
+    if (nue + 2 > 96 || !hyd::make_trans(nle, nue, &L.main) || !hyd::make_trans(nle, um2, &L.minus2) ||
+        !hyd::make_trans(nle, nue + 2, &L.plus2))
+      return fail(SYNTHE_ERANGE, std::string(who) + ": n_upper + 2 beyond the 96-level damping table (line " +
+                                     std::to_string(l) + ")");
+    const double line_wavelength = hl->wavelength[l];
+    L.wl = line_wavelength;
+    L.c = hl->center[l];
+    L.cgf = hyd::kCgf * hl->gf[l] / (hyd::kCNm / line_wavelength);
+    L.elo = hl->elo[l];
+    int ncon = n_lower < 15 ? n_lower : 15;
+    ncon = (ncon > 1 ? ncon : 1) - 1;
+    L.conth_val = kH_conth[ncon];
+    const double ehyd_lower = hyd::ehyd_cm(n_lower);
+    L.wlminus1 = n_upper - 1 > n_lower ? 1.0e7 / (hyd::ehyd_cm(n_upper - 1) - ehyd_lower) : line_wavelength;
+    L.wlminus2 = n_upper - 2 > n_lower ? 1.0e7 / (hyd::ehyd_cm(n_upper - 2) - ehyd_lower) : line_wavelength;
+    L.wlplus1 = 1.0e7 / (hyd::ehyd_cm(n_upper + 1) - ehyd_lower);
+    L.wlplus2 = 1.0e7 / (hyd::ehyd_cm(n_upper + 2) - ehyd_lower);
+    L.redcut = 1.0e7 / (kH_conth[0] - hyd::kRydCm / pow((double)n_upper - 0.8, 2.0) - ehyd_lower);
+    L.bluecut = 1.0e7 / (kH_conth[0] - hyd::kRydCm / pow((double)n_upper + 0.8, 2.0) - ehyd_lower);
+    L.wshift = 1.0e7 / (L.conth_val - hyd::kRydCm / pow(81.0, 2.0));
+    L.simple = nue <= nle + 2;
+  }
+  const double micro = microturb_kms > 0.0 ? microturb_kms / 299792.458 : 0.0;
+  S->depth.assign(D * (6 + hyd::kStateN), 0.0);
+  double* pop = S->depth.data();
+  double *dop = pop + D, *rho = dop + D, *hckt = rho + D, *emerge = hckt + D, *temp = emerge + D,
+         *state = temp + D;
+  const bool has = hd->xnf_he1 != nullptr || hd->xnf_h2 != nullptr;
+  for (long d = 0; d < D; ++d) {
+    pop[d] = hd->pop_h1[d];
+    const double dv = hd->dop_h1[d];
+    dop[d] = micro > 0.0 ? sqrt(dv * dv + micro * micro) : dv;
+    rho[d] = hd->mass_density[d];
+    hckt[d] = hd->hckt[d];
+    const double xne = hd->electron_density[d] > 1e-40 ? hd->electron_density[d] : 1e-40;
+    const double inglis = 1600.0 / pow(xne, 2.0 / 15.0);
+    const double nmerge = inglis - 1.5 > 1.0 ? inglis - 1.5 : 1.0;
+    const double nm2 = nmerge * nmerge;
+    emerge[d] = hyd::kRydCm / (nm2 > 1e-12 ? nm2 : 1e-12);
+    temp[d] = hd->temperature[d];
+    hyd::make_state(hd->temperature[d], hd->electron_density[d], hd->xnf_he1, hd->xnf_h2, hd->xnfph, hd->dopph,
+                    hd->turbulent_velocity, microturb_kms, has, (int)d, state + d * hyd::kStateN);
+  }
+  const size_t o_lines = 0, o_depth = align_up(NH * sizeof(HLine)),
+               o_pairs = o_depth + align_up(S->depth.size() * sizeof(double)),
+               total = o_pairs + align_up((size_t)NH * D * sizeof(HPair));
+  if (S->dev_size < total) {
+    if (S->dev) {
+      HIP_TRY(hipFree(S->dev));
+      S->dev = nullptr;
+      S->dev_size = 0;
+    }
+    if (hipMalloc(&S->dev, total) != hipSuccess) {
+      S->dev = nullptr;
+      return fail(SYNTHE_ENOMEM, std::string(who) + ": staging allocation failed");
+    }
+    S->dev_size = total;
+  }
+  char* base = (char*)S->dev;
+  HLine* d_lines = (HLine*)(base + o_lines);
+  double* d_depth = (double*)(base + o_depth);
+  HPair* d_pairs = (HPair*)(base + o_pairs);
+  HIP_TRY(hipMemcpyAsync(d_lines, S->lines.data(), NH * sizeof(HLine), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(d_depth, S->depth.data(), S->depth.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  HDepthDev dp{d_depth, d_depth + D, d_depth + 2 * D, d_depth + 3 * D, d_depth + 4 * D, d_depth + 6 * D,
+               d_depth + 5 * D};
+  hipLaunchKernelGGL(k_h_extent, dim3((unsigned)NH, (unsigned)D), dim3(64), 0, st, d_lines, dp, (int)D, n_wl, wl,
+                     cont, cutoff, ctx->tb, d_pairs);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_h_gather, dim3((unsigned)((n_wl + 255) / 256), (unsigned)D), dim3(256), 0, st, d_lines,
+                     d_pairs, (int)NH, (int)D, n_wl, wl, dp.temp, dp.state, ctx->tb, asynth);
   HIP_TRY(hipGetLastError());
   return SYNTHE_OK;
 }

@@ -15,7 +15,7 @@ import numpy as np
 
 from . import _native as N
 from .atmosphere import AtmosphereModel, depth_hckt, depth_rho, depth_txnxn, depth_xne
-from .lines import LineCatalog, LineSoA, build_line_soa
+from .lines import LineCatalog, LineSoA, build_line_soa, hydrogen_lines
 
 
 def _torch():
@@ -184,6 +184,8 @@ class SpectrumPlan:
         self.soa = build_line_soa(catalog, self.grid, self.depth.n_elem,
                                   grid_origin=float(self.grid[0]) if self.W else None)
         self.lines = upload_lines(self.soa, self.W)
+        self.hyd = hydrogen_lines(catalog, self.grid)
+        self._hyd_args = None
         f8 = torch.float64
         D, W, L = self.D, self.W, len(self.soa)
         dev = "cuda"
@@ -239,7 +241,13 @@ class SpectrumPlan:
             self.ctx.handle, ct_byref(ls), ct_byref(ds), self.W, N.ptr(self.kapmin), self.cutoff,
             self.microturb, N.ptr(self.transp), N.ptr(self.valid), s))
 
-    def run_asynth(self, stream=None, stats: bool = False):
+    def split_scattering(self):
+        """ALINE = ASYNTH (1 - FSCAT), SIGMAL = ASYNTH FSCAT (engine/opacity.py:2407-2420)."""
+        if self.rhoxj > 0.0:
+            self.line_scat.copy_(self.asynth * self.fscat_col)
+            self.line_abs = self.asynth * self.one_minus_fscat
+
+    def run_asynth(self, stream=None, stats: bool = False, split: bool = True):
         s = stream if stream is not None else N.stream_handle()
         ls, ds = self.lines.struct(), self.depth.struct()
         self.ctx.set_profile_precision(self.precision)
@@ -248,9 +256,8 @@ class SpectrumPlan:
             N.ptr(self.transp), N.ptr(self.valid), N.ptr(self.kapmin), self.cutoff,
             N.ptr(self.lines.corder), N.ptr(self.lines.cstart), N.ptr(self.lines.worder),
             N.ptr(self.asynth), N.ptr(self.stats) if stats else None, s))
-        if self.rhoxj > 0.0:
-            self.line_scat.copy_(self.asynth * self.fscat_col)
-            self.line_abs = self.asynth * self.one_minus_fscat
+        if split:
+            self.split_scattering()
 
     def run_line_opacity(self, stream=None, stats: bool = False):
         """TRANSP and ASYNTH (engine/opacity.py:2320-2363): one fused call
@@ -258,7 +265,9 @@ class SpectrumPlan:
         two drop-in stages."""
         if self.microturb > 0.0:
             self.run_transp(stream)
-            self.run_asynth(stream, stats)
+            self.run_asynth(stream, stats, split=False)
+            self.run_hydrogen(stream)
+            self.split_scattering()
             return
         s = stream if stream is not None else N.stream_handle()
         ls, ds = self.lines.struct(), self.depth.struct()
@@ -267,9 +276,39 @@ class SpectrumPlan:
             self.ctx.handle, ct_byref(ls), ct_byref(ds), self.W, N.ptr(self.d_grid), N.ptr(self.kapmin),
             self.cutoff, N.ptr(self.lines.corder), N.ptr(self.lines.cstart), N.ptr(self.lines.worder),
             N.ptr(self.transp), N.ptr(self.valid), N.ptr(self.asynth), N.ptr(self.stats) if stats else None, s))
-        if self.rhoxj > 0.0:
-            self.line_scat.copy_(self.asynth * self.fscat_col)
-            self.line_abs = self.asynth * self.one_minus_fscat
+        self.run_hydrogen(stream)
+        self.split_scattering()
+
+    def run_hydrogen(self, stream=None):
+        """AHLINE into ASYNTH (engine/opacity.py:2465-2487): the hydrogen lines'
+        HPROF4 profiles, skipped as the reference skips them when the bundle
+        lacks populations, Doppler widths or mass density."""
+        a = self.atm
+        if len(self.hyd) == 0 or a.population_per_ion is None or a.doppler_per_ion is None or \
+                a.mass_density is None or np.asarray(a.population_per_ion).shape[2] < 1:
+            return
+        if self._hyd_args is None:
+            f = lambda v: None if v is None else np.ascontiguousarray(v, np.float64)  # noqa: E731
+            keep = dict(temperature=f(a.temperature), electron_density=f(a.electron_density),
+                        mass_density=f(a.mass_density), hckt=f(depth_hckt(a)),
+                        pop_h1=f(np.asarray(a.population_per_ion)[:, 0, 0]),
+                        dop_h1=f(np.asarray(a.doppler_per_ion)[:, 0, 0]), xnf_he1=f(a.xnf_he1), xnf_h2=f(a.xnf_h2),
+                        xnfph=None if getattr(a, "xnfph", None) is None else
+                        np.ascontiguousarray(np.asarray(a.xnfph, np.float64).reshape(self.D, -1)[:, :2]),
+                        dopph=f(getattr(a, "dopph", None)), turbulent_velocity=f(a.turbulent_velocity))
+            h = self.hyd
+            hl = N.HLines(len(h), *(x.ctypes.data for x in (h.wavelength, h.center, h.gf, h.elo, h.n_lower,
+                                                             h.n_upper)))
+            hd = N.HDepths(self.D, *(None if keep[k] is None else keep[k].ctypes.data
+                                     for k in ("temperature", "electron_density", "mass_density", "hckt", "pop_h1",
+                                               "dop_h1", "xnf_he1", "xnf_h2", "xnfph", "dopph",
+                                               "turbulent_velocity")))
+            self._hyd_args = (keep, hl, hd)
+        _, hl, hd = self._hyd_args
+        s = stream if stream is not None else N.stream_handle()
+        N.check("synthe_hydrogen_lines", self.ctx.lib.synthe_hydrogen_lines(
+            self.ctx.handle, ct_byref(hl), ct_byref(hd), self.microturb, self.cutoff, self.W, N.ptr(self.d_grid),
+            N.ptr(self.kapmin), N.ptr(self.asynth), s))
 
     def run_rt(self, stream=None):
         s = stream if stream is not None else N.stream_handle()
@@ -292,6 +331,8 @@ class SpectrumPlan:
             self.run_line_opacity(stats=stats)
         else:
             self.asynth.zero_()
+            self.run_hydrogen()
+            self.split_scattering()
         self.run_rt()
 
     def result(self):

@@ -496,7 +496,8 @@ def filter_by_range(catalog: LineCatalog, wl_min: float, wl_max: float) -> LineC
 
 def write_gfall(path: Path, wl_nm: np.ndarray, log_gf: np.ndarray, code: np.ndarray,
                 e_low_cm: np.ndarray, gr_log: np.ndarray, gs_log: np.ndarray,
-                gw_log: np.ndarray) -> None:
+                gw_log: np.ndarray, n_lower: Optional[np.ndarray] = None,
+                n_upper: Optional[np.ndarray] = None) -> None:
     """Write 160-column GFALL records with EP = E + 1e7/lambda.
 
     Field layout follows atomic.py:325-345 (F11.4, F7.3, F6.2, F12.3, F5.1,
@@ -509,7 +510,8 @@ def write_gfall(path: Path, wl_nm: np.ndarray, log_gf: np.ndarray, code: np.ndar
         rec = (
             f"{wl_nm[i]:11.4f}{log_gf[i]:7.3f}{code[i]:6.2f}{e_low_cm[i]:12.3f}{0.5:5.1f}"
             f" {'LOWLEV':8s}{'  ':2s}{ep[i]:12.3f}{1.5:5.1f} {'UPLEV':8s}{'  ':2s}"
-            f"{gr_log[i]:6.2f}{gs_log[i]:6.2f}{gw_log[i]:6.2f}{'SYN ':4s}{0:2d}{0:2d}"
+            f"{gr_log[i]:6.2f}{gs_log[i]:6.2f}{gw_log[i]:6.2f}{'SYN ':4s}"
+            f"{0 if n_lower is None else int(n_lower[i]):2d}{0 if n_upper is None else int(n_upper[i]):2d}"
             f"{0:3d}{0.0:6.3f}{0:3d}{0.0:6.3f}{'':10s}{'':10s}{'':10s}{0:6d}"
         )
         assert len(rec) == 160, len(rec)
@@ -623,3 +625,45 @@ def build_line_soa(catalog: LineCatalog, grid: np.ndarray, n_elements: int,
         center_w=np.ascontiguousarray(wing_center_indices(grid, iw, grid_origin),
                                       dtype=np.int64),
     )
+
+
+@dataclass
+class HydrogenLines:
+    """The lines engine/opacity.py:1776-1782 sends to the HPROF4 path (line
+    type -1/-2, ion stage 1), in catalog order, as synthe_hydrogen_lines reads
+    them: the record's wavelength, the centre index of the index wavelength
+    (engine/opacity.py:550-592, -1 / W off the grid), linear gf, lower level
+    energy and NBLO/NBUP (opacity.py:1787-1796 defaults when absent)."""
+
+    index: np.ndarray
+    wavelength: np.ndarray
+    center: np.ndarray
+    gf: np.ndarray
+    elo: np.ndarray
+    n_lower: np.ndarray
+    n_upper: np.ndarray
+
+    def __len__(self) -> int:
+        return int(self.index.size)
+
+
+def hydrogen_lines(catalog: LineCatalog, grid: np.ndarray) -> HydrogenLines:
+    lt = np.asarray(catalog.line_types, np.int64)
+    idx = np.nonzero(((lt == -1) | (lt == -2)) & (np.asarray(catalog.ion_stages, np.int64) == 1))[0]
+    if idx.size and grid.size:
+        center = nearest_grid_indices(grid, np.asarray(catalog.index_wavelength, np.float64)[idx])
+    else:
+        center = np.zeros(idx.size, np.int64)
+    if catalog.n_lower is not None:
+        nlo = np.asarray(catalog.n_lower, np.int64)[idx]
+    else:
+        nlo = np.ones(idx.size, np.int64)
+    if catalog.n_upper is not None:
+        nup = np.asarray(catalog.n_upper, np.int64)[idx]
+    else:
+        nup = nlo + 1
+    c = lambda a, t: np.ascontiguousarray(a, t)  # noqa: E731
+    return HydrogenLines(idx, c(np.asarray(catalog.wavelength)[idx], np.float64), c(center, np.int64),
+                         c(np.asarray(catalog.gf)[idx], np.float64),
+                         c(np.asarray(catalog.excitation_energy)[idx], np.float64), c(nlo, np.int32),
+                         c(nup, np.int32))

@@ -274,6 +274,8 @@ def build_bundle(atm: AtmColumns, abund_delta: Optional[Dict[int, float]] = None
         xnf_h=nh1,
         xnf_he1=xnf[(2, 0)],
         xnf_h2=np.zeros(D),
+        # H I ground level (g = 2 of the partition function) and protons
+        xnfph=np.ascontiguousarray(np.stack([2.0 * pop[:, 0, 0], xnf[(1, 1)]], axis=1)),
         population_per_ion=pop,
         doppler_per_ion=dop,
         continuum_wledge=WLEDGE.copy(),
@@ -302,8 +304,44 @@ def synthetic_lines(n: int, wl0: float, wl1: float, seed: int) -> Dict[str, np.n
     return dict(wl=wl, code=code, loggf=loggf, e_low=e_low, gr=gr, gs=gs, gw=gw)
 
 
+def hydrogen_series(wl0: float, wl1: float, margin_nm: float = 100.0, n_max: int = 6,
+                    m_max: int = 40) -> Dict[str, np.ndarray]:
+    """H I lines n -> m (n <= n_max, m <= m_max) within margin_nm of the window,
+    as GFALL records (code 1.00, NBLO/NBUP set): level energies
+    R_H (1 - 1/n^2) cm^-1, vacuum wavelengths from them, Kramers oscillator
+    strengths f = 32 / (3 pi sqrt 3) / (n^5 m^3 (1/n^2 - 1/m^2)^3) with a
+    Gaunt factor 0.87 and g = 2 n^2."""
+    ryd = 109677.576
+    rows = []
+    for n in range(1, n_max + 1):
+        for m in range(n + 1, m_max + 1):
+            e_n, e_m = ryd * (1.0 - 1.0 / n ** 2), ryd * (1.0 - 1.0 / m ** 2)
+            wl = 1.0e7 / (e_m - e_n)
+            if not (wl0 - margin_nm <= wl <= wl1 + margin_nm):
+                continue
+            f = 0.87 * 32.0 / (3.0 * np.pi * np.sqrt(3.0)) / (n ** 5 * m ** 3 * (1.0 / n ** 2 - 1.0 / m ** 2) ** 3)
+            rows.append((wl, np.log10(2.0 * n * n * f), e_n, n, m))
+    rows.sort()
+    a = np.array(rows, dtype=np.float64).reshape(-1, 5)
+    k = a.shape[0]
+    return dict(wl=a[:, 0], code=np.full(k, 1.00), loggf=a[:, 1], e_low=a[:, 2], gr=np.full(k, 8.5),
+                gs=np.full(k, -5.0), gw=np.full(k, -7.5), n_lower=a[:, 3].astype(np.int64),
+                n_upper=a[:, 4].astype(np.int64))
+
+
+def merge_line_lists(*lists: Dict[str, np.ndarray]) -> Dict[str, np.ndarray]:
+    """Concatenate line draws and sort them by wavelength (GFALL order)."""
+    keys = ("wl", "code", "loggf", "e_low", "gr", "gs", "gw")
+    out = {k: np.concatenate([d[k] for d in lists]) for k in keys}
+    out["n_lower"] = np.concatenate([d.get("n_lower", np.zeros(d["wl"].size, np.int64)) for d in lists])
+    out["n_upper"] = np.concatenate([d.get("n_upper", np.zeros(d["wl"].size, np.int64)) for d in lists])
+    order = np.argsort(out["wl"], kind="stable")
+    return {k: v[order] for k, v in out.items()}
+
+
 def write_line_list(path: Path, d: Dict[str, np.ndarray]) -> None:
-    _lines.write_gfall(path, d["wl"], d["loggf"], d["code"], d["e_low"], d["gr"], d["gs"], d["gw"])
+    _lines.write_gfall(path, d["wl"], d["loggf"], d["code"], d["e_low"], d["gr"], d["gs"], d["gw"],
+                       d.get("n_lower"), d.get("n_upper"))
 
 
 @dataclass

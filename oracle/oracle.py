@@ -60,6 +60,10 @@ def lib():
         L.orc_map1_josh.restype = ct.c_int
         L.orc_josh_iterate.restype = ct.c_int
         L.orc_get_threads.restype = ct.c_int
+        for name in ("orc_vcse1f", "orc_sofbeta", "orc_hf_nm", "orc_hydrogen_profile"):
+            getattr(L, name).restype = ct.c_double
+        L.orc_fine_structure.restype = ct.c_int
+        L.orc_hydrogen_lines.restype = ct.c_int
         _lib = L
     return _lib
 
@@ -112,6 +116,17 @@ class Tables:
         self.struct = _Tables(p(self.h0), p(self.h1), p(self.h2), 2001, p(self.extab),
                               p(self.extabf), 1001)
         lib().orc_set_josh_tables(p(XTAU), p(CH), p(COEFJ))
+        self.h = {k: np.ascontiguousarray(_KT[k], dtype=np.float64) for k in _KT.files
+                  if k.startswith("h_") or k in ("conth", "ehyd_cm")}
+        h = self.h
+        lib().orc_set_hydrogen_tables(
+            p(h["h_propbm"]), p(h["h_c"]), p(h["h_d"]), p(h["h_pp"]), p(h["h_beta_grid"]), p(h["h_stalph"]),
+            p(h["h_stwtal"]), p(h["h_istal"]), p(h["h_lnghal"]), p(h["h_stcomp"]), p(h["h_stcpwt"]),
+            p(h["h_lncomp"]), p(h["h_cutoff_h2_plus"]), ct.c_int(h["h_cutoff_h2_plus"].size),
+            p(h["h_cutoff_h2"]), ct.c_int(h["h_cutoff_h2"].size), p(h["h_asum_lyman"]),
+            ct.c_int(h["h_asum_lyman"].size), p(h["h_asum"]), ct.c_int(h["h_asum"].size), p(h["h_y1wtm"]),
+            p(h["h_xknmtb"]), p(h["conth"]), ct.c_int(h["conth"].size), p(h["ehyd_cm"]), p(self.extab),
+            p(self.extabf), ct.c_int(1001))
 
 
 _TABLES: Optional[Tables] = None
@@ -351,6 +366,93 @@ def lte_frequency(w, grid, atm, cont_abs, cont_scat, line_abs, line_scat, line_s
     return float(ft), float(fc.value), info.copy()
 
 
+# ------------------------------------------------------------- hydrogen lines
+HSTATE_FIELDS = ("pp", "fo", "y1b", "y1s", "t3nhe", "t3nh2", "c1d", "c2d", "gcon1", "gcon2", "xnfph0", "xnfph1",
+                 "dopph", "xne", "has")
+
+
+def vcse1f(x: float) -> float:
+    tables()
+    return lib().orc_vcse1f(ct.c_double(x))
+
+
+def sofbeta(beta: float, p_: float, n: int, m: int) -> float:
+    tables()
+    return lib().orc_sofbeta(ct.c_double(beta), ct.c_double(p_), ct.c_int(n), ct.c_int(m))
+
+
+def hf_nm(n: int, m: int) -> float:
+    return lib().orc_hf_nm(ct.c_int(n), ct.c_int(m))
+
+
+def fine_structure(n: int, m: int):
+    tables()
+    off, wt = np.zeros(40), np.zeros(40)
+    k = lib().orc_fine_structure(ct.c_int(n), ct.c_int(m), p(off), p(wt))
+    return off[:k].copy(), wt[:k].copy()
+
+
+def hydrogen_states(atm, microturb_kms: float = 0.0) -> np.ndarray:
+    """Per-depth hydrogen state [D, 15] (physics/populations.py:96-128, 210-231)."""
+    D = atm.layers
+    he1, h2 = getattr(atm, "xnf_he1", None), getattr(atm, "xnf_h2", None)
+    xnfph, dopph = getattr(atm, "xnfph", None), getattr(atm, "dopph", None)
+    vt = getattr(atm, "turbulent_velocity", None)
+    opt = lambda a: None if a is None else _c(a, np.float64)  # noqa: E731
+    out = np.zeros((D, len(HSTATE_FIELDS)))
+    xnfph = opt(xnfph)
+    if xnfph is not None:
+        xnfph = np.ascontiguousarray(xnfph.reshape(D, -1)[:, :2])
+    args = [opt(he1), opt(h2), xnfph, opt(dopph), opt(vt)]
+    lib().orc_hydrogen_states(ct.c_long(D), p(_c(atm.temperature, np.float64)), p(_xne(atm)),
+                              *[p(a) if a is not None else None for a in args], ct.c_double(microturb_kms),
+                              ct.c_int(int(he1 is not None or h2 is not None)), p(out))
+    return out
+
+
+def hydrogen_profile(n: int, m: int, state: np.ndarray, delta_nm: float) -> float:
+    tables()
+    st = _c(state, np.float64)
+    return lib().orc_hydrogen_profile(ct.c_int(n), ct.c_int(m), p(st), ct.c_double(delta_nm))
+
+
+def hydrogen_line_selection(cat, grid):
+    """The lines engine/opacity.py:1776-1782 takes (line type -1/-2, ion stage
+    1), in catalog order: wavelength, centre index, gf, elo, n_lower, n_upper."""
+    sel = [i for i, r in enumerate(cat.records) if int(r.line_type or 0) in (-1, -2) and r.ion_stage == 1]
+    idx = np.array(sel, dtype=np.int64)
+    centers = nearest_grid_indices(grid, np.ascontiguousarray(cat.index_wavelength, np.float64))
+    return dict(index=idx, wl=_c(cat.wavelength[idx], np.float64), center=_c(centers[idx], np.int64),
+                gf=_c(cat.gf[idx], np.float64), elo=_c(cat.excitation[idx], np.float64),
+                n_lower=np.array([cat.records[i].n_lower for i in sel], np.int32),
+                n_upper=np.array([cat.records[i].n_upper for i in sel], np.int32))
+
+
+def hydrogen_lines(cat, atm, grid, cont, stim, cutoff=1e-3, microturb_kms=0.0):
+    """AHLINE [D, W] (engine/opacity.py:1706-1884) and the profile evaluation count."""
+    t = tables()
+    D, W = atm.layers, grid.size
+    h = hydrogen_line_selection(cat, grid)
+    out = np.zeros((D, W))
+    stats = np.zeros(1)
+    if h["index"].size == 0 or atm.population_per_ion is None or atm.doppler_per_ion is None:
+        return out, 0.0
+    pop = np.ascontiguousarray(atm.population_per_ion[:, 0, 0], np.float64)
+    dop = np.ascontiguousarray(atm.doppler_per_ion[:, 0, 0], np.float64)
+    rho = getattr(atm, "mass_density", None)
+    if rho is None:
+        return out, 0.0
+    st = hydrogen_states(atm, microturb_kms)
+    rc = lib().orc_hydrogen_lines(
+        ct.c_long(h["index"].size), ct.c_long(D), ct.c_long(W), p(_c(grid, np.float64)), p(h["wl"]),
+        p(h["center"], _I64), p(h["gf"]), p(h["elo"]), p(h["n_lower"], _I32), p(h["n_upper"], _I32), p(pop),
+        p(dop), p(_c(rho, np.float64)), p(hckt(atm)), p(_xne(atm)), p(st), p(_c(cont, np.float64)),
+        p(_c(stim, np.float64)), ct.c_double(cutoff), ct.c_double(microturb_kms), p(out), p(stats))
+    if rc != 0:
+        raise ValueError("hydrogen line beyond the HPROF4 damping tables (n_upper + 2 > 96)")
+    return out, float(stats[0])
+
+
 @dataclass
 class OracleResult:
     wavelength: np.ndarray
@@ -396,6 +498,8 @@ def synthesize(atm, lines, wl_start, wl_end, resolution, cutoff=1e-3, microturb_
     tm["transp"] = time.perf_counter() - t0
     t0 = time.perf_counter()
     asy, st = asynth(soa, atm, grid, tr, valid, kapmin, cutoff)
+    ahl, st["h_profile_evals"] = hydrogen_lines(lines, atm, grid, kapmin, stim, cutoff, microturb_kms)
+    asy = asy + ahl  # engine/opacity.py:2483-2487: AHLINE joins ASYNTH before the FSCAT split
     tm["asynth"] = time.perf_counter() - t0
     rhox = atm.depth
     fscat = np.exp(-rhox / rhoxj) if rhoxj > 0.0 else np.zeros_like(rhox)
@@ -412,7 +516,7 @@ def synthesize(atm, lines, wl_start, wl_end, resolution, cutoff=1e-3, microturb_
     lib().orc_flux_to_lambda(ct.c_long(grid.size), p(grid), p(cont))
     stages = {}
     if keep_stages:
-        stages = dict(cont_abs=ca, cont_scat=cs, transp=tr, valid=valid, asynth=asy, slinec=slinec,
+        stages = dict(cont_abs=ca, cont_scat=cs, transp=tr, valid=valid, asynth=asy, ahline=ahl, slinec=slinec,
                       center_t=soa.center_t, center_w=soa.center_w, process=soa.process,
                       iters=lte_spectrum.last_iters)
     return OracleResult(grid, flux, cont, ft, fc, stages, st, tm)

@@ -1,0 +1,107 @@
+"""Hydrogen lines (HPROF4) on the GPU against the oracle.
+
+A 405-415 nm window at R = 300000 (W = 7,317) of the solar-like atmosphere
+(Teff 5750 K, log g 4.44, 72 layers) with 2,000 seeded metal lines and the
+H I series n <= 6 within the reference's 100 nm hydrogen margin (37 Balmer
+lines: H delta at 410.3 nm in the window, its line-centre opacity 14-300
+times the continuum's, the others outside with wings reaching in): the full spectrum through SpectrumPlan (ASYNTH + AHLINE,
+then the RT) against oracle.synthesize, with and without microturbulence
+and line scattering.  AHLINE alone is compared at 1e-11 relative (device
+log/pow/sqrt against glibc's: the walk's stopping points agree exactly, the
+values to a few ulp), the flux at the plain-path bar (1e-6 relative under the
+robust mask |F| > 1e-3 median(C)).
+"""
+
+from __future__ import annotations
+
+import tempfile
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+W0, W1 = 405.0, 415.0
+
+
+def _workload():
+    from kurucz_py_joss_submission_amd import lines as L
+    from kurucz_py_joss_submission_amd import synthetic as S
+
+    d = S.merge_line_lists(S.synthetic_lines(2000, W0 - 1, W1 + 1, seed=7), S.hydrogen_series(W0, W1))
+    td = tempfile.mkdtemp()
+    p = Path(td) / "h.gfall"
+    S.write_line_list(p, d)
+    atm = S.build_bundle(S.make_atm(5750.0, 4.44, 72, 0.0))
+    return atm, L.filter_by_range(L.load_catalog(p), W0, W1), p
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("microturb,rhoxj", [(0.0, 0.0), (1.5, 0.0), (0.0, 0.3)])
+def test_hydrogen_spectrum_matches_oracle(gpu, oracle, microturb, rhoxj):
+    import catalog as C
+    import torch
+
+    from kurucz_py_joss_submission_amd.device import SpectrumPlan
+
+    atm, cat, path = _workload()
+    ocat = C.filter_by_range(C.read_gfall(path), W0, W1)
+    ref = oracle.synthesize(atm, ocat, W0, W1, 300000.0, microturb_kms=microturb, rhoxj=rhoxj)
+    plan = SpectrumPlan(atm, cat, W0, W1, 300000.0, microturb_kms=microturb, rhoxj=rhoxj)
+    assert len(plan.hyd) == 37 == sum(1 for r in ocat.records if r.line_type in (-1, -2))
+    plan.run_continuum()
+    plan.asynth.zero_()
+    plan.run_hydrogen()
+    torch.cuda.synchronize()
+    ah = plan.asynth.cpu().numpy()
+    ah_ref = ref.stages["ahline"]
+    assert np.count_nonzero(ah) == np.count_nonzero(ah_ref) > 0  # the walks stop where the reference's stop
+    nz = ah_ref > 0
+    assert np.max(np.abs(ah[nz] - ah_ref[nz]) / ah_ref[nz]) < 1e-11
+    plan.run()
+    torch.cuda.synchronize()
+    asy = plan.asynth.cpu().numpy()
+    a_ref = ref.stages["asynth"]
+    m = a_ref > 0
+    assert np.max(np.abs(asy[m] - a_ref[m]) / a_ref[m]) < 1e-10
+    f, f_ref = plan.flux.cpu().numpy(), ref.flux
+    floor = 1e-3 * np.median(ref.continuum)
+    assert np.max(np.abs(f - f_ref) / np.maximum(np.abs(f_ref), floor)) < 1e-6
+    # the hydrogen lines change the spectrum: without them the flux differs
+    assert np.max(np.abs(f_ref - _without_h_flux(oracle, atm, ocat, microturb, rhoxj)) / np.abs(f_ref)) > 1e-3
+
+
+def _without_h_flux(oracle, atm, ocat, microturb, rhoxj):
+    import catalog as C
+
+    keep = [r for r in ocat.records if r.line_type not in (-1, -2)]
+    return oracle.synthesize(atm, C.Catalog.of(keep), W0, W1, 300000.0, microturb_kms=microturb, rhoxj=rhoxj,
+                             keep_stages=False).flux
+
+
+@pytest.mark.gpu
+def test_hydrogen_only_catalog(gpu, oracle):
+    """A catalog of hydrogen lines alone (no plain lines: ASYNTH is AHLINE)."""
+    import catalog as C
+    import torch
+
+    from kurucz_py_joss_submission_amd import lines as L
+    from kurucz_py_joss_submission_amd import synthetic as S
+    from kurucz_py_joss_submission_amd.device import SpectrumPlan
+
+    td = tempfile.mkdtemp()
+    p = Path(td) / "h.gfall"
+    S.write_line_list(p, S.hydrogen_series(480.0, 490.0, n_max=4))
+    atm = S.build_bundle(S.make_atm(9500.0, 4.0, 64, 0.0))
+    cat = L.filter_by_range(L.load_catalog(p), 480.0, 490.0)
+    ref = oracle.synthesize(atm, C.filter_by_range(C.read_gfall(p), 480.0, 490.0), 480.0, 490.0, 300000.0)
+    plan = SpectrumPlan(atm, cat, 480.0, 490.0, 300000.0)
+    assert len(plan.soa) == len(cat) and int(plan.soa.process.sum()) == 0
+    plan.run()
+    torch.cuda.synchronize()
+    a_ref = ref.stages["asynth"]
+    asy = plan.asynth.cpu().numpy()
+    assert np.count_nonzero(asy) == np.count_nonzero(a_ref) > 0
+    m = a_ref > 0
+    assert np.max(np.abs(asy[m] - a_ref[m]) / a_ref[m]) < 1e-11
+    floor = 1e-3 * np.median(ref.continuum)
+    assert np.max(np.abs(plan.flux.cpu().numpy() - ref.flux) / np.maximum(np.abs(ref.flux), floor)) < 1e-6
