@@ -487,6 +487,8 @@ def main() -> int:
     pmc = ROOT / "profiles" / f"traffic_{cfg['name']}.json"
     if pmc.exists():
         roof["traffic"] = json.loads(pmc.read_text()).get("k_gather_bytes_per_launch")
+        roof["traffic_source"] = (f"profiles/{pmc.name}: rocprofv3 FETCH_SIZE and WRITE_SIZE of k_gather on this "
+                                  "workload, each scaled by its reading on a 1 GiB device copy")
 
     out = {
         "metric": "spectrum wall-time & line x depth x lambda evals/s (plain FP64 path)",
