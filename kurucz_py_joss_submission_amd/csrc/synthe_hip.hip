@@ -2030,54 +2030,150 @@ __global__ __launch_bounds__(256) void k_near_adj(long W, int D, const WingHead*
   if (qn > 0) flush();
 }
 
-// Far wings: per list entry, Q = sum over its far pixels (N10DOP < |k| <=
-// MAXSTEP) of u / k^2; one wave per (512-pixel tile, depth) as k_gather.
-__global__ __launch_bounds__(64) void k_far_adj(GatherArgs g, const double* __restrict__ u,
-                                                double* __restrict__ q_entry) {
-  constexpr int kP = 8;
-  constexpr long kT = 64L * kP;
-  const int d = blockIdx.y;
-  const long t0 = blockIdx.x * kT, t1 = t0 + kT - 1;
-  const int lane = threadIdx.x;
-  const long W = g.W;
-  const long w0 = t0 + (long)kP * lane;
-  double uu[kP];
+// Tile moments of the adjoint weights for the far-wing expansion: for each
+// (512-pixel tile, depth), m_n = sum over the tile's pixels of u(w) t^n with
+// t = (w - wm) / h (wm the tile's centre, h half its width), n < kFarTerms.
+// One wave per tile, 8 consecutive pixels per lane as k_gather.
+__global__ __launch_bounds__(64) void k_adj_moments(long W, const double* __restrict__ u, double* __restrict__ mom) {
+  const long t = blockIdx.x, ntiles = gridDim.x;
+  const int d = blockIdx.y, lane = threadIdx.x;
+  const long t0 = t * kWT;
+  const double wm = (double)t0 + 0.5 * (kWT - 1);
+  constexpr double kHalf = 0.5 * kWT;
+  double m[kFarTerms];
 #pragma unroll
-  for (int p = 0; p < kP; ++p) uu[p] = w0 + p < W ? u[(long)d * W + w0 + p] : 0.0;
-  const long bound = class_window_search(lane, d, t0, t1, 1L << 40, g.regions, g.list_cm);
-  for (int kk = 0; kk < kClasses; ++kk) {
-    const long lo = uniform_long(__shfl(bound, 2 * kk));
-    const long hi = uniform_long(__shfl(bound, 2 * kk + 1));
-    for (long j = lo; j < hi; ++j) {
-      const WingHead h = g.list_cm[j];
-      if (t1 < h.lo || t0 > h.hi) continue;  // no far wing here
-      const long c = head_c(h), M = h.hi - c;
-      const double n10d = (double)(h.nhi - c), md = (double)M;
-      const double kd = (double)(w0 - c);
-      // sum_p up_p / k_p^2 over the lane's 8 pixels with one shared
-      // reciprocal (the far_chunk scheme): P_p = prod_{i <= p} qk_i, 1/qk_p =
-      // P_{p-1} / P_p; qk = max(k^2, 1) keeps masked pixels (up = 0) finite
-      double up[kP], qk[kP], P[kP];
+  for (int n = 0; n < kFarTerms; ++n) m[n] = 0.0;
 #pragma unroll
-      for (int p = 0; p < kP; ++p) {
-        const double k = kd + (double)p;
-        const double ka = fabs(k);
-        up[p] = (ka > n10d && ka <= md) ? uu[p] : 0.0;
-        qk[p] = fmax(k * k, 1.0);
-        P[p] = p == 0 ? qk[0] : P[p - 1] * qk[p];
-      }
-      double y = __builtin_amdgcn_rcp(P[kP - 1]);
-      y = __builtin_fma(y, __builtin_fma(-P[kP - 1], y, 1.0), y);
-      double q = 0.0;
+  for (int p = 0; p < kWP; ++p) {
+    const long w = t0 + (long)kWP * lane + p;
+    double pw = w < W ? u[(long)d * W + w] : 0.0;
+    const double tt = ((double)w - wm) / kHalf;
 #pragma unroll
-      for (int p = kP - 1; p > 0; --p) {
-        q = __builtin_fma(up[p], y * P[p - 1], q);
-        y *= qk[p];
-      }
-      q = __builtin_fma(up[0], y, q);
-      q = wave_sum(q);
-      if (lane == 0) atomicAdd(q_entry + j, q);
+    for (int n = 0; n < kFarTerms; ++n) {
+      m[n] += pw;
+      pw *= tt;
     }
+  }
+  double mine = 0.0;
+#pragma unroll
+  for (int n = 0; n < kFarTerms; ++n) {
+    const double sn = wave_sum_butterfly(m[n]);
+    mine = lane == n ? sn : mine;
+  }
+  if (lane < kFarTerms) mom[((long)d * ntiles + t) * kFarTerms + lane] = mine;
+}
+
+// Far wings of the adjoint: per list entry, Q = sum over its far pixels
+// (N10DOP < |k| <= MAXSTEP) of u / k^2, the transpose of k_gather's far-wing
+// sum, with the same split of the entry's tiles:
+//  * tiles wholly in the far zone at |K| >= 4.5 h (K the tile centre's
+//    offset from the line centre) take the expansion
+//    u / (K + x)^2 summed = (1/K^2) sum_n (n+1) (-h/K)^n m_n over the tile's
+//    moments (k_adj_moments), one entry per lane;
+//  * the others (the line-centre tiles, the N10DOP and MAXSTEP edges) are
+//    summed pixel by pixel, one entry at a time over the whole wave (8
+//    consecutive pixels per lane, coalesced reads of u), then one wave sum.
+// Each entry's Q is written once, in a fixed order: no atomics, run-to-run
+// identical.  One thread per entry of a depth's lists, 64 entries per wave.
+__global__ __launch_bounds__(64) void k_far_adj(long W, int D, const int64_t* __restrict__ regions,
+                                                const WingHead* __restrict__ list_cm, const double* __restrict__ u,
+                                                const double* __restrict__ mom, long ntiles,
+                                                double* __restrict__ q_entry) {
+  const int d = blockIdx.y, lane = threadIdx.x;
+  const long beg = regions[(long)d * (kClasses + 1)], end = regions[(long)d * (kClasses + 1) + kClasses];
+  constexpr double kHalf = 0.5 * kWT;
+  const double kmin = kFarExpandRatio * kHalf;
+  const long Wl = W;
+  for (long j0 = beg + (long)blockIdx.x * 64; j0 < end; j0 += (long)gridDim.x * 64) {
+    const long j = j0 + lane;
+    WingHead h;
+    h.lo = INT32_MAX;
+    h.hi = INT32_MIN;
+    h.nlo = 0;
+    h.nhi = 0;
+    h.xs = 0.0;
+    h.cd = 0.0;
+    if (j < end) h = list_cm[j];
+    const bool far = h.lo <= h.hi;
+    // expansion part, one entry per lane
+    double qe = 0.0;
+    long t_first = 0, t_last = -1;
+    if (far) {
+      const long lo_px = h.lo > 0 ? h.lo : 0, hi_px = h.hi < Wl - 1 ? h.hi : Wl - 1;
+      if (lo_px <= hi_px) {
+        t_first = lo_px / kWT;
+        t_last = hi_px / kWT;
+      }
+    }
+    const double c = h.cd;
+    for (long t = t_first; t <= t_last; ++t) {
+      const int t0i = (int)(t * kWT), t1i = t0i + kWT - 1;
+      const double K = (double)t0i + 0.5 * (kWT - 1) - c;
+      if (!(far_interior(h, t0i, t1i) && fabs(K) >= kmin)) continue;
+      double y = __builtin_amdgcn_rcp(K);
+      y = __builtin_fma(y, __builtin_fma(-K, y, 1.0), y);
+      y = __builtin_fma(y, __builtin_fma(-K, y, 1.0), y);
+      const double r = -kHalf * y;
+      const double* m = mom + ((long)d * ntiles + t) * kFarTerms;
+      double acc = (double)kFarTerms * m[kFarTerms - 1];
+#pragma unroll
+      for (int n = kFarTerms - 2; n >= 0; --n) acc = __builtin_fma(acc, r, (double)(n + 1) * m[n]);
+      qe += acc * y * y;
+    }
+    // exact part, the wave on one entry at a time
+    unsigned long long todo = __ballot(far && t_first <= t_last);
+    double qx = 0.0;
+    while (todo) {
+      const int src = __builtin_ctzll(todo);
+      todo &= todo - 1;
+      const int lo = __builtin_amdgcn_readlane(h.lo, src), hi = __builtin_amdgcn_readlane(h.hi, src);
+      const int nlo = __builtin_amdgcn_readlane(h.nlo, src), nhi = __builtin_amdgcn_readlane(h.nhi, src);
+      const int tf = __builtin_amdgcn_readlane((int)t_first, src), tl = __builtin_amdgcn_readlane((int)t_last, src);
+      const int M = (hi - lo) >> 1, cc = lo + M, n10 = (nhi - nlo) >> 1;
+      // the candidate tiles: the ends, the tiles within kmin of the centre,
+      // and those holding the Voigt zone's edges; the others expand or hold
+      // no far pixel
+      const int tA0 = (int)floor(((double)cc - kmin) / kWT) - 1, tB0 = (int)floor(((double)cc + kmin) / kWT) + 1;
+      const int tA = tA0 > tf ? tA0 : tf, tB = tB0 < tl ? tB0 : tl;
+      const int zl = (int)floor((double)(cc - n10 - 1) / kWT), zr = (int)floor((double)(cc + n10 + 1) / kWT);
+      // the candidate tiles in increasing order (tf <= zl' <= tA..tB <= zr' <=
+      // tl, zl' = min(zl, tA), zr' = max(zr, tB)), each visited once
+      const int ia[5] = {tf, zl < tA ? zl : tA, tA, zr > tB ? zr : tB, tl};
+      const int ib[5] = {tf, zl < tA ? zl : tA, tB, zr > tB ? zr : tB, tl};
+      double part = 0.0;
+      int last = tf - 1;
+      WingHead hh;
+      hh.lo = lo;
+      hh.hi = hi;
+      hh.nlo = nlo;
+      hh.nhi = nhi;
+      const double n10d = (double)n10, md = (double)M;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        const int ta = ia[k] > last + 1 ? ia[k] : last + 1;
+        const int tb = ib[k] < tl ? ib[k] : tl;
+        for (int t = ta; t <= tb; ++t) {
+          const int t0i = t * kWT, t1i = t0i + kWT - 1;
+          const double K = (double)t0i + 0.5 * (kWT - 1) - (double)cc;
+          if (far_interior(hh, t0i, t1i) && fabs(K) >= kmin) continue;  // expanded above
+          const long w0 = (long)t0i + (long)kWP * lane;
+          // u is padded by a tile past the last depth's row: the 8 loads are
+          // in bounds and pixels past the grid carry weight 0 below
+          const double* up = u + (long)d * W + w0;
+          double uv[kWP];
+#pragma unroll
+          for (int p = 0; p < kWP; ++p) uv[p] = up[p];
+          double inv[kWP];  // 1 / k^2 on the far pixels, 0 elsewhere (one shared reciprocal)
+          far_chunk<kWP, kFarNewton, true>(1.0, (double)(w0 - cc), 0, n10d, md, inv);
+#pragma unroll
+          for (int p = 0; p < kWP; ++p) part = __builtin_fma(w0 + p < Wl ? uv[p] : 0.0, inv[p], part);
+        }
+        last = tb > last ? tb : last;
+      }
+      const double tot = wave_sum_butterfly(part);
+      qx = lane == src ? tot : qx;
+    }
+    if (j < end) q_entry[j] = qe + qx;
   }
 }
 
@@ -3720,8 +3816,9 @@ int synthe_asynth_vjp(synthe_ctx* ctx, const synthe_lines* lines, const synthe_d
   const long pairs = L * (long)D;
   const AsynthLayout lay = asynth_layout(L, D, true);
   const size_t o_u = lay.total;
-  const size_t o_q = align_up(o_u + (size_t)D * n_wl * sizeof(double));
-  const size_t total = align_up(o_q + (size_t)(pairs > 0 ? pairs : 1) * sizeof(double));
+  const size_t o_q = align_up(o_u + ((size_t)D * n_wl + kWT) * sizeof(double));  // u, then one tile of padding
+  const size_t o_mom = align_up(o_q + (size_t)(pairs > 0 ? pairs : 1) * sizeof(double));
+  const size_t total = align_up(o_mom + (size_t)D * ((n_wl + kWT - 1) / kWT) * kFarTerms * sizeof(double));
   rc = ensure_ws(ctx, total);
   if (rc) return rc;
   char* base = (char*)ctx->ws;
@@ -3748,7 +3845,12 @@ int synthe_asynth_vjp(synthe_ctx* ctx, const synthe_lines* lines, const synthe_d
                      b.list_l, b.reg, b.near_reach, ctx->tb, u, wl, s_kappa0, s_adamp, s_dopple);
   HIP_TRY(hipGetLastError());
   const GatherArgs ga = gather_args(lines, depths, n_wl, wl, transp, valid, center_order, center_start, nullptr, b);
-  hipLaunchKernelGGL(k_far_adj, dim3(blocks_for(n_wl, 512), (unsigned)D), dim3(64), 0, st, ga, u, q);
+  (void)ga;
+  const long ntiles = (n_wl + kWT - 1) / kWT;
+  double* mom = (double*)(base + o_mom);
+  hipLaunchKernelGGL(k_adj_moments, dim3((unsigned)ntiles, (unsigned)D), dim3(64), 0, st, n_wl, u, mom);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_far_adj, dim3(2048, (unsigned)D), dim3(64), 0, st, n_wl, D, b.reg, b.lcm, u, mom, ntiles, q);
   HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(k_far_finish, dim3(64, (unsigned)D), dim3(kBlock), 0, st, D, b.reg, b.lp, b.list_l, q, ctx->tb,
                      wl, n_wl, s_kappa0, s_adamp, s_dopple);
