@@ -189,12 +189,14 @@ typedef struct {
 /* Adds AHLINE (post-STIM, [D][W]) into asynth (device, [D][W]): per pixel
  * the lines' terms in catalog order, summed from zero, then added.  cont
  * (device [D][W]) is the total continuum (absorption + scattering) the
- * reference's cutoffs compare with; wl (device [W]) the grid.  Returns
+ * reference's cutoffs compare with; stim (device [D][W], or NULL for
+ * 1 - exp(-h nu / k T) as engine/opacity.py:2215-2228 computes it) the
+ * stimulated-emission factor; wl (device [W]) the grid.  Returns
  * SYNTHE_ERANGE (nothing added) if a line's n_upper + 2 exceeds the
  * radiative-damping table (96 levels), where the reference raises. */
 int synthe_hydrogen_lines(synthe_ctx *ctx, const synthe_hlines *lines, const synthe_hdepths *depths,
                           double microturb_kms, double cutoff, long n_wl, const double *wl,
-                          const double *cont, double *asynth, synthe_stream stream);
+                          const double *cont, const double *stim, double *asynth, synthe_stream stream);
 
 /* engine/radiative.py:282-494 (solve_lte_spectrum) with
  * physics/josh_solver.py:367-767 (solve_josh_flux) per wavelength: the

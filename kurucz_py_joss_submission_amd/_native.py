@@ -123,7 +123,7 @@ def load_library(path: Optional[Path] = None):
         lib.synthe_transp_asynth.argtypes = [_VP, ct.POINTER(Lines), ct.POINTER(Depths), ct.c_long, _VP, _VP,
                                              ct.c_double, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]
         lib.synthe_hydrogen_lines.argtypes = [_VP, ct.POINTER(HLines), ct.POINTER(HDepths), ct.c_double,
-                                              ct.c_double, ct.c_long, _VP, _VP, _VP, _VP]
+                                              ct.c_double, ct.c_long, _VP, _VP, _VP, _VP, _VP]
         lib.synthe_lte_spectrum.argtypes = [_VP, ct.c_long, ct.c_long] + [_VP] * 12
         lib.synthe_lte_spectrum_noalloc.argtypes = [_VP, ct.c_long, ct.c_long] + [_VP] * 12
         lib.synthe_josh_flux_batch.argtypes = [_VP, ct.c_long, ct.c_long] + [_VP] * 10

@@ -3136,7 +3136,8 @@ __device__ __forceinline__ double h_value(const HLine& L, const HPair& pr, const
 // stops at, so the pixels it passes are exactly the reference's.
 __global__ __launch_bounds__(64) void k_h_extent(const HLine* __restrict__ lines, HDepthDev dp, int D, long W,
                                                  const double* __restrict__ grid, const double* __restrict__ cont,
-                                                 double cutoff, DeviceTables tb, HPair* __restrict__ pairs) {
+                                                 const double* __restrict__ stim, double cutoff, DeviceTables tb,
+                                                 HPair* __restrict__ pairs) {
   const int l = blockIdx.x, d = blockIdx.y, lane = threadIdx.x;
   const HLine& L = lines[l];
   HPair pr{0.0, 0.0, 0.0, 0, 0, 0, 0};
@@ -3170,6 +3171,10 @@ __global__ __launch_bounds__(64) void k_h_extent(const HLine* __restrict__ lines
   }
   pr.active = 1;
   const double temp = dp.temp[d];
+  // STIM at a pixel: the caller's, else the engine's 1 - exp(-h nu / k T)
+  auto stim_at = [&](long idx, double wave) {
+    return stim ? stim[(long)d * W + idx] : planck_terms(wave, temp).stim;
+  };
   const bool simple = L.simple != 0;
   const bool taper = !simple && pr.wtail > pr.wcon;
   const long c = L.c;
@@ -3177,7 +3182,7 @@ __global__ __launch_bounds__(64) void k_h_extent(const HLine* __restrict__ lines
   if (c >= 0 && c < W) {
     const double wave = grid[c];
     if (!(!simple && wave < pr.wcon)) {
-      const double v = h_value(L, pr, L.main, L.wl, st, wave, planck_terms(wave, temp).stim, taper, tb);
+      const double v = h_value(L, pr, L.main, L.wl, st, wave, stim_at(c, wave), taper, tb);
       pr.center_add = v >= cont[(long)d * W + c] * cutoff;
     }
   }
@@ -3199,13 +3204,13 @@ __global__ __launch_bounds__(64) void k_h_extent(const HLine* __restrict__ lines
             if (wave > L.wlminus1) {
               stop = true;
             } else if (!(wave < pr.wcon)) {
-              const double sv = planck_terms(wave, temp).stim;
+              const double sv = stim_at(idx, wave);
               double v = h_value(L, pr, L.main, L.wl, st, wave, sv, taper, tb);
               if (wave > L.redcut && h_value(L, pr, L.minus2, L.wlminus2, st, wave, sv, taper, tb) >= v) v = 0.0;
               stop = v <= 0.0 || v < cut;
             }
           } else {
-            const double v = h_value(L, pr, L.main, L.wl, st, wave, planck_terms(wave, temp).stim, false, tb);
+            const double v = h_value(L, pr, L.main, L.wl, st, wave, stim_at(idx, wave), false, tb);
             stop = v <= 0.0 || v < cut;
           }
         }
@@ -3234,7 +3239,7 @@ __global__ __launch_bounds__(64) void k_h_extent(const HLine* __restrict__ lines
           if (!simple && (wave < pr.wcon || wave < L.wlplus1)) {
             stop = true;
           } else {
-            const double sv = planck_terms(wave, temp).stim;
+            const double sv = stim_at(idx, wave);
             double v = h_value(L, pr, L.main, L.wl, st, wave, sv, taper, tb);
             if (!simple && wave < L.bluecut && h_value(L, pr, L.plus2, L.wlplus2, st, wave, sv, taper, tb) >= v)
               v = 0.0;
@@ -3258,12 +3263,13 @@ __global__ __launch_bounds__(64) void k_h_extent(const HLine* __restrict__ lines
 __global__ __launch_bounds__(256) void k_h_gather(const HLine* __restrict__ lines, const HPair* __restrict__ pairs,
                                                   int NH, int D, long W, const double* __restrict__ grid,
                                                   const double* __restrict__ temp, const double* __restrict__ state,
-                                                  DeviceTables tb, double* __restrict__ asynth) {
+                                                  const double* __restrict__ stim, DeviceTables tb,
+                                                  double* __restrict__ asynth) {
   const long w = blockIdx.x * 256L + threadIdx.x;
   const int d = blockIdx.y;
   if (w >= W) return;
   const double wave = grid[w];
-  const double sv = planck_terms(wave, temp[d]).stim;
+  const double sv = stim ? stim[(long)d * W + w] : planck_terms(wave, temp[d]).stim;
   const double* st = state + (long)d * hyd::kStateN;
   double sum = 0.0;
   for (int l = 0; l < NH; ++l) {
@@ -3308,8 +3314,8 @@ HydrogenStage* hstage(synthe_ctx* ctx) {
 extern "C" {
 
 int synthe_hydrogen_lines(synthe_ctx* ctx, const synthe_hlines* hl, const synthe_hdepths* hd, double microturb_kms,
-                          double cutoff, long n_wl, const double* wl, const double* cont, double* asynth,
-                          synthe_stream stream) {
+                          double cutoff, long n_wl, const double* wl, const double* cont, const double* stim,
+                          double* asynth, synthe_stream stream) {
   const char* who = "synthe_hydrogen_lines";
   if (!ctx || !hl || !hd || !wl || !cont || !asynth) return fail(SYNTHE_EINVAL, std::string(who) + ": NULL argument");
   const long NH = hl->n_lines, D = hd->n_depths;
@@ -3397,10 +3403,10 @@ int synthe_hydrogen_lines(synthe_ctx* ctx, const synthe_hlines* hl, const synthe
   HDepthDev dp{d_depth, d_depth + D, d_depth + 2 * D, d_depth + 3 * D, d_depth + 4 * D, d_depth + 6 * D,
                d_depth + 5 * D};
   hipLaunchKernelGGL(k_h_extent, dim3((unsigned)NH, (unsigned)D), dim3(64), 0, st, d_lines, dp, (int)D, n_wl, wl,
-                     cont, cutoff, ctx->tb, d_pairs);
+                     cont, stim, cutoff, ctx->tb, d_pairs);
   HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(k_h_gather, dim3((unsigned)((n_wl + 255) / 256), (unsigned)D), dim3(256), 0, st, d_lines,
-                     d_pairs, (int)NH, (int)D, n_wl, wl, dp.temp, dp.state, ctx->tb, asynth);
+                     d_pairs, (int)NH, (int)D, n_wl, wl, dp.temp, dp.state, stim, ctx->tb, asynth);
   HIP_TRY(hipGetLastError());
   return SYNTHE_OK;
 }

@@ -308,7 +308,7 @@ class SpectrumPlan:
         s = stream if stream is not None else N.stream_handle()
         N.check("synthe_hydrogen_lines", self.ctx.lib.synthe_hydrogen_lines(
             self.ctx.handle, ct_byref(hl), ct_byref(hd), self.microturb, self.cutoff, self.W, N.ptr(self.d_grid),
-            N.ptr(self.kapmin), N.ptr(self.asynth), s))
+            N.ptr(self.kapmin), None, N.ptr(self.asynth), s))
 
     def run_rt(self, stream=None):
         s = stream if stream is not None else N.stream_handle()

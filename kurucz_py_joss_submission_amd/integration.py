@@ -32,6 +32,7 @@ BINDINGS: Tuple[Tuple[str, str, str], ...] = (
     ("engine.radiative", "solve_lte_frequency", "solve_lte_frequency"),
     ("engine.opacity", "solve_lte_spectrum", "solve_lte_spectrum"),
     ("physics.bfudge", "compute_bfudge_and_slinec", "compute_bfudge_and_slinec"),
+    ("engine.opacity", "_compute_hydrogen_line_opacity", "compute_hydrogen_line_opacity"),
 )
 
 

@@ -647,7 +647,8 @@ class HydrogenLines:
         return int(self.index.size)
 
 
-def hydrogen_lines(catalog: LineCatalog, grid: np.ndarray) -> HydrogenLines:
+def hydrogen_lines(catalog, grid: np.ndarray) -> HydrogenLines:
+    catalog = as_line_catalog(catalog)
     lt = np.asarray(catalog.line_types, np.int64)
     idx = np.nonzero(((lt == -1) | (lt == -2)) & (np.asarray(catalog.ion_stages, np.int64) == 1))[0]
     if idx.size and grid.size:

@@ -143,6 +143,53 @@ def compute_asynth_from_transp(transp: np.ndarray, catalog: LineCatalog, atmosph
     return _host(out)
 
 
+def compute_hydrogen_line_opacity(catalog, pops, atmosphere_model: AtmosphereModel, wavelength_grid: np.ndarray,
+                                  continuum: np.ndarray, stim: np.ndarray, cutoff: float,
+                                  microturb_kms: float = 0.0) -> np.ndarray:
+    """engine/opacity.py:1706-1884 (_compute_hydrogen_line_opacity) -> AHLINE [D, W].
+
+    The catalog's H I lines (type -1/-2, ion stage 1) with their HPROF4
+    profiles, walked as engine/opacity.py:1547-1704 walks them.  The per-depth
+    quantities the reference takes from pops (Boltzmann factors, the hydrogen
+    state) are computed from atmosphere_model exactly as
+    physics/populations.py:130-240 computes them, so pops is not read; stim and
+    continuum [D, W] are used as given.  Zero where the reference returns
+    zero (no populations, Doppler widths or mass density)."""
+    from .atmosphere import depth_hckt
+    from .lines import hydrogen_lines
+
+    torch = _t()
+    atm = atmosphere_model
+    wl = np.asarray(wavelength_grid, dtype=np.float64)
+    D, W = atm.layers, wl.size
+    out = torch.zeros((D, W), dtype=torch.float64, device="cuda")
+    h = hydrogen_lines(catalog, wl)
+    pop, dop, rho = (getattr(atm, n, None) for n in ("population_per_ion", "doppler_per_ion", "mass_density"))
+    if len(h) == 0 or pop is None or dop is None or rho is None or np.asarray(pop).shape[2] < 1:
+        return _host(out)
+    f = lambda v: None if v is None else np.ascontiguousarray(v, np.float64)  # noqa: E731
+    xnfph = getattr(atm, "xnfph", None)
+    keep = [f(atm.temperature), f(atm.electron_density), f(rho), f(depth_hckt(atm)),
+            f(np.asarray(pop)[:, 0, 0]), f(np.asarray(dop)[:, 0, 0]), f(getattr(atm, "xnf_he1", None)),
+            f(getattr(atm, "xnf_h2", None)),
+            None if xnfph is None else np.ascontiguousarray(np.asarray(xnfph, np.float64).reshape(D, -1)[:, :2]),
+            f(getattr(atm, "dopph", None)), f(getattr(atm, "turbulent_velocity", None))]
+    hl = N.HLines(len(h), *(x.ctypes.data for x in (h.wavelength, h.center, h.gf, h.elo, h.n_lower, h.n_upper)))
+    hd = N.HDepths(D, *(None if x is None else x.ctypes.data for x in keep))
+    cont = np.asarray(continuum, np.float64)
+    st = np.asarray(stim, np.float64)
+    if cont.shape != (D, W) or st.shape != (D, W):
+        raise ValueError(f"continuum and stim must be [D, W] = {(D, W)}")
+    import ctypes
+
+    ctx = N.context()
+    d_wl, d_c, d_s = _dev(wl), _dev(cont), _dev(st)
+    N.check("synthe_hydrogen_lines", ctx.lib.synthe_hydrogen_lines(
+        ctx.handle, ctypes.byref(hl), ctypes.byref(hd), float(microturb_kms), float(cutoff), W, N.ptr(d_wl),
+        N.ptr(d_c), N.ptr(d_s), N.ptr(out), N.stream_handle()))
+    return _host(out)
+
+
 def _departure_factor(data, layers: int) -> np.ndarray:
     """physics/bfudge.py:26-69 (_prep_factor with one column): a missing table
     is ones; a [D] or [D][k] table gives its first column; another row count

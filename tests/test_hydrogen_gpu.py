@@ -105,3 +105,23 @@ def test_hydrogen_only_catalog(gpu, oracle):
     assert np.max(np.abs(asy[m] - a_ref[m]) / a_ref[m]) < 1e-11
     floor = 1e-3 * np.median(ref.continuum)
     assert np.max(np.abs(plan.flux.cpu().numpy() - ref.flux) / np.maximum(np.abs(ref.flux), floor)) < 1e-6
+
+
+@pytest.mark.gpu
+def test_drop_in_hydrogen_line_opacity(gpu, oracle):
+    """physics.compute_hydrogen_line_opacity (the drop-in for
+    engine/opacity.py:1706) with the caller's continuum and STIM arrays."""
+    import catalog as C
+
+    from kurucz_py_joss_submission_amd import physics
+
+    atm, cat, path = _workload()
+    ocat = C.filter_by_range(C.read_gfall(path), W0, W1)
+    grid = oracle.wavelength_grid(W0, W1, 300000.0)
+    ca, cs = oracle.continuum(atm, grid)
+    _, stim, _, _ = oracle.freq_quantities(grid, atm.temperature)
+    ref, _ = oracle.hydrogen_lines(ocat, atm, grid, ca + cs, stim, 1e-3, 0.7)
+    got = physics.compute_hydrogen_line_opacity(cat, None, atm, grid, ca + cs, stim, 1e-3, 0.7)
+    assert np.count_nonzero(got) == np.count_nonzero(ref) > 0
+    nz = ref > 0
+    assert np.max(np.abs(got[nz] - ref[nz]) / ref[nz]) < 1e-11
