@@ -334,12 +334,12 @@ __global__ void k_cont_src_batch(const ContSrcArgs* __restrict__ a, int n) {
 // (line_opacity.py:1160-1272 prep, 410-537 phases 1-2) for one (line, depth)
 // pair per thread.  mode bit 0: compute transp/valid (else read transp_in /
 // valid_in); bit 1: compute wing parameters and the reach class.
-__global__ void k_transp_wing(LinesDev ln, DepthsDev dp, long W, const double* __restrict__ kapmin,
+__device__ __forceinline__ void k_transp_wing_at(long bx_, int by_, long gx_, LinesDev ln, DepthsDev dp, long W, const double* __restrict__ kapmin,
                               double cutoff, double micro, const double* __restrict__ wl, DeviceTables tb,
                               int mode,
                               double* transp, uint8_t* valid, const double* transp_in,
                               const uint8_t* valid_in, WingParams* wp, uint8_t* cls) {
-  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long i = bx_ * (long)blockDim.x + threadIdx.x;
   long L = ln.n;
   int D = dp.n;
   if (i >= L * D) return;
@@ -489,6 +489,14 @@ __global__ void k_transp_wing(LinesDev ln, DepthsDev dp, long W, const double* _
   cls[i] = k_out;
 }
 
+__global__ void k_transp_wing(LinesDev ln, DepthsDev dp, long W, const double* __restrict__ kapmin,
+                              double cutoff, double micro, const double* __restrict__ wl, DeviceTables tb,
+                              int mode,
+                              double* transp, uint8_t* valid, const double* transp_in,
+                              const uint8_t* valid_in, WingParams* wp, uint8_t* cls) {
+  k_transp_wing_at(blockIdx.x, blockIdx.y, gridDim.x, ln, dp, W, kapmin, cutoff, micro, wl, tb, mode, transp, valid, transp_in, valid_in, wp, cls);
+}
+
 constexpr int kListDepths = 8;  // depths per workgroup row of the class count and scatter
 
 // Per (block of 256 centre-sorted lines, depth): pair counts per reach class.
@@ -502,15 +510,15 @@ __device__ __forceinline__ void load_row_classes(long l, int D, int d0, int d1, 
   for (int t = 0; t < kListDepths; ++t) kc[t] = (l >= 0 && d0 + t < d1) ? cls[l * D + d0 + t] : (uint8_t)0xFF;
 }
 
-__global__ __launch_bounds__(kBlock) void k_class_count(long L, int D, const int64_t* __restrict__ order,
+__device__ __forceinline__ void k_class_count_at(long bx_, int by_, long gx_, long L, int D, const int64_t* __restrict__ order,
                                                         const uint8_t* __restrict__ cls,
                                                         int32_t* __restrict__ counts) {
   __shared__ int sh[kBlock / 64][kClasses];
-  long s = blockIdx.x * (long)kBlock + threadIdx.x;
+  long s = bx_ * (long)kBlock + threadIdx.x;
   long l = s < L ? order[s] : -1;
   int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   // depths are independent: each workgroup row takes kListDepths of them
-  const int d0 = blockIdx.y * kListDepths;
+  const int d0 = by_ * kListDepths;
   const int d1 = min(D, d0 + kListDepths);
   uint8_t kc[kListDepths];
   load_row_classes(l, D, d0, d1, cls, kc);
@@ -527,18 +535,24 @@ __global__ __launch_bounds__(kBlock) void k_class_count(long L, int D, const int
     if (threadIdx.x < kClasses) {
       int t = 0;
       for (int q = 0; q < kBlock / 64; ++q) t += sh[q][threadIdx.x];
-      counts[((long)blockIdx.x * D + d) * kClasses + threadIdx.x] = t;
+      counts[((long)bx_ * D + d) * kClasses + threadIdx.x] = t;
     }
     __syncthreads();
   }
 }
 
+__global__ __launch_bounds__(kBlock) void k_class_count(long L, int D, const int64_t* __restrict__ order,
+                                                        const uint8_t* __restrict__ cls,
+                                                        int32_t* __restrict__ counts) {
+  k_class_count_at(blockIdx.x, blockIdx.y, gridDim.x, L, D, order, cls, counts);
+}
+
 // Exclusive scan over blocks of counts[blk][d][k] for one (d, k) per workgroup;
 // totals[d][k] gets the sum.
-__global__ __launch_bounds__(kBlock) void k_class_scan(long nblk, int D, int32_t* __restrict__ counts,
+__device__ __forceinline__ void k_class_scan_at(long bx_, int by_, long gx_, long nblk, int D, int32_t* __restrict__ counts,
                                                        int64_t* __restrict__ totals) {
   __shared__ int64_t sh[kBlock];
-  int dk = blockIdx.x;
+  int dk = bx_;
   int d = dk / kClasses, kk = dk % kClasses;
   int64_t carry = 0;
   for (long base = 0; base < nblk; base += kBlock) {
@@ -560,9 +574,14 @@ __global__ __launch_bounds__(kBlock) void k_class_scan(long nblk, int D, int32_t
   if (threadIdx.x == 0) totals[dk] = carry;
 }
 
+__global__ __launch_bounds__(kBlock) void k_class_scan(long nblk, int D, int32_t* __restrict__ counts,
+                                                       int64_t* __restrict__ totals) {
+  k_class_scan_at(blockIdx.x, blockIdx.y, gridDim.x, nblk, D, counts, totals);
+}
+
 // regions[d][k] = start of the (d, k) list; regions[d][kClasses] = end.
-__global__ void k_regions(int D, long L, const int64_t* __restrict__ totals, int64_t* __restrict__ regions) {
-  int d = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_regions_at(long bx_, int by_, long gx_, int D, long L, const int64_t* __restrict__ totals, int64_t* __restrict__ regions) {
+  int d = bx_ * blockDim.x + threadIdx.x;
   if (d >= D) return;
   int64_t pos = (int64_t)d * L;
   for (int k = 0; k < kClasses; ++k) {
@@ -572,19 +591,23 @@ __global__ void k_regions(int D, long L, const int64_t* __restrict__ totals, int
   regions[d * (kClasses + 1) + kClasses] = pos;
 }
 
+__global__ void k_regions(int D, long L, const int64_t* __restrict__ totals, int64_t* __restrict__ regions) {
+  k_regions_at(blockIdx.x, blockIdx.y, gridDim.x, D, L, totals, regions);
+}
+
 // Stable scatter of wing pairs into their (depth, class) lists in centre order.
-__global__ __launch_bounds__(kBlock) void k_class_scatter(
+__device__ __forceinline__ void k_class_scatter_at(long bx_, int by_, long gx_, 
     long L, int D, const int64_t* __restrict__ order, const uint8_t* __restrict__ cls,
     const int32_t* __restrict__ offsets, const int64_t* __restrict__ regions,
     const WingParams* __restrict__ wp, WingHead* __restrict__ list_cm, WingParams* __restrict__ list_p,
     int32_t* __restrict__ list_l) {
   __shared__ int sh[kBlock / 64][kClasses];
-  long s = blockIdx.x * (long)kBlock + threadIdx.x;
+  long s = bx_ * (long)kBlock + threadIdx.x;
   long l = s < L ? order[s] : -1;
   int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   unsigned long long ltmask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   // depths are independent: each workgroup row takes kListDepths of them
-  const int d0 = blockIdx.y * kListDepths;
+  const int d0 = by_ * kListDepths;
   const int d1 = min(D, d0 + kListDepths);
   uint8_t kc[kListDepths];
   load_row_classes(l, D, d0, d1, cls, kc);
@@ -603,7 +626,7 @@ __global__ __launch_bounds__(kBlock) void k_class_scatter(
     if (k != 0xFF) {
       int before = 0;
       for (int q = 0; q < wave; ++q) before += sh[q][k];
-      int64_t pos = regions[d * (kClasses + 1) + k] + offsets[((long)blockIdx.x * D + d) * kClasses + k] +
+      int64_t pos = regions[d * (kClasses + 1) + k] + offsets[((long)bx_ * D + d) * kClasses + k] +
                     before + rank;
       const WingParams p = wp[l * D + d];
       // the zone ends at MAXSTEP when MAXPROF caps a far wing below N10DOP
@@ -624,17 +647,25 @@ __global__ __launch_bounds__(kBlock) void k_class_scatter(
   }
 }
 
+__global__ __launch_bounds__(kBlock) void k_class_scatter(
+    long L, int D, const int64_t* __restrict__ order, const uint8_t* __restrict__ cls,
+    const int32_t* __restrict__ offsets, const int64_t* __restrict__ regions,
+    const WingParams* __restrict__ wp, WingHead* __restrict__ list_cm, WingParams* __restrict__ list_p,
+    int32_t* __restrict__ list_l) {
+  k_class_scatter_at(blockIdx.x, blockIdx.y, gridDim.x, L, D, order, cls, offsets, regions, wp, list_cm, list_p, list_l);
+}
+
 // Per depth: the largest Voigt-zone reach over the list entries, i.e.
 // N10DOP for pairs with a far wing and MAXSTEP (<= N10DOP) for the others.
 // near_reach[] must be zero on entry; the result is a max, so it does not
 // depend on the order of the atomics.
-__global__ __launch_bounds__(kBlock) void k_near_reach(int D, const int64_t* __restrict__ regions,
+__device__ __forceinline__ void k_near_reach_at(long bx_, int by_, long gx_, int D, const int64_t* __restrict__ regions,
                                                        const WingHead* __restrict__ list_cm,
                                                        int32_t* __restrict__ near_reach) {
-  const int d = blockIdx.y;
+  const int d = by_;
   const long beg = regions[d * (kClasses + 1)], end = regions[d * (kClasses + 1) + kClasses];
   int m = 0;
-  for (long j = beg + blockIdx.x * (long)kBlock + threadIdx.x; j < end; j += (long)gridDim.x * kBlock) {
+  for (long j = beg + bx_ * (long)kBlock + threadIdx.x; j < end; j += (long)gx_ * kBlock) {
     const WingHead h = list_cm[j];
     const int r = h.nhi - head_c(h);
     m = r > m ? r : m;
@@ -644,6 +675,12 @@ __global__ __launch_bounds__(kBlock) void k_near_reach(int D, const int64_t* __r
     m = o > m ? o : m;
   }
   if ((threadIdx.x & 63) == 0 && m > 0) atomicMax(near_reach + d, m);
+}
+
+__global__ __launch_bounds__(kBlock) void k_near_reach(int D, const int64_t* __restrict__ regions,
+                                                       const WingHead* __restrict__ list_cm,
+                                                       int32_t* __restrict__ near_reach) {
+  k_near_reach_at(blockIdx.x, blockIdx.y, gridDim.x, D, regions, list_cm, near_reach);
 }
 
 __global__ void k_fill_u64(long n, unsigned long long* p) {
@@ -734,17 +771,17 @@ __device__ __forceinline__ void near_flush(int lane, int qn, int& cnt, double& a
 }
 
 template <bool kStats>
-__global__ __launch_bounds__(256) void k_near(long W, int D, const WingHead* __restrict__ list_cm,
+__device__ __forceinline__ void k_near_at(long bx_, int by_, long gx_, long W, int D, const WingHead* __restrict__ list_cm,
                                               const WingParams* __restrict__ list_p,
                                               const int64_t* __restrict__ regions,
                                               const int32_t* __restrict__ near_reach, DeviceTables tb,
                                               double* __restrict__ out, unsigned long long* __restrict__ evals) {
   __shared__ uint64_t s_queue[4][kNearQueue];
   __shared__ double s_vals[4][kNearSlots][64];
-  const int d = blockIdx.y;
+  const int d = by_;
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const long v0 = ((long)blockIdx.x * 4 + wv) * 64;
+  const long v0 = ((long)bx_ * 4 + wv) * 64;
   if (v0 >= W) return;  // whole wave; the waves of a block share no LDS
   const long w = v0 + lane;
   // pixel indices and centres fit 32 bits (n_wl <= 2^30): scalar entry tests
@@ -805,6 +842,15 @@ __global__ __launch_bounds__(256) void k_near(long W, int D, const WingHead* __r
       atomicAdd(evals + 1, nev);
     }
   }
+}
+
+template <bool kStats>
+__global__ __launch_bounds__(256) void k_near(long W, int D, const WingHead* __restrict__ list_cm,
+                                              const WingParams* __restrict__ list_p,
+                                              const int64_t* __restrict__ regions,
+                                              const int32_t* __restrict__ near_reach, DeviceTables tb,
+                                              double* __restrict__ out, unsigned long long* __restrict__ evals) {
+  k_near_at<kStats>(blockIdx.x, blockIdx.y, gridDim.x, W, D, list_cm, list_p, regions, near_reach, tb, out, evals);
 }
 
 #ifndef SYNTHE_GATHER_PIXELS
@@ -2664,6 +2710,84 @@ static int asynth_prepare(const synthe_ctx* ctx, const synthe_lines* lines, cons
   return SYNTHE_OK;
 }
 
+// ------------------------------------------------- batched prepare stages
+// One instance's arguments for the stages asynth_prepare launches per
+// spectrum (TRANSP with the wing set-up, class lists, Voigt zones).  The
+// batch launches each stage once over all instances: block B of stage k
+// belongs to instance i with off[k][i] <= B < off[k][i+1] and runs that
+// instance's block B - off[k][i] of the per-spectrum grid, through the same
+// device body, so each instance's results are those of its own launches.
+struct PrepInst {
+  LinesDev ln;
+  DepthsDev dp;
+  long W, L, nblk;
+  int D;
+  const double *kapmin, *wl;
+  double cutoff;
+  double* transp;
+  uint8_t* valid;
+  const int64_t* wing_order;
+  double* asynth;
+  AsynthBufs b;
+};
+
+constexpr int kPrepStages = 8;  // zero, TRANSP + set-up, count, scan, regions, scatter, reach, Voigt zones
+
+// Blocks of stage k for one instance (per-spectrum grid, flattened x-major).
+static long prep_blocks(int k, const PrepInst& P) {
+  const long pairs = P.L * (long)P.D;
+  const long rows = (P.D + kListDepths - 1) / kListDepths;
+  switch (k) {
+    case 0: return 1;
+    case 1: return pairs > 0 ? (pairs + kBlock - 1) / kBlock : 0;
+    case 2: return pairs > 0 ? P.nblk * rows : 0;
+    case 3: return pairs > 0 ? (long)P.D * kClasses : 0;
+    case 4: return pairs > 0 ? (P.D + kBlock - 1) / kBlock : 0;
+    case 5: return pairs > 0 ? P.nblk * rows : 0;
+    case 6: return pairs > 0 ? 64L * P.D : 0;
+    default: return ((P.W + 255) / 256) * P.D;
+  }
+}
+
+template <int kStage>
+__global__ __launch_bounds__(kBlock) void k_prep_batch(const PrepInst* __restrict__ inst,
+                                                       const int64_t* __restrict__ off, int n, DeviceTables tb) {
+  int lo = 0, hi = n - 1;  // last instance with off[i] <= block
+  const long B = blockIdx.x;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= B) lo = mid;
+    else hi = mid - 1;
+  }
+  const PrepInst& P = inst[lo];
+  const long local = B - off[lo];
+  const AsynthBufs& b = P.b;
+  if constexpr (kStage == 0) {  // the counters, Voigt reaches and (without pairs) the list regions
+    for (int t = threadIdx.x; t < 5; t += kBlock) b.ev[t] = 0ull;
+    for (int t = threadIdx.x; t < P.D; t += kBlock) b.near_reach[t] = 0;
+    if (P.L * (long)P.D == 0)
+      for (long t = threadIdx.x; t < (long)P.D * (kClasses + 1); t += kBlock) b.reg[t] = 0;
+  } else if constexpr (kStage == 1) {
+    k_transp_wing_at(local, 0, local + 1, P.ln, P.dp, P.W, P.kapmin, P.cutoff, 0.0, P.wl, tb, 3, P.transp, P.valid,
+                     nullptr, nullptr, b.wp, b.cls);
+  } else if constexpr (kStage == 2) {
+    k_class_count_at(local % P.nblk, (int)(local / P.nblk), P.nblk, P.L, P.D, P.wing_order, b.cls, b.cnt);
+  } else if constexpr (kStage == 3) {
+    k_class_scan_at(local, 0, (long)P.D * kClasses, P.nblk, P.D, b.cnt, b.tot);
+  } else if constexpr (kStage == 4) {
+    k_regions_at(local, 0, local + 1, P.D, P.L, b.tot, b.reg);
+  } else if constexpr (kStage == 5) {
+    k_class_scatter_at(local % P.nblk, (int)(local / P.nblk), P.nblk, P.L, P.D, P.wing_order, b.cls, b.cnt, b.reg,
+                       b.wp, b.lcm, b.lp, b.list_l);
+  } else if constexpr (kStage == 6) {
+    k_near_reach_at(local % 64, (int)(local / 64), 64, P.D, b.reg, b.lcm, b.near_reach);
+  } else {
+    const long gx = (P.W + 255) / 256;
+    k_near_at<false>(local % gx, (int)(local / gx), gx, P.W, P.D, b.lcm, b.lp, b.reg, b.near_reach, tb, P.asynth,
+                     nullptr);
+  }
+}
+
 // ---------------------------------------------------------------- RT pieces
 struct RtLayout {
   size_t o_flags, o_list, o_cnt, o_args, total;
@@ -3459,6 +3583,9 @@ struct synthe_batch {
   int precision = SYNTHE_PREC_FP64;
   long n_items = 0;              // gather items of the current tile
   std::vector<double> tune_ms;  // [tiles][4 grids], ms, or empty
+  PrepInst* d_prep = nullptr;     // [n] prepare-stage arguments, instance order
+  int64_t* d_poff = nullptr;       // [kPrepStages][n + 1] block offsets of each stage
+  long prep_total[kPrepStages] = {};
   hipStream_t streams[kBatchStreams] = {};
   hipEvent_t ev_fork = nullptr, ev_mid = nullptr;
   hipEvent_t ev_join[kBatchStreams] = {};
@@ -3518,48 +3645,36 @@ int batch_cont_src(synthe_batch* b, hipStream_t st) {
   return SYNTHE_OK;
 }
 
-// Per-instance stages before the gather that follow batch_cont_src: TRANSP
-// with the wing set-up (one pass), class lists, Voigt zones.
-int batch_prepare_one(synthe_batch* b, BatchInst& bi, hipStream_t st) {
-  synthe_ctx* ctx = b->ctx;
-  const synthe_instance& in = bi.in;
-  const long W = in.n_wl;
-  char* base = b->arena;
-  double* kapmin = (double*)(base + bi.o_kapmin);
-  double* asy = (double*)(base + bi.o_asynth);
-  double* transp = (double*)(base + bi.o_transp);
-  uint8_t* valid = (uint8_t*)(base + bi.o_valid);
-  const AsynthBufs bufs = asynth_bufs(base + bi.o_asy, bi.alay);
-  return asynth_prepare(ctx, &in.lines, &in.depths, W, in.wl, transp, valid, kapmin, in.cutoff, in.wing_order, asy,
-                        false, bufs, st, true, transp, valid);
-}
-
-int fork_streams(synthe_batch* b, hipEvent_t ev, hipStream_t s0) {
-  HIP_TRY(hipEventRecord(ev, s0));
-  for (int k = 0; k < kBatchStreams; ++k) HIP_TRY(hipStreamWaitEvent(b->streams[k], ev, 0));
-  return SYNTHE_OK;
-}
-
-int join_streams(synthe_batch* b, hipStream_t s0) {
-  for (int k = 0; k < kBatchStreams; ++k) {
-    HIP_TRY(hipEventRecord(b->ev_join[k], b->streams[k]));
-    HIP_TRY(hipStreamWaitEvent(s0, b->ev_join[k], 0));
+// The prepare stages of every instance, one launch per stage (k_prep_batch).
+int batch_prepare(synthe_batch* b, hipStream_t st) {
+  const int n = (int)b->inst.size();
+#define SYNTHE_PREP(K)                                                                                      \
+  if (b->prep_total[K] > 0) {                                                                               \
+    hipLaunchKernelGGL(k_prep_batch<K>, dim3((unsigned)b->prep_total[K]), dim3(kBlock), 0, st, b->d_prep,   \
+                       b->d_poff + (long)(K) * (n + 1), n, b->ctx->tb);                                    \
+    HIP_TRY(hipGetLastError());                                                                             \
   }
+  SYNTHE_PREP(0)
+  SYNTHE_PREP(1)
+  SYNTHE_PREP(2)
+  SYNTHE_PREP(3)
+  SYNTHE_PREP(4)
+  SYNTHE_PREP(5)
+  SYNTHE_PREP(6)
+  SYNTHE_PREP(7)
+#undef SYNTHE_PREP
   return SYNTHE_OK;
 }
 
-// The whole batch: prepare stages fanned out over the batch's streams, one
-// persistent gather over all instances, one RT over all instances'
-// wavelengths (F_lambda stored directly).  Launches and event records only.
+// The whole batch: each prepare stage once over all instances, one persistent
+// gather over all instances, one RT over all instances' wavelengths
+// (F_lambda stored directly).  Launches only.
 int batch_enqueue(synthe_batch* b, hipStream_t s0, bool prepare, bool gather, bool finish) {
   const int n = (int)b->inst.size();
   int rc;
   if (prepare) {
     if ((rc = batch_cont_src(b, s0))) return rc;
-    if ((rc = fork_streams(b, b->ev_fork, s0))) return rc;
-    for (int i = 0; i < n; ++i)
-      if ((rc = batch_prepare_one(b, b->inst[b->order[i]], b->streams[i % kBatchStreams]))) return rc;
-    if ((rc = join_streams(b, s0))) return rc;
+    if ((rc = batch_prepare(b, s0))) return rc;
   }
   if (gather && (rc = launch_gather_batch(b, b->kwp, b->gather_blocks, s0))) return rc;
   if (finish && (rc = rt_launch_set(b->ctx, b->d_rt, n, b->rt_g, b->rt_base, b->rlay, s0))) return rc;
@@ -3630,6 +3745,8 @@ int synthe_batch_create(synthe_ctx* ctx, int n, const synthe_instance* inst, int
   const size_t o_order = off; off = align_up(off + n * sizeof(int32_t));
   const size_t o_items = off; off = align_up(off + (n + 1) * sizeof(int64_t));
   const size_t o_counter = off; off = align_up(off + sizeof(unsigned long long));
+  const size_t o_prep = off; off = align_up(off + n * sizeof(PrepInst));
+  const size_t o_poff = off; off = align_up(off + (size_t)kPrepStages * (n + 1) * sizeof(int64_t));
   if (hipMalloc(&b->arena, off) != hipSuccess) {
     b->arena = nullptr;
     batch_free(b);
@@ -3644,6 +3761,8 @@ int synthe_batch_create(synthe_ctx* ctx, int n, const synthe_instance* inst, int
   b->d_order = (int32_t*)(b->arena + o_order);
   b->d_items = (int64_t*)(b->arena + o_items);
   b->d_counter = (unsigned long long*)(b->arena + o_counter);
+  b->d_prep = (PrepInst*)(b->arena + o_prep);
+  b->d_poff = (int64_t*)(b->arena + o_poff);
   int rc = SYNTHE_OK;
   auto bail = [&](int code) {
     batch_free(b);
@@ -3661,6 +3780,33 @@ int synthe_batch_create(synthe_ctx* ctx, int n, const synthe_instance* inst, int
     args[i] = gather_args(&in.lines, &in.depths, in.n_wl, in.wl, (double*)(b->arena + bi.o_transp),
                           (uint8_t*)(b->arena + bi.o_valid), in.center_order, in.center_start,
                           (double*)(b->arena + bi.o_asynth), asynth_bufs(b->arena + bi.o_asy, bi.alay));
+  }
+  std::vector<PrepInst> prep(n);
+  std::vector<int64_t> poff((size_t)kPrepStages * (n + 1), 0);
+  for (int i = 0; i < n; ++i) {
+    BatchInst& bi = b->inst[i];
+    const synthe_instance& in = bi.in;
+    PrepInst& P = prep[i];
+    P.ln = to_dev(&in.lines);
+    P.dp = to_dev(&in.depths);
+    P.W = in.n_wl;
+    P.L = in.lines.n_lines;
+    P.D = (int)in.depths.n_depths;
+    P.nblk = (P.L + kBlock - 1) / kBlock;
+    P.kapmin = (const double*)(b->arena + bi.o_kapmin);
+    P.wl = in.wl;
+    P.cutoff = in.cutoff;
+    P.transp = (double*)(b->arena + bi.o_transp);
+    P.valid = (uint8_t*)(b->arena + bi.o_valid);
+    P.wing_order = in.wing_order;
+    P.asynth = (double*)(b->arena + bi.o_asynth);
+    P.b = asynth_bufs(b->arena + bi.o_asy, bi.alay);
+    for (int k = 0; k < kPrepStages; ++k)
+      poff[(size_t)k * (n + 1) + i + 1] = poff[(size_t)k * (n + 1) + i] + prep_blocks(k, P);
+  }
+  for (int k = 0; k < kPrepStages; ++k) {
+    b->prep_total[k] = poff[(size_t)k * (n + 1) + n];
+    if (b->prep_total[k] > 0x7fffffffL) return bail(fail(SYNTHE_ERANGE, "synthe_batch_create: prepare grid too large"));
   }
   std::vector<ContSrcArgs> cs(n);
   for (int i = 0; i < n; ++i) {
@@ -3690,6 +3836,8 @@ int synthe_batch_create(synthe_ctx* ctx, int n, const synthe_instance* inst, int
       hipMemcpy(b->d_args, args.data(), n * sizeof(GatherArgs), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(b->d_cs, cs.data(), n * sizeof(ContSrcArgs), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(b->d_order, b->order.data(), n * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->d_prep, prep.data(), n * sizeof(PrepInst), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->d_poff, poff.data(), poff.size() * sizeof(int64_t), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemset(b->zeros, 0, zeros_n * sizeof(double)) != hipSuccess)
     return bail(fail(SYNTHE_EHIP, "synthe_batch_create: upload failed"));
   int dev = 0;
