@@ -141,54 +141,62 @@ SYNTHE_HD double map1_extrap(double x, double t0, double t1, double f0, double f
   return a + (b + 0.0 * x) * x;
 }
 
-// Source of per-depth values for the stream: Src must provide
-//   void get(int j, double* abtot, double* rho, double* snubar, double* alpha)
-// for 0 <= j < n, in increasing rho order.  Out receives the MAP1 results:
+// INTEG + MAP1 of one column as a stepper, fed one depth at a time in
+// increasing rho order (abtot f, rho x, snubar, alpha): start() with depths 0
+// and 1, step() for j = 1 .. n-1 with the values of depth j+1 (zeros past the
+// last), then finish().  Out receives the MAP1 results
 //   void put(int k, double xsbar, double xalpha)
-// for k = 0..50 in increasing order.  Returns TAUNU(0) (to test the label-401
-// gate) through *tau0.  Requires n >= 4.
-template <class Src, class Out>
-SYNTHE_HD void integ_map1_stream(int n, const double* xtau, Src& src, Out& out, double* tau0) {
-  // rolling input window: index j-1, j, j+1 (f = abtot, x = rho)
-  double fa, fb, fc, xa, xb, xc, sb_n, al_n;
-  double sbb, alb;
+// for k = 0..50 in increasing order.  Several columns can advance in
+// lockstep over one read of the inputs.  Requires n >= 4.
+struct IntegMap1 {
+  int n, k;
+  double fb, fc, xb, xc, sb_n, al_n, tau, x;
   // MAP1 window: tau/snubar/alpha at p-2..p+1 (slot 3 = newest)
   double T[4], S[4], A[4];
-  src.get(0, &fb, &xb, &sbb, &alb);
-  src.get(1, &fc, &xc, &sb_n, &al_n);
-  fa = 0.0;
-  xa = 0.0;
-  double tau = fb * xb;
-  *tau0 = tau;
-  T[3] = tau;
-  S[3] = sbb;
-  A[3] = alb;
-  T[0] = T[1] = T[2] = 0.0;
-  S[0] = S[1] = S[2] = 0.0;
-  A[0] = A[1] = A[2] = 0.0;
-  Coef prev = parcoe_p1(0, n, 0.0, fb, fc, 0.0, xb, xc);
-  int k = 0;
-  double x = xtau[0];
-  for (int j = 1; j < n; ++j) {
-    // window now: fa = f[j-2], fb = f[j-1], fc = f[j]; need f[j+1] for P1(j)
-    double fd = 0.0, xd = 0.0, sbd = 0.0, ald = 0.0;
-    if (j + 1 < n) src.get(j + 1, &fd, &xd, &sbd, &ald);
+  Coef prev;
+  const double* xtau;
+
+  SYNTHE_HD double start(int n_, const double* xtau_, double f0, double x0, double sb0, double al0, double f1,
+                         double x1, double sb1, double al1) {
+    n = n_;
+    xtau = xtau_;
+    fb = f0;
+    xb = x0;
+    fc = f1;
+    xc = x1;
+    sb_n = sb1;
+    al_n = al1;
+    tau = fb * xb;
+    T[3] = tau;
+    S[3] = sb0;
+    A[3] = al0;
+    T[0] = T[1] = T[2] = 0.0;
+    S[0] = S[1] = S[2] = 0.0;
+    A[0] = A[1] = A[2] = 0.0;
+    prev = parcoe_p1(0, n, 0.0, fb, fc, 0.0, xb, xc);
+    k = 0;
+    x = xtau[0];
+    return tau;  // TAUNU(0), the label-401 gate
+  }
+
+  // window: fb = f[j-1], fc = f[j]; fd.. = depth j+1 (zeros past the last)
+  template <class Out>
+  SYNTHE_HD void step(int j, double fd, double xd, double sbd, double ald, Out& out) {
     Coef cur = parcoe_p1(j, n, fb, fc, fd, xb, xc, xd);
     Coef s = parcoe_final(j - 1, n, prev, cur);
     tau = integ_step(tau, s, xb, xc);
     prev = cur;
-    // shift input window
-    fa = fb; xa = xb;
-    fb = fc; xb = xc;
-    fc = fd; xc = xd;
-    // push tau[j], snubar[j], alpha[j]
+    fb = fc;
+    xb = xc;
+    fc = fd;
+    xc = xd;
     T[0] = T[1]; T[1] = T[2]; T[2] = T[3]; T[3] = tau;
     S[0] = S[1]; S[1] = S[2]; S[2] = S[3]; S[3] = sb_n;
     A[0] = A[1]; A[1] = A[2]; A[2] = A[3]; A[3] = al_n;
     sb_n = sbd;
     al_n = ald;
     // bracket p = j - 1 (needs tau[p+1] = tau[j]) for points below tau[p]
-    int p = j - 1;
+    const int p = j - 1;
     if (p >= 1) {
       while (k < kXt && x < T[2]) {
         out.put(k, map1_eval(x, p, n, T, S), map1_eval(x, p, n, T, A));
@@ -197,10 +205,10 @@ SYNTHE_HD void integ_map1_stream(int n, const double* xtau, Src& src, Out& out, 
       }
     }
   }
-  (void)fa;
-  (void)xa;
-  // bracket p = n - 1 (l == nold): window slots 1..3 hold n-3..n-1
-  {
+
+  template <class Out>
+  SYNTHE_HD void finish(Out& out) {
+    // bracket p = n - 1 (l == nold): window slots 1..3 hold n-3..n-1
     double Tb[4] = {T[1], T[2], T[3], 0.0};
     double Sb[4] = {S[1], S[2], S[3], 0.0};
     double Ab[4] = {A[1], A[2], A[3], 0.0};
@@ -209,12 +217,30 @@ SYNTHE_HD void integ_map1_stream(int n, const double* xtau, Src& src, Out& out, 
       ++k;
       x = k < kXt ? xtau[k] : 0.0;
     }
+    while (k < kXt) {
+      out.put(k, map1_extrap(x, T[2], T[3], S[2], S[3]), map1_extrap(x, T[2], T[3], A[2], A[3]));
+      ++k;
+      x = k < kXt ? xtau[k] : 0.0;
+    }
   }
-  while (k < kXt) {
-    out.put(k, map1_extrap(x, T[2], T[3], S[2], S[3]), map1_extrap(x, T[2], T[3], A[2], A[3]));
-    ++k;
-    x = k < kXt ? xtau[k] : 0.0;
+};
+
+// One column through the stepper.  Src must provide
+//   void get(int j, double* abtot, double* rho, double* snubar, double* alpha)
+// for 0 <= j < n, in increasing rho order.  Returns TAUNU(0) through *tau0.
+template <class Src, class Out>
+SYNTHE_HD void integ_map1_stream(int n, const double* xtau, Src& src, Out& out, double* tau0) {
+  double f0, x0, sb0, al0, f1, x1, sb1, al1;
+  src.get(0, &f0, &x0, &sb0, &al0);
+  src.get(1, &f1, &x1, &sb1, &al1);
+  IntegMap1 st;
+  *tau0 = st.start(n, xtau, f0, x0, sb0, al0, f1, x1, sb1, al1);
+  for (int j = 1; j < n; ++j) {
+    double fd = 0.0, xd = 0.0, sbd = 0.0, ald = 0.0;
+    if (j + 1 < n) src.get(j + 1, &fd, &xd, &sbd, &ald);
+    st.step(j, fd, xd, sbd, ald, out);
   }
+  st.finish(out);
 }
 
 }  // namespace rt

@@ -1563,11 +1563,18 @@ struct ColSrc {
   long W;
   double freq, f3;
   bool total;  // false: continuum-only solve (aline = sigmal = 0, sline = B)
+  bool check;  // test every depth read (the inputs finite and below 3.4e38, rho > 0) into ok
+  bool ok;
   __device__ __forceinline__ void get(int j, double* f, double* x, double* sb, double* al) {
     long i = (long)j * W;
     double ac = ca[i], sc = cs[i];
     double aln = total ? la[i] : 0.0;
     double sln = total ? ls[i] : 0.0;
+    if (check) {
+      const double r = rhox[j], maxr4 = 3.4e38;
+      ok = ok && r > 0.0 && isfinite(r) && isfinite(ac) && isfinite(sc) && isfinite(aln) && isfinite(sln) &&
+           ac < maxr4 && sc < maxr4 && aln < maxr4 && sln < maxr4;
+    }
     double B = planck_one(freq, f3, temp[j]);
     double sl = B;
     if (total && src) {
@@ -1613,8 +1620,11 @@ struct TauOnlyOut {
 
 constexpr int kRtBlock = 256;
 
-// Fast-path set-up of both JOSH solves per wavelength: validity check, then
-// the streaming INTEG + MAP1.  flags[w]: 1 = needs the general kernel.
+// Fast-path set-up of both JOSH solves per wavelength: the streaming INTEG +
+// MAP1 of the total solve first, whose reads of the five inputs also check
+// every depth, then the continuum-only solve (two of the five again): 7
+// reads per depth.  flags[w]: 1 = needs the general kernel (a depth fails
+// the checks, D < 4, rho not increasing, or either solve takes label 401).
 __global__ __launch_bounds__(kRtBlock, 3) void k_rt_prep(const RtArgs* __restrict__ rt, int n_rt, long G,
                                                       const double* __restrict__ xtau, float* ws,
                                                       uint8_t* __restrict__ flags,
@@ -1626,40 +1636,31 @@ __global__ __launch_bounds__(kRtBlock, 3) void k_rt_prep(const RtArgs* __restric
   const int D = a.D;
   const double *wl = a.wl, *temp = a.temp, *rhox = a.rhox, *ca = a.ca, *cs = a.cs, *la = a.la, *ls = a.ls,
                *src = a.src;
-  const double maxr4 = 3.4e38;
-  bool ok = D >= 4 && rhox[0] < rhox[D - 1];
-  for (int d = 0; d < D && ok; ++d) {
-    long i = (long)d * W + w;
-    double r = rhox[d], a1 = ca[i], a2 = cs[i], a3 = la[i], a4 = ls[i];
-    ok = r > 0.0 && isfinite(r) && isfinite(a1) && isfinite(a2) && isfinite(a3) && isfinite(a4) &&
-         a1 < maxr4 && a2 < maxr4 && a3 < maxr4 && a4 < maxr4;
-  }
-  if (!ok) {
-    flags[g] = 1;
-    list[atomicAdd(list_n, 1)] = g;  // order is irrelevant: one slot per wavelength
-    return;
-  }
-  double wv = wl[w];
-  double freq = kCLightNm / (wv > 1e-12 ? wv : 1e-12);
-  double f3 = pow(freq / 1.0e15, 3.0);
-  uint8_t fl = 0;
-  for (int s = 0; s < 2; ++s) {
-    ColSrc in{ca + w, cs + w, la + w, ls + w, src ? src + w : nullptr, temp, rhox, W, freq, f3, s == 1};
-    double f0, x0, sb0, al0;
-    in.get(0, &f0, &x0, &sb0, &al0);
-    float* base = ws + (long)s * 3 * kNxtau * G + g;
-    ColOut out{base, base + (long)kNxtau * G, base + 2L * kNxtau * G, G, xtau, 0.0,
-               sb0 > kEps38 ? sb0 : kEps38, al0 < 0.0 ? 0.0 : (al0 > 1.0 ? 1.0 : al0)};
-    out.tau0 = f0 * x0;
-    if (out.tau0 > xtau[kNxtau - 1]) {
-      fl = 1;  // label 401: general kernel
-      break;
+  bool fl = !(D >= 4 && rhox[0] < rhox[D - 1]);
+  if (!fl) {
+    double wv = wl[w];
+    double freq = kCLightNm / (wv > 1e-12 ? wv : 1e-12);
+    double f3 = pow(freq / 1.0e15, 3.0);
+    for (int s = 1; s >= 0 && !fl; --s) {
+      ColSrc in{ca + w, cs + w, la + w, ls + w, src ? src + w : nullptr, temp, rhox, W, freq, f3, s == 1, s == 1,
+                true};
+      double f0, x0, sb0, al0;
+      in.get(0, &f0, &x0, &sb0, &al0);
+      float* base = ws + (long)s * 3 * kNxtau * G + g;
+      ColOut out{base, base + (long)kNxtau * G, base + 2L * kNxtau * G, G, xtau, 0.0,
+                 sb0 > kEps38 ? sb0 : kEps38, al0 < 0.0 ? 0.0 : (al0 > 1.0 ? 1.0 : al0)};
+      out.tau0 = f0 * x0;
+      if (out.tau0 > xtau[kNxtau - 1] || !in.ok) {
+        fl = true;  // label 401 or a failed check: general kernel
+        break;
+      }
+      double tau0;
+      rt::integ_map1_stream(D, xtau, in, out, &tau0);
+      fl = !in.ok;
     }
-    double tau0;
-    rt::integ_map1_stream(D, xtau, in, out, &tau0);
   }
-  flags[g] = fl;
-  if (fl) list[atomicAdd(list_n, 1)] = g;
+  flags[g] = fl ? 1 : 0;
+  if (fl) list[atomicAdd(list_n, 1)] = g;  // order is irrelevant: one slot per wavelength
 }
 
 __global__ void k_zero_int(int* p) {
