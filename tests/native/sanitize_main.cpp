@@ -1,5 +1,5 @@
 // Memory and undefined-behaviour check of the CPU code that the tests trust:
-// the oracle (oracle/synthe_oracle.c) and the CPU builds of the device headers
+// the oracle (oracle/synthe_oracle.c, oracle/hydrogen_oracle.c) and the CPU builds of the device headers
 // (near_queue_check.cpp, rt_core_check.cpp, rt_adj_check.cpp).  Built with
 // -fsanitize=address,undefined into one executable by
 // tests/test_sanitize_cpu.py, which then runs it; any report fails the run.
@@ -7,7 +7,9 @@
 // sizes follow the oracle's own calling conventions (oracle/oracle.py).
 //
 //   sanitize_main <tables.bin>
-// tables.bin: int64 n_tabvi, tabvi[n], tabh1[n], xtau[51], ch[51], coefj[51*51].
+// tables.bin: int64 n_tabvi, tabvi[n], tabh1[n], xtau[51], ch[51], coefj[51*51],
+// then the hydrogen tables, each as int64 size + doubles, in the order of
+// orc_set_hydrogen_tables (tests/test_sanitize_cpu.py writes it).
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -71,6 +73,22 @@ void orc_lte_spectrum(long D, long W, const double* wl, const double* temp, cons
                       const double* line_scat, const double* line_source, double* flux_total, double* flux_cont,
                       double* iters, int32_t* iters_w);
 void orc_flux_to_lambda(long W, const double* wl, double* flux);
+void orc_set_hydrogen_tables(const double* propbm, const double* c, const double* d, const double* pp,
+                             const double* beta, const double* stalph, const double* stwtal, const double* istal,
+                             const double* lnghal, const double* stcomp, const double* stcpwt, const double* lncomp,
+                             const double* cut_h2p, int ncut_h2p, const double* cut_h2, int ncut_h2,
+                             const double* asum_lyman, int nasum_lyman, const double* asum, int nasum,
+                             const double* y1wtm, const double* xknmtb, const double* conth, int nconth,
+                             const double* ehyd, const double* extab, const double* extabf, int nfastex);
+void orc_hydrogen_states(long D, const double* temperature, const double* electron_density, const double* xnf_he1,
+                         const double* xnf_h2, const double* xnfph, const double* dopph,
+                         const double* turbulent_velocity, double microturb_kms, int has_he1_h2, double* out);
+double orc_hydrogen_profile(int n, int m, const double* h, double delta_lambda_nm);
+int orc_hydrogen_lines(long NH, long D, long W, const double* grid, const double* wl, const int64_t* center,
+                       const double* gf, const double* elo, const int32_t* n_lower_in, const int32_t* n_upper_in,
+                       const double* pop_h1, const double* dop_h1, const double* rho, const double* hckt,
+                       const double* electron_density, const double* states, const double* cont,
+                       const double* stim, double cutoff, double microturb_kms, double* ahline, double* stats);
 
 // CPU builds of the device headers
 int nq_roundtrip(long j_rel, long ka, int lane, int tag, long* out);
@@ -225,6 +243,62 @@ void oracle_kernels(const orc_tables& tb) {
   orc_set_josh_fp64(0, 0.0);
 }
 
+// Hydrogen lines: states with and without the optional arrays, profiles from
+// Lyman alpha to high members, and the walk over a small grid with lines on,
+// at and beyond its ends.
+void hydrogen(const orc_tables& tb) {
+  rng.seed(5);
+  const long D = 6;
+  std::vector<double> T(D), xne(D), he1(D), h2(D), xnfph(2 * D), dopph(D), vt(D), states(D * 15), st2(D * 15);
+  for (long d = 0; d < D; ++d) {
+    T[d] = uni(4000.0, 20000.0);
+    xne[d] = pow(10.0, uni(10.0, 15.0));
+    he1[d] = pow(10.0, uni(12.0, 15.0));
+    h2[d] = pow(10.0, uni(0.0, 12.0));
+    xnfph[2 * d] = pow(10.0, uni(12.0, 16.0));
+    xnfph[2 * d + 1] = pow(10.0, uni(10.0, 14.0));
+    dopph[d] = uni(1e-5, 4e-5);
+    vt[d] = uni(0.0, 2e5);
+  }
+  orc_hydrogen_states(D, T.data(), xne.data(), he1.data(), h2.data(), xnfph.data(), dopph.data(), vt.data(), 1.0,
+                      1, states.data());
+  orc_hydrogen_states(D, T.data(), xne.data(), nullptr, h2.data(), nullptr, nullptr, nullptr, 0.0, 1, st2.data());
+  double acc = 0.0;
+  const int pairs[][2] = {{1, 2}, {1, 3}, {1, 5}, {2, 3}, {2, 4}, {2, 7}, {2, 30}, {3, 4}, {3, 9}, {4, 5}, {5, 11}};
+  for (const auto& nm : pairs)
+    for (long d = 0; d < D; ++d)
+      for (double dl : {0.0, 0.003, -0.2, 5.0, -60.0}) acc += orc_hydrogen_profile(nm[0], nm[1], &states[d * 15], dl);
+  const long W = orc_wavelength_grid(480.0, 481.0, 300000.0, nullptr, 0);
+  std::vector<double> grid(W);
+  orc_wavelength_grid(480.0, 481.0, 300000.0, grid.data(), W);
+  const double wl[] = {479.0, 480.5, 486.1, 481.0, 434.0};
+  const int64_t center[] = {-1, W / 2, W, W - 1, -1};
+  const double gf[] = {0.1, 0.9, 0.5, 0.05, 0.3}, elo[] = {82259.0, 82259.0, 97492.0, 82259.0, 82259.0};
+  const int32_t nlo[] = {2, 2, 3, 2, 2}, nup[] = {9, 4, 12, 6, 5};
+  std::vector<double> pop(D), dop(D), rho(D), hckt(D), cont(D * W), stim(D * W), ah(D * W, 0.0);
+  for (long d = 0; d < D; ++d) {
+    pop[d] = pow(10.0, uni(12.0, 16.0));
+    dop[d] = uni(1e-5, 4e-5);
+    rho[d] = pow(10.0, uni(-9.0, -7.0));
+    hckt[d] = 1.4388 / T[d];
+    for (long w = 0; w < W; ++w) {
+      cont[d * W + w] = pow(10.0, uni(-3.0, 0.0));
+      stim[d * W + w] = uni(0.5, 1.0);
+    }
+  }
+  double stats = 0.0;
+  const int rc = orc_hydrogen_lines(5, D, W, grid.data(), wl, center, gf, elo, nlo, nup, pop.data(), dop.data(),
+                                    rho.data(), hckt.data(), xne.data(), states.data(), cont.data(), stim.data(),
+                                    1e-3, 1.5, ah.data(), &stats);
+  for (double v : ah) acc += v;
+  const int32_t bad[] = {95};  // n_upper + 2 beyond the damping tables: refused
+  const int rc2 = orc_hydrogen_lines(1, D, W, grid.data(), wl, center, gf, elo, nlo, bad, pop.data(), dop.data(),
+                                     rho.data(), hckt.data(), xne.data(), states.data(), cont.data(), stim.data(),
+                                     1e-3, 0.0, ah.data(), &stats);
+  (void)tb;
+  printf("hydrogen rc=%d rc_bad=%d evals=%.0f checksum=%.6e\n", rc, rc2, stats, acc);
+}
+
 void header_builds(const std::vector<double>& xtau, const std::vector<double>& ch,
                    const std::vector<double>& coefj) {
   std::vector<float> coefj32(coefj.begin(), coefj.end());
@@ -278,6 +352,15 @@ int main(int argc, char** argv) {
   }
   const std::vector<double> tabvi = read_doubles(f, nv), tabh1 = read_doubles(f, nv);
   const std::vector<double> xtau = read_doubles(f, 51), ch = read_doubles(f, 51), coefj = read_doubles(f, 51 * 51);
+  std::vector<std::vector<double>> ht;
+  for (int i = 0; i < 20; ++i) {  // the hydrogen tables, each size-prefixed
+    int64_t sz = 0;
+    if (fread(&sz, sizeof sz, 1, f) != 1 || sz < 1 || sz > 100000) {
+      fprintf(stderr, "bad hydrogen table %d\n", i);
+      return 2;
+    }
+    ht.push_back(read_doubles(f, (size_t)sz));
+  }
   fclose(f);
   orc_set_threads(2);
   std::vector<double> h0(2001), h1(2001), h2(2001), ex(1001), exf(1001);
@@ -286,6 +369,12 @@ int main(int argc, char** argv) {
   orc_set_josh_tables(xtau.data(), ch.data(), coefj.data());
   const orc_tables tb = {h0.data(), h1.data(), h2.data(), 2001, ex.data(), exf.data(), 1001};
   oracle_kernels(tb);
+  orc_set_hydrogen_tables(ht[0].data(), ht[1].data(), ht[2].data(), ht[3].data(), ht[4].data(), ht[5].data(),
+                          ht[6].data(), ht[7].data(), ht[8].data(), ht[9].data(), ht[10].data(), ht[11].data(),
+                          ht[12].data(), (int)ht[12].size(), ht[13].data(), (int)ht[13].size(), ht[14].data(),
+                          (int)ht[14].size(), ht[15].data(), (int)ht[15].size(), ht[16].data(), ht[17].data(),
+                          ht[18].data(), (int)ht[18].size(), ht[19].data(), ex.data(), exf.data(), 1001);
+  hydrogen(tb);
   oracle_spectrum(tb, 500.0, 500.4, 60, 12, 1);
   oracle_spectrum(tb, 1200.0, 1200.2, 25, 5, 2);
   oracle_spectrum(tb, 300.0, 300.05, 8, 3, 3);
