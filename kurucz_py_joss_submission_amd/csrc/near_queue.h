@@ -1,13 +1,12 @@
 // The record of one queued (list entry, pixel) pair of the Voigt-zone kernels
-// (k_near, k_near_adj): 64 bits, packed and unpacked here so that a CPU build
-// can check the layout at its bounds.
+// (k_near): 64 bits, packed and unpacked here so that a CPU build can check
+// the layout at its bounds.
 //
 //   bits  0-31  the entry's index within its depth's lists (< 2^32)
 //   bits 32-51  |k|, the pixel's offset from the line centre, 1 <= |k| <= the
 //               zone's half-width <= MAXPROF = 10^6 < 2^20
 //   bits 52-57  the pixel's lane in the wave (0..63)
-//   bits 58-63  a tag (0..63): the pixel's slot in k_near, the entry's run
-//               in k_near_adj
+//   bits 58-63  a tag (0..63): the pixel's slot in k_near
 //
 // The header compiles as HIP device code and as plain C++ (for CPU checks).
 #pragma once

@@ -1980,104 +1980,48 @@ __global__ void k_adj_centre(LinesDev ln, DepthsDev dp, long W, const double* __
   s_a[i] += uc * post * dvc;
 }
 
-// A queued Voigt-zone pair of the adjoint: bits 0-31 the entry's index within
-// its depth's lists, 32-51 |k| (< 2^20), 52-57 the pixel's lane, 58-63 the
-// entry's run in the current flush.
-constexpr int kNearAdjQueue = 256;  // pairs per wave between flushes
-constexpr int kNearAdjRuns = 64;    // entries per flush (6 bits)
-
-__device__ __forceinline__ uint64_t near_adj_record(long j_rel, long ka, int lane, int run) {
-  return nq::pack(j_rel, ka, lane, run);
-}
-
 // Per entry (line, depth): dL/dkappa0, dL/da and dL/d(Doppler width) summed
-// over its Voigt-zone pixels (the k_near set; c = kappa0 H(k dvoigt, a)).  As in k_near, one wave per 64 pixels of a
-// depth queues every (entry, pixel) pair of the zones reaching it in LDS and
-// evaluates the queue 64 pairs at a time (no idle lanes); each pair adds its
-// three partials to its entry's run in LDS, and each run adds its sums to the
-// entry's pair with one global atomic each (run-to-run last-bit differences,
-// as any floating-point atomics).
-__global__ __launch_bounds__(256) void k_near_adj(long W, int D, const WingHead* __restrict__ list_cm,
-                                                  const WingParams* __restrict__ list_p,
-                                                  const int32_t* __restrict__ list_l,
-                                                  const int64_t* __restrict__ regions,
-                                                  const int32_t* __restrict__ near_reach, DeviceTables tb,
-                                                  const double* __restrict__ u, const double* __restrict__ wl,
-                                                  double* __restrict__ s_k0, double* __restrict__ s_a,
-                                                  double* __restrict__ s_dv) {
-  __shared__ uint64_t s_queue[4][kNearAdjQueue];
-  __shared__ double s_run[4][kNearAdjRuns][3];
-  __shared__ int32_t s_runj[4][kNearAdjRuns];
-  __shared__ double s_u[4][64];
+// over its Voigt-zone pixels 1 <= |k| <= r (the k_near set, r = min(N10DOP,
+// MAXSTEP) with a far wing, MAXSTEP without; c = kappa0 H(k dvoigt, a)).  One
+// entry per lane, the depth's lists in order: the profile's partials depend
+// on |k| only, so each is evaluated once and weighted by u at both c - k and
+// c + k (pixels off the grid weigh 0).  Each pair has one list entry, and
+// the kernels writing the pair's sums run in stream order, so the sums are
+// plain adds: run-to-run identical.
+__global__ __launch_bounds__(64) void k_near_adj(long W, int D, const WingHead* __restrict__ list_cm,
+                                                 const WingParams* __restrict__ list_p,
+                                                 const int32_t* __restrict__ list_l,
+                                                 const int64_t* __restrict__ regions, DeviceTables tb,
+                                                 const double* __restrict__ u, const double* __restrict__ wl,
+                                                 double* __restrict__ s_k0, double* __restrict__ s_a,
+                                                 double* __restrict__ s_dv) {
   const int d = blockIdx.y;
-  const int lane = threadIdx.x & 63;
-  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const long v0 = ((long)blockIdx.x * 4 + wv) * 64;
-  if (v0 >= W) return;  // whole wave; the waves of a block share no LDS
-  const long w = v0 + lane;
-  const int v0i = (int)v0, wi = v0i + lane;  // pixel indices fit 32 bits (n_wl <= 2^30)
+  const long beg = regions[(long)d * (kClasses + 1)], end = regions[(long)d * (kClasses + 1) + kClasses];
   const double resolu = W > 1 && wl[1] / wl[0] > 1.0 ? 1.0 / (wl[1] / wl[0] - 1.0) : 300000.0;
-  s_u[wv][lane] = w < W ? u[(long)d * W + w] : 0.0;
-  const long bound = class_window_search(lane, d, v0, v0 + 63, (long)near_reach[d], regions, list_cm);
-  const long base = regions[(long)d * (kClasses + 1)];  // the depth's lists start here
-  const WingParams* __restrict__ lp = list_p + base;
-  uint64_t* queue = s_queue[wv];
-  double(*run)[3] = s_run[wv];
-  int32_t* runj = s_runj[wv];
-  const double* uu = s_u[wv];
-  int qn = 0, nr = 0;  // queued pairs, runs (wave-uniform)
-  const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-  auto flush = [&]() {
-    if (lane < nr) run[lane][0] = run[lane][1] = run[lane][2] = 0.0;
-    wave_lds_sync();
-    for (int q = lane; q < qn; q += 64) {
-      const nq::Record rec = nq::unpack(queue[q]);
-      const long j_rel = rec.j_rel;
-      const long ka = rec.ka;
-      const int pl = rec.lane;
-      const int rr = rec.tag;
-      const WingParams pr = lp[j_rel];
+  const double* ud = u + (long)d * W;
+  for (long j = beg + (long)blockIdx.x * 64 + threadIdx.x; j < end; j += (long)gridDim.x * 64) {
+    const WingHead h = list_cm[j];
+    const int lim = (h.nhi - h.nlo) >> 1;  // the zone's half-width
+    const long c = head_c(h);
+    if (lim < 1) continue;
+    const WingParams pr = list_p[j];
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int ka = 1; ka <= lim; ++ka) {
+      const long wb = c - ka, wr = c + ka;
+      const double uw = (wb >= 0 && wb < W ? ud[wb] : 0.0) + (wr >= 0 && wr < W ? ud[wr] : 0.0);
+      if (uw == 0.0) continue;
       double H, dHa, dHx;
       near_wing_partials(pr, ka, tb, &H, &dHa, &dHx);
       const double x = (double)ka * pr.dvoigt;
-      const double uw = uu[pl];
-      atomicAdd(&run[rr][0], uw * H);
-      atomicAdd(&run[rr][1], uw * pr.kappa0 * dHa);
-      atomicAdd(&run[rr][2], uw * pr.kappa0 * dHx * (-x * pr.dvoigt * resolu));
+      s0 += uw * H;
+      s1 += uw * pr.kappa0 * dHa;
+      s2 += uw * pr.kappa0 * dHx * (-x * pr.dvoigt * resolu);
     }
-    wave_lds_sync();
-    if (lane < nr) {
-      const long pair = (long)list_l[base + runj[lane]] * D + d;
-      atomicAdd(s_k0 + pair, run[lane][0]);
-      atomicAdd(s_a + pair, run[lane][1]);
-      atomicAdd(s_dv + pair, run[lane][2]);
-    }
-    wave_lds_sync();
-    qn = 0;
-    nr = 0;
-  };
-  wave_lds_sync();  // s_u visible to the wave
-  for (int kk = 0; kk < kClasses; ++kk) {
-    const long lo = uniform_long(__shfl(bound, 2 * kk));
-    const long hi = uniform_long(__shfl(bound, 2 * kk + 1));
-    for (long j = lo; j < hi; ++j) {
-      const WingHead h = list_cm[j];
-      if (v0i + 63 < h.nlo || v0i > h.nhi) continue;  // wave-uniform
-      const int lim = (h.nhi - h.nlo) >> 1;          // the zone's half-width
-      const int c = h.nlo + lim;
-      const int ka = abs(wi - c);
-      const bool act = (unsigned)(ka - 1) < (unsigned)lim && w < W;  // 1 <= |k| <= lim
-      const unsigned long long m = __ballot(act);
-      if (m == 0) continue;
-      const int np = __popcll(m);
-      if (qn + np > kNearAdjQueue || nr == kNearAdjRuns) flush();
-      if (lane == 0) runj[nr] = (int32_t)(j - base);
-      if (act) queue[qn + __popcll(m & below)] = near_adj_record(j - base, ka, lane, nr);
-      qn += np;
-      ++nr;
-    }
+    const long pair = (long)list_l[j] * D + d;
+    s_k0[pair] += s0;
+    s_a[pair] += s1;
+    s_dv[pair] += s2;
   }
-  if (qn > 0) flush();
 }
 
 // Tile moments of the adjoint weights for the far-wing expansion: for each
@@ -3999,8 +3943,8 @@ int synthe_asynth_vjp(synthe_ctx* ctx, const synthe_lines* lines, const synthe_d
   hipLaunchKernelGGL(k_adj_centre, dim3(blocks_for(pairs, kBlock)), dim3(kBlock), 0, st, to_dev(lines),
                      to_dev(depths), n_wl, transp, valid, u, ctx->tb, s_kappa0, s_adamp, kappa0);
   HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(k_near_adj, dim3(blocks_for(n_wl, 256), (unsigned)D), dim3(256), 0, st, n_wl, D, b.lcm, b.lp,
-                     b.list_l, b.reg, b.near_reach, ctx->tb, u, wl, s_kappa0, s_adamp, s_dopple);
+  hipLaunchKernelGGL(k_near_adj, dim3(2048, (unsigned)D), dim3(64), 0, st, n_wl, D, b.lcm, b.lp, b.list_l, b.reg,
+                     ctx->tb, u, wl, s_kappa0, s_adamp, s_dopple);
   HIP_TRY(hipGetLastError());
   const GatherArgs ga = gather_args(lines, depths, n_wl, wl, transp, valid, center_order, center_start, nullptr, b);
   (void)ga;
