@@ -250,3 +250,16 @@ def test_adjoint_with_line_scattering(gpu, oracle, adj_case):
         print(f"rhoxj {rhoxj}: {name}: adjoint {adj:.6e} fp64-fd {fd:.6e} (rhoxj 0: {adj0:.6e})")
         assert abs(adj - fd) <= 2e-3 * abs(fd), name
     assert max(abs(a - a0) / abs(a0) for _, a, _, a0 in checks) > 1e-2
+
+
+@pytest.mark.gpu
+def test_adjoint_is_run_to_run_identical(gpu, adj_case):
+    """The library's profile adjoints write each list entry's and pair's sums
+    once in a fixed order (no floating-point atomics): two backward passes
+    give the same bits."""
+    cols, cat, _ = adj_case
+    _, f1, g1 = _model_grads(cols, cat, "fp64")
+    _, f2, g2 = _model_grads(cols, cat, "fp64")
+    assert np.array_equal(f1, f2)
+    for k in g1:
+        assert np.array_equal(g1[k], g2[k]), k
