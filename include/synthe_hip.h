@@ -198,6 +198,51 @@ int synthe_hydrogen_lines(synthe_ctx *ctx, const synthe_hlines *lines, const syn
                           double microturb_kms, double cutoff, long n_wl, const double *wl,
                           const double *cont, const double *stim, double *asynth, synthe_stream stream);
 
+/* fort.19 lines, engine/opacity.py:1335-1546 (_add_fort19_asynth) with
+ * :760-963 (_accumulate_metal_profile_kernel, _accumulate_merged_continuum,
+ * _accumulate_autoionizing_profile) and :646-757 (WCON/WTAIL, merged
+ * continuum limits): the records rgfall routes to fort.19
+ * (io/lines/fort19.py:233-420) that are normal lines with NBLO/NBUP,
+ * autoionising lines (Shore profiles) or merged-continuum ramps, in fort.19
+ * order.  All host arrays. */
+typedef struct {
+  int64_t n_lines;
+  const int32_t *kind;      /* [n] 0 normal, 1 autoionising, 2 merged continuum */
+  const double *wavelength; /* [n] the record's vacuum wavelength, nm */
+  const int64_t *center;    /* [n] nearest grid index of that wavelength, >= -1 and < n_wl */
+  const double *gf;         /* [n] the record's float32 gf (CGF for a normal line), as double */
+  const double *gamma_rad, *gamma_stark, *gamma_vdw; /* [n] the record's float32 damping, as double */
+  const int32_t *elem;      /* [n] Z - 1 of the matched catalog line */
+  const int32_t *ion;       /* [n] its ion stage (1-based) */
+  const int32_t *nelion;    /* [n] the record's NELION */
+  const int32_t *nlast;     /* [n] the record's TYPE (NLAST of a merged continuum) */
+  const double *cont_val;   /* [n] CONTX(NCON, NELIONX), 0 where the record has no WCON */
+  const double *cat_wavelength; /* [n] matched catalog line's wavelength (NULL if depths->doppler given) */
+  const double *elo;        /* [n] its lower level energy, cm^-1 (NULL if depths->boltz given) */
+} synthe_f19lines;
+
+typedef struct {
+  int32_t n_depths, n_ion, n_elem;
+  const double *pop;        /* [D][n_ion][n_elem] population_per_ion */
+  const double *mass_density, *electron_density, *txnxn, *temperature; /* [D] */
+  const double *emerge, *emerge_h; /* [D] the engine's merge energies (engine/opacity.py:2369-2372) */
+  const double *boltz;      /* [n][D] FASTEX(E_low HCKT) of each record's catalog line, or NULL: from elo, hckt */
+  const double *hckt;       /* [D] (used when boltz is NULL) */
+  const double *doppler;    /* [n][D] Doppler width, nm, of each record's catalog line, or NULL */
+  const double *vscale;     /* [D] Doppler width / catalog wavelength (used when doppler is NULL) */
+} synthe_f19depths;
+
+/* Adds the fort.19 terms into asynth (device, [D][W]): per pixel the records'
+ * terms in fort.19 order, summed from zero, times stim, added (the reference
+ * sums a depth's records into a buffer and adds buffer * STIM).  cont
+ * (device [D][W]) is the total continuum the cutoffs compare with, stim
+ * (device [D][W], or NULL for 1 - exp(-h nu / k T)) and wl (device [W]) the
+ * grid.  SYNTHE_ERANGE (nothing added) for a record whose element or stage
+ * lies outside the population table, where the reference's lookup raises. */
+int synthe_fort19_lines(synthe_ctx *ctx, const synthe_f19lines *lines, const synthe_f19depths *depths,
+                        double cutoff, long n_wl, const double *wl, const double *cont, const double *stim,
+                        double *asynth, synthe_stream stream);
+
 /* engine/radiative.py:282-494 (solve_lte_spectrum) with
  * physics/josh_solver.py:367-767 (solve_josh_flux) per wavelength: the
  * continuum-only and the total JOSH solves.  All [D][W] except wl [W],

@@ -28,7 +28,7 @@ SYMBOLS = (
     "synthe_flux_to_lambda", "synthe_diag_div_pos", "synthe_set_kernel_timing",
     "synthe_kernel_times", "synthe_set_profile_precision", "synthe_batch_create", "synthe_batch_run",
     "synthe_batch_info", "synthe_batch_destroy", "synthe_asynth_vjp", "synthe_lte_vjp",
-    "synthe_lte_spectrum_noalloc", "synthe_transp_asynth", "synthe_hydrogen_lines",
+    "synthe_lte_spectrum_noalloc", "synthe_transp_asynth", "synthe_hydrogen_lines", "synthe_fort19_lines",
 )
 
 SYNTHE_BATCH_AUTOTUNE = 1
@@ -81,6 +81,21 @@ class HDepths(ct.Structure):
                 ("xnfph", _VP), ("dopph", _VP), ("turbulent_velocity", _VP)]
 
 
+class F19Lines(ct.Structure):
+    """synthe_f19lines (host arrays)."""
+    _fields_ = [("n_lines", ct.c_int64), ("kind", _VP), ("wavelength", _VP), ("center", _VP), ("gf", _VP),
+                ("gamma_rad", _VP), ("gamma_stark", _VP), ("gamma_vdw", _VP), ("elem", _VP), ("ion", _VP),
+                ("nelion", _VP), ("nlast", _VP), ("cont_val", _VP), ("cat_wavelength", _VP), ("elo", _VP)]
+
+
+class F19Depths(ct.Structure):
+    """synthe_f19depths (host arrays)."""
+    _fields_ = [("n_depths", ct.c_int32), ("n_ion", ct.c_int32), ("n_elem", ct.c_int32), ("pop", _VP),
+                ("mass_density", _VP), ("electron_density", _VP), ("txnxn", _VP), ("temperature", _VP),
+                ("emerge", _VP), ("emerge_h", _VP), ("boltz", _VP), ("hckt", _VP), ("doppler", _VP),
+                ("vscale", _VP)]
+
+
 class Instance(ct.Structure):
     """synthe_instance: one spectrum of a batch (include/synthe_hip.h)."""
 
@@ -124,6 +139,8 @@ def load_library(path: Optional[Path] = None):
                                              ct.c_double, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]
         lib.synthe_hydrogen_lines.argtypes = [_VP, ct.POINTER(HLines), ct.POINTER(HDepths), ct.c_double,
                                               ct.c_double, ct.c_long, _VP, _VP, _VP, _VP, _VP]
+        lib.synthe_fort19_lines.argtypes = [_VP, ct.POINTER(F19Lines), ct.POINTER(F19Depths), ct.c_double,
+                                            ct.c_long, _VP, _VP, _VP, _VP, _VP]
         lib.synthe_lte_spectrum.argtypes = [_VP, ct.c_long, ct.c_long] + [_VP] * 12
         lib.synthe_lte_spectrum_noalloc.argtypes = [_VP, ct.c_long, ct.c_long] + [_VP] * 12
         lib.synthe_josh_flux_batch.argtypes = [_VP, ct.c_long, ct.c_long] + [_VP] * 10

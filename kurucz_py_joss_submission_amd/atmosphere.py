@@ -34,6 +34,8 @@ from dataclasses import dataclass
 from pathlib import Path
 from typing import Dict, Optional
 
+import math
+
 import numpy as np
 
 
@@ -149,6 +151,34 @@ def depth_xne(atm) -> np.ndarray:
     v = _opt(atm, "electron_density")
     return np.zeros(atm.temperature.size) if v is None else v
 
+
+
+def depth_doppler_scale(atm, microturb_kms: float = 0.0, mass_amu: float = 56.0) -> np.ndarray:
+    """Doppler width / line wavelength per depth as physics/populations.py:172-190
+    computes it for a line of mass_amu (the engine passes no elements, so every
+    line takes the default 56): sqrt(v_turb^2 + 2kT/m) / c with v_turb the
+    model's turbulence combined with the synthesis microturbulence."""
+    temp = np.asarray(atm.temperature, dtype=np.float64)
+    vt = _opt(atm, "turbulent_velocity")
+    out = np.empty(temp.size, dtype=np.float64)
+    mass = np.array([mass_amu], dtype=np.float64)
+    for i in range(temp.size):
+        t = max(float(temp[i]), 1.0)
+        thermal = np.sqrt(2.0 * 1.380649e-16 * t / (mass * 1.66054e-24)) / 2.99792458e10
+        v_model = (float(vt[i]) if vt is not None and vt.size > i else 0.0) / 1e5 / 299_792.458
+        micro = microturb_kms / 299_792.458 if microturb_kms > 0 else 0.0
+        turb = math.sqrt(v_model * v_model + micro * micro)
+        out[i] = float(np.sqrt(turb ** 2 + thermal ** 2)[0])
+    return out
+
+
+def merge_energies(atm):
+    """(EMERGE, EMERGE_H) per depth, cm^-1 (engine/opacity.py:2368-2372): the
+    Inglis-Teller last series member from the electron density."""
+    ne = np.maximum(np.asarray(atm.electron_density, dtype=np.float64), 1e-40)
+    inglis = 1600.0 / np.power(ne, 2.0 / 15.0)
+    nmerge = np.maximum(inglis - 1.5, 1.0)
+    return 109737.312 / np.maximum(nmerge ** 2, 1e-12), 109677.576 / np.maximum(nmerge ** 2, 1e-12)
 
 
 def _xnfph(xnfph, qxnfpel):

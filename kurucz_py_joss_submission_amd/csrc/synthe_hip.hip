@@ -1817,6 +1817,8 @@ struct synthe_ctx {
   void* scratch = nullptr;  // device address of the static RT scratch on this device
   void* hstage = nullptr;   // hydrogen-line staging (synthe_hydrogen_lines), freed by hstage_free
   void (*hstage_free)(void*) = nullptr;
+  void* f19stage = nullptr;  // fort.19 staging (synthe_fort19_lines), freed by f19stage_free
+  void (*f19stage_free)(void*) = nullptr;
   // kernel timing: event pairs around each wing-gather launch while enabled
   static constexpr int kTimedMax = 256;
   int precision = SYNTHE_PREC_FP64;  // profile precision of synthe_asynth
@@ -2894,6 +2896,7 @@ int synthe_destroy(synthe_ctx* ctx) {
   if (ctx->d_tab32) (void)hipFree(ctx->d_tab32);
   if (ctx->ws) (void)hipFree(ctx->ws);
   if (ctx->hstage_free) ctx->hstage_free(ctx->hstage);
+  if (ctx->f19stage_free) ctx->f19stage_free(ctx->f19stage);
   for (int i = 0; i < synthe_ctx::kTimedMax; ++i) {
     if (ctx->ev_beg[i]) (void)hipEventDestroy(ctx->ev_beg[i]);
     if (ctx->ev_end[i]) (void)hipEventDestroy(ctx->ev_end[i]);
@@ -3479,6 +3482,450 @@ int synthe_hydrogen_lines(synthe_ctx* ctx, const synthe_hlines* hl, const synthe
   HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(k_h_gather, dim3((unsigned)((n_wl + 255) / 256), (unsigned)D), dim3(256), 0, st, d_lines,
                      d_pairs, (int)NH, (int)D, n_wl, wl, dp.temp, dp.state, stim, ctx->tb, asynth);
+  HIP_TRY(hipGetLastError());
+  return SYNTHE_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ fort.19 lines
+namespace {
+
+// One fort.19 record the ASYNTH add takes (engine/opacity.py:1364-1546),
+// packed on the host: the record's wavelength, float32 gf and damping (as
+// doubles), centre index, the matched catalog line's Z - 1 and stage, the
+// record's NELION and NLAST, CONTX(NCON, NELIONX) (0 where there is no WCON),
+// and the matched line's wavelength and lower level energy for the Doppler
+// width and Boltzmann factor when the caller does not give them.
+struct F19Line {
+  double wl, gf, gr, gs, gw, cont_val, lam_cat, elo;
+  long c;
+  int kind, elem, ion, nelion, nlast;
+};
+
+// A (record, depth) pair after its gates: what the per-pixel sum needs.
+struct F19Pair {
+  double kappa0;  // KAPPA0 (normal, autoionising) or KAPPA (merged continuum)
+  double adamp, dopple, resolu, x_far, kapcen, wcon, wtail;
+  long cpix;      // normal: the clamped centre; autoionising: the pixel its centre term lands on
+  long lo, hi;    // normal: wing steps [1, hi] red (from lo) and [1, blue] blue; continuum: pixels [lo, hi)
+  long blue;      // normal: last blue step; autoionising: blue walk extent
+  long red;       // autoionising: red walk extent
+  long merge, tail, denom;  // continuum: IDX_MERGE, IDX_TAIL, the ramp's denominator
+  int active, center_add, n10dop, nstep, early, wing;
+};
+
+struct F19DepthDev {
+  const double *pop, *rho, *xne, *txnxn, *emerge, *emerge_h, *temp, *hckt, *vscale, *boltz, *doppler;
+  int n_ion, n_elem;
+};
+
+// profile(k) of the metal wing loop (engine/opacity.py:150-198): the near
+// wing by the TABI walk (a < 0.2) or the Voigt function, the far wing X / k^2.
+__device__ __forceinline__ double f19_profile(const F19Pair& p, long k, const DeviceTables& tb) {
+  if (p.early || k <= p.n10dop) {
+    if (p.adamp < 0.2) {
+      const double dr = p.dopple * p.resolu;
+      const double tabstep = dr > 0.0 ? 200.0 / dr : 200.0;
+      long itab = tabi_index(tabstep, k);
+      itab = itab < 0 ? 0 : (itab > kVoigtN - 1 ? kVoigtN - 1 : itab);
+      return p.kappa0 * (tb.h0[itab] + p.adamp * tb.h1[itab]);
+    }
+    const double dvoigt = p.dopple * p.resolu > 0.0 ? 1.0 / p.dopple / p.resolu : 1e-6;
+    return p.kappa0 * voigt((double)k * dvoigt, p.adamp, tb.h0, tb.h1, tb.h2);
+  }
+  return p.x_far / ((double)k * (double)k);
+}
+
+// The normal record's wing value at a pixel step k, with WTAIL's taper.
+__device__ __forceinline__ double f19_normal_value(const F19Pair& p, long k, double wave, double line_wl,
+                                                   const DeviceTables& tb) {
+  double v = f19_profile(p, k, tb);
+  if (p.wtail > 0.0 && wave < p.wtail) {
+    const double base = p.wcon > 0.0 ? p.wcon : line_wl;
+    const double span = p.wtail - base;
+    v = v * (wave - base) / (span > 1e-12 ? span : 1e-12);
+  }
+  return v;
+}
+
+// Shore profile of an autoionising line (engine/opacity.py:895-963).
+__device__ __forceinline__ double f19_auto_value(const F19Line& L, double kappa0, double wave) {
+  const double fc = kCLightCm / (L.wl * kNmToCm);
+  const double gamma = fabs(L.gr) > 1e-30 ? fabs(L.gr) : 1e-30;
+  const double bshore = fabs(L.gw) < 1e-30 ? 1e-30 : L.gw;
+  const double eps = 2.0 * (kCLightCm / (wave * kNmToCm) - fc) / gamma;
+  return kappa0 * (L.gs * eps + bshore) / (eps * eps + 1.0) / bshore;
+}
+
+// First index with grid[i] >= x (side "left") or > x (side "right").
+__device__ __forceinline__ long f19_search(const double* grid, long W, double x, bool right) {
+  long lo = 0, hi = W;
+  while (lo < hi) {
+    const long mid = (lo + hi) >> 1;
+    if (right ? grid[mid] <= x : grid[mid] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Gates and set-up of one (record, depth) pair (engine/opacity.py:1383-1546),
+// one wave per pair: lane 0 runs the normal record's near-wing probe; the
+// autoionising and merged-continuum walks take 64 offsets per step and stop
+// where the reference's loops stop.
+__global__ __launch_bounds__(64) void k_f19_extent(const F19Line* __restrict__ lines, F19DepthDev dp, int N, int D,
+                                                   long W, const double* __restrict__ grid,
+                                                   const double* __restrict__ cont, double cutoff, DeviceTables tb,
+                                                   F19Pair* __restrict__ pairs) {
+  const int r = blockIdx.x, d = blockIdx.y, lane = threadIdx.x;
+  const F19Line& L = lines[r];
+  F19Pair p{};
+  const double* crow = cont + (long)d * W;
+  const double pop_val = dp.pop[((long)d * dp.n_ion + (L.ion - 1)) * dp.n_elem + L.elem];
+  const double boltz = dp.boltz ? dp.boltz[(long)r * D + d] : fastex(L.elo * dp.hckt[d], tb.extab, tb.extabf);
+  const double rho = dp.rho[d];
+  bool ok = pop_val > 0.0 && rho > 0.0;
+  if (ok && L.kind == 0) {  // normal (opacity.py:1407-1478)
+    const double dw = dp.doppler ? dp.doppler[(long)r * D + d] : L.lam_cat * dp.vscale[d];
+    ok = L.wl > 0.0 && dw > 0.0;
+    const double dopple = ok ? dw / L.wl : 0.0;
+    ok = ok && dopple > 0.0;
+    if (ok) {
+      const double xnfdop = pop_val / (rho * dopple);
+      const double kappa0_pre = L.gf * xnfdop;
+      const long cc = L.c < 0 ? 0 : (L.c > W - 1 ? W - 1 : L.c);
+      const double kapmin = crow[cc] * cutoff;
+      const double kappa0 = kappa0_pre * boltz;
+      ok = !(kappa0_pre < kapmin) && !(kappa0 < kapmin);
+      if (ok) {
+        const double gamma_total = L.gr + L.gs * dp.xne[d] + L.gw * dp.txnxn[d];
+        const double adamp = gamma_total / dopple;
+        const double kapcen = adamp < 0.2 ? kappa0 * (1.0 - 1.128 * adamp) : kappa0 * voigt(0.0, adamp, tb.h0, tb.h1, tb.h2);
+        p.active = 1;
+        p.cpix = cc;
+        p.kapcen = kapcen;
+        p.center_add = kapcen >= kapmin;
+        p.kappa0 = kappa0;
+        p.dopple = dopple;
+        p.wcon = p.wtail = -1.0;
+        if (L.cont_val > 0.0) {  // WCON / WTAIL (opacity.py:646-686), vacuum wavelengths
+          const double emerge_line = L.nelion == 1 ? dp.emerge_h[d] : dp.emerge[d];
+          const double den = L.cont_val - emerge_line;
+          if (fabs(den) > 1e-8) {
+            const double wcon = 1.0e7 / den;
+            const double dt = den - 500.0;
+            double wtail = -1.0;
+            if (fabs(dt) > 1e-8) {
+              wtail = 1.0e7 / dt;
+              if (wtail < 0.0) wtail = 2.0 * wcon;
+              wtail = 2.0 * wcon < wtail ? 2.0 * wcon : wtail;
+            }
+            if (wtail > 0.0 && wtail <= wcon) wtail = -1.0;
+            p.wcon = wcon > 0.0 ? wcon : -1.0;
+            p.wtail = wtail > 0.0 ? wtail : -1.0;
+          }
+        }
+        // the metal wing loop's set-up and near-wing probe (opacity.py:128-198)
+        p.adamp = adamp > 1e-12 ? adamp : 1e-12;
+        if (cc < W - 1) p.resolu = 1.0 / (grid[cc + 1] / grid[cc] - 1.0);
+        else p.resolu = cc > 0 ? 1.0 / (grid[cc] / grid[cc - 1] - 1.0) : 300000.0;
+        long n10 = (long)(10.0 * dopple * p.resolu);
+        n10 = n10 < kMaxProfileSteps ? n10 : kMaxProfileSteps;
+        p.n10dop = (int)n10;
+        long nstep = 0;
+        if (lane != 0) return;
+        if (kappa0 > 0.0) {  // else no wing (opacity.py:128-129)
+          const double kmin = cutoff * crow[cc];
+          double last = 0.0;
+          for (long k = 1; k <= n10; ++k) {
+            p.early = 1;  // provisional: f19_profile takes the near wing for any k
+            last = f19_profile(p, k, tb);
+            p.early = 0;
+            if (last < kmin) {
+              nstep = k;
+              p.early = 1;
+              break;
+            }
+          }
+          if (!p.early) {
+            const double xf = n10 > 0 && last > 0.0 ? last * ((double)n10 * (double)n10) : 0.0;
+            long maxstep = n10;
+            if (xf > 0.0 && kmin > 0.0) {
+              const double ms = sqrt(xf / kmin) + 1.0;  // int() of it, capped at MAX_PROFILE_STEPS
+              maxstep = ms < 2.0e6 ? (long)ms : kMaxProfileSteps;
+              maxstep = maxstep < kMaxProfileSteps ? maxstep : kMaxProfileSteps;
+            }
+            p.x_far = xf;
+            nstep = maxstep;
+          }
+        }
+        const long c = L.c;
+        p.nstep = (int)nstep;
+        p.wing = !(c + nstep < 0 || c - nstep >= W) && nstep > 0;
+        p.lo = c >= 0 ? 1 : -c;  // MINRED
+        p.hi = c < W - 1 ? (c >= 0 ? (W - 1 - c < nstep ? W - 1 - c : nstep) : (W - 1 < nstep ? W - 1 : nstep)) : 0;
+        p.blue = c > 0 ? (c < nstep ? c : nstep) : 0;
+        pairs[(long)r * D + d] = p;
+        return;
+      }
+    }
+  } else if (ok && L.kind == 2) {  // merged continuum (opacity.py:1480-1513, 850-892)
+    const double kappa = L.gf * (pop_val / rho) * boltz;
+    double wmerge = L.wl, wtail = L.wl;
+    if (L.wl > 0.0 && L.nlast > 0) {
+      const double ryd = L.nelion == 1 ? 109677.576 : 109737.312;
+      const double ds = 1.0e7 / L.wl - ryd / (double)(L.nlast * L.nlast);
+      const double emerge_line = L.nelion == 1 ? dp.emerge_h[d] : dp.emerge[d];
+      const double dm = 1.0e7 / L.wl - emerge_line;
+      if (fabs(ds) > 1e-12 && fabs(dm) > 1e-12) {
+        const double wshift = 1.0e7 / ds;
+        double wm = 1.0e7 / dm;
+        if (wm < 0.0) wm = wshift + wshift;
+        wm = wm > wshift ? wm : wshift;
+        wm = wshift + wshift < wm ? wshift + wshift : wm;
+        wmerge = wm;
+        const double dt = 1.0e7 / wm - 500.0;
+        if (fabs(dt) > 1e-12) {
+          double wt = 1.0e7 / dt;
+          if (wt < 0.0) wt = wm + wm;
+          wtail = wm + wm < wt ? wm + wm : wt;
+        }
+      }
+    }
+    ok = kappa > 0.0;
+    const long start = L.c > 0 ? L.c : 0;
+    const long merge = f19_search(grid, W, wmerge, false);
+    long tail = f19_search(grid, W, wtail, true);
+    tail = tail < W ? tail : W;
+    ok = ok && tail > start;
+    if (ok) {
+      const long m0 = merge > start ? merge : start;
+      p.denom = tail - m0 > 1 ? tail - m0 : 1;
+      p.kappa0 = kappa;
+      p.merge = merge;
+      p.tail = tail;
+      p.lo = start;
+      long stop = tail;
+      for (long i0 = start; i0 < tail; i0 += 64) {
+        const long i = i0 + lane;
+        bool s = false;
+        if (i < tail) {
+          const double wave = grid[i];
+          if (!(wave < L.wl)) {
+            double v = kappa;
+            if (i >= merge) v *= (double)(tail - i > 0 ? tail - i : 0) / (double)p.denom;
+            s = v < crow[i] * cutoff;
+          }
+        }
+        const unsigned long long m = __ballot(s);
+        if (m) {
+          stop = i0 + __builtin_ctzll(m);
+          break;
+        }
+      }
+      p.hi = stop;
+      p.active = stop > start;
+    }
+  } else if (ok && L.kind == 1) {  // autoionising (opacity.py:1515-1545, 895-963)
+    const double kappa0 = L.gw * L.gf * (pop_val / rho) * boltz;
+    const long c = L.c;
+    const long cpix = c >= 0 ? c : W + c;  // the reference indexes buffer[-1] for a centre below the grid
+    ok = !(kappa0 < crow[cpix] * cutoff) && kappa0 > 0.0;
+    if (ok) {
+      p.active = 1;
+      p.kappa0 = kappa0;
+      p.cpix = cpix;
+      p.center_add = 1;
+      const long max_steps = c > W - c - 1 ? c : W - c - 1;
+      for (int side = 0; side < 2; ++side) {
+        long ext = 0;
+        for (long o0 = 1;; o0 += 64) {
+          const long o = o0 + lane;
+          bool stop = o > max_steps;
+          if (!stop) {
+            const long idx = side == 0 ? c + o : c - o;
+            if (idx >= W || idx < 0) {
+              stop = true;
+            } else {
+              const double v = f19_auto_value(L, kappa0, grid[idx]);
+              stop = v <= 0.0 || v < crow[idx] * cutoff;
+            }
+          }
+          const unsigned long long m = __ballot(stop);
+          if (m) {
+            ext = o0 + __builtin_ctzll(m) - 1;
+            break;
+          }
+        }
+        if (side == 0) p.red = ext;
+        else p.blue = ext;
+      }
+    }
+  }
+  if (lane == 0) pairs[(long)r * D + d] = p;
+}
+
+// The records' sum per pixel, in fort.19 order from zero (engine/opacity.py
+// :1362-1546), then ASYNTH += sum * STIM (:1545-1546).
+__global__ __launch_bounds__(256) void k_f19_gather(const F19Line* __restrict__ lines,
+                                                    const F19Pair* __restrict__ pairs, int N, int D, long W,
+                                                    const double* __restrict__ grid, const double* __restrict__ temp,
+                                                    const double* __restrict__ stim, DeviceTables tb,
+                                                    double* __restrict__ asynth) {
+  const long w = blockIdx.x * 256L + threadIdx.x;
+  const int d = blockIdx.y;
+  if (w >= W) return;
+  const double wave = grid[w];
+  double sum = 0.0;
+  for (int r = 0; r < N; ++r) {
+    const F19Pair& p = pairs[(long)r * D + d];
+    if (!p.active) continue;
+    const F19Line& L = lines[r];
+    if (L.kind == 0) {
+      if (p.center_add && w == p.cpix) sum += p.kapcen;
+      if (p.wing && !(p.wcon > 0.0 && wave <= p.wcon)) {
+        const long k = w - L.c;
+        if (k > 0 ? (k >= p.lo && k <= p.hi) : (-k >= 1 && -k <= p.blue)) sum += f19_normal_value(p, k > 0 ? k : -k, wave, L.wl, tb);
+      }
+    } else if (L.kind == 1) {
+      if (w == p.cpix) sum += p.kappa0;
+      const long o = w - L.c;
+      if ((o > 0 && o <= p.red) || (o < 0 && -o <= p.blue)) sum += f19_auto_value(L, p.kappa0, wave);
+    } else {
+      if (w >= p.lo && w < p.hi && !(wave < L.wl)) {
+        double v = p.kappa0;
+        if (w >= p.merge) v *= (double)(p.tail - w > 0 ? p.tail - w : 0) / (double)p.denom;
+        sum += v;
+      }
+    }
+  }
+  if (sum != 0.0) {
+    const double sv = stim ? stim[(long)d * W + w] : planck_terms(wave, temp[d]).stim;
+    asynth[(long)d * W + w] += sum * sv;
+  }
+}
+
+struct Fort19Stage {
+  std::vector<F19Line> lines;
+  std::vector<double> depth;
+  void* dev = nullptr;
+  size_t dev_size = 0;
+};
+void f19stage_free(void* p) {
+  Fort19Stage* S = (Fort19Stage*)p;
+  if (S->dev) (void)hipFree(S->dev);
+  delete S;
+}
+
+}  // namespace
+
+extern "C" {
+
+int synthe_fort19_lines(synthe_ctx* ctx, const synthe_f19lines* fl, const synthe_f19depths* fd, double cutoff,
+                        long n_wl, const double* wl, const double* cont, const double* stim, double* asynth,
+                        synthe_stream stream) {
+  const char* who = "synthe_fort19_lines";
+  if (!ctx || !fl || !fd || !wl || !cont || !asynth) return fail(SYNTHE_EINVAL, std::string(who) + ": NULL argument");
+  const long N = fl->n_lines, D = fd->n_depths;
+  if (N < 0 || D <= 0 || D > 65535 || n_wl <= 0 || n_wl > 0x7fffffffL || fd->n_ion <= 0 || fd->n_elem <= 0)
+    return fail(SYNTHE_EINVAL, std::string(who) + ": bad sizes");
+  if (N == 0) return SYNTHE_OK;
+  if (N > 65535) return fail(SYNTHE_ERANGE, std::string(who) + ": more than 65535 fort.19 records");
+  if (!fl->kind || !fl->wavelength || !fl->center || !fl->gf || !fl->gamma_rad || !fl->gamma_stark ||
+      !fl->gamma_vdw || !fl->elem || !fl->ion || !fl->nelion || !fl->nlast || !fl->cont_val || !fd->pop ||
+      !fd->mass_density || !fd->electron_density || !fd->txnxn || !fd->emerge || !fd->emerge_h ||
+      !fd->temperature)
+    return fail(SYNTHE_EINVAL, std::string(who) + ": NULL array");
+  if (!fd->boltz && (!fd->hckt || !fl->elo)) return fail(SYNTHE_EINVAL, std::string(who) + ": no Boltzmann factors");
+  if (!fd->doppler && (!fd->vscale || !fl->cat_wavelength))
+    return fail(SYNTHE_EINVAL, std::string(who) + ": no Doppler widths");
+  hipStream_t st = (hipStream_t)stream;
+  if (!ctx->f19stage) {
+    ctx->f19stage = new Fort19Stage();
+    ctx->f19stage_free = f19stage_free;
+  }
+  Fort19Stage* S = (Fort19Stage*)ctx->f19stage;
+  HIP_TRY(hipStreamSynchronize(st));  // the previous call's copies have left the staging vectors
+  S->lines.resize(N);
+  for (long r = 0; r < N; ++r) {
+    F19Line& L = S->lines[r];
+    L.kind = fl->kind[r];
+    if (L.kind < 0 || L.kind > 2) return fail(SYNTHE_EINVAL, std::string(who) + ": bad record kind");
+    L.elem = fl->elem[r];
+    L.ion = fl->ion[r];
+    if (L.elem < 0 || L.elem >= fd->n_elem || L.ion < 1 || L.ion > fd->n_ion)
+      return fail(SYNTHE_ERANGE, std::string(who) + ": element or stage outside the population table (record " +
+                                     std::to_string(r) + ")");
+    L.c = fl->center[r];
+    if (L.c < -1 || L.c >= n_wl)
+      return fail(SYNTHE_ERANGE, std::string(who) + ": centre index outside [-1, n_wl) (record " + std::to_string(r) + ")");
+    L.wl = fl->wavelength[r];
+    L.gf = fl->gf[r];
+    L.gr = fl->gamma_rad[r];
+    L.gs = fl->gamma_stark[r];
+    L.gw = fl->gamma_vdw[r];
+    L.cont_val = fl->cont_val[r];
+    L.lam_cat = fl->cat_wavelength ? fl->cat_wavelength[r] : 0.0;
+    L.elo = fl->elo ? fl->elo[r] : 0.0;
+    L.nelion = fl->nelion[r];
+    L.nlast = fl->nlast[r];
+  }
+  const long P = D * fd->n_ion * fd->n_elem;
+  const long nd = 9 * D + P + (fd->boltz ? N * D : 0) + (fd->doppler ? N * D : 0);
+  S->depth.assign(nd, 0.0);
+  double* h = S->depth.data();
+  const double* srcs[9] = {fd->mass_density, fd->electron_density, fd->txnxn, fd->emerge, fd->emerge_h,
+                           fd->temperature, fd->hckt, fd->vscale, nullptr};
+  for (int a = 0; a < 8; ++a)
+    if (srcs[a]) std::copy(srcs[a], srcs[a] + D, h + a * D);
+  std::copy(fd->pop, fd->pop + P, h + 9 * D);
+  long off = 9 * D + P;
+  if (fd->boltz) {
+    std::copy(fd->boltz, fd->boltz + N * D, h + off);
+    off += N * D;
+  }
+  if (fd->doppler) std::copy(fd->doppler, fd->doppler + N * D, h + off);
+  const size_t o_depth = align_up(N * sizeof(F19Line)), o_pairs = o_depth + align_up(nd * sizeof(double)),
+               total = o_pairs + align_up((size_t)N * D * sizeof(F19Pair));
+  if (S->dev_size < total) {
+    if (S->dev) {
+      HIP_TRY(hipFree(S->dev));
+      S->dev = nullptr;
+      S->dev_size = 0;
+    }
+    if (hipMalloc(&S->dev, total) != hipSuccess) {
+      S->dev = nullptr;
+      return fail(SYNTHE_ENOMEM, std::string(who) + ": staging allocation failed");
+    }
+    S->dev_size = total;
+  }
+  char* base = (char*)S->dev;
+  F19Line* d_lines = (F19Line*)base;
+  double* dd = (double*)(base + o_depth);
+  F19Pair* d_pairs = (F19Pair*)(base + o_pairs);
+  HIP_TRY(hipMemcpyAsync(d_lines, S->lines.data(), N * sizeof(F19Line), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(dd, S->depth.data(), nd * sizeof(double), hipMemcpyHostToDevice, st));
+  F19DepthDev dp;
+  dp.rho = dd;
+  dp.xne = dd + D;
+  dp.txnxn = dd + 2 * D;
+  dp.emerge = dd + 3 * D;
+  dp.emerge_h = dd + 4 * D;
+  dp.temp = dd + 5 * D;
+  dp.hckt = dd + 6 * D;
+  dp.vscale = dd + 7 * D;
+  dp.pop = dd + 9 * D;
+  off = 9 * D + P;
+  dp.boltz = fd->boltz ? dd + off : nullptr;
+  if (fd->boltz) off += N * D;
+  dp.doppler = fd->doppler ? dd + off : nullptr;
+  dp.n_ion = fd->n_ion;
+  dp.n_elem = fd->n_elem;
+  hipLaunchKernelGGL(k_f19_extent, dim3((unsigned)N, (unsigned)D), dim3(64), 0, st, d_lines, dp, (int)N, (int)D,
+                     n_wl, wl, cont, cutoff, ctx->tb, d_pairs);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_f19_gather, dim3((unsigned)((n_wl + 255) / 256), (unsigned)D), dim3(256), 0, st, d_lines,
+                     d_pairs, (int)N, (int)D, n_wl, wl, dp.temp, stim, ctx->tb, asynth);
   HIP_TRY(hipGetLastError());
   return SYNTHE_OK;
 }

@@ -8,7 +8,8 @@ POTION ionisation potentials in cm^-1 that the GFALL damping defaults use
 the HPROF4 hydrogen-profile tables (``physics/profiles/hydrogen.py:47-1406``:
 Stark PROPBM corrections, C/D coefficients, fine-structure components,
 quasi-static cutoffs, radiative damping sums, XKNMTB, Y1WTM), the hydrogen
-series limits CONTH (``physics/tables.py:152``) and the EHYD level energies
+series limits CONTH (``physics/tables.py:152``), the WCON merge table CONTX
+(``physics/tables.py:163-204``) and the EHYD level energies
 (``engine/opacity.py:49-61``).
 The source tables are read as text with ``ast`` and the archive with
 ``numpy.load(allow_pickle=False)`` (nothing from the reference is imported
@@ -117,6 +118,7 @@ def main(ref_root: str = "/root/reference") -> None:
         "tabh1": voigt["tabh1"],
         "potion": potion,
         "conth": voigt["_METAL_CONTH"],
+        "contx": voigt["_METAL_CONTX"],
         "ehyd_cm": engine["_EHYD_CM"],
     }
     for k in ("propbm", "c", "d", "pp", "beta_grid", "stalph", "stwtal", "istal", "lnghal", "stcomp", "stcpwt",
@@ -125,6 +127,7 @@ def main(ref_root: str = "/root/reference") -> None:
     assert tables["h_propbm"].shape == (7, 5, 15) and tables["h_c"].shape == (5, 7)
     assert tables["h_stcomp"].shape == (5, 4) and tables["h_xknmtb"].shape == (4, 3)
     assert tables["ehyd_cm"].shape == (8,)
+    assert tables["contx"].shape == (26, 17)
     assert tables["xtau"].shape == (51,) and tables["coefj"].shape == (51, 51)
     assert tables["tabvi"].shape == (81,) and tables["tabh1"].shape == (81,)
     assert tables["potion"].shape == (999,)

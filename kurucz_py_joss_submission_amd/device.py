@@ -14,8 +14,9 @@ from typing import Dict, List, Optional, Sequence, Tuple
 import numpy as np
 
 from . import _native as N
-from .atmosphere import AtmosphereModel, depth_hckt, depth_rho, depth_txnxn, depth_xne
-from .lines import LineCatalog, LineSoA, build_line_soa, hydrogen_lines
+from .atmosphere import (AtmosphereModel, depth_doppler_scale, depth_hckt, depth_rho, depth_txnxn, depth_xne,
+                         merge_energies)
+from .lines import LineCatalog, LineSoA, build_line_soa, fort19_lines, fort19_records, hydrogen_lines
 
 
 def _torch():
@@ -186,6 +187,13 @@ class SpectrumPlan:
         self.lines = upload_lines(self.soa, self.W)
         self.hyd = hydrogen_lines(catalog, self.grid)
         self._hyd_args = None
+        # fort.19 records (rgfall's routing) of the window, matched to the catalog
+        self.f19 = None
+        if atm.population_per_ion is not None:
+            pop_shape = np.asarray(atm.population_per_ion).shape
+            self.f19 = fort19_lines(catalog, fort19_records(catalog, wl_start, wl_end, resolution), self.grid,
+                                    pop_shape[1], pop_shape[2])
+        self._f19_args = None
         f8 = torch.float64
         D, W, L = self.D, self.W, len(self.soa)
         dev = "cuda"
@@ -266,6 +274,7 @@ class SpectrumPlan:
         if self.microturb > 0.0:
             self.run_transp(stream)
             self.run_asynth(stream, stats, split=False)
+            self.run_fort19(stream)
             self.run_hydrogen(stream)
             self.split_scattering()
             return
@@ -276,8 +285,38 @@ class SpectrumPlan:
             self.ctx.handle, ct_byref(ls), ct_byref(ds), self.W, N.ptr(self.d_grid), N.ptr(self.kapmin),
             self.cutoff, N.ptr(self.lines.corder), N.ptr(self.lines.cstart), N.ptr(self.lines.worder),
             N.ptr(self.transp), N.ptr(self.valid), N.ptr(self.asynth), N.ptr(self.stats) if stats else None, s))
+        self.run_fort19(stream)
         self.run_hydrogen(stream)
         self.split_scattering()
+
+    def run_fort19(self, stream=None):
+        """The fort.19 records' terms into ASYNTH (engine/opacity.py:2366-2387,
+        before the hydrogen lines): normal lines with NBLO/NBUP, autoionising
+        lines and merged-continuum ramps, times STIM."""
+        f = self.f19
+        a = self.atm
+        if f is None or len(f) == 0:
+            return
+        if self._f19_args is None:
+            c = np.ascontiguousarray
+            pop = c(a.population_per_ion, np.float64)
+            emerge, emerge_h = merge_energies(a)
+            rho = np.zeros(self.D) if a.mass_density is None else c(a.mass_density, np.float64)
+            keep = dict(pop=pop, rho=rho, xne=c(a.electron_density, np.float64), txnxn=c(depth_txnxn(a)),
+                        temp=c(a.temperature, np.float64), emerge=c(emerge), emerge_h=c(emerge_h),
+                        hckt=c(depth_hckt(a)), vscale=c(depth_doppler_scale(a, self.microturb)))
+            fl = N.F19Lines(len(f), *(x.ctypes.data for x in (f.kind, f.wavelength, f.center, f.gf, f.gamma_rad,
+                                                             f.gamma_stark, f.gamma_vdw, f.elem, f.ion, f.nelion,
+                                                             f.nlast, f.cont_val, f.cat_wavelength, f.elo)))
+            fd = N.F19Depths(self.D, pop.shape[1], pop.shape[2], *(keep[k].ctypes.data for k in (
+                "pop", "rho", "xne", "txnxn", "temp", "emerge", "emerge_h")), None, keep["hckt"].ctypes.data,
+                None, keep["vscale"].ctypes.data)
+            self._f19_args = (keep, fl, fd)
+        _, fl, fd = self._f19_args
+        s = stream if stream is not None else N.stream_handle()
+        N.check("synthe_fort19_lines", self.ctx.lib.synthe_fort19_lines(
+            self.ctx.handle, ct_byref(fl), ct_byref(fd), self.cutoff, self.W, N.ptr(self.d_grid),
+            N.ptr(self.kapmin), None, N.ptr(self.asynth), s))
 
     def run_hydrogen(self, stream=None):
         """AHLINE into ASYNTH (engine/opacity.py:2465-2487): the hydrogen lines'
@@ -331,6 +370,7 @@ class SpectrumPlan:
             self.run_line_opacity(stats=stats)
         else:
             self.asynth.zero_()
+            self.run_fort19()
             self.run_hydrogen()
             self.split_scattering()
         self.run_rt()
@@ -387,6 +427,9 @@ class BatchPlan:
             check_wing_reach(grid, atm.doppler_per_ion)
             depth = upload_atmosphere(atm) if depth_arrays is None else depth_arrays[i]
             cat = filter_by_range(catalog, wl0, wl1)
+            if len(hydrogen_lines(cat, grid)) or len(fort19_records(cat, wl0, wl1, resolution)):
+                raise NotImplementedError(f"instance {i}: a batch takes plain lines only; hydrogen and fort.19 "
+                                          "lines run in SpectrumPlan")
             soa = build_line_soa(cat, grid, depth.n_elem, grid_origin=float(grid[0]) if grid.size else None)
             lines = upload_lines(soa, grid.size)
             d_grid = to_device(grid, np.float64)

@@ -33,6 +33,7 @@ BINDINGS: Tuple[Tuple[str, str, str], ...] = (
     ("engine.opacity", "solve_lte_spectrum", "solve_lte_spectrum"),
     ("physics.bfudge", "compute_bfudge_and_slinec", "compute_bfudge_and_slinec"),
     ("engine.opacity", "_compute_hydrogen_line_opacity", "compute_hydrogen_line_opacity"),
+    ("engine.opacity", "_add_fort19_asynth", "add_fort19_asynth"),
 )
 
 

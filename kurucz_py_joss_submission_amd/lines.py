@@ -76,6 +76,17 @@ class LineCatalog:
     n_upper: np.ndarray = field(default=None)
     code: np.ndarray = field(default=None)
     line_size: np.ndarray = field(default=None)
+    # rgfall metadata the fort.19 records are built from (atomic.py:202-232):
+    # ISO1/ISO2, the J values, LABELP starting "CONTINUU" and the raw log
+    # GR/GS/GW columns (0 where blank)
+    iso1: np.ndarray = field(default=None)
+    iso2: np.ndarray = field(default=None)
+    xj: np.ndarray = field(default=None)
+    xjp: np.ndarray = field(default=None)
+    continuum_label: np.ndarray = field(default=None)  # uint8
+    gr_log: np.ndarray = field(default=None)
+    gs_log: np.ndarray = field(default=None)
+    gw_log: np.ndarray = field(default=None)
 
     def __len__(self) -> int:
         return int(self.wavelength.size)
@@ -275,8 +286,8 @@ def load_catalog(path: Path) -> LineCatalog:
     gr_log, ok5 = _floats(_field(block, 80, 86))
     gs_log, ok6 = _floats(_field(block, 86, 92))
     gw_log, ok7 = _floats(_field(block, 92, 98))
-    _, ok8 = _floats(_field(block, 36, 41))  # J values: parsed, a bad one fails the record
-    _, ok9 = _floats(_field(block, 64, 69))
+    xj, ok8 = _floats(_field(block, 36, 41))  # J values: parsed, a bad one fails the record
+    xjp, ok9 = _floats(_field(block, 64, 69))
     good = fixed & ok0 & ok1 & ok2 & ok3 & ok4 & ok5 & ok6 & ok7 & ok8 & ok9
     good &= (_field(block, 0, 11) != b"") & (_field(block, 11, 18) != b"") & (_field(block, 18, 24) != b"")
     good &= e_col != b""
@@ -294,6 +305,9 @@ def load_catalog(path: Path) -> LineCatalog:
     nblo = nblo.astype(np.int16)
     nbup = nbup.astype(np.int16)
     iso1 = np.where(lens >= 109, _ints(_field(block, 106, 109)), 0)
+    iso2 = np.where(lens >= 119, _ints(_field(block, 116, 119)), 0)
+    labelp = np.where(lens >= 78, _field(block, 70, 78), b"")
+    continuum_label = np.char.startswith(np.char.upper(labelp), b"CONTINUU").astype(np.uint8)
     isoshift = np.where(lens >= 160, _ints(_field(block, 154, 160)), 0)
     hyper = np.ascontiguousarray(block[:, 124:134]).view("S10").ravel()
     has_hyper = (lens >= 134) & (np.char.strip(hyper) != b"")
@@ -377,6 +391,14 @@ def load_catalog(path: Path) -> LineCatalog:
         n_upper=nbup,
         code=code,
         line_size=line_size,
+        iso1=iso1.astype(np.int16),
+        iso2=iso2.astype(np.int16),
+        xj=xj,
+        xjp=xjp,
+        continuum_label=continuum_label,
+        gr_log=gr_log,
+        gs_log=gs_log,
+        gw_log=gw_log,
     )
     if not np.all(good):
         cat = _merge_whitespace(cat, recs, good)
@@ -412,14 +434,18 @@ def _merge_whitespace(cat: LineCatalog, recs, good: np.ndarray) -> LineCatalog:
         out.n_lower[j], out.n_upper[j] = r["nlo"], r["nup"]
         out.code[j] = 0.0
         out.line_size[j] = 0
+        out.iso1[j] = out.iso2[j] = 0
+        out.xj[j] = out.xjp[j] = 0.0
+        out.continuum_label[j] = 0
+        out.gr_log[j] = out.gs_log[j] = out.gw_log[j] = 0.0
     return out
 
 
 def _empty_catalog() -> LineCatalog:
     e = np.zeros(0, dtype=np.float64)
-    return LineCatalog(e, e, e, e, e, e, e, e, np.zeros(0, np.int32), np.zeros(0, np.int16),
-                       np.zeros(0, np.int8), np.zeros(0, np.int16), np.zeros(0, np.int16), e,
-                       np.zeros(0, np.int16))
+    i16 = np.zeros(0, np.int16)
+    return LineCatalog(e, e, e, e, e, e, e, e, np.zeros(0, np.int32), i16, np.zeros(0, np.int8), i16, i16, e,
+                       i16, i16, i16, e, e, np.zeros(0, np.uint8), e, e, e)
 
 
 def as_line_catalog(catalog) -> LineCatalog:
@@ -465,6 +491,15 @@ def as_line_catalog(catalog) -> LineCatalog:
         n_upper=arr("n_upper", np.int16),
         code=from_records("code", np.float64),
         line_size=from_records("line_size", np.int16),
+        iso1=from_records("iso1", np.int16),
+        iso2=from_records("iso2", np.int16),
+        xj=from_records("xj", np.float64),
+        xjp=from_records("xjp", np.float64),
+        continuum_label=np.array([str(getattr(r, "labelp", "") or "").strip().upper().startswith("CONTINUU")
+                                  for r in recs] if recs is not None else np.zeros(n), dtype=np.uint8),
+        gr_log=from_records("gamma_rad_log", np.float64),
+        gs_log=from_records("gamma_stark_log", np.float64),
+        gw_log=from_records("gamma_vdw_log", np.float64),
     )
 
 
@@ -668,3 +703,242 @@ def hydrogen_lines(catalog, grid: np.ndarray) -> HydrogenLines:
                          c(np.asarray(catalog.gf)[idx], np.float64),
                          c(np.asarray(catalog.excitation_energy)[idx], np.float64), c(nlo, np.int32),
                          c(nup, np.int32))
+
+
+# ------------------------------------------------------------- fort.19 lines
+# Element/ion codes with a WCON table column, 1-based (io/lines/fort19.py:19-22)
+_CODEX = np.array([1.0, 2.0, 2.01, 6.0, 6.01, 12.0, 12.01, 13.0, 13.01, 14.0, 14.01, 20.0, 20.01, 8.0, 11.0,
+                   5.0, 19.0], dtype=np.float64)
+# fort.19 line kinds the ASYNTH add handles (engine/opacity.py:1374-1381)
+F19_NORMAL, F19_AUTOIONIZING, F19_CONTINUUM = 0, 1, 2
+
+
+@dataclass
+class Fort19Records:
+    """The fort.19 wing records rgfall writes for a window (io/lines/fort19.py:73-90),
+    in catalog order, with the catalog index each was built from.  The float
+    fields are float32, as the record stores them."""
+
+    source: np.ndarray  # int64 catalog index
+    wavelength_vacuum: np.ndarray  # float64, nm
+    energy_lower: np.ndarray  # float32
+    oscillator_strength: np.ndarray  # float32: CGF for TYPE <= 3 (not 1), else gf
+    n_lower: np.ndarray
+    n_upper: np.ndarray
+    ion_index: np.ndarray  # NELION
+    line_type: np.ndarray
+    continuum_index: np.ndarray  # NCON
+    element_index: np.ndarray  # NELIONX
+    gamma_rad: np.ndarray  # float32
+    gamma_stark: np.ndarray
+    gamma_vdw: np.ndarray
+    nbuff: np.ndarray
+    limb: np.ndarray
+
+    def __len__(self) -> int:
+        return int(self.wavelength_vacuum.size)
+
+
+def _opt(catalog: LineCatalog, name: str, dtype) -> np.ndarray:
+    v = getattr(catalog, name, None)
+    return np.zeros(len(catalog), dtype) if v is None else np.asarray(v).astype(dtype)
+
+
+def fort19_records(catalog, wlbeg: float, wlend: float, resolution: float) -> Fort19Records:
+    """rgfall's fort.19 selection and fields (io/lines/fort19.py:233-420).
+
+    A record goes to fort.19 when its TYPE is 1 (autoionising) or above 3
+    (merged continuum: LABELP "CONTINUU", TYPE = the last member NLAST, gf
+    times 2J + 1), or when it carries NBLO/NBUP; coronal lines (TYPE 2) are
+    dropped; the DELLIM margin is the window filter's.  TYPE 1 and > 3 keep
+    their damping on the file's unnormalised scale (a positive log GS meaning
+    a negative Shore A); the others keep the catalog's normalised constants
+    and store CGF in place of gf.  The float fields are rounded to float32."""
+    catalog = as_line_catalog(catalog)
+    n = len(catalog)
+    ratiolg = math.log(1.0 + 1.0 / resolution)
+    ixwlbeg = math.floor(math.log(wlbeg) / ratiolg)
+    if math.exp(ixwlbeg * ratiolg) < wlbeg:
+        ixwlbeg += 1
+    delfactor = 1.0 if wlbeg <= 500.0 else wlbeg / 500.0
+    wl = np.asarray(catalog.wavelength, np.float64)
+    code_raw = _opt(catalog, "code", np.float64)
+    size = np.maximum(_opt(catalog, "line_size", np.int64), 0)
+    lim = np.minimum(8 - size, 7)
+    lim = np.where(np.abs(np.where(code_raw > 0.0, code_raw, 0.0) - 1.0) < 1.0e-6, 1, lim)
+    margin = _DELLIM[lim - 1] * delfactor
+    nblo = np.abs(_opt(catalog, "n_lower", np.int64))
+    nbup = np.abs(_opt(catalog, "n_upper", np.int64))
+    lt = _opt(catalog, "line_types", np.int64)
+    cont = _opt(catalog, "continuum_label", np.uint8) != 0
+    xj, xjp = _opt(catalog, "xj", np.float64), _opt(catalog, "xjp", np.float64)
+    ltype = np.where(cont, np.where(xjp > 0.0, xjp.astype(np.int64), nbup), lt)
+    cand = (wl >= wlbeg - margin) & (wl <= wlend + margin) & (ltype != 2)
+    cand &= (ltype == 1) | (ltype > 3) | (nblo + nbup != 0)
+    idx = np.nonzero(cand)[0]
+    gr_log, gs_log, gw_log = (_opt(catalog, k, np.float64) for k in ("gr_log", "gs_log", "gw_log"))
+    z = _opt(catalog, "atomic_number", np.int64)
+    stage = _opt(catalog, "ion_stages", np.int64)
+    iso1, iso2 = _opt(catalog, "iso1", np.int64), _opt(catalog, "iso2", np.int64)
+    cols = {k: [] for k in ("gf", "nelion", "nelionx", "ncon", "gr", "gs", "gw", "nbuff")}
+    for i in idx.tolist():  # the reference's scalar arithmetic, record by record
+        wlvac = float(wl[i])
+        t = int(ltype[i])
+        gf = 10.0 ** float(catalog.log_gf[i])
+        if cont[i]:
+            gf *= 2.0 * float(xj[i]) + 1.0
+        code = float(code_raw[i]) if code_raw[i] > 0.0 else 0.0
+        if code <= 0.0 and z[i] > 0:  # CODE from the element and stage
+            code = int(z[i]) + (int(stage[i]) - 1) * 0.01
+        nelem = int(code + 1.0e-6) if code > 0.0 else 0
+        icharge = int((code - nelem) * 100.0 + 0.1) if code > 0.0 else 0
+        nelion = nelem * 6 - 6 + icharge + 1
+        if 19 < nelem < 29 and icharge > 5:
+            nelion = 6 * (nelem + icharge * 10 - 30) - 1
+        nelionx = 0
+        if code > 0.0:
+            hit = np.nonzero(np.isclose(_CODEX, code, rtol=0.0, atol=1e-3))[0]
+            nelionx = int(hit[0]) + 1 if hit.size else 0
+        freq = C_LIGHT_NM / max(wlvac, 1.0e-30)
+        denom = 12.5664 * freq
+        if t == 1 or t > 3:
+            gr = 10.0 ** gr_log[i] if gr_log[i] != 0.0 else catalog.gamma_rad[i] * denom
+            if gs_log[i] > 0.0:
+                gs = -10.0 ** (-gs_log[i])
+            elif gs_log[i] < 0.0:
+                gs = 10.0 ** gs_log[i]
+            else:
+                gs = catalog.gamma_stark[i] * denom
+            gw = 10.0 ** gw_log[i] if gw_log[i] != 0.0 else catalog.gamma_vdw[i] * denom
+        else:
+            gr, gs, gw = catalog.gamma_rad[i], catalog.gamma_stark[i], catalog.gamma_vdw[i]
+            gf = CGF_CONSTANT * gf / freq
+        cols["gf"].append(gf)
+        cols["nelion"].append(nelion)
+        cols["nelionx"].append(nelionx)
+        cols["ncon"].append(int(iso2[i]) if iso1[i] == 0 and iso2[i] > 0 else 0)
+        cols["gr"].append(float(gr))
+        cols["gs"].append(float(gs))
+        cols["gw"].append(float(gw))
+        cols["nbuff"].append(int(math.log(wlvac) / ratiolg + 0.5) - int(ixwlbeg) + 1)
+    f4 = lambda k: np.asarray(cols[k], np.float32)  # noqa: E731
+    return Fort19Records(
+        source=idx.astype(np.int64), wavelength_vacuum=wl[idx].copy(),
+        energy_lower=np.asarray(catalog.excitation_energy, np.float64)[idx].astype(np.float32),
+        oscillator_strength=f4("gf"), n_lower=nblo[idx].astype(np.int16), n_upper=nbup[idx].astype(np.int16),
+        ion_index=np.asarray(cols["nelion"], np.int16), line_type=ltype[idx].astype(np.int16),
+        continuum_index=np.asarray(cols["ncon"], np.int16), element_index=np.asarray(cols["nelionx"], np.int16),
+        gamma_rad=f4("gr"), gamma_stark=f4("gs"), gamma_vdw=f4("gw"),
+        nbuff=np.asarray(cols["nbuff"], np.int32), limb=lim[idx].astype(np.int32))
+
+
+def match_catalog_to_fort19(catalog_wavelength: np.ndarray, meta_wavelength: np.ndarray,
+                            tolerance: float = 1e-3) -> dict:
+    """catalog index -> fort.19 index (engine/opacity.py:595-620): each record
+    takes the nearer of its two sorted neighbours within the tolerance, the
+    first record to claim a catalog line keeps it."""
+    mapping = {}
+    cw = np.asarray(catalog_wavelength, np.float64)
+    mw = np.asarray(meta_wavelength, np.float64)
+    if cw.size == 0 or mw.size == 0:
+        return mapping
+    order = np.argsort(cw)
+    sw = cw[order]
+    for m, pos in enumerate(np.searchsorted(sw, mw).tolist()):
+        best, best_d = None, tolerance
+        for cand in (pos, pos - 1):
+            if 0 <= cand < sw.size:
+                dlt = abs(sw[cand] - mw[m])
+                if dlt < best_d:
+                    best_d, best = dlt, cand
+        if best is not None:
+            ci = int(order[best])
+            if ci not in mapping:
+                mapping[ci] = m
+    return mapping
+
+
+# WCON table values by (NCON, NELIONX), cm^-1 (physics/tables.py:163-204; zero rows omitted)
+def _contx() -> np.ndarray:
+    with np.load(Path(__file__).resolve().parent / "data" / "kurucz_tables.npz", allow_pickle=False) as z:
+        return np.asarray(z["contx"], np.float64)
+
+
+@dataclass
+class Fort19Lines:
+    """The fort.19 records the ASYNTH add takes (engine/opacity.py:1364-1397),
+    in fort.19 order, as synthe_fort19_lines reads them: kind (F19_*), the
+    record's wavelength and centre index (engine/opacity.py:550-592), its
+    float32 gf and damping as doubles, the matched catalog line's wavelength
+    (the Doppler width's scale), lower level energy, Z - 1 and stage, the
+    record's NELION and NLAST, and CONTX(NCON, NELIONX) (0 where no WCON)."""
+
+    kind: np.ndarray
+    wavelength: np.ndarray
+    center: np.ndarray
+    gf: np.ndarray
+    gamma_rad: np.ndarray
+    gamma_stark: np.ndarray
+    gamma_vdw: np.ndarray
+    cat_wavelength: np.ndarray
+    elo: np.ndarray
+    elem: np.ndarray
+    ion: np.ndarray
+    nelion: np.ndarray
+    nlast: np.ndarray
+    cont_val: np.ndarray
+    cat_index: np.ndarray  # the matched catalog line, for per-line population quantities
+
+    def __len__(self) -> int:
+        return int(self.kind.size)
+
+
+def fort19_lines(catalog, records, grid: np.ndarray, n_ion: int, n_elem: int,
+                 catalog_to_fort19: Optional[dict] = None, contx: Optional[np.ndarray] = None) -> Fort19Lines:
+    """Select and pack the records (engine/opacity.py:1364-1397): wing types
+    normal, autoionising and merged continuum, matched to a catalog line of a
+    known element and positive stage, centre not past the grid's end.  A
+    matched line whose element or stage lies outside the bundle's population
+    table raises IndexError, where the reference's table lookup fails.
+
+    records: Fort19Records or the reference's Fort19Data (the same field
+    names); catalog_to_fort19: the engine's match (computed here when None);
+    contx: the WCON table (the reference's metal_wing_tables().contx by default)."""
+    catalog = as_line_catalog(catalog)
+    grid = np.asarray(grid, np.float64)
+    rwl = np.asarray(records.wavelength_vacuum, np.float64)
+    if catalog_to_fort19 is None:
+        catalog_to_fort19 = match_catalog_to_fort19(catalog.wavelength, rwl)
+    to_cat = {v: k for k, v in catalog_to_fort19.items()}
+    centers = nearest_grid_indices(grid, rwl) if rwl.size else np.zeros(0, np.int64)
+    contx = _contx() if contx is None else np.asarray(contx, np.float64)
+    out = {k: [] for k in Fort19Lines.__dataclass_fields__}
+    for f in range(rwl.size):
+        t = int(records.line_type[f])
+        kind = F19_NORMAL if t == 0 else F19_AUTOIONIZING if t == 1 else F19_CONTINUUM if t > 3 else -1
+        ci = to_cat.get(f)
+        if kind < 0 or ci is None:
+            continue
+        zc, st = int(catalog.atomic_number[ci]), int(catalog.ion_stages[ci])
+        if zc <= 0 or st <= 0:
+            continue
+        if st - 1 >= n_ion or zc - 1 >= n_elem:
+            raise IndexError(f"fort.19 record {f}: element {zc} stage {st} outside the population table "
+                             f"({n_ion} stages, {n_elem} elements)")
+        c = int(centers[f])
+        if c >= grid.size:
+            continue
+        ncon, nx = int(records.continuum_index[f]), int(records.element_index[f])
+        cv = 0.0
+        if kind == F19_NORMAL and ncon > 0 and nx > 0 and nx <= contx.shape[1] and ncon <= contx.shape[0]:
+            cv = float(contx[ncon - 1, nx - 1])
+        vals = dict(kind=kind, wavelength=float(rwl[f]), center=c,
+                    gf=float(records.oscillator_strength[f]), gamma_rad=float(records.gamma_rad[f]),
+                    gamma_stark=float(records.gamma_stark[f]), gamma_vdw=float(records.gamma_vdw[f]),
+                    cat_wavelength=float(catalog.wavelength[ci]), elo=float(catalog.excitation_energy[ci]),
+                    elem=zc - 1, ion=st, nelion=int(records.ion_index[f]), nlast=t, cont_val=cv, cat_index=ci)
+        for k, v in vals.items():
+            out[k].append(v)
+    i32, f8 = np.int32, np.float64
+    types = dict(kind=i32, center=np.int64, elem=i32, ion=i32, nelion=i32, nlast=i32, cat_index=np.int64)
+    return Fort19Lines(**{k: np.ascontiguousarray(np.asarray(v, types.get(k, f8))) for k, v in out.items()})

@@ -38,7 +38,7 @@ import numpy as np
 from . import lines as L
 
 PARSED_CACHE_SCHEMA = "synthe-hip-soa-1"
-PARSED_CACHE_LOGIC_VERSION = 1
+PARSED_CACHE_LOGIC_VERSION = 2
 # what reading a damaged or foreign cache file can raise
 _UNREADABLE = (OSError, ValueError, KeyError, TypeError, EOFError, zipfile.BadZipFile)
 

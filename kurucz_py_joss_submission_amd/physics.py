@@ -190,6 +190,64 @@ def compute_hydrogen_line_opacity(catalog, pops, atmosphere_model: AtmosphereMod
     return _host(out)
 
 
+def add_fort19_asynth(asynth: np.ndarray, stim: np.ndarray, wavelength: np.ndarray, continuum: np.ndarray,
+                      contx: np.ndarray, emerge: np.ndarray, emerge_h: np.ndarray, catalog, fort19_data,
+                      catalog_to_fort19, pops, atm: AtmosphereModel, cutoff: float) -> None:
+    """engine/opacity.py:1335-1546 (_add_fort19_asynth): the fort.19 records'
+    terms (normal lines with NBLO/NBUP, autoionising Shore profiles, merged-
+    continuum ramps) times STIM, added into asynth [D, W] in place.  The
+    per-line Boltzmann factors, Doppler widths, electron densities and TXNXN
+    are taken from pops as the reference takes them (computed from atm when
+    pops is None); continuum, stim, emerge and contx are used as given."""
+    from .atmosphere import depth_doppler_scale, depth_hckt, depth_txnxn
+    from .lines import fort19_lines
+
+    if fort19_data is None or len(fort19_data.wavelength_vacuum) == 0:
+        return
+    if atm.population_per_ion is None:
+        return
+    wl = np.asarray(wavelength, dtype=np.float64)
+    pop = np.ascontiguousarray(atm.population_per_ion, np.float64)
+    D, W = atm.layers, wl.size
+    f = fort19_lines(catalog, fort19_data, wl, pop.shape[1], pop.shape[2], catalog_to_fort19, contx)
+    if len(f) == 0:
+        return
+    c = np.ascontiguousarray
+    rho = np.zeros(D) if atm.mass_density is None else c(atm.mass_density, np.float64)
+    if pops is not None:
+        layers = [pops.layers[d] for d in range(D)]
+        boltz = c(np.array([[float(layers[d].boltzmann_factor[i]) for d in range(D)] for i in f.cat_index]).reshape(
+            len(f), D))
+        dopp = c(np.array([[float(layers[d].doppler_width[i]) for d in range(D)] for i in f.cat_index]).reshape(
+            len(f), D))
+        xne = c([float(layers[d].electron_density) for d in range(D)], np.float64)
+        tx = c([float(layers[d].txnxn) for d in range(D)], np.float64)
+        hckt = vscale = None
+    else:
+        boltz = dopp = None
+        xne = c(atm.electron_density, np.float64)
+        tx = c(depth_txnxn(atm))
+        hckt, vscale = c(depth_hckt(atm)), c(depth_doppler_scale(atm))
+    keep = [pop, rho, xne, tx, c(atm.temperature, np.float64), c(emerge, np.float64), c(emerge_h, np.float64),
+            boltz, hckt, dopp, vscale]
+    fl = N.F19Lines(len(f), *(x.ctypes.data for x in (f.kind, f.wavelength, f.center, f.gf, f.gamma_rad,
+                                                     f.gamma_stark, f.gamma_vdw, f.elem, f.ion, f.nelion, f.nlast,
+                                                     f.cont_val, f.cat_wavelength, f.elo)))
+    fd = N.F19Depths(D, pop.shape[1], pop.shape[2], *(None if x is None else x.ctypes.data for x in keep))
+    cont = np.asarray(continuum, np.float64)
+    st = np.asarray(stim, np.float64)
+    if cont.shape != (D, W) or st.shape != (D, W) or np.shape(asynth) != (D, W):
+        raise ValueError(f"asynth, continuum and stim must be [D, W] = {(D, W)}")
+    import ctypes
+
+    ctx = N.context()
+    d_wl, d_c, d_s, d_a = _dev(wl), _dev(cont), _dev(st), _dev(asynth)
+    N.check("synthe_fort19_lines", ctx.lib.synthe_fort19_lines(
+        ctx.handle, ctypes.byref(fl), ctypes.byref(fd), float(cutoff), W, N.ptr(d_wl), N.ptr(d_c), N.ptr(d_s),
+        N.ptr(d_a), N.stream_handle()))
+    asynth[...] = _host(d_a)
+
+
 def _departure_factor(data, layers: int) -> np.ndarray:
     """physics/bfudge.py:26-69 (_prep_factor with one column): a missing table
     is ones; a [D] or [D][k] table gives its first column; another row count

@@ -14,6 +14,8 @@ the checker.
 * line_soa          physics/line_opacity.py:700-784 (the TRANSP prep loop);
                     centre indices from the C restatements of
                     engine/opacity.py:550-592 and physics/line_opacity.py:1030-1059
+* fort19_build      io/lines/fort19.py:233-420 (rgfall's fort.19 records)
+* match_fort19      engine/opacity.py:595-620 (catalog line of each record)
 
 The ionisation potentials (POTION, cm^-1) are the reference's data table
 (synthe_py/data/fortran_data.npz), carried in the package's table archive.
@@ -131,6 +133,14 @@ class Record:
     n_upper: int = 0
     code: float = 0.0
     line_size: int = 0
+    iso1: int = 0
+    iso2: int = 0
+    labelp: str = ""
+    xj: float = 0.0
+    xjp: float = 0.0
+    gr_log: float = 0.0
+    gs_log: float = 0.0
+    gw_log: float = 0.0
 
 
 def _int_or(text: str, default: int = 0) -> int:
@@ -207,6 +217,8 @@ def _fixed_width(line: str) -> Record:
         except ValueError:
             pass
     iso1 = _int_or(col(106, 109)) if n >= 109 and col(106, 109) else 0
+    iso2 = _int_or(col(116, 119)) if n >= 119 and col(116, 119) else 0
+    labelp = col(70, 78) if n >= 78 else ""
     size, tag = 0, ""
     tail = line[134:144] if n >= 144 else ""
     if tail.strip():
@@ -225,10 +237,10 @@ def _fixed_width(line: str) -> Record:
         ltype = 0
     # the J values are parsed last; a malformed one sends the record to the
     # whitespace reader
-    float(xj) if xj else 0.0
-    float(xjp) if xjp else 0.0
+    xj_val = float(xj) if xj else 0.0
+    xjp_val = float(xjp) if xjp else 0.0
     return Record(wavelength, wavelength, element, stage, log_gf, e_lo, g_rad, g_stark, g_vdw, ltype,
-                  n_lo, n_up, code, size)
+                  n_lo, n_up, code, size, iso1, iso2, labelp, xj_val, xjp_val, glog[0], glog[1], glog[2])
 
 
 def _whitespace(line: str) -> Optional[Record]:
@@ -394,3 +406,103 @@ def line_soa(cat: Catalog, grid: np.ndarray, n_elem: int, grid_origin: Optional[
     return Soa(wl, gf, cgf, gr, gs, gw, np.ascontiguousarray(cat.excitation, np.float64), elem, ion, ltype,
                process, nearest_grid_indices(grid, idx),
                wing_center_indices(grid, idx, grid_origin if grid_origin else float(grid[0])))
+
+
+# ------------------------------------------------------------- fort.19 records
+CODEX = (1.0, 2.0, 2.01, 6.0, 6.01, 12.0, 12.01, 13.0, 13.01, 14.0, 14.01, 20.0, 20.01, 8.0, 11.0, 5.0, 19.0)
+
+
+@dataclass
+class F19:
+    """One fort.19 record (io/lines/fort19.py:73-90); gf and damping rounded to float32."""
+
+    source: int
+    wavelength: float
+    gf: float
+    n_lower: int
+    n_upper: int
+    nelion: int
+    line_type: int
+    ncon: int
+    nelionx: int
+    gamma_rad: float
+    gamma_stark: float
+    gamma_vdw: float
+
+
+def _f32(x: float) -> float:
+    return float(np.float32(x))
+
+
+def fort19_build(cat: Catalog, wlbeg: float, wlend: float, resolution: float) -> List[F19]:
+    """build_from_catalog (io/lines/fort19.py:233-420), record by record."""
+    out: List[F19] = []
+    ratiolg = math.log(1.0 + 1.0 / resolution)
+    delfactor = 1.0 if wlbeg <= 500.0 else wlbeg / 500.0
+    for i, rec in enumerate(cat.records):
+        wlvac = float(rec.wavelength)
+        lim = min(8 - (rec.line_size if rec.line_size > 0 else 0), 7)
+        if abs((float(rec.code) if rec.code > 0.0 else 0.0) - 1.0) < 1.0e-6:
+            lim = 1
+        margin = DELLIM[lim - 1] * delfactor
+        if wlvac < wlbeg - margin or wlvac > wlend + margin:
+            continue
+        line_type = int(rec.line_type)
+        gf = 10.0 ** float(rec.log_gf)
+        if rec.labelp.strip().upper().startswith("CONTINUU"):
+            line_type = int(rec.xjp) if rec.xjp > 0.0 else int(rec.n_upper)
+            gf *= 2.0 * float(rec.xj) + 1.0
+        if line_type == 2:
+            continue
+        code = float(rec.code) if rec.code > 0.0 else 0.0
+        if code <= 0.0:
+            nel = PARSER_SYMBOLS.index(rec.element) if rec.element in PARSER_SYMBOLS else 0
+            if nel > 0:
+                code = nel + (rec.ion_stage - 1) * 0.01
+        nelem = int(code + 1.0e-6) if code > 0.0 else 0
+        icharge = int((code - nelem) * 100.0 + 0.1) if code > 0.0 else 0
+        nelion = nelem * 6 - 6 + icharge + 1
+        if 19 < nelem < 29 and icharge > 5:
+            nelion = 6 * (nelem + icharge * 10 - 30) - 1
+        nelionx = 0
+        if code > 0.0:
+            for k, cx in enumerate(CODEX):
+                if abs(cx - code) <= 1e-3:
+                    nelionx = k + 1
+                    break
+        nblo, nbup = abs(int(rec.n_lower)), abs(int(rec.n_upper))
+        if not (line_type == 1 or line_type > 3 or nblo + nbup != 0):
+            continue
+        denom = 12.5664 * (C_NM / max(wlvac, 1.0e-30))
+        if line_type == 1 or line_type > 3:
+            gr = 10.0 ** rec.gr_log if rec.gr_log != 0.0 else rec.gamma_rad * denom
+            if rec.gs_log > 0.0:
+                gs = -10.0 ** (-rec.gs_log)
+            elif rec.gs_log < 0.0:
+                gs = 10.0 ** rec.gs_log
+            else:
+                gs = rec.gamma_stark * denom
+            gw = 10.0 ** rec.gw_log if rec.gw_log != 0.0 else rec.gamma_vdw * denom
+        else:
+            gr, gs, gw = rec.gamma_rad, rec.gamma_stark, rec.gamma_vdw
+            gf = CGF_FACTOR * gf / (C_NM / max(wlvac, 1.0e-30))
+        out.append(F19(i, wlvac, _f32(gf), nblo, nbup, nelion, line_type,
+                       rec.iso2 if rec.iso1 == 0 and rec.iso2 > 0 else 0, nelionx, _f32(gr), _f32(gs), _f32(gw)))
+    return out
+
+
+def match_fort19(cat_wavelength: np.ndarray, f19_wavelength: np.ndarray, tol: float = 1e-3) -> dict:
+    """_match_catalog_to_fort19 (engine/opacity.py:595-620): catalog index -> record index."""
+    mapping: dict = {}
+    if cat_wavelength.size == 0 or f19_wavelength.size == 0:
+        return mapping
+    order = np.argsort(cat_wavelength)
+    srt = cat_wavelength[order]
+    for m, pos in enumerate(np.searchsorted(srt, f19_wavelength)):
+        best, best_d = None, tol
+        for cand in (pos, pos - 1):
+            if 0 <= cand < srt.size and abs(srt[cand] - f19_wavelength[m]) < best_d:
+                best_d, best = abs(srt[cand] - f19_wavelength[m]), cand
+        if best is not None and int(order[best]) not in mapping:
+            mapping[int(order[best])] = m
+    return mapping

@@ -13,6 +13,7 @@ frequency quantities, TRANSP, ASYNTH, LTE source, RT, unit conversion.
 from __future__ import annotations
 
 import ctypes as ct
+import math
 import os
 import subprocess
 from dataclasses import dataclass, field
@@ -40,7 +41,7 @@ class _Tables(ct.Structure):
 
 def build(force: bool = False) -> Path:
     """Compile liboracle.so with the committed Makefile."""
-    if force or not _LIB_PATH.exists() or _LIB_PATH.stat().st_mtime < (_HERE / "synthe_oracle.c").stat().st_mtime:
+    if force or not _LIB_PATH.exists() or _LIB_PATH.stat().st_mtime < max(f.stat().st_mtime for f in _HERE.glob("*.c")):
         subprocess.run(["make", "-C", str(_HERE), "-B" if force else "liboracle.so"], check=True,
                        stdout=subprocess.DEVNULL)
     return _LIB_PATH
@@ -453,6 +454,129 @@ def hydrogen_lines(cat, atm, grid, cont, stim, cutoff=1e-3, microturb_kms=0.0):
     return out, float(stats[0])
 
 
+# ------------------------------------------------------------- fort.19 lines
+class _F19Tables(ct.Structure):
+    _fields_ = [("h0", _D), ("h1", _D), ("h2", _D), ("nvoigt", ct.c_int),
+                ("extab", _D), ("extabf", _D), ("nfastex", ct.c_int)]
+
+
+def doppler_scale(atm, microturb_kms: float = 0.0) -> np.ndarray:
+    """Doppler width / wavelength per depth for the engine's default line mass,
+    56 amu (physics/populations.py:131-190; the engine passes no elements)."""
+    out = []
+    vt = getattr(atm, "turbulent_velocity", None)
+    for i, t in enumerate(np.asarray(atm.temperature, np.float64).tolist()):
+        t = max(t, 1.0)
+        th = np.sqrt(2.0 * 1.380649e-16 * t / (np.array([56.0]) * 1.66054e-24)) / 2.99792458e10
+        v = (float(vt[i]) if vt is not None and len(vt) > i else 0.0) / 1e5 / 299792.458
+        m = microturb_kms / 299792.458 if microturb_kms > 0 else 0.0
+        turb = math.sqrt(v * v + m * m)
+        out.append(float(np.sqrt(turb ** 2 + th ** 2)[0]))
+    return np.array(out, np.float64)
+
+
+def fort19_selection(cat, f19, grid, n_ion, n_elem):
+    """The records _add_fort19_asynth takes (engine/opacity.py:1364-1397), with
+    their matched catalog index and centre."""
+    import catalog as C_
+
+    fw = np.array([r.wavelength for r in f19], np.float64)
+    to_cat = {v: k for k, v in C_.match_fort19(np.asarray(cat.wavelength, np.float64), fw).items()}
+    centers = nearest_grid_indices(grid, fw) if fw.size else np.zeros(0, np.int64)
+    sel = []
+    for f, r in enumerate(f19):
+        kind = 0 if r.line_type == 0 else 1 if r.line_type == 1 else 2 if r.line_type > 3 else -1
+        ci = to_cat.get(f)
+        if kind < 0 or ci is None:
+            continue
+        rec = cat.records[ci]
+        z = C_._ENGINE_Z.get(str(rec.element).strip().upper().replace(" ", ""))
+        if z is None or rec.ion_stage <= 0:
+            continue
+        if int(centers[f]) >= grid.size:
+            continue
+        if rec.ion_stage - 1 >= n_ion or z - 1 >= n_elem:
+            raise IndexError("fort.19 record outside the population table")
+        sel.append((f, kind, ci, z - 1, rec.ion_stage, int(centers[f])))
+    return sel
+
+
+def fort19_asynth(cat, atm, grid, cont, stim, asynth_arr, wl_start, wl_end, resolution, cutoff=1e-3,
+                  microturb_kms=0.0):
+    """_add_fort19_asynth (engine/opacity.py:1335-1546) into asynth_arr [D, W],
+    in place; returns the number of records taken."""
+    import catalog as C_
+
+    t = tables()
+    if atm.population_per_ion is None:
+        return 0
+    f19 = C_.fort19_build(cat, wl_start, wl_end, resolution)
+    pop = _c(atm.population_per_ion, np.float64)
+    sel = fort19_selection(cat, f19, grid, pop.shape[1], pop.shape[2])
+    if not sel:
+        return 0
+    D, W, N = atm.layers, grid.size, len(sel)
+    hk = hckt(atm)
+    vs = doppler_scale(atm, microturb_kms)
+    boltz = np.zeros((N, D))
+    dopp = np.zeros((N, D))
+    cols = {k: [] for k in ("kind", "wl", "center", "gf", "gr", "gs", "gw", "elem", "ion", "nelion", "nlast",
+                            "ncon", "nelionx")}
+    for j, (f, kind, ci, el, st, c) in enumerate(sel):
+        r = f19[f]
+        e = float(cat.excitation[ci])
+        for d in range(D):
+            boltz[j, d] = lib().orc_fastex(ct.c_double(e * hk[d]), p(t.extab), p(t.extabf), ct.c_int(1001))
+            dopp[j, d] = float(cat.wavelength[ci]) * vs[d]
+        for k, v in (("kind", kind), ("wl", r.wavelength), ("center", c), ("gf", r.gf), ("gr", r.gamma_rad),
+                     ("gs", r.gamma_stark), ("gw", r.gamma_vdw), ("elem", el), ("ion", st), ("nelion", r.nelion),
+                     ("nlast", r.line_type), ("ncon", r.ncon), ("nelionx", r.nelionx)):
+            cols[k].append(v)
+    ne = np.maximum(_xne(atm), 1e-40)
+    nmerge = np.maximum(1600.0 / np.power(ne, 2.0 / 15.0) - 1.5, 1.0)
+    emerge = 109737.312 / np.maximum(nmerge ** 2, 1e-12)
+    emerge_h = 109677.576 / np.maximum(nmerge ** 2, 1e-12)
+    rho = getattr(atm, "mass_density", None)
+    rho = np.zeros(D) if rho is None else _c(rho, np.float64)
+    depth = dict(pop=pop, rho=rho, xne=_xne(atm), txnxn=txnxn(atm), emerge=emerge, emerge_h=emerge_h,
+                 boltz=boltz, doppler=dopp)
+    fort19_raw(grid, cols, depth, cont, stim, asynth_arr, cutoff)
+    return N
+
+
+def fort19_raw(grid, rec: dict, depth: dict, cont, stim, asynth_arr, cutoff=1e-3):
+    """orc_fort19_asynth on explicit arrays: rec holds the per-record columns
+    (kind, wl, center, gf, gr, gs, gw, elem, ion, nelion, nlast, ncon, nelionx),
+    depth the per-depth ones (pop [D][nion][nelem], rho, xne, txnxn, emerge,
+    emerge_h) and boltz, doppler [N][D]; asynth_arr [D, W] is added to in place."""
+    t = tables()
+    i32 = {k: np.ascontiguousarray(rec[k], np.int32) for k in ("kind", "elem", "ion", "nelion", "nlast", "ncon",
+                                                               "nelionx")}
+    f8 = {k: np.ascontiguousarray(rec[k], np.float64) for k in ("wl", "gf", "gr", "gs", "gw")}
+    center = np.ascontiguousarray(rec["center"], np.int64)
+    N = center.size
+    pop = _c(depth["pop"], np.float64)
+    D, W = pop.shape[0], grid.size
+    with np.load(_HERE.parent / "kurucz_py_joss_submission_amd" / "data" / "kurucz_tables.npz",
+                 allow_pickle=False) as z:
+        contx = np.ascontiguousarray(z["contx"], np.float64)
+    tb = _F19Tables(p(t.h0), p(t.h1), p(t.h2), 2001, p(t.extab), p(t.extabf), 1001)
+    out = _c(asynth_arr, np.float64)
+    dd = {k: _c(depth[k], np.float64) for k in ("rho", "xne", "txnxn", "emerge", "emerge_h", "boltz", "doppler")}
+    lib().orc_fort19_asynth(
+        ct.c_long(N), ct.c_long(D), ct.c_long(W), p(_c(grid, np.float64)), p(i32["kind"], _I32), p(f8["wl"]),
+        p(center, _I64), p(f8["gf"]), p(f8["gr"]), p(f8["gs"]), p(f8["gw"]), p(i32["elem"], _I32),
+        p(i32["ion"], _I32), p(i32["nelion"], _I32), p(i32["nlast"], _I32), p(i32["ncon"], _I32),
+        p(i32["nelionx"], _I32), p(contx), p(dd["boltz"]), p(dd["doppler"]), p(pop), ct.c_long(pop.shape[1]),
+        ct.c_long(pop.shape[2]), p(dd["rho"]), p(dd["xne"]), p(dd["txnxn"]), p(dd["emerge"]), p(dd["emerge_h"]),
+        p(_c(cont, np.float64)), p(_c(stim, np.float64)), ct.c_double(cutoff), ct.byref(tb), p(out))
+    if out is not asynth_arr:
+        asynth_arr[...] = out
+    if out is not asynth_arr:
+        asynth_arr[...] = out
+    return N
+
+
 @dataclass
 class OracleResult:
     wavelength: np.ndarray
@@ -498,6 +622,9 @@ def synthesize(atm, lines, wl_start, wl_end, resolution, cutoff=1e-3, microturb_
     tm["transp"] = time.perf_counter() - t0
     t0 = time.perf_counter()
     asy, st = asynth(soa, atm, grid, tr, valid, kapmin, cutoff)
+    # engine/opacity.py:2366-2387: the fort.19 records join ASYNTH before the hydrogen lines
+    st["fort19_records"] = float(fort19_asynth(lines, atm, grid, kapmin, stim, asy, wl_start, wl_end, resolution,
+                                               cutoff, microturb_kms))
     ahl, st["h_profile_evals"] = hydrogen_lines(lines, atm, grid, kapmin, stim, cutoff, microturb_kms)
     asy = asy + ahl  # engine/opacity.py:2483-2487: AHLINE joins ASYNTH before the FSCAT split
     tm["asynth"] = time.perf_counter() - t0

@@ -1,0 +1,110 @@
+"""fort.19 lines on the GPU (synthe_fort19_lines) against the oracle:
+normal lines with NBLO/NBUP (with and without WCON/WTAIL), autoionising
+Shore profiles and merged-continuum ramps, mixed into synthetic line lists,
+through SpectrumPlan and through the engine's drop-in."""
+
+from __future__ import annotations
+
+import tempfile
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from conftest import make_small
+from gfall_special import record
+
+pytestmark = pytest.mark.gpu
+
+
+def _specials(w0: float, w1: float, seed: int):
+    """Twelve records around the window, strong enough to show against the
+    continuum without swamping it (high lower levels, small gf)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(12):
+        wl = float(rng.uniform(w0 - 0.05, w1 + 0.05))
+        kind = i % 3
+        if kind == 0:  # autoionising (Ca I), Shore A of either sign
+            out.append(record(wl, float(rng.uniform(-9, -7)), 20.00, float(rng.uniform(60000, 80000)),
+                              gr=float(rng.uniform(10.0, 11.3)), gs=float(rng.choice([1.5, -2.0, 0.0])),
+                              gw=float(rng.uniform(-0.5, 0.7)), tag="AUT"))
+        elif kind == 1:  # merged continuum (Mg I): TYPE = XJP
+            out.append(record(wl, float(rng.uniform(-13, -11)), 12.00, float(rng.uniform(60000, 80000)),
+                              xj=0.5, xjp=float(rng.integers(20, 90)), labelp="CONTINUU"))
+        else:  # normal line with NBLO/NBUP (Mg I, NCON 4: WCON near 252-259 nm)
+            out.append(record(wl, float(rng.uniform(-6, -4)), 12.00, float(rng.uniform(0, 20000)),
+                              gw=float(rng.uniform(-7.8, -6.5)), n_lower=3, n_upper=5, iso2=4))
+    return out
+
+
+def _workload(name, w0, w1, seed, teff=5750.0, logg=4.44):
+    from kurucz_py_joss_submission_amd import lines as L
+    from kurucz_py_joss_submission_amd import synthetic as S
+
+    wk = make_small(name, teff, logg, 1500, w0, w1, seed=seed)
+    td = Path(tempfile.mkdtemp(prefix="f19_"))
+    gf = S.write_workload_gfall(wk, td / "lines.gfall")
+    text = gf.read_text().splitlines() + _specials(w0, w1, seed)
+    gf.write_text("\n".join(sorted(text, key=lambda r: float(r[:11]))) + "\n")
+    return wk, L.filter_by_range(L.load_catalog(gf), w0, w1), gf
+
+
+@pytest.mark.parametrize("window,teff,logg", [((500.0, 500.8), 5750.0, 4.44), ((253.0, 258.5), 5750.0, 4.44),
+                                              ((400.0, 400.6), 9500.0, 3.5)])
+def test_fort19_spectrum_matches_oracle(oracle, window, teff, logg):
+    from kurucz_py_joss_submission_amd.device import SpectrumPlan
+
+    wk, cat, gf = _workload(f"f19_{window[0]:g}", *window, seed=int(window[0]), teff=teff, logg=logg)
+    plan = SpectrumPlan(wk.atmosphere, cat, *window, 300000.0)
+    assert len(plan.f19) >= 8 and set(plan.f19.kind.tolist()) == {0, 1, 2}
+    plan.run()
+    ref = oracle.synthesize(wk.atmosphere, str(gf), *window, 300000.0)
+    assert ref.stats["fort19_records"] == len(plan.f19)
+    asy = plan.asynth.cpu().numpy()
+    ra = ref.stages["asynth"]
+    assert np.max(np.abs(asy - ra) / np.maximum(np.abs(ra), 1e-300)) < 1e-12
+    flux = plan.flux.cpu().numpy()
+    assert np.max(np.abs(flux - ref.flux) / np.abs(ref.flux)) < 1e-6
+    # the fort.19 terms are a visible part of ASYNTH
+    assert np.max(_oracle_f19(oracle, wk, gf, window, ref) / np.maximum(ra, 1e-300)) > 0.01
+
+
+def _oracle_f19(oracle, wk, gf, window, ref):
+    import catalog as C
+
+    cat = C.filter_by_range(C.read_gfall(gf), *window)
+    add = np.zeros_like(ref.stages["asynth"])
+    _, stim = oracle.freq_quantities(ref.wavelength, wk.atmosphere.temperature)[:2]
+    oracle.fort19_asynth(cat, wk.atmosphere, ref.wavelength, ref.stages["cont_abs"] + ref.stages["cont_scat"], stim,
+                         add, *window, 300000.0)
+    assert add.max() > 0
+    return add
+
+
+def test_fort19_drop_in_matches_oracle(oracle):
+    """physics.add_fort19_asynth (the engine's _add_fort19_asynth) on the
+    reference's argument list, pops None (per-line quantities from atm)."""
+    from kurucz_py_joss_submission_amd import lines as L
+    from kurucz_py_joss_submission_amd import physics
+    from kurucz_py_joss_submission_amd.atmosphere import merge_energies
+
+    window = (500.0, 500.8)
+    wk, cat, gf = _workload("f19_dropin", *window, seed=9)
+    grid = oracle.wavelength_grid(*window, 300000.0)
+    ca, cs = oracle.continuum(wk.atmosphere, grid)
+    _, stim, _, _ = oracle.freq_quantities(grid, wk.atmosphere.temperature)
+    base = np.random.default_rng(1).uniform(0.0, 1.0, (wk.atmosphere.layers, grid.size))
+    rec = L.fort19_records(cat, *window, 300000.0)
+    em, emh = merge_energies(wk.atmosphere)
+    got = base.copy()
+    physics.add_fort19_asynth(got, stim, grid, ca + cs, L._contx(), em, emh, cat, rec, None, None,
+                              wk.atmosphere, 1e-3)
+    import catalog as C
+
+    want = base.copy()
+    n = oracle.fort19_asynth(C.filter_by_range(C.read_gfall(gf), *window), wk.atmosphere, grid, ca + cs, stim,
+                             want, *window, 300000.0)
+    assert n >= 8
+    assert np.max(np.abs(got - want) / np.abs(want)) < 1e-13
+    assert np.max(np.abs(want - base)) > 0

@@ -46,14 +46,17 @@ def ref_layout():
     rad = _module(f"{PKG}.engine.radiative", solve_josh_flux=js.solve_josh_flux,
                   solve_lte_spectrum=orig("lte_spectrum"), solve_lte_frequency=orig("lte_frequency"))
     # engine/opacity.py:23-35: module imports plus `from .radiative import solve_lte_spectrum`
-    # and its own _compute_hydrogen_line_opacity (:1706), called by name at :2467
+    # and its own _compute_hydrogen_line_opacity (:1706) and _add_fort19_asynth (:1335),
+    # called by name at :2467 and :2375
     op = _module(f"{PKG}.engine.opacity", line_opacity=lo, continuum=co, bfudge=bf,
-                 solve_lte_spectrum=rad.solve_lte_spectrum, _compute_hydrogen_line_opacity=orig("hydrogen"))
+                 solve_lte_spectrum=rad.solve_lte_spectrum, _compute_hydrogen_line_opacity=orig("hydrogen"),
+                 _add_fort19_asynth=orig("fort19"))
 
-    def run_synthesis():  # the engine's call sites (2236, 2320, 2352, 2467, 3788)
+    def run_synthesis():  # the engine's call sites (2236, 2320, 2352, 2375, 2467, 3788)
         g = op.__dict__
         return [g["continuum"].build_depth_continuum(), g["line_opacity"].compute_transp(),
-                g["line_opacity"].compute_asynth_from_transp(), g["_compute_hydrogen_line_opacity"](),
+                g["line_opacity"].compute_asynth_from_transp(), g["_add_fort19_asynth"](),
+                g["_compute_hydrogen_line_opacity"](),
                 g["solve_lte_spectrum"](), rad.solve_josh_flux(), g["bfudge"].compute_bfudge_and_slinec()]
 
     op.run_synthesis = run_synthesis
@@ -69,20 +72,22 @@ def test_install_reaches_every_engine_call_site(ref_layout):
     impl = types.SimpleNamespace()
     for name in {b[2] for b in integration.BINDINGS}:
         setattr(impl, name, (lambda n: (lambda *a, **k: calls.append(("gpu", n)) or n))(name))
-    assert op.run_synthesis() == ["continuum", "transp", "asynth", "hydrogen", "lte_spectrum", "josh", "bfudge"]
+    assert op.run_synthesis() == ["continuum", "transp", "asynth", "fort19", "hydrogen", "lte_spectrum", "josh",
+                                  "bfudge"]
     assert all(who == "ref" for who, _ in calls)
     calls.clear()
     saved = integration.install(PKG, impl)
     assert len(saved) == len(integration.BINDINGS)
     op.run_synthesis()
     assert calls == [("gpu", "build_depth_continuum"), ("gpu", "compute_transp"),
-                     ("gpu", "compute_asynth_from_transp"), ("gpu", "compute_hydrogen_line_opacity"),
+                     ("gpu", "compute_asynth_from_transp"), ("gpu", "add_fort19_asynth"),
+                     ("gpu", "compute_hydrogen_line_opacity"),
                      ("gpu", "solve_lte_spectrum"),
                      ("gpu", "solve_josh_flux"), ("gpu", "compute_bfudge_and_slinec")]
     calls.clear()
     integration.uninstall(saved)
     op.run_synthesis()
-    assert all(who == "ref" for who, _ in calls) and len(calls) == 7
+    assert all(who == "ref" for who, _ in calls) and len(calls) == 8
 
 
 def test_default_impl_is_the_drop_in_module(ref_layout):
