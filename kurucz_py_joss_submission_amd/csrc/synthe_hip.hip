@@ -3549,6 +3549,20 @@ __device__ __forceinline__ double f19_normal_value(const F19Pair& p, long k, dou
   return v;
 }
 
+// A helium line's wing value (engine/opacity.py:998-1053, the TYPE < -2
+// branch of _apply_fort19_profile): KAPPA0 H(a, |dλ| / Δλ_D), with WTAIL's
+// taper (base WCON, else the line).
+__device__ __forceinline__ double f19_he_value(const F19Pair& p, double wave, double line_wl,
+                                               const DeviceTables& tb) {
+  double v = p.kappa0 * voigt(fabs(wave - line_wl) / p.dopple, p.adamp, tb.h0, tb.h1, tb.h2);
+  if (p.wtail > 0.0 && wave < p.wtail) {
+    const double base = p.wcon > 0.0 ? p.wcon : line_wl;
+    const double span = p.wtail - base;
+    v = v * (wave - base) / (span > 1e-12 ? span : 1e-12);
+  }
+  return v;
+}
+
 // Shore profile of an autoionising line (engine/opacity.py:895-963).
 __device__ __forceinline__ double f19_auto_value(const F19Line& L, double kappa0, double wave) {
   const double fc = kCLightCm / (L.wl * kNmToCm);
@@ -3567,6 +3581,27 @@ __device__ __forceinline__ long f19_search(const double* grid, long W, double x,
     else hi = mid;
   }
   return lo;
+}
+
+// WCON / WTAIL of a record at depth d (opacity.py:646-686), vacuum
+// wavelengths; -1 where the reference has None.
+__device__ __forceinline__ void f19_limits(const F19Line& L, const F19DepthDev& dp, int d, F19Pair* p) {
+  p->wcon = p->wtail = -1.0;
+  if (!(L.cont_val > 0.0)) return;
+  const double emerge_line = L.nelion == 1 ? dp.emerge_h[d] : dp.emerge[d];
+  const double den = L.cont_val - emerge_line;
+  if (!(fabs(den) > 1e-8)) return;
+  const double wcon = 1.0e7 / den;
+  const double dt = den - 500.0;
+  double wtail = -1.0;
+  if (fabs(dt) > 1e-8) {
+    wtail = 1.0e7 / dt;
+    if (wtail < 0.0) wtail = 2.0 * wcon;
+    wtail = 2.0 * wcon < wtail ? 2.0 * wcon : wtail;
+  }
+  if (wtail > 0.0 && wtail <= wcon) wtail = -1.0;
+  p->wcon = wcon > 0.0 ? wcon : -1.0;
+  p->wtail = wtail > 0.0 ? wtail : -1.0;
 }
 
 // Gates and set-up of one (record, depth) pair (engine/opacity.py:1383-1546),
@@ -3607,24 +3642,7 @@ __global__ __launch_bounds__(64) void k_f19_extent(const F19Line* __restrict__ l
         p.center_add = kapcen >= kapmin;
         p.kappa0 = kappa0;
         p.dopple = dopple;
-        p.wcon = p.wtail = -1.0;
-        if (L.cont_val > 0.0) {  // WCON / WTAIL (opacity.py:646-686), vacuum wavelengths
-          const double emerge_line = L.nelion == 1 ? dp.emerge_h[d] : dp.emerge[d];
-          const double den = L.cont_val - emerge_line;
-          if (fabs(den) > 1e-8) {
-            const double wcon = 1.0e7 / den;
-            const double dt = den - 500.0;
-            double wtail = -1.0;
-            if (fabs(dt) > 1e-8) {
-              wtail = 1.0e7 / dt;
-              if (wtail < 0.0) wtail = 2.0 * wcon;
-              wtail = 2.0 * wcon < wtail ? 2.0 * wcon : wtail;
-            }
-            if (wtail > 0.0 && wtail <= wcon) wtail = -1.0;
-            p.wcon = wcon > 0.0 ? wcon : -1.0;
-            p.wtail = wtail > 0.0 ? wtail : -1.0;
-          }
-        }
+        f19_limits(L, dp, d, &p);
         // the metal wing loop's set-up and near-wing probe (opacity.py:128-198)
         p.adamp = adamp > 1e-12 ? adamp : 1e-12;
         if (cc < W - 1) p.resolu = 1.0 / (grid[cc + 1] / grid[cc] - 1.0);
@@ -3761,6 +3779,76 @@ __global__ __launch_bounds__(64) void k_f19_extent(const F19Line* __restrict__ l
         else p.blue = ext;
       }
     }
+  } else if (ok && L.kind == 3) {  // helium line (opacity.py:2716-3066, then :998-1053)
+    const double dop_val = dp.doppler[(long)r * D + d];  // DOPPLE of the line's stage
+    const double dw = dop_val * L.wl;
+    const long c = L.c;
+    const double* grow = grid;
+    ok = dop_val > 0.0 && !(dw <= 0.0);
+    const double xnfdop = pop_val / (rho * dop_val);
+    const double kapmin = crow[c] * cutoff;
+    const double kappa0_pre = L.gf * xnfdop;
+    const double kappa0 = kappa0_pre * boltz;
+    ok = ok && !(kappa0_pre < kapmin) && !(kappa0 < kapmin);
+    if (ok) {
+      f19_limits(L, dp, d, &p);
+      double doppler = dw > 1e-12 ? dw : 1e-12;
+      double keff = kappa0;
+      if (L.nlast == -4) {  // 3He
+        keff /= 1.155;
+        doppler *= 1.155;
+      }
+      const double dl = doppler / L.wl;
+      const double damping = (L.gr + L.gs * dp.xne[d] + L.gw * dp.txnxn[d]) / (dl > 1e-40 ? dl : 1e-40);
+      p.active = 1;
+      p.kappa0 = keff;
+      p.dopple = doppler;
+      p.adamp = damping > 1e-12 ? damping : 1e-12;
+      p.cpix = c;
+      // red wing: c .. W-1; a pixel at or below WCON is passed over, the
+      // first value below the cutoff ends the wing (not added)
+      long red_end = c;
+      if (L.wl <= grow[W - 1]) {
+        red_end = W;
+        for (long i0 = c; i0 < W; i0 += 64) {
+          const long i = i0 + lane;
+          bool s = false;
+          if (i < W) {
+            const double wave = grow[i];
+            if (!(p.wcon > 0.0 && wave <= p.wcon)) s = f19_he_value(p, wave, L.wl, tb) < crow[i] * cutoff;
+          }
+          const unsigned long long m = __ballot(s);
+          if (m) {
+            red_end = i0 + __builtin_ctzll(m);
+            break;
+          }
+        }
+      }
+      p.hi = red_end;
+      // blue wing: c - 1 .. 0; ends at a pixel at or below WCON (not added) or
+      // after the first value below the cutoff (added)
+      long blue_lo = c;
+      if (c > 0 && L.wl >= grow[0]) {
+        blue_lo = 0;
+        for (long i0 = c - 1; i0 >= 0; i0 -= 64) {
+          const long i = i0 - lane;
+          int s = 0;  // 1: stop before i, 2: stop after i
+          if (i >= 0) {
+            const double wave = grow[i];
+            if (p.wcon > 0.0 && wave <= p.wcon) s = 1;
+            else if (f19_he_value(p, wave, L.wl, tb) < crow[i] * cutoff) s = 2;
+          }
+          const unsigned long long m = __ballot(s != 0);
+          if (m) {
+            const int first = __builtin_ctzll(m);
+            const int how = __shfl(s, first);
+            blue_lo = i0 - first + (how == 1 ? 1 : 0);
+            break;
+          }
+        }
+      }
+      p.lo = blue_lo;
+    }
   }
   if (lane == 0) pairs[(long)r * D + d] = p;
 }
@@ -3791,6 +3879,9 @@ __global__ __launch_bounds__(256) void k_f19_gather(const F19Line* __restrict__ 
       if (w == p.cpix) sum += p.kappa0;
       const long o = w - L.c;
       if ((o > 0 && o <= p.red) || (o < 0 && -o <= p.blue)) sum += f19_auto_value(L, p.kappa0, wave);
+    } else if (L.kind == 3) {
+      if ((w >= p.cpix && w < p.hi && !(p.wcon > 0.0 && wave <= p.wcon)) || (w < p.cpix && w >= p.lo))
+        sum += f19_he_value(p, wave, L.wl, tb);
     } else {
       if (w >= p.lo && w < p.hi && !(wave < L.wl)) {
         double v = p.kappa0;
@@ -3850,7 +3941,10 @@ int synthe_fort19_lines(synthe_ctx* ctx, const synthe_f19lines* fl, const synthe
   for (long r = 0; r < N; ++r) {
     F19Line& L = S->lines[r];
     L.kind = fl->kind[r];
-    if (L.kind < 0 || L.kind > 2) return fail(SYNTHE_EINVAL, std::string(who) + ": bad record kind");
+    if (L.kind < 0 || L.kind > 3) return fail(SYNTHE_EINVAL, std::string(who) + ": bad record kind");
+    if (L.kind == 3 && (!fd->doppler || fl->center[r] < 0))
+      return fail(SYNTHE_EINVAL, std::string(who) + ": a helium line needs DOPPLE (depths->doppler) and a centre on "
+                                                    "the grid");
     L.elem = fl->elem[r];
     L.ion = fl->ion[r];
     if (L.elem < 0 || L.elem >= fd->n_elem || L.ion < 1 || L.ion > fd->n_ion)

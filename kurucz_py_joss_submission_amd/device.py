@@ -16,7 +16,8 @@ import numpy as np
 from . import _native as N
 from .atmosphere import (AtmosphereModel, depth_doppler_scale, depth_hckt, depth_rho, depth_txnxn, depth_xne,
                          merge_energies)
-from .lines import LineCatalog, LineSoA, build_line_soa, fort19_lines, fort19_records, hydrogen_lines
+from .lines import (LineCatalog, LineSoA, build_line_soa, fort19_lines, fort19_records, helium_wing_lines,
+                    hydrogen_lines)
 
 
 def _torch():
@@ -167,7 +168,8 @@ class SpectrumPlan:
     def __init__(self, atm: AtmosphereModel, catalog: LineCatalog, wl_start: float, wl_end: float,
                  resolution: float, cutoff: float = 1e-3, microturb_kms: float = 0.0,
                  rhoxj: float = 0.0, grid: Optional[np.ndarray] = None, precision: str = "fp64",
-                 bfudge: Optional[np.ndarray] = None):
+                 bfudge: Optional[np.ndarray] = None, helium_wings: bool = True):
+        """helium_wings: the engine's helium pass (SynthesisConfig.enable_helium_wings)."""
         torch = _torch()
         self.ctx = N.context()
         if precision not in ("fp64", "fp32"):
@@ -188,12 +190,15 @@ class SpectrumPlan:
         self.hyd = hydrogen_lines(catalog, self.grid)
         self._hyd_args = None
         # fort.19 records (rgfall's routing) of the window, matched to the catalog
-        self.f19 = None
+        self.f19 = self.he = None
         if atm.population_per_ion is not None:
             pop_shape = np.asarray(atm.population_per_ion).shape
-            self.f19 = fort19_lines(catalog, fort19_records(catalog, wl_start, wl_end, resolution), self.grid,
-                                    pop_shape[1], pop_shape[2])
+            recs = fort19_records(catalog, wl_start, wl_end, resolution)
+            self.f19 = fort19_lines(catalog, recs, self.grid, pop_shape[1], pop_shape[2])
+            if helium_wings and atm.doppler_per_ion is not None and pop_shape[2] >= 2:
+                self.he = helium_wing_lines(catalog, self.grid, recs, pop_shape[1])
         self._f19_args = None
+        self._he_args = None
         f8 = torch.float64
         D, W, L = self.D, self.W, len(self.soa)
         dev = "cuda"
@@ -276,6 +281,7 @@ class SpectrumPlan:
             self.run_asynth(stream, stats, split=False)
             self.run_fort19(stream)
             self.run_hydrogen(stream)
+            self.run_helium(stream)
             self.split_scattering()
             return
         s = stream if stream is not None else N.stream_handle()
@@ -287,6 +293,7 @@ class SpectrumPlan:
             N.ptr(self.transp), N.ptr(self.valid), N.ptr(self.asynth), N.ptr(self.stats) if stats else None, s))
         self.run_fort19(stream)
         self.run_hydrogen(stream)
+        self.run_helium(stream)
         self.split_scattering()
 
     def run_fort19(self, stream=None):
@@ -313,6 +320,38 @@ class SpectrumPlan:
                 None, keep["vscale"].ctypes.data)
             self._f19_args = (keep, fl, fd)
         _, fl, fd = self._f19_args
+        s = stream if stream is not None else N.stream_handle()
+        N.check("synthe_fort19_lines", self.ctx.lib.synthe_fort19_lines(
+            self.ctx.handle, ct_byref(fl), ct_byref(fd), self.cutoff, self.W, N.ptr(self.d_grid),
+            N.ptr(self.kapmin), None, N.ptr(self.asynth), s))
+
+    def run_helium(self, stream=None):
+        """The engine's helium pass (engine/opacity.py:2534-2546, 3551-3590,
+        3651-3659): each He line's Voigt wings (the TYPE < -2 branch of
+        _apply_fort19_profile) summed per depth, times STIM, into the line
+        opacity after the hydrogen lines."""
+        h = self.he
+        a = self.atm
+        if h is None or len(h) == 0:
+            return
+        if self._he_args is None:
+            c = np.ascontiguousarray
+            pop = c(a.population_per_ion, np.float64)
+            dop = np.asarray(a.doppler_per_ion, np.float64)
+            emerge, emerge_h = merge_energies(a)
+            rho = np.ones(self.D) if a.mass_density is None else c(a.mass_density, np.float64)
+            dopple = c(np.stack([dop[:, h.ion[j] - 1, h.elem[j]] for j in range(len(h))]), np.float64)
+            keep = dict(pop=pop, rho=rho, xne=c(a.electron_density, np.float64), txnxn=c(depth_txnxn(a)),
+                        temp=c(a.temperature, np.float64), emerge=c(emerge), emerge_h=c(emerge_h),
+                        hckt=c(depth_hckt(a)), dopple=dopple)
+            fl = N.F19Lines(len(h), *(x.ctypes.data for x in (h.kind, h.wavelength, h.center, h.gf, h.gamma_rad,
+                                                             h.gamma_stark, h.gamma_vdw, h.elem, h.ion, h.nelion,
+                                                             h.nlast, h.cont_val, h.cat_wavelength, h.elo)))
+            fd = N.F19Depths(self.D, pop.shape[1], pop.shape[2], *(keep[k].ctypes.data for k in (
+                "pop", "rho", "xne", "txnxn", "temp", "emerge", "emerge_h")), None, keep["hckt"].ctypes.data,
+                keep["dopple"].ctypes.data, None)
+            self._he_args = (keep, fl, fd)
+        _, fl, fd = self._he_args
         s = stream if stream is not None else N.stream_handle()
         N.check("synthe_fort19_lines", self.ctx.lib.synthe_fort19_lines(
             self.ctx.handle, ct_byref(fl), ct_byref(fd), self.cutoff, self.W, N.ptr(self.d_grid),
@@ -372,6 +411,7 @@ class SpectrumPlan:
             self.asynth.zero_()
             self.run_fort19()
             self.run_hydrogen()
+            self.run_helium()
             self.split_scattering()
         self.run_rt()
 

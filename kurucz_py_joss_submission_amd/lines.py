@@ -942,3 +942,46 @@ def fort19_lines(catalog, records, grid: np.ndarray, n_ion: int, n_elem: int,
     i32, f8 = np.int32, np.float64
     types = dict(kind=i32, center=np.int64, elem=i32, ion=i32, nelion=i32, nlast=i32, cat_index=np.int64)
     return Fort19Lines(**{k: np.ascontiguousarray(np.asarray(v, types.get(k, f8))) for k, v in out.items()})
+
+
+def helium_wing_lines(catalog, grid: np.ndarray, records=None, n_ion: int = 6,
+                      contx: Optional[np.ndarray] = None) -> Fort19Lines:
+    """The lines of the engine's helium pass (engine/opacity.py:2534-2546 and
+    2686-3066), packed for synthe_fort19_lines as kind 3, in catalog order:
+    TYPE -3, -4 or -6 (the matched fort.19 record's TYPE where there is one),
+    centre (of the catalog wavelength) on the grid, ion stage within the
+    population table.  gf holds CGF = 0.026538/1.77245 gf / nu, nelion the
+    ion stage, nlast the TYPE, cont_val CONTX of the matched record's NCON and
+    NELIONX; records: fort19_records of the window (matched here)."""
+    catalog = as_line_catalog(catalog)
+    grid = np.asarray(grid, np.float64)
+    lt = np.asarray(catalog.line_types, np.int64).copy()
+    ncon = np.zeros(lt.size, np.int64)
+    nelionx = np.zeros(lt.size, np.int64)
+    if records is not None and len(records):
+        for ci, f in match_catalog_to_fort19(catalog.wavelength, records.wavelength_vacuum).items():
+            lt[ci] = int(records.line_type[f])
+            ncon[ci] = int(records.continuum_index[f])
+            nelionx[ci] = int(records.element_index[f])
+    wl = np.asarray(catalog.wavelength, np.float64)
+    z = np.asarray(catalog.atomic_number, np.int64)
+    st = np.asarray(catalog.ion_stages, np.int64)
+    centers = nearest_grid_indices(grid, wl) if wl.size else np.zeros(0, np.int64)
+    pick = np.isin(lt, (-3, -4, -6)) & (centers >= 0) & (centers < grid.size) & ~((z == 1) & (st == 1))
+    pick &= (st >= 1) & (st <= n_ion) & (z > 0)
+    idx = np.nonzero(pick)[0]
+    contx = _contx() if contx is None else np.asarray(contx, np.float64)
+    cv = np.zeros(idx.size)
+    for j, i in enumerate(idx.tolist()):
+        if ncon[i] > 0 and nelionx[i] > 0 and nelionx[i] <= contx.shape[1] and ncon[i] <= contx.shape[0]:
+            cv[j] = contx[ncon[i] - 1, nelionx[i] - 1]
+    w = wl[idx]
+    cgf = CGF_CONSTANT * np.asarray(catalog.gf, np.float64)[idx] / (C_LIGHT_NM / w)
+    c = np.ascontiguousarray
+    return Fort19Lines(kind=np.full(idx.size, 3, np.int32), wavelength=c(w), center=c(centers[idx], np.int64),
+                       gf=c(cgf), gamma_rad=c(np.asarray(catalog.gamma_rad, np.float64)[idx]),
+                       gamma_stark=c(np.asarray(catalog.gamma_stark, np.float64)[idx]),
+                       gamma_vdw=c(np.asarray(catalog.gamma_vdw, np.float64)[idx]), cat_wavelength=c(w),
+                       elo=c(np.asarray(catalog.excitation_energy, np.float64)[idx]),
+                       elem=c(z[idx] - 1, np.int32), ion=c(st[idx], np.int32), nelion=c(st[idx], np.int32),
+                       nlast=c(lt[idx], np.int32), cont_val=c(cv), cat_index=c(idx, np.int64))

@@ -305,3 +305,88 @@ void orc_fort19_asynth(long N, long D, long W, const double *grid, const int32_t
     free(profile);
     free(buf);
 }
+
+/* The engine's helium pass (engine/opacity.py:2534-2546, 2686-3066 with the
+ * TYPE < -2 branch of _apply_fort19_profile, :998-1053), one depth at a
+ * time: every helium line in catalog order adds its Voigt wings to
+ * HELIUM_WINGS; then line opacity += HELIUM_WINGS * STIM (:3651-3659).
+ * Per line: cgf (CGF of the catalog gf), its TYPE (-4: 3He), the matched
+ * fort.19 record's NCON / NELIONX (0 without one), the ion stage; per (line,
+ * depth): the population of that stage, its DOPPLE and FASTEX(E HCKT). */
+void orc_helium_wings(long N, long D, long W, const double *grid, const double *wl, const int64_t *center,
+                      const double *cgf, const double *gr, const double *gs, const double *gw,
+                      const int32_t *ltype, const int32_t *ion, const int32_t *ncon, const int32_t *nelionx,
+                      const double *contx, const double *pop_nd, const double *dopple_nd,
+                      const double *boltz_nd, const double *rho_d, const double *xne, const double *txnxn,
+                      const double *emerge, const double *emerge_h, const double *cont_all, const double *stim,
+                      double cutoff, const f19_tables *tb, double *out) {
+    double *wings = (double *)malloc(sizeof(double) * (size_t)W);
+    double *tmp = (double *)malloc(sizeof(double) * (size_t)W);
+    for (long d = 0; d < D; ++d) {
+        const double *cont = cont_all + d * W;
+        memset(wings, 0, sizeof(double) * (size_t)W);
+        for (long l = 0; l < N; ++l) {
+            long c = center[l];
+            if (c < 0 || c >= W) continue;
+            double pop_val = pop_nd[l * D + d], dop_val = dopple_nd[l * D + d];
+            if (pop_val <= 0.0 || dop_val <= 0.0) continue;
+            double rho = rho_d[d];
+            if (!(rho > 0.0)) continue;
+            double lw = wl[l];
+            double xnfdop = pop_val / (rho * dop_val);
+            double dw = dop_val * lw;
+            double kmin = cont[c] * cutoff;
+            double k0pre = cgf[l] * xnfdop;
+            if (k0pre < kmin || dw <= 0.0) continue;
+            double kappa0 = k0pre * boltz_nd[l * D + d];
+            if (kappa0 < kmin) continue;
+            double wcon = -1.0, wtail = -1.0;
+            continuum_limits(ncon[l], ion[l], nelionx[l], emerge[d], emerge_h[d], contx, &wcon, &wtail);
+            /* _apply_fort19_profile, TYPE < -2 */
+            memset(tmp, 0, sizeof(double) * (size_t)W);
+            double doppler = dw > 1e-12 ? dw : 1e-12;
+            double keff = kappa0;
+            if (ltype[l] == -4) {
+                keff /= 1.155;
+                doppler *= 1.155;
+            }
+            double dl = doppler / lw;
+            double damping = (gr[l] + gs[l] * xne[d] + gw[l] * txnxn[d]) / (dl > 1e-40 ? dl : 1e-40);
+            double adamp = damping > 1e-12 ? damping : 1e-12;
+            long cc = c;
+            double base = wcon > 0.0 ? wcon : lw;
+            if (lw <= grid[W - 1]) {
+                for (long i = cc; i < W; ++i) {
+                    double wave = grid[i];
+                    if (wcon > 0.0 && wave <= wcon) continue;
+                    double x = fabs(wave - lw) / doppler;
+                    double v = keff * orc_voigt(x, adamp, tb->h0, tb->h1, tb->h2, tb->nvoigt);
+                    if (wtail > 0.0 && wave < wtail) {
+                        double span = wtail - base;
+                        v = v * (wave - base) / (span > 1e-12 ? span : 1e-12);
+                    }
+                    if (v < cont[i] * cutoff) break;
+                    tmp[i] += v;
+                }
+            }
+            if (cc > 0 && lw >= grid[0]) {
+                for (long i = cc - 1; i >= 0; --i) {
+                    double wave = grid[i];
+                    if (wcon > 0.0 && wave <= wcon) break;
+                    double x = fabs(wave - lw) / doppler;
+                    double v = keff * orc_voigt(x, adamp, tb->h0, tb->h1, tb->h2, tb->nvoigt);
+                    if (wtail > 0.0 && wave < wtail) {
+                        double span = wtail - base;
+                        v = v * (wave - base) / (span > 1e-12 ? span : 1e-12);
+                    }
+                    tmp[i] += v;
+                    if (v < cont[i] * cutoff) break;
+                }
+            }
+            for (long i = 0; i < W; ++i) wings[i] += tmp[i];
+        }
+        for (long i = 0; i < W; ++i) out[d * W + i] += wings[i] * stim[d * W + i];
+    }
+    free(tmp);
+    free(wings);
+}

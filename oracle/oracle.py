@@ -577,6 +577,79 @@ def fort19_raw(grid, rec: dict, depth: dict, cont, stim, asynth_arr, cutoff=1e-3
     return N
 
 
+def helium_asynth(cat, atm, grid, cont, stim, out, wl_start, wl_end, resolution, cutoff=1e-3):
+    """The engine's helium pass (engine/opacity.py:2534-2546, 2686-3066,
+    3651-3659): each He line's Voigt wings, summed per depth, times STIM,
+    added to out [D, W] in place.  Returns the number of lines taken."""
+    import catalog as C_
+
+    t = tables()
+    pop, dop = getattr(atm, "population_per_ion", None), getattr(atm, "doppler_per_ion", None)
+    if pop is None or dop is None:
+        return 0
+    pop, dop = np.asarray(pop, np.float64), np.asarray(dop, np.float64)
+    f19 = C_.fort19_build(cat, wl_start, wl_end, resolution)
+    match = C_.match_fort19(np.asarray(cat.wavelength, np.float64), np.array([r.wavelength for r in f19]))
+    centers = nearest_grid_indices(grid, np.ascontiguousarray(cat.wavelength, np.float64))
+    D, W = atm.layers, grid.size
+    hk = hckt(atm)
+    cols = {k: [] for k in ("wl", "center", "cgf", "gr", "gs", "gw", "ltype", "ion", "ncon", "nelionx")}
+    popn, dopn, bz = [], [], []
+    for i, rec in enumerate(cat.records):
+        m = match.get(i)
+        lt = f19[m].line_type if m is not None else int(rec.line_type)
+        if lt not in (-3, -4, -6):
+            continue
+        sym = str(rec.element).strip().upper()
+        if sym in {"H", "H I", "HI"} and rec.ion_stage == 1:
+            continue
+        c = int(centers[i])
+        if c < 0 or c >= W:
+            continue
+        z = C_._ENGINE_Z.get(sym.replace(" ", ""))
+        if z is None or z - 1 >= pop.shape[2] or rec.ion_stage > pop.shape[1]:
+            continue
+        lw = float(cat.wavelength[i])
+        e = float(cat.excitation[i])
+        for k, v in (("wl", lw), ("center", c), ("cgf", C_.CGF_FACTOR * float(cat.gf[i]) / (C_.C_NM / lw)),
+                     ("gr", float(cat.gamma_rad[i])), ("gs", float(cat.gamma_stark[i])),
+                     ("gw", float(cat.gamma_vdw[i])), ("ltype", lt), ("ion", rec.ion_stage),
+                     ("ncon", f19[m].ncon if m is not None else 0),
+                     ("nelionx", f19[m].nelionx if m is not None else 0)):
+            cols[k].append(v)
+        popn.append(pop[:, rec.ion_stage - 1, z - 1])
+        dopn.append(dop[:, rec.ion_stage - 1, z - 1])
+        bz.append([lib().orc_fastex(ct.c_double(e * hk[d]), p(t.extab), p(t.extabf), ct.c_int(1001))
+                   for d in range(D)])
+    N = len(cols["wl"])
+    if N == 0:
+        return 0
+    f8 = {k: np.ascontiguousarray(cols[k], np.float64) for k in ("wl", "cgf", "gr", "gs", "gw")}
+    i32 = {k: np.ascontiguousarray(cols[k], np.int32) for k in ("ltype", "ion", "ncon", "nelionx")}
+    center = np.ascontiguousarray(cols["center"], np.int64)
+    ne = np.maximum(_xne(atm), 1e-40)
+    nmerge = np.maximum(1600.0 / np.power(ne, 2.0 / 15.0) - 1.5, 1.0)
+    emerge = 109737.312 / np.maximum(nmerge ** 2, 1e-12)
+    emerge_h = 109677.576 / np.maximum(nmerge ** 2, 1e-12)
+    rho = getattr(atm, "mass_density", None)
+    rho = np.ones(D) if rho is None else _c(rho, np.float64)
+    with np.load(_HERE.parent / "kurucz_py_joss_submission_amd" / "data" / "kurucz_tables.npz",
+                 allow_pickle=False) as z:
+        contx = np.ascontiguousarray(z["contx"], np.float64)
+    tb = _F19Tables(p(t.h0), p(t.h1), p(t.h2), 2001, p(t.extab), p(t.extabf), 1001)
+    res = _c(out, np.float64)
+    lib().orc_helium_wings(
+        ct.c_long(N), ct.c_long(D), ct.c_long(W), p(_c(grid, np.float64)), p(f8["wl"]), p(center, _I64),
+        p(f8["cgf"]), p(f8["gr"]), p(f8["gs"]), p(f8["gw"]), p(i32["ltype"], _I32), p(i32["ion"], _I32),
+        p(i32["ncon"], _I32), p(i32["nelionx"], _I32), p(contx), p(_c(np.array(popn), np.float64)),
+        p(_c(np.array(dopn), np.float64)), p(_c(np.array(bz), np.float64)), p(rho), p(_xne(atm)), p(txnxn(atm)),
+        p(emerge), p(emerge_h), p(_c(cont, np.float64)), p(_c(stim, np.float64)), ct.c_double(cutoff),
+        ct.byref(tb), p(res))
+    if res is not out:
+        out[...] = res
+    return N
+
+
 @dataclass
 class OracleResult:
     wavelength: np.ndarray
@@ -590,7 +663,8 @@ class OracleResult:
 
 
 def synthesize(atm, lines, wl_start, wl_end, resolution, cutoff=1e-3, microturb_kms=0.0,
-               rhoxj=0.0, keep_stages=True, threads: Optional[int] = None) -> OracleResult:
+               rhoxj=0.0, keep_stages=True, threads: Optional[int] = None,
+               helium_wings: bool = True) -> OracleResult:
     """Full hot path on the CPU (engine/opacity.py:2084-3861 stage order).
 
     lines: the oracle's own catalog (catalog.Catalog, from catalog.read_gfall)
@@ -627,6 +701,9 @@ def synthesize(atm, lines, wl_start, wl_end, resolution, cutoff=1e-3, microturb_
                                                cutoff, microturb_kms))
     ahl, st["h_profile_evals"] = hydrogen_lines(lines, atm, grid, kapmin, stim, cutoff, microturb_kms)
     asy = asy + ahl  # engine/opacity.py:2483-2487: AHLINE joins ASYNTH before the FSCAT split
+    # engine/opacity.py:3551-3590, 3651-3659: the helium pass, HELIUM_WINGS * STIM, after the hydrogen lines
+    st["helium_lines"] = float(helium_asynth(lines, atm, grid, kapmin, stim, asy, wl_start, wl_end, resolution,
+                                             cutoff) if helium_wings else 0)
     tm["asynth"] = time.perf_counter() - t0
     rhox = atm.depth
     fscat = np.exp(-rhox / rhoxj) if rhoxj > 0.0 else np.zeros_like(rhox)
