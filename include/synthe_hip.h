@@ -204,18 +204,23 @@ int synthe_hydrogen_lines(synthe_ctx *ctx, const synthe_hlines *lines, const syn
  * continuum limits): the records rgfall routes to fort.19
  * (io/lines/fort19.py:233-420) that are normal lines with NBLO/NBUP,
  * autoionising lines (Shore profiles) or merged-continuum ramps, in fort.19
- * order.  All host arrays. */
+ * order.  Kind 3 is the engine's helium pass instead (engine/opacity.py
+ * :2534-2546, 2686-3066, 3551-3590, 3651-3659, with the TYPE < -2 branch of
+ * _apply_fort19_profile, :998-1053): the catalog's He lines (TYPE -3/-4/-6),
+ * each a Voigt profile walked to the cutoff, the sum per pixel times STIM;
+ * a call takes either fort.19 records or helium lines, as the engine adds
+ * them at different points.  All host arrays. */
 typedef struct {
   int64_t n_lines;
-  const int32_t *kind;      /* [n] 0 normal, 1 autoionising, 2 merged continuum */
+  const int32_t *kind;      /* [n] 0 normal, 1 autoionising, 2 merged continuum, 3 helium line */
   const double *wavelength; /* [n] the record's vacuum wavelength, nm */
   const int64_t *center;    /* [n] nearest grid index of that wavelength, >= -1 and < n_wl */
-  const double *gf;         /* [n] the record's float32 gf (CGF for a normal line), as double */
+  const double *gf;         /* [n] the record's float32 gf (CGF for a normal line), as double; CGF of a helium line */
   const double *gamma_rad, *gamma_stark, *gamma_vdw; /* [n] the record's float32 damping, as double */
   const int32_t *elem;      /* [n] Z - 1 of the matched catalog line */
   const int32_t *ion;       /* [n] its ion stage (1-based) */
-  const int32_t *nelion;    /* [n] the record's NELION */
-  const int32_t *nlast;     /* [n] the record's TYPE (NLAST of a merged continuum) */
+  const int32_t *nelion;    /* [n] the record's NELION (a helium line's ion stage) */
+  const int32_t *nlast;     /* [n] the record's TYPE (NLAST of a merged continuum; -4 is 3He) */
   const double *cont_val;   /* [n] CONTX(NCON, NELIONX), 0 where the record has no WCON */
   const double *cat_wavelength; /* [n] matched catalog line's wavelength (NULL if depths->doppler given) */
   const double *elo;        /* [n] its lower level energy, cm^-1 (NULL if depths->boltz given) */
@@ -228,7 +233,8 @@ typedef struct {
   const double *emerge, *emerge_h; /* [D] the engine's merge energies (engine/opacity.py:2369-2372) */
   const double *boltz;      /* [n][D] FASTEX(E_low HCKT) of each record's catalog line, or NULL: from elo, hckt */
   const double *hckt;       /* [D] (used when boltz is NULL) */
-  const double *doppler;    /* [n][D] Doppler width, nm, of each record's catalog line, or NULL */
+  const double *doppler;    /* [n][D] Doppler width, nm, of each record's catalog line, or NULL;
+                               for a helium line DOPPLE (v/c) of its stage, required */
   const double *vscale;     /* [D] Doppler width / catalog wavelength (used when doppler is NULL) */
 } synthe_f19depths;
 
