@@ -362,6 +362,20 @@ int synthe_batch_run(synthe_batch *b, synthe_stream stream);
  * size.  Any pointer may be NULL. */
 int synthe_batch_info(const synthe_batch *b, int *pixels_per_lane, int *gather_blocks, double *tune_ms,
                       int tune_cap, int *n_tuned, double *arena_bytes);
+/* Device arrays of instance i of a batch after synthe_batch_run, for the
+ * adjoint of a batched forward pass (synthe_lte_vjp and synthe_asynth_vjp take
+ * them per instance): the continuum [D][W] (absorption, scattering, their sum
+ * KAPMIN), the line source, ASYNTH (post-STIM, the line absorption of the
+ * LTE solve), the all-zero line scattering, and TRANSP / valid [L][D].  Valid
+ * until the next synthe_batch_run or synthe_batch_destroy.  No counterpart in
+ * the reference (which has no batches). */
+typedef struct {
+  double *cont_abs, *cont_scat, *kapmin, *slinec, *asynth, *line_scat, *transp;
+  uint8_t *valid;
+} synthe_batch_view;
+
+int synthe_batch_arrays(const synthe_batch *b, int i, synthe_batch_view *out);
+
 int synthe_batch_destroy(synthe_batch *b);
 
 /* ------------------------------------------------------------------ adjoint

@@ -29,6 +29,7 @@ SYMBOLS = (
     "synthe_kernel_times", "synthe_set_profile_precision", "synthe_batch_create", "synthe_batch_run",
     "synthe_batch_info", "synthe_batch_destroy", "synthe_asynth_vjp", "synthe_lte_vjp",
     "synthe_lte_spectrum_noalloc", "synthe_transp_asynth", "synthe_hydrogen_lines", "synthe_fort19_lines",
+    "synthe_batch_arrays",
 )
 
 SYNTHE_BATCH_AUTOTUNE = 1
@@ -106,6 +107,12 @@ class Instance(ct.Structure):
                 ("cutoff", ct.c_double), ("flux", _VP), ("cont", _VP), ("iters", _VP)]
 
 
+class BatchView(ct.Structure):
+    """synthe_batch_view: device arrays of one batch instance."""
+    _fields_ = [("cont_abs", _VP), ("cont_scat", _VP), ("kapmin", _VP), ("slinec", _VP), ("asynth", _VP),
+                ("line_scat", _VP), ("transp", _VP), ("valid", _VP)]
+
+
 _lib = None
 _lib_lock = threading.Lock()
 
@@ -155,6 +162,7 @@ def load_library(path: Optional[Path] = None):
         lib.synthe_batch_info.argtypes = [_VP, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int), _VP, ct.c_int,
                                           ct.POINTER(ct.c_int), ct.POINTER(ct.c_double)]
         lib.synthe_batch_destroy.argtypes = [_VP]
+        lib.synthe_batch_arrays.argtypes = [_VP, ct.c_int, ct.POINTER(BatchView)]
         lib.synthe_asynth_vjp.argtypes = [_VP, ct.POINTER(Lines), ct.POINTER(Depths), ct.c_long, _VP, _VP, _VP,
                                           _VP, ct.c_double, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]
         lib.synthe_lte_vjp.argtypes = [_VP, ct.c_long, ct.c_long] + [_VP] * 15

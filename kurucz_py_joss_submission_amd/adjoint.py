@@ -188,6 +188,21 @@ class LayerModel:
         return dict(pop=pop, dop=dop, rho=rho, xne=ne, txnxn=txnxn, hckt=hckt, cabs=cabs, cscat=cscat)
 
 
+def continuum_lagrange(grid: np.ndarray):
+    """Edge interval e [W] and Lagrange weights [W][3] of k_continuum's
+    interpolation in the synthetic bundle's edge table: log10 kappa at a pixel
+    is sum_j w_j coeff[e][j]."""
+    x = np.asarray(grid, np.float64)
+    half = 0.5 * (S.WLEDGE[:-1] + S.WLEDGE[1:])
+    e = np.clip(np.searchsorted(np.abs(S.WLEDGE), np.abs(x), side="right") - 1, 0, S.WLEDGE.size - 2)
+    left, right = np.abs(S.WLEDGE[e]), np.abs(S.WLEDGE[e + 1])
+    h, dl = half[e], 0.5 * (S.WLEDGE[1:] - S.WLEDGE[:-1])[e] ** 2
+    dl = np.where(dl == 0.0, 1e-20, dl)
+    cw = np.stack([(x - h) * (x - right) / dl, (left - x) * (x - right) * 2.0 / dl,
+                   (x - left) * (x - h) / dl], axis=1)
+    return e, cw
+
+
 class SpectrumModel:
     """flux(T, dlog_abund, log_gf, log_gvdw) for one column, line list and window."""
 
@@ -233,13 +248,7 @@ class SpectrumModel:
         self.half = to_device(half)
         self.delta = to_device(0.5 * (S.WLEDGE[1:] - S.WLEDGE[:-1]) ** 2)
         # Lagrange weights of the continuum interpolation (k_continuum) per pixel
-        x = self.grid
-        e = np.clip(np.searchsorted(np.abs(S.WLEDGE), np.abs(x), side="right") - 1, 0, S.WLEDGE.size - 2)
-        left, right = np.abs(S.WLEDGE[e]), np.abs(S.WLEDGE[e + 1])
-        h, dl = half[e], 0.5 * (S.WLEDGE[1:] - S.WLEDGE[:-1])[e] ** 2
-        dl = np.where(dl == 0.0, 1e-20, dl)
-        cw = np.stack([(x - h) * (x - right) / dl, (left - x) * (x - right) * 2.0 / dl,
-                       (x - left) * (x - h) / dl], axis=1)
+        e, cw = continuum_lagrange(self.grid)
         self.cont_w = torch.tensor(cw, dtype=torch.float64, device="cuda")
         # pixels of one edge interval are contiguous (the grid ascends): (interval, first, end)
         cuts = np.flatnonzero(np.diff(e)) + 1

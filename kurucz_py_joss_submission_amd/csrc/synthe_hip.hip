@@ -4430,6 +4430,21 @@ int synthe_batch_info(const synthe_batch* b, int* pixels_per_lane, int* gather_b
   return SYNTHE_OK;
 }
 
+int synthe_batch_arrays(const synthe_batch* b, int i, synthe_batch_view* out) {
+  if (!b || !out) return fail(SYNTHE_EINVAL, "synthe_batch_arrays: NULL argument");
+  if (i < 0 || i >= (int)b->inst.size()) return fail(SYNTHE_EINVAL, "synthe_batch_arrays: no such instance");
+  const BatchInst& bi = b->inst[i];
+  out->cont_abs = (double*)(b->arena + bi.o_cabs);
+  out->cont_scat = (double*)(b->arena + bi.o_cscat);
+  out->kapmin = (double*)(b->arena + bi.o_kapmin);
+  out->slinec = (double*)(b->arena + bi.o_slinec);
+  out->asynth = (double*)(b->arena + bi.o_asynth);
+  out->transp = (double*)(b->arena + bi.o_transp);
+  out->valid = (uint8_t*)(b->arena + bi.o_valid);
+  out->line_scat = b->zeros;
+  return SYNTHE_OK;
+}
+
 int synthe_batch_destroy(synthe_batch* b) {
   if (b) {
     (void)hipDeviceSynchronize();

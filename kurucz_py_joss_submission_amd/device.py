@@ -512,6 +512,16 @@ class BatchPlan:
         return dict(pixels_per_lane=kwp.value, tile=64 * kwp.value, gather_workgroups=blocks.value,
                     autotune_ms=tune[:nt.value].tolist(), arena_bytes=arena.value)
 
+    def arrays(self, i: int) -> "N.BatchView":
+        """Device pointers of instance i's forward arrays after run() (continuum,
+        KAPMIN, line source, ASYNTH, zero line scattering, TRANSP, valid), for
+        the per-instance adjoint entry points."""
+        import ctypes
+
+        v = N.BatchView()
+        N.check("synthe_batch_arrays", self.ctx.lib.synthe_batch_arrays(self.handle, int(i), ctypes.byref(v)))
+        return v
+
     def results(self) -> List[dict]:
         return [dict(wavelength=it["grid"].copy(), flux=it["flux"].cpu().numpy(),
                      continuum=it["cont"].cpu().numpy()) for it in self.items]

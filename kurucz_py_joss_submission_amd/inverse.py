@@ -147,6 +147,23 @@ class BatchResponse:
         self.sr = responses[0].sr
         self.zidx = responses[0].zidx
 
+    def arrays(self, delta) -> Dict[str, object]:
+        """The same closed forms as write(), as new tensors (differentiable in
+        delta [n][20]): pop [n][Dmax][ion][Z], rho and txnxn [n][Dmax],
+        coeff_scat [n][Dmax][E-1][3]."""
+        torch = _torch()
+        ab = self.ab0 * torch.pow(10.0, delta)
+        asum = ab.sum(dim=1, keepdim=True)
+        nz = (ab / asum)[:, None, :] * self.nnuc[:, :, None]
+        popz = nz[:, :, None, :] * self.unit                        # [n][D][ion][20]
+        mu = (ab * self.mass[None, :]).sum(dim=1, keepdim=True) / asum
+        r = self.nnuc * mu * AMU
+        nh1 = nz[:, :, 0] * self.fr_h1
+        he1 = nz[:, :, 1] * self.fr_he1
+        sig = (self.ne_st[:, :, None, None] + nh1[:, :, None, None] * self.sr[None, None]) / r[:, :, None, None]
+        return dict(popz=popz, rho=r, txnxn=(nh1 + 0.42 * he1) * self.tfac,
+                    coeff_scat=torch.log10(torch.clamp(sig, min=1e-30)))
+
     def write(self, delta, pop, rho, txnxn, coeff_scat) -> None:
         torch = _torch()
         delta = torch.as_tensor(delta, dtype=torch.float64, device=self.ab0.device).reshape(self.n, -1)
@@ -281,17 +298,123 @@ class InverseProblem:
         # cores would dominate relative residuals)
         self.f_scale = [float(np.median(t.flux)) for t in self.targets]
         self.evaluations = 0
+        self.cutoff = float(cutoff)
+        self.last_delta = None
 
     def evaluate(self, delta: np.ndarray) -> List[object]:
         """Fluxes (torch, F_lambda) of every spectrum at offsets delta[n][20];
         self.valid gets each spectrum's pixels regular in target and model."""
         torch = _torch()
+        self.last_delta = np.array(delta, dtype=np.float64, copy=True)
         self.batch_resp.write(delta, self.pop, self.rho, self.txnxn, self.coeff_scat)
         self.plan.run()
         self.evaluations += 1
         torch.cuda.synchronize()
         self.valid = [(it["iters"] < 1000) & tr for it, tr in zip(self.plan.items, self.t_regular)]
         return [it["flux"].clone() for it in self.plan.items]
+
+    def _vjp_setup(self):
+        """Per-spectrum constants of the adjoint chain, built once: the
+        continuum interpolation as a [W][3(E-1)] matrix and each spectrum's
+        lines grouped by species of the abundance elements (one-hot rows)."""
+        from .adjoint import continuum_lagrange
+
+        torch = _torch()
+        f8 = torch.float64
+        if getattr(self, "_geo", None) is not None:
+            return
+        zpos = {z - 1: k for k, z in enumerate(S.ABUNDANCE_Z)}
+        E3 = 3 * (S.WLEDGE.size - 1)
+        self._geo = []
+        for it in self.plan.items:
+            ln, grid = it["lines"], it["grid"]
+            e, cw = continuum_lagrange(grid)
+            M = np.zeros((grid.size, E3))
+            for j in range(3):
+                M[np.arange(grid.size), 3 * e + j] = cw[:, j]
+            soa = ln.soa
+            live = (soa.process != 0) & (soa.elem >= 0)
+            ion = np.where(live, soa.ion - 1, 0)
+            el = np.where(live, soa.elem, 0)
+            key = np.where(live, ion * S.N_ELEMENTS + el, -1)
+            species = [k for k in np.unique(key) if k >= 0 and (k % S.N_ELEMENTS) in zpos]
+            onehot = np.zeros((len(species), key.size))
+            for r, k in enumerate(species):
+                onehot[r, key == k] = 1.0
+            t = lambda a, dt=f8: torch.tensor(np.ascontiguousarray(a), dtype=dt, device="cuda")  # noqa: E731
+            self._geo.append(dict(
+                M=t(M), onehot=t(onehot), sp_ion=t([k // S.N_ELEMENTS for k in species], torch.int64),
+                sp_z=t([zpos[k % S.N_ELEMENTS] for k in species], torch.int64), live=t(live, torch.bool),
+                ion=t(ion, torch.int64), el=t(el, torch.int64)))
+
+    def gradient(self, res, masks) -> np.ndarray:
+        """dL/d delta [n][20] of L = 1/2 sum_i sum_w (m_i r_i)^2 at the offsets of
+        the last evaluate(), by the batched adjoint: for every spectrum the RT
+        and line-opacity adjoints (synthe_lte_vjp, synthe_asynth_vjp) run on the
+        batch's own forward arrays (synthe_batch_arrays), one spectrum after
+        another on the current stream (the RT adjoint's kernels keep ~23 KB of
+        private memory per lane: several streams of them at once exhaust the
+        runtime's scratch); their outputs are contracted to
+        dL/d(populations, rho, TXNXN) and dL/d(continuum scattering), and one
+        autograd pass through the closed-form bundle response
+        (BatchResponse.arrays) takes them to the offsets.  Branches (KAPMIN
+        gates, MAXSTEP, table indices, label 401) are held at their forward
+        values, as in SpectrumModel."""
+        import ctypes
+
+        from . import _native as N
+        from .adjoint import C_NM
+
+        torch = _torch()
+        f8 = torch.float64
+        self._vjp_setup()
+        lib, h = self.plan.ctx.lib, self.plan.ctx.handle
+        st = N.stream_handle()
+        p = N.ptr
+        dvar = torch.tensor(self.last_delta, dtype=f8, device="cuda", requires_grad=True)
+        arr = self.batch_resp.arrays(dvar)
+        E3 = 3 * (S.WLEDGE.size - 1)
+        terms = []
+        for i, it in enumerate(self.plan.items):
+            geo = self._geo[i]
+            dp, ln = it["depth"], it["lines"]
+            D, W, L = dp.n, it["grid"].size, len(ln.soa)
+            v = self.plan.arrays(i)
+            wl = it["d_grid"]
+            g_flam = (res[i] * masks[i].to(f8)) / self.f_scale[i]
+            vnu = (g_flam * (C_NM / torch.clamp(wl * wl, min=1e-40))).contiguous()
+            g_ca, g_cs, g_la, g_b, g_src = (torch.empty((D, W), dtype=f8, device="cuda") for _ in range(5))
+            N.check("synthe_lte_vjp", lib.synthe_lte_vjp(
+                h, D, W, p(wl), p(dp.temperature), p(dp.rhox), v.cont_abs, v.cont_scat, v.asynth, v.line_scat,
+                v.slinec, p(vnu), p(g_ca), p(g_cs), p(g_la), p(g_b), p(g_src), st))
+            # continuum scattering: d/d coeff of 10^(sum_j w_j coeff_j), through a surrogate
+            lcs = arr["coeff_scat"][i, :D].reshape(D, E3) @ geo["M"].T
+            cs = torch.pow(10.0, torch.clamp(lcs, max=38.0))
+            term = (g_cs * cs).sum()
+            if L > 0:
+                s_k0, s_a, s_dv, k0 = (torch.empty((L, D), dtype=f8, device="cuda") for _ in range(4))
+                ls, ds = ln.struct(), dp.struct()
+                N.check("synthe_asynth_vjp", lib.synthe_asynth_vjp(
+                    h, ctypes.byref(ls), ctypes.byref(ds), W, p(wl), v.transp, v.valid, v.kapmin, self.cutoff,
+                    p(ln.corder), p(ln.cstart), p(ln.worder), p(g_la), p(s_k0), p(s_a), p(s_dv), p(k0), st))
+                live, ion, el = geo["live"], geo["ion"], geo["el"]
+                pp = dp.pop[:, ion, el].T                       # [L][D]
+                dv = dp.dop[:, ion, el].T
+                gk = torch.where(live[:, None], k0 * s_k0, torch.zeros_like(s_k0))  # dL/d ln KAPPA0
+                gpp = torch.where(pp > 0, gk / torch.where(pp > 0, pp, torch.ones_like(pp)),
+                                  torch.zeros_like(gk))
+                # KAPPA0 ~ pop / rho; the damping a ~ gamma_W TXNXN / DOPPLE
+                g_rho = -(gk / dp.rho[None, :]).sum(dim=0)
+                safe_dv = torch.where(dv > 0, dv, torch.ones_like(dv))
+                g_tx = torch.where(live[:, None] & (dv > 0), ln.t["gamma_vdw"][:, None] / safe_dv * s_a,
+                                   torch.zeros_like(s_a)).sum(dim=0)
+                g_sp = geo["onehot"] @ gpp                      # per species of the abundance elements
+                popz = arr["popz"][i, :D][:, geo["sp_ion"], geo["sp_z"]]  # [D][species]
+                term = term + (g_sp.T * popz).sum() + (g_rho * arr["rho"][i, :D]).sum() \
+                    + (g_tx * arr["txnxn"][i, :D]).sum()
+            terms.append(term)
+        g = torch.autograd.grad(torch.stack(terms).sum(), dvar)[0]
+        return g.detach().cpu().numpy()
 
     def residuals(self, flux) -> List[object]:
         """Full residual vectors (F - F_target) / median(F_target); the
@@ -308,13 +431,21 @@ class InverseProblem:
 
     def solve(self, max_iter: int = 12, step: float = 1e-3, tol: float = 1e-6, rcond: float = 1e-6,
               cfloor: float = 1e-3, line_floor: float = 2e-3, max_tries: int = 6, max_step: float = 0.5,
-              bound: float = 2.0) -> dict:
+              bound: float = 2.0, gradient: str = "differences", jac_every: Optional[int] = None) -> dict:
         """Gauss-Newton from zero offsets, with Levenberg damping on failure.
 
         Per iteration: a forward-difference Jacobian (one batched evaluation
         per abundance, over the pixels valid at the point and at every step)
-        and, per spectrum, the SVD of its columns; the step is
-        -V diag(s / (s^2 + mu s_0^2)) U^T r (mu = 0 first: plain Gauss-Newton).
+        and, per spectrum, the SVD J = U S V^T of its columns; the step is
+        -V diag(1 / (s^2 + mu s_0^2)) V^T g with g = J^T r (mu = 0 first:
+        plain Gauss-Newton).  gradient="adjoint" takes g from the batched
+        adjoint (gradient()) at every point instead, starts from the
+        Gauss-Newton Hessian J^T J of the difference Jacobian and updates it
+        by BFGS from each accepted step and the change of the adjoint
+        gradient, and recomputes the difference Jacobian only after a
+        spectrum's trials all failed (or every jac_every iterations when
+        given): an iteration then costs the batched forward and adjoint of the
+        trial point rather than 20 forwards.
         A spectrum keeps its step if the misfit over the pixels valid at both
         points drops (mu / 10), else
         mu = max(10 mu, 1e-6) and a new trial with the same Jacobian.
@@ -325,6 +456,8 @@ class InverseProblem:
         the spectrum only through sum(A) and the mean molecular weight, when
         theirs clears cfloor times the largest.  A step is shortened to
         max_step dex, and offsets stay within +-bound."""
+        if gradient not in ("differences", "adjoint"):
+            raise ValueError("gradient must be 'differences' or 'adjoint'")
         torch = _torch()
         n, nz = self.n, self.nz
         delta = np.zeros((n, nz))
@@ -333,51 +466,91 @@ class InverseProblem:
         valid = list(self.valid)
         cost = self._costs(res, valid)
         history = [float(cost.sum())]
+        per_target = [cost.copy()]
+        refreshes = 0
         mu = np.zeros(n)
         active = cost > 0
         sens = np.zeros((n, nz))
+        svd = [None] * n
+        hess = [None] * n   # adjoint mode: Hessian estimates of the live columns
+        last = [None] * n   # adjoint mode: (accepted step, gradient before it)
+        common = list(valid)
+        refresh = True
+        adjoint_calls = 0
         it = 0
         for it in range(1, max_iter + 1):
-            # Jacobian columns over the pixels valid at delta and at every step
-            common = [v.clone() for v in valid]
-            cols = []
-            for k in range(nz):
-                if k == self.h_index:
-                    cols.append(None)
-                    continue
-                dk = delta.copy()
-                dk[:, k] += step
-                rk = self.residuals(self.evaluate(dk))
+            if refresh or gradient == "differences" or (jac_every and (it - 1) % jac_every == 0):
+                # Jacobian columns over the pixels valid at delta and at every step
+                common = [v.clone() for v in valid]
+                cols = []
+                for k in range(nz):
+                    if k == self.h_index:
+                        cols.append(None)
+                        continue
+                    dk = delta.copy()
+                    dk[:, k] += step
+                    rk = self.residuals(self.evaluate(dk))
+                    for i in range(n):
+                        common[i] &= self.valid[i]
+                    cols.append([(rk[i] - res[i]) / step for i in range(n)])
+                refreshes += 1
+                svd = []
                 for i in range(n):
-                    common[i] &= self.valid[i]
-                cols.append([(rk[i] - res[i]) / step for i in range(n)])
-            svd = []
-            for i in range(n):
-                m = common[i].to(torch.float64)
-                idx = [k for k in range(nz) if cols[k] is not None]
-                J = torch.stack([cols[k][i] for k in idx], dim=1) * m[:, None]  # [W][nz - 1]
-                c = torch.linalg.vector_norm(J, dim=0)
-                cn = c.cpu().numpy()
-                live = (cn > cfloor * cn.max()) | ((cn > line_floor) & self.line_mask[idx])
-                sens[i, idx] = cn
-                idx = [k for k, lv in zip(idx, live) if lv]
-                if not idx:
-                    svd.append(None)
-                    continue
-                u, s_, vh = torch.linalg.svd(J[:, torch.tensor(np.nonzero(live)[0], device=J.device)],
-                                             full_matrices=False)
-                svd.append((idx, s_.cpu().numpy(), vh.T.cpu().numpy(), (u.T @ (res[i] * m)).cpu().numpy()))
+                    m = common[i].to(torch.float64)
+                    idx = [k for k in range(nz) if cols[k] is not None]
+                    J = torch.stack([cols[k][i] for k in idx], dim=1) * m[:, None]  # [W][nz - 1]
+                    c = torch.linalg.vector_norm(J, dim=0)
+                    cn = c.cpu().numpy()
+                    live = (cn > cfloor * cn.max()) | ((cn > line_floor) & self.line_mask[idx])
+                    sens[i, idx] = cn
+                    idx = [k for k, lv in zip(idx, live) if lv]
+                    if not idx:
+                        svd.append(None)
+                        continue
+                    u, s_, vh = torch.linalg.svd(J[:, torch.tensor(np.nonzero(live)[0], device=J.device)],
+                                                 full_matrices=False)
+                    s_np = s_.cpu().numpy()
+                    svd.append((idx, s_np, vh.T.cpu().numpy(), s_np * (u.T @ (res[i] * m)).cpu().numpy()))
+                    # Gauss-Newton Hessian J^T J of the live columns, the start of the quasi-Newton updates
+                    V_np = vh.T.cpu().numpy()
+                    hess[i] = (V_np * s_np ** 2) @ V_np.T
+                    last[i] = None
+                refresh = False
+            if gradient == "adjoint":
+                if self.last_delta is None or not np.array_equal(self.last_delta, delta):
+                    res = self.residuals(self.evaluate(delta))
+                    valid = list(self.valid)
+                g = self.gradient(res, valid)
+                adjoint_calls += 1
+                for i in range(n):
+                    if svd[i] is None:
+                        continue
+                    idx = svd[i][0]
+                    gi = g[i, idx]
+                    if last[i] is not None:  # BFGS update of the Hessian from the accepted step
+                        sk, gk = last[i]
+                        yk = gi - gk
+                        sy = float(sk @ yk)
+                        Hs = hess[i] @ sk
+                        sHs = float(sk @ Hs)
+                        if sy > 1e-12 * np.linalg.norm(sk) * np.linalg.norm(yk) and sHs > 0:
+                            hess[i] = hess[i] - np.outer(Hs, Hs) / sHs + np.outer(yk, yk) / sy
+                    lam, Q = np.linalg.eigh(0.5 * (hess[i] + hess[i].T))
+                    lam = np.maximum(lam, 0.0)[::-1]
+                    Q = Q[:, ::-1]
+                    svd[i] = (idx, np.sqrt(lam), Q, Q.T @ gi)
+                    last[i] = (np.zeros(len(idx)), gi)
             improved = np.zeros(n, bool)
             for _ in range(max_tries):
                 dstep = np.zeros((n, nz))
                 for i in np.nonzero(active & ~improved)[0]:
                     if svd[i] is None:
                         continue
-                    idx, s_, V, Ur = svd[i]
+                    idx, s_, V, Vg = svd[i]
                     keep = s_ > rcond * s_[0]
                     f = np.zeros_like(s_)
-                    f[keep] = s_[keep] / (s_[keep] ** 2 + mu[i] * s_[0] ** 2)
-                    dstep[i, idx] = -(V @ (f * Ur))
+                    f[keep] = 1.0 / (s_[keep] ** 2 + mu[i] * s_[0] ** 2)
+                    dstep[i, idx] = -(V @ (f * Vg))
                     big = float(np.abs(dstep[i]).max())
                     if big > max_step:
                         dstep[i] *= max_step / big
@@ -385,13 +558,16 @@ class InverseProblem:
                 dstep = trial - delta
                 rt = self.residuals(self.evaluate(trial))
                 vt = self.valid
-                both = [cm & v for cm, v in zip(common, vt)]  # compare on pixels valid at both points
+                base = common if gradient == "differences" else valid
+                both = [cm & v for cm, v in zip(base, vt)]  # compare on pixels valid at both points
                 ct = self._costs(rt, both)
                 c0 = self._costs(res, both)
                 cnew = self._costs(rt, vt)
                 for i in np.nonzero(active & ~improved)[0]:
                     if ct[i] < c0[i]:
                         improved[i] = True
+                        if gradient == "adjoint" and svd[i] is not None and last[i] is not None:
+                            last[i] = (dstep[i, svd[i][0]].copy(), last[i][1])
                         delta[i] = trial[i]
                         res[i] = rt[i]
                         valid[i] = vt[i]
@@ -405,13 +581,17 @@ class InverseProblem:
                     break
             # a spectrum no trial improves gets a fresh Jacobian at the raised damping, until the
             # damping leaves no step worth taking
+            if (active & ~improved).any():
+                refresh = True
             active &= improved | (mu < 1e6)
             history.append(float(cost.sum()))
+            per_target.append(cost.copy())
             if not active.any():
                 break
         elapsed = time.perf_counter() - t0
         return dict(delta=delta, iterations=it, seconds=elapsed, evaluations=self.evaluations,
-                    cost_history=history, cost=cost.copy(), sensitivity=sens)
+                    adjoint_evaluations=adjoint_calls, jacobians=refreshes, cost_history=history,
+                    cost_per_target=np.array(per_target), cost=cost.copy(), sensitivity=sens, gradient=gradient)
 
 
 def report(targets: Sequence[Target], delta: np.ndarray, sensitivity: np.ndarray,

@@ -10,7 +10,8 @@ median flux).  Checked: the total misfit falls by five orders of magnitude and
 every target whose starting misfit exceeds the numerical floor (1e-9) by two;
 the constrained line elements come back (median error below 1e-3 dex); and the
 FP64 plain path, run independently at the recovered offsets, reproduces the
-targets to 1e-5 on the fitted pixels.
+targets to 1e-5 on the fitted pixels; the batched adjoint's gradient of the
+misfit equals the per-spectrum differentiable model's.
 """
 
 from __future__ import annotations
@@ -40,4 +41,23 @@ def test_c5_inverse_problem_full_size(gpu):
     check = INV.fp64_plain_check(targets, sol["delta"], indices=list(range(4)) + order[:4].tolist())
     print("FP64 plain path at the recovered offsets:", check)
     assert check < 1e-5
+    # the batched adjoint at full size, at the recovered offsets: every spectrum's gradient of the
+    # misfit against the per-spectrum differentiable model, on two spectra
+    import torch
+
+    from kurucz_py_joss_submission_amd.adjoint import SpectrumModel
+
+    res = prob.residuals(prob.evaluate(sol["delta"]))
+    masks = list(prob.valid)
+    g = prob.gradient(res, masks)
+    for i in (int(order[0]), int(np.argmax(sol["sensitivity"].sum(axis=1)))):
+        t = targets[i]
+        m = SpectrumModel(t.cols, t.catalog, t.wl_start, t.wl_end, precision="fp64")
+        T, _, lgf, lgw = m.base_parameters()
+        da = torch.tensor(sol["delta"][i], dtype=torch.float64, device="cuda", requires_grad=True)
+        r = masks[i].to(torch.float64) * (m(T, da, lgf, lgw) - prob.f_target[i]) / prob.f_scale[i]
+        (0.5 * (r * r).sum()).backward()
+        ref = da.grad.cpu().numpy()
+        print("batched adjoint vs model, spectrum", i, np.abs(g[i] - ref).max() / np.abs(ref).max())
+        assert np.abs(g[i] - ref).max() <= 1e-6 * np.abs(ref).max()
     prob.plan.close()
