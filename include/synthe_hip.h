@@ -413,6 +413,20 @@ int synthe_lte_vjp(synthe_ctx *ctx, long n_depths, long n_wl, const double *wl, 
                    double *g_cont_scat, double *g_line_abs, double *g_planck, double *g_line_source,
                    synthe_stream stream);
 
+/* synthe_lte_vjp of n spectra in one launch per stage: each stage's lanes
+ * run over every spectrum's wavelengths (a global index; each lane finds its
+ * spectrum), so many small spectra fill the GPU together; per spectrum the
+ * results are those of synthe_lte_vjp.  All arrays device pointers, as
+ * synthe_lte_vjp takes them.  No counterpart in the reference (it has no
+ * gradients or batches). */
+typedef struct {
+  long n_depths, n_wl;
+  const double *wl, *temperature, *rhox, *cont_abs, *cont_scat, *line_abs, *line_scat, *line_source, *v_flux;
+  double *g_cont_abs, *g_cont_scat, *g_line_abs, *g_planck, *g_line_source;
+} synthe_vjp_instance;
+
+int synthe_lte_vjp_batch(synthe_ctx *ctx, int n, const synthe_vjp_instance *inst, synthe_stream stream);
+
 #ifdef __cplusplus
 }
 #endif

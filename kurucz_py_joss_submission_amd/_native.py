@@ -29,7 +29,7 @@ SYMBOLS = (
     "synthe_kernel_times", "synthe_set_profile_precision", "synthe_batch_create", "synthe_batch_run",
     "synthe_batch_info", "synthe_batch_destroy", "synthe_asynth_vjp", "synthe_lte_vjp",
     "synthe_lte_spectrum_noalloc", "synthe_transp_asynth", "synthe_hydrogen_lines", "synthe_fort19_lines",
-    "synthe_batch_arrays",
+    "synthe_batch_arrays", "synthe_lte_vjp_batch",
 )
 
 SYNTHE_BATCH_AUTOTUNE = 1
@@ -113,6 +113,14 @@ class BatchView(ct.Structure):
                 ("line_scat", _VP), ("transp", _VP), ("valid", _VP)]
 
 
+class VjpInstance(ct.Structure):
+    """synthe_vjp_instance: one spectrum of a batched RT vjp (device pointers)."""
+    _fields_ = [("n_depths", ct.c_long), ("n_wl", ct.c_long)] + [
+        (k, _VP) for k in ("wl", "temperature", "rhox", "cont_abs", "cont_scat", "line_abs", "line_scat",
+                           "line_source", "v_flux", "g_cont_abs", "g_cont_scat", "g_line_abs", "g_planck",
+                           "g_line_source")]
+
+
 _lib = None
 _lib_lock = threading.Lock()
 
@@ -163,6 +171,7 @@ def load_library(path: Optional[Path] = None):
                                           ct.POINTER(ct.c_int), ct.POINTER(ct.c_double)]
         lib.synthe_batch_destroy.argtypes = [_VP]
         lib.synthe_batch_arrays.argtypes = [_VP, ct.c_int, ct.POINTER(BatchView)]
+        lib.synthe_lte_vjp_batch.argtypes = [_VP, ct.c_int, ct.POINTER(VjpInstance), _VP]
         lib.synthe_asynth_vjp.argtypes = [_VP, ct.POINTER(Lines), ct.POINTER(Depths), ct.c_long, _VP, _VP, _VP,
                                           _VP, ct.c_double, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]
         lib.synthe_lte_vjp.argtypes = [_VP, ct.c_long, ct.c_long] + [_VP] * 15

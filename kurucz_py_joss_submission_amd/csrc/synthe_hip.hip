@@ -2238,27 +2238,61 @@ __device__ int vjp_depths(int D, long W, long w, const double* __restrict__ rhox
 // (or none) first, label-401 ones (TAUNU(0) = abtot rho at the top above
 // XTAU(51)) last, so that a wave runs one branch.  Only the order depends on
 // the atomics; each wavelength's result does not.
-__global__ void k_vjp_order(int D, long W, const double* __restrict__ rhox, const double* __restrict__ ca,
-                            const double* __restrict__ cs, const double* __restrict__ la,
-                            const double* __restrict__ ls, const double* __restrict__ v, const double* xtau,
-                            int* __restrict__ counts, int32_t* __restrict__ order) {
-  const long w = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (w >= W) return;
+// One spectrum of an RT vjp (device pointers): D depths, W wavelengths, its
+// first slot g0 in the batch's global wavelength index, inputs and outputs.
+struct VjpArgs {
+  int D;
+  long W, g0;
+  const double *wl, *temp, *rhox, *ca, *cs, *la, *ls, *src, *v;
+  double *g_ca, *g_cs, *g_la, *g_b, *g_src;
+};
+
+// The spectrum holding global slot g: the last i with A[i].g0 <= g.
+__device__ __forceinline__ int vjp_inst(const VjpArgs* __restrict__ A, int n, long g) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (A[mid].g0 <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// XTAU-branch wavelengths first, label-401 ones last (counts[0], counts[1]).
+__device__ __forceinline__ void vjp_order_at(const VjpArgs& a, long w, const double* xtau, int* __restrict__ counts,
+                                             int32_t* __restrict__ order) {
+  const int D = a.D;
+  const long W = a.W;
   bool b401 = false;
-  if (v[w] != 0.0) {
+  if (a.v[w] != 0.0) {
     int idx[kAdjMaxDepth];
-    const int n = vjp_depths(D, W, w, rhox, ca, cs, la, ls, idx);
+    const int n = vjp_depths(D, W, w, a.rhox, a.ca, a.cs, a.la, a.ls, idx);
     if (n >= 4) {
       const long i = (long)idx[0] * W + w;
-      double ab = ca[i] + la[i] + cs[i] + ls[i];
+      double ab = a.ca[i] + a.la[i] + a.cs[i] + a.ls[i];
       ab = ab > kEps38 ? ab : kEps38;
-      b401 = ab * rhox[idx[0]] > xtau[adj::kXt - 1];
+      b401 = ab * a.rhox[idx[0]] > xtau[adj::kXt - 1];
     }
   }
   if (b401)
     order[W - 1 - atomicAdd(&counts[1], 1)] = (int32_t)w;
   else
     order[atomicAdd(&counts[0], 1)] = (int32_t)w;
+}
+
+__global__ void k_vjp_order(VjpArgs a, const double* xtau, int* __restrict__ counts, int32_t* __restrict__ order) {
+  const long w = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (w < a.W) vjp_order_at(a, w, xtau, counts, order);
+}
+
+// Every spectrum of a batch (counts [2n], order [G] in each spectrum's slots).
+__global__ void k_vjp_order_batch(const VjpArgs* __restrict__ A, int n, long G, const double* xtau,
+                                  int* __restrict__ counts, int32_t* __restrict__ order) {
+  const long g = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (g >= G) return;
+  const int i = vjp_inst(A, n, g);
+  const VjpArgs a = A[i];
+  vjp_order_at(a, g - a.g0, xtau, counts + 2 * i, order + a.g0);
 }
 
 // The vjp of one wavelength in stages, each with the parallelism it has (the
@@ -2301,29 +2335,21 @@ __device__ __forceinline__ void vjp_load(const VjpRecord& R, VjpColumn& col, Vjp
   }
 }
 
-__global__ __launch_bounds__(kVjpBlock) void k_vjp_solve(int D, long W, const int32_t* __restrict__ order,
-                                                         const double* __restrict__ wl,
-                                                         const double* __restrict__ temp,
-                                                         const double* __restrict__ rhox,
-                                                         const double* __restrict__ ca, const double* __restrict__ cs,
-                                                         const double* __restrict__ la, const double* __restrict__ ls,
-                                                         const double* __restrict__ src, const double* __restrict__ v,
-                                                         DeviceTables tb, VjpRecord* __restrict__ recs) {
-  __shared__ float s_cj[adj::kXt * adj::kXt];
-  for (int i = threadIdx.x; i < adj::kXt * adj::kXt; i += kVjpBlock) s_cj[i] = tb.coefj32[i];
-  __syncthreads();  // the only barrier: lanes leave independently below
-  const long t = blockIdx.x * (long)kVjpBlock + threadIdx.x;
-  if (t >= W) return;
-  VjpRecord& R = recs[t];
+// The column of one wavelength, its FP64 solve and (XTAU branch) the fixed
+// point's adjoint, into record R; t indexes the spectrum's order.
+__device__ __forceinline__ void vjp_solve_at(const VjpArgs& a, long t, const int32_t* __restrict__ order,
+                                             const DeviceTables& tb, const float* s_cj, VjpRecord& R) {
+  const int D = a.D;
+  const long W = a.W;
   const long w = order[t];
-  const double vw = v[w];
+  const double vw = a.v[w];
   int idx[kAdjMaxDepth];
-  const int n = vw != 0.0 ? vjp_depths(D, W, w, rhox, ca, cs, la, ls, idx) : 0;
+  const int n = vw != 0.0 ? vjp_depths(D, W, w, a.rhox, a.ca, a.cs, a.la, a.ls, idx) : 0;
   R.flag = -1;
   R.n = n;
   R.vw = vw;
   if (vw == 0.0 || n < 4) return;
-  const double wv = wl[w];
+  const double wv = a.wl[w];
   const double freq = kCLightNm / (wv > 1e-12 ? wv : 1e-12);
   const double f3 = pow(freq / 1.0e15, 3.0);
   VjpColumn col;
@@ -2332,19 +2358,19 @@ __global__ __launch_bounds__(kVjpBlock) void k_vjp_solve(int D, long W, const in
   for (int j = 0; j < n; ++j) {
     const int d = idx[j];
     const long i = (long)d * W + w;
-    const double B = planck_one(freq, f3, temp[d]);
+    const double B = planck_one(freq, f3, a.temp[d]);
     double S = B;
-    if (src) {
-      const double s0 = src[i];
+    if (a.src) {
+      const double s0 = a.src[i];
       S = isfinite(s0) ? s0 : B;
     }
-    const double ac = ca[i], al = la[i], sc = cs[i], sgl = ls[i];
+    const double ac = a.ca[i], al = a.la[i], sc = a.cs[i], sgl = a.ls[i];
     double ab = ac + al + sc + sgl;
     ab = ab > kEps38 ? ab : kEps38;
     double alpha = ab > 0.0 ? (sc + sgl) / ab : 0.0;
     alpha = alpha < 0.0 ? 0.0 : (alpha > 1.0 ? 1.0 : alpha);
     const double dn = ac + al;
-    col.rho[j] = rhox[d];
+    col.rho[j] = a.rhox[d];
     col.abtot[j] = ab;
     col.alpha[j] = alpha;
     col.snubar[j] = dn > 0 ? (ac * B + al * S) / dn : B;
@@ -2382,6 +2408,30 @@ __global__ __launch_bounds__(kVjpBlock) void k_vjp_solve(int D, long W, const in
     }
   }
   R.flag = took401;
+}
+
+__device__ __forceinline__ void vjp_stage_coefj(const DeviceTables& tb, float* s_cj) {
+  for (int i = threadIdx.x; i < adj::kXt * adj::kXt; i += kVjpBlock) s_cj[i] = tb.coefj32[i];
+  __syncthreads();  // the only barrier: lanes leave independently afterwards
+}
+
+__global__ __launch_bounds__(kVjpBlock) void k_vjp_solve(VjpArgs a, const int32_t* __restrict__ order,
+                                                         DeviceTables tb, VjpRecord* __restrict__ recs) {
+  __shared__ float s_cj[adj::kXt * adj::kXt];
+  vjp_stage_coefj(tb, s_cj);
+  const long t = blockIdx.x * (long)kVjpBlock + threadIdx.x;
+  if (t < a.W) vjp_solve_at(a, t, order, tb, s_cj, recs[t]);
+}
+
+__global__ __launch_bounds__(kVjpBlock) void k_vjp_solve_batch(const VjpArgs* __restrict__ A, int n, long G,
+                                                               const int32_t* __restrict__ order, DeviceTables tb,
+                                                               VjpRecord* __restrict__ recs) {
+  __shared__ float s_cj[adj::kXt * adj::kXt];
+  vjp_stage_coefj(tb, s_cj);
+  const long g = blockIdx.x * (long)kVjpBlock + threadIdx.x;
+  if (g >= G) return;
+  const VjpArgs a = A[vjp_inst(A, n, g)];
+  vjp_solve_at(a, g - a.g0, order + a.g0, tb, s_cj, recs[g]);
 }
 
 __global__ __launch_bounds__(64) void k_vjp_pass(long W, DeviceTables tb, VjpRecord* __restrict__ recs) {
@@ -2469,18 +2519,13 @@ __global__ __launch_bounds__(64) void k_vjp_tau401(long W, VjpRecord* __restrict
   for (int i = r; i < n; i += 5) R.g_tau[i] = gt[i];
 }
 
-__global__ __launch_bounds__(64) void k_vjp_finish(int D, long W, const int32_t* __restrict__ order,
-                                                   const double* __restrict__ wl, const double* __restrict__ temp,
-                                                   const double* __restrict__ ca, const double* __restrict__ la,
-                                                   const double* __restrict__ src,
-                                                   const VjpRecord* __restrict__ recs, double* __restrict__ g_ca,
-                                                   double* __restrict__ g_cs, double* __restrict__ g_la,
-                                                   double* __restrict__ g_b, double* __restrict__ g_src) {
-  const long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (u >= W * 4) return;
-  const long t = u / 4;
-  const int r = (int)(u % 4);
-  const VjpRecord& R = recs[t];
+// INTEG/PARCOE and the chain to the outputs of one wavelength for the depths
+// j = r (mod 4); t indexes the spectrum's order.
+__device__ __forceinline__ void vjp_finish_at(const VjpArgs& a, long t, int r, const int32_t* __restrict__ order,
+                                              const VjpRecord& R) {
+  const long W = a.W;
+  const double *wl = a.wl, *temp = a.temp, *ca = a.ca, *la = a.la, *src = a.src;
+  double *g_ca = a.g_ca, *g_cs = a.g_cs, *g_la = a.g_la, *g_b = a.g_b, *g_src = a.g_src;
   if (R.flag < 0) return;
   VjpColumn col;
   VjpWork wk;
@@ -2529,6 +2574,23 @@ __global__ __launch_bounds__(64) void k_vjp_finish(int D, long W, const int32_t*
       g_b[i] = gB + gS;
     }
   }
+}
+
+__global__ __launch_bounds__(64) void k_vjp_finish(VjpArgs a, const int32_t* __restrict__ order,
+                                                   const VjpRecord* __restrict__ recs) {
+  const long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (u >= a.W * 4) return;
+  vjp_finish_at(a, u / 4, (int)(u % 4), order, recs[u / 4]);
+}
+
+__global__ __launch_bounds__(64) void k_vjp_finish_batch(const VjpArgs* __restrict__ A, int n, long G,
+                                                         const int32_t* __restrict__ order,
+                                                         const VjpRecord* __restrict__ recs) {
+  const long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (u >= G * 4) return;
+  const long g = u / 4;
+  const VjpArgs a = A[vjp_inst(A, n, g)];
+  vjp_finish_at(a, g - a.g0, (int)(u % 4), order + a.g0, recs[g]);
 }
 
 // ------------------------------------------------------------ ASYNTH pieces
@@ -4546,12 +4608,12 @@ int synthe_lte_vjp(synthe_ctx* ctx, long n_depths, long n_wl, const double* wl, 
   VjpRecord* recs = (VjpRecord*)((char*)ctx->ws + o_rec);
   hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, 2, counts);
   HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(k_vjp_order, dim3(blocks_for(n_wl, 256)), dim3(256), 0, st, (int)n_depths, n_wl, rhox,
-                     cont_abs, cont_scat, line_abs, line_scat, v_flux, ctx->tb.xtau, counts, order);
+  const VjpArgs a{(int)n_depths, n_wl, 0, wl, temperature, rhox, cont_abs, cont_scat, line_abs, line_scat,
+                  line_source, v_flux, g_cont_abs, g_cont_scat, g_line_abs, g_planck, g_line_source};
+  hipLaunchKernelGGL(k_vjp_order, dim3(blocks_for(n_wl, 256)), dim3(256), 0, st, a, ctx->tb.xtau, counts, order);
   HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(k_vjp_solve, dim3(blocks_for(n_wl, kVjpBlock)), dim3(kVjpBlock), 0, st, (int)n_depths,
-                     n_wl, order, wl, temperature, rhox, cont_abs, cont_scat, line_abs, line_scat, line_source,
-                     v_flux, ctx->tb, recs);
+  hipLaunchKernelGGL(k_vjp_solve, dim3(blocks_for(n_wl, kVjpBlock)), dim3(kVjpBlock), 0, st, a, order, ctx->tb,
+                     recs);
   HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(k_vjp_pass, dim3(blocks_for(n_wl * 5, 64)), dim3(64), 0, st, n_wl, ctx->tb, recs);
   HIP_TRY(hipGetLastError());
@@ -4559,9 +4621,66 @@ int synthe_lte_vjp(synthe_ctx* ctx, long n_depths, long n_wl, const double* wl, 
   HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(k_vjp_tau401, dim3(blocks_for(n_wl * 5, 64)), dim3(64), 0, st, n_wl, recs);
   HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(k_vjp_finish, dim3(blocks_for(n_wl * 4, 64)), dim3(64), 0, st, (int)n_depths, n_wl, order,
-                     wl, temperature, cont_abs, line_abs, line_source, recs, g_cont_abs, g_cont_scat,
-                     g_line_abs, g_planck, g_line_source);
+  hipLaunchKernelGGL(k_vjp_finish, dim3(blocks_for(n_wl * 4, 64)), dim3(64), 0, st, a, order, recs);
+  HIP_TRY(hipGetLastError());
+  return SYNTHE_OK;
+}
+
+int synthe_lte_vjp_batch(synthe_ctx* ctx, int n, const synthe_vjp_instance* inst, synthe_stream stream) {
+  const char* who = "synthe_lte_vjp_batch";
+  if (!ctx || (n > 0 && !inst)) return fail(SYNTHE_EINVAL, std::string(who) + ": NULL argument");
+  if (n < 0) return fail(SYNTHE_EINVAL, std::string(who) + ": bad count");
+  if (n == 0) return SYNTHE_OK;
+  hipStream_t st = (hipStream_t)stream;
+  std::vector<VjpArgs> args(n);
+  long G = 0;
+  for (int i = 0; i < n; ++i) {
+    const synthe_vjp_instance& q = inst[i];
+    if (q.n_depths <= 0 || q.n_wl <= 0) return fail(SYNTHE_EINVAL, std::string(who) + ": bad sizes");
+    if (q.n_depths > kAdjMaxDepth)
+      return fail(SYNTHE_ERANGE, std::string(who) + ": more than " + std::to_string(kAdjMaxDepth) + " depths");
+    if (!q.wl || !q.temperature || !q.rhox || !q.cont_abs || !q.cont_scat || !q.line_abs || !q.line_scat ||
+        !q.v_flux)
+      return fail(SYNTHE_EINVAL, std::string(who) + ": NULL input");
+    args[i] = VjpArgs{(int)q.n_depths, q.n_wl, G, q.wl, q.temperature, q.rhox, q.cont_abs, q.cont_scat,
+                      q.line_abs, q.line_scat, q.line_source, q.v_flux, q.g_cont_abs, q.g_cont_scat, q.g_line_abs,
+                      q.g_planck, q.g_line_source};
+    G += q.n_wl;
+    if (G > 0x7fffffffL) return fail(SYNTHE_ERANGE, std::string(who) + ": more than 2^31 wavelengths");
+    const long dw = q.n_depths * q.n_wl;
+    for (double* z : {q.g_cont_abs, q.g_cont_scat, q.g_line_abs, q.g_planck, q.g_line_source}) {
+      if (!z) continue;
+      hipLaunchKernelGGL(k_fill_u64, dim3(blocks_for(dw, kBlock)), dim3(kBlock), 0, st, dw, (unsigned long long*)z);
+      HIP_TRY(hipGetLastError());
+    }
+  }
+  const size_t o_ord = align_up((size_t)n * sizeof(VjpArgs));
+  const size_t o_cnt = o_ord + align_up((size_t)G * sizeof(int32_t));
+  const size_t o_rec = o_cnt + align_up((size_t)2 * n * sizeof(int));
+  int rc = ensure_ws(ctx, o_rec + (size_t)G * sizeof(VjpRecord));
+  if (rc) return rc;
+  char* base = (char*)ctx->ws;
+  VjpArgs* d_args = (VjpArgs*)base;
+  int32_t* order = (int32_t*)(base + o_ord);
+  int* counts = (int*)(base + o_cnt);
+  VjpRecord* recs = (VjpRecord*)(base + o_rec);
+  HIP_TRY(hipMemcpyAsync(d_args, args.data(), (size_t)n * sizeof(VjpArgs), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipStreamSynchronize(st));  // args (host) is freed when this call returns
+  hipLaunchKernelGGL(k_zero_i32, dim3(blocks_for(2L * n, 64)), dim3(64), 0, st, 2 * n, counts);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_vjp_order_batch, dim3(blocks_for(G, 256)), dim3(256), 0, st, d_args, n, G, ctx->tb.xtau,
+                     counts, order);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_vjp_solve_batch, dim3(blocks_for(G, kVjpBlock)), dim3(kVjpBlock), 0, st, d_args, n, G, order,
+                     ctx->tb, recs);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_vjp_pass, dim3(blocks_for(G * 5, 64)), dim3(64), 0, st, G, ctx->tb, recs);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_vjp_adj401, dim3(blocks_for(G, 64)), dim3(64), 0, st, G, recs);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_vjp_tau401, dim3(blocks_for(G * 5, 64)), dim3(64), 0, st, G, recs);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_vjp_finish_batch, dim3(blocks_for(G * 4, 64)), dim3(64), 0, st, d_args, n, G, order, recs);
   HIP_TRY(hipGetLastError());
   return SYNTHE_OK;
 }

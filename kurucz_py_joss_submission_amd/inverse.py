@@ -315,8 +315,8 @@ class InverseProblem:
 
     def _vjp_setup(self):
         """Per-spectrum constants of the adjoint chain, built once: the
-        continuum interpolation as a [W][3(E-1)] matrix and each spectrum's
-        lines grouped by species of the abundance elements (one-hot rows)."""
+        continuum interpolation as a [3(E-1)][W] matrix and each spectrum's
+        lines ordered by species of the abundance elements (one slice each)."""
         from .adjoint import continuum_lagrange
 
         torch = _torch()
@@ -338,12 +338,16 @@ class InverseProblem:
             el = np.where(live, soa.elem, 0)
             key = np.where(live, ion * S.N_ELEMENTS + el, -1)
             species = [k for k in np.unique(key) if k >= 0 and (k % S.N_ELEMENTS) in zpos]
-            onehot = np.zeros((len(species), key.size))
+            # lines of those species ordered by species: each species one slice
+            sid = np.full(key.size, -1)
             for r, k in enumerate(species):
-                onehot[r, key == k] = 1.0
+                sid[key == k] = r
+            order = np.nonzero(sid >= 0)[0][np.argsort(sid[sid >= 0], kind="stable")]
+            bounds = np.searchsorted(sid[order], np.arange(len(species) + 1))
             t = lambda a, dt=f8: torch.tensor(np.ascontiguousarray(a), dtype=dt, device="cuda")  # noqa: E731
             self._geo.append(dict(
-                M=t(M), onehot=t(onehot), sp_ion=t([k // S.N_ELEMENTS for k in species], torch.int64),
+                Mt=t(M.T), sp_order=t(order, torch.int64), sp_slices=list(zip(bounds[:-1].tolist(), bounds[1:].tolist())),
+                sp_ion=t([k // S.N_ELEMENTS for k in species], torch.int64),
                 sp_z=t([zpos[k % S.N_ELEMENTS] for k in species], torch.int64), live=t(live, torch.bool),
                 ion=t(ion, torch.int64), el=t(el, torch.int64)))
 
@@ -351,10 +355,10 @@ class InverseProblem:
         """dL/d delta [n][20] of L = 1/2 sum_i sum_w (m_i r_i)^2 at the offsets of
         the last evaluate(), by the batched adjoint: for every spectrum the RT
         and line-opacity adjoints (synthe_lte_vjp, synthe_asynth_vjp) run on the
-        batch's own forward arrays (synthe_batch_arrays), one spectrum after
-        another on the current stream (the RT adjoint's kernels keep ~23 KB of
-        private memory per lane: several streams of them at once exhaust the
-        runtime's scratch); their outputs are contracted to
+        batch's own forward arrays (synthe_batch_arrays): the RT adjoints of all
+        spectra in one batched call (synthe_lte_vjp_batch, one launch per
+        stage over every spectrum's wavelengths), then each spectrum's line
+        adjoint; their outputs are contracted to
         dL/d(populations, rho, TXNXN) and dL/d(continuum scattering), and one
         autograd pass through the closed-form bundle response
         (BatchResponse.arrays) takes them to the offsets.  Branches (KAPMIN
@@ -374,21 +378,32 @@ class InverseProblem:
         dvar = torch.tensor(self.last_delta, dtype=f8, device="cuda", requires_grad=True)
         arr = self.batch_resp.arrays(dvar)
         E3 = 3 * (S.WLEDGE.size - 1)
+        # the RT adjoints of every spectrum in one batched call (one launch per stage)
+        items = self.plan.items
+        views = [self.plan.arrays(i) for i in range(len(items))]
+        outs = []
+        vi = (N.VjpInstance * len(items))()
+        for i, it in enumerate(items):
+            dp, wl = it["depth"], it["d_grid"]
+            D, W = dp.n, it["grid"].size
+            v = views[i]
+            g_flam = (res[i] * masks[i].to(f8)) / self.f_scale[i]
+            vnu = (g_flam * (C_NM / torch.clamp(wl * wl, min=1e-40))).contiguous()
+            g = [torch.empty((D, W), dtype=f8, device="cuda") for _ in range(5)]  # ca, cs, la, planck, source
+            outs.append((vnu, g))
+            vi[i] = N.VjpInstance(D, W, p(wl), p(dp.temperature), p(dp.rhox), v.cont_abs, v.cont_scat, v.asynth,
+                                  v.line_scat, v.slinec, p(vnu), *(p(x) for x in g))
+        N.check("synthe_lte_vjp_batch", lib.synthe_lte_vjp_batch(h, len(items), vi, st))
         terms = []
-        for i, it in enumerate(self.plan.items):
+        for i, it in enumerate(items):
             geo = self._geo[i]
             dp, ln = it["depth"], it["lines"]
             D, W, L = dp.n, it["grid"].size, len(ln.soa)
-            v = self.plan.arrays(i)
+            v = views[i]
             wl = it["d_grid"]
-            g_flam = (res[i] * masks[i].to(f8)) / self.f_scale[i]
-            vnu = (g_flam * (C_NM / torch.clamp(wl * wl, min=1e-40))).contiguous()
-            g_ca, g_cs, g_la, g_b, g_src = (torch.empty((D, W), dtype=f8, device="cuda") for _ in range(5))
-            N.check("synthe_lte_vjp", lib.synthe_lte_vjp(
-                h, D, W, p(wl), p(dp.temperature), p(dp.rhox), v.cont_abs, v.cont_scat, v.asynth, v.line_scat,
-                v.slinec, p(vnu), p(g_ca), p(g_cs), p(g_la), p(g_b), p(g_src), st))
+            _, (g_ca, g_cs, g_la, g_b, g_src) = outs[i]
             # continuum scattering: d/d coeff of 10^(sum_j w_j coeff_j), through a surrogate
-            lcs = arr["coeff_scat"][i, :D].reshape(D, E3) @ geo["M"].T
+            lcs = arr["coeff_scat"][i, :D].reshape(D, E3) @ geo["Mt"]
             cs = torch.pow(10.0, torch.clamp(lcs, max=38.0))
             term = (g_cs * cs).sum()
             if L > 0:
@@ -401,17 +416,18 @@ class InverseProblem:
                 pp = dp.pop[:, ion, el].T                       # [L][D]
                 dv = dp.dop[:, ion, el].T
                 gk = torch.where(live[:, None], k0 * s_k0, torch.zeros_like(s_k0))  # dL/d ln KAPPA0
-                gpp = torch.where(pp > 0, gk / torch.where(pp > 0, pp, torch.ones_like(pp)),
-                                  torch.zeros_like(gk))
+                gpp = torch.where(pp > 0, gk / torch.where(pp > 0, pp, torch.ones_like(pp)), torch.zeros_like(gk))
                 # KAPPA0 ~ pop / rho; the damping a ~ gamma_W TXNXN / DOPPLE
                 g_rho = -(gk / dp.rho[None, :]).sum(dim=0)
                 safe_dv = torch.where(dv > 0, dv, torch.ones_like(dv))
                 g_tx = torch.where(live[:, None] & (dv > 0), ln.t["gamma_vdw"][:, None] / safe_dv * s_a,
                                    torch.zeros_like(s_a)).sum(dim=0)
-                g_sp = geo["onehot"] @ gpp                      # per species of the abundance elements
-                popz = arr["popz"][i, :D][:, geo["sp_ion"], geo["sp_z"]]  # [D][species]
-                term = term + (g_sp.T * popz).sum() + (g_rho * arr["rho"][i, :D]).sum() \
-                    + (g_tx * arr["txnxn"][i, :D]).sum()
+                term = term + (g_rho * arr["rho"][i, :D]).sum() + (g_tx * arr["txnxn"][i, :D]).sum()
+                if geo["sp_slices"]:  # per species of the abundance elements
+                    gs = gpp.index_select(0, geo["sp_order"])
+                    g_sp = torch.stack([gs[a:b].sum(dim=0) for a, b in geo["sp_slices"]])  # [species][D]
+                    popz = arr["popz"][i, :D][:, geo["sp_ion"], geo["sp_z"]]              # [D][species]
+                    term = term + (g_sp.T * popz).sum()
             terms.append(term)
         g = torch.autograd.grad(torch.stack(terms).sum(), dvar)[0]
         return g.detach().cpu().numpy()

@@ -124,3 +124,47 @@ def test_lte_vjp_matches_cpu_reverse_mode(gpu, oracle, cpu_vjp):
         assert np.all(np.isfinite(got[q])), name
         assert err < 1e-9, (name, err)
     assert np.all(got[0][10, 7] == 0.0) and np.all(got[0][:, 5] == 0.0)
+
+
+@pytest.mark.gpu
+def test_lte_vjp_batch_equals_single_calls(gpu):
+    """synthe_lte_vjp_batch over spectra of different depth counts and
+    widths equals synthe_lte_vjp called on each, bit for bit (each lane runs
+    the same device body on its own spectrum's arrays)."""
+    import torch
+
+    from kurucz_py_joss_submission_amd import _native as N
+
+    ctx = N.context()
+    lib, h = ctx.lib, ctx.handle
+    st = N.stream_handle()
+    rng = np.random.default_rng(77)
+    dev = lambda a: torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda")  # noqa: E731
+    specs = []
+    for D, W in ((56, 96), (72, 300), (40, 17), (64, 1000)):
+        rhox, T, ca, cs, la, ls = _columns(D, W, rng)
+        src = None if W == 17 else np.abs(la) * 0 + 1e-3 * (1 + rng.random((D, W)))
+        specs.append(dict(D=D, W=W, wl=dev(np.linspace(500.0, 505.0, W)), T=dev(T), rhox=dev(rhox), ca=dev(ca),
+                          cs=dev(cs), la=dev(la), ls=dev(ls), src=None if src is None else dev(src),
+                          v=dev(rng.uniform(-1.0, 1.0, W))))
+    p = N.ptr
+    single = []
+    for sp in specs:
+        g = [torch.empty((sp["D"], sp["W"]), dtype=torch.float64, device="cuda") for _ in range(5)]
+        N.check("synthe_lte_vjp", lib.synthe_lte_vjp(h, sp["D"], sp["W"], p(sp["wl"]), p(sp["T"]), p(sp["rhox"]),
+                                                     p(sp["ca"]), p(sp["cs"]), p(sp["la"]), p(sp["ls"]), p(sp["src"]),
+                                                     p(sp["v"]), *(p(x) for x in g), st))
+        single.append([x.clone() for x in g])
+    vi = (N.VjpInstance * len(specs))()
+    batch = []
+    for i, sp in enumerate(specs):
+        g = [torch.full((sp["D"], sp["W"]), 7.0, dtype=torch.float64, device="cuda") for _ in range(5)]
+        batch.append(g)
+        vi[i] = N.VjpInstance(sp["D"], sp["W"], p(sp["wl"]), p(sp["T"]), p(sp["rhox"]), p(sp["ca"]), p(sp["cs"]),
+                              p(sp["la"]), p(sp["ls"]), p(sp["src"]), p(sp["v"]), *(p(x) for x in g))
+    N.check("synthe_lte_vjp_batch", lib.synthe_lte_vjp_batch(h, len(specs), vi, st))
+    torch.cuda.synchronize()
+    for a, b in zip(single, batch):
+        for x, y in zip(a, b):
+            assert torch.equal(torch.nan_to_num(x, nan=-1.0), torch.nan_to_num(y, nan=-1.0))
+    assert any(float(x.abs().max()) > 0 for x in single[1])
