@@ -492,7 +492,10 @@ def main() -> int:
     g_ms = float(np.mean(gather_ms)) if len(gather_ms) else float("nan")
     achieved = flops / (g_ms * 1e-3) / 1e12
     roof = {
-        "bound": "mfma",
+        # compute-bound on the FP64 vector ALU: the far-wing sums are not a
+        # dense contraction, so no matrix core is used; the peak is MI355X's
+        # FP64 vector rate (78.6 TFLOP/s, equal to its FP64 matrix rate)
+        "bound": "fp64-valu",
         "resource": "FP64 VALU (no dense contraction on this path; MI355X FP64 vector peak equals "
                     "its FP64 matrix peak, 78.6 TFLOP/s)",
         "kernel": "k_gather",
