@@ -887,14 +887,15 @@ __device__ __forceinline__ double far_quotient(double x, double b) {
   else return div_fast(x, b);
 }
 
-// waves per SIMD the gather's register budget is sized for: the FP64 form's
-// 24 expansion moments need ~100 VGPRs; the FP32 form fits more waves but
-// gains nothing from them (its tiles thrash L2 at 7 waves: 46 ms against 26)
+// waves per SIMD the gather's register budget is sized for: the tile and row
+// expansions (24 moments, their wave sums and Horner evaluation) need about
+// 130 VGPRs, so four waves (128 VGPRs each) spill little and five spill
+// heavily
 #ifndef SYNTHE_GATHER_WAVES
-#define SYNTHE_GATHER_WAVES 5
+#define SYNTHE_GATHER_WAVES 4
 #endif
 #ifndef SYNTHE_GATHER_WAVES32
-#define SYNTHE_GATHER_WAVES32 5
+#define SYNTHE_GATHER_WAVES32 4
 #endif
 template <int kPrec>
 constexpr int gather_waves() {
@@ -937,36 +938,113 @@ __device__ __forceinline__ bool far_interior(const WingHead& h, int t0i, int t1i
 // adds the line centres (catalog order, line_opacity.py:1091-1102), then the
 // far wings X / k^2 (N10DOP < |k| <= MAXSTEP, line_opacity.py:562-617) of
 // every list entry that reaches the tile, and multiplies by STIM
-// (line_opacity.py:1338).  One wave per (64 kWP pixel tile, depth), kWP
-// consecutive pixels per lane.  Two passes over the class windows, each
-// taking 64 entries per step with one coalesced load of their heads:
-//  1. entries whose far zone covers the tile at |K| >= 4.5 h add their
-//     expansion moments (one entry per lane); one wave reduction, then each
-//     pixel evaluates the expansion (Horner in u/h);
-//     Pass 1 also queues, in LDS, the list positions of the other entries
-//     that reach the tile (about one in seven visits), in the order met;
-//  2. the queued entries are taken one at a time, 64 heads per gathered load,
-//     each broadcast from the lane that loaded it: where the tile lies in
-//     one far wing every lane adds X / k^2 branch-free, its pixels sharing one
-//     reciprocal (far_chunk); on the tiles holding a wing's N10DOP or MAXSTEP
-//     edge, pixels outside the far zone add +0.  If the queue fills, pass 1
-//     notes where, and pass 2 rescans the class windows from that step on, so
-//     the order of the exact terms is the same either way.
-// Each pixel's sum has a fixed order (Voigt zones, centres, expansions, exact
-// far wings by class and centre), no atomics, so results are run-to-run
-// identical.
-// List positions (relative to the depth's lists) of one tile's exactly
-// evaluated entries: 7 KiB of LDS per wave, so the 20 waves a CU holds at
-// five per SIMD fit in its 160 KiB.
+// (line_opacity.py:1338).  One wave per (tile, depth); a tile is kWP rows of
+// 64 pixels and lane l holds pixel t0 + 64 r + l of row r, so a row is one
+// coalesced access and one pixel per lane.  The far wings are summed at three
+// granularities:
+//  1. tile: the class windows are walked 64 entries per step with one
+//     coalesced load of their heads; entries whose far zone covers the tile
+//     at |K| >= 4.5 h (K: tile centre - line centre, h: half the tile) add
+//     their expansion moments (one entry per lane), reduced over the wave and
+//     evaluated per pixel (Horner in u/h).  The other entries that reach the
+//     tile are queued in LDS (their list positions, in the order met);
+//  2. row: for each row, the queued entries whose far zone covers the row at
+//     |K_r| >= 4.5 h_r (h_r = 32) add the same expansion about the row
+//     centre;
+//  3. exact: each queued entry, broadcast to the wave, adds X / k^2 on the
+//     rows it reaches and could not expand, one quotient per pixel; on rows
+//     holding a wing's N10DOP or MAXSTEP edge, pixels outside the far zone
+//     add +0.
+// Steps 2 and 3 run when the queue is full (after the tile moments met so far
+// are added to the pixels) and at the end.  Each pixel's sum has a fixed
+// order that depends only on the lists, no atomics, so results are
+// run-to-run identical.
+// List positions (relative to the depth's lists) of one tile's queued
+// entries: 7 KiB of LDS per wave, so the 20 waves a CU holds at five per SIMD
+// fit in its 160 KiB.
 #ifndef SYNTHE_GATHER_QUEUE
 #define SYNTHE_GATHER_QUEUE 1792
 #endif
 constexpr int kGatherQueue = SYNTHE_GATHER_QUEUE;
+static_assert(kGatherQueue >= 64 && kGatherQueue % 64 == 0, "queue holds whole steps of 64 entries");
+
+// Tile expansion: adds (n + 1) * sum_lanes(mom[n]) t^n to every pixel of the
+// tile's row segment, with t = (pixel - tile centre) / h (exact: h is a power
+// of two).
+template <int kT, int kWP>
+__device__ __forceinline__ void add_tile_expansion(const double* mom, double* __restrict__ row, long t0, long W,
+                                                   int lane) {
+  constexpr double kCentre = 0.5 * (64 * kWP - 1), kInvH = 1.0 / (32.0 * kWP);
+  double coef[kT];
+#pragma unroll
+  for (int n = 0; n < kT; ++n) {
+    coef[n] = (double)(n + 1) * wave_sum_butterfly(mom[n]);
+    // one reduction at a time: interleaving all kT of them for latency
+    // costs more registers than the kernel's occupancy allows
+    __builtin_amdgcn_sched_barrier(0);
+  }
+#pragma unroll
+  for (int r = 0; r < kWP; ++r) {
+    const double t = ((double)(64 * r + lane) - kCentre) * kInvH;
+    double s = coef[kT - 1];
+#pragma unroll
+    for (int n = kT - 2; n >= 0; --n) s = __builtin_fma(s, t, coef[n]);
+    const long w = t0 + 64 * r + lane;
+    if (w < W) row[w] += s;
+  }
+}
+
+// Row expansion: the same about the centre of a row (h = 32), the value at
+// the lane's pixel of that row.
+template <int kT>
+__device__ __forceinline__ double row_expansion(const double* mom, int lane) {
+  const double t = ((double)lane - 31.5) * (1.0 / 32.0);
+  double s = (double)kT * wave_sum_butterfly(mom[kT - 1]);
+#pragma unroll
+  for (int n = kT - 2; n >= 0; --n) {
+    __builtin_amdgcn_sched_barrier(0);  // one reduction at a time (registers)
+    s = __builtin_fma(s, t, (double)(n + 1) * wave_sum_butterfly(mom[n]));
+  }
+  return s;
+}
+
+// The value of v in lane src, for every lane.
+__device__ __forceinline__ double bcast_double(double v, int src) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), src);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// One expansion step: with y = 1/K, adds X/K^2 (-h/K)^n to mom[n] (x = 0
+// adds nothing; K must be nonzero).
+template <int kT>
+__device__ __forceinline__ void expansion_moments(double x, double kq, double half, double* mom) {
+  double y = __builtin_amdgcn_rcp(kq);  // 1/K to an ulp: two Newton steps
+  y = __builtin_fma(y, __builtin_fma(-kq, y, 1.0), y);
+  y = __builtin_fma(y, __builtin_fma(-kq, y, 1.0), y);
+  double q = x * y * y;
+  const double rr = -half * y;
+  mom[0] += q;
+#pragma unroll
+  for (int n = 1; n < kT; ++n) {
+    q *= rr;
+    mom[n] += q;
+  }
+}
+
+// The row r0..r1 lies in the entry's far zone and its centre is at least
+// kmin_r from the line centre (the row-expansion test).
+__device__ __forceinline__ bool row_expands(const WingHead& h, int r0, int r1, double wr, double kmin_r) {
+  return far_interior(h, r0, r1) && fabs(wr - h.cd) >= kmin_r && h.xs >= 0.0;
+}
 
 template <bool kStats, int kPrec, int kWP>
 __device__ __forceinline__ void gather_tile(const GatherArgs& g, int d, long t0, int lane,
                                             unsigned long long* __restrict__ evals) {
   constexpr int kWT = 64 * kWP;
+  constexpr int kT = kPrec == kPrecFP32 ? kFarTerms32 : kFarTerms;
+  constexpr double kRatio = kPrec == kPrecFP32 ? kFarExpandRatio32 : kFarExpandRatio;
   const long W = g.W;
   const int D = g.D;
   const int64_t* __restrict__ corder = g.corder;
@@ -978,23 +1056,25 @@ __device__ __forceinline__ void gather_tile(const GatherArgs& g, int d, long t0,
   const double* __restrict__ grid = g.grid;
   const double* __restrict__ temp = g.temp;
   double* __restrict__ asynth = g.asynth;
-  const long w0 = t0 + (long)kWP * lane;
+  // Each pixel's running sum lives in its own element of the tile's asynth
+  // row (L2-resident), read and written back only by its lane, so no
+  // register holds the tile's sums across the steps below; pixels off the
+  // grid have no element and their sums are not formed.
   double* row = asynth + (long)d * W;
-  double acc[kWP];
-#pragma unroll
-  for (int p = 0; p < kWP; ++p) acc[p] = w0 + p < W ? row[w0 + p] : 0.0;
   unsigned long long nev = 0;
 #pragma unroll
-  for (int p = 0; p < kWP; ++p) {
-    long w = w0 + p;
+  for (int r = 0; r < kWP; ++r) {
+    const long w = t0 + 64 * r + lane;
     if (w < W) {
+      double a = row[w];
       for (long j = cstart[w]; j < cstart[w + 1]; ++j) {
         long l = corder[j];
         if (valid[l * D + d]) {
-          acc[p] += transp[l * D + d];
+          a += transp[l * D + d];
           if (kStats) ++nev;
         }
       }
+      row[w] = a;
     }
   }
   // Pixel indices, centres and offsets fit in 32 bits (W <= 2^30, centres
@@ -1002,9 +1082,11 @@ __device__ __forceinline__ void gather_tile(const GatherArgs& g, int d, long t0,
   const int t0i = (int)t0, t1i = (int)t0 + kWT - 1;
   const long bound = class_window_search(lane, d, t0, t0 + kWT - 1, 1L << 40, regions, list_cm);
   const long npix = W - t0 < kWT ? W - t0 : kWT;  // tile pixels on the grid
-  constexpr double kHalf = 0.5 * kWT;             // a power of two: u / h is exact
+  constexpr double kHalf = 0.5 * kWT;             // powers of two: u / h is exact
+  constexpr double kHalfRow = 32.0;
   const double wm = (double)t0 + 0.5 * (kWT - 1);
-  const double kmin = (kPrec == kPrecFP32 ? kFarExpandRatio32 : kFarExpandRatio) * kHalf;
+  const double kmin = kRatio * kHalf;
+  const double kmin_r = kRatio * kHalfRow;
   WingHead none;
   none.lo = INT32_MAX;
   none.hi = INT32_MIN;
@@ -1016,168 +1098,194 @@ __device__ __forceinline__ void gather_tile(const GatherArgs& g, int d, long t0,
   __shared__ uint32_t s_exact[kGatherQueue];
   const long base = regions[(long)d * (kClasses + 1)];  // the depth's lists start here
   const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-  int qn = 0;                    // queued entries (wave-uniform)
-  int spill_kk = kClasses;       // the class and step at which the queue filled
-  long spill_j0 = 0;
+  int qn = 0;             // queued entries (wave-uniform)
+  bool tile_any = false;  // a tile expansion since the moments were last added
+  double mom[kT];
+#pragma unroll
+  for (int n = 0; n < kT; ++n) mom[n] = 0.0;
 
-  // ---- pass 1: expansion moments m_n = sum X/K^2 (-h/K)^n
-  {
-    constexpr int kT = kPrec == kPrecFP32 ? kFarTerms32 : kFarTerms;
-    double mom[kT];
+  // Steps 2 and 3 on the qn queued entries, then an empty queue.  `ln` is
+  // the lane index made opaque, so that the per-row pixel values below are
+  // formed at each flush instead of being hoisted out of the scan loop and
+  // held in registers throughout it.
+  auto flush = [&]() {
+    int ln = lane;
+    __asm__ volatile("" : "+v"(ln));
+    __syncthreads();  // the queue's writes before its reads (one wave per workgroup)
+    // ---- step 2: row expansions
+#pragma unroll 1
+    for (int r = 0; r < kWP; ++r) {
+      const int r0 = t0i + 64 * r, r1 = r0 + 63;
+      const double wr = (double)r0 + 31.5;
+      double rm[kT];
 #pragma unroll
-    for (int n = 0; n < kT; ++n) mom[n] = 0.0;
-    for (int kk = 0; kk < kClasses; ++kk) {
-      const long lo = uniform_long(__shfl(bound, 2 * kk));
-      const long hi = uniform_long(__shfl(bound, 2 * kk + 1));
-      for (long j0 = lo; j0 < hi; j0 += 64) {
-        const long j = j0 + lane;
-        const WingHead h = j < hi ? list_cm[j] : none;
-        const double K = wm - h.cd;
-        const bool expand = far_interior(h, t0i, t1i) && fabs(K) >= kmin && h.xs >= 0.0;
-        const bool exact = !(t1i < h.lo || t0i > h.hi) && !expand;
-        const unsigned long long em = __ballot(exact);
-        if (em != 0ull && spill_kk == kClasses) {
-          const int ne = __popcll(em);
-          if (qn + ne <= kGatherQueue) {
-            if (exact) s_exact[qn + __popcll(em & below)] = (uint32_t)(j - base);
-            qn += ne;
-          } else {
-            spill_kk = kk;
-            spill_j0 = j0;
-          }
+      for (int n = 0; n < kT; ++n) rm[n] = 0.0;
+      bool any = false;
+#pragma unroll 1
+      for (int i0 = 0; i0 < qn; i0 += 64) {
+        const int i = i0 + ln;
+        const WingHead h = i < qn ? list_cm[base + s_exact[i]] : none;
+        const bool ex = row_expands(h, r0, r1, wr, kmin_r);
+        const unsigned long long em = __ballot(ex);
+        if (em == 0ull) continue;
+        any = true;
+        expansion_moments<kT>(ex ? h.xs : 0.0, ex ? wr - h.cd : kmin_r, kHalfRow, rm);
+        if (kStats && ex) {
+          const long nr = W - r0 < 64 ? W - r0 : 64;
+          nev += nr > 0 ? nr : 0;
         }
-        const double x = expand ? h.xs : 0.0;
-        const double kq = expand ? K : kmin;
-        double y = __builtin_amdgcn_rcp(kq);  // 1/K to an ulp: two Newton steps
-        y = __builtin_fma(y, __builtin_fma(-kq, y, 1.0), y);
-        y = __builtin_fma(y, __builtin_fma(-kq, y, 1.0), y);
-        double q = x * y * y;
-        const double rr = -kHalf * y;
-        mom[0] += q;
+      }
+      if (any) {
+        const double s = row_expansion<kT>(rm, ln);
+        if (r0 + ln < W) row[r0 + ln] += s;
+      }
+    }
+    // ---- step 3: exact rows.  Each lane first classifies its entry's rows
+    // (bit r of ex_int: the row lies in one far side and does not expand, so
+    // every pixel adds X / k^2; of ex_edge: the row holds a zone edge, or X
+    // lies below 2^-500, so pixels are masked); then, row by row, the
+    // entries with that row's bit are broadcast in queue order (interior
+    // rows first), two per iteration: their quotients are independent and
+    // are added one after the other (a missing second entry adds +0, which
+    // leaves the non-negative sum unchanged).  The row sums stay in
+    // registers for the whole step.
+    double acc[kWP];
 #pragma unroll
-        for (int n = 1; n < kT; ++n) {
-          q *= rr;
-          mom[n] += q;
+    for (int r = 0; r < kWP; ++r) {
+      const long w = t0 + 64 * r + ln;
+      acc[r] = w < W ? row[w] : 0.0;
+    }
+#pragma unroll 1
+    for (int i0 = 0; i0 < qn; i0 += 64) {
+      const int i = i0 + ln;
+      const WingHead hv = i < qn ? list_cm[base + s_exact[i]] : none;
+      const bool chunk_l = __double_as_longlong(hv.xs) >= 0;  // X >= 2^-500 (or 0)
+      unsigned ex_int = 0u, ex_edge = 0u;
+#pragma unroll
+      for (int r = 0; r < kWP; ++r) {
+        const int r0 = t0i + 64 * r, r1 = r0 + 63;
+        // the row meets the far zone [lo, nlo - 1] u [nhi + 1, hi]
+        const bool meets = (r0 < hv.nlo && r1 >= hv.lo) || (r1 > hv.nhi && r0 <= hv.hi);
+        const bool inside = far_interior(hv, r0, r1);
+        const bool expands = inside && fabs((double)r0 + 31.5 - hv.cd) >= kmin_r && hv.xs >= 0.0;
+        ex_int |= (meets && inside && chunk_l && !expands) ? (1u << r) : 0u;
+        ex_edge |= (meets && !(inside && chunk_l)) ? (1u << r) : 0u;
+      }
+      if (i >= qn) ex_int = ex_edge = 0u;
+      // the values other lanes read with readlane: formed here in every lane
+      // (the opaque asm keeps the compiler from sinking them into the
+      // branches below, where the lane read may be inactive)
+      double x_l = chunk_l ? hv.xs : -hv.xs;
+      double cd_l = hv.cd;
+      double n10_l = (double)((hv.nhi - hv.nlo) >> 1), m_l = (double)((hv.hi - hv.lo) >> 1);
+      __asm__ volatile("" : "+v"(x_l), "+v"(cd_l), "+v"(n10_l), "+v"(m_l));
+#pragma unroll
+      for (int r = 0; r < kWP; ++r) {
+        const double px = (double)(t0i + 64 * r + ln);  // the lane's pixel in row r
+        const bool on = t0 + 64 * r + ln < W;
+        unsigned long long todo = __ballot((ex_int >> r) & 1u);
+        while (todo) {
+          const int s1 = __builtin_ctzll(todo);
+          todo &= todo - 1;
+          const bool two = todo != 0ull;
+          const int s2 = two ? __builtin_ctzll(todo) : s1;
+          if (two) todo &= todo - 1;
+          const double k1 = px - bcast_double(cd_l, s1);  // |k| >= N10DOP + 1 >= 1
+          const double k2 = px - bcast_double(cd_l, s2);
+          const double x1 = bcast_double(x_l, s1), x2 = bcast_double(x_l, s2);
+          const double v1 = far_quotient<kPrec>(x1, k1 * k1);
+          const double v2 = far_quotient<kPrec>(two ? x2 : 0.0, k2 * k2);
+          acc[r] += v1;
+          acc[r] += v2;
+          if (kStats) nev += on ? (two ? 2 : 1) : 0;
         }
+        todo = __ballot((ex_edge >> r) & 1u);
+        while (todo) {
+          const int s1 = __builtin_ctzll(todo);
+          todo &= todo - 1;
+          const bool two = todo != 0ull;
+          const int s2 = two ? __builtin_ctzll(todo) : s1;
+          if (two) todo &= todo - 1;
+          // every broadcast before any lane-dependent choice
+          const double k1 = px - bcast_double(cd_l, s1);
+          const double k2 = px - bcast_double(cd_l, s2);
+          const double x1 = bcast_double(x_l, s1), x2 = bcast_double(x_l, s2);
+          const double n1 = bcast_double(n10_l, s1), n2 = bcast_double(n10_l, s2);
+          const double m1 = bcast_double(m_l, s1), m2 = bcast_double(m_l, s2);
+          const bool keep1 = (fabs(k1) > n1) & (fabs(k1) <= m1);
+          const bool keep2 = two & (fabs(k2) > n2) & (fabs(k2) <= m2);
+          const double v1 = far_quotient<kPrec>(keep1 ? x1 : 0.0, fmax(k1 * k1, 1.0));
+          const double v2 = far_quotient<kPrec>(keep2 ? x2 : 0.0, fmax(k2 * k2, 1.0));
+          acc[r] += v1;
+          acc[r] += v2;
+          if (kStats) nev += on ? (keep1 ? 1 : 0) + (keep2 ? 1 : 0) : 0;
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < kWP; ++r) {
+      const long w = t0 + 64 * r + ln;
+      if (w < W) row[w] = acc[r];
+    }
+    __syncthreads();  // the queue's reads before its next writes
+    qn = 0;
+  };
+
+  // ---- step 1: tile expansions, queueing the other entries that reach the
+  // tile; steps 2 and 3 whenever the next step could overflow the queue
+  int kk = 0;
+  long j0 = uniform_long(__shfl(bound, 0));
+  long hi = uniform_long(__shfl(bound, 1));
+  for (;;) {
+    bool done = false;
+    while (qn + 64 <= kGatherQueue) {
+      while (j0 >= hi && kk < kClasses) {
+        if (++kk < kClasses) {
+          j0 = uniform_long(__shfl(bound, 2 * kk));
+          hi = uniform_long(__shfl(bound, 2 * kk + 1));
+        }
+      }
+      if (kk == kClasses) {
+        done = true;
+        break;
+      }
+      const long j = j0 + lane;
+      const WingHead h = j < hi ? list_cm[j] : none;
+      const double K = wm - h.cd;
+      const bool expand = far_interior(h, t0i, t1i) && fabs(K) >= kmin && h.xs >= 0.0;
+      const bool exact = !(t1i < h.lo || t0i > h.hi) && !expand;
+      const unsigned long long em = __ballot(exact);
+      if (exact) s_exact[qn + __popcll(em & below)] = (uint32_t)(j - base);
+      qn += __popcll(em);
+      const unsigned long long xm = __ballot(expand);
+      if (xm != 0ull) {
+        tile_any = true;
+        expansion_moments<kT>(expand ? h.xs : 0.0, expand ? K : kmin, kHalf, mom);
         if (kStats && expand) nev += npix > 0 ? npix : 0;
       }
+      j0 += 64;
     }
-    double coef[kT];
-#pragma unroll
-    for (int n = 0; n < kT; ++n) coef[n] = (double)(n + 1) * wave_sum_butterfly(mom[n]);
-#pragma unroll
-    for (int p = 0; p < kWP; ++p) {
-      const double t = ((double)(w0 + p) - wm) / kHalf;
-      double s = coef[kT - 1];
-#pragma unroll
-      for (int n = kT - 2; n >= 0; --n) s = __builtin_fma(s, t, coef[n]);
-      acc[p] += s;
+    // the tile moments met so far go to the pixels first, so that they are
+    // not live across steps 2 and 3
+    if (tile_any) {
+      int ln = lane;  // opaque, as in flush
+      __asm__ volatile("" : "+v"(ln));
+      add_tile_expansion<kT, kWP>(mom, row, t0, W, ln);
     }
+    flush();
+    if (done) break;
+    tile_any = false;
+#pragma unroll
+    for (int n = 0; n < kT; ++n) mom[n] = 0.0;
   }
-
-  // ---- pass 2: the remaining entries, exactly
-  const double kdl = (double)(int)w0;  // lane's first pixel
-  // pixels per shared reciprocal: the largest divisor of kWP up to 8 (4 on
-  // edge tiles, which need more registers)
-  constexpr int kChunk = kWP % 8 == 0 ? 8 : (kWP % 6 == 0 ? 6 : (kWP % 4 == 0 ? 4 : 1));
-#ifndef SYNTHE_EDGE_CHUNK
-#define SYNTHE_EDGE_CHUNK 4
-#endif
-  constexpr int kChunkEdge = kWP % SYNTHE_EDGE_CHUNK == 0 ? SYNTHE_EDGE_CHUNK : (kWP % 2 == 0 ? 2 : 1);
-  constexpr int kNewton = kPrec == kPrecFP32 ? 0 : kFarNewton;
-  double edge[kWP];  // edge tiles' terms, kept apart so each sum updates in place
-#pragma unroll
-  for (int p = 0; p < kWP; ++p) edge[p] = 0.0;
-  // every entry of `todo`, in lane order, with its head in lane src's hv
-  auto take = [&](const WingHead& hv, unsigned long long todo) {
-    while (todo) {
-      const int src = __builtin_ctzll(todo);
-      todo &= todo - 1;
-      WingHead h;  // the entry's head, from the lane that loaded it
-      h.lo = __builtin_amdgcn_readlane(hv.lo, src);
-      h.hi = __builtin_amdgcn_readlane(hv.hi, src);
-      h.nlo = __builtin_amdgcn_readlane(hv.nlo, src);
-      h.nhi = __builtin_amdgcn_readlane(hv.nhi, src);
-      {
-        const long long xb = __double_as_longlong(hv.xs);
-        const long long cb = __double_as_longlong(hv.cd);
-        const int xlo = __builtin_amdgcn_readlane((int)(xb & 0xffffffff), src);
-        const int xhi = __builtin_amdgcn_readlane((int)(xb >> 32), src);
-        const int clo = __builtin_amdgcn_readlane((int)(cb & 0xffffffff), src);
-        const int chi = __builtin_amdgcn_readlane((int)(cb >> 32), src);
-        h.xs = __longlong_as_double((long long)(((unsigned long long)(unsigned)xhi << 32) | (unsigned)xlo));
-        h.cd = __longlong_as_double((long long)(((unsigned long long)(unsigned)chi << 32) | (unsigned)clo));
-      }
-      double kd = kdl - h.cd;  // offset of the lane's first pixel from the centre
-      __asm__("" : "+v"(kd));  // keep k = kd + p in double (no per-pixel int conversion)
-      const bool interior = far_interior(h, t0i, t1i);
-      const bool chunk = __double_as_longlong(h.xs) >= 0;
-      const int n10 = (h.nhi - h.nlo) >> 1, M = (h.hi - h.lo) >> 1;
-      if (chunk && interior) {  // |k| >= N10DOP + 1 >= 1 on every pixel of the tile
-#pragma unroll
-        for (int p0 = 0; p0 < kWP; p0 += kChunk)
-          far_chunk_acc<kChunk, kNewton>(h.xs, kd, p0, acc + p0);
-      } else {
-        double add[kWP];
-        if (chunk) {
-          const double n10d = (double)n10, md = (double)M;
-#pragma unroll
-          for (int p0 = 0; p0 < kWP; p0 += kChunkEdge)
-            far_chunk<kChunkEdge, kNewton, true>(h.xs, kd, p0, n10d, md, add + p0);
-        } else {  // X below 2^-500: one quotient per pixel
-          const double n10d = (double)n10, md = (double)M, x = -h.xs;
-#pragma unroll
-          for (int p = 0; p < kWP; ++p) {
-            const double k = kd + (double)p;
-            const double xp = (fabs(k) > n10d && fabs(k) <= md) ? x : 0.0;
-            add[p] = far_quotient<kPrec>(xp, fmax(k * k, 1.0));
-          }
-        }
-#pragma unroll
-        for (int p = 0; p < kWP; ++p) edge[p] += add[p];
-      }
-      if (kStats) {
-        if (interior) {
-          long nv = W - w0 < kWP ? W - w0 : kWP;
-          nev += nv > 0 ? nv : 0;
-        } else {
-          const int c = h.lo + M;
-          for (int p = 0; p < kWP; ++p) {
-            const int ka = abs((int)w0 - c + p);
-            if (ka > n10 && ka <= M && w0 + p < W) ++nev;
-          }
-        }
-      }
-    }
-  };
-  __syncthreads();  // the queue's writes before its reads (one wave per workgroup)
-  for (int i0 = 0; i0 < qn; i0 += 64) {
-    const int i = i0 + lane;
-    const WingHead hv = i < qn ? list_cm[base + s_exact[i]] : none;
-    take(hv, __ballot(i < qn));
-  }
-  for (int kk = spill_kk; kk < kClasses; ++kk) {  // only when the queue filled
-    const long lo = kk == spill_kk ? spill_j0 : uniform_long(__shfl(bound, 2 * kk));
-    const long hi = uniform_long(__shfl(bound, 2 * kk + 1));
-    for (long j0 = lo; j0 < hi; j0 += 64) {
-      const long jl = j0 + lane;
-      const WingHead hv = jl < hi ? list_cm[jl] : none;
-      const bool reach = !(t1i < hv.lo || t0i > hv.hi);
-      const bool expand = far_interior(hv, t0i, t1i) && fabs(wm - hv.cd) >= kmin && hv.xs >= 0.0;
-      take(hv, __ballot(reach && !expand));
-    }
-  }
-  __syncthreads();  // the queue's reads before a later tile's writes
-#pragma unroll
-  for (int p = 0; p < kWP; ++p) acc[p] += edge[p];
   double T = temp[d] > 1.0 ? temp[d] : 1.0;
   double hkt = kHPlanck / (kKBoltz * T);
 #pragma unroll
-  for (int p = 0; p < kWP; ++p) {
-    long w = w0 + p;
+  for (int r = 0; r < kWP; ++r) {
+    const long w = t0 + 64 * r + lane;
     if (w < W) {
       double fg = kCLightNm / grid[w];
-      row[w] = acc[p] * (1.0 - exp(-fg * hkt));
+      row[w] = row[w] * (1.0 - exp(-fg * hkt));
     }
   }
   if (kStats) {
