@@ -959,14 +959,48 @@ __device__ __forceinline__ bool far_interior(const WingHead& h, int t0i, int t1i
 // are added to the pixels) and at the end.  Each pixel's sum has a fixed
 // order that depends only on the lists, no atomics, so results are
 // run-to-run identical.
-// List positions (relative to the depth's lists) of one tile's queued
-// entries: 7 KiB of LDS per wave, so the 20 waves a CU holds at five per SIMD
-// fit in its 160 KiB.
+// The queue holds one 16-byte record per entry: X (as stored in the head)
+// and, packed in 61 bits, the centre relative to the tile's first pixel
+// (21-bit two's complement) and the two half-widths r (Voigt zone) and
+// MAXSTEP (20 bits each).  An entry that reaches the tile has |c - t0| <=
+// MAXSTEP + tile and MAXSTEP <= MAXPROF = 10^6 < 2^20.  640 records are 10
+// KiB of LDS per wave, so the 16 waves a CU holds at four per SIMD fill its
+// 160 KiB (a full queue is flushed: 384, 512, 576 and 640 records measured
+// 21.9, 20.7, 20.5 and 20.0 ms on the full window).
 #ifndef SYNTHE_GATHER_QUEUE
-#define SYNTHE_GATHER_QUEUE 1792
+#define SYNTHE_GATHER_QUEUE 640
 #endif
 constexpr int kGatherQueue = SYNTHE_GATHER_QUEUE;
 static_assert(kGatherQueue >= 64 && kGatherQueue % 64 == 0, "queue holds whole steps of 64 entries");
+static_assert(kMaxProfileSteps + 64 * 64 < (1L << 20), "queue record fields hold MAXPROF plus a tile");
+
+struct QueueRec {
+  double xs;
+  uint64_t pk;
+};
+
+__device__ __forceinline__ QueueRec queue_pack(const WingHead& h, int t0i) {
+  const int dc = head_c(h) - t0i;
+  const unsigned r = (unsigned)((h.nhi - h.nlo) >> 1), m = (unsigned)((h.hi - h.lo) >> 1);
+  QueueRec q;
+  q.xs = h.xs;
+  q.pk = (uint64_t)((unsigned)dc & 0x1fffffu) | ((uint64_t)r << 21) | ((uint64_t)m << 41);
+  return q;
+}
+
+// The head of a queued entry in the tile's own coordinates (pixel t0 is 0).
+__device__ __forceinline__ WingHead queue_unpack(const QueueRec& q) {
+  const int dc = ((int)(unsigned)(q.pk << 11)) >> 11;  // bits 0-20, sign-extended
+  const int r = (int)((q.pk >> 21) & 0xfffffu), m = (int)((q.pk >> 41) & 0xfffffu);
+  WingHead h;
+  h.lo = dc - m;
+  h.hi = dc + m;
+  h.nlo = dc - r;
+  h.nhi = dc + r;
+  h.xs = q.xs;
+  h.cd = (double)dc;
+  return h;
+}
 
 // Tile expansion: adds (n + 1) * sum_lanes(mom[n]) t^n to every pixel of the
 // tile's row segment, with t = (pixel - tile centre) / h (exact: h is a power
@@ -977,12 +1011,7 @@ __device__ __forceinline__ void add_tile_expansion(const double* mom, double* __
   constexpr double kCentre = 0.5 * (64 * kWP - 1), kInvH = 1.0 / (32.0 * kWP);
   double coef[kT];
 #pragma unroll
-  for (int n = 0; n < kT; ++n) {
-    coef[n] = (double)(n + 1) * wave_sum_butterfly(mom[n]);
-    // one reduction at a time: interleaving all kT of them for latency
-    // costs more registers than the kernel's occupancy allows
-    __builtin_amdgcn_sched_barrier(0);
-  }
+  for (int n = 0; n < kT; ++n) coef[n] = (double)(n + 1) * wave_sum_butterfly(mom[n]);
 #pragma unroll
   for (int r = 0; r < kWP; ++r) {
     const double t = ((double)(64 * r + lane) - kCentre) * kInvH;
@@ -1001,10 +1030,7 @@ __device__ __forceinline__ double row_expansion(const double* mom, int lane) {
   const double t = ((double)lane - 31.5) * (1.0 / 32.0);
   double s = (double)kT * wave_sum_butterfly(mom[kT - 1]);
 #pragma unroll
-  for (int n = kT - 2; n >= 0; --n) {
-    __builtin_amdgcn_sched_barrier(0);  // one reduction at a time (registers)
-    s = __builtin_fma(s, t, (double)(n + 1) * wave_sum_butterfly(mom[n]));
-  }
+  for (int n = kT - 2; n >= 0; --n) s = __builtin_fma(s, t, (double)(n + 1) * wave_sum_butterfly(mom[n]));
   return s;
 }
 
@@ -1095,8 +1121,7 @@ __device__ __forceinline__ void gather_tile(const GatherArgs& g, int d, long t0,
   none.xs = 0.0;
   none.cd = 0.0;
 
-  __shared__ uint32_t s_exact[kGatherQueue];
-  const long base = regions[(long)d * (kClasses + 1)];  // the depth's lists start here
+  __shared__ QueueRec s_q[kGatherQueue];
   const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   int qn = 0;             // queued entries (wave-uniform)
   bool tile_any = false;  // a tile expansion since the moments were last added
@@ -1104,10 +1129,12 @@ __device__ __forceinline__ void gather_tile(const GatherArgs& g, int d, long t0,
 #pragma unroll
   for (int n = 0; n < kT; ++n) mom[n] = 0.0;
 
-  // Steps 2 and 3 on the qn queued entries, then an empty queue.  `ln` is
-  // the lane index made opaque, so that the per-row pixel values below are
-  // formed at each flush instead of being hoisted out of the scan loop and
-  // held in registers throughout it.
+  // Steps 2 and 3 on the qn queued entries, then an empty queue.  The
+  // queued heads are in the tile's own coordinates (queue_unpack), and so are
+  // the rows here (row r: pixels 64 r .. 64 r + 63).  `ln` is the lane index
+  // made opaque, so that the per-row pixel values below are formed at each
+  // flush instead of being hoisted out of the scan loop and held in
+  // registers throughout it.
   auto flush = [&]() {
     int ln = lane;
     __asm__ volatile("" : "+v"(ln));
@@ -1115,8 +1142,9 @@ __device__ __forceinline__ void gather_tile(const GatherArgs& g, int d, long t0,
     // ---- step 2: row expansions
 #pragma unroll 1
     for (int r = 0; r < kWP; ++r) {
-      const int r0 = t0i + 64 * r, r1 = r0 + 63;
+      const int r0 = 64 * r, r1 = r0 + 63;
       const double wr = (double)r0 + 31.5;
+      const long wa = t0 + r0;  // the row's first pixel on the grid
       double rm[kT];
 #pragma unroll
       for (int n = 0; n < kT; ++n) rm[n] = 0.0;
@@ -1124,20 +1152,20 @@ __device__ __forceinline__ void gather_tile(const GatherArgs& g, int d, long t0,
 #pragma unroll 1
       for (int i0 = 0; i0 < qn; i0 += 64) {
         const int i = i0 + ln;
-        const WingHead h = i < qn ? list_cm[base + s_exact[i]] : none;
+        const WingHead h = i < qn ? queue_unpack(s_q[i]) : none;
         const bool ex = row_expands(h, r0, r1, wr, kmin_r);
         const unsigned long long em = __ballot(ex);
         if (em == 0ull) continue;
         any = true;
         expansion_moments<kT>(ex ? h.xs : 0.0, ex ? wr - h.cd : kmin_r, kHalfRow, rm);
         if (kStats && ex) {
-          const long nr = W - r0 < 64 ? W - r0 : 64;
+          const long nr = W - wa < 64 ? W - wa : 64;
           nev += nr > 0 ? nr : 0;
         }
       }
       if (any) {
         const double s = row_expansion<kT>(rm, ln);
-        if (r0 + ln < W) row[r0 + ln] += s;
+        if (wa + ln < W) row[wa + ln] += s;
       }
     }
     // ---- step 3: exact rows.  Each lane first classifies its entry's rows
@@ -1158,12 +1186,12 @@ __device__ __forceinline__ void gather_tile(const GatherArgs& g, int d, long t0,
 #pragma unroll 1
     for (int i0 = 0; i0 < qn; i0 += 64) {
       const int i = i0 + ln;
-      const WingHead hv = i < qn ? list_cm[base + s_exact[i]] : none;
+      const WingHead hv = i < qn ? queue_unpack(s_q[i]) : none;
       const bool chunk_l = __double_as_longlong(hv.xs) >= 0;  // X >= 2^-500 (or 0)
       unsigned ex_int = 0u, ex_edge = 0u;
 #pragma unroll
       for (int r = 0; r < kWP; ++r) {
-        const int r0 = t0i + 64 * r, r1 = r0 + 63;
+        const int r0 = 64 * r, r1 = r0 + 63;
         // the row meets the far zone [lo, nlo - 1] u [nhi + 1, hi]
         const bool meets = (r0 < hv.nlo && r1 >= hv.lo) || (r1 > hv.nhi && r0 <= hv.hi);
         const bool inside = far_interior(hv, r0, r1);
@@ -1181,7 +1209,7 @@ __device__ __forceinline__ void gather_tile(const GatherArgs& g, int d, long t0,
       __asm__ volatile("" : "+v"(x_l), "+v"(cd_l), "+v"(n10_l), "+v"(m_l));
 #pragma unroll
       for (int r = 0; r < kWP; ++r) {
-        const double px = (double)(t0i + 64 * r + ln);  // the lane's pixel in row r
+        const double px = (double)(64 * r + ln);  // the lane's pixel in row r
         const bool on = t0 + 64 * r + ln < W;
         unsigned long long todo = __ballot((ex_int >> r) & 1u);
         while (todo) {
@@ -1212,8 +1240,9 @@ __device__ __forceinline__ void gather_tile(const GatherArgs& g, int d, long t0,
           const double x1 = bcast_double(x_l, s1), x2 = bcast_double(x_l, s2);
           const double n1 = bcast_double(n10_l, s1), n2 = bcast_double(n10_l, s2);
           const double m1 = bcast_double(m_l, s1), m2 = bcast_double(m_l, s2);
-          const bool keep1 = (fabs(k1) > n1) & (fabs(k1) <= m1);
-          const bool keep2 = two & (fabs(k2) > n2) & (fabs(k2) <= m2);
+          // bitwise, not short-circuit: no branch between the broadcasts
+          const bool keep1 = ((int)(fabs(k1) > n1) & (int)(fabs(k1) <= m1)) != 0;
+          const bool keep2 = ((int)two & (int)(fabs(k2) > n2) & (int)(fabs(k2) <= m2)) != 0;
           const double v1 = far_quotient<kPrec>(keep1 ? x1 : 0.0, fmax(k1 * k1, 1.0));
           const double v2 = far_quotient<kPrec>(keep2 ? x2 : 0.0, fmax(k2 * k2, 1.0));
           acc[r] += v1;
@@ -1255,7 +1284,7 @@ __device__ __forceinline__ void gather_tile(const GatherArgs& g, int d, long t0,
       const bool expand = far_interior(h, t0i, t1i) && fabs(K) >= kmin && h.xs >= 0.0;
       const bool exact = !(t1i < h.lo || t0i > h.hi) && !expand;
       const unsigned long long em = __ballot(exact);
-      if (exact) s_exact[qn + __popcll(em & below)] = (uint32_t)(j - base);
+      if (exact) s_q[qn + __popcll(em & below)] = queue_pack(h, t0i);
       qn += __popcll(em);
       const unsigned long long xm = __ballot(expand);
       if (xm != 0ull) {
@@ -1273,10 +1302,10 @@ __device__ __forceinline__ void gather_tile(const GatherArgs& g, int d, long t0,
       add_tile_expansion<kT, kWP>(mom, row, t0, W, ln);
     }
     flush();
-    if (done) break;
     tile_any = false;
 #pragma unroll
     for (int n = 0; n < kT; ++n) mom[n] = 0.0;
+    if (done) break;
   }
   double T = temp[d] > 1.0 ? temp[d] : 1.0;
   double hkt = kHPlanck / (kKBoltz * T);
