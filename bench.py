@@ -310,7 +310,7 @@ def bench_inverse(args) -> dict:
     adjoint_ms = (time.perf_counter() - t0) / 3 * 1e3
     prob.evaluations = 0
     sol = prob.solve(max_iter=20)
-    rep = INV.report(targets, sol["delta"], sol["sensitivity"])
+    rep = INV.report(targets, sol["delta"], sol["sensitivity"], marginal=sol["marginal"])
     check = INV.fp64_plain_check(targets, sol["delta"], indices=range(8))
     # adjoint cross-check at the solution, on the spectrum whose lines
     # constrain the line elements most: d(sum w F)/d(offsets)

@@ -487,6 +487,7 @@ class InverseProblem:
         mu = np.zeros(n)
         active = cost > 0
         sens = np.zeros((n, nz))
+        marg = np.zeros((n, nz))  # marginal sensitivities 1 / sqrt([(J^T J)^-1]_kk)
         svd = [None] * n
         hess = [None] * n   # adjoint mode: Hessian estimates of the live columns
         last = [None] * n   # adjoint mode: (accepted step, gradient before it)
@@ -527,6 +528,13 @@ class InverseProblem:
                                                  full_matrices=False)
                     s_np = s_.cpu().numpy()
                     svd.append((idx, s_np, vh.T.cpu().numpy(), s_np * (u.T @ (res[i] * m)).cpu().numpy()))
+                    # how far the residual moves per dex of an element when the other live
+                    # elements re-adjust: 1 / sigma_k, sigma_k^2 = sum_j V_kj^2 / s_j^2
+                    Vn = vh.T.cpu().numpy()
+                    with np.errstate(divide="ignore"):
+                        var = (Vn ** 2 / np.where(s_np > 0, s_np, 0.0) ** 2).sum(axis=1)
+                    marg[i] = 0.0
+                    marg[i, idx] = np.where(np.isfinite(var) & (var > 0), 1.0 / np.sqrt(var), 0.0)
                     # Gauss-Newton Hessian J^T J of the live columns, the start of the quasi-Newton updates
                     V_np = vh.T.cpu().numpy()
                     hess[i] = (V_np * s_np ** 2) @ V_np.T
@@ -607,22 +615,27 @@ class InverseProblem:
         elapsed = time.perf_counter() - t0
         return dict(delta=delta, iterations=it, seconds=elapsed, evaluations=self.evaluations,
                     adjoint_evaluations=adjoint_calls, jacobians=refreshes, cost_history=history,
-                    cost_per_target=np.array(per_target), cost=cost.copy(), sensitivity=sens, gradient=gradient)
+                    cost_per_target=np.array(per_target), cost=cost.copy(), sensitivity=sens, marginal=marg,
+                    gradient=gradient)
 
 
 def report(targets: Sequence[Target], delta: np.ndarray, sensitivity: np.ndarray,
-           min_sensitivity: float = 2e-3) -> dict:
+           min_sensitivity: float = 2e-3, marginal: Optional[np.ndarray] = None) -> dict:
     """Abundance errors (dex).  A spectrum constrains a line element (Fe, Ti,
     Cr, Ca, Ni) when its residual moves by at least min_sensitivity (L2, in
     units of the median flux) per dex of that element -- 0.05 dex then changes the misfit
     by 2.5e-3 min_sensitivity^2 or more; neutral-metal lines in hot
-    atmospheres are too weak for that.  Errors are given for the constrained
-    and the unconstrained pairs, and over all 19 offsets (hydrogen is the
-    reference element)."""
+    atmospheres are too weak for that.  With `marginal` (solve()'s
+    1 / sqrt([(J^T J)^-1]_kk)) the move is the one left when the spectrum's
+    other elements re-adjust, so two elements whose lines trade off against
+    each other (a fit to 1e-12 of the misfit with both off) are not counted
+    as constrained; without it, the Jacobian column's own norm (sensitivity).
+    Errors are given for the constrained and the unconstrained pairs, and
+    over all 19 offsets (hydrogen is the reference element)."""
     truth = np.stack([t.delta_true for t in targets])
     err = np.abs(delta - truth)
     li = [S.ABUNDANCE_Z.index(z) for z in LINE_ELEMENTS]
-    fit = sensitivity[:, li] >= min_sensitivity
+    fit = (marginal if marginal is not None else sensitivity)[:, li] >= min_sensitivity
     # the number fraction log(A_z / sum A): what line strengths measure when
     # nothing else in the spectrum pins sum A
     def lsum(d, t):
@@ -634,6 +647,8 @@ def report(targets: Sequence[Target], delta: np.ndarray, sensitivity: np.ndarray
     e_fit, e_not = err[:, li][fit], err[:, li][~fit]
     free = [k for k in range(len(S.ABUNDANCE_Z)) if S.ABUNDANCE_Z[k] != 1]
     out = dict(line_pairs=int(fit.size), line_pairs_constrained=int(fit.sum()),
+               constraint="marginal" if marginal is not None else "column",
+               line_pairs_column_over_floor=int((sensitivity[:, li] >= min_sensitivity).sum()),
                max_err_constrained_line_dex=float(e_fit.max()) if e_fit.size else None,
                median_err_constrained_line_dex=float(np.median(e_fit)) if e_fit.size else None,
                p90_err_constrained_line_dex=float(np.percentile(e_fit, 90)) if e_fit.size else None,

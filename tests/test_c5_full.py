@@ -30,13 +30,15 @@ def test_c5_inverse_problem_full_size(gpu):
     c0 = prob._costs(prob.residuals(prob.evaluate(zero)), prob.valid)
     sol = prob.solve(max_iter=20)
     cf = sol["cost"]
-    rep = INV.report(targets, sol["delta"], sol["sensitivity"])
+    rep = INV.report(targets, sol["delta"], sol["sensitivity"], marginal=sol["marginal"])
     live = c0 > 1e-9
     print(f"misfit {c0.sum():.4g} -> {cf.sum():.4g}; worst ratio over {int(live.sum())} live targets "
           f"{np.max(cf[live] / c0[live]):.3g}; {rep}")
     assert cf.sum() < 1e-5 * c0.sum()
     assert np.all(cf[live] < 1e-2 * c0[live])
     assert rep["line_pairs_constrained"] >= 40 and rep["median_err_constrained_line_dex"] < 1e-3
+    # constrained by the marginal sensitivity: no element pair that trades off against another
+    assert rep["max_err_constrained_line_dex"] < 0.05
     order = np.argsort(-cf / np.maximum(c0, 1e-300))
     check = INV.fp64_plain_check(targets, sol["delta"], indices=list(range(4)) + order[:4].tolist())
     print("FP64 plain path at the recovered offsets:", check)

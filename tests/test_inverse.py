@@ -29,7 +29,7 @@ def test_gauss_newton_recovers_line_elements(small_problem):
     from kurucz_py_joss_submission_amd.inverse import report
 
     targets, prob, sol = small_problem
-    rep = report(targets, sol["delta"], sol["sensitivity"])
+    rep = report(targets, sol["delta"], sol["sensitivity"], marginal=sol["marginal"])
     print(sol["iterations"], sol["evaluations"], sol["cost"], rep)
     c0 = prob._costs(prob.residuals(prob.evaluate(np.zeros_like(sol["delta"]))), prob.valid)
     assert np.all(sol["cost"] < 1e-6 * c0)
@@ -125,7 +125,7 @@ def test_adjoint_gauss_newton_recovers_line_elements(small_problem):
     targets, prob, sol = small_problem
     p2 = InverseProblem(targets, autotune=False)
     sol2 = p2.solve(max_iter=40, gradient="adjoint")
-    rep = report(targets, sol2["delta"], sol2["sensitivity"])
+    rep = report(targets, sol2["delta"], sol2["sensitivity"], marginal=sol2["marginal"])
     print(sol2["iterations"], sol2["evaluations"], sol2["adjoint_evaluations"], sol2["jacobians"], sol2["cost"], rep,
           "differences:", sol["evaluations"])
     np.set_printoptions(linewidth=200, precision=3)
