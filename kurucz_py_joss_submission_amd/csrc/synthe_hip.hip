@@ -1261,38 +1261,54 @@ __device__ __forceinline__ void gather_tile(const GatherArgs& g, int d, long t0,
   };
 
   // ---- step 1: tile expansions, queueing the other entries that reach the
-  // tile; steps 2 and 3 whenever the next step could overflow the queue
+  // tile; steps 2 and 3 whenever the next step could overflow the queue.
+  // (kk, j0, hi): the class, first entry and window end of the current step.
+  // The next step's heads are loaded while this one is processed, as long as
+  // the queue has room for both steps; a step that may be followed by a
+  // flush loads its own heads, so no prefetched head is live across one.
+  auto settle = [&](int& c, long& s0, long& e) {  // the next non-empty window
+    while (s0 >= e && c < kClasses) {
+      if (++c < kClasses) {
+        s0 = uniform_long(__shfl(bound, 2 * c));
+        e = uniform_long(__shfl(bound, 2 * c + 1));
+      }
+    }
+  };
   int kk = 0;
   long j0 = uniform_long(__shfl(bound, 0));
   long hi = uniform_long(__shfl(bound, 1));
+  settle(kk, j0, hi);
   for (;;) {
-    bool done = false;
-    while (qn + 64 <= kGatherQueue) {
-      while (j0 >= hi && kk < kClasses) {
-        if (++kk < kClasses) {
-          j0 = uniform_long(__shfl(bound, 2 * kk));
-          hi = uniform_long(__shfl(bound, 2 * kk + 1));
+    const bool done = kk == kClasses;
+    if (!done && qn + 64 <= kGatherQueue) {
+      WingHead h = j0 + lane < hi ? list_cm[j0 + lane] : none;
+      for (;;) {
+        int kk2 = kk;
+        long j02 = j0 + 64, hi2 = hi;
+        settle(kk2, j02, hi2);
+        // the next step runs without a flush in between only if this step
+        // leaves room for 64 more entries whatever it queues
+        const bool ahead = kk2 < kClasses && qn + 128 <= kGatherQueue;
+        const WingHead hn = ahead && j02 + lane < hi2 ? list_cm[j02 + lane] : none;
+        const double K = wm - h.cd;
+        const bool expand = far_interior(h, t0i, t1i) && fabs(K) >= kmin && h.xs >= 0.0;
+        const bool exact = !(t1i < h.lo || t0i > h.hi) && !expand;
+        const unsigned long long em = __ballot(exact);
+        if (exact) s_q[qn + __popcll(em & below)] = queue_pack(h, t0i);
+        qn += __popcll(em);
+        const unsigned long long xm = __ballot(expand);
+        if (xm != 0ull) {
+          tile_any = true;
+          expansion_moments<kT>(expand ? h.xs : 0.0, expand ? K : kmin, kHalf, mom);
+          if (kStats && expand) nev += npix > 0 ? npix : 0;
         }
+        kk = kk2;
+        j0 = j02;
+        hi = hi2;
+        if (!ahead) break;
+        h = hn;
       }
-      if (kk == kClasses) {
-        done = true;
-        break;
-      }
-      const long j = j0 + lane;
-      const WingHead h = j < hi ? list_cm[j] : none;
-      const double K = wm - h.cd;
-      const bool expand = far_interior(h, t0i, t1i) && fabs(K) >= kmin && h.xs >= 0.0;
-      const bool exact = !(t1i < h.lo || t0i > h.hi) && !expand;
-      const unsigned long long em = __ballot(exact);
-      if (exact) s_q[qn + __popcll(em & below)] = queue_pack(h, t0i);
-      qn += __popcll(em);
-      const unsigned long long xm = __ballot(expand);
-      if (xm != 0ull) {
-        tile_any = true;
-        expansion_moments<kT>(expand ? h.xs : 0.0, expand ? K : kmin, kHalf, mom);
-        if (kStats && expand) nev += npix > 0 ? npix : 0;
-      }
-      j0 += 64;
+      if (kk < kClasses && qn + 64 <= kGatherQueue) continue;  // the next step loads its own heads
     }
     // the tile moments met so far go to the pixels first, so that they are
     // not live across steps 2 and 3
@@ -1305,7 +1321,7 @@ __device__ __forceinline__ void gather_tile(const GatherArgs& g, int d, long t0,
     tile_any = false;
 #pragma unroll
     for (int n = 0; n < kT; ++n) mom[n] = 0.0;
-    if (done) break;
+    if (kk == kClasses) break;
   }
   double T = temp[d] > 1.0 ? temp[d] : 1.0;
   double hkt = kHPlanck / (kKBoltz * T);
