@@ -2462,16 +2462,90 @@ __global__ void k_vjp_order_batch(const VjpArgs* __restrict__ A, int n, long G, 
 // A wavelength's state lives in its record between stages; each entry is
 // owned by one lane of a stage, which reads, updates and writes it back, so
 // every sum runs in the order adj::total_flux_vjp runs it.
+// The per-wavelength records the vjp stages pass on (about 13 KB each), stored
+// field-major: of field f, record t's element j is at f's region + j G + t
+// (G records), so the lanes of a wave, which hold consecutive records (or
+// share one), read and write each element with one coalesced access.  Held
+// whole, one record per lane, every access was a transaction of its own and
+// the stages waited on memory 56-87% of their cycles.
+enum VjpField {
+  kFVw, kFRho, kFAbtot, kFAlpha, kFSnubar, kFTaunu, kFS, kFLam, kFGxb, kFGxa, kFBand, kFGTau, kFGSnubar,
+  kFGAlpha, kFN, kFFlag, kFIdx, kFBase, kFCount
+};
+
+// elements per record and element size of field f
+__host__ __device__ constexpr size_t vjp_field_elems(int f) {
+  return f == kFVw || f == kFN || f == kFFlag ? 1
+         : f == kFGxb || f == kFGxa || f == kFBase ? (size_t)adj::kXt
+         : f == kFBand ? (size_t)5 * kAdjMaxDepth
+                       : (size_t)kAdjMaxDepth;
+}
+__host__ __device__ constexpr size_t vjp_field_size(int f) { return f >= kFN ? sizeof(int) : sizeof(double); }
+
+// byte offset of field f's region for G records (regions 256-byte aligned)
+__host__ __device__ inline size_t vjp_field_offset(int f, long G) {
+  size_t off = 0;
+  for (int i = 0; i < f; ++i) off += (vjp_field_elems(i) * vjp_field_size(i) * (size_t)G + 255) & ~(size_t)255;
+  return off;
+}
+
+struct VjpStore {
+  char* base;  // 256-byte aligned
+  long G;      // records
+};
+
+static size_t vjp_store_bytes(long G) { return vjp_field_offset(kFCount, G); }
+
+template <typename T>
+struct FieldRef {
+  T* r;
+  long g, t;
+  __device__ __forceinline__ T& operator[](int j) const { return r[(long)j * g + t]; }
+};
+
+// band[j][o], o < 5
+struct BandRef {
+  double* r;
+  long g, t;
+  struct Row {
+    double* r;
+    long g, t;
+    __device__ __forceinline__ double& operator[](int o) const { return r[(long)o * g + t]; }
+  };
+  __device__ __forceinline__ Row operator[](int j) const { return Row{r + (long)j * 5 * g, g, t}; }
+};
+
+// Record t of a store, with the fields of the former per-wavelength struct.
 struct VjpRecord {
-  int n, flag;  // depths in the solve; 0 XTAU branch, 1 label 401, -1 no work
-  double vw;
-  int idx[kAdjMaxDepth];
-  int base[adj::kXt];
-  double rho[kAdjMaxDepth], abtot[kAdjMaxDepth], alpha[kAdjMaxDepth], snubar[kAdjMaxDepth];
-  double taunu[kAdjMaxDepth], S[kAdjMaxDepth], lam[kAdjMaxDepth];
-  double gxb[adj::kXt], gxa[adj::kXt];
-  double band[kAdjMaxDepth][5];
-  double g_tau[kAdjMaxDepth], g_snubar[kAdjMaxDepth], g_alpha[kAdjMaxDepth];
+  int& n;     // depths in the solve
+  int& flag;  // 0 XTAU branch, 1 label 401, -1 no work
+  double& vw;
+  FieldRef<int> idx, base;
+  FieldRef<double> rho, abtot, alpha, snubar, taunu, S, lam, gxb, gxa, g_tau, g_snubar, g_alpha;
+  BandRef band;
+  template <typename T>
+  __device__ __forceinline__ static T* field(const VjpStore& s, int f) {
+    return (T*)(s.base + vjp_field_offset(f, s.G));
+  }
+  __device__ __forceinline__ VjpRecord(const VjpStore& s, long t)
+      : n(field<int>(s, kFN)[t]),
+        flag(field<int>(s, kFFlag)[t]),
+        vw(field<double>(s, kFVw)[t]),
+        idx{field<int>(s, kFIdx), s.G, t},
+        base{field<int>(s, kFBase), s.G, t},
+        rho{field<double>(s, kFRho), s.G, t},
+        abtot{field<double>(s, kFAbtot), s.G, t},
+        alpha{field<double>(s, kFAlpha), s.G, t},
+        snubar{field<double>(s, kFSnubar), s.G, t},
+        taunu{field<double>(s, kFTaunu), s.G, t},
+        S{field<double>(s, kFS), s.G, t},
+        lam{field<double>(s, kFLam), s.G, t},
+        gxb{field<double>(s, kFGxb), s.G, t},
+        gxa{field<double>(s, kFGxa), s.G, t},
+        g_tau{field<double>(s, kFGTau), s.G, t},
+        g_snubar{field<double>(s, kFGSnubar), s.G, t},
+        g_alpha{field<double>(s, kFGAlpha), s.G, t},
+        band{field<double>(s, kFBand), s.G, t} {}
 };
 
 using VjpColumn = adj::Column<kAdjMaxDepth>;
@@ -2491,7 +2565,7 @@ __device__ __forceinline__ void vjp_load(const VjpRecord& R, VjpColumn& col, Vjp
 // The column of one wavelength, its FP64 solve and (XTAU branch) the fixed
 // point's adjoint, into record R; t indexes the spectrum's order.
 __device__ __forceinline__ void vjp_solve_at(const VjpArgs& a, long t, const int32_t* __restrict__ order,
-                                             const DeviceTables& tb, const float* s_cj, VjpRecord& R) {
+                                             const DeviceTables& tb, const float* s_cj, const VjpRecord& R) {
   const int D = a.D;
   const long W = a.W;
   const long w = order[t];
@@ -2569,28 +2643,28 @@ __device__ __forceinline__ void vjp_stage_coefj(const DeviceTables& tb, float* s
 }
 
 __global__ __launch_bounds__(kVjpBlock) void k_vjp_solve(VjpArgs a, const int32_t* __restrict__ order,
-                                                         DeviceTables tb, VjpRecord* __restrict__ recs) {
+                                                         DeviceTables tb, VjpStore recs) {
   __shared__ float s_cj[adj::kXt * adj::kXt];
   vjp_stage_coefj(tb, s_cj);
   const long t = blockIdx.x * (long)kVjpBlock + threadIdx.x;
-  if (t < a.W) vjp_solve_at(a, t, order, tb, s_cj, recs[t]);
+  if (t < a.W) vjp_solve_at(a, t, order, tb, s_cj, VjpRecord(recs, t));
 }
 
 __global__ __launch_bounds__(kVjpBlock) void k_vjp_solve_batch(const VjpArgs* __restrict__ A, int n, long G,
                                                                const int32_t* __restrict__ order, DeviceTables tb,
-                                                               VjpRecord* __restrict__ recs) {
+                                                               VjpStore recs) {
   __shared__ float s_cj[adj::kXt * adj::kXt];
   vjp_stage_coefj(tb, s_cj);
   const long g = blockIdx.x * (long)kVjpBlock + threadIdx.x;
   if (g >= G) return;
   const VjpArgs a = A[vjp_inst(A, n, g)];
-  vjp_solve_at(a, g - a.g0, order + a.g0, tb, s_cj, recs[g]);
+  vjp_solve_at(a, g - a.g0, order + a.g0, tb, s_cj, VjpRecord(recs, g));
 }
 
-__global__ __launch_bounds__(64) void k_vjp_pass(long W, DeviceTables tb, VjpRecord* __restrict__ recs) {
+__global__ __launch_bounds__(64) void k_vjp_pass(long W, DeviceTables tb, VjpStore recs) {
   const long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (u >= W * 5) return;
-  VjpRecord& R = recs[u / 5];
+  const VjpRecord R(recs, u / 5);
   const int r = (int)(u % 5);
   const int flag = R.flag;
   if (flag < 0 || (flag == 0 && r == 4)) return;
@@ -2627,10 +2701,10 @@ __global__ __launch_bounds__(64) void k_vjp_pass(long W, DeviceTables tb, VjpRec
   }
 }
 
-__global__ __launch_bounds__(64) void k_vjp_adj401(long W, VjpRecord* __restrict__ recs) {
+__global__ __launch_bounds__(64) void k_vjp_adj401(long W, VjpStore recs) {
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (t >= W) return;
-  VjpRecord& R = recs[t];
+  const VjpRecord R(recs, t);
   if (R.flag != 1) return;
   VjpColumn col;
   VjpWork wk;
@@ -2643,7 +2717,9 @@ __global__ __launch_bounds__(64) void k_vjp_adj401(long W, VjpRecord* __restrict
     ga[j] = R.g_alpha[j];
     gt[j] = R.g_tau[j];
   }
-  adj::l401_adjoint(col, wk, R.band, gs, ga, gt);
+  for (int j = 0; j < n; ++j)  // the band, into the lane's own memory
+    for (int o = 0; o < 5; ++o) wk.s.band[j][o] = R.band[j][o];
+  adj::l401_adjoint(col, wk, wk.s.band, gs, ga, gt);
   for (int j = 0; j < n; ++j) {
     R.g_snubar[j] = gs[j];
     R.g_alpha[j] = ga[j];
@@ -2652,10 +2728,10 @@ __global__ __launch_bounds__(64) void k_vjp_adj401(long W, VjpRecord* __restrict
   }
 }
 
-__global__ __launch_bounds__(64) void k_vjp_tau401(long W, VjpRecord* __restrict__ recs) {
+__global__ __launch_bounds__(64) void k_vjp_tau401(long W, VjpStore recs) {
   const long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (u >= W * 5) return;
-  VjpRecord& R = recs[u / 5];
+  const VjpRecord R(recs, u / 5);
   const int r = (int)(u % 5);
   if (R.flag != 1) return;
   VjpColumn col;
@@ -2729,21 +2805,19 @@ __device__ __forceinline__ void vjp_finish_at(const VjpArgs& a, long t, int r, c
   }
 }
 
-__global__ __launch_bounds__(64) void k_vjp_finish(VjpArgs a, const int32_t* __restrict__ order,
-                                                   const VjpRecord* __restrict__ recs) {
+__global__ __launch_bounds__(64) void k_vjp_finish(VjpArgs a, const int32_t* __restrict__ order, VjpStore recs) {
   const long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (u >= a.W * 4) return;
-  vjp_finish_at(a, u / 4, (int)(u % 4), order, recs[u / 4]);
+  vjp_finish_at(a, u / 4, (int)(u % 4), order, VjpRecord(recs, u / 4));
 }
 
 __global__ __launch_bounds__(64) void k_vjp_finish_batch(const VjpArgs* __restrict__ A, int n, long G,
-                                                         const int32_t* __restrict__ order,
-                                                         const VjpRecord* __restrict__ recs) {
+                                                         const int32_t* __restrict__ order, VjpStore recs) {
   const long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (u >= G * 4) return;
   const long g = u / 4;
   const VjpArgs a = A[vjp_inst(A, n, g)];
-  vjp_finish_at(a, g - a.g0, (int)(u % 4), order + a.g0, recs[g]);
+  vjp_finish_at(a, g - a.g0, (int)(u % 4), order + a.g0, VjpRecord(recs, g));
 }
 
 // ------------------------------------------------------------ ASYNTH pieces
@@ -4753,12 +4827,12 @@ int synthe_lte_vjp(synthe_ctx* ctx, long n_depths, long n_wl, const double* wl, 
   }
   const size_t o_cnt = align_up((size_t)n_wl * sizeof(int32_t));
   const size_t o_rec = o_cnt + 256;
-  int rc = ensure_ws(ctx, o_rec + (size_t)n_wl * sizeof(VjpRecord));
+  int rc = ensure_ws(ctx, o_rec + vjp_store_bytes(n_wl));
   if (rc) return rc;
   hipStream_t st = (hipStream_t)stream;
   int32_t* order = (int32_t*)ctx->ws;
   int* counts = (int*)((char*)ctx->ws + o_cnt);
-  VjpRecord* recs = (VjpRecord*)((char*)ctx->ws + o_rec);
+  const VjpStore recs{(char*)ctx->ws + o_rec, n_wl};
   hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, 2, counts);
   HIP_TRY(hipGetLastError());
   const VjpArgs a{(int)n_depths, n_wl, 0, wl, temperature, rhox, cont_abs, cont_scat, line_abs, line_scat,
@@ -4810,13 +4884,13 @@ int synthe_lte_vjp_batch(synthe_ctx* ctx, int n, const synthe_vjp_instance* inst
   const size_t o_ord = align_up((size_t)n * sizeof(VjpArgs));
   const size_t o_cnt = o_ord + align_up((size_t)G * sizeof(int32_t));
   const size_t o_rec = o_cnt + align_up((size_t)2 * n * sizeof(int));
-  int rc = ensure_ws(ctx, o_rec + (size_t)G * sizeof(VjpRecord));
+  int rc = ensure_ws(ctx, o_rec + vjp_store_bytes(G));
   if (rc) return rc;
   char* base = (char*)ctx->ws;
   VjpArgs* d_args = (VjpArgs*)base;
   int32_t* order = (int32_t*)(base + o_ord);
   int* counts = (int*)(base + o_cnt);
-  VjpRecord* recs = (VjpRecord*)(base + o_rec);
+  const VjpStore recs{base + o_rec, G};
   HIP_TRY(hipMemcpyAsync(d_args, args.data(), (size_t)n * sizeof(VjpArgs), hipMemcpyHostToDevice, st));
   HIP_TRY(hipStreamSynchronize(st));  // args (host) is freed when this call returns
   hipLaunchKernelGGL(k_zero_i32, dim3(blocks_for(2L * n, 64)), dim3(64), 0, st, 2 * n, counts);
