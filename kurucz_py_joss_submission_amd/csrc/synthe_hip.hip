@@ -26,7 +26,7 @@
 
 #include "_tables_gen.h"
 #include "rt_adjoint.h"
-#include "near_queue.h"
+#include "gather_queue.h"
 #include "rt_core.h"
 #include "synthe_device.h"
 #include "hydrogen.h"
@@ -727,47 +727,17 @@ __device__ __forceinline__ long class_window_search(int lane, int d, long lo_px,
 // Voigt zone of every wing: pixels 1 <= |k| <= N10DOP of pairs with a far
 // wing and 1 <= |k| <= MAXSTEP of the others, valued as
 // physics/line_opacity.py:410-560 (near_wing).  One wave per 64 pixels of one
-// depth; entries are walked with wave-uniform loads in class-then-centre
-// order.  A zone covers a few tens of the 64 pixels, so evaluating it lane per
-// pixel would leave most lanes idle: instead each (entry, pixel) pair is
-// queued in LDS, with the pixel's next slot, and the queue is evaluated 64
-// pairs at a time when a pixel runs out of slots (and at the end); each pixel
-// then adds its slots in order.  The sum order is the entry order, the same
-// as adding entry by entry.  Writes the pre-STIM sum to out [D][W] (0 where no
-// wing reaches), which k_gather continues from.
-#ifndef SYNTHE_NEAR_SLOTS
-#define SYNTHE_NEAR_SLOTS 6
-#endif
-constexpr int kNearSlots = SYNTHE_NEAR_SLOTS;  // queued values per pixel
-constexpr int kNearQueue = kNearSlots * 64;    // pairs per wave between flushes
-static_assert(kNearSlots >= 1 && kNearSlots <= 16, "slots per pixel: 1..16");
-static_assert(nq::kMaxZone == kMaxProfileSteps, "queue records hold offsets up to MAXPROF");
-
+// depth, one pixel per lane.  The class windows are walked 64 entries per step
+// (one coalesced load of their zone bounds); the wing parameters of the
+// entries whose zone meets the window are staged in LDS, each lane builds the
+// mask of those entries that reach its own pixel (two broadcasts and a range
+// test per entry), then adds their values in list order.  Each pixel's sum
+// runs in class-then-centre order, entry by entry.  Writes the pre-STIM sum
+// to out [D][W] (0 where no wing reaches), which k_gather continues from.
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// A queued pair: its entry, |k|, the pixel's lane and slot (near_queue.h).
-__device__ __forceinline__ uint64_t near_record(long j_rel, long ka, int lane, int slot) {
-  return nq::pack(j_rel, ka, lane, slot);
-}
-
-// Evaluates the qn queued pairs into vals (64 at a time), then adds each
-// lane's cnt slots to acc in slot order and empties the queue.
-__device__ __forceinline__ void near_flush(int lane, int qn, int& cnt, double& acc, const uint64_t* queue,
-                                           double (*vals)[64], const WingParams* __restrict__ list_p,
-                                           const DeviceTables& tb) {
-  wave_lds_sync();
-  for (int q = lane; q < qn; q += 64) {
-    const nq::Record r = nq::unpack(queue[q]);
-    vals[r.tag][r.lane] = near_wing(list_p[r.j_rel], r.ka, tb.h0, tb.h1, tb.h2);
-  }
-  wave_lds_sync();
-  for (int sl = 0; sl < cnt; ++sl) acc += vals[sl][lane];
-  wave_lds_sync();
-  cnt = 0;
 }
 
 template <bool kStats>
@@ -776,32 +746,23 @@ __device__ __forceinline__ void k_near_at(long bx_, int by_, long gx_, long W, i
                                               const int64_t* __restrict__ regions,
                                               const int32_t* __restrict__ near_reach, DeviceTables tb,
                                               double* __restrict__ out, unsigned long long* __restrict__ evals) {
-  __shared__ uint64_t s_queue[4][kNearQueue];
-  __shared__ double s_vals[4][kNearSlots][64];
+  __shared__ WingParams s_p[4][64];  // one step's meeting entries, per wave
   const int d = by_;
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const long v0 = ((long)bx_ * 4 + wv) * 64;
   if (v0 >= W) return;  // whole wave; the waves of a block share no LDS
   const long w = v0 + lane;
-  // pixel indices and centres fit 32 bits (n_wl <= 2^30): scalar entry tests
+  // pixel indices and centres fit 32 bits (n_wl <= 2^30): 32-bit entry tests
   const int v0i = (int)v0, wi = v0i + lane;
   const bool on_grid = w < W;
   const long bound = class_window_search(lane, d, v0, v0 + 63, (long)near_reach[d], regions, list_cm);
-  const long base = regions[(long)d * (kClasses + 1)];  // the depth's lists start here
-  const WingParams* __restrict__ lp = list_p + base;
-  uint64_t* queue = s_queue[wv];
-  double(*vals)[64] = s_vals[wv];
+  WingParams* sp = s_p[wv];
   double acc = 0.0;
-  int cnt = 0;  // this pixel's queued slots
-  int qn = 0;   // queued pairs (wave-uniform)
   unsigned long long nev = 0;
-  const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   for (int kk = 0; kk < kClasses; ++kk) {
     const long lo = uniform_long(__shfl(bound, 2 * kk));
     const long hi = uniform_long(__shfl(bound, 2 * kk + 1));
-    // 64 entries per step: one coalesced load of their zone bounds, a ballot
-    // of those whose zone meets the window, then each of those in list order
     for (long j0 = lo; j0 < hi; j0 += 64) {
       const long jl = j0 + lane;
       int2 z = make_int2(INT32_MAX, INT32_MIN);
@@ -809,31 +770,32 @@ __device__ __forceinline__ void k_near_at(long bx_, int by_, long gx_, long W, i
         const WingHead hv = list_cm[jl];
         z = make_int2(hv.nlo, hv.nhi);
       }
-      unsigned long long meet = __ballot(!(v0i + 63 < z.x || v0i > z.y));
+      const bool mine = !(v0i + 63 < z.x || v0i > z.y);  // my entry's zone meets the window
+      unsigned long long meet = __ballot(mine);
+      if (meet == 0ull) continue;
+      if (mine) sp[lane] = list_p[jl];
+      // the entries that reach this lane's pixel, 1 <= |k| <= the zone's half-width
+      unsigned long long reach = 0ull;
       while (meet) {
         const int src = __builtin_ctzll(meet);
         meet &= meet - 1;
         const int nlo = __builtin_amdgcn_readlane(z.x, src);
         const int nhi = __builtin_amdgcn_readlane(z.y, src);
         const int lim = (nhi - nlo) >> 1;  // the zone's half-width
-        const int c = nlo + lim;
-        const int ka = abs(wi - c);
-        const bool act = (unsigned)(ka - 1) < (unsigned)lim && on_grid;  // 1 <= |k| <= lim
-        if (__ballot(act && cnt == kNearSlots)) {  // a pixel is out of slots
-          near_flush(lane, qn, cnt, acc, queue, vals, lp, tb);
-          qn = 0;
-        }
-        const unsigned long long m = __ballot(act);
-        if (act) {
-          queue[qn + __popcll(m & below)] = near_record(j0 + src - base, ka, lane, cnt);
-          ++cnt;
-          if (kStats) ++nev;
-        }
-        qn += __popcll(m);
+        const int ka = abs(wi - (nlo + lim));
+        reach |= ((unsigned)(ka - 1) < (unsigned)lim && on_grid) ? (1ull << src) : 0ull;
       }
+      wave_lds_sync();  // the staged parameters before their reads
+      while (reach) {
+        const int src = __builtin_ctzll(reach);
+        reach &= reach - 1;
+        const WingParams pr = sp[src];
+        acc += near_wing(pr, abs(wi - pr.c), tb.h0, tb.h1, tb.h2);
+        if (kStats) ++nev;
+      }
+      wave_lds_sync();  // the reads before the next step's writes
     }
   }
-  near_flush(lane, qn, cnt, acc, queue, vals, lp, tb);
   if (w < W) out[(long)d * W + w] = acc;
   if (kStats) {
     for (int off = 32; off > 0; off >>= 1) nev += __shfl_down(nev, off);
@@ -959,20 +921,19 @@ __device__ __forceinline__ bool far_interior(const WingHead& h, int t0i, int t1i
 // are added to the pixels) and at the end.  Each pixel's sum has a fixed
 // order that depends only on the lists, no atomics, so results are
 // run-to-run identical.
-// The queue holds one 16-byte record per entry: X (as stored in the head)
-// and, packed in 61 bits, the centre relative to the tile's first pixel
-// (21-bit two's complement) and the two half-widths r (Voigt zone) and
-// MAXSTEP (20 bits each).  An entry that reaches the tile has |c - t0| <=
-// MAXSTEP + tile and MAXSTEP <= MAXPROF = 10^6 < 2^20.  640 records are 10
-// KiB of LDS per wave, so the 16 waves a CU holds at four per SIMD fill its
-// 160 KiB (a full queue is flushed: 384, 512, 576 and 640 records measured
-// 21.9, 20.7, 20.5 and 20.0 ms on the full window).
+// The queue holds one 16-byte record per entry (gather_queue.h): X as the
+// head stores it, and the centre relative to the tile with the two
+// half-widths packed in 61 bits.  640 records are 10 KiB of LDS per wave, so
+// the 16 waves a CU holds at four per SIMD fill its 160 KiB (a full queue is
+// flushed: 384, 512, 576 and 640 records measured 21.9, 20.7, 20.5 and 20.0 ms
+// on the full window).
 #ifndef SYNTHE_GATHER_QUEUE
 #define SYNTHE_GATHER_QUEUE 640
 #endif
 constexpr int kGatherQueue = SYNTHE_GATHER_QUEUE;
 static_assert(kGatherQueue >= 64 && kGatherQueue % 64 == 0, "queue holds whole steps of 64 entries");
-static_assert(kMaxProfileSteps + 64 * 64 < (1L << 20), "queue record fields hold MAXPROF plus a tile");
+static_assert(kMaxProfileSteps == gq::kMaxZone, "queue record fields hold MAXPROF");
+static_assert(64 * kWP <= gq::kMaxTile, "queue record centre field holds a tile");
 
 struct QueueRec {
   double xs;
@@ -980,25 +941,22 @@ struct QueueRec {
 };
 
 __device__ __forceinline__ QueueRec queue_pack(const WingHead& h, int t0i) {
-  const int dc = head_c(h) - t0i;
-  const unsigned r = (unsigned)((h.nhi - h.nlo) >> 1), m = (unsigned)((h.hi - h.lo) >> 1);
   QueueRec q;
   q.xs = h.xs;
-  q.pk = (uint64_t)((unsigned)dc & 0x1fffffu) | ((uint64_t)r << 21) | ((uint64_t)m << 41);
+  q.pk = gq::pack(head_c(h) - t0i, (h.nhi - h.nlo) >> 1, (h.hi - h.lo) >> 1);
   return q;
 }
 
 // The head of a queued entry in the tile's own coordinates (pixel t0 is 0).
 __device__ __forceinline__ WingHead queue_unpack(const QueueRec& q) {
-  const int dc = ((int)(unsigned)(q.pk << 11)) >> 11;  // bits 0-20, sign-extended
-  const int r = (int)((q.pk >> 21) & 0xfffffu), m = (int)((q.pk >> 41) & 0xfffffu);
+  const gq::Fields f = gq::unpack(q.pk);
   WingHead h;
-  h.lo = dc - m;
-  h.hi = dc + m;
-  h.nlo = dc - r;
-  h.nhi = dc + r;
+  h.lo = f.dc - f.m;
+  h.hi = f.dc + f.m;
+  h.nlo = f.dc - f.r;
+  h.nhi = f.dc + f.r;
   h.xs = q.xs;
-  h.cd = (double)dc;
+  h.cd = (double)f.dc;
   return h;
 }
 

@@ -1,6 +1,6 @@
 // Memory and undefined-behaviour check of the CPU code that the tests trust:
 // the oracle (oracle/synthe_oracle.c, oracle/hydrogen_oracle.c) and the CPU builds of the device headers
-// (near_queue_check.cpp, rt_core_check.cpp, rt_adj_check.cpp).  Built with
+// (gather_queue_check.cpp, rt_core_check.cpp, rt_adj_check.cpp).  Built with
 // -fsanitize=address,undefined into one executable by
 // tests/test_sanitize_cpu.py, which then runs it; any report fails the run.
 // Test infrastructure: every call runs on small seeded inputs whose array
@@ -91,8 +91,8 @@ int orc_hydrogen_lines(long NH, long D, long W, const double* grid, const double
                        const double* stim, double cutoff, double microturb_kms, double* ahline, double* stats);
 
 // CPU builds of the device headers
-int nq_roundtrip(long j_rel, long ka, int lane, int tag, long* out);
-long nq_max_zone(void);
+int gq_roundtrip(int dc, int r, int m, long* out);
+long gq_max_zone(void);
 double rtc_integ_map1(int n, const double* xtau, const double* abtot, const double* rho, const double* snubar,
                       const double* alpha, double* xsbar, double* xalpha);
 double adj_flux64(int n, const double* rho, const double* abtot, const double* alpha, const double* snubar,
@@ -302,16 +302,15 @@ void hydrogen(const orc_tables& tb) {
 void header_builds(const std::vector<double>& xtau, const std::vector<double>& ch,
                    const std::vector<double>& coefj) {
   std::vector<float> coefj32(coefj.begin(), coefj.end());
-  long out[4];
-  const long zones[] = {1, 2, 999999, 1000000};
-  for (long ka : zones)
-    for (int lane : {0, 63})
-      if (nq_roundtrip(4294967295L, ka, lane, 63, out) != 0 || out[0] != 4294967295L || out[1] != ka ||
-          out[2] != lane || out[3] != 63) {
+  long out[3];
+  const int zones[] = {0, 1, 999999, 1000000};
+  for (int z : zones)
+    for (int dc : {-1000000, -1, 0, 1004095})
+      if (gq_roundtrip(dc, z, z, out) != 0 || out[0] != dc || out[1] != z || out[2] != z) {
         fprintf(stderr, "queue record did not round-trip\n");
         exit(3);
       }
-  (void)nq_max_zone();
+  (void)gq_max_zone();
   for (int t = 0; t < 60; ++t) {
     const int n = (int)pick(4, 99);
     std::vector<double> rho(n), abtot(n), snubar(n), alpha(n), xs(51), xa(51), g1(n), g2(n), g3(n);

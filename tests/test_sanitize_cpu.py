@@ -47,7 +47,7 @@ def test_oracle_and_harnesses_are_sanitizer_clean(tmp_path):
         subprocess.run(["gcc", "-std=c11", "-fopenmp", *FLAGS, "-c", str(ROOT / "oracle" / src), "-o", str(obj)],
                        check=True)
         objs.append(obj)
-    for name in ("sanitize_main", "near_queue_check", "rt_core_check", "rt_adj_check"):
+    for name in ("sanitize_main", "gather_queue_check", "rt_core_check", "rt_adj_check"):
         obj = tmp_path / f"{name}.o"
         subprocess.run(["g++", "-std=c++17", *FLAGS, f"-I{INC}", "-c", str(NATIVE / f"{name}.cpp"), "-o", str(obj)],
                        check=True)
