@@ -1819,15 +1819,21 @@ __global__ void k_check_div(long n, int variant, const double* x, const double* 
 
 // Gauss-Seidel scattering iteration (josh_solver.py:24-92, Numba's float32
 // arrays with float64 accumulation) and flux = CH . XS, both solves.
-__global__ __launch_bounds__(kRtLanes) void k_rt_iter(const RtArgs* __restrict__ rt, int n_rt, long G,
-                                                      const uint8_t* __restrict__ flags,
-                                                      const float* __restrict__ ws, DeviceTables tb) {
-  __shared__ float s_xs[kNxtau][kRtLanes];
+// Four waves per workgroup share one LDS copy of COEFJ (10.6 KB) beside their
+// own XS columns (51 floats per lane): 62.8 KB per workgroup, two per CU, so
+// the CU holds the 8 waves its registers allow (64-lane workgroups, each with
+// its own COEFJ, fitted 6).
+constexpr int kRtIterBlock = 256;
+
+__global__ __launch_bounds__(kRtIterBlock) void k_rt_iter(const RtArgs* __restrict__ rt, int n_rt, long G,
+                                                          const uint8_t* __restrict__ flags,
+                                                          const float* __restrict__ ws, DeviceTables tb) {
+  __shared__ float s_xs[kNxtau][kRtIterBlock];
   __shared__ float s_cj[kNxtau * kNxtau + kNxtau];  // COEFJ rows, then its diagonal
-  for (int i = threadIdx.x; i < kNxtau * kNxtau; i += kRtLanes) s_cj[i] = tb.coefj32[i];
+  for (int i = threadIdx.x; i < kNxtau * kNxtau; i += kRtIterBlock) s_cj[i] = tb.coefj32[i];
   if (threadIdx.x < kNxtau) s_cj[kNxtau * kNxtau + threadIdx.x] = tb.diag32[threadIdx.x];
   __syncthreads();
-  const long g = blockIdx.x * (long)kRtLanes + threadIdx.x;
+  const long g = blockIdx.x * (long)kRtIterBlock + threadIdx.x;
   if (g >= G) return;
   if (flags[g] != 0) return;
   const int lane = threadIdx.x;
@@ -3014,7 +3020,7 @@ static int rt_launch_set(const synthe_ctx* ctx, const RtArgs* rt, int n_rt, long
   hipLaunchKernelGGL(k_rt_prep, dim3(blocks_for(G, kRtBlock)), dim3(kRtBlock), 0, st, rt, n_rt, G, ctx->tb.xtau, ws,
                      flags, list, list_n);
   HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(k_rt_iter, dim3(blocks_for(G, kRtLanes)), dim3(kRtLanes), 0, st, rt, n_rt, G, flags, ws,
+  hipLaunchKernelGGL(k_rt_iter, dim3(blocks_for(G, kRtIterBlock)), dim3(kRtIterBlock), 0, st, rt, n_rt, G, flags, ws,
                      ctx->tb);
   HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(k_lte, dim3(blocks_for(G, kRtLanes)), dim3(kRtLanes), 0, st, rt, n_rt, list, list_n, ctx->tb);
