@@ -2326,6 +2326,9 @@ __global__ void k_far_finish(int D, const int64_t* __restrict__ regions, const W
 // the three vectors live in LDS interleaved by lane (88 KB per 64-lane
 // workgroup), the rest in private memory.
 constexpr int kVjpBlock = 64;
+// From this many wavelengths in a call, the colour stages run one lane per
+// wavelength (1,024 waves and more fill the GPU without a lane per colour).
+constexpr long kVjpSeqMin = 65536;
 
 // The depths of wavelength w that enter the solve, in increasing rho (the
 // masks of engine/radiative.py:42-218): writes idx[0..n), returns n.
@@ -2625,11 +2628,16 @@ __global__ __launch_bounds__(kVjpBlock) void k_vjp_solve_batch(const VjpArgs* __
   vjp_solve_at(a, g - a.g0, order + a.g0, tb, s_cj, VjpRecord(recs, g));
 }
 
-__global__ __launch_bounds__(64) void k_vjp_pass(long W, DeviceTables tb, VjpStore recs) {
+// The colour passes: one lane per (wavelength, colour), or with seq one lane
+// per wavelength running its colours in turn on the column it loaded once
+// (the CPU order; each pass writes only its own entries, so the two forms
+// give the same results).  seq suits calls whose wavelengths alone fill the
+// GPU: the column is then read once instead of once per colour.
+__global__ __launch_bounds__(64) void k_vjp_pass(long W, DeviceTables tb, VjpStore recs, int seq) {
   const long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (u >= W * 5) return;
-  const VjpRecord R(recs, u / 5);
-  const int r = (int)(u % 5);
+  if (u >= W * (seq ? 1 : 5)) return;
+  const VjpRecord R(recs, seq ? u : u / 5);
+  const int r = seq ? -1 : (int)(u % 5);  // -1: every colour
   const int flag = R.flag;
   if (flag < 0 || (flag == 0 && r == 4)) return;
   VjpColumn col;
@@ -2642,25 +2650,37 @@ __global__ __launch_bounds__(64) void k_vjp_pass(long W, DeviceTables tb, VjpSto
       wk.x.gxb[k] = R.gxb[k];
       wk.x.gxa[k] = R.gxa[k];
     }
+    const int j0 = r < 0 ? 0 : r, js = r < 0 ? 1 : 4;
     double gs[kAdjMaxDepth], ga[kAdjMaxDepth], gt[kAdjMaxDepth];
-    for (int j = r; j < n; j += 4) {
+    for (int j = j0; j < n; j += js) {
       gs[j] = R.g_snubar[j];
       ga[j] = R.g_alpha[j];
       gt[j] = R.g_tau[j];
     }
     const adj::JoshTab jt{tb.xtau, tb.ch, tb.coefj32};
-    adj::xtau_pass(col, jt, wk, r, gs, ga, gt);
-    for (int j = r; j < n; j += 4) {
+    if (r < 0) {
+      for (int c = 0; c < 4; ++c) adj::xtau_pass(col, jt, wk, c, gs, ga, gt);
+    } else {
+      adj::xtau_pass(col, jt, wk, r, gs, ga, gt);
+    }
+    for (int j = j0; j < n; j += js) {
       R.g_snubar[j] = gs[j];
       R.g_alpha[j] = ga[j];
       R.g_tau[j] = gt[j];
     }
   } else {
     for (int j = 0; j < n; ++j) wk.pa[j] = R.S[j];
-    adj::l401_band(col, wk, r, wk.s.band);
+    if (r < 0) {
+      for (int c = 0; c < 5; ++c) adj::l401_band(col, wk, c, wk.s.band);
+    } else {
+      adj::l401_band(col, wk, r, wk.s.band);
+    }
     for (int j = 0; j < n; ++j) {
-      const int i = adj::seeded_in(j - 2, r, 5);
-      if (i >= 0 && i < n) R.band[j][i - j + 2] = wk.s.band[j][i - j + 2];
+      for (int c = 0; c < 5; ++c) {
+        if (r >= 0 && c != r) continue;
+        const int i = adj::seeded_in(j - 2, c, 5);
+        if (i >= 0 && i < n) R.band[j][i - j + 2] = wk.s.band[j][i - j + 2];
+      }
     }
   }
 }
@@ -2692,11 +2712,11 @@ __global__ __launch_bounds__(64) void k_vjp_adj401(long W, VjpStore recs) {
   }
 }
 
-__global__ __launch_bounds__(64) void k_vjp_tau401(long W, VjpStore recs) {
+__global__ __launch_bounds__(64) void k_vjp_tau401(long W, VjpStore recs, int seq) {
   const long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (u >= W * 5) return;
-  const VjpRecord R(recs, u / 5);
-  const int r = (int)(u % 5);
+  if (u >= W * (seq ? 1 : 5)) return;
+  const VjpRecord R(recs, seq ? u : u / 5);
+  const int r = seq ? -1 : (int)(u % 5);  // -1: every colour in turn
   if (R.flag != 1) return;
   VjpColumn col;
   VjpWork wk;
@@ -2707,13 +2727,19 @@ __global__ __launch_bounds__(64) void k_vjp_tau401(long W, VjpStore recs) {
     wk.pa[j] = R.S[j];
     wk.s.lam[j] = R.lam[j];
   }
-  for (int i = r; i < n; i += 5) gt[i] = R.g_tau[i];
-  adj::l401_tau(col, wk, r, gt);
-  for (int i = r; i < n; i += 5) R.g_tau[i] = gt[i];
+  const int i0 = r < 0 ? 0 : r, is = r < 0 ? 1 : 5;
+  for (int i = i0; i < n; i += is) gt[i] = R.g_tau[i];
+  if (r < 0) {
+    for (int c = 0; c < 5; ++c) adj::l401_tau(col, wk, c, gt);
+  } else {
+    adj::l401_tau(col, wk, r, gt);
+  }
+  for (int i = i0; i < n; i += is) R.g_tau[i] = gt[i];
 }
 
 // INTEG/PARCOE and the chain to the outputs of one wavelength for the depths
-// j = r (mod 4); t indexes the spectrum's order.
+// j = r (mod 4), or for every depth with the colours in turn when r < 0; t
+// indexes the spectrum's order.
 __device__ __forceinline__ void vjp_finish_at(const VjpArgs& a, long t, int r, const int32_t* __restrict__ order,
                                               const VjpRecord& R) {
   const long W = a.W;
@@ -2727,14 +2753,19 @@ __device__ __forceinline__ void vjp_finish_at(const VjpArgs& a, long t, int r, c
   for (int j = 0; j < n; ++j) wk.g_tau[j] = R.g_tau[j];
   adj::tau_suffix(n, wk);
   double gab[kAdjMaxDepth];
-  for (int j = r; j < n; j += 4) gab[j] = 0.0;
-  adj::integ_pass(col, wk, r, gab);
+  const int j0 = r < 0 ? 0 : r, js = r < 0 ? 1 : 4;
+  for (int j = j0; j < n; j += js) gab[j] = 0.0;
+  if (r < 0) {
+    for (int c = 0; c < 4; ++c) adj::integ_pass(col, wk, c, gab);
+  } else {
+    adj::integ_pass(col, wk, r, gab);
+  }
   const long w = order[t];
   const double vw = R.vw;
   const double wv = wl[w];
   const double freq = kCLightNm / (wv > 1e-12 ? wv : 1e-12);
   const double f3 = pow(freq / 1.0e15, 3.0);
-  for (int j = r; j < n; j += 4) {
+  for (int j = j0; j < n; j += js) {
     const int d = R.idx[j];
     const long i = (long)d * W + w;
     const double g_ab = gab[j] * vw, g_al = R.g_alpha[j] * vw, g_sb = R.g_snubar[j] * vw;
@@ -2769,19 +2800,23 @@ __device__ __forceinline__ void vjp_finish_at(const VjpArgs& a, long t, int r, c
   }
 }
 
-__global__ __launch_bounds__(64) void k_vjp_finish(VjpArgs a, const int32_t* __restrict__ order, VjpStore recs) {
+__global__ __launch_bounds__(64) void k_vjp_finish(VjpArgs a, const int32_t* __restrict__ order, VjpStore recs,
+                                                   int seq) {
   const long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (u >= a.W * 4) return;
-  vjp_finish_at(a, u / 4, (int)(u % 4), order, VjpRecord(recs, u / 4));
+  const int per = seq ? 1 : 4;
+  if (u >= a.W * per) return;
+  vjp_finish_at(a, u / per, seq ? -1 : (int)(u % 4), order, VjpRecord(recs, u / per));
 }
 
 __global__ __launch_bounds__(64) void k_vjp_finish_batch(const VjpArgs* __restrict__ A, int n, long G,
-                                                         const int32_t* __restrict__ order, VjpStore recs) {
+                                                         const int32_t* __restrict__ order, VjpStore recs,
+                                                         int seq) {
   const long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (u >= G * 4) return;
-  const long g = u / 4;
+  const int per = seq ? 1 : 4;
+  if (u >= G * per) return;
+  const long g = u / per;
   const VjpArgs a = A[vjp_inst(A, n, g)];
-  vjp_finish_at(a, g - a.g0, (int)(u % 4), order + a.g0, VjpRecord(recs, g));
+  vjp_finish_at(a, g - a.g0, seq ? -1 : (int)(u % 4), order + a.g0, VjpRecord(recs, g));
 }
 
 // ------------------------------------------------------------ ASYNTH pieces
@@ -4806,13 +4841,16 @@ int synthe_lte_vjp(synthe_ctx* ctx, long n_depths, long n_wl, const double* wl, 
   hipLaunchKernelGGL(k_vjp_solve, dim3(blocks_for(n_wl, kVjpBlock)), dim3(kVjpBlock), 0, st, a, order, ctx->tb,
                      recs);
   HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(k_vjp_pass, dim3(blocks_for(n_wl * 5, 64)), dim3(64), 0, st, n_wl, ctx->tb, recs);
+  const int seq = n_wl >= kVjpSeqMin;  // one lane per wavelength, its colours in turn
+  hipLaunchKernelGGL(k_vjp_pass, dim3(blocks_for(n_wl * (seq ? 1 : 5), 64)), dim3(64), 0, st, n_wl, ctx->tb, recs,
+                     seq);
   HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(k_vjp_adj401, dim3(blocks_for(n_wl, 64)), dim3(64), 0, st, n_wl, recs);
   HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(k_vjp_tau401, dim3(blocks_for(n_wl * 5, 64)), dim3(64), 0, st, n_wl, recs);
+  hipLaunchKernelGGL(k_vjp_tau401, dim3(blocks_for(n_wl * (seq ? 1 : 5), 64)), dim3(64), 0, st, n_wl, recs, seq);
   HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(k_vjp_finish, dim3(blocks_for(n_wl * 4, 64)), dim3(64), 0, st, a, order, recs);
+  hipLaunchKernelGGL(k_vjp_finish, dim3(blocks_for(n_wl * (seq ? 1 : 4), 64)), dim3(64), 0, st, a, order, recs,
+                     seq);
   HIP_TRY(hipGetLastError());
   return SYNTHE_OK;
 }
@@ -4865,13 +4903,15 @@ int synthe_lte_vjp_batch(synthe_ctx* ctx, int n, const synthe_vjp_instance* inst
   hipLaunchKernelGGL(k_vjp_solve_batch, dim3(blocks_for(G, kVjpBlock)), dim3(kVjpBlock), 0, st, d_args, n, G, order,
                      ctx->tb, recs);
   HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(k_vjp_pass, dim3(blocks_for(G * 5, 64)), dim3(64), 0, st, G, ctx->tb, recs);
+  const int seq = G >= kVjpSeqMin;  // one lane per wavelength, its colours in turn
+  hipLaunchKernelGGL(k_vjp_pass, dim3(blocks_for(G * (seq ? 1 : 5), 64)), dim3(64), 0, st, G, ctx->tb, recs, seq);
   HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(k_vjp_adj401, dim3(blocks_for(G, 64)), dim3(64), 0, st, G, recs);
   HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(k_vjp_tau401, dim3(blocks_for(G * 5, 64)), dim3(64), 0, st, G, recs);
+  hipLaunchKernelGGL(k_vjp_tau401, dim3(blocks_for(G * (seq ? 1 : 5), 64)), dim3(64), 0, st, G, recs, seq);
   HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(k_vjp_finish_batch, dim3(blocks_for(G * 4, 64)), dim3(64), 0, st, d_args, n, G, order, recs);
+  hipLaunchKernelGGL(k_vjp_finish_batch, dim3(blocks_for(G * (seq ? 1 : 4), 64)), dim3(64), 0, st, d_args, n, G,
+                     order, recs, seq);
   HIP_TRY(hipGetLastError());
   return SYNTHE_OK;
 }

@@ -126,11 +126,9 @@ def test_lte_vjp_matches_cpu_reverse_mode(gpu, oracle, cpu_vjp):
     assert np.all(got[0][10, 7] == 0.0) and np.all(got[0][:, 5] == 0.0)
 
 
-@pytest.mark.gpu
-def test_lte_vjp_batch_equals_single_calls(gpu):
-    """synthe_lte_vjp_batch over spectra of different depth counts and
-    widths equals synthe_lte_vjp called on each, bit for bit (each lane runs
-    the same device body on its own spectrum's arrays)."""
+def _single_and_batch(shapes, seed):
+    """synthe_lte_vjp on each (D, W) spectrum alone, and synthe_lte_vjp_batch
+    on all of them: the five gradient arrays of each, per spectrum."""
     import torch
 
     from kurucz_py_joss_submission_amd import _native as N
@@ -138,10 +136,10 @@ def test_lte_vjp_batch_equals_single_calls(gpu):
     ctx = N.context()
     lib, h = ctx.lib, ctx.handle
     st = N.stream_handle()
-    rng = np.random.default_rng(77)
+    rng = np.random.default_rng(seed)
     dev = lambda a: torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda")  # noqa: E731
     specs = []
-    for D, W in ((56, 96), (72, 300), (40, 17), (64, 1000)):
+    for D, W in shapes:
         rhox, T, ca, cs, la, ls = _columns(D, W, rng)
         src = None if W == 17 else np.abs(la) * 0 + 1e-3 * (1 + rng.random((D, W)))
         specs.append(dict(D=D, W=W, wl=dev(np.linspace(500.0, 505.0, W)), T=dev(T), rhox=dev(rhox), ca=dev(ca),
@@ -164,7 +162,34 @@ def test_lte_vjp_batch_equals_single_calls(gpu):
                               p(sp["la"]), p(sp["ls"]), p(sp["src"]), p(sp["v"]), *(p(x) for x in g))
     N.check("synthe_lte_vjp_batch", lib.synthe_lte_vjp_batch(h, len(specs), vi, st))
     torch.cuda.synchronize()
+    return single, batch
+
+
+def _bitwise(single, batch):
+    import torch
+
     for a, b in zip(single, batch):
         for x, y in zip(a, b):
             assert torch.equal(torch.nan_to_num(x, nan=-1.0), torch.nan_to_num(y, nan=-1.0))
+
+
+@pytest.mark.gpu
+def test_lte_vjp_batch_equals_single_calls(gpu):
+    """synthe_lte_vjp_batch over spectra of different depth counts and
+    widths equals synthe_lte_vjp called on each, bit for bit (each lane runs
+    the same device body on its own spectrum's arrays)."""
+    single, batch = _single_and_batch(((56, 96), (72, 300), (40, 17), (64, 1000)), 77)
+    _bitwise(single, batch)
     assert any(float(x.abs().max()) > 0 for x in single[1])
+
+
+@pytest.mark.gpu
+def test_lte_vjp_colours_in_turn_equal_lanes_per_colour(gpu):
+    """From 65,536 wavelengths in a call the colour stages run one lane per
+    wavelength with its colours in turn; below, one lane per (wavelength,
+    colour).  A batch of two 35,000-wavelength spectra (70,000 in the call:
+    colours in turn) equals the two spectra called alone (a lane per colour),
+    bit for bit, over ordinary, label-401 and strongly scattering columns."""
+    single, batch = _single_and_batch(((40, 35000), (48, 35000)), 78)
+    _bitwise(single, batch)
+    assert all(float(x.abs().max()) > 0 for x in single[0][:3])
